@@ -1,0 +1,118 @@
+"""Loader for the gfx950 library (codim-ipc_amd/libcipc.so) and its C ABI (include/cipc.h).
+
+The library is the only compute path: if it is missing, every entry point raises.
+There is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libcipc.so"))
+
+D = ctypes.POINTER(ctypes.c_double)
+I32 = ctypes.POINTER(ctypes.c_int32)
+
+CIPC_OK = 0
+ERRORS = {
+    -1: "invalid argument",
+    -2: "HIP runtime failure",
+    -3: "non-positive contact distance",
+    -4: "zero time of impact",
+    -5: "unsupported configuration",
+    -6: "no convergence",
+    -7: "buffer too small",
+}
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_vertices", ctypes.c_int32),
+        ("x0", D),
+        ("v0", D),
+        ("n_triangles", ctypes.c_int32),
+        ("triangles", I32),
+        ("n_dbc", ctypes.c_int32),
+        ("dbc_index", I32),
+        ("dbc_target", D),
+        ("density", ctypes.c_double),
+        ("youngs", ctypes.c_double),
+        ("poisson", ctypes.c_double),
+        ("thickness", ctypes.c_double),
+        ("bending_stiff_mult", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("dhat", ctypes.c_double),
+        ("offset", ctypes.c_double),
+        ("newton_tol", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
+        ("kappa_stiff_mult", ctypes.c_double),
+        ("pcg_tol", ctypes.c_double),
+        ("pcg_max_iter", ctypes.c_int32),
+        ("with_collision", ctypes.c_int32),
+    ]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [
+        ("newton_iters", ctypes.c_int32),
+        ("contacts", ctypes.c_int32),
+        ("pcg_iters", ctypes.c_int32),
+        ("gd_fallbacks", ctypes.c_int32),
+        ("ls_trials", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+        ("kappa", ctypes.c_double),
+        ("last_residual", ctypes.c_double),
+        ("energy", ctypes.c_double),
+        ("t_total_ms", ctypes.c_double),
+        ("t_constraint_set_ms", ctypes.c_double),
+        ("t_ccd_ms", ctypes.c_double),
+        ("t_hessian_ms", ctypes.c_double),
+        ("t_solve_ms", ctypes.c_double),
+        ("t_energy_ms", ctypes.c_double),
+        ("t_gradient_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# exported symbols and their prototypes (restype, argtypes)
+P = ctypes.c_void_p
+PROTOTYPES = {
+    "cipc_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "cipc_destroy": (None, [P]),
+    "cipc_last_error": (ctypes.c_char_p, [P]),
+    "cipc_get_sizes": (ctypes.c_int, [P, I32]),
+    "cipc_get_scalars": (ctypes.c_int, [P, D]),
+    "cipc_get_setup": (ctypes.c_int, [P, D, I32, D, I32]),
+    "cipc_set_state": (ctypes.c_int, [P, D, D]),
+    "cipc_get_state": (ctypes.c_int, [P, D, D]),
+    "cipc_set_kappa": (ctypes.c_int, [P, ctypes.c_double]),
+    "cipc_step": (ctypes.c_int, [P, ctypes.POINTER(StepStats)]),
+    "cipc_constraint_set": (ctypes.c_int, [P, D, I32, ctypes.c_int32, I32]),
+    "cipc_barrier_energy": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, D]),
+    "cipc_barrier_gradient": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, D]),
+    "cipc_elastic_energy": (ctypes.c_int, [P, D, D]),
+    "cipc_elastic_gradient": (ctypes.c_int, [P, D, D]),
+    "cipc_hessian": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, I32, I32, D, ctypes.c_int32, I32]),
+    "cipc_min_dist2": (ctypes.c_int, [P, D, I32, ctypes.c_int32, D, D]),
+    "cipc_ccd_stepsize": (ctypes.c_int, [P, D, D, ctypes.c_double, D, D]),
+    "cipc_linear_solve": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, D, D, ctypes.c_double, I32, D]),
+}
+
+_LIB = None
+
+
+def load():
+    """Load libcipc.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libcipc.so not found at %s: build it first (__graft_entry__.build())" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib

@@ -1,0 +1,175 @@
+"""Synthetic scenes for the contact-and-Newton step.
+
+The reference drives its FEMShell scenes from OBJ meshes and Python scene scripts
+(Projects/FEMShell/*.py, e.g. 18_sphere_on_cloth_stack.py); the benchmark scenes here
+are generated instead, from recorded seeds, so that every consumer (the HIP path,
+the CPU oracle, the benchmark) sees identical bits:
+
+* ``cloth_grid``: an n x n regular grid over [-0.5, 0.5]^2 in the xz plane, two
+  triangles per quad with a consistent +y normal (same topology family as
+  Projects/FEMShell/input/square41.obj);
+* ``icosphere``: the level-k subdivided icosahedron (level 4: 2,562 vertices,
+  5,120 faces);
+* ``ground``: two triangles spanning [-1, 1]^2 at y = 0;
+* vertex jitter: uniform(-1e-4, 1e-4) m per coordinate from a splitmix64 stream.
+
+``scene_c1`` and ``scene_c2`` build the two cloth-on-sphere workloads (41x41 cloth,
+and four 201x201 sheets 5 mm apart); sphere and ground vertices are Dirichlet-fixed.
+"""
+import math
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64_uniform(seed, count):
+    """``count`` doubles in [0, 1) from a splitmix64 stream (53-bit mantissa)."""
+    with np.errstate(over="ignore"):
+        i = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed & MASK64) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def cloth_grid(n, y=0.5):
+    idx = np.arange(n)
+    xs = -0.5 + idx / (n - 1)
+    X = np.zeros((n * n, 3))
+    for j in range(n):
+        X[j * n:(j + 1) * n, 0] = xs
+        X[j * n:(j + 1) * n, 2] = -0.5 + j / (n - 1)
+    X[:, 1] = y
+    tris = []
+    for j in range(n - 1):
+        for i in range(n - 1):
+            v00 = i + n * j
+            v10 = v00 + 1
+            v01 = v00 + n
+            v11 = v01 + 1
+            tris.append((v00, v11, v10))
+            tris.append((v00, v01, v11))
+    return X, np.array(tris, dtype=np.int32)
+
+
+def icosphere(level=4, radius=0.2, center=(0.0, 0.0, 0.0)):
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0),
+             (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11),
+             (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8),
+             (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
+             (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for (a, b, c) in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    X = np.array(verts) * radius + np.asarray(center, dtype=np.float64)
+    return X, np.array(faces, dtype=np.int32)
+
+
+def ground(half=1.0, y=0.0):
+    X = np.array([[-half, y, -half], [half, y, -half], [half, y, half], [-half, y, half]], dtype=np.float64)
+    T = np.array([[0, 2, 1], [0, 3, 2]], dtype=np.int32)
+    return X, T
+
+
+class SceneData:
+    """Arrays and parameters of one scene (all float64 / int32, vertex-major xyz)."""
+
+    def __init__(self, X, tris, dbc_idx, dbc_target, params, name=""):
+        self.X = np.ascontiguousarray(X, dtype=np.float64)
+        self.tris = np.ascontiguousarray(tris, dtype=np.int32)
+        self.dbc_idx = np.ascontiguousarray(dbc_idx, dtype=np.int32)
+        self.dbc_target = np.ascontiguousarray(dbc_target, dtype=np.float64)
+        self.params = dict(params)
+        self.name = name
+
+    @property
+    def nV(self):
+        return self.X.shape[0]
+
+    @property
+    def nT(self):
+        return self.tris.shape[0]
+
+    def bbox_diag(self):
+        return float(np.linalg.norm(self.X.max(axis=0) - self.X.min(axis=0)))
+
+
+DEFAULT_PARAMS = dict(rho=200.0, E=1e6, nu=0.3, thick=1e-3, bendMult=1.0, dt=0.01, dHat=1e-3, xi=0.0,
+                      gravity=(0.0, -9.81, 0.0), kappaStiffMult=1.0, collision=1)
+
+
+def cloth_on_sphere(sheets=1, n=41, y0=0.5, dy=0.005, seed=1, sphere_level=4, sphere_radius=0.2,
+                    ground_half=1.0, jitter=1e-4, name="", twist_deg=0.0, twists=None):
+    parts_X, parts_T = [], []
+    nv = 0
+    for s in range(sheets):
+        X, T = cloth_grid(n, y0 + s * dy)
+        tw = twists[s] if twists is not None else twist_deg * s
+        if tw:
+            a = math.radians(tw)
+            c, sn = math.cos(a), math.sin(a)
+            X = X.copy()
+            X[:, 0], X[:, 2] = c * X[:, 0] - sn * X[:, 2], sn * X[:, 0] + c * X[:, 2]
+        parts_X.append(X)
+        parts_T.append(T + nv)
+        nv += X.shape[0]
+    n_cloth = nv
+    Xc = np.concatenate(parts_X)
+    if jitter:
+        u = splitmix64_uniform(seed, Xc.size).reshape(Xc.shape)
+        Xc = Xc + (2.0 * u - 1.0) * jitter
+    Xs, Ts = icosphere(sphere_level, sphere_radius)
+    Xg, Tg = ground(ground_half)
+    X = np.concatenate([Xc, Xs, Xg])
+    T = np.concatenate(parts_T + [Ts + n_cloth, Tg + n_cloth + Xs.shape[0]])
+    dbc = np.arange(n_cloth, X.shape[0], dtype=np.int32)
+    params = dict(DEFAULT_PARAMS)
+    sd = SceneData(X, T, dbc, X[dbc].copy(), params, name)
+    sd.params["PNTol"] = 1e-2 * sd.bbox_diag()
+    sd.n_cloth = n_cloth
+    return sd
+
+
+def scene_c1(seed=1):
+    """41x41 cloth (1,681 v / 3,200 tris) over the icosphere and ground, 4,247 vertices."""
+    return cloth_on_sphere(sheets=1, n=41, seed=seed, name="c1_cloth41_sphere")
+
+
+def scene_c2(seed=2):
+    """Four 201x201 sheets 5 mm apart (161,604 cloth v / 320,000 tris) over the icosphere and ground."""
+    return cloth_on_sphere(sheets=4, n=201, seed=seed, name="c2_cloth4x201_sphere")
+
+
+def scene_small(n=9, sheets=2, seed=3, y0=0.22, dy=0.004, twist_deg=0.0, sphere_level=2):
+    """A few-hundred-vertex cloth stack just above the sphere: contact within a few steps."""
+    return cloth_on_sphere(sheets=sheets, n=n, y0=y0, dy=dy, seed=seed, sphere_level=sphere_level, name="small_stack",
+                           twist_deg=twist_deg)
+
+
+def scene_contact_stack(n=12, seed=5):
+    """Three sheets 0.8 mm apart (inside d_hat = 1 mm) resting 0.8 mm above the sphere top; the two
+    lower sheets are aligned (parallel edges: mollified edge-edge / point-edge / point-point) and the top
+    one is twisted by 10 degrees (general point-triangle and edge-edge), so every constraint kind is active
+    in the first state."""
+    sd = cloth_on_sphere(sheets=3, n=n, y0=0.2008, dy=0.0008, seed=seed, sphere_level=3, name="contact_stack",
+                         twists=[0.0, 0.0, 10.0])
+    return sd

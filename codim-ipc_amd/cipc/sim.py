@@ -1,0 +1,209 @@
+"""Host-side object over the C ABI: one simulated scene on one GPU.
+
+``Simulator`` owns a ``cipc_ctx``; its methods map one-to-one onto the reference
+functions the hot path is built from (names in the docstrings), taking and returning
+numpy arrays (vertex-major xyz, float64).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class CipcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("cipc error %d (%s): %s" % (code, _lib.ERRORS.get(code, "?"), msg))
+        self.code = code
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_lib.D)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_lib.I32)
+
+
+class Simulator:
+    def __init__(self, sd, pcg_tol=1e-5, pcg_max_iter=100000):
+        """Initialize_Shell_Hinge_EIPC + Initialize_OIPC (DISCRETE_SHELL.h:218-357, 555-577)."""
+        self.lib = _lib.load()
+        P = sd.params
+        self._keep = [sd.X, sd.tris, sd.dbc_idx, sd.dbc_target]
+        desc = _lib.SceneDesc()
+        desc.n_vertices = sd.nV
+        desc.x0 = _d(sd.X)
+        desc.v0 = None
+        desc.n_triangles = sd.nT
+        desc.triangles = _i(sd.tris)
+        desc.n_dbc = len(sd.dbc_idx)
+        desc.dbc_index = _i(sd.dbc_idx)
+        desc.dbc_target = _d(sd.dbc_target)
+        desc.density = P["rho"]
+        desc.youngs = P["E"]
+        desc.poisson = P["nu"]
+        desc.thickness = P["thick"]
+        desc.bending_stiff_mult = P["bendMult"]
+        desc.dt = P["dt"]
+        desc.dhat = P["dHat"]
+        desc.offset = P["xi"]
+        desc.newton_tol = P["PNTol"]
+        g = P.get("gravity", (0.0, -9.81, 0.0))
+        desc.gravity[0], desc.gravity[1], desc.gravity[2] = g
+        desc.kappa_stiff_mult = P.get("kappaStiffMult", 1.0)
+        desc.pcg_tol = pcg_tol
+        desc.pcg_max_iter = pcg_max_iter
+        desc.with_collision = int(P.get("collision", 1))
+        h = ctypes.c_void_p()
+        rc = self.lib.cipc_create(ctypes.byref(desc), ctypes.byref(h))
+        if rc != 0:
+            raise CipcError(rc, "cipc_create failed")
+        self.h = h
+        self.nV = sd.nV
+        s = np.zeros(6, dtype=np.int32)
+        self.lib.cipc_get_sizes(self.h, _i(s))
+        self.nT, self.nBN, self.nBE, self.nH = int(s[1]), int(s[2]), int(s[3]), int(s[4])
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cipc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CipcError(rc, self.lib.cipc_last_error(self.h).decode())
+
+    # ------------------------------------------------------------ state
+    def scalars(self):
+        out = np.zeros(6)
+        self._check(self.lib.cipc_get_scalars(self.h, _d(out)))
+        return dict(kappa=out[0], kappaMax=out[1], dHat2=out[2], lam=out[3], mu=out[4], kBend=out[5])
+
+    def setup_arrays(self):
+        mass = np.zeros(self.nV)
+        hinge = np.zeros(4 * self.nH, dtype=np.int32)
+        hinfo = np.zeros(3 * self.nH)
+        bedge = np.zeros(2 * self.nBE, dtype=np.int32)
+        self._check(self.lib.cipc_get_setup(self.h, _d(mass), _i(hinge), _d(hinfo), _i(bedge)))
+        return dict(mass=mass, hinge=hinge.reshape(-1, 4), hinge_info=hinfo.reshape(-1, 3), bedge=bedge.reshape(-1, 2))
+
+    def get_state(self):
+        x = np.zeros(3 * self.nV)
+        v = np.zeros(3 * self.nV)
+        self._check(self.lib.cipc_get_state(self.h, _d(x), _d(v)))
+        return x, v
+
+    def set_state(self, x=None, v=None):
+        xx = None if x is None else np.ascontiguousarray(x, dtype=np.float64).ravel()
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64).ravel()
+        self._check(self.lib.cipc_set_state(self.h, _d(xx), _d(vv)))
+
+    def set_kappa(self, k):
+        self._check(self.lib.cipc_set_kappa(self.h, float(k)))
+
+    # ------------------------------------------------------------ step
+    def step(self):
+        """Advance_One_Step_IE_Discrete_Shell<double,3,false,false> (IMPLICIT_EULER.h:151-902)."""
+        st = _lib.StepStats()
+        rc = self.lib.cipc_step(self.h, ctypes.byref(st))
+        self._check(rc)
+        return st.as_dict()
+
+    # ------------------------------------------------------------ components
+    @staticmethod
+    def _x(x):
+        return np.ascontiguousarray(x, dtype=np.float64).ravel()
+
+    @staticmethod
+    def _cs(C):
+        return np.ascontiguousarray(C, dtype=np.int32).reshape(-1, 4)
+
+    def constraint_set(self, x):
+        """Compute_Constraint_Set (IPC.h:19-740)."""
+        x = self._x(x)
+        n = ctypes.c_int32(0)
+        cap = 1 << 16
+        while True:
+            out = np.zeros(4 * cap, dtype=np.int32)
+            rc = self.lib.cipc_constraint_set(self.h, _d(x), _i(out), cap, ctypes.byref(n))
+            if rc == -7:
+                cap = n.value
+                continue
+            self._check(rc)
+            return out[:4 * n.value].reshape(-1, 4)
+
+    def barrier_energy(self, x, C, kappa):
+        """Compute_Barrier (IPC.h:742-941)."""
+        x, C = self._x(x), self._cs(C)
+        e = np.zeros(1)
+        self._check(self.lib.cipc_barrier_energy(self.h, _d(x), _i(C), len(C), float(kappa), _d(e)))
+        return e[0]
+
+    def barrier_gradient(self, x, C, kappa):
+        """Compute_Barrier_Gradient (IPC.h:943-1256)."""
+        x, C = self._x(x), self._cs(C)
+        g = np.zeros(3 * self.nV)
+        self._check(self.lib.cipc_barrier_gradient(self.h, _d(x), _i(C), len(C), float(kappa), _d(g)))
+        return g
+
+    def elastic_energy(self, x):
+        """Compute_Membrane_Energy + Compute_Bending_Energy: (membrane, bending)."""
+        out = np.zeros(2)
+        self._check(self.lib.cipc_elastic_energy(self.h, _d(self._x(x)), _d(out)))
+        return out
+
+    def elastic_gradient(self, x):
+        g = np.zeros(3 * self.nV)
+        self._check(self.lib.cipc_elastic_gradient(self.h, _d(self._x(x)), _d(g)))
+        return g
+
+    def hessian(self, x, C, kappa, which=3):
+        """Compute_IncPotential_Hessian (INC_POTENTIAL.h:277-395) as a scipy CSR matrix."""
+        import scipy.sparse as sp
+        x, C = self._x(x), self._cs(C)
+        nb = ctypes.c_int32(0)
+        rc = self.lib.cipc_hessian(self.h, _d(x), _i(C), len(C), float(kappa), which, None, None, None, 0, ctypes.byref(nb))
+        if rc not in (0, -7):
+            self._check(rc)
+        cap = nb.value
+        br = np.zeros(max(cap, 1), dtype=np.int32)
+        bc = np.zeros(max(cap, 1), dtype=np.int32)
+        bv = np.zeros((max(cap, 1), 9))
+        self._check(self.lib.cipc_hessian(self.h, _d(x), _i(C), len(C), float(kappa), which, _i(br), _i(bc), _d(bv), cap,
+                                          ctypes.byref(nb)))
+        n = nb.value
+        rows = (3 * br[:n, None] + np.array([0, 0, 0, 1, 1, 1, 2, 2, 2])[None, :]).ravel()
+        cols = (3 * bc[:n, None] + np.array([0, 1, 2, 0, 1, 2, 0, 1, 2])[None, :]).ravel()
+        return sp.csr_matrix((bv[:n].ravel(), (rows, cols)), shape=(3 * self.nV, 3 * self.nV))
+
+    def min_dist2(self, x, C):
+        """Compute_Min_Dist2 (IPC.h:2246-2388): (per-constraint d^2, min d^2 - offset^2)."""
+        x, C = self._x(x), self._cs(C)
+        d2 = np.zeros(max(len(C), 1))
+        m = np.zeros(1)
+        self._check(self.lib.cipc_min_dist2(self.h, _d(x), _i(C), len(C), _d(d2), _d(m)))
+        return d2[:len(C)], m[0]
+
+    def ccd_stepsize(self, x, p, alpha=1.0):
+        """Compute_Intersection_Free_StepSize (IPC.h:1879-2244): (alpha, alpha after the hash clamp)."""
+        a = np.zeros(1)
+        cl = np.zeros(1)
+        self._check(self.lib.cipc_ccd_stepsize(self.h, _d(self._x(x)), _d(self._x(p)), float(alpha), _d(a), _d(cl)))
+        return a[0], cl[0]
+
+    def linear_solve(self, x, C, kappa, b, tol=1e-5):
+        """Assemble H(x, C) and solve H s = b by block-Jacobi PCG (stands in for Solve_Direct)."""
+        x, C, b = self._x(x), self._cs(C), self._x(b)
+        sol = np.zeros(3 * self.nV)
+        it = ctypes.c_int32(0)
+        rr = np.zeros(1)
+        self._check(self.lib.cipc_linear_solve(self.h, _d(x), _i(C), len(C), float(kappa), _d(b), _d(sol), float(tol),
+                                               ctypes.byref(it), _d(rr)))
+        return sol, it.value, rr[0]

@@ -1,0 +1,332 @@
+// C ABI (include/cipc.h).  Every entry point converts HIP failures and invalid
+// arguments into negative status codes and records a message for cipc_last_error().
+#include "ctx.hpp"
+#include <cstring>
+#include <exception>
+
+using namespace cipc;
+
+struct cipc_ctx {
+    Ctx c;
+    DevBuf<double> qx, qp, qb, qsol, qout;
+    DevBuf<int> qC;
+};
+
+namespace cipc {
+void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri);
+void elastic_gradient_only(Ctx& c, const double* x, double* g);
+}
+
+template <class F>
+static int guarded(cipc_ctx* ctx, F f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        if (ctx) ctx->c.lastError = e.what();
+        return CIPC_E_HIP;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->c.lastError = e.what();
+        return CIPC_E_INVALID;
+    }
+}
+
+static void upload_x(cipc_ctx* ctx, DevBuf<double>& b, const double* h, size_t n) {
+    b.ensure(n);
+    CIPC_HIP(hipMemcpyAsync(b.p, h, n * sizeof(double), hipMemcpyHostToDevice, ctx->c.s));
+}
+// constraints arrive as int4 rows; the device keeps them SoA [4][n]
+static void upload_cs(cipc_ctx* ctx, const int32_t* cs, int32_t n) {
+    std::vector<int> soa(4 * (size_t)std::max(n, 1));
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 4; ++k) soa[(size_t)k * n + i] = cs[4 * (size_t)i + k];
+    ctx->qC.ensure(soa.size());
+    CIPC_HIP(hipMemcpyAsync(ctx->qC.p, soa.data(), soa.size() * sizeof(int), hipMemcpyHostToDevice, ctx->c.s));
+    CIPC_HIP(hipStreamSynchronize(ctx->c.s));
+}
+
+extern "C" {
+
+int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
+    if (!d || !out || d->n_vertices <= 0 || !d->x0 || d->n_triangles < 0 || (d->n_triangles && !d->triangles) ||
+        d->n_dbc < 0 || (d->n_dbc && (!d->dbc_index || !d->dbc_target)))
+        return CIPC_E_INVALID;
+    for (int i = 0; i < 3 * d->n_triangles; ++i)
+        if (d->triangles[i] < 0 || d->triangles[i] >= d->n_vertices) return CIPC_E_INVALID;
+    for (int i = 0; i < d->n_dbc; ++i)
+        if (d->dbc_index[i] < 0 || d->dbc_index[i] >= d->n_vertices) return CIPC_E_INVALID;
+    if (d->n_vertices >= (1 << 21) - 1) return CIPC_E_INVALID;   // contact-key packing limit
+    cipc_ctx* ctx = new cipc_ctx();
+    int rc = guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        Params& P = c.P;
+        P.rho = d->density; P.E = d->youngs; P.nu = d->poisson; P.thick = d->thickness; P.bendMult = d->bending_stiff_mult;
+        P.dt = d->dt; P.dHat = d->dhat; P.xi = d->offset; P.PNTol = d->newton_tol;
+        for (int k = 0; k < 3; ++k) P.grav[k] = d->gravity[k];
+        P.kappaStiffMult = d->kappa_stiff_mult;
+        P.pcgTol = d->pcg_tol > 0 ? d->pcg_tol : 1e-5;
+        P.pcgMaxIter = d->pcg_max_iter > 0 ? d->pcg_max_iter : 100000;
+        P.collision = d->with_collision;
+        CIPC_HIP(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
+        CIPC_HIP(hipHostMalloc((void**)&c.hScal, 64 * sizeof(double), hipHostMallocDefault));
+        CIPC_HIP(hipHostMalloc((void**)&c.hCnt, 64 * sizeof(uint32_t), hipHostMallocDefault));
+        c.nV = d->n_vertices;
+        c.nDBC = d->n_dbc;
+        c.hDbcIdx.assign(d->dbc_index, d->dbc_index + d->n_dbc);
+        c.hDbcTarget.assign(d->dbc_target, d->dbc_target + 3 * (size_t)d->n_dbc);
+        std::vector<double> X(d->x0, d->x0 + 3 * (size_t)c.nV);
+        std::vector<int> T(d->triangles, d->triangles + 3 * (size_t)d->n_triangles);
+        ctx_setup(c, X, T);
+        CIPC_HIP(hipMemcpyAsync(c.x.p, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice, c.s));
+        if (d->v0) CIPC_HIP(hipMemcpyAsync(c.v.p, d->v0, X.size() * sizeof(double), hipMemcpyHostToDevice, c.s));
+        else CIPC_HIP(hipMemsetAsync(c.v.p, 0, X.size() * sizeof(double), c.s));
+        CIPC_HIP(hipMemcpyAsync(c.xt.p, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        return CIPC_OK;
+    });
+    if (rc != CIPC_OK) {
+        cipc_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return CIPC_OK;
+}
+
+void cipc_destroy(cipc_ctx* ctx) {
+    if (!ctx) return;
+    Ctx& c = ctx->c;
+    if (c.s) (void)hipStreamSynchronize(c.s);
+    if (c.hScal) (void)hipHostFree(c.hScal);
+    if (c.hCnt) (void)hipHostFree(c.hCnt);
+    hipStream_t s = c.s;
+    delete ctx;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char* cipc_last_error(const cipc_ctx* ctx) { return ctx ? ctx->c.lastError.c_str() : "null context"; }
+
+int cipc_get_sizes(const cipc_ctx* ctx, int32_t* out) {
+    if (!ctx || !out) return CIPC_E_INVALID;
+    const Ctx& c = ctx->c;
+    out[0] = c.nV; out[1] = c.nT; out[2] = c.nBN; out[3] = c.nBE; out[4] = c.nH; out[5] = c.nC;
+    return CIPC_OK;
+}
+int cipc_get_scalars(const cipc_ctx* ctx, double* out) {
+    if (!ctx || !out) return CIPC_E_INVALID;
+    const Ctx& c = ctx->c;
+    out[0] = c.kappa; out[1] = c.kappaMax; out[2] = c.dHat2; out[3] = c.lam; out[4] = c.mu; out[5] = c.kBend;
+    return CIPC_OK;
+}
+int cipc_get_setup(const cipc_ctx* ctx, double* mass, int32_t* hinges, double* hinge_info, int32_t* bedges) {
+    if (!ctx) return CIPC_E_INVALID;
+    const Ctx& c = ctx->c;
+    if (mass) std::memcpy(mass, c.hMass.data(), c.hMass.size() * sizeof(double));
+    if (hinges) std::memcpy(hinges, c.hHinge.data(), c.hHinge.size() * sizeof(int));
+    if (hinge_info) std::memcpy(hinge_info, c.hHingeInfo.data(), c.hHingeInfo.size() * sizeof(double));
+    if (bedges) std::memcpy(bedges, c.hBEdge.data(), c.hBEdge.size() * sizeof(int));
+    return CIPC_OK;
+}
+int cipc_set_state(cipc_ctx* ctx, const double* x, const double* v) {
+    if (!ctx) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        size_t n = 3 * (size_t)c.nV;
+        if (x) CIPC_HIP(hipMemcpyAsync(c.x.p, x, n * sizeof(double), hipMemcpyHostToDevice, c.s));
+        if (v) CIPC_HIP(hipMemcpyAsync(c.v.p, v, n * sizeof(double), hipMemcpyHostToDevice, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        return CIPC_OK;
+    });
+}
+int cipc_get_state(const cipc_ctx* cctx, double* x, double* v) {
+    cipc_ctx* ctx = const_cast<cipc_ctx*>(cctx);
+    if (!ctx) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        size_t n = 3 * (size_t)c.nV;
+        if (x) CIPC_HIP(hipMemcpyAsync(x, c.x.p, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        if (v) CIPC_HIP(hipMemcpyAsync(v, c.v.p, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        return CIPC_OK;
+    });
+}
+int cipc_set_kappa(cipc_ctx* ctx, double kappa) {
+    if (!ctx || !(kappa > 0)) return CIPC_E_INVALID;
+    ctx->c.kappa = kappa;
+    return CIPC_OK;
+}
+
+int cipc_step(cipc_ctx* ctx, cipc_step_stats* stats) {
+    if (!ctx) return CIPC_E_INVALID;
+    int rc = guarded(ctx, [&]() { return advance_one_step(ctx->c); });
+    ctx->c.stats.status = rc;
+    if (stats) *stats = ctx->c.stats;
+    return rc;
+}
+
+int cipc_constraint_set(cipc_ctx* ctx, const double* x, int32_t* out, int32_t cap, int32_t* n) {
+    if (!ctx || !x || !n) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        upload_x(ctx, ctx->qx, x, 3 * (size_t)c.nV);
+        int nC = 0;
+        DevBuf<int>& outD = ctx->qC;
+        constraint_set(c, ctx->qx.p, outD, nC);
+        *n = nC;
+        if (nC > cap || !out) return nC > cap ? CIPC_E_CAPACITY : CIPC_OK;
+        std::vector<int> soa(4 * (size_t)std::max(nC, 1));
+        CIPC_HIP(hipMemcpyAsync(soa.data(), outD.p, 4 * (size_t)nC * sizeof(int), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        for (int i = 0; i < nC; ++i)
+            for (int k = 0; k < 4; ++k) out[4 * (size_t)i + k] = soa[(size_t)k * nC + i];
+        return CIPC_OK;
+    });
+}
+
+int cipc_barrier_energy(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, double* e) {
+    if (!ctx || !x || !e || n_cs < 0 || (n_cs && !cs)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        upload_x(ctx, ctx->qx, x, 3 * (size_t)c.nV);
+        upload_cs(ctx, cs, n_cs);
+        int bad = 0;
+        *e = barrier_energy(c, ctx->qx.p, ctx->qC.p, n_cs, kappa, &bad);
+        if (bad) { c.lastError = "non-positive constraint distance"; return CIPC_E_PENETRATION; }
+        return CIPC_OK;
+    });
+}
+
+int cipc_barrier_gradient(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, double* g) {
+    if (!ctx || !x || !g || n_cs < 0 || (n_cs && !cs)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        size_t n = 3 * (size_t)c.nV;
+        upload_x(ctx, ctx->qx, x, n);
+        upload_cs(ctx, cs, n_cs);
+        double* gd = ctx->qout.ensure(n);
+        CIPC_HIP(hipMemsetAsync(gd, 0, n * sizeof(double), c.s));
+        barrier_gradient(c, ctx->qx.p, ctx->qC.p, n_cs, kappa, gd);
+        CIPC_HIP(hipMemcpyAsync(g, gd, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        return CIPC_OK;
+    });
+}
+
+int cipc_elastic_energy(cipc_ctx* ctx, const double* x, double* out) {
+    if (!ctx || !x || !out) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        upload_x(ctx, ctx->qx, x, 3 * (size_t)c.nV);
+        elastic_energy(c, ctx->qx.p, &out[0], &out[1]);
+        return CIPC_OK;
+    });
+}
+
+int cipc_elastic_gradient(cipc_ctx* ctx, const double* x, double* g) {
+    if (!ctx || !x || !g) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        size_t n = 3 * (size_t)c.nV;
+        upload_x(ctx, ctx->qx, x, n);
+        double* gd = ctx->qout.ensure(n);
+        elastic_gradient_only(c, ctx->qx.p, gd);
+        CIPC_HIP(hipMemcpyAsync(g, gd, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        return CIPC_OK;
+    });
+}
+
+static void download_bsr(Ctx& c, const BSR& A, int32_t* brow, int32_t* bcol, double* bval, int off) {
+    if (A.nnzb == 0) return;
+    std::vector<int> r(A.nnzb), cl(A.nnzb);
+    std::vector<double> v(9 * (size_t)A.nnzb);
+    CIPC_HIP(hipMemcpyAsync(r.data(), A.rowOf.p, A.nnzb * sizeof(int), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipMemcpyAsync(cl.data(), A.col.p, A.nnzb * sizeof(int), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipMemcpyAsync(v.data(), A.val.p, v.size() * sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    for (int b = 0; b < A.nnzb; ++b) {
+        brow[off + b] = r[b];
+        bcol[off + b] = cl[b];
+        for (int k = 0; k < 9; ++k) bval[9 * (size_t)(off + b) + k] = v[(size_t)k * A.nnzb + b];
+    }
+}
+
+int cipc_hessian(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t which, int32_t* brow,
+                 int32_t* bcol, double* bval, int32_t cap, int32_t* nblocks_out) {
+    if (!ctx || !x || !nblocks_out || n_cs < 0 || (n_cs && !cs)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        upload_x(ctx, ctx->qx, x, 3 * (size_t)c.nV);
+        int ne = 0, nk = 0;
+        if (which & 1) { elastic_hessian(c, ctx->qx.p); ne = c.Ael.nnzb; }
+        if (which & 2) {
+            upload_cs(ctx, cs, n_cs);
+            barrier_hessian(c, ctx->qx.p, ctx->qC.p, n_cs, kappa);
+            nk = c.Acon.nnzb;
+        }
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        *nblocks_out = ne + nk;
+        if (ne + nk > cap || !brow || !bcol || !bval) return (ne + nk > cap) ? CIPC_E_CAPACITY : CIPC_OK;
+        if (which & 1) download_bsr(c, c.Ael, brow, bcol, bval, 0);
+        if (which & 2) download_bsr(c, c.Acon, brow, bcol, bval, ne);
+        return CIPC_OK;
+    });
+}
+
+int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double* d2, double* min_val) {
+    if (!ctx || !x || n_cs < 0 || (n_cs && !cs)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        upload_x(ctx, ctx->qx, x, 3 * (size_t)c.nV);
+        upload_cs(ctx, cs, n_cs);
+        double* dd = ctx->qout.ensure(std::max(n_cs, 1));
+        double m = min_dist2(c, ctx->qx.p, ctx->qC.p, n_cs, dd);
+        if (min_val) *min_val = m;
+        if (d2 && n_cs) {
+            CIPC_HIP(hipMemcpyAsync(d2, dd, n_cs * sizeof(double), hipMemcpyDeviceToHost, c.s));
+            CIPC_HIP(hipStreamSynchronize(c.s));
+        }
+        return CIPC_OK;
+    });
+}
+
+int cipc_ccd_stepsize(cipc_ctx* ctx, const double* x, const double* p, double alpha_in, double* alpha_out, double* alpha_clamped) {
+    if (!ctx || !x || !p || !alpha_out || !(alpha_in > 0)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        size_t n = 3 * (size_t)c.nV;
+        upload_x(ctx, ctx->qx, x, n);
+        upload_x(ctx, ctx->qp, p, n);
+        int z = 0;
+        double cl = 0;
+        *alpha_out = ccd_stepsize(c, ctx->qx.p, ctx->qp.p, alpha_in, &cl, &z);
+        if (alpha_clamped) *alpha_clamped = cl;
+        if (z) { c.lastError = "zero time of impact"; return CIPC_E_ZERO_TOI; }
+        return CIPC_OK;
+    });
+}
+
+int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, const double* b, double* sol,
+                      double tol, int32_t* iters, double* relres) {
+    if (!ctx || !x || !b || !sol || n_cs < 0 || (n_cs && !cs)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        size_t n = 3 * (size_t)c.nV;
+        upload_x(ctx, ctx->qx, x, n);
+        upload_x(ctx, ctx->qb, b, n);
+        upload_cs(ctx, cs, n_cs);
+        elastic_hessian(c, ctx->qx.p);
+        barrier_hessian(c, ctx->qx.p, ctx->qC.p, n_cs, kappa);
+        double* sd = ctx->qsol.ensure(n);
+        int it = 0;
+        double rr = 0;
+        int rc = pcg_solve(c, ctx->qb.p, sd, tol > 0 ? tol : c.P.pcgTol, c.P.pcgMaxIter, &it, &rr);
+        CIPC_HIP(hipMemcpyAsync(sol, sd, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        if (iters) *iters = it;
+        if (relres) *relres = rr;
+        if (rc == 1) { c.lastError = "matrix not positive definite"; return CIPC_E_NO_CONVERGENCE; }
+        if (rc == 2) { c.lastError = "PCG iteration limit"; return CIPC_E_NO_CONVERGENCE; }
+        return CIPC_OK;
+    });
+}
+
+}  // extern "C"

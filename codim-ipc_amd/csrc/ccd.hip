@@ -1,0 +1,463 @@
+// Intersection-free step size: FEM/IPC.h:1879-2244 (Compute_Intersection_Free_StepSize)
+// with the CCD spatial hash of Grid/SPATIAL_HASH.h:432-622 and the additive CCD of
+// Math/Distance/CCD.h:188-395.
+//
+// The reference's candidate pairs are those whose swept voxel ranges overlap on its
+// CCD voxel grid (voxel = mean boundary-edge length at x, origin = min corner of the
+// swept boundary nodes minus offset/2); that grid, the step clamp
+// (alpha /= alpha * mean|p| / voxel when > 1) and the overlap test are reproduced
+// exactly, so the candidate set, and with it the step size, matches the reference
+// rule.  Candidates are enumerated through cells of f x f x f voxels (f grown until the
+// cell table fits), then filtered by the exact voxel-range overlap.
+//
+// Each candidate runs the additive CCD in FP64 with the running per-primitive step as
+// its cut-off, as the reference does; every loop stops after at most ACCD_MAX_ITERS
+// advances (touching stencils would otherwise never end) with the conservative time
+// accumulated so far.  The global step is a min-reduction over positive doubles
+// through their bit patterns (atomicMin on uint64).
+#include "ctx.hpp"
+#include "dev_math.hpp"
+#include <algorithm>
+#include <cmath>
+
+namespace cipc {
+
+static constexpr long ACCD_MAX_ITERS = 1000000;
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ bool pt_ccd_broadphase(d3 p, d3 t0, d3 t1, d3 t2, d3 dp, d3 dt0, d3 dt1, d3 dt2, double dist) {
+    d3 pe = p + dp, q0 = t0 + dt0, q1 = t1 + dt1, q2 = t2 + dt2;
+    double maxp, minp, mx, mn;
+    maxp = dmax(p.x, pe.x); minp = dmin(p.x, pe.x);
+    mx = dmax(dmax(dmax(dmax(dmax(t0.x, t1.x), t2.x), q0.x), q1.x), q2.x);
+    mn = dmin(dmin(dmin(dmin(dmin(t0.x, t1.x), t2.x), q0.x), q1.x), q2.x);
+    if (minp - mx > dist || mn - maxp > dist) return false;
+    maxp = dmax(p.y, pe.y); minp = dmin(p.y, pe.y);
+    mx = dmax(dmax(dmax(dmax(dmax(t0.y, t1.y), t2.y), q0.y), q1.y), q2.y);
+    mn = dmin(dmin(dmin(dmin(dmin(t0.y, t1.y), t2.y), q0.y), q1.y), q2.y);
+    if (minp - mx > dist || mn - maxp > dist) return false;
+    maxp = dmax(p.z, pe.z); minp = dmin(p.z, pe.z);
+    mx = dmax(dmax(dmax(dmax(dmax(t0.z, t1.z), t2.z), q0.z), q1.z), q2.z);
+    mn = dmin(dmin(dmin(dmin(dmin(t0.z, t1.z), t2.z), q0.z), q1.z), q2.z);
+    if (minp - mx > dist || mn - maxp > dist) return false;
+    return true;
+}
+__device__ __forceinline__ bool ee_ccd_broadphase(d3 a0, d3 a1, d3 b0, d3 b1, d3 da0, d3 da1, d3 db0, d3 db1, double dist) {
+    d3 A2 = a0 + da0, A3 = a1 + da1, B2 = b0 + db0, B3 = b1 + db1;
+    double mxa, mna, mxb, mnb;
+    mxa = dmax(dmax(dmax(a0.x, a1.x), A2.x), A3.x); mna = dmin(dmin(dmin(a0.x, a1.x), A2.x), A3.x);
+    mxb = dmax(dmax(dmax(b0.x, b1.x), B2.x), B3.x); mnb = dmin(dmin(dmin(b0.x, b1.x), B2.x), B3.x);
+    if (mna - mxb > dist || mnb - mxa > dist) return false;
+    mxa = dmax(dmax(dmax(a0.y, a1.y), A2.y), A3.y); mna = dmin(dmin(dmin(a0.y, a1.y), A2.y), A3.y);
+    mxb = dmax(dmax(dmax(b0.y, b1.y), B2.y), B3.y); mnb = dmin(dmin(dmin(b0.y, b1.y), B2.y), B3.y);
+    if (mna - mxb > dist || mnb - mxa > dist) return false;
+    mxa = dmax(dmax(dmax(a0.z, a1.z), A2.z), A3.z); mna = dmin(dmin(dmin(a0.z, a1.z), A2.z), A3.z);
+    mxb = dmax(dmax(dmax(b0.z, b1.z), B2.z), B3.z); mnb = dmin(dmin(dmin(b0.z, b1.z), B2.z), B3.z);
+    if (mna - mxb > dist || mnb - mxa > dist) return false;
+    return true;
+}
+
+// CCD.h:280-328
+__device__ bool pt_accd(d3 p, d3 t0, d3 t1, d3 t2, d3 dp, d3 dt0, d3 dt1, d3 dt2, double eta, double th, double& toc) {
+    d3 mov = {(dt0.x + dt1.x + dt2.x + dp.x) / 4, (dt0.y + dt1.y + dt2.y + dp.y) / 4, (dt0.z + dt1.z + dt2.z + dp.z) / 4};
+    dt0 = dt0 - mov; dt1 = dt1 - mov; dt2 = dt2 - mov; dp = dp - mov;
+    double m2 = dmax(dmax(sq(dt0), sq(dt1)), sq(dt2));
+    double maxDispMag = sqrt(sq(dp)) + sqrt(m2);
+    if (maxDispMag == 0) return false;
+    double dist2_cur = dist2_pt_unclassified(p, t0, t1, t2);
+    double dist_cur = sqrt(dist2_cur);
+    double gap = eta * (dist2_cur - th * th) / (dist_cur + th);
+    double toc_prev = toc;
+    double t = 0;
+    for (long it = 0;; ++it) {
+        if (it >= ACCD_MAX_ITERS) { toc = t; return true; }
+        double lb = (1 - eta) * (dist2_cur - th * th) / ((dist_cur + th) * maxDispMag);
+        p = p + lb * dp; t0 = t0 + lb * dt0; t1 = t1 + lb * dt1; t2 = t2 + lb * dt2;
+        dist2_cur = dist2_pt_unclassified(p, t0, t1, t2);
+        dist_cur = sqrt(dist2_cur);
+        if (t != 0 && ((dist2_cur - th * th) / (dist_cur + th) < gap)) break;
+        t += lb;
+        if (t > toc_prev) { toc = t; return false; }
+    }
+    toc = t;
+    return true;
+}
+// CCD.h:331-395
+__device__ bool ee_accd(d3 ea0, d3 ea1, d3 eb0, d3 eb1, d3 dea0, d3 dea1, d3 deb0, d3 deb1, double eta, double th, double& toc) {
+    d3 mov = {(dea0.x + dea1.x + deb0.x + deb1.x) / 4, (dea0.y + dea1.y + deb0.y + deb1.y) / 4, (dea0.z + dea1.z + deb0.z + deb1.z) / 4};
+    dea0 = dea0 - mov; dea1 = dea1 - mov; deb0 = deb0 - mov; deb1 = deb1 - mov;
+    double maxDispMag = sqrt(dmax(sq(dea0), sq(dea1))) + sqrt(dmax(sq(deb0), sq(deb1)));
+    if (maxDispMag == 0) return false;
+    double th2 = th * th;
+    double dist2_cur = dist2_ee_unclassified(ea0, ea1, eb0, eb1);
+    double dFunc = dist2_cur - th2;
+    if (dFunc <= 0) {
+        double l[4] = {sq(ea0 - eb0), sq(ea0 - eb1), sq(ea1 - eb0), sq(ea1 - eb1)};
+        double m = l[0];
+        for (int i = 1; i < 4; ++i) if (l[i] < m) m = l[i];
+        dist2_cur = m; dFunc = dist2_cur - th2;
+    }
+    double dist_cur = sqrt(dist2_cur);
+    double gap = eta * dFunc / (dist_cur + th);
+    double toc_prev = toc;
+    double t = 0;
+    for (long it = 0;; ++it) {
+        if (it >= ACCD_MAX_ITERS) { toc = t; return true; }
+        double lb = (1 - eta) * dFunc / ((dist_cur + th) * maxDispMag);
+        ea0 = ea0 + lb * dea0; ea1 = ea1 + lb * dea1; eb0 = eb0 + lb * deb0; eb1 = eb1 + lb * deb1;
+        dist2_cur = dist2_ee_unclassified(ea0, ea1, eb0, eb1);
+        dFunc = dist2_cur - th2;
+        if (dFunc <= 0) {
+            double l[4] = {sq(ea0 - eb0), sq(ea0 - eb1), sq(ea1 - eb0), sq(ea1 - eb1)};
+            double m = l[0];
+            for (int i = 1; i < 4; ++i) if (l[i] < m) m = l[i];
+            dist2_cur = m; dFunc = dist2_cur - th2;
+        }
+        dist_cur = sqrt(dist2_cur);
+        if (t != 0 && (dFunc / (dist_cur + th) < gap)) break;
+        t += lb;
+        if (t > toc_prev) { toc = t; return false; }
+    }
+    toc = t;
+    return true;
+}
+
+// ------------------------------------------------------------------ clamp inputs
+__global__ void k_edge_len(const double* __restrict__ x, const int* __restrict__ bEdge, int nBE, double* __restrict__ len) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nBE) return;
+    len[e] = sqrt(sq(ld3(x, bEdge[e]) - ld3(x, bEdge[nBE + e])));
+}
+__global__ void k_pabs(const double* __restrict__ p, const int* __restrict__ bNode, int nBN, double* __restrict__ out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nBN) return;
+    int v = bNode[s];
+    out[s] = (fabs(p[3 * v]) + fabs(p[3 * v + 1])) + fabs(p[3 * v + 2]);
+}
+// swept bounding box of the boundary nodes (x and x + alpha p)
+__global__ void k_swept_partial(const double* __restrict__ x, const double* __restrict__ p, double alpha, const int* __restrict__ bNode,
+                                int nBN, double* __restrict__ part) {
+    __shared__ double lo[3][kBlock], hi[3][kBlock];
+    int t = threadIdx.x;
+    double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
+    for (int s = blockIdx.x * blockDim.x + t; s < nBN; s += gridDim.x * blockDim.x) {
+        int v = bNode[s];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double a = x[3 * v + d], b = x[3 * v + d] + alpha * p[3 * v + d];
+            double mn = dmin(a, b), mx = dmax(a, b);
+            l[d] = mn < l[d] ? mn : l[d]; h[d] = mx > h[d] ? mx : h[d];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { lo[d][t] = l[d]; hi[d][t] = h[d]; }
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (t < st)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                lo[d][t] = lo[d][t + st] < lo[d][t] ? lo[d][t + st] : lo[d][t];
+                hi[d][t] = hi[d][t + st] > hi[d][t] ? hi[d][t + st] : hi[d][t];
+            }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int d = 0; d < 3; ++d) { part[6 * blockIdx.x + d] = lo[d][0]; part[6 * blockIdx.x + 3 + d] = hi[d][0]; }
+}
+__global__ void k_minmax_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
+    for (int b = 0; b < nb; ++b)
+        for (int d = 0; d < 3; ++d) {
+            l[d] = part[6 * b + d] < l[d] ? part[6 * b + d] : l[d];
+            h[d] = part[6 * b + 3 + d] > h[d] ? part[6 * b + 3 + d] : h[d];
+        }
+    for (int d = 0; d < 3; ++d) { out[d] = l[d]; out[3 + d] = h[d]; }
+}
+
+struct VoxGrid { double lb[3]; double inv; double halfXi; double alpha; int f; int edim[3]; };
+
+// per boundary node: voxel index range of its swept, offset-padded box (SPATIAL_HASH.h:546-553)
+__global__ void k_node_vox(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bNode, int nBN, VoxGrid G,
+                           int* __restrict__ vlo, int* __restrict__ vhi, int nV) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nBN) return;
+    int v = bNode[s];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double a = x[3 * v + d], b = x[3 * v + d] + G.alpha * p[3 * v + d];
+        double mn = dmin(a, b) - G.halfXi, mx = dmax(a, b) + G.halfXi;
+        vlo[d * nV + v] = (int)floor((mn - G.lb[d]) * G.inv);
+        vhi[d * nV + v] = (int)floor((mx - G.lb[d]) * G.inv);
+    }
+}
+#pragma clang fp contract(on)
+
+__device__ __forceinline__ int ecell(int vox, int f, int dim) {
+    int c = vox >= 0 ? vox / f : 0;
+    return c < dim ? c : dim - 1;
+}
+// primitive voxel range = union of its nodes' ranges; count of enumeration cells
+__global__ void k_prim_vox(const int* __restrict__ verts, int n, int nv, const int* __restrict__ vlo, const int* __restrict__ vhi, int nV,
+                           VoxGrid G, int* __restrict__ plo, int* __restrict__ phi, uint32_t* __restrict__ cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c = 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int lo = 1 << 30, hi = -(1 << 30);
+        for (int k = 0; k < nv; ++k) {
+            int v = verts[(long)k * n + i];
+            lo = min(lo, vlo[d * nV + v]);
+            hi = max(hi, vhi[d * nV + v]);
+        }
+        plo[(long)d * n + i] = lo;
+        phi[(long)d * n + i] = hi;
+        c *= (uint32_t)(ecell(hi, G.f, G.edim[d]) - ecell(lo, G.f, G.edim[d]) + 1);
+    }
+    cnt[i] = c;
+}
+__global__ void k_prim_vox_emit(int n, const int* __restrict__ plo, const int* __restrict__ phi, VoxGrid G, const uint32_t* __restrict__ off,
+                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int a[3], b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { a[d] = ecell(plo[(long)d * n + i], G.f, G.edim[d]); b[d] = ecell(phi[(long)d * n + i], G.f, G.edim[d]); }
+    uint32_t o = off[i];
+    for (int iz = a[2]; iz <= b[2]; ++iz)
+        for (int iy = a[1]; iy <= b[1]; ++iy)
+            for (int ix = a[0]; ix <= b[0]; ++ix) {
+                keys[o] = (uint64_t)ix + (uint64_t)G.edim[0] * ((uint64_t)iy + (uint64_t)G.edim[1] * (uint64_t)iz);
+                vals[o] = (uint32_t)i;
+                ++o;
+            }
+}
+__global__ void k_cell_bounds2(const uint64_t* __restrict__ keys, long n, uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) start[k] = (uint32_t)i;
+    if (i == n - 1 || keys[i + 1] != k) end[k] = (uint32_t)(i + 1);
+}
+
+__device__ __forceinline__ void atomic_min_pos(unsigned long long* a, double v) {
+    atomicMin(a, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void k_ccd_pt(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bNode, int nBN,
+                         const int* __restrict__ tri, int nT, const uint8_t* __restrict__ dbc, const int* __restrict__ vlo,
+                         const int* __restrict__ vhi, int nV, const int* __restrict__ tlo, const int* __restrict__ thi, VoxGrid G,
+                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals,
+                         double alpha0, double xi, unsigned long long* __restrict__ gmin, uint32_t* __restrict__ zero) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nBN) return;
+    int vI = bNode[s];
+    int nlo[3], nhi[3], qlo[3], qhi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        nlo[d] = vlo[d * nV + vI]; nhi[d] = vhi[d * nV + vI];
+        qlo[d] = ecell(nlo[d], G.f, G.edim[d]); qhi[d] = ecell(nhi[d], G.f, G.edim[d]);
+    }
+    d3 P = ld3(x, vI), dP = ld3(p, vI);
+    double largest = alpha0;
+    for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
+        for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
+            for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
+                uint64_t cell = (uint64_t)ix + (uint64_t)G.edim[0] * ((uint64_t)iy + (uint64_t)G.edim[1] * (uint64_t)iz);
+                uint32_t st = start[cell], en = end[cell];
+                for (uint32_t k = st; k < en; ++k) {
+                    int t = (int)vals[k];
+                    int l0 = tlo[t], l1 = tlo[nT + t], l2 = tlo[2 * nT + t];
+                    int h0 = thi[t], h1 = thi[nT + t], h2 = thi[2 * nT + t];
+                    if (ix != max(qlo[0], ecell(l0, G.f, G.edim[0])) || iy != max(qlo[1], ecell(l1, G.f, G.edim[1])) ||
+                        iz != max(qlo[2], ecell(l2, G.f, G.edim[2])))
+                        continue;
+                    if (nlo[0] > h0 || l0 > nhi[0] || nlo[1] > h1 || l1 > nhi[1] || nlo[2] > h2 || l2 > nhi[2]) continue;
+                    int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+                    if (vI == a || vI == b || vI == c) continue;
+                    if (dbc[vI] && dbc[a] && dbc[b] && dbc[c]) continue;
+                    d3 t0 = ld3(x, a), t1 = ld3(x, b), t2 = ld3(x, c);
+                    d3 d0 = ld3(p, a), d1 = ld3(p, b), d2 = ld3(p, c);
+                    if (!pt_ccd_broadphase(P, t0, t1, t2, dP, d0, d1, d2, xi)) continue;
+                    double la = largest;
+                    if (pt_accd(P, t0, t1, t2, dP, d0, d1, d2, 0.1, xi, la)) {
+                        if (largest > la) largest = la;
+                        if (la == 0) atomicOr(zero, 1u);
+                    }
+                }
+            }
+    if (largest < alpha0) atomic_min_pos(gmin, largest);
+}
+
+__global__ void k_ccd_ee(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bEdge, int nBE,
+                         const uint8_t* __restrict__ dbc, const int* __restrict__ elo, const int* __restrict__ ehi, VoxGrid G,
+                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals,
+                         double alpha0, double xi, unsigned long long* __restrict__ gmin, uint32_t* __restrict__ zero) {
+    int eI = blockIdx.x * blockDim.x + threadIdx.x;
+    if (eI >= nBE) return;
+    int a0 = bEdge[eI], a1 = bEdge[nBE + eI];
+    int nlo[3], nhi[3], qlo[3], qhi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        nlo[d] = elo[(long)d * nBE + eI]; nhi[d] = ehi[(long)d * nBE + eI];
+        qlo[d] = ecell(nlo[d], G.f, G.edim[d]); qhi[d] = ecell(nhi[d], G.f, G.edim[d]);
+    }
+    d3 A0 = ld3(x, a0), A1 = ld3(x, a1), dA0 = ld3(p, a0), dA1 = ld3(p, a1);
+    double largest = alpha0;
+    for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
+        for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
+            for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
+                uint64_t cell = (uint64_t)ix + (uint64_t)G.edim[0] * ((uint64_t)iy + (uint64_t)G.edim[1] * (uint64_t)iz);
+                uint32_t st = start[cell], en = end[cell];
+                for (uint32_t k = st; k < en; ++k) {
+                    int eJ = (int)vals[k];
+                    if (eJ <= eI) continue;
+                    int l0 = elo[eJ], l1 = elo[nBE + eJ], l2 = elo[2l * nBE + eJ];
+                    int h0 = ehi[eJ], h1 = ehi[nBE + eJ], h2 = ehi[2l * nBE + eJ];
+                    if (ix != max(qlo[0], ecell(l0, G.f, G.edim[0])) || iy != max(qlo[1], ecell(l1, G.f, G.edim[1])) ||
+                        iz != max(qlo[2], ecell(l2, G.f, G.edim[2])))
+                        continue;
+                    if (nlo[0] > h0 || l0 > nhi[0] || nlo[1] > h1 || l1 > nhi[1] || nlo[2] > h2 || l2 > nhi[2]) continue;
+                    int b0 = bEdge[eJ], b1 = bEdge[nBE + eJ];
+                    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
+                    if (dbc[a0] && dbc[a1] && dbc[b0] && dbc[b1]) continue;
+                    d3 B0 = ld3(x, b0), B1 = ld3(x, b1), dB0 = ld3(p, b0), dB1 = ld3(p, b1);
+                    if (!ee_ccd_broadphase(A0, A1, B0, B1, dA0, dA1, dB0, dB1, xi)) continue;
+                    double la = largest;
+                    if (ee_accd(A0, A1, B0, B1, dA0, dA1, dB0, dB1, 0.1, xi, la)) {
+                        if (largest > la) largest = la;
+                        if (la == 0) atomicOr(zero, 1u);
+                    }
+                }
+            }
+    if (largest < alpha0) atomic_min_pos(gmin, largest);
+}
+
+static int bits_for2(uint64_t n) {
+    int b = 1;
+    while (b < 64 && (1ull << b) < n) ++b;
+    return b;
+}
+
+static long build_vox_cells(Ctx& c, const int* verts, int n, int nv, const int* vlo, const int* vhi, const VoxGrid& G, DevBuf<int>& plo,
+                            DevBuf<int>& phi, DevBuf<uint64_t>& keys, DevBuf<uint32_t>& vals, DevBuf<uint32_t>& start,
+                            DevBuf<uint32_t>& end) {
+    long nCells = (long)G.edim[0] * G.edim[1] * G.edim[2];
+    start.ensure(nCells);
+    end.ensure(nCells);
+    CIPC_HIP(hipMemsetAsync(start.p, 0, nCells * sizeof(uint32_t), c.s));
+    CIPC_HIP(hipMemsetAsync(end.p, 0, nCells * sizeof(uint32_t), c.s));
+    if (n == 0) return 0;
+    plo.ensure(3l * n);
+    phi.ensure(3l * n);
+    uint32_t* cnt = c.u32a.ensure(n + 1);
+    uint32_t* off = c.u32b.ensure(n + 1);
+    hipLaunchKernelGGL(k_prim_vox, dim3(nblocks(n)), dim3(kBlock), 0, c.s, verts, n, nv, vlo, vhi, c.nV, G, plo.p, phi.p, cnt);
+    CIPC_LAUNCH_CHECK();
+    scan_exclusive(cnt, off, n, c.counters.p + 9, c.pw, c.s);
+    CIPC_HIP(hipMemcpyAsync(c.hCnt + 9, c.counters.p + 9, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    long total = c.hCnt[9];
+    keys.ensure(total);
+    vals.ensure(total);
+    hipLaunchKernelGGL(k_prim_vox_emit, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, plo.p, phi.p, G, off, keys.p, vals.p);
+    CIPC_LAUNCH_CHECK();
+    radix_sort_pairs(keys.p, vals.p, total, bits_for2((uint64_t)nCells), c.pw, c.s);
+    hipLaunchKernelGGL(k_cell_bounds2, dim3(nblocks(total)), dim3(kBlock), 0, c.s, keys.p, total, start.p, end.p);
+    CIPC_LAUNCH_CHECK();
+    return total;
+}
+
+double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, double* clamped, int* zeroToi) {
+    if (zeroToi) *zeroToi = 0;
+    const double xi = c.P.xi;
+    // step clamp, SPATIAL_HASH.h:454-482 (deterministic chunked sums)
+    double voxelSize = 1.0;
+    {
+        double* len = c.dscr2.ensure(std::max(c.nBE, c.nBN) + 1);
+        if (c.nBE) {
+            hipLaunchKernelGGL(k_edge_len, dim3(nblocks(c.nBE)), dim3(kBlock), 0, c.s, x, c.bEdge.p, c.nBE, len);
+            CIPC_LAUNCH_CHECK();
+            chunk_sum(len, c.nBE, c.scal.p + 8, c.pw, c.s);
+        }
+        double* pa = c.dscr.ensure(c.nBN + 1);
+        hipLaunchKernelGGL(k_pabs, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, p, c.bNode.p, c.nBN, pa);
+        CIPC_LAUNCH_CHECK();
+        chunk_sum(pa, c.nBN, c.scal.p + 9, c.pw, c.s);
+        CIPC_HIP(hipMemcpyAsync(c.hScal + 8, c.scal.p + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        if (c.nBE) voxelSize *= c.hScal[8] / c.nBE;
+        double pSize = c.hScal[9] / (c.nBN * 3);
+        double spanSize = alpha * pSize / voxelSize;
+        if (spanSize > 1) alpha /= spanSize;
+    }
+    if (clamped) *clamped = alpha;
+    // voxel grid, SPATIAL_HASH.h:484-544
+    VoxGrid G;
+    {
+        int nb = std::min(nblocks(c.nBN), 1024);
+        double* part = c.dscr2.ensure(6l * nb);
+        hipLaunchKernelGGL(k_swept_partial, dim3(nb), dim3(kBlock), 0, c.s, x, p, alpha, c.bNode.p, c.nBN, part);
+        CIPC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, c.s, part, nb, c.scal.p + 10);
+        CIPC_LAUNCH_CHECK();
+        CIPC_HIP(hipMemcpyAsync(c.hScal + 10, c.scal.p + 10, 6 * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        double lb[3], range[3];
+        for (int d = 0; d < 3; ++d) {
+            lb[d] = c.hScal[10 + d] - xi / 2;
+            double rt = c.hScal[13 + d] + xi / 2;
+            range[d] = rt - lb[d];
+        }
+        double inv = 1.0 / voxelSize;
+        double vcl = 1;
+        for (int d = 0; d < 3; ++d) vcl *= std::max(1.0, std::ceil(range[d] * inv));
+        if (vcl > 1e9) { voxelSize *= std::pow(vcl / 1.0e9, 1.0 / 3); inv = 1.0 / voxelSize; }
+        for (int d = 0; d < 3; ++d) G.lb[d] = lb[d];
+        G.inv = inv;
+        G.halfXi = xi / 2;
+        G.alpha = alpha;
+        long vc[3];
+        for (int d = 0; d < 3; ++d) vc[d] = std::max(1L, (long)std::ceil(range[d] * inv)) + 1;
+        G.f = 1;
+        for (;;) {
+            double cells = 1;
+            for (int d = 0; d < 3; ++d) cells *= (double)((vc[d] + G.f - 1) / G.f);
+            if (cells <= (double)(1 << 23)) break;
+            G.f++;
+        }
+        for (int d = 0; d < 3; ++d) G.edim[d] = (int)((vc[d] + G.f - 1) / G.f);
+    }
+    int* vlo = c.i32a.ensure(3l * c.nV);
+    int* vhi = c.i32b.ensure(3l * c.nV);
+    hipLaunchKernelGGL(k_node_vox, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, p, c.bNode.p, c.nBN, G, vlo, vhi, c.nV);
+    CIPC_LAUNCH_CHECK();
+    static_assert(sizeof(unsigned long long) == sizeof(double), "");
+    unsigned long long* gmin = (unsigned long long*)(c.scal.p + 24);
+    double a0 = alpha;
+    CIPC_HIP(hipMemcpyAsync(gmin, &a0, sizeof(double), hipMemcpyHostToDevice, c.s));
+    CIPC_HIP(hipMemsetAsync(c.counters.p + 13, 0, sizeof(uint32_t), c.s));
+    // point-triangle
+    build_vox_cells(c, c.tri.p, c.nT, 3, vlo, vhi, G, c.ccdTlo, c.ccdThi, c.k64a, c.u32c, c.cellStart, c.cellEnd);
+    if (c.nBN && c.nT) {
+        hipLaunchKernelGGL(k_ccd_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, p, c.bNode.p, c.nBN, c.tri.p, c.nT, c.dbc.p, vlo, vhi, c.nV,
+                           c.ccdTlo.p, c.ccdThi.p, G, c.cellStart.p, c.cellEnd.p, c.u32c.p, alpha, xi, gmin, c.counters.p + 13);
+        CIPC_LAUNCH_CHECK();
+    }
+    // edge-edge
+    build_vox_cells(c, c.bEdge.p, c.nBE, 2, vlo, vhi, G, c.ccdElo, c.ccdEhi, c.k64b, c.u32d, c.ecellStart, c.ecellEnd);
+    if (c.nBE) {
+        hipLaunchKernelGGL(k_ccd_ee, dim3(nblocks(c.nBE)), dim3(kBlock), 0, c.s, x, p, c.bEdge.p, c.nBE, c.dbc.p, c.ccdElo.p, c.ccdEhi.p, G,
+                           c.ecellStart.p, c.ecellEnd.p, c.u32d.p, alpha, xi, gmin, c.counters.p + 13);
+        CIPC_LAUNCH_CHECK();
+    }
+    CIPC_HIP(hipMemcpyAsync(c.hScal + 24, gmin, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipMemcpyAsync(c.hCnt + 13, c.counters.p + 13, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    if (zeroToi && c.hCnt[13]) *zeroToi = 1;
+    return c.hScal[24];
+}
+
+}  // namespace cipc

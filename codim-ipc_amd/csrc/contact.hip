@@ -1,0 +1,833 @@
+// Contact kernels: spatial-grid broad phase, point-triangle and edge-edge narrow
+// phase with compaction, PP/PE deduplication with multiplicity, barrier energy /
+// gradient / Hessian (per-stencil PSD projection), minimum distance.
+//
+// Reference: FEM/IPC.h:19-740 (Compute_Constraint_Set), :742-941, :943-1256,
+// :1258-1731 (Compute_Barrier{,_Gradient,_Hessian}), :2246-2388 (Compute_Min_Dist2).
+//
+// Broad phase: a uniform grid (cell = mean rest boundary-edge length, grown so the
+// grid has at most 2^23 cells) holds each triangle / edge in every cell its AABB
+// touches; entries are sorted by cell (stable radix sort) and each cell's run is
+// found through dense start/end tables.  A query visits the cells of its box and
+// accepts a candidate only in the first cell common to both boxes, so every pair is
+// tested once; indices are clamped into the grid, which keeps the search complete
+// for points outside it.  The active set is defined by the reference's predicates
+// (AABB filter with d_hat + offset, region type, d^2 < (d_hat + offset)^2), so it does
+// not depend on the grid.
+//
+// Output order (deterministic): point-triangle by (boundary node, triangle),
+// edge-edge family by (edge, edge), then deduplicated PP/PE entries in the
+// lexicographic int4 order of the reference's std::map.
+#include "ctx.hpp"
+#include "dev_math.hpp"
+#include <algorithm>
+#include <cmath>
+
+namespace cipc {
+
+__device__ __forceinline__ int cell_of(double v, double lb, double inv, int dim) {
+    double f = floor((v - lb) * inv);
+    if (!(f >= 0.0)) return 0;
+    if (f >= (double)(dim - 1)) return dim - 1;
+    return (int)f;
+}
+
+// ------------------------------------------------------------------ grid build
+// kind 0: triangles (tri SoA), kind 1: boundary edges (bEdge SoA)
+__global__ void k_prim_range(const double* __restrict__ x, const int* __restrict__ verts, int n, int nv, GridParams G,
+                             int* __restrict__ lo, uint32_t* __restrict__ count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int k = 0; k < nv; ++k) {
+        int v = verts[(long)k * n + i];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { double c = x[3 * v + d]; mn[d] = c < mn[d] ? c : mn[d]; mx[d] = c > mx[d] ? c : mx[d]; }
+    }
+    uint32_t cnt = 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int a = cell_of(mn[d], G.lb[d], G.inv, G.dim[d]), b = cell_of(mx[d], G.lb[d], G.inv, G.dim[d]);
+        lo[(long)d * n + i] = a;
+        cnt *= (uint32_t)(b - a + 1);
+    }
+    count[i] = cnt;
+}
+__global__ void k_prim_emit(const double* __restrict__ x, const int* __restrict__ verts, int n, int nv, GridParams G,
+                            const uint32_t* __restrict__ off, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int k = 0; k < nv; ++k) {
+        int v = verts[(long)k * n + i];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { double c = x[3 * v + d]; mn[d] = c < mn[d] ? c : mn[d]; mx[d] = c > mx[d] ? c : mx[d]; }
+    }
+    int a[3], b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { a[d] = cell_of(mn[d], G.lb[d], G.inv, G.dim[d]); b[d] = cell_of(mx[d], G.lb[d], G.inv, G.dim[d]); }
+    uint32_t o = off[i];
+    for (int iz = a[2]; iz <= b[2]; ++iz)
+        for (int iy = a[1]; iy <= b[1]; ++iy)
+            for (int ix = a[0]; ix <= b[0]; ++ix) {
+                keys[o] = (uint64_t)ix + (uint64_t)G.dim[0] * ((uint64_t)iy + (uint64_t)G.dim[1] * (uint64_t)iz);
+                vals[o] = (uint32_t)i;
+                ++o;
+            }
+}
+__global__ void k_cell_bounds(const uint64_t* __restrict__ keys, long n, uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) start[k] = (uint32_t)i;
+    if (i == n - 1 || keys[i + 1] != k) end[k] = (uint32_t)(i + 1);
+}
+
+static int bits_for(uint64_t n) {
+    int b = 1;
+    while (b < 64 && (1ull << b) < n) ++b;
+    return b;
+}
+
+// builds the sorted cell lists of one primitive kind; returns number of entries
+static long build_cells(Ctx& c, const double* x, const int* verts, int n, int nv, const GridParams& G, DevBuf<int>& lo,
+                        DevBuf<uint64_t>& keys, DevBuf<uint32_t>& vals, DevBuf<uint32_t>& start, DevBuf<uint32_t>& end) {
+    long nCells = (long)G.dim[0] * G.dim[1] * G.dim[2];
+    start.ensure(nCells);
+    end.ensure(nCells);
+    CIPC_HIP(hipMemsetAsync(start.p, 0, nCells * sizeof(uint32_t), c.s));
+    CIPC_HIP(hipMemsetAsync(end.p, 0, nCells * sizeof(uint32_t), c.s));
+    if (n == 0) return 0;
+    lo.ensure((size_t)3 * n);
+    uint32_t* cnt = c.u32a.ensure(n + 1);
+    uint32_t* off = c.u32b.ensure(n + 1);
+    hipLaunchKernelGGL(k_prim_range, dim3(nblocks(n)), dim3(kBlock), 0, c.s, x, verts, n, nv, G, lo.p, cnt);
+    CIPC_LAUNCH_CHECK();
+    scan_exclusive(cnt, off, n, c.counters.p + 8, c.pw, c.s);
+    CIPC_HIP(hipMemcpyAsync(c.hCnt + 8, c.counters.p + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    long total = c.hCnt[8];
+    keys.ensure(total);
+    vals.ensure(total);
+    hipLaunchKernelGGL(k_prim_emit, dim3(nblocks(n)), dim3(kBlock), 0, c.s, x, verts, n, nv, G, off, keys.p, vals.p);
+    CIPC_LAUNCH_CHECK();
+    radix_sort_pairs(keys.p, vals.p, total, bits_for((uint64_t)nCells), c.pw, c.s);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(nblocks(total)), dim3(kBlock), 0, c.s, keys.p, total, start.p, end.p);
+    CIPC_LAUNCH_CHECK();
+    return total;
+}
+
+// ------------------------------------------------------------------ narrow phase
+struct CSOut {
+    uint64_t* key; int* c4; uint32_t* cnt; uint32_t cap;   // kept entries
+    uint64_t* dkey; uint32_t* dcnt; uint32_t dcap;          // PP/PE entries to deduplicate
+};
+
+__device__ __forceinline__ uint64_t dup_key(int c0, int c1, int c2, int nV) {
+    // lexicographic order of (c0 < 0, c1 >= 0, c2 >= -1) as signed ints
+    uint64_t a = (uint64_t)(nV - 1 - (-c0 - 1)), b = (uint64_t)c1, d = (uint64_t)(c2 + 1);
+    return (a * (uint64_t)nV + b) * (uint64_t)(nV + 1) + d;
+}
+__device__ __forceinline__ void emit_kept(const CSOut& o, uint64_t key, int c0, int c1, int c2, int c3) {
+    uint32_t i = atomicAdd(o.cnt, 1u);
+    if (i < o.cap) {
+        o.key[i] = key;
+        o.c4[4 * (long)i] = c0; o.c4[4 * (long)i + 1] = c1; o.c4[4 * (long)i + 2] = c2; o.c4[4 * (long)i + 3] = c3;
+    }
+}
+__device__ __forceinline__ void emit_dup(const CSOut& o, uint64_t key) {
+    uint32_t i = atomicAdd(o.dcnt, 1u);
+    if (i < o.dcap) o.dkey[i] = key;
+}
+
+#pragma clang fp contract(off)
+__device__ __forceinline__ bool pt_cd_broadphase(d3 p, d3 t0, d3 t1, d3 t2, double dist) {
+    double mx, mn;
+    mx = dmax(dmax(t0.x, t1.x), t2.x); mn = dmin(dmin(t0.x, t1.x), t2.x);
+    if (p.x - mx > dist || mn - p.x > dist) return false;
+    mx = dmax(dmax(t0.y, t1.y), t2.y); mn = dmin(dmin(t0.y, t1.y), t2.y);
+    if (p.y - mx > dist || mn - p.y > dist) return false;
+    mx = dmax(dmax(t0.z, t1.z), t2.z); mn = dmin(dmin(t0.z, t1.z), t2.z);
+    if (p.z - mx > dist || mn - p.z > dist) return false;
+    return true;
+}
+__device__ __forceinline__ bool ee_cd_broadphase(d3 a0, d3 a1, d3 b0, d3 b1, double dist) {
+    if (dmin(a0.x, a1.x) - dmax(b0.x, b1.x) > dist || dmin(b0.x, b1.x) - dmax(a0.x, a1.x) > dist) return false;
+    if (dmin(a0.y, a1.y) - dmax(b0.y, b1.y) > dist || dmin(b0.y, b1.y) - dmax(a0.y, a1.y) > dist) return false;
+    if (dmin(a0.z, a1.z) - dmax(b0.z, b1.z) > dist || dmin(b0.z, b1.z) - dmax(a0.z, a1.z) > dist) return false;
+    return true;
+}
+
+__global__ void k_query_pt(const double* __restrict__ x, const int* __restrict__ bNode, int nBN, const int* __restrict__ tri,
+                           int nT, const uint8_t* __restrict__ dbc, GridParams G, const uint32_t* __restrict__ start,
+                           const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals, const int* __restrict__ triLo,
+                           double dHat, double dHat2, int nV, CSOut o) {
+    int svI = blockIdx.x * blockDim.x + threadIdx.x;
+    if (svI >= nBN) return;
+    int vI = bNode[svI];
+    d3 p = ld3(x, vI);
+    int qlo[3], qhi[3];
+    double pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { qlo[d] = cell_of(pc[d] - dHat, G.lb[d], G.inv, G.dim[d]); qhi[d] = cell_of(pc[d] + dHat, G.lb[d], G.inv, G.dim[d]); }
+    for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
+        for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
+            for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
+                uint64_t cell = (uint64_t)ix + (uint64_t)G.dim[0] * ((uint64_t)iy + (uint64_t)G.dim[1] * (uint64_t)iz);
+                uint32_t s = start[cell], e = end[cell];
+                for (uint32_t k = s; k < e; ++k) {
+                    int t = (int)vals[k];
+                    int l0 = triLo[t], l1 = triLo[nT + t], l2 = triLo[2 * nT + t];
+                    if (ix != max(qlo[0], l0) || iy != max(qlo[1], l1) || iz != max(qlo[2], l2)) continue;
+                    int a = tri[t], b = tri[nT + t], cc = tri[2 * nT + t];
+                    if (vI == a || vI == b || vI == cc) continue;
+                    if (dbc[vI] && dbc[a] && dbc[b] && dbc[cc]) continue;
+                    d3 t0 = ld3(x, a), t1 = ld3(x, b), t2 = ld3(x, cc);
+                    if (!pt_cd_broadphase(p, t0, t1, t2, dHat)) continue;
+                    int ty = pt_type(p, t0, t1, t2);
+                    double d;
+                    int c1 = -1, c2 = -1;
+                    switch (ty) {
+                        case 0: d = dist2_pp(p, t0); c1 = a; break;
+                        case 1: d = dist2_pp(p, t1); c1 = b; break;
+                        case 2: d = dist2_pp(p, t2); c1 = cc; break;
+                        case 3: d = dist2_pe(p, t0, t1); c1 = a; c2 = b; break;
+                        case 4: d = dist2_pe(p, t1, t2); c1 = b; c2 = cc; break;
+                        case 5: d = dist2_pe(p, t2, t0); c1 = cc; c2 = a; break;
+                        default: d = dist2_pt(p, t0, t1, t2); break;
+                    }
+                    if (!(d < dHat2)) continue;
+                    if (ty == 6) emit_kept(o, (uint64_t)svI * (uint64_t)nT + (uint64_t)t, -vI - 1, a, b, cc);
+                    else emit_dup(o, dup_key(-vI - 1, c1, c2, nV));
+                }
+            }
+}
+
+__global__ void k_query_ee(const double* __restrict__ x, const double* __restrict__ X0, const int* __restrict__ bEdge, int nBE,
+                           const uint8_t* __restrict__ dbc, GridParams G, const uint32_t* __restrict__ start,
+                           const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals, const int* __restrict__ edgeLo,
+                           double dHat, double dHat2, int nV, CSOut o) {
+    int eI = blockIdx.x * blockDim.x + threadIdx.x;
+    if (eI >= nBE) return;
+    int a0 = bEdge[eI], a1 = bEdge[nBE + eI];
+    d3 ea0 = ld3(x, a0), ea1 = ld3(x, a1);
+    d3 ra0 = ld3(X0, a0), ra1 = ld3(X0, a1);
+    int qlo[3], qhi[3];
+    double mn[3] = {dmin(ea0.x, ea1.x), dmin(ea0.y, ea1.y), dmin(ea0.z, ea1.z)};
+    double mx[3] = {dmax(ea0.x, ea1.x), dmax(ea0.y, ea1.y), dmax(ea0.z, ea1.z)};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { qlo[d] = cell_of(mn[d] - dHat, G.lb[d], G.inv, G.dim[d]); qhi[d] = cell_of(mx[d] + dHat, G.lb[d], G.inv, G.dim[d]); }
+    for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
+        for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
+            for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
+                uint64_t cell = (uint64_t)ix + (uint64_t)G.dim[0] * ((uint64_t)iy + (uint64_t)G.dim[1] * (uint64_t)iz);
+                uint32_t s = start[cell], e = end[cell];
+                for (uint32_t k = s; k < e; ++k) {
+                    int eJ = (int)vals[k];
+                    if (eJ <= eI) continue;
+                    int l0 = edgeLo[eJ], l1 = edgeLo[nBE + eJ], l2 = edgeLo[2 * nBE + eJ];
+                    if (ix != max(qlo[0], l0) || iy != max(qlo[1], l1) || iz != max(qlo[2], l2)) continue;
+                    int b0 = bEdge[eJ], b1 = bEdge[nBE + eJ];
+                    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
+                    if (dbc[a0] && dbc[a1] && dbc[b0] && dbc[b1]) continue;
+                    d3 eb0 = ld3(x, b0), eb1 = ld3(x, b1);
+                    if (!ee_cd_broadphase(ea0, ea1, eb0, eb1, dHat)) continue;
+                    double crossN = ee_cross_norm2(ea0, ea1, eb0, eb1);
+                    double eps = ee_mollifier_threshold(ra0, ra1, ld3(X0, b0), ld3(X0, b1));
+                    bool moll = crossN < eps;
+                    double d;
+                    int c[4];
+                    switch (ee_type(ea0, ea1, eb0, eb1)) {
+                        case 0: d = dist2_pp(ea0, eb0);
+                            if (moll) { c[0] = a0; c[1] = b0; c[2] = -a1 - 1; c[3] = -b1 - 1; } else { c[0] = -a0 - 1; c[1] = b0; c[2] = -1; c[3] = -1; } break;
+                        case 1: d = dist2_pp(ea0, eb1);
+                            if (moll) { c[0] = a0; c[1] = b1; c[2] = -a1 - 1; c[3] = -b0 - 1; } else { c[0] = -a0 - 1; c[1] = b1; c[2] = -1; c[3] = -1; } break;
+                        case 2: d = dist2_pe(ea0, eb0, eb1);
+                            if (moll) { c[0] = a0; c[1] = b0; c[2] = b1; c[3] = -a1 - 1; } else { c[0] = -a0 - 1; c[1] = b0; c[2] = b1; c[3] = -1; } break;
+                        case 3: d = dist2_pp(ea1, eb0);
+                            if (moll) { c[0] = a1; c[1] = b0; c[2] = -a0 - 1; c[3] = -b1 - 1; } else { c[0] = -a1 - 1; c[1] = b0; c[2] = -1; c[3] = -1; } break;
+                        case 4: d = dist2_pp(ea1, eb1);
+                            if (moll) { c[0] = a1; c[1] = b1; c[2] = -a0 - 1; c[3] = -b0 - 1; } else { c[0] = -a1 - 1; c[1] = b1; c[2] = -1; c[3] = -1; } break;
+                        case 5: d = dist2_pe(ea1, eb0, eb1);
+                            if (moll) { c[0] = a1; c[1] = b0; c[2] = b1; c[3] = -a0 - 1; } else { c[0] = -a1 - 1; c[1] = b0; c[2] = b1; c[3] = -1; } break;
+                        case 6: d = dist2_pe(eb0, ea0, ea1);
+                            if (moll) { c[0] = b0; c[1] = a0; c[2] = a1; c[3] = -b1 - 1; } else { c[0] = -b0 - 1; c[1] = a0; c[2] = a1; c[3] = -1; } break;
+                        case 7: d = dist2_pe(eb1, ea0, ea1);
+                            if (moll) { c[0] = b1; c[1] = a0; c[2] = a1; c[3] = -b0 - 1; } else { c[0] = -b1 - 1; c[1] = a0; c[2] = a1; c[3] = -1; } break;
+                        default: d = dist2_ee(ea0, ea1, eb0, eb1);
+                            if (moll) { c[0] = a0; c[1] = a1; c[2] = -b0 - 1; c[3] = b1; } else { c[0] = a0; c[1] = a1; c[2] = b0; c[3] = b1; } break;
+                    }
+                    if (!(d < dHat2)) continue;
+                    if (c[0] >= 0) emit_kept(o, (uint64_t)eI * (uint64_t)nBE + (uint64_t)eJ, c[0], c[1], c[2], c[3]);
+                    else emit_dup(o, dup_key(c[0], c[1], c[2], nV));
+                }
+            }
+}
+#pragma clang fp contract(on)
+
+// gather sorted kept entries into the SoA constraint array
+__global__ void k_gather_c4(const uint32_t* __restrict__ idx, const int* __restrict__ c4, int n, int* __restrict__ C, int nC,
+                            int offset) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long s = idx[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) C[(long)k * nC + offset + i] = c4[4 * s + k];
+}
+__global__ void k_run_flags(const uint64_t* __restrict__ key, int n, uint32_t* __restrict__ head) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    head[i] = (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
+}
+__global__ void k_run_emit(const uint64_t* __restrict__ key, int n, const uint32_t* __restrict__ pos, int* __restrict__ C, int nC,
+                           int offset, int nV) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!(i == 0 || key[i] != key[i - 1])) return;
+    int j = i + 1;
+    while (j < n && key[j] == key[i]) ++j;
+    uint64_t k = key[i];
+    uint64_t d = k % (uint64_t)(nV + 1), r = k / (uint64_t)(nV + 1);
+    uint64_t b = r % (uint64_t)nV, a = r / (uint64_t)nV;
+    int c0 = -((int)(nV - 1 - (int)a)) - 1;
+    long o = offset + pos[i];
+    C[o] = c0;
+    C[(long)nC + o] = (int)b;
+    C[2l * nC + o] = (int)d - 1;
+    C[3l * nC + o] = -(j - i);
+}
+
+void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
+    const double dHat = std::sqrt(c.dHat2) + c.P.xi;     // IPC.h:53-54
+    const double dHat2 = dHat * dHat;
+    // grid over the current bounding box of all vertices
+    {
+        double* bb = c.scal.p + 16;
+        bbox_minmax(c, x, bb);
+        CIPC_HIP(hipMemcpyAsync(c.hScal + 16, bb, 6 * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+    }
+    GridParams G;
+    double cs = c.cellSize;
+    double rng[3];
+    for (int d = 0; d < 3; ++d) { G.lb[d] = c.hScal[16 + d]; rng[d] = c.hScal[19 + d] - c.hScal[16 + d]; }
+    for (;;) {
+        double nc = 1;
+        for (int d = 0; d < 3; ++d) nc *= std::floor(rng[d] / cs) + 1;
+        if (nc <= (double)(1 << 23)) break;
+        cs *= 1.25;
+    }
+    G.inv = 1.0 / cs;
+    for (int d = 0; d < 3; ++d) G.dim[d] = (int)std::floor(rng[d] / cs) + 1;
+
+    build_cells(c, x, c.tri.p, c.nT, 3, G, c.triLo, c.k64a, c.u32c, c.cellStart, c.cellEnd);
+    build_cells(c, x, c.bEdge.p, c.nBE, 2, G, c.edgeLo, c.k64b, c.u32d, c.ecellStart, c.ecellEnd);
+
+    // narrow phase with capacity retry
+    CSOut o;
+    uint32_t cap = c.csCap;
+    for (;;) {
+        o.cap = cap; o.dcap = cap;
+        o.key = c.csKey.ensure(2 * (size_t)cap);
+        o.c4 = c.csC4.ensure(8 * (size_t)cap);
+        o.dkey = c.csDKey.ensure((size_t)cap);
+        o.cnt = c.counters.p;
+        o.dcnt = c.counters.p + 1;
+        CIPC_HIP(hipMemsetAsync(c.counters.p, 0, 4 * sizeof(uint32_t), c.s));
+        // point-triangle kept entries fill [0, cap), edge-edge kept entries [cap, 2 cap)
+        hipLaunchKernelGGL(k_query_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, c.bNode.p, c.nBN, c.tri.p, c.nT, c.dbc.p, G,
+                           c.cellStart.p, c.cellEnd.p, c.u32c.p, c.triLo.p, dHat, dHat2, c.nV, o);
+        CIPC_LAUNCH_CHECK();
+        CSOut oEE = o;
+        oEE.key = o.key + cap; oEE.c4 = o.c4 + 4 * (size_t)cap; oEE.cnt = c.counters.p + 2;
+        hipLaunchKernelGGL(k_query_ee, dim3(nblocks(c.nBE)), dim3(kBlock), 0, c.s, x, c.X0.p, c.bEdge.p, c.nBE, c.dbc.p, G,
+                           c.ecellStart.p, c.ecellEnd.p, c.u32d.p, c.edgeLo.p, dHat, dHat2, c.nV, oEE);
+        CIPC_LAUNCH_CHECK();
+        CIPC_HIP(hipMemcpyAsync(c.hCnt, c.counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        uint32_t nPT = c.hCnt[0], nDup = c.hCnt[1], nEE = c.hCnt[2];
+        uint32_t need = std::max(std::max(nPT, nDup), nEE);
+        if (need > cap) {
+            while (cap < need) cap *= 2;
+            c.csCap = cap;
+            continue;
+        }
+        // deduplicate PP/PE entries: sort, run heads, scan
+        uint32_t nUnique = 0;
+        uint32_t* dpos = c.csPos.ensure((size_t)std::max<uint32_t>(nDup, 1) + 1);
+        uint32_t* si = c.csIdx.ensure((size_t)std::max<uint32_t>(std::max(std::max(nPT, nEE), nDup), 1));
+        if (nDup) {
+            fill_iota(si, nDup, c.s);
+            radix_sort_pairs(o.dkey, si, nDup, bits_for((uint64_t)c.nV * c.nV * (c.nV + 1)), c.pw, c.s);
+            uint32_t* h = c.csHead.ensure(nDup);
+            hipLaunchKernelGGL(k_run_flags, dim3(nblocks(nDup)), dim3(kBlock), 0, c.s, o.dkey, (int)nDup, h);
+            CIPC_LAUNCH_CHECK();
+            scan_exclusive(h, dpos, nDup, c.counters.p + 4, c.pw, c.s);
+            CIPC_HIP(hipMemcpyAsync(c.hCnt + 4, c.counters.p + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+            CIPC_HIP(hipStreamSynchronize(c.s));
+            nUnique = c.hCnt[4];
+        }
+        int nC = (int)(nPT + nEE + nUnique);
+        out.ensure((size_t)4 * std::max(nC, 1));
+        if (nPT) {
+            fill_iota(si, nPT, c.s);
+            radix_sort_pairs(o.key, si, nPT, bits_for((uint64_t)c.nBN * c.nT), c.pw, c.s);
+            hipLaunchKernelGGL(k_gather_c4, dim3(nblocks(nPT)), dim3(kBlock), 0, c.s, si, o.c4, (int)nPT, out.p, nC, 0);
+            CIPC_LAUNCH_CHECK();
+        }
+        if (nEE) {
+            fill_iota(si, nEE, c.s);
+            radix_sort_pairs(oEE.key, si, nEE, bits_for((uint64_t)c.nBE * c.nBE), c.pw, c.s);
+            hipLaunchKernelGGL(k_gather_c4, dim3(nblocks(nEE)), dim3(kBlock), 0, c.s, si, oEE.c4, (int)nEE, out.p, nC, (int)nPT);
+            CIPC_LAUNCH_CHECK();
+        }
+        if (nDup) {
+            hipLaunchKernelGGL(k_run_emit, dim3(nblocks(nDup)), dim3(kBlock), 0, c.s, o.dkey, (int)nDup, dpos, out.p, nC,
+                               (int)(nPT + nEE), c.nV);
+            CIPC_LAUNCH_CHECK();
+        }
+        nOut = nC;
+        return;
+    }
+}
+
+// ------------------------------------------------------------------ per-constraint evaluation
+struct LocalC {
+    int nv;
+    int vid[4];
+    double d2;
+};
+
+// decode a constraint (reference encoding) into its 4-vertex stencil; for mollified
+// entries the stencil is the two edges (edgeVInd of IPC.h:831, 845, 859)
+__device__ __forceinline__ void decode(const int* __restrict__ C, int nC, int i, int* cc, int& kind, int* vid, int& nv, double& mult) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cc[k] = C[(long)k * nC + i];
+    mult = 1;
+    if (cc[0] >= 0) {
+        nv = 4;
+        if (cc[2] >= 0 && cc[3] >= 0) { kind = 0; vid[0] = cc[0]; vid[1] = cc[1]; vid[2] = cc[2]; vid[3] = cc[3]; }                  // EE
+        else if (cc[3] >= 0) { kind = 1; vid[0] = cc[0]; vid[1] = cc[1]; vid[2] = -cc[2] - 1; vid[3] = cc[3]; }                      // mollified EE
+        else if (cc[2] >= 0) { kind = 2; vid[0] = cc[0]; vid[1] = -cc[3] - 1; vid[2] = cc[1]; vid[3] = cc[2]; }                      // mollified PE
+        else { kind = 3; vid[0] = cc[0]; vid[1] = -cc[2] - 1; vid[2] = cc[1]; vid[3] = -cc[3] - 1; }                                 // mollified PP
+    } else {
+        vid[0] = -cc[0] - 1;
+        if (cc[3] >= 0) { kind = 4; nv = 4; vid[1] = cc[1]; vid[2] = cc[2]; vid[3] = cc[3]; }                                        // PT
+        else if (cc[2] >= 0) { kind = 5; nv = 3; vid[1] = cc[1]; vid[2] = cc[2]; vid[3] = -1; mult = -cc[3]; }                       // PE
+        else { kind = 6; nv = 2; vid[1] = cc[1]; vid[2] = -1; vid[3] = -1; mult = -cc[3]; }                                          // PP
+    }
+}
+__device__ __forceinline__ double con_dist2(const double* __restrict__ x, int kind, const int* vid) {
+#pragma clang fp contract(off)
+    switch (kind) {
+        case 0: case 1: return dist2_ee(ld3(x, vid[0]), ld3(x, vid[1]), ld3(x, vid[2]), ld3(x, vid[3]));
+        case 2: return dist2_pe(ld3(x, vid[0]), ld3(x, vid[2]), ld3(x, vid[3]));
+        case 3: return dist2_pp(ld3(x, vid[0]), ld3(x, vid[2]));
+        case 4: return dist2_pt(ld3(x, vid[0]), ld3(x, vid[1]), ld3(x, vid[2]), ld3(x, vid[3]));
+        case 5: return dist2_pe(ld3(x, vid[0]), ld3(x, vid[1]), ld3(x, vid[2]));
+        default: return dist2_pp(ld3(x, vid[0]), ld3(x, vid[1]));
+    }
+}
+
+__global__ void k_barrier_energy(const double* __restrict__ x, const double* __restrict__ X0, const int* __restrict__ C, int nC,
+                                 double kappa, double xi2, double shat, double* __restrict__ partial, uint32_t* __restrict__ bad) {
+    __shared__ double lds[4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (i < nC) {
+        int cc[4], vid[4], kind, nv;
+        double mult;
+        decode(C, nC, i, cc, kind, vid, nv, mult);
+        double d2 = con_dist2(x, kind, vid);
+        double s = d2 - xi2;
+        if (s <= 0) {
+            atomicOr(bad, 1u);
+        } else {
+            double b = barrier_e(s, shat, kappa);
+            if (kind >= 1 && kind <= 3) {
+                double eps = ee_mollifier_threshold(ld3(X0, vid[0]), ld3(X0, vid[1]), ld3(X0, vid[2]), ld3(X0, vid[3]));
+                double m = ee_mollifier(ld3(x, vid[0]), ld3(x, vid[1]), ld3(x, vid[2]), ld3(x, vid[3]), eps, nullptr, nullptr);
+                e = b * m;
+            } else {
+                e = (cc[3] < -1) ? b * mult : b;
+            }
+        }
+    }
+    double t = block_sum_256(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// distance gradient / Hessian of a constraint embedded into its 4-vertex stencil
+__device__ __forceinline__ void con_dist_derivs(const double* __restrict__ x, int kind, const int* vid, double* gd, double* Hd) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) gd[i] = 0;
+    if (Hd) {
+#pragma unroll
+        for (int i = 0; i < 144; ++i) Hd[i] = 0;
+    }
+    switch (kind) {
+        case 0: case 1:
+            dist2_ee_gh(ld3(x, vid[0]), ld3(x, vid[1]), ld3(x, vid[2]), ld3(x, vid[3]), gd, Hd);
+            break;
+        case 4:
+            dist2_pt_gh(ld3(x, vid[0]), ld3(x, vid[1]), ld3(x, vid[2]), ld3(x, vid[3]), gd, Hd);
+            break;
+        case 2: {   // PE over stencil slots (0, 2, 3)
+            double g9[9], H9[81];
+            dist2_pe_gh(ld3(x, vid[0]), ld3(x, vid[2]), ld3(x, vid[3]), g9, Hd ? H9 : nullptr);
+            const int sl[3] = {0, 2, 3};
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    gd[sl[a] * 3 + k] = g9[a * 3 + k];
+                    if (Hd) {
+#pragma unroll
+                        for (int b = 0; b < 3; ++b)
+#pragma unroll
+                            for (int l = 0; l < 3; ++l) Hd[(sl[a] * 3 + k) * 12 + sl[b] * 3 + l] = H9[(a * 3 + k) * 9 + b * 3 + l];
+                    }
+                }
+            break;
+        }
+        case 3: {   // PP over stencil slots (0, 2)
+            double g6[6], H6[36];
+            dist2_pp_gh(ld3(x, vid[0]), ld3(x, vid[2]), g6, Hd ? H6 : nullptr);
+            const int sl[2] = {0, 2};
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    gd[sl[a] * 3 + k] = g6[a * 3 + k];
+                    if (Hd) {
+#pragma unroll
+                        for (int b = 0; b < 2; ++b)
+#pragma unroll
+                            for (int l = 0; l < 3; ++l) Hd[(sl[a] * 3 + k) * 12 + sl[b] * 3 + l] = H6[(a * 3 + k) * 6 + b * 3 + l];
+                    }
+                }
+            break;
+        }
+        case 5: {   // PE, native 3-vertex layout
+            double H9[81];
+            dist2_pe_gh(ld3(x, vid[0]), ld3(x, vid[1]), ld3(x, vid[2]), gd, Hd ? H9 : nullptr);
+            if (Hd) {
+#pragma unroll
+                for (int i = 0; i < 9; ++i)
+#pragma unroll
+                    for (int j = 0; j < 9; ++j) Hd[(i / 3 * 3 + i % 3) * 12 + j] = H9[i * 9 + j];
+            }
+            break;
+        }
+        default: {  // PP
+            double H6[36];
+            dist2_pp_gh(ld3(x, vid[0]), ld3(x, vid[1]), gd, Hd ? H6 : nullptr);
+            if (Hd) {
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) Hd[i * 12 + j] = H6[i * 6 + j];
+            }
+            break;
+        }
+    }
+}
+
+// local gradients: SoA [3][4 nC] plus the vertex of each slot (-1 when unused or
+// when the barrier is not evaluated)
+__global__ void k_barrier_grad_local(const double* __restrict__ x, const double* __restrict__ X0, const int* __restrict__ C, int nC,
+                                     double kappa, double xi2, double shat, double* __restrict__ gl, int* __restrict__ gv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nC) return;
+    int cc[4], vid[4], kind, nv;
+    double mult;
+    decode(C, nC, i, cc, kind, vid, nv, mult);
+    double d2 = con_dist2(x, kind, vid);
+    double s = d2 - xi2;
+    long ns = 4l * nC;
+    if (!(s > 0)) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) gv[4l * i + a] = -1;
+        return;
+    }
+    double gd[12];
+    con_dist_derivs(x, kind, vid, gd, nullptr);
+    double bg = barrier_g(s, shat, kappa);
+    double gout[12];
+    if (kind >= 1 && kind <= 3) {
+        double eps = ee_mollifier_threshold(ld3(X0, vid[0]), ld3(X0, vid[1]), ld3(X0, vid[2]), ld3(X0, vid[3]));
+        double ge[12];
+        double m = ee_mollifier(ld3(x, vid[0]), ld3(x, vid[1]), ld3(x, vid[2]), ld3(x, vid[3]), eps, ge, nullptr);
+        double b = barrier_e(s, shat, kappa);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) gout[k] = m * bg * gd[k] + b * ge[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) gout[k] = mult * bg * gd[k];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        bool used = a < nv;
+        gv[4l * i + a] = used ? vid[a] : -1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gl[k * ns + 4l * i + a] = used ? gout[a * 3 + k] : 0.0;
+    }
+}
+
+// local Hessian blocks: SoA [9][16 nC]; block (a, b) of constraint i at slot 16 i + 4 a + b
+template <int N>
+__device__ __forceinline__ void psd_store(double* Hf, double* __restrict__ hl, long ns, long base, int nvv) {
+    double H[N * N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) H[i * N + j] = Hf[i * 12 + j];
+    make_pd<N>(H);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    double v = (a < nvv && b < nvv) ? H[(a * 3 + k) * N + b * 3 + l] : 0.0;
+                    hl[(long)(k * 3 + l) * ns + base + 4 * a + b] = v;
+                }
+}
+
+__global__ void __launch_bounds__(64) k_barrier_hess_local(const double* __restrict__ x, const double* __restrict__ X0,
+                                                           const int* __restrict__ C, int nC, double kappa, double xi2, double shat,
+                                                           double* __restrict__ hl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nC) return;
+    int cc[4], vid[4], kind, nv;
+    double mult;
+    decode(C, nC, i, cc, kind, vid, nv, mult);
+    double d2 = con_dist2(x, kind, vid);
+    double s = d2 - xi2;
+    long ns = 16l * nC;
+    double H[144];
+    if (!(s > 0)) {
+#pragma unroll
+        for (int k = 0; k < 144; ++k) H[k] = 0;
+        psd_store<12>(H, hl, ns, 16l * i, 0);
+        return;
+    }
+    double gd[12], Hd[144];
+    con_dist_derivs(x, kind, vid, gd, Hd);
+    double bg = barrier_g(s, shat, kappa), bh = barrier_h(s, shat, kappa);
+    if (kind >= 1 && kind <= 3) {
+        double eps = ee_mollifier_threshold(ld3(X0, vid[0]), ld3(X0, vid[1]), ld3(X0, vid[2]), ld3(X0, vid[3]));
+        double ge[12], He[144];
+        double m = ee_mollifier(ld3(x, vid[0]), ld3(x, vid[1]), ld3(x, vid[2]), ld3(x, vid[3]), eps, ge, He);
+        double b = barrier_e(s, shat, kappa);
+#pragma unroll
+        for (int r = 0; r < 12; ++r)
+#pragma unroll
+            for (int q = 0; q < 12; ++q)
+                H[r * 12 + q] = bg * gd[r] * ge[q] + bg * ge[r] * gd[q] + b * He[r * 12 + q] + m * bh * gd[r] * gd[q] + m * bg * Hd[r * 12 + q];
+        psd_store<12>(H, hl, ns, 16l * i, 4);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 12; ++r)
+#pragma unroll
+            for (int q = 0; q < 12; ++q) H[r * 12 + q] = mult * (bh * gd[r] * gd[q] + bg * Hd[r * 12 + q]);
+        if (nv == 4) psd_store<12>(H, hl, ns, 16l * i, 4);
+        else if (nv == 3) psd_store<9>(H, hl, ns, 16l * i, 3);
+        else psd_store<6>(H, hl, ns, 16l * i, 2);
+    }
+}
+
+__global__ void k_con_dist2(const double* __restrict__ x, const int* __restrict__ C, int nC, double* __restrict__ d2,
+                            double* __restrict__ partial) {
+    __shared__ double lds[4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 1e300;
+    if (i < nC) {
+        int cc[4], vid[4], kind, nv;
+        double mult;
+        decode(C, nC, i, cc, kind, vid, nv, mult);
+        v = con_dist2(x, kind, vid);
+        if (d2) d2[i] = v;
+    }
+    double m = block_min_256(v, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+__global__ void k_min_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
+    if (threadIdx.x == 0) {
+        double m = 1e300;
+        for (int i = 0; i < n; ++i) m = partial[i] < m ? partial[i] : m;
+        *out = m;
+    }
+}
+
+// ------------------------------------------------------------------ host wrappers
+double barrier_energy(Ctx& c, const double* x, const int* C, int nC, double kappa, int* bad) {
+    if (nC == 0) { if (bad) *bad = 0; return 0.0; }
+    const double xi2 = c.P.xi * c.P.xi, shat = c.dHat2 + 2 * std::sqrt(c.dHat2) * c.P.xi;   // IPC.h:756-757
+    int nb = nblocks(nC);
+    double* part = c.red.ensure(nb);
+    CIPC_HIP(hipMemsetAsync(c.counters.p + 12, 0, sizeof(uint32_t), c.s));
+    hipLaunchKernelGGL(k_barrier_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, part, c.counters.p + 12);
+    CIPC_LAUNCH_CHECK();
+    sum_partials(part, nb, c.scal.p + 1, c.s);
+    CIPC_HIP(hipMemcpyAsync(c.hScal + 1, c.scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipMemcpyAsync(c.hCnt + 12, c.counters.p + 12, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    if (bad) *bad = c.hCnt[12] ? 1 : 0;
+    return c.hScal[1];
+}
+
+__global__ void k_vert_bounds(const uint64_t* __restrict__ key, long n, int nV, uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = key[i];
+    if (k >= (uint64_t)nV) return;
+    if (i == 0 || key[i - 1] != k) start[k] = (uint32_t)i;
+    if (i == n - 1 || key[i + 1] != k) end[k] = (uint32_t)(i + 1);
+}
+__global__ void k_grad_keys(const int* __restrict__ gv, long n, int nV, uint64_t* __restrict__ key) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = gv[i];
+    key[i] = v < 0 ? (uint64_t)nV : (uint64_t)v;
+}
+__global__ void k_grad_gather(const double* __restrict__ gl, long ns, const uint32_t* __restrict__ slot, const uint32_t* __restrict__ start,
+                              const uint32_t* __restrict__ end, int nV, double* __restrict__ g) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    uint32_t s = start[v], e = end[v];
+    double a = 0, b = 0, d = 0;
+    for (uint32_t k = s; k < e; ++k) {
+        long q = slot[k];
+        a += gl[q]; b += gl[ns + q]; d += gl[2 * ns + q];
+    }
+    g[3 * v] += a; g[3 * v + 1] += b; g[3 * v + 2] += d;
+}
+
+// g += barrier gradient (deterministic: slots sorted by vertex, summed in slot order)
+void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kappa, double* g) {
+    if (nC == 0) return;
+    const double xi2 = c.P.xi * c.P.xi, shat = c.dHat2 + 2 * std::sqrt(c.dHat2) * c.P.xi;
+    long ns = 4l * nC;
+    double* gl = c.conGLocal.ensure(3 * ns);
+    int* gv = c.conGVert.ensure(ns);
+    hipLaunchKernelGGL(k_barrier_grad_local, dim3(nblocks(nC)), dim3(kBlock), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, gl, gv);
+    CIPC_LAUNCH_CHECK();
+    uint64_t* key = c.k64a.ensure(ns);
+    uint32_t* slot = c.u32c.ensure(ns);
+    hipLaunchKernelGGL(k_grad_keys, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, gv, ns, c.nV, key);
+    CIPC_LAUNCH_CHECK();
+    fill_iota(slot, ns, c.s);
+    radix_sort_pairs(key, slot, ns, bits_for((uint64_t)c.nV + 1), c.pw, c.s);
+    uint32_t* st = c.u32a.ensure(c.nV);
+    uint32_t* en = c.u32b.ensure(c.nV);
+    CIPC_HIP(hipMemsetAsync(st, 0, c.nV * sizeof(uint32_t), c.s));
+    CIPC_HIP(hipMemsetAsync(en, 0, c.nV * sizeof(uint32_t), c.s));
+    hipLaunchKernelGGL(k_vert_bounds, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, key, ns, c.nV, st, en);
+    CIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_grad_gather, dim3(nblocks(c.nV)), dim3(kBlock), 0, c.s, gl, ns, slot, st, en, c.nV, g);
+    CIPC_LAUNCH_CHECK();
+}
+
+__global__ void k_hess_keys(const int* __restrict__ C, int nC, int nV, uint64_t* __restrict__ key) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 16l * nC) return;
+    int i = (int)(q / 16), a = (int)((q / 4) % 4), b = (int)(q % 4);
+    int cc[4], vid[4], kind, nv;
+    double mult;
+    decode(C, nC, i, cc, kind, vid, nv, mult);
+    key[q] = (a < nv && b < nv) ? (uint64_t)vid[a] * (uint64_t)nV + (uint64_t)vid[b] : (uint64_t)nV * (uint64_t)nV;
+}
+__global__ void k_bsr_from_sorted(const uint64_t* __restrict__ key, long n, int nV, uint32_t* __restrict__ head) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    head[i] = (key[i] < (uint64_t)nV * nV && (i == 0 || key[i] != key[i - 1])) ? 1u : 0u;
+}
+// sum the local blocks of each unique (row, col) into the contact BSR; Dirichlet
+// rows/columns are zeroed (Project_DBC)
+__global__ void k_bsr_sum(const uint64_t* __restrict__ key, long n, int nV, const uint32_t* __restrict__ pos,
+                          const uint32_t* __restrict__ slot, const double* __restrict__ hl, long ns, const uint8_t* __restrict__ dbc,
+                          int nnzb, int* __restrict__ rowOf, int* __restrict__ col, double* __restrict__ val) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = key[i];
+    if (k >= (uint64_t)nV * nV) return;
+    if (!(i == 0 || key[i - 1] != k)) return;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (long j = i; j < n && key[j] == k; ++j) {
+        long q = slot[j];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) acc[m] += hl[m * ns + q];
+    }
+    int r = (int)(k / (uint64_t)nV), cl = (int)(k % (uint64_t)nV);
+    bool z = dbc[r] || dbc[cl];
+    int o = (int)pos[i];
+    rowOf[o] = r;
+    col[o] = cl;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) val[(long)m * nnzb + o] = z ? 0.0 : acc[m];
+}
+__global__ void k_rowptr(const int* __restrict__ rowOf, int nnzb, int nrow, int* __restrict__ rowptr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nnzb) return;
+    int prev = (i == 0) ? -1 : rowOf[i - 1];
+    int cur = (i == nnzb) ? nrow : rowOf[i];
+    for (int r = prev + 1; r <= cur; ++r) rowptr[r] = i;
+}
+
+void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa) {
+    BSR& A = c.Acon;
+    A.nrow = c.nV;
+    if (nC == 0) {
+        A.nnzb = 0;
+        A.rowptr.ensure(c.nV + 1);
+        CIPC_HIP(hipMemsetAsync(A.rowptr.p, 0, (c.nV + 1) * sizeof(int), c.s));
+        return;
+    }
+    const double xi2 = c.P.xi * c.P.xi, shat = c.dHat2 + 2 * std::sqrt(c.dHat2) * c.P.xi;
+    long ns = 16l * nC;
+    double* hl = c.conLocal.ensure(9 * ns);
+    hipLaunchKernelGGL(k_barrier_hess_local, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, hl);
+    CIPC_LAUNCH_CHECK();
+    uint64_t* key = c.k64b.ensure(ns);
+    uint32_t* slot = c.u32d.ensure(ns);
+    hipLaunchKernelGGL(k_hess_keys, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, C, nC, c.nV, key);
+    CIPC_LAUNCH_CHECK();
+    fill_iota(slot, ns, c.s);
+    radix_sort_pairs(key, slot, ns, bits_for((uint64_t)c.nV * c.nV + 1), c.pw, c.s);
+    uint32_t* head = c.u32a.ensure(ns);
+    uint32_t* pos = c.u32b.ensure(ns);
+    hipLaunchKernelGGL(k_bsr_from_sorted, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, key, ns, c.nV, head);
+    CIPC_LAUNCH_CHECK();
+    scan_exclusive(head, pos, ns, c.counters.p + 5, c.pw, c.s);
+    CIPC_HIP(hipMemcpyAsync(c.hCnt + 5, c.counters.p + 5, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    int nnzb = (int)c.hCnt[5];
+    A.nnzb = nnzb;
+    A.rowOf.ensure(std::max(nnzb, 1));
+    A.col.ensure(std::max(nnzb, 1));
+    A.val.ensure(9l * std::max(nnzb, 1));
+    A.rowptr.ensure(c.nV + 1);
+    hipLaunchKernelGGL(k_bsr_sum, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, key, ns, c.nV, pos, slot, hl, ns, c.dbc.p, nnzb,
+                       A.rowOf.p, A.col.p, A.val.p);
+    CIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_rowptr, dim3(nblocks(nnzb + 1)), dim3(kBlock), 0, c.s, A.rowOf.p, nnzb, c.nV, A.rowptr.p);
+    CIPC_LAUNCH_CHECK();
+}
+
+double min_dist2(Ctx& c, const double* x, const int* C, int nC, double* d2out) {
+    if (nC == 0) return 1e300;
+    int nb = nblocks(nC);
+    double* part = c.red.ensure(nb);
+    hipLaunchKernelGGL(k_con_dist2, dim3(nb), dim3(kBlock), 0, c.s, x, C, nC, d2out, part);
+    CIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_min_partials, dim3(1), dim3(64), 0, c.s, part, nb, c.scal.p + 2);
+    CIPC_LAUNCH_CHECK();
+    CIPC_HIP(hipMemcpyAsync(c.hScal + 2, c.scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    return c.hScal[2] - c.P.xi * c.P.xi;
+}
+
+}  // namespace cipc

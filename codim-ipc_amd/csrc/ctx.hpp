@@ -1,0 +1,126 @@
+// Device-resident scene state and the host-side stages of one implicit-Euler step.
+//
+// Layout in HBM: per-vertex vectors (positions, velocities, gradients, Newton
+// directions, Krylov vectors) are DOF-ordered double[3 nV] (x0 y0 z0 x1 ...), the order
+// of the linear system and of the 3x3 BSR blocks.  Element and stencil attributes are
+// structure-of-arrays (tri vertex k of all triangles contiguous, inverse rest metric
+// component k contiguous, ...), so stencil kernels read them coalesced.
+#pragma once
+#include "common.hpp"
+#include "../../include/cipc.h"
+
+namespace cipc {
+
+struct Params {
+    double rho = 200, E = 1e6, nu = 0.3, thick = 1e-3, bendMult = 1;
+    double dt = 0.01, dHat = 1e-3, xi = 0, PNTol = 1e-2;
+    double grav[3] = {0, -9.81, 0};
+    double kappaStiffMult = 1;
+    double pcgTol = 1e-5;
+    int pcgMaxIter = 100000;
+    int collision = 1;
+};
+
+// spatial grid used by the constraint-set broad phase and the CCD candidate search
+struct GridParams {
+    double lb[3];
+    double inv;        // 1 / cell size
+    int dim[3];
+};
+
+// sparse symmetric matrix in 3x3 block CSR with structure-of-arrays block storage:
+// val[k * nnzb + b] is entry k (row-major in the block) of block b
+struct BSR {
+    int nrow = 0, nnzb = 0;
+    DevBuf<int> rowptr, col, rowOf;
+    DevBuf<double> val;
+};
+
+struct Ctx {
+    hipStream_t s = nullptr;
+    PrimWork pw;
+    Params P;
+    std::string lastError;
+
+    int nV = 0, nT = 0, nH = 0, nBN = 0, nBE = 0, nDBC = 0;
+    double kappa = 0, kappaMax = 0, dHat2 = 0, lam = 0, mu = 0, kBend = 0, cellSize = 0;
+
+    // host mirrors of the topology
+    std::vector<int> hTri, hHinge, hBNode, hBEdge, hDbcIdx;
+    std::vector<double> hDbcTarget, hMass, hHingeInfo, hBinv, hVol;
+
+    // device state
+    DevBuf<double> x, xn, xt, v, xprev, X0, mass, p, g, rhs;
+    DevBuf<uint8_t> dbc;
+    DevBuf<int> tri;          // SoA [3][nT]
+    DevBuf<double> Binv;      // SoA [3][nT]
+    DevBuf<double> vol;
+    DevBuf<int> hinge;        // SoA [4][nH]
+    DevBuf<double> hInfo;     // SoA [3][nH]
+    DevBuf<int> bNode, bEdge; // bEdge SoA [2][nBE]
+
+    // constraint set (reference int4 encoding, SoA [4][nC])
+    int nC = 0, nCprev = 0;
+    DevBuf<int> C, Cprev;
+    DevBuf<double> d2prev;
+
+    // scratch
+    DevBuf<double> dscr, dscr2, red, scal;
+    DevBuf<uint32_t> u32a, u32b, u32c, u32d, counters;
+    DevBuf<uint64_t> k64a, k64b;
+    DevBuf<int> i32a, i32b, i32c;
+    DevBuf<uint32_t> cellStart, cellEnd, ecellStart, ecellEnd;
+    uint32_t csCap = 1u << 16;
+    DevBuf<uint64_t> csKey, csDKey;
+    DevBuf<int> csC4;
+    DevBuf<uint32_t> csPos, csIdx, csHead;
+    DevBuf<int> triLo, edgeLo;
+    DevBuf<int> ccdTlo, ccdThi, ccdElo, ccdEhi;
+
+    // elastic Hessian: static BSR pattern + gather plan
+    BSR Ael;
+    DevBuf<int> planOff, planSlot;     // per BSR block: list of local block slots
+    DevBuf<double> elLocal;            // local 3x3 blocks, SoA [9][nSlots]
+    int nElSlots = 0, nTriSlots = 0;
+    DevBuf<int> gplanOff, gplanSlot;   // per vertex: gradient slots (tri 3 per, hinge 4 per)
+    DevBuf<double> gLocal;             // local gradient 3-vectors, SoA [3][nGSlots]
+    int nGSlots = 0;
+
+    // contact Hessian (rebuilt per Newton iteration)
+    BSR Acon;
+    DevBuf<double> conLocal;           // SoA [9][16 nC]
+    DevBuf<int> conPlanSlot;           // sorted slot order
+    DevBuf<double> conGLocal;          // SoA [3][4 nC]
+    DevBuf<int> conGVert;              // vertex of each gradient slot (4 nC; -1 unused)
+
+    // PCG
+    DevBuf<double> pr, pz, pp, pq, pdinv, psol;
+
+    // host pinned scalars
+    double* hScal = nullptr;           // 64 doubles
+    uint32_t* hCnt = nullptr;          // 64 counters
+
+    // statistics of the last step
+    cipc_step_stats stats{};
+};
+
+// ---- utilities (util.hip)
+void fill_iota(uint32_t* a, long n, hipStream_t s);
+void bbox_minmax(Ctx& c, const double* x, double* out6);   // min xyz, max xyz into device memory
+
+// ---- stages (implemented in the .hip files)
+void ctx_init_device(Ctx& c);                                      // upload + derived data
+void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut);
+double barrier_energy(Ctx& c, const double* x, const int* C, int nC, double kappa, int* bad);
+void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kappa, double* g);
+void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa);   // fills Acon
+double min_dist2(Ctx& c, const double* x, const int* C, int nC, double* d2out);
+double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, double* clamped, int* zeroToi);
+double elastic_energy(Ctx& c, const double* x, double* memb, double* bend);
+double inertia_energy(Ctx& c, const double* x);
+void elastic_gradient(Ctx& c, const double* x, double* g);        // g = elastic + inertia (overwrites)
+void elastic_hessian(Ctx& c, const double* x);                    // fills Ael (incl. mass, DBC projection)
+int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres);
+int advance_one_step(Ctx& c);
+
+}  // namespace cipc

@@ -1,0 +1,357 @@
+// Shell elasticity and inertia: Neo-Hookean membrane (FEM/Shell/MEMBRANE.h:7-290),
+// hinge bending (FEM/Shell/BENDING.h:52-75, 177-208, 441-496), inertia
+// (FEM/Shell/INC_POTENTIAL.h:132-144, 250-269).  All carry the h^2 of the incremental
+// potential.  Triangles / hinges whose nodes are all Dirichlet contribute nothing.
+//
+// Assembly is deterministic and atomic-free: every stencil writes its local gradient
+// / Hessian blocks to its own slots, and a precomputed plan (built once per scene,
+// the topology is static) lists for each vertex / each BSR block the slots to sum in
+// a fixed order.  The BSR pattern holds every (row, col) pair of a triangle or hinge
+// plus all diagonal blocks (the lumped mass lands there); Dirichlet rows and
+// columns become identity / zero (Project_DBC, Math/CSR_MATRIX.h:130-141).
+#include "ctx.hpp"
+#include "dev_math.hpp"
+#include <algorithm>
+#include <cmath>
+
+namespace cipc {
+
+__device__ __forceinline__ bool all_dbc3(const uint8_t* dbc, int a, int b, int c) { return dbc[a] && dbc[b] && dbc[c]; }
+
+__global__ void k_memb_energy(const double* __restrict__ x, const int* __restrict__ tri, int nT, const double* __restrict__ Binv,
+                              const double* __restrict__ vol, double lam, double mu, double h2, const uint8_t* __restrict__ dbc,
+                              double* __restrict__ partial) {
+    __shared__ double lds[4];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (t < nT) {
+        int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+        if (!all_dbc3(dbc, a, b, c))
+            e = h2 * membrane_energy(ld3(x, a), ld3(x, b), ld3(x, c), Binv[t], Binv[nT + t], Binv[2 * nT + t], vol[t], lam, mu);
+    }
+    double s = block_sum_256(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+__global__ void k_bend_energy(const double* __restrict__ x, const int* __restrict__ hg, int nH, const double* __restrict__ hi,
+                              double kb, double h2, const uint8_t* __restrict__ dbc, double* __restrict__ partial) {
+    __shared__ double lds[4];
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (h < nH) {
+        int v0 = hg[h], v1 = hg[nH + h], v2 = hg[2 * nH + h], v3 = hg[3 * nH + h];
+        if (!(dbc[v0] && dbc[v1] && dbc[v2] && dbc[v3])) {
+            double th = dihedral_angle(ld3(x, v0), ld3(x, v1), ld3(x, v2), ld3(x, v3));
+            double tb = hi[h], el = hi[nH + h], hb = hi[2 * nH + h];
+            e = h2 * kb * (th - tb) * (th - tb) * el / hb;
+        }
+    }
+    double s = block_sum_256(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+__global__ void k_inertia_energy(const double* __restrict__ x, const double* __restrict__ xt, const double* __restrict__ m, int nV,
+                                 double* __restrict__ partial) {
+    __shared__ double lds[4];
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (v < nV) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double d = x[3 * v + k] - xt[3 * v + k];
+            e += 0.5 * m[v] * d * d;
+        }
+    }
+    double s = block_sum_256(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+static double finish_sum(Ctx& c, const double* part, int nb, int slot) {
+    sum_partials(part, nb, c.scal.p + slot, c.s);
+    CIPC_HIP(hipMemcpyAsync(c.hScal + slot, c.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    return c.hScal[slot];
+}
+
+double elastic_energy(Ctx& c, const double* x, double* memb, double* bend) {
+    const double h2 = c.P.dt * c.P.dt;
+    double Em = 0, Eb = 0;
+    if (c.nT) {
+        int nb = nblocks(c.nT);
+        double* part = c.red.ensure(nb);
+        hipLaunchKernelGGL(k_memb_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.tri.p, c.nT, c.Binv.p, c.vol.p, c.lam, c.mu, h2, c.dbc.p, part);
+        CIPC_LAUNCH_CHECK();
+        Em = finish_sum(c, part, nb, 3);
+    }
+    if (c.nH && c.P.bendMult != 0) {
+        int nb = nblocks(c.nH);
+        double* part = c.red.ensure(nb);
+        hipLaunchKernelGGL(k_bend_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.hinge.p, c.nH, c.hInfo.p, c.kBend, h2, c.dbc.p, part);
+        CIPC_LAUNCH_CHECK();
+        Eb = finish_sum(c, part, nb, 4);
+    }
+    if (memb) *memb = Em;
+    if (bend) *bend = Eb;
+    return Em + Eb;
+}
+double inertia_energy(Ctx& c, const double* x) {
+    int nb = nblocks(c.nV);
+    double* part = c.red.ensure(nb);
+    hipLaunchKernelGGL(k_inertia_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.xt.p, c.mass.p, c.nV, part);
+    CIPC_LAUNCH_CHECK();
+    return finish_sum(c, part, nb, 5);
+}
+
+// ------------------------------------------------------------------ gradient
+__global__ void k_memb_grad_local(const double* __restrict__ x, const int* __restrict__ tri, int nT, const double* __restrict__ Binv,
+                                  const double* __restrict__ vol, double lam, double mu, double h2, const uint8_t* __restrict__ dbc,
+                                  double* __restrict__ gl, long ns) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+    double g[9];
+    if (all_dbc3(dbc, a, b, c)) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) g[k] = 0;
+    } else {
+        membrane_grad(ld3(x, a), ld3(x, b), ld3(x, c), Binv[t], Binv[nT + t], Binv[2 * nT + t], vol[t], lam, mu, g);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gl[k * ns + 3l * t + q] = h2 * g[3 * q + k];
+}
+__global__ void k_bend_grad_local(const double* __restrict__ x, const int* __restrict__ hg, int nH, const double* __restrict__ hi,
+                                  double kb, double h2, const uint8_t* __restrict__ dbc, double* __restrict__ gl, long ns, long off) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nH) return;
+    int v[4] = {hg[h], hg[nH + h], hg[2 * nH + h], hg[3 * nH + h]};
+    double g[12];
+    if (dbc[v[0]] && dbc[v[1]] && dbc[v[2]] && dbc[v[3]]) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) g[k] = 0;
+    } else {
+        d3 p0 = ld3(x, v[0]), p1 = ld3(x, v[1]), p2 = ld3(x, v[2]), p3 = ld3(x, v[3]);
+        double th = dihedral_angle(p0, p1, p2, p3);
+        dihedral_angle_grad(p0, p1, p2, p3, g);
+        double s = h2 * kb * 2 * (th - hi[h]) * hi[nH + h] / hi[2 * nH + h];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) g[k] *= s;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gl[k * ns + off + 4l * h + q] = g[3 * q + k];
+}
+// g[v] = sum of local slots + m (x - xt)
+__global__ void k_el_grad_gather(const double* __restrict__ gl, long ns, const int* __restrict__ off, const int* __restrict__ slot,
+                                 const double* __restrict__ x, const double* __restrict__ xt, const double* __restrict__ m, int nV,
+                                 int withInertia, double* __restrict__ g) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    double a = 0, b = 0, d = 0;
+    for (int k = off[v]; k < off[v + 1]; ++k) {
+        long q = slot[k];
+        a += gl[q]; b += gl[ns + q]; d += gl[2 * ns + q];
+    }
+    if (withInertia) {
+        a += m[v] * (x[3 * v] - xt[3 * v]);
+        b += m[v] * (x[3 * v + 1] - xt[3 * v + 1]);
+        d += m[v] * (x[3 * v + 2] - xt[3 * v + 2]);
+    }
+    g[3 * v] = a; g[3 * v + 1] = b; g[3 * v + 2] = d;
+}
+
+static void elastic_grad_impl(Ctx& c, const double* x, double* g, int withInertia) {
+    const double h2 = c.P.dt * c.P.dt;
+    long ns = c.nGSlots;
+    double* gl = c.gLocal.ensure(3 * std::max<long>(ns, 1));
+    if (c.nT) {
+        hipLaunchKernelGGL(k_memb_grad_local, dim3(nblocks(c.nT)), dim3(kBlock), 0, c.s, x, c.tri.p, c.nT, c.Binv.p, c.vol.p, c.lam, c.mu,
+                           h2, c.dbc.p, gl, ns);
+        CIPC_LAUNCH_CHECK();
+    }
+    if (c.nH) {
+        double kb = c.P.bendMult != 0 ? c.kBend : 0.0;
+        hipLaunchKernelGGL(k_bend_grad_local, dim3(nblocks(c.nH)), dim3(kBlock), 0, c.s, x, c.hinge.p, c.nH, c.hInfo.p, kb, h2, c.dbc.p,
+                           gl, ns, 3l * c.nT);
+        CIPC_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_el_grad_gather, dim3(nblocks(c.nV)), dim3(kBlock), 0, c.s, gl, ns, c.gplanOff.p, c.gplanSlot.p, x, c.xt.p,
+                       c.mass.p, c.nV, withInertia, g);
+    CIPC_LAUNCH_CHECK();
+}
+void elastic_gradient(Ctx& c, const double* x, double* g) { elastic_grad_impl(c, x, g, 1); }
+void elastic_gradient_only(Ctx& c, const double* x, double* g) { elastic_grad_impl(c, x, g, 0); }
+
+// ------------------------------------------------------------------ Hessian
+// local blocks SoA [9][nSlots]; triangle t block (a, b) at 9 t + 3 a + b, hinge h
+// block (a, b) at 9 nT + 16 h + 4 a + b
+__global__ void __launch_bounds__(64) k_memb_hess_local(const double* __restrict__ x, const int* __restrict__ tri, int nT,
+                                                        const double* __restrict__ Binv, const double* __restrict__ vol, double lam,
+                                                        double mu, double h2, const uint8_t* __restrict__ dbc, double* __restrict__ hl,
+                                                        long ns) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+    double H[81];
+    if (all_dbc3(dbc, a, b, c)) {
+#pragma unroll
+        for (int k = 0; k < 81; ++k) H[k] = 0;
+    } else {
+        membrane_hess(ld3(x, a), ld3(x, b), ld3(x, c), Binv[t], Binv[nT + t], Binv[2 * nT + t], vol[t], lam, mu, H);
+#pragma unroll
+        for (int k = 0; k < 81; ++k) H[k] *= h2;
+        make_pd<9>(H);
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) hl[(long)(3 * k + l) * ns + 9l * t + 3 * p + q] = H[(3 * p + k) * 9 + 3 * q + l];
+}
+__global__ void __launch_bounds__(64) k_bend_hess_local(const double* __restrict__ x, const int* __restrict__ hg, int nH,
+                                                        const double* __restrict__ hi, double kb, double h2,
+                                                        const uint8_t* __restrict__ dbc, double* __restrict__ hl, long ns, long off) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nH) return;
+    int v[4] = {hg[h], hg[nH + h], hg[2 * nH + h], hg[3 * nH + h]};
+    double H[144];
+    if (kb == 0.0 || (dbc[v[0]] && dbc[v[1]] && dbc[v[2]] && dbc[v[3]])) {
+#pragma unroll
+        for (int k = 0; k < 144; ++k) H[k] = 0;
+    } else {
+        d3 p0 = ld3(x, v[0]), p1 = ld3(x, v[1]), p2 = ld3(x, v[2]), p3 = ld3(x, v[3]);
+        double th = dihedral_angle(p0, p1, p2, p3);
+        double g[12];
+        dihedral_angle_grad(p0, p1, p2, p3, g);
+        dihedral_angle_hess(p0, p1, p2, p3, H);
+        double s1 = h2 * kb * 2 * (th - hi[h]) * hi[nH + h] / hi[2 * nH + h];
+        double s2 = h2 * kb * 2 * hi[nH + h] / hi[2 * nH + h];
+#pragma unroll
+        for (int i = 0; i < 12; ++i)
+#pragma unroll
+            for (int j = 0; j < 12; ++j) H[i * 12 + j] = s1 * H[i * 12 + j] + s2 * g[i] * g[j];
+        make_pd<12>(H);
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) hl[(long)(3 * k + l) * ns + off + 16l * h + 4 * p + q] = H[(3 * p + k) * 12 + 3 * q + l];
+}
+__global__ void k_el_assemble(const double* __restrict__ hl, long ns, const int* __restrict__ off, const int* __restrict__ slot,
+                              const int* __restrict__ rowOf, const int* __restrict__ col, int nnzb, const double* __restrict__ m,
+                              const uint8_t* __restrict__ dbc, double* __restrict__ val) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nnzb) return;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = off[b]; k < off[b + 1]; ++k) {
+        long q = slot[k];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[e] += hl[e * ns + q];
+    }
+    int r = rowOf[b], cl = col[b];
+    if (r == cl) { acc[0] += m[r]; acc[4] += m[r]; acc[8] += m[r]; }
+    if (dbc[r] || dbc[cl]) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[e] = (r == cl && (e == 0 || e == 4 || e == 8)) ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) val[(long)e * nnzb + b] = acc[e];
+}
+
+void elastic_hessian(Ctx& c, const double* x) {
+    const double h2 = c.P.dt * c.P.dt;
+    long ns = c.nElSlots;
+    double* hl = c.elLocal.ensure(9 * std::max<long>(ns, 1));
+    if (c.nT) {
+        hipLaunchKernelGGL(k_memb_hess_local, dim3(nblocks(c.nT, 64)), dim3(64), 0, c.s, x, c.tri.p, c.nT, c.Binv.p, c.vol.p, c.lam, c.mu,
+                           h2, c.dbc.p, hl, ns);
+        CIPC_LAUNCH_CHECK();
+    }
+    if (c.nH) {
+        double kb = c.P.bendMult != 0 ? c.kBend : 0.0;
+        hipLaunchKernelGGL(k_bend_hess_local, dim3(nblocks(c.nH, 64)), dim3(64), 0, c.s, x, c.hinge.p, c.nH, c.hInfo.p, kb, h2, c.dbc.p,
+                           hl, ns, 9l * c.nT);
+        CIPC_LAUNCH_CHECK();
+    }
+    BSR& A = c.Ael;
+    hipLaunchKernelGGL(k_el_assemble, dim3(nblocks(A.nnzb)), dim3(kBlock), 0, c.s, hl, ns, c.planOff.p, c.planSlot.p, A.rowOf.p,
+                       A.col.p, A.nnzb, c.mass.p, c.dbc.p, A.val.p);
+    CIPC_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ static plans (host, once per scene)
+void build_elastic_plans(Ctx& c) {
+    const int nT = c.nT, nH = c.nH, nV = c.nV;
+    // gradient plan: slot -> vertex
+    {
+        long ns = 3l * nT + 4l * nH;
+        c.nGSlots = (int)ns;
+        std::vector<std::pair<int, int>> vs;
+        vs.reserve(ns);
+        for (int t = 0; t < nT; ++t)
+            for (int a = 0; a < 3; ++a) vs.push_back({c.hTri[3 * t + a], (int)(3l * t + a)});
+        for (int h = 0; h < nH; ++h)
+            for (int a = 0; a < 4; ++a) vs.push_back({c.hHinge[4 * h + a], (int)(3l * nT + 4l * h + a)});
+        std::sort(vs.begin(), vs.end());
+        std::vector<int> off(nV + 1, 0), slot(ns);
+        for (auto& p : vs) off[p.first + 1]++;
+        for (int v = 0; v < nV; ++v) off[v + 1] += off[v];
+        for (long i = 0; i < ns; ++i) slot[i] = vs[i].second;
+        upload(c.gplanOff, off, c.s);
+        upload(c.gplanSlot, slot, c.s);
+    }
+    // Hessian pattern and plan
+    {
+        long ns = 9l * nT + 16l * nH;
+        c.nElSlots = (int)ns;
+        std::vector<std::pair<uint64_t, int>> ks;
+        ks.reserve(ns + nV);
+        for (int t = 0; t < nT; ++t)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    ks.push_back({(uint64_t)c.hTri[3 * t + a] * nV + c.hTri[3 * t + b], (int)(9l * t + 3 * a + b)});
+        for (int h = 0; h < nH; ++h)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    ks.push_back({(uint64_t)c.hHinge[4 * h + a] * nV + c.hHinge[4 * h + b], (int)(9l * nT + 16l * h + 4 * a + b)});
+        for (int v = 0; v < nV; ++v) ks.push_back({(uint64_t)v * nV + v, -1});
+        std::sort(ks.begin(), ks.end());
+        std::vector<int> rowOf, col, off, slot;
+        for (size_t i = 0; i < ks.size();) {
+            size_t j = i;
+            off.push_back((int)slot.size());
+            while (j < ks.size() && ks[j].first == ks[i].first) {
+                if (ks[j].second >= 0) slot.push_back(ks[j].second);
+                ++j;
+            }
+            rowOf.push_back((int)(ks[i].first / nV));
+            col.push_back((int)(ks[i].first % nV));
+            i = j;
+        }
+        off.push_back((int)slot.size());
+        int nnzb = (int)rowOf.size();
+        std::vector<int> rowptr(nV + 1, 0);
+        for (int b = 0; b < nnzb; ++b) rowptr[rowOf[b] + 1]++;
+        for (int v = 0; v < nV; ++v) rowptr[v + 1] += rowptr[v];
+        if (slot.empty()) slot.push_back(0);
+        BSR& A = c.Ael;
+        A.nrow = nV;
+        A.nnzb = nnzb;
+        upload(A.rowptr, rowptr, c.s);
+        upload(A.rowOf, rowOf, c.s);
+        upload(A.col, col, c.s);
+        A.val.ensure_exact(9l * nnzb);
+        upload(c.planOff, off, c.s);
+        upload(c.planSlot, slot, c.s);
+    }
+    CIPC_HIP(hipStreamSynchronize(c.s));
+}
+
+}  // namespace cipc

@@ -1,0 +1,422 @@
+// Scene set-up and the implicit-Euler step driven from the host.
+//
+// ctx_setup restates Initialize_Discrete_Shell<double,3,KL=false>
+// (FEM/Shell/DISCRETE_SHELL.h:218-357), surface primitive extraction
+// (Utils/MESHIO.h:728-835) and Initialize_EIPC<double,false> (DISCRETE_SHELL.h:555-577).
+// advance_one_step restates Advance_One_Step_IE_Discrete_Shell<double,3,false,false>
+// (FEM/Shell/IMPLICIT_EULER.h:151-902) and Line_Search (:9-149) for static or
+// CCD-unblocked Dirichlet nodes without strain limiting, fibres, tets or friction;
+// every array operation runs on the device, the host only sequences the stages and
+// reads back the scalars the control flow branches on (energies, step sizes,
+// residuals, counts).
+#include "ctx.hpp"
+#include "dev_math.hpp"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <map>
+#include <set>
+
+namespace cipc {
+
+void build_elastic_plans(Ctx& c);
+void elastic_gradient_only(Ctx& c, const double* x, double* g);
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------------ set-up (host)
+static inline void hv(const std::vector<double>& X, int i, double* o) { o[0] = X[3 * i]; o[1] = X[3 * i + 1]; o[2] = X[3 * i + 2]; }
+static inline double hdot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void hsub(const double* a, const double* b, double* o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; }
+static inline void hcross(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double h_dihedral(const double* x0, const double* x1, const double* x2, const double* x3) {
+    double a[3], b[3], n1[3], n2[3], t[3];
+    hsub(x1, x0, a); hsub(x2, x0, b); hcross(a, b, n1);
+    hsub(x2, x3, a); hsub(x1, x3, b); hcross(a, b, n2);
+    double cth = hdot(n1, n2) / std::sqrt(hdot(n1, n1) * hdot(n2, n2));
+    cth = std::max(-1.0, std::min(1.0, cth));
+    double th = std::acos(cth);
+    hcross(n2, n1, t);
+    hsub(x1, x2, a);
+    if (hdot(t, a) < 0) th = -th;
+    return th;
+}
+
+void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& triIn) {
+    const Params& P = c.P;
+    // degenerate filter, DISCRETE_SHELL.h:228-244
+    std::vector<int> tri;
+    for (size_t t = 0; t < triIn.size() / 3; ++t) {
+        double a[3], b[3], cc[3], e1[3], e2[3];
+        hv(X, triIn[3 * t], a); hv(X, triIn[3 * t + 1], b); hv(X, triIn[3 * t + 2], cc);
+        hsub(b, a, e1); hsub(cc, a, e2);
+        double I00 = hdot(e1, e1), I01 = hdot(e1, e2), I11 = hdot(e2, e2);
+        if (I00 * I11 - I01 * I01 != 0) for (int k = 0; k < 3; ++k) tri.push_back(triIn[3 * t + k]);
+    }
+    c.hTri = tri;
+    c.nT = (int)tri.size() / 3;
+    // edge2tri + primitives, MESHIO.h:728-766
+    std::map<std::pair<int, int>, int> edge2tri;
+    std::set<std::pair<int, int>> edgeSet;
+    std::vector<char> isBN(c.nV, 0);
+    for (int t = 0; t < c.nT; ++t) {
+        const int* v = &tri[3 * t];
+        edge2tri[{v[0], v[1]}] = t; edge2tri[{v[1], v[2]}] = t; edge2tri[{v[2], v[0]}] = t;
+    }
+    for (int t = 0; t < c.nT; ++t) {
+        const int* v = &tri[3 * t];
+        if (!edgeSet.count({v[1], v[0]})) edgeSet.insert({v[0], v[1]});
+        if (!edgeSet.count({v[2], v[1]})) edgeSet.insert({v[1], v[2]});
+        if (!edgeSet.count({v[0], v[2]})) edgeSet.insert({v[2], v[0]});
+        isBN[v[0]] = isBN[v[1]] = isBN[v[2]] = 1;
+    }
+    c.hBNode.clear(); c.hBEdge.clear();
+    for (int i = 0; i < c.nV; ++i) if (isBN[i]) c.hBNode.push_back(i);
+    for (auto& e : edgeSet) { c.hBEdge.push_back(e.first); c.hBEdge.push_back(e.second); }
+    c.nBN = (int)c.hBNode.size();
+    c.nBE = (int)c.hBEdge.size() / 2;
+    // inverse rest metric, hinges, DISCRETE_SHELL.h:132-213
+    c.hBinv.resize(3 * c.nT);
+    for (int t = 0; t < c.nT; ++t) {
+        double a[3], b[3], cc[3], e1[3], e2[3];
+        hv(X, tri[3 * t], a); hv(X, tri[3 * t + 1], b); hv(X, tri[3 * t + 2], cc);
+        hsub(b, a, e1); hsub(cc, a, e2);
+        double I00 = hdot(e1, e1), I01 = hdot(e1, e2), I11 = hdot(e2, e2);
+        double det = I00 * I11 - I01 * I01;
+        c.hBinv[3 * t] = I11 / det; c.hBinv[3 * t + 1] = -I01 / det; c.hBinv[3 * t + 2] = I00 / det;
+    }
+    c.hHinge.clear(); c.hHingeInfo.clear();
+    for (auto& e : edgeSet) {
+        auto fa = edge2tri.find({e.first, e.second});
+        if (fa == edge2tri.end()) continue;
+        const int* A = &tri[3 * fa->second];
+        int v0 = -1;
+        for (int j = 0; j < 3; ++j) if (A[j] == e.second) { v0 = A[(j + 1) % 3]; break; }
+        auto fb = edge2tri.find({e.second, e.first});
+        if (fb == edge2tri.end()) continue;
+        const int* B = &tri[3 * fb->second];
+        int v3 = B[0];
+        for (int j = 0; j < 3; ++j) if (B[j] == e.first) { v3 = B[(j + 1) % 3]; break; }
+        int st[4] = {v0, e.first, e.second, v3};
+        for (int k = 0; k < 4; ++k) c.hHinge.push_back(st[k]);
+        double p0[3], p1[3], p2[3], p3[3], a[3], b[3], n1[3], n2[3];
+        hv(X, st[0], p0); hv(X, st[1], p1); hv(X, st[2], p2); hv(X, st[3], p3);
+        double th = h_dihedral(p0, p1, p2, p3);
+        hsub(p1, p2, a);
+        double el = std::sqrt(hdot(a, a));
+        hsub(p1, p0, a); hsub(p2, p0, b); hcross(a, b, n1);
+        hsub(p2, p3, a); hsub(p1, p3, b); hcross(a, b, n2);
+        c.hHingeInfo.push_back(th);
+        c.hHingeInfo.push_back(el);
+        c.hHingeInfo.push_back((std::sqrt(hdot(n1, n1)) + std::sqrt(hdot(n2, n2))) / (el * 6));
+    }
+    c.nH = (int)c.hHinge.size() / 4;
+    // lumped mass, DISCRETE_SHELL.h:279-299
+    c.hMass.assign(c.nV, 0.0);
+    c.hVol.resize(c.nT);
+    for (int t = 0; t < c.nT; ++t) {
+        const int* v = &tri[3 * t];
+        double a[3], b[3], cc[3], e1[3], e2[3], n[3];
+        hv(X, v[0], a); hv(X, v[1], b); hv(X, v[2], cc);
+        hsub(b, a, e1); hsub(cc, a, e2); hcross(e1, e2, n);
+        double area = std::sqrt(hdot(n, n)) / 2;
+        double mp = area * P.thick * P.rho / 3;
+        c.hMass[v[0]] += mp; c.hMass[v[1]] += mp; c.hMass[v[2]] += mp;
+        c.hVol[t] = area * P.thick;
+    }
+    c.lam = P.E * P.nu / (1.0 - P.nu * P.nu);
+    c.mu = P.E / (2.0 * (1.0 + P.nu));
+    c.kBend = P.bendMult * (P.E * std::pow(P.thick, 3) / (24 * (1.0 - P.nu * P.nu)));
+    // Initialize_EIPC<double,false>, DISCRETE_SHELL.h:563-574
+    c.dHat2 = P.dHat * P.dHat;
+    double t2 = 1.0e-16 - c.dHat2;
+    double Hb = 1.0 * ((std::log(1.0e-16 / c.dHat2) * -2.0 - t2 * 4.0 / 1.0e-16) + 1.0 / (1.0e-16 * 1.0e-16) * (t2 * t2));
+    double mmean = 0;
+    for (int i = 0; i < c.nV; ++i) mmean += 3 * c.hMass[i];
+    mmean /= 3 * c.nV;
+    c.kappa = P.kappaStiffMult * 1.0e11 * mmean * 3 / (4.0e-16 * Hb);
+    c.kappaMax = 100 * c.kappa;
+    // broad-phase cell size: mean rest boundary-edge length
+    double sum = 0;
+    for (int e = 0; e < c.nBE; ++e) {
+        double a[3], b[3], d[3];
+        hv(X, c.hBEdge[2 * e], a); hv(X, c.hBEdge[2 * e + 1], b); hsub(a, b, d);
+        sum += std::sqrt(hdot(d, d));
+    }
+    c.cellSize = c.nBE ? sum / c.nBE : 1.0;
+    if (!(c.cellSize > 0)) c.cellSize = 1.0;
+
+    // ---- uploads (SoA element attributes)
+    auto soa_i = [](const std::vector<int>& a, int n, int k) {
+        std::vector<int> o((size_t)n * k);
+        for (int i = 0; i < n; ++i) for (int j = 0; j < k; ++j) o[(size_t)j * n + i] = a[(size_t)k * i + j];
+        return o;
+    };
+    auto soa_d = [](const std::vector<double>& a, int n, int k) {
+        std::vector<double> o((size_t)n * k);
+        for (int i = 0; i < n; ++i) for (int j = 0; j < k; ++j) o[(size_t)j * n + i] = a[(size_t)k * i + j];
+        return o;
+    };
+    hipStream_t s = c.s;
+    upload(c.X0, X, s);
+    upload(c.mass, c.hMass, s);
+    std::vector<uint8_t> dbc(c.nV, 0);
+    for (int d : c.hDbcIdx) dbc[d] = 1;
+    upload(c.dbc, dbc, s);
+    upload(c.tri, soa_i(tri, c.nT, 3), s);
+    upload(c.Binv, soa_d(c.hBinv, c.nT, 3), s);
+    upload(c.vol, c.hVol, s);
+    upload(c.hinge, soa_i(c.hHinge, c.nH, 4), s);
+    upload(c.hInfo, soa_d(c.hHingeInfo, c.nH, 3), s);
+    upload(c.bNode, c.hBNode, s);
+    upload(c.bEdge, soa_i(c.hBEdge, c.nBE, 2), s);
+    const size_t n3 = 3 * (size_t)c.nV;
+    for (DevBuf<double>* b : {&c.x, &c.xn, &c.xt, &c.v, &c.xprev, &c.p, &c.g, &c.rhs}) b->ensure_exact(n3);
+    c.scal.ensure_exact(64);
+    c.counters.ensure_exact(64);
+    CIPC_HIP(hipMemsetAsync(c.counters.p, 0, 64 * sizeof(uint32_t), s));
+    build_elastic_plans(c);
+}
+
+// ------------------------------------------------------------------ vector kernels
+__global__ void k_predict(int nV, const double* __restrict__ x, const double* __restrict__ v, const double* __restrict__ m, double h,
+                          double gx, double gy, double gz, double* __restrict__ xt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nV) return;
+    double g[3] = {gx, gy, gz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double a = m[i] > 0 ? g[k] : 0.0;
+        xt[3 * i + k] = x[3 * i + k] + h * v[3 * i + k] + h * h * a;
+    }
+}
+__global__ void k_dbc_disp(int n, const int* __restrict__ idx, const double* __restrict__ tgt, const double* __restrict__ x,
+                           double* __restrict__ disp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = idx[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) disp[3 * v + k] = tgt[3 * i + k] - x[3 * v + k];
+}
+__global__ void k_dbc_set(int n, const int* __restrict__ idx, const double* __restrict__ tgt, double* __restrict__ x) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = idx[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[3 * v + k] = tgt[3 * i + k];
+}
+// rhs = -g with Dirichlet rows zeroed (IMPLICIT_EULER.h:476-491)
+__global__ void k_rhs(int nV, double* __restrict__ g, const uint8_t* __restrict__ dbc, double* __restrict__ rhs) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nV) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double gv = dbc[i] ? 0.0 : g[3 * i + k];
+        g[3 * i + k] = gv;
+        rhs[3 * i + k] = -gv;
+    }
+}
+__global__ void k_axpy_to(long n, const double* __restrict__ a, double s, const double* __restrict__ b, double* __restrict__ out) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] + s * b[i];
+}
+__global__ void k_sumsq(long n, const double* __restrict__ a, double* __restrict__ partial) {
+    __shared__ double lds[4];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = i < n ? a[i] * a[i] : 0.0;
+    double s = block_sum_256(v, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+__global__ void k_velocity(long n, const double* __restrict__ x, const double* __restrict__ xn, double h, double* __restrict__ v) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (x[i] - xn[i]) / h;
+}
+__global__ void k_kappa_flag(int n, const double* __restrict__ d2prev, const double* __restrict__ d2cur, uint32_t* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (d2prev[i] < 1e-18 && d2cur[i] < d2prev[i]) atomicOr(flag, 1u);
+}
+
+static double sumsq(Ctx& c, const double* a, long n) {
+    int nb = nblocks(n);
+    double* part = c.red.ensure(nb);
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kBlock), 0, c.s, n, a, part);
+    CIPC_LAUNCH_CHECK();
+    sum_partials(part, nb, c.scal.p + 6, c.s);
+    CIPC_HIP(hipMemcpyAsync(c.hScal + 6, c.scal.p + 6, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    return c.hScal[6];
+}
+
+// incremental potential at x with constraint set C (INC_POTENTIAL.h:15-152)
+static double inc_potential(Ctx& c, const double* x, const int* C, int nC, int* bad) {
+    double E = elastic_energy(c, x, nullptr, nullptr) + inertia_energy(c, x);
+    if (c.P.collision) E += barrier_energy(c, x, C, nC, c.kappa, bad);
+    else if (bad) *bad = 0;
+    return E;
+}
+
+// ------------------------------------------------------------------ the step
+int advance_one_step(Ctx& c) {
+    const Params& P = c.P;
+    const long n = 3l * c.nV;
+    const double h = P.dt;
+    cipc_step_stats& st = c.stats;
+    st = cipc_step_stats{};
+    double t0 = now_ms(), ta;
+    hipStream_t s = c.s;
+    DevBuf<int>& dbcIdxD = c.i32c;
+    static_assert(sizeof(int) == 4, "");
+    // x_n, x_tilde (IMPLICIT_EULER.h:195-217)
+    CIPC_HIP(hipMemcpyAsync(c.xn.p, c.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_predict, dim3(nblocks(c.nV)), dim3(kBlock), 0, s, c.nV, c.x.p, c.v.p, c.mass.p, h, P.grav[0], P.grav[1], P.grav[2],
+                       c.xt.p);
+    CIPC_LAUNCH_CHECK();
+    // Dirichlet motion and its CCD (IMPLICIT_EULER.h:304-401)
+    if (c.nDBC) {
+        dbcIdxD.ensure(c.nDBC);
+        CIPC_HIP(hipMemcpyAsync(dbcIdxD.p, c.hDbcIdx.data(), c.nDBC * sizeof(int), hipMemcpyHostToDevice, s));
+        double* tgt = c.dscr.ensure(3l * c.nDBC);
+        CIPC_HIP(hipMemcpyAsync(tgt, c.hDbcTarget.data(), 3l * c.nDBC * sizeof(double), hipMemcpyHostToDevice, s));
+        CIPC_HIP(hipMemsetAsync(c.p.p, 0, n * sizeof(double), s));
+        hipLaunchKernelGGL(k_dbc_disp, dim3(nblocks(c.nDBC)), dim3(kBlock), 0, s, c.nDBC, dbcIdxD.p, tgt, c.x.p, c.p.p);
+        CIPC_LAUNCH_CHECK();
+        double dbcAlpha = 1;
+        if (P.collision) {
+            int z = 0;
+            dbcAlpha = ccd_stepsize(c, c.x.p, c.p.p, 1.0, nullptr, &z);
+        }
+        if (dbcAlpha != 1) { c.lastError = "Dirichlet motion blocked by contact (augmented Lagrangian path not covered)"; return CIPC_E_UNSUPPORTED; }
+        tgt = c.dscr.ensure(3l * c.nDBC);
+        CIPC_HIP(hipMemcpyAsync(tgt, c.hDbcTarget.data(), 3l * c.nDBC * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_dbc_set, dim3(nblocks(c.nDBC)), dim3(kBlock), 0, s, c.nDBC, dbcIdxD.p, tgt, c.x.p);
+        CIPC_LAUNCH_CHECK();
+    }
+    c.nC = 0;
+    c.nCprev = 0;
+    if (P.collision) { ta = now_ms(); constraint_set(c, c.x.p, c.C, c.nC); st.t_constraint_set_ms += now_ms() - ta; }
+    int bad = 0;
+    double Eprev = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
+    if (bad) { c.lastError = "non-positive distance in the initial state"; return CIPC_E_PENETRATION; }
+    std::deque<double> resRecord;
+    bool useGD = false;
+    double L2Norm = 0;
+    int PNIter = 0;
+    do {
+        // gradient (INC_POTENTIAL.h:154-275)
+        ta = now_ms();
+        elastic_gradient(c, c.x.p, c.g.p);
+        if (P.collision) barrier_gradient(c, c.x.p, c.C.p, c.nC, c.kappa, c.g.p);
+        hipLaunchKernelGGL(k_rhs, dim3(nblocks(c.nV)), dim3(kBlock), 0, s, c.nV, c.g.p, c.dbc.p, c.rhs.p);
+        CIPC_LAUNCH_CHECK();
+        CIPC_HIP(hipStreamSynchronize(s));
+        st.t_gradient_ms += now_ms() - ta;
+        // Hessian and solve (IMPLICIT_EULER.h:494-554)
+        if (!useGD) {
+            ta = now_ms();
+            elastic_hessian(c, c.x.p);
+            if (P.collision) barrier_hessian(c, c.x.p, c.C.p, c.nC, c.kappa);
+            else c.Acon.nnzb = 0;
+            CIPC_HIP(hipStreamSynchronize(s));
+            st.t_hessian_ms += now_ms() - ta;
+            ta = now_ms();
+            int it = 0;
+            double rr = 0;
+            int rc = pcg_solve(c, c.rhs.p, c.p.p, P.pcgTol, P.pcgMaxIter, &it, &rr);
+            st.pcg_iters += it;
+            st.t_solve_ms += now_ms() - ta;
+            if (rc == 1) { useGD = true; st.gd_fallbacks++; }
+        }
+        if (useGD) CIPC_HIP(hipMemcpyAsync(c.p.p, c.rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        // line search (IMPLICIT_EULER.h:9-149)
+        CIPC_HIP(hipMemcpyAsync(c.xprev.p, c.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        double alpha = 1.0;
+        if (P.collision) {
+            ta = now_ms();
+            int z = 0;
+            alpha = ccd_stepsize(c, c.x.p, c.p.p, alpha, nullptr, &z);
+            st.t_ccd_ms += now_ms() - ta;
+            if (alpha == 0 || z) { c.lastError = "CCD returned a zero step"; return CIPC_E_ZERO_TOI; }
+        }
+        double feasibleAlpha = alpha;
+        double E = Eprev;
+        do {
+            hipLaunchKernelGGL(k_axpy_to, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c.xprev.p, alpha, c.p.p, c.x.p);
+            CIPC_LAUNCH_CHECK();
+            st.ls_trials++;
+            if (P.collision) {
+                ta = now_ms();
+                constraint_set(c, c.x.p, c.C, c.nC);
+                st.t_constraint_set_ms += now_ms() - ta;
+                if (c.nC) {
+                    double mn = min_dist2(c, c.x.p, c.C.p, c.nC, nullptr);
+                    if (mn <= 0) { alpha /= 2; E = Eprev + 1; continue; }
+                }
+            }
+            ta = now_ms();
+            E = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
+            st.t_energy_ms += now_ms() - ta;
+            if (bad) { c.lastError = "non-positive distance in line search"; return CIPC_E_PENETRATION; }
+            alpha /= 2.0;
+        } while (E > Eprev);
+        Eprev = E;
+        // kappa adaptation (IMPLICIT_EULER.h:566-595)
+        if (c.nCprev > 0) {
+            double* cur = c.dscr2.ensure(c.nCprev);
+            min_dist2(c, c.x.p, c.Cprev.p, c.nCprev, cur);
+            CIPC_HIP(hipMemsetAsync(c.counters.p + 14, 0, sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_kappa_flag, dim3(nblocks(c.nCprev)), dim3(kBlock), 0, s, c.nCprev, c.d2prev.p, cur, c.counters.p + 14);
+            CIPC_LAUNCH_CHECK();
+            CIPC_HIP(hipMemcpyAsync(c.hCnt + 14, c.counters.p + 14, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            CIPC_HIP(hipStreamSynchronize(s));
+            if (c.hCnt[14] && c.kappa < c.kappaMax) {
+                c.kappa *= 2;
+                Eprev = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
+            }
+        }
+        // remember the constraint set and its distances
+        c.nCprev = c.nC;
+        if (c.nC) {
+            c.Cprev.ensure(4l * c.nC);
+            CIPC_HIP(hipMemcpyAsync(c.Cprev.p, c.C.p, 4l * c.nC * sizeof(int), hipMemcpyDeviceToDevice, s));
+            c.d2prev.ensure(c.nC);
+            min_dist2(c, c.x.p, c.C.p, c.nC, c.d2prev.p);
+        }
+        // stopping rule (IMPLICIT_EULER.h:647-710, 866)
+        double ss = sumsq(c, c.p.p, n);
+        L2Norm = std::sqrt(ss / (c.nV - c.nDBC)) / h;
+        PNIter++;
+        resRecord.push_back(L2Norm);
+        if (resRecord.size() > 3) resRecord.pop_front();
+        double cur = L2Norm;
+        L2Norm = *std::max_element(resRecord.begin(), resRecord.end());
+        if (useGD) L2Norm = P.PNTol * 10;
+        if (alpha * 2 < 1e-8 && feasibleAlpha > 1e-8) {
+            if (!useGD) {
+                if (cur < P.PNTol) resRecord.assign(3, cur);
+                else useGD = true;
+            }
+        } else {
+            useGD = false;
+        }
+        st.last_residual = cur;
+        if (PNIter > 2000) { c.lastError = "Newton iteration limit"; return CIPC_E_NO_CONVERGENCE; }
+    } while (resRecord.size() < 3 || L2Norm > P.PNTol);
+    hipLaunchKernelGGL(k_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c.x.p, c.xn.p, h, c.v.p);
+    CIPC_LAUNCH_CHECK();
+    CIPC_HIP(hipStreamSynchronize(s));
+    st.newton_iters = PNIter;
+    st.contacts = c.nC;
+    st.kappa = c.kappa;
+    st.energy = Eprev;
+    st.t_total_ms = now_ms() - t0;
+    st.status = CIPC_OK;
+    return CIPC_OK;
+}
+
+}  // namespace cipc

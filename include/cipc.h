@@ -1,0 +1,120 @@
+/*
+ * cipc.h -- C ABI of the MI355X (gfx950) implementation of the C-IPC
+ * contact-and-Newton step (ipc-sim/Codim-IPC, discrete-shell hinge path).
+ *
+ * Plain pointers and sizes only.  All arrays are caller-owned host memory;
+ * per-vertex arrays are vertex-major xyz (double[3 n]).  Every entry point
+ * returns CIPC_OK or a negative error code; cipc_last_error() describes the
+ * last failure of a context.  A context owns one HIP stream and its device
+ * state; it is not thread safe (one context per host thread).
+ *
+ * The step entry point replaces the reference's pybind11 binding
+ *   FEM.DiscreteShell.Advance_One_Step_IE_Hinge
+ *     = Advance_One_Step_IE_Discrete_Shell<double,3,KL=false,elasticIPC=false>
+ *   (Library/FEM/Shell/DISCRETE_SHELL.h:1108, body IMPLICIT_EULER.h:151-902),
+ * called from Python/Drivers/FEMDiscreteShellBase.py:353-361.  Scene set-up
+ * replaces Initialize_Shell_Hinge_EIPC + Initialize_OIPC
+ * (DISCRETE_SHELL.h:218-357, 555-577).  The component entry points expose the
+ * reference functions the step is built from, for parity testing.
+ */
+#ifndef CIPC_H
+#define CIPC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CIPC_OK 0
+#define CIPC_E_INVALID (-1)      /* bad argument or size */
+#define CIPC_E_HIP (-2)          /* HIP runtime failure */
+#define CIPC_E_PENETRATION (-3)  /* a constraint distance <= offset (IPC.h:773-776 exit) */
+#define CIPC_E_ZERO_TOI (-4)     /* CCD returned a zero step (IPC.h:2014-2032 exit) */
+#define CIPC_E_UNSUPPORTED (-5)  /* configuration outside this build (e.g. blocked Dirichlet motion) */
+#define CIPC_E_NO_CONVERGENCE (-6)
+#define CIPC_E_CAPACITY (-7)     /* caller buffer too small; required size returned */
+
+typedef struct cipc_ctx cipc_ctx;
+
+/* Scene description: geometry, Dirichlet nodes and the parameters the reference
+ * passes to Initialize_Shell_Hinge_EIPC / Initialize_OIPC / Advance_One_Step. */
+typedef struct cipc_scene_desc {
+    int32_t n_vertices;
+    const double* x0;          /* rest = initial positions, 3 n_vertices */
+    const double* v0;          /* initial velocities or NULL (zero) */
+    int32_t n_triangles;
+    const int32_t* triangles;  /* 3 n_triangles, consistently oriented */
+    int32_t n_dbc;
+    const int32_t* dbc_index;  /* Dirichlet vertex ids */
+    const double* dbc_target;  /* 3 n_dbc target positions */
+    double density, youngs, poisson, thickness, bending_stiff_mult;
+    double dt, dhat, offset, newton_tol;
+    double gravity[3];
+    double kappa_stiff_mult;   /* Initialize_OIPC stiffMult */
+    double pcg_tol;            /* relative residual of the Newton linear solve */
+    int32_t pcg_max_iter;
+    int32_t with_collision;
+} cipc_scene_desc;
+
+typedef struct cipc_step_stats {
+    int32_t newton_iters, contacts, pcg_iters, gd_fallbacks, ls_trials, status;
+    double kappa, last_residual, energy;
+    double t_total_ms, t_constraint_set_ms, t_ccd_ms, t_hessian_ms, t_solve_ms, t_energy_ms, t_gradient_ms;
+} cipc_step_stats;
+
+/* ---- lifetime ------------------------------------------------------------ */
+/* Initialize_Shell_Hinge_EIPC + Initialize_OIPC (DISCRETE_SHELL.h:218-357, 555-577) */
+int cipc_create(const cipc_scene_desc* desc, cipc_ctx** out);
+void cipc_destroy(cipc_ctx* ctx);
+const char* cipc_last_error(const cipc_ctx* ctx);
+/* out[0..5] = n_vertices, n_triangles, n_boundary_nodes, n_boundary_edges, n_hinges, n_constraints */
+int cipc_get_sizes(const cipc_ctx* ctx, int32_t* out);
+/* out[0..5] = kappa, kappa_max, dhat^2, lambda, mu, hinge k (DISCRETE_SHELL.h:322-342, 568-574) */
+int cipc_get_scalars(const cipc_ctx* ctx, double* out);
+/* derived set-up arrays: lumped mass (n_vertices), hinge stencils (4 n_hinges) and
+ * rest data (3 n_hinges: theta_bar, |e_bar|, h_bar), boundary edges (2 n_boundary_edges) */
+int cipc_get_setup(const cipc_ctx* ctx, double* mass, int32_t* hinges, double* hinge_info, int32_t* bedges);
+int cipc_set_state(cipc_ctx* ctx, const double* x, const double* v);
+int cipc_get_state(const cipc_ctx* ctx, double* x, double* v);
+int cipc_set_kappa(cipc_ctx* ctx, double kappa);
+
+/* ---- the step ------------------------------------------------------------ */
+/* Advance_One_Step_IE_Discrete_Shell<double,3,false,false> (IMPLICIT_EULER.h:151-902):
+ * one implicit-Euler step; mutates positions, velocities and kappa. */
+int cipc_step(cipc_ctx* ctx, cipc_step_stats* stats);
+
+/* ---- components at caller positions x (parity) ---------------------------- */
+/* Compute_Constraint_Set (IPC.h:19-740): int4 encoding of the reference, order:
+ * point-triangle by (boundary node, triangle), edge-edge family by (edge, edge),
+ * deduplicated point-point/point-edge lexicographically with -multiplicity. */
+int cipc_constraint_set(cipc_ctx* ctx, const double* x, int32_t* out, int32_t cap, int32_t* n);
+/* Compute_Barrier (IPC.h:742-941) */
+int cipc_barrier_energy(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, double* e);
+/* Compute_Barrier_Gradient (IPC.h:943-1256); g overwritten (3 n_vertices) */
+int cipc_barrier_gradient(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, double* g);
+/* Compute_Membrane_Energy + Compute_Bending_Energy (MEMBRANE.h:7-50, BENDING.h:52-75): out[0..1] */
+int cipc_elastic_energy(cipc_ctx* ctx, const double* x, double* out);
+/* Compute_Membrane_Gradient + Compute_Bending_Gradient (MEMBRANE.h:52-119, BENDING.h:177-208) */
+int cipc_elastic_gradient(cipc_ctx* ctx, const double* x, double* g);
+/* Compute_IncPotential_Hessian (INC_POTENTIAL.h:277-395): which = 1 elastic (+mass,
+ * Dirichlet projection), 2 barrier, 3 both.  Blocks as COO (duplicates to be summed):
+ * brow/bcol block indices, bval 9 doubles (row-major) per block. */
+int cipc_hessian(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t which,
+                 int32_t* brow, int32_t* bcol, double* bval, int32_t cap, int32_t* nblocks);
+/* Compute_Min_Dist2 (IPC.h:2246-2388): per-constraint d^2 (no offset) and min(d^2) - offset^2 */
+int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double* d2, double* min_val);
+/* Compute_Intersection_Free_StepSize (IPC.h:1879-2244) incl. the CCD hash step clamp
+ * (SPATIAL_HASH.h:466-482); alpha_clamped = step after the clamp */
+int cipc_ccd_stepsize(cipc_ctx* ctx, const double* x, const double* p, double alpha_in, double* alpha_out,
+                      double* alpha_clamped);
+/* Newton linear solve: assemble H(x, cs) (elastic + barrier + mass, Dirichlet
+ * projected) and solve H sol = b by block-Jacobi PCG to ||r|| <= tol ||b||
+ * (stands in for Solve_Direct, DIRECT_SOLVER.h:14-66). */
+int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, const double* b,
+                      double* sol, double tol, int32_t* iters, double* relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CIPC_H */

@@ -1,0 +1,211 @@
+// Oracle C entry points for ctypes (tests/, smoke, bench cpu_baseline only).
+// TEST INFRASTRUCTURE ONLY (see oracle.h).
+#include "oracle.h"
+#include <cstring>
+
+using namespace orc;
+
+static V3 P3(const double* p, int i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+extern "C" {
+
+// ---------------------------------------------------------------- primitives
+// kind: 0 PP (2 pts), 1 PE (3), 2 PT (4), 3 EE (4)
+double orc_dist2(int kind, const double* p) {
+    switch (kind) {
+        case 0: return dist2_pp(P3(p, 0), P3(p, 1));
+        case 1: return dist2_pe(P3(p, 0), P3(p, 1), P3(p, 2));
+        case 2: return dist2_pt(P3(p, 0), P3(p, 1), P3(p, 2), P3(p, 3));
+        default: return dist2_ee(P3(p, 0), P3(p, 1), P3(p, 2), P3(p, 3));
+    }
+}
+void orc_dist2_gh(int kind, const double* p, double* g, double* H) {
+    switch (kind) {
+        case 0: dist2_pp_gh(P3(p, 0), P3(p, 1), g, H); break;
+        case 1: dist2_pe_gh(P3(p, 0), P3(p, 1), P3(p, 2), g, H); break;
+        case 2: dist2_pt_gh(P3(p, 0), P3(p, 1), P3(p, 2), P3(p, 3), g, H); break;
+        default: dist2_ee_gh(P3(p, 0), P3(p, 1), P3(p, 2), P3(p, 3), g, H); break;
+    }
+}
+// kind: 1 PE, 2 PT, 3 EE; 4 PT unclassified distance, 5 EE unclassified, 6 PE unclassified
+int orc_type(int kind, const double* p) {
+    switch (kind) {
+        case 1: return pe_type(P3(p, 0), P3(p, 1), P3(p, 2));
+        case 2: return pt_type(P3(p, 0), P3(p, 1), P3(p, 2), P3(p, 3));
+        default: return ee_type(P3(p, 0), P3(p, 1), P3(p, 2), P3(p, 3));
+    }
+}
+double orc_dist2_unclassified(int kind, const double* p) {
+    switch (kind) {
+        case 1: return dist2_pe_unclassified(P3(p, 0), P3(p, 1), P3(p, 2));
+        case 2: return dist2_pt_unclassified(P3(p, 0), P3(p, 1), P3(p, 2), P3(p, 3));
+        default: return dist2_ee_unclassified(P3(p, 0), P3(p, 1), P3(p, 2), P3(p, 3));
+    }
+}
+void orc_mollifier(const double* p, const double* prest, double* e, double* g, double* H, double* eps) {
+    *eps = ee_mollifier_threshold(P3(prest, 0), P3(prest, 1), P3(prest, 2), P3(prest, 3));
+    ee_mollifier(P3(p, 0), P3(p, 1), P3(p, 2), P3(p, 3), *eps, e, g, H);
+}
+void orc_barrier(double d, double dhat, double kappa, double* out3) {
+    out3[0] = barrier(d, dhat, kappa);
+    out3[1] = barrier_g(d, dhat, kappa);
+    out3[2] = barrier_h(d, dhat, kappa);
+}
+void orc_sym_eig(int n, const double* A, double* w, double* V) { sym_eig(n, A, w, V); }
+void orc_make_pd(int n, double* A) { make_pd(n, A); }
+void orc_membrane(const double* x, const double* B, double vol, double lam, double mu, double* E, double* g, double* H) {
+    *E = membrane_energy(P3(x, 0), P3(x, 1), P3(x, 2), B, vol, lam, mu);
+    membrane_grad(P3(x, 0), P3(x, 1), P3(x, 2), B, vol, lam, mu, g);
+    membrane_hess(P3(x, 0), P3(x, 1), P3(x, 2), B, vol, lam, mu, H);
+}
+void orc_dihedral(const double* x, double* th, double* g, double* H) {
+    *th = dihedral_angle(P3(x, 0), P3(x, 1), P3(x, 2), P3(x, 3));
+    dihedral_angle_grad(P3(x, 0), P3(x, 1), P3(x, 2), P3(x, 3), g);
+    dihedral_angle_hess(P3(x, 0), P3(x, 1), P3(x, 2), P3(x, 3), H);
+}
+
+// ---------------------------------------------------------------- scenes
+// params: rho, E, nu, thick, bendMult, dt, dHat, xi, PNTol, gx, gy, gz, kappaStiffMult, pcgTol
+// iparams: collision, solver, pcgMaxIter, threads
+void* orc_scene_create(int nV, const double* X0, int nT, const int* tri, int nDBC, const int* dbcIdx,
+                       const double* dbcTarget, const double* params, const int* iparams) {
+    Scene* S = new Scene();
+    Params& P = S->P;
+    P.rho = params[0]; P.E = params[1]; P.nu = params[2]; P.thick = params[3]; P.bendMult = params[4];
+    P.dt = params[5]; P.dHat = params[6]; P.xi = params[7]; P.PNTol = params[8];
+    P.grav[0] = params[9]; P.grav[1] = params[10]; P.grav[2] = params[11];
+    P.kappaStiffMult = params[12]; P.pcgTol = params[13];
+    P.collision = iparams[0]; P.solver = iparams[1]; P.pcgMaxIter = iparams[2]; P.threads = iparams[3] > 0 ? iparams[3] : 1;
+    S->nV = nV;
+    S->X0.assign(X0, X0 + 3 * nV);
+    S->tri.assign(tri, tri + 3 * nT);
+    S->dbcIdx.assign(dbcIdx, dbcIdx + nDBC);
+    S->dbcTarget.assign(dbcTarget, dbcTarget + 3 * nDBC);
+    scene_init(*S);
+    return S;
+}
+void orc_scene_destroy(void* h) { delete (Scene*)h; }
+void orc_set_threads(void* h, int t) { ((Scene*)h)->P.threads = t > 0 ? t : 1; }
+void orc_set_solver(void* h, int solver, double tol) { ((Scene*)h)->P.solver = solver; ((Scene*)h)->P.pcgTol = tol; }
+// sizes: nT, nBN, nBE, nBT, nHinge
+void orc_scene_sizes(void* h, int* out) {
+    Scene* S = (Scene*)h;
+    out[0] = S->nT; out[1] = (int)S->bNode.size(); out[2] = (int)S->bEdge.size() / 2;
+    out[3] = (int)S->bTri.size() / 3; out[4] = (int)S->hinge.size() / 4;
+}
+// scalars: kappa, kappaMax, dHat2, lambda, mu, kBend
+void orc_scene_scalars(void* h, double* out) {
+    Scene* S = (Scene*)h;
+    out[0] = S->kappa; out[1] = S->kappaMax; out[2] = S->dHat2; out[3] = S->lam; out[4] = S->mu; out[5] = S->kBend;
+}
+void orc_scene_arrays(void* h, double* mass, double* Binv, double* vol, int* tri, int* hinge, double* hingeInfo,
+                      int* bNode, int* bEdge) {
+    Scene* S = (Scene*)h;
+    if (mass) std::memcpy(mass, S->mass.data(), sizeof(double) * S->mass.size());
+    if (Binv) std::memcpy(Binv, S->Binv.data(), sizeof(double) * S->Binv.size());
+    if (vol) std::memcpy(vol, S->vol.data(), sizeof(double) * S->vol.size());
+    if (tri) std::memcpy(tri, S->tri.data(), sizeof(int) * S->tri.size());
+    if (hinge) std::memcpy(hinge, S->hinge.data(), sizeof(int) * S->hinge.size());
+    if (hingeInfo) std::memcpy(hingeInfo, S->hingeInfo.data(), sizeof(double) * S->hingeInfo.size());
+    if (bNode) std::memcpy(bNode, S->bNode.data(), sizeof(int) * S->bNode.size());
+    if (bEdge) std::memcpy(bEdge, S->bEdge.data(), sizeof(int) * S->bEdge.size());
+}
+void orc_get_state(void* h, double* x, double* v) {
+    Scene* S = (Scene*)h;
+    if (x) std::memcpy(x, S->x.data(), sizeof(double) * S->x.size());
+    if (v) std::memcpy(v, S->v.data(), sizeof(double) * S->v.size());
+}
+void orc_set_state(void* h, const double* x, const double* v) {
+    Scene* S = (Scene*)h;
+    if (x) std::memcpy(S->x.data(), x, sizeof(double) * S->x.size());
+    if (v) std::memcpy(S->v.data(), v, sizeof(double) * S->v.size());
+}
+void orc_set_kappa(void* h, double k) { ((Scene*)h)->kappa = k; }
+
+// constraint set at x: returns count, writes up to cap int4 entries
+int orc_constraint_set(void* h, const double* x, int brute, int* out, int cap) {
+    std::vector<Constraint> C;
+    constraint_set(*(Scene*)h, x, C, brute != 0);
+    int n = (int)C.size();
+    for (int i = 0; i < n && i < cap; ++i) std::memcpy(out + 4 * i, C[i].c, 4 * sizeof(int));
+    return n;
+}
+static std::vector<Constraint> toC(const int* c, int n) {
+    std::vector<Constraint> C(n);
+    for (int i = 0; i < n; ++i) std::memcpy(C[i].c, c + 4 * i, 4 * sizeof(int));
+    return C;
+}
+// energies: [membrane, bending, inertia, barrier]; returns 1 if a constraint is non-positive
+int orc_energy(void* h, const double* x, const double* xt, const int* c, int nC, double kappa, double* out4) {
+    Scene& S = *(Scene*)h;
+    double m, b;
+    elastic_energy(S, x, &m, &b);
+    out4[0] = m; out4[1] = b;
+    out4[2] = xt ? inertia_energy(S, x, xt) : 0.0;
+    int bad = 0;
+    out4[3] = barrier_energy(S, x, toC(c, nC), kappa, &bad);
+    return bad;
+}
+// gradient parts: g_el (elastic), g_bar (barrier)
+void orc_gradient(void* h, const double* x, const int* c, int nC, double kappa, double* gel, double* gbar) {
+    Scene& S = *(Scene*)h;
+    if (gel) { std::memset(gel, 0, sizeof(double) * 3 * S.nV); elastic_grad(S, x, gel); }
+    if (gbar) { std::memset(gbar, 0, sizeof(double) * 3 * S.nV); barrier_grad(S, x, toC(c, nC), kappa, gbar); }
+}
+// assembled Hessian (elastic + barrier + M, DBC projected) in CSR; call with rp only to get nnz
+static CSR g_lastA;
+int orc_hessian(void* h, const double* x, const int* c, int nC, double kappa, int which) {
+    Scene& S = *(Scene*)h;
+    std::vector<Block> blocks;
+    if (which & 1) elastic_hess(S, x, blocks);
+    if (which & 2) barrier_hess(S, x, toC(c, nC), kappa, blocks);
+    assemble(S, blocks, g_lastA);
+    return (int)g_lastA.val.size();
+}
+void orc_hessian_fetch(int* rp, int* ci, double* val) {
+    std::memcpy(rp, g_lastA.rp.data(), sizeof(int) * g_lastA.rp.size());
+    std::memcpy(ci, g_lastA.ci.data(), sizeof(int) * g_lastA.ci.size());
+    std::memcpy(val, g_lastA.val.data(), sizeof(double) * g_lastA.val.size());
+}
+// local barrier Hessian blocks before assembly (for per-constraint parity)
+int orc_barrier_blocks(void* h, const double* x, const int* c, int nC, double kappa, int* rc, double* blk, int cap) {
+    Scene& S = *(Scene*)h;
+    std::vector<Block> blocks;
+    barrier_hess(S, x, toC(c, nC), kappa, blocks);
+    int n = (int)blocks.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        rc[2 * i] = blocks[i].r; rc[2 * i + 1] = blocks[i].c;
+        std::memcpy(blk + 9 * i, blocks[i].b, 9 * sizeof(double));
+    }
+    return n;
+}
+void orc_dist2_list(void* h, const double* x, const int* c, int nC, double* d2) {
+    std::vector<double> d;
+    constraint_dist2(*(Scene*)h, x, toC(c, nC), d);
+    std::memcpy(d2, d.data(), sizeof(double) * d.size());
+}
+double orc_ccd(void* h, const double* x, const double* p, double alpha, double* clamped) {
+    return ccd_stepsize(*(Scene*)h, x, p, alpha, false, clamped);
+}
+int orc_solve(int n, const int* rp, const int* ci, const double* val, const double* b, double* x, int solver, double tol,
+              int* iters) {
+    CSR A; A.n = n;
+    A.rp.assign(rp, rp + n + 1);
+    A.ci.assign(ci, ci + rp[n]);
+    A.val.assign(val, val + rp[n]);
+    if (solver == 0) return solve_direct(A, b, x);
+    return solve_pcg(A, b, x, tol, 1000000, iters);
+}
+// stats: newtonIters, nContacts, pcgIters, useGD, lsTrials, kappa, lastRes, E, tHess, tSolve, tCS, tCCD, tEnergy, tGrad, tTotal
+int orc_step(void* h, double* stats) {
+    StepStats st;
+    int rc = advance_one_step(*(Scene*)h, st);
+    if (stats) {
+        double s[15] = {(double)st.newtonIters, (double)st.nContacts, (double)st.pcgIters, (double)st.useGDCount, (double)st.lsTrials,
+                        st.kappa, st.lastRes, st.E, st.tHessian, st.tSolve, st.tCS, st.tCCD, st.tEnergy, st.tGrad, st.tTotal};
+        std::memcpy(stats, s, sizeof(s));
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+// CPU FP64 oracle for the C-IPC contact-and-Newton hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This is a plain C++17 restatement of the reference
+// algorithm (ipc-sim/Codim-IPC, Library/FEM/Shell/IMPLICIT_EULER.h and the files it
+// calls).  It is the checker for the HIP path and the CPU baseline; nothing in the
+// product links or calls it.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it.
+//
+// Every function names the reference file:line it restates.  Small fixed-size
+// linear algebra is written here (no Eigen).  Compiled with -ffp-contract=off so
+// that classification predicates round exactly like the HIP kernels.
+//
+// Parity status of each part is listed in the header comment of the file that
+// implements it.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <vector>
+#include <array>
+#include <string>
+
+namespace orc {
+
+// ---------------------------------------------------------------- vec3
+struct V3 { double x, y, z; };
+inline V3 v3(const double* p) { return {p[0], p[1], p[2]}; }
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double sq(V3 a) { return dot(a, a); }
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// ---------------------------------------------------------------- distances
+// squared distances and region classifiers (Math/Distance/*.h)
+double dist2_pp(V3 a, V3 b);
+double dist2_pe(V3 p, V3 e0, V3 e1);
+double dist2_pt(V3 p, V3 t0, V3 t1, V3 t2);
+double dist2_ee(V3 a0, V3 a1, V3 b0, V3 b1);
+int pe_type(V3 p, V3 e0, V3 e1);
+int pt_type(V3 p, V3 t0, V3 t1, V3 t2);
+int ee_type(V3 a0, V3 a1, V3 b0, V3 b1);
+double dist2_pt_unclassified(V3 p, V3 t0, V3 t1, V3 t2);
+double dist2_ee_unclassified(V3 a0, V3 a1, V3 b0, V3 b1);
+double dist2_pe_unclassified(V3 p, V3 e0, V3 e1);
+
+// derivatives: g has 3*n entries, H is (3n)x(3n) row-major, n = 2,3,4 vertices
+void dist2_pp_gh(V3 a, V3 b, double* g, double* H);
+void dist2_pe_gh(V3 p, V3 e0, V3 e1, double* g, double* H);
+void dist2_pt_gh(V3 p, V3 t0, V3 t1, V3 t2, double* g, double* H);
+void dist2_ee_gh(V3 a0, V3 a1, V3 b0, V3 b1, double* g, double* H);
+
+// edge-edge mollifier (Math/Distance/EDGE_EDGE_MOLLIFIER.h)
+double ee_cross_norm2(V3 a0, V3 a1, V3 b0, V3 b1);
+double ee_mollifier_threshold(V3 a0r, V3 a1r, V3 b0r, V3 b1r);
+// e, 12-gradient, 12x12 Hessian (gradient/Hessian zero when c >= eps)
+void ee_mollifier(V3 a0, V3 a1, V3 b0, V3 b1, double eps, double* e, double* g, double* H);
+
+// log barrier, Math/BARRIER.h:10-62 (non-elastic branch)
+double barrier(double d, double dhat, double kappa);
+double barrier_g(double d, double dhat, double kappa);
+double barrier_h(double d, double dhat, double kappa);
+
+// symmetric eigen-decomposition (Householder tridiagonalisation + implicit QL)
+// eigenvalues ascending in w, eigenvectors in columns of V (row-major n x n)
+void sym_eig(int n, const double* A, double* w, double* V);
+// Math/UTILS.h:10-27: clamp negative eigenvalues to zero
+void make_pd(int n, double* A);
+
+// ---------------------------------------------------------------- elasticity
+// Neo-Hookean membrane, FEM/Shell/MEMBRANE.h:7-290 (useNH = true)
+double membrane_energy(V3 x0, V3 x1, V3 x2, const double* B, double vol, double lam, double mu);
+void membrane_grad(V3 x0, V3 x1, V3 x2, const double* B, double vol, double lam, double mu, double* g);
+void membrane_hess(V3 x0, V3 x1, V3 x2, const double* B, double vol, double lam, double mu, double* H);
+
+// dihedral angle, Math/DIHEDRAL_ANGLE.h:10-24 (angle and sign convention)
+double dihedral_angle(V3 x0, V3 x1, V3 x2, V3 x3);
+void dihedral_angle_grad(V3 x0, V3 x1, V3 x2, V3 x3, double* g);
+void dihedral_angle_hess(V3 x0, V3 x1, V3 x2, V3 x3, double* H);
+
+// ---------------------------------------------------------------- scene
+struct Params {
+    double rho = 200, E = 1e6, nu = 0.3, thick = 1e-3, bendMult = 1;
+    double dt = 0.01, dHat = 1e-3, xi = 0, PNTol = 1e-2;
+    double grav[3] = {0, -9.81, 0};
+    double kappaStiffMult = 1;
+    int collision = 1;
+    int solver = 0;            // 0 = direct (sparse Cholesky), 1 = block-Jacobi PCG
+    double pcgTol = 1e-5;
+    int pcgMaxIter = 100000;
+    int threads = 1;
+};
+
+struct Scene {
+    Params P;
+    int nV = 0, nT = 0;
+    std::vector<double> X0;            // rest positions (3 nV) = nodeAttr.x0
+    std::vector<double> x, v;          // state
+    std::vector<int> tri;              // 3 nT (after degenerate filter)
+    std::vector<char> isDBC;
+    std::vector<int> dbcIdx;
+    std::vector<double> dbcTarget;     // 3 per DBC entry
+    // derived (DISCRETE_SHELL.h:218-357, 555-577; MESHIO.h:728-835)
+    std::vector<double> mass;          // lumped, per vertex
+    std::vector<double> Binv;          // inverse rest first fundamental form (3 per tri: 00, 01, 11)
+    std::vector<double> vol;           // area * thickness
+    double lam = 0, mu = 0, kBend = 0;
+    std::vector<int> hinge;            // 4 per hinge
+    std::vector<double> hingeInfo;     // theta_bar, |e_bar|, h_bar
+    std::vector<int> bNode, bEdge, bTri;
+    double kappa = 0, kappaMax = 0, dHat2 = 0;
+    double bboxDiag = 0;
+};
+
+// constraint: 4 ints in the reference encoding (FEM/IPC.h:144-661)
+struct Constraint { int c[4]; };
+
+void scene_init(Scene& S);                       // A22
+// contact
+void constraint_set(const Scene& S, const double* x, std::vector<Constraint>& out,
+                    bool bruteForce = false);
+double barrier_energy(const Scene& S, const double* x, const std::vector<Constraint>& C, double kappa,
+                      int* bad = nullptr);
+void barrier_grad(const Scene& S, const double* x, const std::vector<Constraint>& C, double kappa, double* g);
+// local Hessian blocks per constraint, appended as (row vertex, col vertex, 3x3 block)
+struct Block { int r, c; double b[9]; };
+void barrier_hess(const Scene& S, const double* x, const std::vector<Constraint>& C, double kappa,
+                  std::vector<Block>& out);
+void constraint_dist2(const Scene& S, const double* x, const std::vector<Constraint>& C, std::vector<double>& d2);
+double ccd_stepsize(const Scene& S, const double* x, const double* p, double alpha, bool bruteForce = false,
+                    double* clampedAlpha = nullptr);
+// elastic + inertia
+double elastic_energy(const Scene& S, const double* x, double* memb = nullptr, double* bend = nullptr);
+void elastic_grad(const Scene& S, const double* x, double* g);
+void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out);
+double inertia_energy(const Scene& S, const double* x, const double* xt);
+
+// assembled system in CSR (scalar) form
+struct CSR { int n = 0; std::vector<int> rp, ci; std::vector<double> val; };
+void assemble(const Scene& S, const std::vector<Block>& blocks, CSR& A);   // + M, DBC projection
+int solve_direct(const CSR& A, const double* b, double* x);                // 0 ok, 1 not SPD
+int solve_pcg(const CSR& A, const double* b, double* x, double tol, int maxIter, int* iters);
+
+struct StepStats {
+    int newtonIters = 0, nContacts = 0, pcgIters = 0, useGDCount = 0, lsTrials = 0;
+    double kappa = 0, lastRes = 0, E = 0;
+    double tHessian = 0, tSolve = 0, tCS = 0, tCCD = 0, tEnergy = 0, tGrad = 0, tTotal = 0;
+};
+int advance_one_step(Scene& S, StepStats& st);
+
+}  // namespace orc

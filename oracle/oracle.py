@@ -1,0 +1,280 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker -- never by the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.POINTER(ctypes.c_int)
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(D)
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(I)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        L.orc_dist2.restype = ctypes.c_double
+        L.orc_dist2_unclassified.restype = ctypes.c_double
+        L.orc_scene_create.restype = ctypes.c_void_p
+        L.orc_ccd.restype = ctypes.c_double
+        for name in ("orc_scene_destroy", "orc_scene_sizes", "orc_scene_scalars", "orc_scene_arrays", "orc_get_state",
+                     "orc_set_state", "orc_set_kappa", "orc_constraint_set", "orc_energy", "orc_gradient", "orc_hessian",
+                     "orc_barrier_blocks", "orc_dist2_list", "orc_ccd", "orc_step", "orc_set_threads", "orc_set_solver"):
+            getattr(L, name).argtypes = None
+        L.orc_set_kappa.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_ccd.argtypes = [ctypes.c_void_p, D, D, ctypes.c_double, D]
+        L.orc_set_solver.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.orc_barrier.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, D]
+        L.orc_energy.argtypes = [ctypes.c_void_p, D, D, I, ctypes.c_int, ctypes.c_double, D]
+        L.orc_gradient.argtypes = [ctypes.c_void_p, D, I, ctypes.c_int, ctypes.c_double, D, D]
+        L.orc_hessian.argtypes = [ctypes.c_void_p, D, I, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.orc_barrier_blocks.argtypes = [ctypes.c_void_p, D, I, ctypes.c_int, ctypes.c_double, I, D, ctypes.c_int]
+        L.orc_membrane.argtypes = [D, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D, D, D]
+        L.orc_solve.argtypes = [ctypes.c_int, I, I, D, D, D, ctypes.c_int, ctypes.c_double, I]
+        _LIB = L
+    return _LIB
+
+
+# ------------------------------------------------------------------ primitives
+KIND = {"pp": 0, "pe": 1, "pt": 2, "ee": 3}
+
+
+def dist2(kind, pts):
+    p = np.ascontiguousarray(pts, dtype=np.float64)
+    return lib().orc_dist2(KIND[kind], _dp(p))
+
+
+def dist2_gh(kind, pts):
+    p = np.ascontiguousarray(pts, dtype=np.float64)
+    n = p.size
+    g = np.zeros(n)
+    H = np.zeros((n, n))
+    lib().orc_dist2_gh(KIND[kind], _dp(p), _dp(g), _dp(H))
+    return g, H
+
+
+def dtype(kind, pts):
+    p = np.ascontiguousarray(pts, dtype=np.float64)
+    return lib().orc_type(KIND[kind], _dp(p))
+
+
+def dist2_unclassified(kind, pts):
+    p = np.ascontiguousarray(pts, dtype=np.float64)
+    return lib().orc_dist2_unclassified(KIND[kind], _dp(p))
+
+
+def mollifier(pts, rest):
+    p = np.ascontiguousarray(pts, dtype=np.float64)
+    r = np.ascontiguousarray(rest, dtype=np.float64)
+    e = np.zeros(1)
+    eps = np.zeros(1)
+    g = np.zeros(12)
+    H = np.zeros((12, 12))
+    lib().orc_mollifier(_dp(p), _dp(r), _dp(e), _dp(g), _dp(H), _dp(eps))
+    return e[0], g, H, eps[0]
+
+
+def barrier(d, dhat, kappa):
+    out = np.zeros(3)
+    lib().orc_barrier(d, dhat, kappa, _dp(out))
+    return out
+
+
+def sym_eig(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w = np.zeros(n)
+    V = np.zeros((n, n))
+    lib().orc_sym_eig(n, _dp(A), _dp(w), _dp(V))
+    return w, V
+
+
+def make_pd(A):
+    A = np.array(A, dtype=np.float64, copy=True)
+    lib().orc_make_pd(A.shape[0], _dp(A))
+    return A
+
+
+def membrane(x, B, vol, lam, mu):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    E = np.zeros(1)
+    g = np.zeros(9)
+    H = np.zeros((9, 9))
+    lib().orc_membrane(_dp(x), _dp(B), vol, lam, mu, _dp(E), _dp(g), _dp(H))
+    return E[0], g, H
+
+
+def dihedral(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    th = np.zeros(1)
+    g = np.zeros(12)
+    H = np.zeros((12, 12))
+    lib().orc_dihedral(_dp(x), _dp(th), _dp(g), _dp(H))
+    return th[0], g, H
+
+
+def solve(A_csr, b, solver=0, tol=1e-5):
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A_csr)
+    A.sort_indices()
+    rp = A.indptr.astype(np.int32)
+    ci = A.indices.astype(np.int32)
+    val = A.data.astype(np.float64)
+    x = np.zeros(A.shape[0])
+    it = np.zeros(1, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    rc = lib().orc_solve(A.shape[0], _ip(rp), _ip(ci), _dp(val), _dp(b), _dp(x), solver, tol, _ip(it))
+    return rc, x, int(it[0])
+
+
+# ------------------------------------------------------------------ scenes
+class OracleScene:
+    """A scene held by the oracle; mirrors the state kept by the HIP context."""
+
+    def __init__(self, sd, solver=0, pcg_tol=1e-5, threads=1):
+        P = sd.params
+        g = P.get("gravity", (0.0, -9.81, 0.0))
+        self._params = np.array([P["rho"], P["E"], P["nu"], P["thick"], P["bendMult"], P["dt"], P["dHat"], P["xi"],
+                                 P["PNTol"], g[0], g[1], g[2], P.get("kappaStiffMult", 1.0), pcg_tol], dtype=np.float64)
+        self._iparams = np.array([P.get("collision", 1), solver, 1000000, threads], dtype=np.int32)
+        L = lib()
+        self.sd = sd
+        self.h = ctypes.c_void_p(L.orc_scene_create(
+            ctypes.c_int(sd.nV), _dp(sd.X), ctypes.c_int(sd.nT), _ip(sd.tris), ctypes.c_int(len(sd.dbc_idx)),
+            _ip(sd.dbc_idx), _dp(sd.dbc_target), _dp(self._params), _ip(self._iparams)))
+        s = np.zeros(5, dtype=np.int32)
+        L.orc_scene_sizes(self.h, _ip(s))
+        self.nT, self.nBN, self.nBE, self.nBT, self.nH = [int(v) for v in s]
+        self.nV = sd.nV
+
+    def __del__(self):
+        try:
+            lib().orc_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_threads(self, t):
+        lib().orc_set_threads(self.h, ctypes.c_int(t))
+
+    def set_solver(self, solver, tol=1e-5):
+        lib().orc_set_solver(self.h, solver, tol)
+
+    def scalars(self):
+        out = np.zeros(6)
+        lib().orc_scene_scalars(self.h, _dp(out))
+        return dict(kappa=out[0], kappaMax=out[1], dHat2=out[2], lam=out[3], mu=out[4], kBend=out[5])
+
+    def arrays(self):
+        mass = np.zeros(self.nV)
+        Binv = np.zeros(3 * self.nT)
+        vol = np.zeros(self.nT)
+        tri = np.zeros(3 * self.nT, dtype=np.int32)
+        hinge = np.zeros(4 * self.nH, dtype=np.int32)
+        hinfo = np.zeros(3 * self.nH)
+        bnode = np.zeros(self.nBN, dtype=np.int32)
+        bedge = np.zeros(2 * self.nBE, dtype=np.int32)
+        lib().orc_scene_arrays(self.h, _dp(mass), _dp(Binv), _dp(vol), _ip(tri), _ip(hinge), _dp(hinfo), _ip(bnode),
+                               _ip(bedge))
+        return dict(mass=mass, Binv=Binv.reshape(-1, 3), vol=vol, tri=tri.reshape(-1, 3), hinge=hinge.reshape(-1, 4),
+                    hinge_info=hinfo.reshape(-1, 3), bnode=bnode, bedge=bedge.reshape(-1, 2))
+
+    def get_state(self):
+        x = np.zeros(3 * self.nV)
+        v = np.zeros(3 * self.nV)
+        lib().orc_get_state(self.h, _dp(x), _dp(v))
+        return x, v
+
+    def set_state(self, x=None, v=None):
+        xx = None if x is None else np.ascontiguousarray(x, dtype=np.float64).ravel()
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64).ravel()
+        lib().orc_set_state(self.h, _dp(xx), _dp(vv))
+
+    def set_kappa(self, k):
+        lib().orc_set_kappa(self.h, k)
+
+    def constraint_set(self, x, brute=False):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        cap = 1 << 16
+        while True:
+            out = np.zeros(4 * cap, dtype=np.int32)
+            n = lib().orc_constraint_set(self.h, _dp(x), int(brute), _ip(out), cap)
+            if n <= cap:
+                return out[:4 * n].reshape(-1, 4)
+            cap = n
+
+    def energy(self, x, xt, C, kappa):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        xt = None if xt is None else np.ascontiguousarray(xt, dtype=np.float64).ravel()
+        C = np.ascontiguousarray(C, dtype=np.int32).reshape(-1, 4)
+        out = np.zeros(4)
+        bad = lib().orc_energy(self.h, _dp(x), _dp(xt), _ip(C), len(C), kappa, _dp(out))
+        return out, bad
+
+    def gradient(self, x, C, kappa):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        C = np.ascontiguousarray(C, dtype=np.int32).reshape(-1, 4)
+        ge = np.zeros(3 * self.nV)
+        gb = np.zeros(3 * self.nV)
+        lib().orc_gradient(self.h, _dp(x), _ip(C), len(C), kappa, _dp(ge), _dp(gb))
+        return ge, gb
+
+    def hessian(self, x, C, kappa, which=3):
+        import scipy.sparse as sp
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        C = np.ascontiguousarray(C, dtype=np.int32).reshape(-1, 4)
+        nnz = lib().orc_hessian(self.h, _dp(x), _ip(C), len(C), kappa, which)
+        n = 3 * self.nV
+        rp = np.zeros(n + 1, dtype=np.int32)
+        ci = np.zeros(nnz, dtype=np.int32)
+        val = np.zeros(nnz)
+        lib().orc_hessian_fetch(_ip(rp), _ip(ci), _dp(val))
+        return sp.csr_matrix((val, ci, rp), shape=(n, n))
+
+    def barrier_blocks(self, x, C, kappa):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        C = np.ascontiguousarray(C, dtype=np.int32).reshape(-1, 4)
+        cap = 16 * len(C) + 1
+        rc = np.zeros(2 * cap, dtype=np.int32)
+        blk = np.zeros(9 * cap)
+        n = lib().orc_barrier_blocks(self.h, _dp(x), _ip(C), len(C), kappa, _ip(rc), _dp(blk), cap)
+        return rc[:2 * n].reshape(-1, 2), blk[:9 * n].reshape(-1, 3, 3)
+
+    def dist2_list(self, x, C):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        C = np.ascontiguousarray(C, dtype=np.int32).reshape(-1, 4)
+        d2 = np.zeros(len(C))
+        lib().orc_dist2_list(self.h, _dp(x), _ip(C), len(C), _dp(d2))
+        return d2
+
+    def ccd(self, x, p, alpha=1.0):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        p = np.ascontiguousarray(p, dtype=np.float64).ravel()
+        cl = np.zeros(1)
+        a = lib().orc_ccd(self.h, _dp(x), _dp(p), alpha, _dp(cl))
+        return a, cl[0]
+
+    STAT_NAMES = ("newton_iters", "contacts", "pcg_iters", "gd_fallbacks", "ls_trials", "kappa", "last_res", "E",
+                  "t_hessian", "t_solve", "t_constraint_set", "t_ccd", "t_energy", "t_grad", "t_total")
+
+    def step(self):
+        st = np.zeros(15)
+        rc = lib().orc_step(self.h, _dp(st))
+        return rc, dict(zip(self.STAT_NAMES, st.tolist()))

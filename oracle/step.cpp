@@ -1,0 +1,272 @@
+// Oracle: scene set-up and one implicit-Euler step by projected Newton.
+// TEST INFRASTRUCTURE ONLY (see oracle.h).
+//
+// scene_init restates Initialize_Discrete_Shell<double,3,KL=false>
+// (FEM/Shell/DISCRETE_SHELL.h:218-357: degenerate-triangle filter, edge2tri, hinge
+// stencils and rest data :169-213, lumped mass, lambda/mu, hinge k), the surface
+// primitive extraction (Utils/MESHIO.h:728-835) and Initialize_EIPC<double,false>
+// (DISCRETE_SHELL.h:555-577: kappa from the mean lumped mass).
+//
+// advance_one_step restates Advance_One_Step_IE_Discrete_Shell<double,3,false,false,
+// false> (FEM/Shell/IMPLICIT_EULER.h:151-902) for the configuration this build
+// covers: no strain limiting, no fibres, no tets, no friction, no stitches, static
+// or CCD-unblocked Dirichlet nodes, and Line_Search (IMPLICIT_EULER.h:9-149).
+// Parity pins: energy non-increase per accepted step, intersection-free states,
+// stopping rule; trajectories are the reference for the HIP path.
+#include "oracle.h"
+#include <algorithm>
+#include <map>
+#include <set>
+#include <chrono>
+#include <deque>
+#include <cstdio>
+
+namespace orc {
+
+static inline V3 X3(const double* x, int i) { return {x[3 * i], x[3 * i + 1], x[3 * i + 2]}; }
+static double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+void scene_init(Scene& S) {
+    const Params& P = S.P;
+    // degenerate filter, DISCRETE_SHELL.h:228-244
+    std::vector<int> kept;
+    for (int t = 0; t < (int)S.tri.size() / 3; ++t) {
+        V3 a = X3(S.X0.data(), S.tri[3 * t]), b = X3(S.X0.data(), S.tri[3 * t + 1]), c = X3(S.X0.data(), S.tri[3 * t + 2]);
+        V3 e1 = b - a, e2 = c - a;
+        double I00 = sq(e1), I01 = dot(e1, e2), I11 = sq(e2);
+        if (I00 * I11 - I01 * I01 != 0) for (int k = 0; k < 3; ++k) kept.push_back(S.tri[3 * t + k]);
+    }
+    S.tri = kept;
+    S.nT = (int)S.tri.size() / 3;
+    if (S.x.empty()) S.x = S.X0;
+    if (S.v.empty()) S.v.assign(3 * S.nV, 0.0);
+    S.isDBC.assign(S.nV, 0);
+    for (int d : S.dbcIdx) S.isDBC[d] = 1;
+    // edge2tri and surface primitives, MESHIO.h:728-766
+    std::map<std::pair<int, int>, int> edge2tri;
+    std::set<std::pair<int, int>> edgeSet;
+    std::vector<char> isBN(S.nV, 0);
+    for (int t = 0; t < S.nT; ++t) {
+        const int* v = &S.tri[3 * t];
+        edge2tri[{v[0], v[1]}] = t; edge2tri[{v[1], v[2]}] = t; edge2tri[{v[2], v[0]}] = t;
+    }
+    for (int t = 0; t < S.nT; ++t) {
+        const int* v = &S.tri[3 * t];
+        if (!edgeSet.count({v[1], v[0]})) edgeSet.insert({v[0], v[1]});
+        if (!edgeSet.count({v[2], v[1]})) edgeSet.insert({v[1], v[2]});
+        if (!edgeSet.count({v[0], v[2]})) edgeSet.insert({v[2], v[0]});
+        isBN[v[0]] = isBN[v[1]] = isBN[v[2]] = 1;
+    }
+    S.bNode.clear(); S.bEdge.clear();
+    for (int i = 0; i < S.nV; ++i) if (isBN[i]) S.bNode.push_back(i);
+    for (auto& e : edgeSet) { S.bEdge.push_back(e.first); S.bEdge.push_back(e.second); }
+    S.bTri = S.tri;
+    // rest first fundamental form (inverted) and hinges, DISCRETE_SHELL.h:132-213
+    S.Binv.resize(3 * S.nT);
+    for (int t = 0; t < S.nT; ++t) {
+        V3 a = X3(S.X0.data(), S.tri[3 * t]), b = X3(S.X0.data(), S.tri[3 * t + 1]), c = X3(S.X0.data(), S.tri[3 * t + 2]);
+        V3 e1 = b - a, e2 = c - a;
+        double I00 = sq(e1), I01 = dot(e1, e2), I11 = sq(e2);
+        double det = I00 * I11 - I01 * I01;
+        S.Binv[3 * t] = I11 / det; S.Binv[3 * t + 1] = -I01 / det; S.Binv[3 * t + 2] = I00 / det;
+    }
+    S.hinge.clear(); S.hingeInfo.clear();
+    for (auto& e : edgeSet) {
+        auto fa = edge2tri.find({e.first, e.second});
+        if (fa == edge2tri.end()) continue;
+        const int* A = &S.tri[3 * fa->second];
+        int v0 = -1;
+        for (int j = 0; j < 3; ++j) if (A[j] == e.second) { v0 = A[(j + 1) % 3]; break; }
+        auto fb = edge2tri.find({e.second, e.first});
+        if (fb == edge2tri.end()) continue;
+        const int* B = &S.tri[3 * fb->second];
+        int v3 = B[0];
+        for (int j = 0; j < 3; ++j) if (B[j] == e.first) { v3 = B[(j + 1) % 3]; break; }
+        int st[4] = {v0, e.first, e.second, v3};
+        for (int k = 0; k < 4; ++k) S.hinge.push_back(st[k]);
+        V3 X0 = X3(S.X0.data(), st[0]), X1 = X3(S.X0.data(), st[1]), X2 = X3(S.X0.data(), st[2]), Xq = X3(S.X0.data(), st[3]);
+        double th = dihedral_angle(X0, X1, X2, Xq);
+        double el = std::sqrt(sq(X1 - X2));
+        V3 n1 = cross(X1 - X0, X2 - X0), n2 = cross(X2 - Xq, X1 - Xq);
+        S.hingeInfo.push_back(th);
+        S.hingeInfo.push_back(el);
+        S.hingeInfo.push_back((std::sqrt(sq(n1)) + std::sqrt(sq(n2))) / (el * 6));
+    }
+    // lumped mass, DISCRETE_SHELL.h:279-299
+    S.mass.assign(S.nV, 0.0);
+    S.vol.resize(S.nT);
+    for (int t = 0; t < S.nT; ++t) {
+        const int* v = &S.tri[3 * t];
+        V3 a = X3(S.X0.data(), v[0]), b = X3(S.X0.data(), v[1]), c = X3(S.X0.data(), v[2]);
+        double area = std::sqrt(sq(cross(b - a, c - a))) / 2;
+        double mp = area * P.thick * P.rho / 3;
+        S.mass[v[0]] += mp; S.mass[v[1]] += mp; S.mass[v[2]] += mp;
+        S.vol[t] = area * P.thick;
+    }
+    S.lam = P.E * P.nu / (1.0 - P.nu * P.nu);
+    S.mu = P.E / (2.0 * (1.0 + P.nu));
+    S.kBend = P.bendMult * (P.E * std::pow(P.thick, 3) / (24 * (1.0 - P.nu * P.nu)));
+    // Initialize_EIPC<double,false>, DISCRETE_SHELL.h:563-574
+    S.dHat2 = P.dHat * P.dHat;
+    double Hb = barrier_h(1.0e-16, S.dHat2, 1.0);
+    double mmean = 0;
+    for (int i = 0; i < S.nV; ++i) mmean += 3 * S.mass[i];
+    mmean /= 3 * S.nV;
+    S.kappa = P.kappaStiffMult * 1.0e11 * mmean * 3 / (4.0e-16 * Hb);
+    S.kappaMax = 100 * S.kappa;
+}
+
+// full incremental potential at x for constraint set C
+static double inc_potential(const Scene& S, const double* x, const double* xt, const std::vector<Constraint>& C,
+                            double kappa, bool withContact, int* bad) {
+    double E = elastic_energy(S, x) + inertia_energy(S, x, xt);
+    if (withContact) E += barrier_energy(S, x, C, kappa, bad);
+    return E;
+}
+
+int advance_one_step(Scene& S, StepStats& st) {
+    const Params& P = S.P;
+    const int n = 3 * S.nV;
+    const double h = P.dt;
+    double t0 = now();
+    st = StepStats();
+    std::vector<double>& x = S.x;
+    std::vector<double> xn = x, xt(n);
+    // x_tilde = x + h v + h^2 M^{-1} b, IMPLICIT_EULER.h:195-217 (b = m g)
+    for (int i = 0; i < S.nV; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double a = S.mass[i] > 0 ? P.grav[k] : 0.0;
+            xt[3 * i + k] = x[3 * i + k] + h * S.v[3 * i + k] + h * h * a;
+        }
+    // Dirichlet targets, IMPLICIT_EULER.h:304-401
+    std::vector<double> disp(n, 0.0);
+    for (size_t k = 0; k < S.dbcIdx.size(); ++k) {
+        int v = S.dbcIdx[k];
+        for (int d = 0; d < 3; ++d) disp[3 * v + d] = S.dbcTarget[3 * k + d] - x[3 * v + d];
+    }
+    double dbcAlpha = 1;
+    if (P.collision) dbcAlpha = ccd_stepsize(S, x.data(), disp.data(), 1.0);
+    if (dbcAlpha != 1) return -2;   // augmented-Lagrangian Dirichlet path is not covered
+    for (size_t k = 0; k < S.dbcIdx.size(); ++k)
+        for (int d = 0; d < 3; ++d) x[3 * S.dbcIdx[k] + d] = S.dbcTarget[3 * k + d];
+
+    std::vector<Constraint> C, Cprev;
+    std::vector<double> d2prev;
+    if (P.collision) constraint_set(S, x.data(), C);
+    int bad = 0;
+    double Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad);
+    if (bad) return -1;
+    std::deque<double> resRecord;
+    bool useGD = false;
+    double L2Norm = 0;
+    int PNIter = 0;
+    std::vector<double> g(n), rhs(n), sol(n), xprev(n);
+    CSR A;
+    do {
+        // gradient, INC_POTENTIAL.h:154-275, DBC rows zeroed IMPLICIT_EULER.h:476-491
+        double ta = now();
+        std::fill(g.begin(), g.end(), 0.0);
+        elastic_grad(S, x.data(), g.data());
+        for (int i = 0; i < S.nV; ++i)
+            for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
+        if (P.collision) barrier_grad(S, x.data(), C, S.kappa, g.data());
+        for (int v : S.dbcIdx) for (int k = 0; k < 3; ++k) g[3 * v + k] = 0;
+        for (int i = 0; i < n; ++i) rhs[i] = -g[i];
+        st.tGrad += now() - ta;
+        // Hessian + solve, IMPLICIT_EULER.h:494-554
+        bool solved = false;
+        if (!useGD) {
+            ta = now();
+            std::vector<Block> blocks;
+            elastic_hess(S, x.data(), blocks);
+            if (P.collision) barrier_hess(S, x.data(), C, S.kappa, blocks);
+            assemble(S, blocks, A);
+            st.tHessian += now() - ta;
+            ta = now();
+            int rc;
+            if (P.solver == 0) rc = solve_direct(A, rhs.data(), sol.data());
+            else { int it = 0; rc = solve_pcg(A, rhs.data(), sol.data(), P.pcgTol, P.pcgMaxIter, &it); st.pcgIters += it; }
+            st.tSolve += now() - ta;
+            solved = (rc == 0);
+            if (!solved) { useGD = true; st.useGDCount++; }
+        }
+        if (useGD) sol = rhs;
+        // line search, IMPLICIT_EULER.h:9-149
+        xprev = x;
+        double alpha = 1.0;
+        if (P.collision) {
+            ta = now();
+            alpha = ccd_stepsize(S, x.data(), sol.data(), alpha);
+            st.tCCD += now() - ta;
+            if (alpha == 0) return -3;
+        }
+        double feasibleAlpha = alpha;
+        double E;
+        bool valid = true;
+        do {
+            for (int i = 0; i < n; ++i) x[i] = xprev[i] + alpha * sol[i];
+            st.lsTrials++;
+            if (P.collision) {
+                ta = now();
+                constraint_set(S, x.data(), C);
+                st.tCS += now() - ta;
+                if (!C.empty()) {
+                    std::vector<double> d2;
+                    constraint_dist2(S, x.data(), C, d2);
+                    double mn = *std::min_element(d2.begin(), d2.end()) - P.xi * P.xi;
+                    if (mn <= 0) { alpha /= 2; E = Eprev + 1; continue; }
+                }
+            }
+            ta = now();
+            E = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad);
+            st.tEnergy += now() - ta;
+            if (bad) return -1;
+            alpha /= 2.0;
+        } while (E > Eprev || !valid);
+        Eprev = E;
+        // kappa adaptation, IMPLICIT_EULER.h:566-595
+        if (!Cprev.empty()) {
+            std::vector<double> cur;
+            constraint_dist2(S, x.data(), Cprev, cur);
+            bool upd = false;
+            for (size_t i = 0; i < cur.size(); ++i)
+                if (d2prev[i] < 1e-18 && cur[i] < d2prev[i]) { upd = true; break; }
+            if (upd && S.kappa < S.kappaMax) {
+                S.kappa *= 2;
+                Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad);
+            }
+        }
+        std::vector<double> d2;
+        if (!C.empty()) constraint_dist2(S, x.data(), C, d2);
+        Cprev = C; d2prev = d2;
+        // stopping rule, IMPLICIT_EULER.h:647-710, 866
+        double s = 0;
+        for (int i = 0; i < n; ++i) s += sol[i] * sol[i];
+        L2Norm = std::sqrt(s / (S.nV - (int)S.dbcIdx.size())) / h;
+        PNIter++;
+        resRecord.push_back(L2Norm);
+        if (resRecord.size() > 3) resRecord.pop_front();
+        double cur = L2Norm;
+        L2Norm = *std::max_element(resRecord.begin(), resRecord.end());
+        if (useGD) L2Norm = P.PNTol * 10;
+        if (alpha * 2 < 1e-8 && feasibleAlpha > 1e-8) {
+            if (!useGD) {
+                if (cur < P.PNTol) { resRecord.assign(3, cur); }
+                else useGD = true;
+            }
+        } else {
+            useGD = false;
+        }
+        st.lastRes = cur;
+        if (PNIter > 2000) return -4;
+    } while (resRecord.size() < 3 || L2Norm > P.PNTol);
+    for (int i = 0; i < n; ++i) S.v[i] = (x[i] - xn[i]) / h;
+    st.newtonIters = PNIter;
+    st.nContacts = (int)C.size();
+    st.kappa = S.kappa;
+    st.E = Eprev;
+    st.tTotal = now() - t0;
+    return PNIter;
+}
+
+}  // namespace orc

@@ -1,0 +1,167 @@
+"""HIP path vs the CPU FP64 oracle, through the C ABI (include/cipc.h).
+
+Bars (stated per test): integer/index results bit-exact (constraint sets, set-up
+topology); distances bit-exact (same IEEE operation order, no contraction); energies,
+gradients and Hessians to a relative tolerance (different summation order, different
+eigensolver in the PSD projection); CCD step never later than the oracle's; PCG
+solutions to the stated relative residual of the oracle's own assembled matrix.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from cipc import Simulator, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    den = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / den)
+
+
+@pytest.fixture(scope="module")
+def stack(require_gpu):
+    sd = scenes.scene_contact_stack()
+    gpu = Simulator(sd)
+    orc = O.OracleScene(sd, threads=4)
+    # states: the initial one (every constraint kind active) and two after oracle steps
+    states = []
+    x, _ = orc.get_state()
+    states.append(x.copy())
+    for _ in range(2):
+        rc, _ = orc.step()
+        assert rc > 0
+        states.append(orc.get_state()[0].copy())
+    return sd, gpu, orc, states
+
+
+def test_setup_matches_oracle(stack):
+    sd, gpu, orc, _ = stack
+    a, g = orc.arrays(), gpu.setup_arrays()
+    assert np.array_equal(g["hinge"], a["hinge"])
+    assert np.array_equal(g["bedge"], a["bedge"])
+    assert np.array_equal(g["mass"], a["mass"])
+    assert np.array_equal(g["hinge_info"], a["hinge_info"])
+    so, sg = orc.scalars(), gpu.scalars()
+    for k in ("kappa", "kappaMax", "dHat2", "lam", "mu", "kBend"):
+        assert sg[k] == so[k], k
+
+
+def test_constraint_set_bit_exact(stack):
+    sd, gpu, orc, states = stack
+    for x in states:
+        Cg = gpu.constraint_set(x)
+        Co = orc.constraint_set(x)
+        Cb = orc.constraint_set(x, brute=True)
+        assert np.array_equal(Co, Cb)
+        assert Cg.shape == Co.shape
+        assert np.array_equal(Cg, Co)
+    assert len(gpu.constraint_set(states[0])) > 1000
+
+
+def test_min_dist2_bit_exact(stack):
+    sd, gpu, orc, states = stack
+    x = states[0]
+    C = orc.constraint_set(x)
+    d2g, mg = gpu.min_dist2(x, C)
+    d2o = orc.dist2_list(x, C)
+    assert np.array_equal(d2g, d2o)
+    assert mg == d2o.min() - sd.params["xi"] ** 2
+
+
+def test_barrier_energy_gradient(stack):
+    sd, gpu, orc, states = stack
+    kappa = orc.scalars()["kappa"]
+    for x in states:
+        C = orc.constraint_set(x)
+        eo, bad = orc.energy(x, None, C, kappa)
+        assert bad == 0
+        eg = gpu.barrier_energy(x, C, kappa)
+        assert abs(eg - eo[3]) <= 1e-11 * abs(eo[3])          # summation order only
+        _, gbo = orc.gradient(x, C, kappa)
+        gbg = gpu.barrier_gradient(x, C, kappa)
+        assert rel(gbg, gbo) <= 1e-11
+
+
+def test_barrier_hessian(stack):
+    """Barrier blocks after PSD projection (Jacobi on the GPU, Householder-QL in the oracle)."""
+    sd, gpu, orc, states = stack
+    kappa = orc.scalars()["kappa"]
+    none = np.zeros((0, 4), np.int32)
+    for x in states[:2]:
+        C = orc.constraint_set(x)
+        # the oracle always adds the mass diagonal and projects Dirichlet rows; remove that part
+        D = orc.hessian(x, C, kappa, which=2) - orc.hessian(x, none, kappa, which=0)
+        Hg = gpu.hessian(x, C, kappa, which=2)
+        assert rel(Hg.toarray(), D.toarray()) <= 1e-9
+
+
+def test_elastic_energy_gradient_hessian(stack):
+    sd, gpu, orc, states = stack
+    x = states[2]
+    eo, _ = orc.energy(x, None, np.zeros((0, 4), np.int32), 1.0)
+    eg = gpu.elastic_energy(x)
+    # near the rest shape the per-triangle energies are differences of O(1) terms, so the
+    # summed energy carries cancellation error: relative bar 1e-8
+    assert abs(eg[0] - eo[0]) <= 1e-8 * abs(eo[0])
+    assert abs(eg[1] - eo[1]) <= 1e-8 * abs(eo[1]) + 1e-30
+    geo, _ = orc.gradient(x, np.zeros((0, 4), np.int32), 1.0)
+    geg = gpu.elastic_gradient(x)
+    assert rel(geg, geo) <= 1e-10
+    Ho = orc.hessian(x, np.zeros((0, 4), np.int32), 1.0, which=1)
+    Hg = gpu.hessian(x, np.zeros((0, 4), np.int32), 1.0, which=1)
+    assert rel(Hg.toarray(), Ho.toarray()) <= 1e-9
+
+
+def test_ccd_matches_oracle(stack):
+    sd, gpu, orc, states = stack
+    rng = np.random.default_rng(7)
+    x = states[0]
+    n = x.size
+    dbc = np.zeros(sd.nV, bool)
+    dbc[sd.dbc_idx] = True
+    for scale in (1e-4, 1e-3, 1e-2):
+        p = rng.standard_normal(n) * scale
+        p.reshape(-1, 3)[dbc] = 0
+        ao, clo = orc.ccd(x, p, 1.0)
+        ag, clg = gpu.ccd_stepsize(x, p, 1.0)
+        assert clg == clo
+        assert ag <= ao * (1 + 1e-12)
+        assert ag >= ao * (1 - 1e-9)
+
+
+def test_linear_solve_residual(stack):
+    sd, gpu, orc, states = stack
+    import scipy.sparse.linalg as spla
+    kappa = orc.scalars()["kappa"]
+    x = states[0]
+    C = orc.constraint_set(x)
+    H = orc.hessian(x, C, kappa, which=3)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(x.size)
+    b.reshape(-1, 3)[sd.dbc_idx] = 0
+    sol, it, rr = gpu.linear_solve(x, C, kappa, b, tol=1e-8)
+    assert rr <= 1e-8
+    assert np.linalg.norm(H @ sol - b) <= 2e-8 * np.linalg.norm(b)
+    ref = spla.spsolve(H.tocsc(), b)
+    assert rel(sol, ref) <= 1e-5
+
+
+def test_step_trajectory(require_gpu):
+    """Five steps of the contact stack, tight Newton tolerance, FP64 on both sides."""
+    sd = scenes.scene_contact_stack()
+    sd.params["PNTol"] = 1e-6 * sd.bbox_diag()
+    gpu = Simulator(sd, pcg_tol=1e-10)
+    orc = O.OracleScene(sd, threads=4)
+    for s in range(5):
+        st = gpu.step()
+        rc, so = orc.step()
+        assert rc > 0 and st["status"] == 0
+        xg, vg = gpu.get_state()
+        xo, vo = orc.get_state()
+        disp = np.abs(xo - sd.X.ravel()).max()
+        assert np.abs(xg - xo).max() <= 1e-6 * max(disp, 1e-3), s
+        d2, m = gpu.min_dist2(xg, gpu.constraint_set(xg)) if len(gpu.constraint_set(xg)) else (None, 1.0)
+        assert m > 0
