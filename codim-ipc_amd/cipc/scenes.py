@@ -173,3 +173,26 @@ def scene_contact_stack(n=12, seed=5):
     sd = cloth_on_sphere(sheets=3, n=n, y0=0.2008, dy=0.0008, seed=seed, sphere_level=3, name="contact_stack",
                          twists=[0.0, 0.0, 10.0])
     return sd
+
+
+def scene_cloth_on_ground(n=40, gap=6e-4, seed=9, twist_deg=0.0, ground_half=1.0):
+    """One n x n sheet `gap` above the 2-triangle ground (no sphere).  The ground triangles
+    and edges cover thousands of grid cells, so they take the large-primitive path of the
+    broad phases; the sheet's diagonals run parallel to the ground diagonal, so mollified
+    edge-edge entries appear next to point-triangle and edge-edge ones."""
+    X, T = cloth_grid(n, gap)
+    if twist_deg:
+        a = math.radians(twist_deg)
+        c, sn = math.cos(a), math.sin(a)
+        X[:, 0], X[:, 2] = c * X[:, 0] - sn * X[:, 2], sn * X[:, 0] + c * X[:, 2]
+    u = splitmix64_uniform(seed, X.size).reshape(X.shape)
+    X = X + (2.0 * u - 1.0) * np.array([1e-4, 1e-4 * min(1.0, gap / 1e-3) * 0.5, 1e-4])
+    n_cloth = X.shape[0]
+    Xg, Tg = ground(ground_half)
+    Xa = np.concatenate([X, Xg])
+    Ta = np.concatenate([T, Tg + n_cloth])
+    dbc = np.arange(n_cloth, Xa.shape[0], dtype=np.int32)
+    sd = SceneData(Xa, Ta, dbc, Xa[dbc].copy(), dict(DEFAULT_PARAMS), "cloth_on_ground")
+    sd.params["PNTol"] = 1e-2 * sd.bbox_diag()
+    sd.n_cloth = n_cloth
+    return sd

@@ -116,6 +116,16 @@ class Simulator:
         self._check(rc)
         return st.as_dict()
 
+    def set_profiling(self, enable=True):
+        self._check(self.lib.cipc_set_profiling(self.h, 1 if enable else 0))
+
+    def profile(self):
+        """Event timing of the elastic SpMV inside the Newton PCG (the step's dominant kernel)."""
+        out = np.zeros(6)
+        self._check(self.lib.cipc_get_profile(self.h, _d(out)))
+        return dict(launches=int(out[0]), total_ms=out[1], block_rows=int(out[2]), nnzb=int(out[3]),
+                    bytes_per_launch=out[4], flops_per_launch=out[5])
+
     # ------------------------------------------------------------ components
     @staticmethod
     def _x(x):

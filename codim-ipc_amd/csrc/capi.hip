@@ -95,6 +95,7 @@ void cipc_destroy(cipc_ctx* ctx) {
     if (!ctx) return;
     Ctx& c = ctx->c;
     if (c.s) (void)hipStreamSynchronize(c.s);
+    for (hipEvent_t e : c.evPool) (void)hipEventDestroy(e);
     if (c.hScal) (void)hipHostFree(c.hScal);
     if (c.hCnt) (void)hipHostFree(c.hCnt);
     hipStream_t s = c.s;
@@ -327,6 +328,29 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
         if (rc == 2) { c.lastError = "PCG iteration limit"; return CIPC_E_NO_CONVERGENCE; }
         return CIPC_OK;
     });
+}
+
+int cipc_set_profiling(cipc_ctx* ctx, int32_t enable) {
+    if (!ctx) return CIPC_E_INVALID;
+    ctx->c.profiling = enable ? 1 : 0;
+    ctx->c.profLaunches = 0;
+    ctx->c.profMs = 0;
+    ctx->c.profBytes = 0;
+    ctx->c.profFlops = 0;
+    return CIPC_OK;
+}
+int cipc_get_profile(const cipc_ctx* ctx, double* out) {
+    if (!ctx || !out) return CIPC_E_INVALID;
+    const Ctx& c = ctx->c;
+    double nrow = c.Ael.nrow, nnzb = c.Ael.nnzb;
+    out[0] = (double)c.profLaunches;
+    out[1] = c.profMs;
+    out[2] = nrow;
+    out[3] = nnzb;
+    const double L = c.profLaunches > 0 ? (double)c.profLaunches : 1.0;
+    out[4] = c.profBytes / L;
+    out[5] = c.profFlops / L;
+    return CIPC_OK;
 }
 
 }  // extern "C"

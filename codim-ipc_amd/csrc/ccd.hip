@@ -199,11 +199,13 @@ __device__ __forceinline__ int ecell(int vox, int f, int dim) {
     return c < dim ? c : dim - 1;
 }
 // primitive voxel range = union of its nodes' ranges; count of enumeration cells
+// (0 for primitives covering more than kLargeCells cells, which go to the large list)
 __global__ void k_prim_vox(const int* __restrict__ verts, int n, int nv, const int* __restrict__ vlo, const int* __restrict__ vhi, int nV,
-                           VoxGrid G, int* __restrict__ plo, int* __restrict__ phi, uint32_t* __restrict__ cnt) {
+                           VoxGrid G, int* __restrict__ plo, int* __restrict__ phi, uint32_t* __restrict__ cnt,
+                           uint8_t* __restrict__ lgFlag, int* __restrict__ lgList, uint32_t* __restrict__ lgCount) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t c = 1;
+    uint64_t c = 1;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         int lo = 1 << 30, hi = -(1 << 30);
@@ -214,14 +216,17 @@ __global__ void k_prim_vox(const int* __restrict__ verts, int n, int nv, const i
         }
         plo[(long)d * n + i] = lo;
         phi[(long)d * n + i] = hi;
-        c *= (uint32_t)(ecell(hi, G.f, G.edim[d]) - ecell(lo, G.f, G.edim[d]) + 1);
+        c *= (uint64_t)(ecell(hi, G.f, G.edim[d]) - ecell(lo, G.f, G.edim[d]) + 1);
     }
-    cnt[i] = c;
+    const bool large = c > kLargeCells;
+    lgFlag[i] = large ? 1 : 0;
+    cnt[i] = large ? 0u : (uint32_t)c;
+    if (large) lgList[atomicAdd(lgCount, 1u)] = i;
 }
-__global__ void k_prim_vox_emit(int n, const int* __restrict__ plo, const int* __restrict__ phi, VoxGrid G, const uint32_t* __restrict__ off,
-                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__global__ void k_prim_vox_emit(int n, const int* __restrict__ plo, const int* __restrict__ phi, VoxGrid G, const uint8_t* __restrict__ lgFlag,
+                                const uint32_t* __restrict__ off, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || lgFlag[i]) return;
     int a[3], b[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) { a[d] = ecell(plo[(long)d * n + i], G.f, G.edim[d]); b[d] = ecell(phi[(long)d * n + i], G.f, G.edim[d]); }
@@ -246,11 +251,55 @@ __device__ __forceinline__ void atomic_min_pos(unsigned long long* a, double v) 
     atomicMin(a, (unsigned long long)__double_as_longlong(v));
 }
 
+// point vI against triangle t (IPC.h:2120-2164): exact voxel-range overlap, shared
+// vertex and Dirichlet filters, swept-box broad phase, additive CCD with the running
+// step `largest` as cut-off
+__device__ __forceinline__ void pt_ccd_candidate(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ tri,
+                                                 int nT, const uint8_t* __restrict__ dbc, const int* __restrict__ tlo,
+                                                 const int* __restrict__ thi, int vI, const int* nlo, const int* nhi, d3 P, d3 dP,
+                                                 int t, double xi, double& largest, uint32_t* __restrict__ zero) {
+    int l0 = tlo[t], l1 = tlo[nT + t], l2 = tlo[2 * nT + t];
+    int h0 = thi[t], h1 = thi[nT + t], h2 = thi[2 * nT + t];
+    if (nlo[0] > h0 || l0 > nhi[0] || nlo[1] > h1 || l1 > nhi[1] || nlo[2] > h2 || l2 > nhi[2]) return;
+    int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+    if (vI == a || vI == b || vI == c) return;
+    if (dbc[vI] && dbc[a] && dbc[b] && dbc[c]) return;
+    d3 t0 = ld3(x, a), t1 = ld3(x, b), t2 = ld3(x, c);
+    d3 d0 = ld3(p, a), d1 = ld3(p, b), d2 = ld3(p, c);
+    if (!pt_ccd_broadphase(P, t0, t1, t2, dP, d0, d1, d2, xi)) return;
+    double la = largest;
+    if (pt_accd(P, t0, t1, t2, dP, d0, d1, d2, 0.1, xi, la)) {
+        if (largest > la) largest = la;
+        if (la == 0) atomicOr(zero, 1u);
+    }
+}
+// edge pair eA < eB (IPC.h:2166-2242), edge A first in the additive CCD
+__device__ __forceinline__ void ee_ccd_candidate(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bEdge,
+                                                 int nBE, const uint8_t* __restrict__ dbc, const int* __restrict__ elo,
+                                                 const int* __restrict__ ehi, int eA, int eB, double xi, double& largest,
+                                                 uint32_t* __restrict__ zero) {
+    if (elo[eA] > ehi[eB] || elo[eB] > ehi[eA] || elo[nBE + eA] > ehi[nBE + eB] || elo[nBE + eB] > ehi[nBE + eA] ||
+        elo[2l * nBE + eA] > ehi[2l * nBE + eB] || elo[2l * nBE + eB] > ehi[2l * nBE + eA])
+        return;
+    int a0 = bEdge[eA], a1 = bEdge[nBE + eA], b0 = bEdge[eB], b1 = bEdge[nBE + eB];
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
+    if (dbc[a0] && dbc[a1] && dbc[b0] && dbc[b1]) return;
+    d3 A0 = ld3(x, a0), A1 = ld3(x, a1), dA0 = ld3(p, a0), dA1 = ld3(p, a1);
+    d3 B0 = ld3(x, b0), B1 = ld3(x, b1), dB0 = ld3(p, b0), dB1 = ld3(p, b1);
+    if (!ee_ccd_broadphase(A0, A1, B0, B1, dA0, dA1, dB0, dB1, xi)) return;
+    double la = largest;
+    if (ee_accd(A0, A1, B0, B1, dA0, dA1, dB0, dB1, 0.1, xi, la)) {
+        if (largest > la) largest = la;
+        if (la == 0) atomicOr(zero, 1u);
+    }
+}
+
 __global__ void k_ccd_pt(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bNode, int nBN,
                          const int* __restrict__ tri, int nT, const uint8_t* __restrict__ dbc, const int* __restrict__ vlo,
                          const int* __restrict__ vhi, int nV, const int* __restrict__ tlo, const int* __restrict__ thi, VoxGrid G,
                          const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals,
-                         double alpha0, double xi, unsigned long long* __restrict__ gmin, uint32_t* __restrict__ zero) {
+                         const int* __restrict__ lgList, const uint32_t* __restrict__ lgCount, double alpha0, double xi,
+                         unsigned long long* __restrict__ gmin, uint32_t* __restrict__ zero) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nBN) return;
     int vI = bNode[s];
@@ -269,69 +318,60 @@ __global__ void k_ccd_pt(const double* __restrict__ x, const double* __restrict_
                 uint32_t st = start[cell], en = end[cell];
                 for (uint32_t k = st; k < en; ++k) {
                     int t = (int)vals[k];
-                    int l0 = tlo[t], l1 = tlo[nT + t], l2 = tlo[2 * nT + t];
-                    int h0 = thi[t], h1 = thi[nT + t], h2 = thi[2 * nT + t];
-                    if (ix != max(qlo[0], ecell(l0, G.f, G.edim[0])) || iy != max(qlo[1], ecell(l1, G.f, G.edim[1])) ||
-                        iz != max(qlo[2], ecell(l2, G.f, G.edim[2])))
+                    if (ix != max(qlo[0], ecell(tlo[t], G.f, G.edim[0])) || iy != max(qlo[1], ecell(tlo[nT + t], G.f, G.edim[1])) ||
+                        iz != max(qlo[2], ecell(tlo[2 * nT + t], G.f, G.edim[2])))
                         continue;
-                    if (nlo[0] > h0 || l0 > nhi[0] || nlo[1] > h1 || l1 > nhi[1] || nlo[2] > h2 || l2 > nhi[2]) continue;
-                    int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
-                    if (vI == a || vI == b || vI == c) continue;
-                    if (dbc[vI] && dbc[a] && dbc[b] && dbc[c]) continue;
-                    d3 t0 = ld3(x, a), t1 = ld3(x, b), t2 = ld3(x, c);
-                    d3 d0 = ld3(p, a), d1 = ld3(p, b), d2 = ld3(p, c);
-                    if (!pt_ccd_broadphase(P, t0, t1, t2, dP, d0, d1, d2, xi)) continue;
-                    double la = largest;
-                    if (pt_accd(P, t0, t1, t2, dP, d0, d1, d2, 0.1, xi, la)) {
-                        if (largest > la) largest = la;
-                        if (la == 0) atomicOr(zero, 1u);
-                    }
+                    pt_ccd_candidate(x, p, tri, nT, dbc, tlo, thi, vI, nlo, nhi, P, dP, t, xi, largest, zero);
                 }
             }
+    const uint32_t nl = *lgCount;
+    for (uint32_t k = 0; k < nl; ++k) pt_ccd_candidate(x, p, tri, nT, dbc, tlo, thi, vI, nlo, nhi, P, dP, lgList[k], xi, largest, zero);
     if (largest < alpha0) atomic_min_pos(gmin, largest);
 }
 
+// pair coverage as in k_query_ee: grid-listed edges take grid-listed partners of
+// larger index plus every large edge; large edges take large partners of larger index
 __global__ void k_ccd_ee(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bEdge, int nBE,
                          const uint8_t* __restrict__ dbc, const int* __restrict__ elo, const int* __restrict__ ehi, VoxGrid G,
                          const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals,
+                         const uint8_t* __restrict__ lgFlag, const int* __restrict__ lgList, const uint32_t* __restrict__ lgCount,
                          double alpha0, double xi, unsigned long long* __restrict__ gmin, uint32_t* __restrict__ zero) {
     int eI = blockIdx.x * blockDim.x + threadIdx.x;
     if (eI >= nBE) return;
-    int a0 = bEdge[eI], a1 = bEdge[nBE + eI];
-    int nlo[3], nhi[3], qlo[3], qhi[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        nlo[d] = elo[(long)d * nBE + eI]; nhi[d] = ehi[(long)d * nBE + eI];
-        qlo[d] = ecell(nlo[d], G.f, G.edim[d]); qhi[d] = ecell(nhi[d], G.f, G.edim[d]);
-    }
-    d3 A0 = ld3(x, a0), A1 = ld3(x, a1), dA0 = ld3(p, a0), dA1 = ld3(p, a1);
     double largest = alpha0;
-    for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
-        for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
-            for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
-                uint64_t cell = (uint64_t)ix + (uint64_t)G.edim[0] * ((uint64_t)iy + (uint64_t)G.edim[1] * (uint64_t)iz);
-                uint32_t st = start[cell], en = end[cell];
-                for (uint32_t k = st; k < en; ++k) {
-                    int eJ = (int)vals[k];
-                    if (eJ <= eI) continue;
-                    int l0 = elo[eJ], l1 = elo[nBE + eJ], l2 = elo[2l * nBE + eJ];
-                    int h0 = ehi[eJ], h1 = ehi[nBE + eJ], h2 = ehi[2l * nBE + eJ];
-                    if (ix != max(qlo[0], ecell(l0, G.f, G.edim[0])) || iy != max(qlo[1], ecell(l1, G.f, G.edim[1])) ||
-                        iz != max(qlo[2], ecell(l2, G.f, G.edim[2])))
-                        continue;
-                    if (nlo[0] > h0 || l0 > nhi[0] || nlo[1] > h1 || l1 > nhi[1] || nlo[2] > h2 || l2 > nhi[2]) continue;
-                    int b0 = bEdge[eJ], b1 = bEdge[nBE + eJ];
-                    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
-                    if (dbc[a0] && dbc[a1] && dbc[b0] && dbc[b1]) continue;
-                    d3 B0 = ld3(x, b0), B1 = ld3(x, b1), dB0 = ld3(p, b0), dB1 = ld3(p, b1);
-                    if (!ee_ccd_broadphase(A0, A1, B0, B1, dA0, dA1, dB0, dB1, xi)) continue;
-                    double la = largest;
-                    if (ee_accd(A0, A1, B0, B1, dA0, dA1, dB0, dB1, 0.1, xi, la)) {
-                        if (largest > la) largest = la;
-                        if (la == 0) atomicOr(zero, 1u);
+    const uint32_t nl = *lgCount;
+    if (lgFlag[eI]) {
+        for (uint32_t k = 0; k < nl; ++k) {
+            int eJ = lgList[k];
+            if (eJ > eI) ee_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, eI, eJ, xi, largest, zero);
+        }
+    } else {
+        int qlo[3], qhi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            qlo[d] = ecell(elo[(long)d * nBE + eI], G.f, G.edim[d]);
+            qhi[d] = ecell(ehi[(long)d * nBE + eI], G.f, G.edim[d]);
+        }
+        for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
+            for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
+                for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
+                    uint64_t cell = (uint64_t)ix + (uint64_t)G.edim[0] * ((uint64_t)iy + (uint64_t)G.edim[1] * (uint64_t)iz);
+                    uint32_t st = start[cell], en = end[cell];
+                    for (uint32_t k = st; k < en; ++k) {
+                        int eJ = (int)vals[k];
+                        if (eJ <= eI) continue;
+                        if (ix != max(qlo[0], ecell(elo[eJ], G.f, G.edim[0])) || iy != max(qlo[1], ecell(elo[nBE + eJ], G.f, G.edim[1])) ||
+                            iz != max(qlo[2], ecell(elo[2l * nBE + eJ], G.f, G.edim[2])))
+                            continue;
+                        ee_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, eI, eJ, xi, largest, zero);
                     }
                 }
-            }
+        for (uint32_t k = 0; k < nl; ++k) {
+            int eJ = lgList[k];
+            if (eJ > eI) ee_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, eI, eJ, xi, largest, zero);
+            else ee_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, eJ, eI, xi, largest, zero);
+        }
+    }
     if (largest < alpha0) atomic_min_pos(gmin, largest);
 }
 
@@ -343,26 +383,31 @@ static int bits_for2(uint64_t n) {
 
 static long build_vox_cells(Ctx& c, const int* verts, int n, int nv, const int* vlo, const int* vhi, const VoxGrid& G, DevBuf<int>& plo,
                             DevBuf<int>& phi, DevBuf<uint64_t>& keys, DevBuf<uint32_t>& vals, DevBuf<uint32_t>& start,
-                            DevBuf<uint32_t>& end) {
+                            DevBuf<uint32_t>& end, DevBuf<uint8_t>& lgFlag, DevBuf<int>& lgList, uint32_t* lgCount) {
     long nCells = (long)G.edim[0] * G.edim[1] * G.edim[2];
     start.ensure(nCells);
     end.ensure(nCells);
     CIPC_HIP(hipMemsetAsync(start.p, 0, nCells * sizeof(uint32_t), c.s));
     CIPC_HIP(hipMemsetAsync(end.p, 0, nCells * sizeof(uint32_t), c.s));
+    CIPC_HIP(hipMemsetAsync(lgCount, 0, sizeof(uint32_t), c.s));
+    lgFlag.ensure((size_t)n + 1);
+    lgList.ensure((size_t)n + 1);
     if (n == 0) return 0;
     plo.ensure(3l * n);
     phi.ensure(3l * n);
     uint32_t* cnt = c.u32a.ensure(n + 1);
     uint32_t* off = c.u32b.ensure(n + 1);
-    hipLaunchKernelGGL(k_prim_vox, dim3(nblocks(n)), dim3(kBlock), 0, c.s, verts, n, nv, vlo, vhi, c.nV, G, plo.p, phi.p, cnt);
+    hipLaunchKernelGGL(k_prim_vox, dim3(nblocks(n)), dim3(kBlock), 0, c.s, verts, n, nv, vlo, vhi, c.nV, G, plo.p, phi.p, cnt, lgFlag.p,
+                       lgList.p, lgCount);
     CIPC_LAUNCH_CHECK();
     scan_exclusive(cnt, off, n, c.counters.p + 9, c.pw, c.s);
     CIPC_HIP(hipMemcpyAsync(c.hCnt + 9, c.counters.p + 9, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
     CIPC_HIP(hipStreamSynchronize(c.s));
     long total = c.hCnt[9];
+    if (total == 0) return 0;
     keys.ensure(total);
     vals.ensure(total);
-    hipLaunchKernelGGL(k_prim_vox_emit, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, plo.p, phi.p, G, off, keys.p, vals.p);
+    hipLaunchKernelGGL(k_prim_vox_emit, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, plo.p, phi.p, G, lgFlag.p, off, keys.p, vals.p);
     CIPC_LAUNCH_CHECK();
     radix_sort_pairs(keys.p, vals.p, total, bits_for2((uint64_t)nCells), c.pw, c.s);
     hipLaunchKernelGGL(k_cell_bounds2, dim3(nblocks(total)), dim3(kBlock), 0, c.s, keys.p, total, start.p, end.p);
@@ -440,17 +485,23 @@ double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, doub
     CIPC_HIP(hipMemcpyAsync(gmin, &a0, sizeof(double), hipMemcpyHostToDevice, c.s));
     CIPC_HIP(hipMemsetAsync(c.counters.p + 13, 0, sizeof(uint32_t), c.s));
     // point-triangle
-    build_vox_cells(c, c.tri.p, c.nT, 3, vlo, vhi, G, c.ccdTlo, c.ccdThi, c.k64a, c.u32c, c.cellStart, c.cellEnd);
+    uint32_t* lgCountT = c.counters.p + 18;
+    uint32_t* lgCountE = c.counters.p + 19;
+    build_vox_cells(c, c.tri.p, c.nT, 3, vlo, vhi, G, c.ccdTlo, c.ccdThi, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT,
+                    lgCountT);
     if (c.nBN && c.nT) {
         hipLaunchKernelGGL(k_ccd_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, p, c.bNode.p, c.nBN, c.tri.p, c.nT, c.dbc.p, vlo, vhi, c.nV,
-                           c.ccdTlo.p, c.ccdThi.p, G, c.cellStart.p, c.cellEnd.p, c.u32c.p, alpha, xi, gmin, c.counters.p + 13);
+                           c.ccdTlo.p, c.ccdThi.p, G, c.cellStart.p, c.cellEnd.p, c.u32c.p, c.lgListT.p, lgCountT, alpha, xi, gmin,
+                           c.counters.p + 13);
         CIPC_LAUNCH_CHECK();
     }
     // edge-edge
-    build_vox_cells(c, c.bEdge.p, c.nBE, 2, vlo, vhi, G, c.ccdElo, c.ccdEhi, c.k64b, c.u32d, c.ecellStart, c.ecellEnd);
+    build_vox_cells(c, c.bEdge.p, c.nBE, 2, vlo, vhi, G, c.ccdElo, c.ccdEhi, c.k64b, c.u32d, c.ecellStart, c.ecellEnd, c.lgFlagE,
+                    c.lgListE, lgCountE);
     if (c.nBE) {
         hipLaunchKernelGGL(k_ccd_ee, dim3(nblocks(c.nBE)), dim3(kBlock), 0, c.s, x, p, c.bEdge.p, c.nBE, c.dbc.p, c.ccdElo.p, c.ccdEhi.p, G,
-                           c.ecellStart.p, c.ecellEnd.p, c.u32d.p, alpha, xi, gmin, c.counters.p + 13);
+                           c.ecellStart.p, c.ecellEnd.p, c.u32d.p, c.lgFlagE.p, c.lgListE.p, lgCountE, alpha, xi, gmin,
+                           c.counters.p + 13);
         CIPC_LAUNCH_CHECK();
     }
     CIPC_HIP(hipMemcpyAsync(c.hScal + 24, gmin, sizeof(double), hipMemcpyDeviceToHost, c.s));

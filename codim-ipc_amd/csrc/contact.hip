@@ -33,9 +33,11 @@ __device__ __forceinline__ int cell_of(double v, double lb, double inv, int dim)
 }
 
 // ------------------------------------------------------------------ grid build
-// kind 0: triangles (tri SoA), kind 1: boundary edges (bEdge SoA)
+// kind 0: triangles (tri SoA), kind 1: boundary edges (bEdge SoA).  Primitives whose
+// box covers more than kLargeCells cells get count 0 and go to the large list.
 __global__ void k_prim_range(const double* __restrict__ x, const int* __restrict__ verts, int n, int nv, GridParams G,
-                             int* __restrict__ lo, uint32_t* __restrict__ count) {
+                             int* __restrict__ lo, uint32_t* __restrict__ count, uint8_t* __restrict__ lgFlag,
+                             int* __restrict__ lgList, uint32_t* __restrict__ lgCount) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
@@ -44,19 +46,23 @@ __global__ void k_prim_range(const double* __restrict__ x, const int* __restrict
 #pragma unroll
         for (int d = 0; d < 3; ++d) { double c = x[3 * v + d]; mn[d] = c < mn[d] ? c : mn[d]; mx[d] = c > mx[d] ? c : mx[d]; }
     }
-    uint32_t cnt = 1;
+    uint64_t cnt = 1;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         int a = cell_of(mn[d], G.lb[d], G.inv, G.dim[d]), b = cell_of(mx[d], G.lb[d], G.inv, G.dim[d]);
         lo[(long)d * n + i] = a;
-        cnt *= (uint32_t)(b - a + 1);
+        cnt *= (uint64_t)(b - a + 1);
     }
-    count[i] = cnt;
+    const bool large = cnt > kLargeCells;
+    lgFlag[i] = large ? 1 : 0;
+    count[i] = large ? 0u : (uint32_t)cnt;
+    if (large) lgList[atomicAdd(lgCount, 1u)] = i;
 }
 __global__ void k_prim_emit(const double* __restrict__ x, const int* __restrict__ verts, int n, int nv, GridParams G,
-                            const uint32_t* __restrict__ off, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                            const uint8_t* __restrict__ lgFlag, const uint32_t* __restrict__ off, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || lgFlag[i]) return;
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
     for (int k = 0; k < nv; ++k) {
         int v = verts[(long)k * n + i];
@@ -89,27 +95,33 @@ static int bits_for(uint64_t n) {
     return b;
 }
 
-// builds the sorted cell lists of one primitive kind; returns number of entries
+// builds the sorted cell lists of one primitive kind (large primitives listed apart,
+// their count left in device memory at lgCount); returns number of cell entries
 static long build_cells(Ctx& c, const double* x, const int* verts, int n, int nv, const GridParams& G, DevBuf<int>& lo,
-                        DevBuf<uint64_t>& keys, DevBuf<uint32_t>& vals, DevBuf<uint32_t>& start, DevBuf<uint32_t>& end) {
+                        DevBuf<uint64_t>& keys, DevBuf<uint32_t>& vals, DevBuf<uint32_t>& start, DevBuf<uint32_t>& end,
+                        DevBuf<uint8_t>& lgFlag, DevBuf<int>& lgList, uint32_t* lgCount) {
     long nCells = (long)G.dim[0] * G.dim[1] * G.dim[2];
     start.ensure(nCells);
     end.ensure(nCells);
     CIPC_HIP(hipMemsetAsync(start.p, 0, nCells * sizeof(uint32_t), c.s));
     CIPC_HIP(hipMemsetAsync(end.p, 0, nCells * sizeof(uint32_t), c.s));
+    CIPC_HIP(hipMemsetAsync(lgCount, 0, sizeof(uint32_t), c.s));
+    lgFlag.ensure((size_t)n + 1);
+    lgList.ensure((size_t)n + 1);
     if (n == 0) return 0;
     lo.ensure((size_t)3 * n);
     uint32_t* cnt = c.u32a.ensure(n + 1);
     uint32_t* off = c.u32b.ensure(n + 1);
-    hipLaunchKernelGGL(k_prim_range, dim3(nblocks(n)), dim3(kBlock), 0, c.s, x, verts, n, nv, G, lo.p, cnt);
+    hipLaunchKernelGGL(k_prim_range, dim3(nblocks(n)), dim3(kBlock), 0, c.s, x, verts, n, nv, G, lo.p, cnt, lgFlag.p, lgList.p, lgCount);
     CIPC_LAUNCH_CHECK();
     scan_exclusive(cnt, off, n, c.counters.p + 8, c.pw, c.s);
     CIPC_HIP(hipMemcpyAsync(c.hCnt + 8, c.counters.p + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
     CIPC_HIP(hipStreamSynchronize(c.s));
     long total = c.hCnt[8];
+    if (total == 0) return 0;
     keys.ensure(total);
     vals.ensure(total);
-    hipLaunchKernelGGL(k_prim_emit, dim3(nblocks(n)), dim3(kBlock), 0, c.s, x, verts, n, nv, G, off, keys.p, vals.p);
+    hipLaunchKernelGGL(k_prim_emit, dim3(nblocks(n)), dim3(kBlock), 0, c.s, x, verts, n, nv, G, lgFlag.p, off, keys.p, vals.p);
     CIPC_LAUNCH_CHECK();
     radix_sort_pairs(keys.p, vals.p, total, bits_for((uint64_t)nCells), c.pw, c.s);
     hipLaunchKernelGGL(k_cell_bounds, dim3(nblocks(total)), dim3(kBlock), 0, c.s, keys.p, total, start.p, end.p);
@@ -158,10 +170,76 @@ __device__ __forceinline__ bool ee_cd_broadphase(d3 a0, d3 a1, d3 b0, d3 b1, dou
     return true;
 }
 
+// point vI (boundary-node slot svI) against triangle t: IPC.h:57-140
+__device__ __forceinline__ void pt_candidate(const CSOut& o, const double* __restrict__ x, const int* __restrict__ tri, int nT,
+                                             const uint8_t* __restrict__ dbc, int svI, int vI, d3 p, int t, double dHat,
+                                             double dHat2, int nV) {
+    int a = tri[t], b = tri[nT + t], cc = tri[2 * nT + t];
+    if (vI == a || vI == b || vI == cc) return;
+    if (dbc[vI] && dbc[a] && dbc[b] && dbc[cc]) return;
+    d3 t0 = ld3(x, a), t1 = ld3(x, b), t2 = ld3(x, cc);
+    if (!pt_cd_broadphase(p, t0, t1, t2, dHat)) return;
+    int ty = pt_type(p, t0, t1, t2);
+    double d;
+    int c1 = -1, c2 = -1;
+    switch (ty) {
+        case 0: d = dist2_pp(p, t0); c1 = a; break;
+        case 1: d = dist2_pp(p, t1); c1 = b; break;
+        case 2: d = dist2_pp(p, t2); c1 = cc; break;
+        case 3: d = dist2_pe(p, t0, t1); c1 = a; c2 = b; break;
+        case 4: d = dist2_pe(p, t1, t2); c1 = b; c2 = cc; break;
+        case 5: d = dist2_pe(p, t2, t0); c1 = cc; c2 = a; break;
+        default: d = dist2_pt(p, t0, t1, t2); break;
+    }
+    if (!(d < dHat2)) return;
+    if (ty == 6) emit_kept(o, (uint64_t)svI * (uint64_t)nT + (uint64_t)t, -vI - 1, a, b, cc);
+    else emit_dup(o, dup_key(-vI - 1, c1, c2, nV));
+}
+
+// edge pair eA < eB: IPC.h:142-330 (edge A supplies a0/a1 of the encoding)
+__device__ __forceinline__ void ee_candidate(const CSOut& o, const double* __restrict__ x, const double* __restrict__ X0,
+                                             const int* __restrict__ bEdge, int nBE, const uint8_t* __restrict__ dbc, int eA, int a0,
+                                             int a1, d3 ea0, d3 ea1, d3 ra0, d3 ra1, int eB, double dHat, double dHat2, int nV) {
+    int b0 = bEdge[eB], b1 = bEdge[nBE + eB];
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
+    if (dbc[a0] && dbc[a1] && dbc[b0] && dbc[b1]) return;
+    d3 eb0 = ld3(x, b0), eb1 = ld3(x, b1);
+    if (!ee_cd_broadphase(ea0, ea1, eb0, eb1, dHat)) return;
+    double crossN = ee_cross_norm2(ea0, ea1, eb0, eb1);
+    double eps = ee_mollifier_threshold(ra0, ra1, ld3(X0, b0), ld3(X0, b1));
+    bool moll = crossN < eps;
+    double d;
+    int c[4];
+    switch (ee_type(ea0, ea1, eb0, eb1)) {
+        case 0: d = dist2_pp(ea0, eb0);
+            if (moll) { c[0] = a0; c[1] = b0; c[2] = -a1 - 1; c[3] = -b1 - 1; } else { c[0] = -a0 - 1; c[1] = b0; c[2] = -1; c[3] = -1; } break;
+        case 1: d = dist2_pp(ea0, eb1);
+            if (moll) { c[0] = a0; c[1] = b1; c[2] = -a1 - 1; c[3] = -b0 - 1; } else { c[0] = -a0 - 1; c[1] = b1; c[2] = -1; c[3] = -1; } break;
+        case 2: d = dist2_pe(ea0, eb0, eb1);
+            if (moll) { c[0] = a0; c[1] = b0; c[2] = b1; c[3] = -a1 - 1; } else { c[0] = -a0 - 1; c[1] = b0; c[2] = b1; c[3] = -1; } break;
+        case 3: d = dist2_pp(ea1, eb0);
+            if (moll) { c[0] = a1; c[1] = b0; c[2] = -a0 - 1; c[3] = -b1 - 1; } else { c[0] = -a1 - 1; c[1] = b0; c[2] = -1; c[3] = -1; } break;
+        case 4: d = dist2_pp(ea1, eb1);
+            if (moll) { c[0] = a1; c[1] = b1; c[2] = -a0 - 1; c[3] = -b0 - 1; } else { c[0] = -a1 - 1; c[1] = b1; c[2] = -1; c[3] = -1; } break;
+        case 5: d = dist2_pe(ea1, eb0, eb1);
+            if (moll) { c[0] = a1; c[1] = b0; c[2] = b1; c[3] = -a0 - 1; } else { c[0] = -a1 - 1; c[1] = b0; c[2] = b1; c[3] = -1; } break;
+        case 6: d = dist2_pe(eb0, ea0, ea1);
+            if (moll) { c[0] = b0; c[1] = a0; c[2] = a1; c[3] = -b1 - 1; } else { c[0] = -b0 - 1; c[1] = a0; c[2] = a1; c[3] = -1; } break;
+        case 7: d = dist2_pe(eb1, ea0, ea1);
+            if (moll) { c[0] = b1; c[1] = a0; c[2] = a1; c[3] = -b0 - 1; } else { c[0] = -b1 - 1; c[1] = a0; c[2] = a1; c[3] = -1; } break;
+        default: d = dist2_ee(ea0, ea1, eb0, eb1);
+            if (moll) { c[0] = a0; c[1] = a1; c[2] = -b0 - 1; c[3] = b1; } else { c[0] = a0; c[1] = a1; c[2] = b0; c[3] = b1; } break;
+    }
+    if (!(d < dHat2)) return;
+    if (c[0] >= 0) emit_kept(o, (uint64_t)eA * (uint64_t)nBE + (uint64_t)eB, c[0], c[1], c[2], c[3]);
+    else emit_dup(o, dup_key(c[0], c[1], c[2], nV));
+}
+
 __global__ void k_query_pt(const double* __restrict__ x, const int* __restrict__ bNode, int nBN, const int* __restrict__ tri,
                            int nT, const uint8_t* __restrict__ dbc, GridParams G, const uint32_t* __restrict__ start,
                            const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals, const int* __restrict__ triLo,
-                           double dHat, double dHat2, int nV, CSOut o) {
+                           const int* __restrict__ lgList, const uint32_t* __restrict__ lgCount, double dHat, double dHat2,
+                           int nV, CSOut o) {
     int svI = blockIdx.x * blockDim.x + threadIdx.x;
     if (svI >= nBN) return;
     int vI = bNode[svI];
@@ -179,39 +257,34 @@ __global__ void k_query_pt(const double* __restrict__ x, const int* __restrict__
                     int t = (int)vals[k];
                     int l0 = triLo[t], l1 = triLo[nT + t], l2 = triLo[2 * nT + t];
                     if (ix != max(qlo[0], l0) || iy != max(qlo[1], l1) || iz != max(qlo[2], l2)) continue;
-                    int a = tri[t], b = tri[nT + t], cc = tri[2 * nT + t];
-                    if (vI == a || vI == b || vI == cc) continue;
-                    if (dbc[vI] && dbc[a] && dbc[b] && dbc[cc]) continue;
-                    d3 t0 = ld3(x, a), t1 = ld3(x, b), t2 = ld3(x, cc);
-                    if (!pt_cd_broadphase(p, t0, t1, t2, dHat)) continue;
-                    int ty = pt_type(p, t0, t1, t2);
-                    double d;
-                    int c1 = -1, c2 = -1;
-                    switch (ty) {
-                        case 0: d = dist2_pp(p, t0); c1 = a; break;
-                        case 1: d = dist2_pp(p, t1); c1 = b; break;
-                        case 2: d = dist2_pp(p, t2); c1 = cc; break;
-                        case 3: d = dist2_pe(p, t0, t1); c1 = a; c2 = b; break;
-                        case 4: d = dist2_pe(p, t1, t2); c1 = b; c2 = cc; break;
-                        case 5: d = dist2_pe(p, t2, t0); c1 = cc; c2 = a; break;
-                        default: d = dist2_pt(p, t0, t1, t2); break;
-                    }
-                    if (!(d < dHat2)) continue;
-                    if (ty == 6) emit_kept(o, (uint64_t)svI * (uint64_t)nT + (uint64_t)t, -vI - 1, a, b, cc);
-                    else emit_dup(o, dup_key(-vI - 1, c1, c2, nV));
+                    pt_candidate(o, x, tri, nT, dbc, svI, vI, p, t, dHat, dHat2, nV);
                 }
             }
+    const uint32_t nl = *lgCount;
+    for (uint32_t k = 0; k < nl; ++k) pt_candidate(o, x, tri, nT, dbc, svI, vI, p, lgList[k], dHat, dHat2, nV);
 }
 
+// Every unordered edge pair is tested once: a grid-listed edge finds grid-listed
+// edges of larger index in the first common cell and every large edge from the
+// list; a large edge only tests large edges of larger index.
 __global__ void k_query_ee(const double* __restrict__ x, const double* __restrict__ X0, const int* __restrict__ bEdge, int nBE,
                            const uint8_t* __restrict__ dbc, GridParams G, const uint32_t* __restrict__ start,
                            const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals, const int* __restrict__ edgeLo,
+                           const uint8_t* __restrict__ lgFlag, const int* __restrict__ lgList, const uint32_t* __restrict__ lgCount,
                            double dHat, double dHat2, int nV, CSOut o) {
     int eI = blockIdx.x * blockDim.x + threadIdx.x;
     if (eI >= nBE) return;
     int a0 = bEdge[eI], a1 = bEdge[nBE + eI];
     d3 ea0 = ld3(x, a0), ea1 = ld3(x, a1);
     d3 ra0 = ld3(X0, a0), ra1 = ld3(X0, a1);
+    const uint32_t nl = *lgCount;
+    if (lgFlag[eI]) {
+        for (uint32_t k = 0; k < nl; ++k) {
+            int eJ = lgList[k];
+            if (eJ > eI) ee_candidate(o, x, X0, bEdge, nBE, dbc, eI, a0, a1, ea0, ea1, ra0, ra1, eJ, dHat, dHat2, nV);
+        }
+        return;
+    }
     int qlo[3], qhi[3];
     double mn[3] = {dmin(ea0.x, ea1.x), dmin(ea0.y, ea1.y), dmin(ea0.z, ea1.z)};
     double mx[3] = {dmax(ea0.x, ea1.x), dmax(ea0.y, ea1.y), dmax(ea0.z, ea1.z)};
@@ -227,41 +300,18 @@ __global__ void k_query_ee(const double* __restrict__ x, const double* __restric
                     if (eJ <= eI) continue;
                     int l0 = edgeLo[eJ], l1 = edgeLo[nBE + eJ], l2 = edgeLo[2 * nBE + eJ];
                     if (ix != max(qlo[0], l0) || iy != max(qlo[1], l1) || iz != max(qlo[2], l2)) continue;
-                    int b0 = bEdge[eJ], b1 = bEdge[nBE + eJ];
-                    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
-                    if (dbc[a0] && dbc[a1] && dbc[b0] && dbc[b1]) continue;
-                    d3 eb0 = ld3(x, b0), eb1 = ld3(x, b1);
-                    if (!ee_cd_broadphase(ea0, ea1, eb0, eb1, dHat)) continue;
-                    double crossN = ee_cross_norm2(ea0, ea1, eb0, eb1);
-                    double eps = ee_mollifier_threshold(ra0, ra1, ld3(X0, b0), ld3(X0, b1));
-                    bool moll = crossN < eps;
-                    double d;
-                    int c[4];
-                    switch (ee_type(ea0, ea1, eb0, eb1)) {
-                        case 0: d = dist2_pp(ea0, eb0);
-                            if (moll) { c[0] = a0; c[1] = b0; c[2] = -a1 - 1; c[3] = -b1 - 1; } else { c[0] = -a0 - 1; c[1] = b0; c[2] = -1; c[3] = -1; } break;
-                        case 1: d = dist2_pp(ea0, eb1);
-                            if (moll) { c[0] = a0; c[1] = b1; c[2] = -a1 - 1; c[3] = -b0 - 1; } else { c[0] = -a0 - 1; c[1] = b1; c[2] = -1; c[3] = -1; } break;
-                        case 2: d = dist2_pe(ea0, eb0, eb1);
-                            if (moll) { c[0] = a0; c[1] = b0; c[2] = b1; c[3] = -a1 - 1; } else { c[0] = -a0 - 1; c[1] = b0; c[2] = b1; c[3] = -1; } break;
-                        case 3: d = dist2_pp(ea1, eb0);
-                            if (moll) { c[0] = a1; c[1] = b0; c[2] = -a0 - 1; c[3] = -b1 - 1; } else { c[0] = -a1 - 1; c[1] = b0; c[2] = -1; c[3] = -1; } break;
-                        case 4: d = dist2_pp(ea1, eb1);
-                            if (moll) { c[0] = a1; c[1] = b1; c[2] = -a0 - 1; c[3] = -b0 - 1; } else { c[0] = -a1 - 1; c[1] = b1; c[2] = -1; c[3] = -1; } break;
-                        case 5: d = dist2_pe(ea1, eb0, eb1);
-                            if (moll) { c[0] = a1; c[1] = b0; c[2] = b1; c[3] = -a0 - 1; } else { c[0] = -a1 - 1; c[1] = b0; c[2] = b1; c[3] = -1; } break;
-                        case 6: d = dist2_pe(eb0, ea0, ea1);
-                            if (moll) { c[0] = b0; c[1] = a0; c[2] = a1; c[3] = -b1 - 1; } else { c[0] = -b0 - 1; c[1] = a0; c[2] = a1; c[3] = -1; } break;
-                        case 7: d = dist2_pe(eb1, ea0, ea1);
-                            if (moll) { c[0] = b1; c[1] = a0; c[2] = a1; c[3] = -b0 - 1; } else { c[0] = -b1 - 1; c[1] = a0; c[2] = a1; c[3] = -1; } break;
-                        default: d = dist2_ee(ea0, ea1, eb0, eb1);
-                            if (moll) { c[0] = a0; c[1] = a1; c[2] = -b0 - 1; c[3] = b1; } else { c[0] = a0; c[1] = a1; c[2] = b0; c[3] = b1; } break;
-                    }
-                    if (!(d < dHat2)) continue;
-                    if (c[0] >= 0) emit_kept(o, (uint64_t)eI * (uint64_t)nBE + (uint64_t)eJ, c[0], c[1], c[2], c[3]);
-                    else emit_dup(o, dup_key(c[0], c[1], c[2], nV));
+                    ee_candidate(o, x, X0, bEdge, nBE, dbc, eI, a0, a1, ea0, ea1, ra0, ra1, eJ, dHat, dHat2, nV);
                 }
             }
+    for (uint32_t k = 0; k < nl; ++k) {
+        int eJ = lgList[k];
+        if (eJ > eI) {
+            ee_candidate(o, x, X0, bEdge, nBE, dbc, eI, a0, a1, ea0, ea1, ra0, ra1, eJ, dHat, dHat2, nV);
+        } else {
+            int b0 = bEdge[eJ], b1 = bEdge[nBE + eJ];
+            ee_candidate(o, x, X0, bEdge, nBE, dbc, eJ, b0, b1, ld3(x, b0), ld3(x, b1), ld3(X0, b0), ld3(X0, b1), eI, dHat, dHat2, nV);
+        }
+    }
 }
 #pragma clang fp contract(on)
 
@@ -320,8 +370,10 @@ void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
     G.inv = 1.0 / cs;
     for (int d = 0; d < 3; ++d) G.dim[d] = (int)std::floor(rng[d] / cs) + 1;
 
-    build_cells(c, x, c.tri.p, c.nT, 3, G, c.triLo, c.k64a, c.u32c, c.cellStart, c.cellEnd);
-    build_cells(c, x, c.bEdge.p, c.nBE, 2, G, c.edgeLo, c.k64b, c.u32d, c.ecellStart, c.ecellEnd);
+    uint32_t* lgCountT = c.counters.p + 16;
+    uint32_t* lgCountE = c.counters.p + 17;
+    build_cells(c, x, c.tri.p, c.nT, 3, G, c.triLo, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT, lgCountT);
+    build_cells(c, x, c.bEdge.p, c.nBE, 2, G, c.edgeLo, c.k64b, c.u32d, c.ecellStart, c.ecellEnd, c.lgFlagE, c.lgListE, lgCountE);
 
     // narrow phase with capacity retry
     CSOut o;
@@ -336,12 +388,13 @@ void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
         CIPC_HIP(hipMemsetAsync(c.counters.p, 0, 4 * sizeof(uint32_t), c.s));
         // point-triangle kept entries fill [0, cap), edge-edge kept entries [cap, 2 cap)
         hipLaunchKernelGGL(k_query_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, c.bNode.p, c.nBN, c.tri.p, c.nT, c.dbc.p, G,
-                           c.cellStart.p, c.cellEnd.p, c.u32c.p, c.triLo.p, dHat, dHat2, c.nV, o);
+                           c.cellStart.p, c.cellEnd.p, c.u32c.p, c.triLo.p, c.lgListT.p, lgCountT, dHat, dHat2, c.nV, o);
         CIPC_LAUNCH_CHECK();
         CSOut oEE = o;
         oEE.key = o.key + cap; oEE.c4 = o.c4 + 4 * (size_t)cap; oEE.cnt = c.counters.p + 2;
         hipLaunchKernelGGL(k_query_ee, dim3(nblocks(c.nBE)), dim3(kBlock), 0, c.s, x, c.X0.p, c.bEdge.p, c.nBE, c.dbc.p, G,
-                           c.ecellStart.p, c.ecellEnd.p, c.u32d.p, c.edgeLo.p, dHat, dHat2, c.nV, oEE);
+                           c.ecellStart.p, c.ecellEnd.p, c.u32d.p, c.edgeLo.p, c.lgFlagE.p, c.lgListE.p, lgCountE, dHat, dHat2, c.nV,
+                           oEE);
         CIPC_LAUNCH_CHECK();
         CIPC_HIP(hipMemcpyAsync(c.hCnt, c.counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
         CIPC_HIP(hipStreamSynchronize(c.s));

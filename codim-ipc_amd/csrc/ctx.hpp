@@ -22,6 +22,10 @@ struct Params {
 };
 
 // spatial grid used by the constraint-set broad phase and the CCD candidate search
+// A primitive whose box covers more grid cells than this is not entered in the cell
+// lists; queries test it directly from a compact list instead.
+constexpr uint32_t kLargeCells = 512;
+
 struct GridParams {
     double lb[3];
     double inv;        // 1 / cell size
@@ -76,6 +80,10 @@ struct Ctx {
     DevBuf<uint32_t> csPos, csIdx, csHead;
     DevBuf<int> triLo, edgeLo;
     DevBuf<int> ccdTlo, ccdThi, ccdElo, ccdEhi;
+    // primitives spanning more than kLargeCells grid cells are kept out of the cell
+    // lists and tested by every query directly (flag per primitive + compact list)
+    DevBuf<uint8_t> lgFlagT, lgFlagE;
+    DevBuf<int> lgListT, lgListE;
 
     // elastic Hessian: static BSR pattern + gather plan
     BSR Ael;
@@ -102,6 +110,13 @@ struct Ctx {
 
     // statistics of the last step
     cipc_step_stats stats{};
+
+    // optional event timing of the dominant PCG kernel (elastic SpMV)
+    int profiling = 0;
+    std::vector<hipEvent_t> evPool;
+    long profLaunches = 0;
+    double profMs = 0;
+    double profBytes = 0, profFlops = 0;   // algorithmic totals over the timed launches
 };
 
 // ---- utilities (util.hip)

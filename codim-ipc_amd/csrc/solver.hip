@@ -5,184 +5,247 @@
 // rule ||r|| <= tol ||b|| follows the reference's iterative-solver precedent
 // (AMGCL_SOLVER.h:53-65, rel-tol 1e-5).
 //
-// SpMV: a group of LANES lanes per block row, each lane strides over the row's
-// blocks; blocks are stored structure-of-arrays (entry k of all blocks contiguous)
-// so the lanes of a wave read consecutive addresses; the group reduces with
-// cross-lane shuffles.  Scalars (alpha, beta, residual) never leave the device
-// inside an iteration: dot products finish in the last block to arrive (ticket
-// counter, partials summed in a fixed order), and every kernel returns early once
-// the on-device "done" flag is set, so the host only polls every few iterations.
+// One iteration is two kernels and no atomics:
+//   k_pcg_spmv   q = A (z + beta p_old) for every block row, p_new written, partial
+//                sums of p_new . q per workgroup;
+//   k_pcg_update x += alpha p, r -= alpha q, z = Dinv r, partial sums of r.z, r.r.
+// Each kernel starts by summing the previous kernel's per-workgroup partials itself
+// (every workgroup, same fixed order, so every workgroup holds the identical scalar);
+// kernel boundaries on the one stream order the partials' writes before those reads,
+// so no fence or ticket counter is needed.  The SpMV keeps A z and A p_old in separate
+// accumulators so the scalar reduction's latency overlaps the matrix stream, and beta
+// is applied when the row is finished.
+//
+// Matrix blocks are stored structure-of-arrays (entry k of all blocks contiguous);
+// LANES lanes share a block row and stride over its blocks, so the lanes of a wave
+// read consecutive addresses.  Rows are mapped XCD-aware: workgroup b runs on XCD
+// b % 8 and XCD x owns one contiguous eighth of the rows, so the gathered neighbour
+// values stay in that XCD's L2.
+//
+// The host launches iterations in chunks and polls an on-device done flag between
+// chunks; kernels launched after convergence return at once.
 #include "ctx.hpp"
+#include <hip/hip_ext.h>
 #include <cmath>
 
 namespace cipc {
 
-enum { S_RZ = 0, S_PQ, S_RZNEW, S_RR, S_BETA, S_BNORM, S_TOL, S_ALPHA };
-enum { F_DONE = 0, F_ITERS, F_TICKET, F_FAIL, F_MAXIT };
+// solver scalars (sc[]) and flags
+enum { S_RZ0 = 0, S_RZ1, S_BB, S_RR };
+enum { F_DONE = 0, F_ITERS, F_FAIL };
+
+constexpr int kSpmvLanes = 8;
+constexpr int kSpmvBlock = 512;
+constexpr int kUpdBlock = 256;
+constexpr int kXcd = 8;
+
+// Sum over the workgroup; every thread receives the same value (butterfly shuffles
+// give bit-identical results in all lanes, waves are added in a fixed order).
+template <int BS>
+__device__ __forceinline__ double block_allsum(double v, double* lds) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) r += lds[w];
+    __syncthreads();
+    return r;
+}
+
+// rows per XCD, rounded up to whole workgroups of `rowsPerBlock`
+inline int rows_per_xcd(int nrow, int rowsPerBlock) {
+    int r = (nrow + kXcd - 1) / kXcd;
+    return (r + rowsPerBlock - 1) / rowsPerBlock * rowsPerBlock;
+}
+__device__ __forceinline__ int xcd_row_base(int rowsPerXcd, int rowsPerBlock) {
+    return (int)(blockIdx.x % kXcd) * rowsPerXcd + (int)(blockIdx.x / kXcd) * rowsPerBlock;
+}
 
 template <int LANES>
-__global__ void k_spmv(const int* __restrict__ rowptr, const int* __restrict__ col, const double* __restrict__ val, int nnzb, int nrow,
-                       const double* __restrict__ p, double* __restrict__ q, int accumulate, const int* __restrict__ flags) {
-    if (flags && flags[F_DONE]) return;
-    long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int row = (int)(gid / LANES), lane = (int)(gid % LANES);
-    double s0 = 0, s1 = 0, s2 = 0;
+__device__ __forceinline__ void bsr_row_acc(const int* __restrict__ rp, const int* __restrict__ col, const double* __restrict__ val,
+                                            long nnzb, int row, int lane, const double* __restrict__ z,
+                                            const double* __restrict__ p, bool withP, double* az, double* ap) {
+    const int e = rp[row + 1];
+    for (int b = rp[row] + lane; b < e; b += LANES) {
+        const int c = col[b];
+        double a[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a[k] = val[k * nnzb + b];
+        const double z0 = z[3 * c], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
+        az[0] += a[0] * z0 + a[1] * z1 + a[2] * z2;
+        az[1] += a[3] * z0 + a[4] * z1 + a[5] * z2;
+        az[2] += a[6] * z0 + a[7] * z1 + a[8] * z2;
+        if (withP) {
+            const double p0 = p[3 * c], p1 = p[3 * c + 1], p2 = p[3 * c + 2];
+            ap[0] += a[0] * p0 + a[1] * p1 + a[2] * p2;
+            ap[1] += a[3] * p0 + a[4] * p1 + a[5] * p2;
+            ap[2] += a[6] * p0 + a[7] * p1 + a[8] * p2;
+        }
+    }
+}
+
+// Iteration `it`: p_new = z + beta p_old (p_new = z at it = 0), q = A p_new, partial
+// p_new . q.  Convergence of the residual left by the previous update is decided here.
+template <int LANES, int BS>
+__global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2, const int* __restrict__ erp,
+                                                 const int* __restrict__ ecol, const double* __restrict__ ev, int enz,
+                                                 const int* __restrict__ crp, const int* __restrict__ ccol,
+                                                 const double* __restrict__ cv, int cnz, int nrow, int rowsPerXcd,
+                                                 const double* __restrict__ z, const double* __restrict__ pold,
+                                                 double* __restrict__ pnew, double* __restrict__ q,
+                                                 const double* __restrict__ partU, int nU, double* __restrict__ partS,
+                                                 double* __restrict__ sc, int* __restrict__ flags) {
+    if (flags[F_DONE]) return;
+    __shared__ double lds[BS / 64];
+    double arz = 0, arr = 0;
+    for (int i = threadIdx.x; i < nU; i += BS) { arz += partU[i]; arr += partU[nU + i]; }
+
+    constexpr int RPB = BS / LANES;
+    const int row = xcd_row_base(rowsPerXcd, RPB) + (int)threadIdx.x / LANES;
+    const int lane = (int)threadIdx.x % LANES;
+    const bool withP = it > 0;
+    double az[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
     if (row < nrow) {
-        int e = rowptr[row + 1];
-        for (int b = rowptr[row] + lane; b < e; b += LANES) {
-            int c = col[b];
-            double x0 = p[3 * c], x1 = p[3 * c + 1], x2 = p[3 * c + 2];
-            s0 += val[b] * x0 + val[(long)nnzb + b] * x1 + val[2l * nnzb + b] * x2;
-            s1 += val[3l * nnzb + b] * x0 + val[4l * nnzb + b] * x1 + val[5l * nnzb + b] * x2;
-            s2 += val[6l * nnzb + b] * x0 + val[7l * nnzb + b] * x1 + val[8l * nnzb + b] * x2;
-        }
+        bsr_row_acc<LANES>(erp, ecol, ev, enz, row, lane, z, pold, withP, az, ap);
+        if (cnz > 0) bsr_row_acc<LANES>(crp, ccol, cv, cnz, row, lane, z, pold, withP, az, ap);
     }
 #pragma unroll
-    for (int o = LANES / 2; o > 0; o >>= 1) {
-        s0 += __shfl_xor(s0, o, LANES);
-        s1 += __shfl_xor(s1, o, LANES);
-        s2 += __shfl_xor(s2, o, LANES);
-    }
-    if (row < nrow && lane == 0) {
-        if (accumulate) { q[3 * row] += s0; q[3 * row + 1] += s1; q[3 * row + 2] += s2; }
-        else { q[3 * row] = s0; q[3 * row + 1] = s1; q[3 * row + 2] = s2; }
-    }
-}
-
-// finish a block reduction of up to two values; the last block to arrive sums the
-// per-block partials in a fixed order and calls fin(total0, total1)
-template <class Fin>
-__device__ __forceinline__ void last_block_finish(double v0, double v1, double* part, int* flags, Fin fin) {
-    __shared__ double lds[4];
-    __shared__ int amLast;
-    double b0 = block_sum_256(v0, lds);
-    double b1 = block_sum_256(v1, lds);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = b0;
-        part[2 * blockIdx.x + 1] = b1;
-        __threadfence();
-        int t = atomicAdd(&flags[F_TICKET], 1);
-        amLast = (t == (int)gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!amLast) return;
-    __threadfence();
-    double a0 = 0, a1 = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-        // agent-scope atomic loads: the partials come from other CUs and must not be
-        // served from this CU's (non-coherent) vector L1
-        a0 += __hip_atomic_load(&part[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a1 += __hip_atomic_load(&part[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    double t0 = block_sum_256(a0, lds);
-    double t1 = block_sum_256(a1, lds);
-    if (threadIdx.x == 0) {
-        flags[F_TICKET] = 0;
-        fin(t0, t1);
-    }
-}
-
-// row-wise: q += A_con p (over all rows), then pq = p . q
-__global__ void k_spmv_con_dot(const int* __restrict__ rowptr, const int* __restrict__ col, const double* __restrict__ val, int nnzb,
-                               int nrow, const double* __restrict__ p, double* __restrict__ q, double* __restrict__ sc,
-                               double* __restrict__ part, int* __restrict__ flags) {
-    if (flags[F_DONE]) return;
-    constexpr int LANES = 4;
-    long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int row = (int)(gid / LANES), lane = (int)(gid % LANES);
-    double s0 = 0, s1 = 0, s2 = 0;
-    if (row < nrow && nnzb > 0) {
-        int e = rowptr[row + 1];
-        for (int b = rowptr[row] + lane; b < e; b += LANES) {
-            int c = col[b];
-            double x0 = p[3 * c], x1 = p[3 * c + 1], x2 = p[3 * c + 2];
-            s0 += val[b] * x0 + val[(long)nnzb + b] * x1 + val[2l * nnzb + b] * x2;
-            s1 += val[3l * nnzb + b] * x0 + val[4l * nnzb + b] * x1 + val[5l * nnzb + b] * x2;
-            s2 += val[6l * nnzb + b] * x0 + val[7l * nnzb + b] * x1 + val[8l * nnzb + b] * x2;
-        }
-    }
-#pragma unroll
-    for (int o = LANES / 2; o > 0; o >>= 1) {
-        s0 += __shfl_xor(s0, o, LANES);
-        s1 += __shfl_xor(s1, o, LANES);
-        s2 += __shfl_xor(s2, o, LANES);
-    }
-    double d = 0;
-    if (row < nrow && lane == 0) {
-        double q0 = q[3 * row] + s0, q1 = q[3 * row + 1] + s1, q2 = q[3 * row + 2] + s2;
-        q[3 * row] = q0; q[3 * row + 1] = q1; q[3 * row + 2] = q2;
-        d = p[3 * row] * q0 + p[3 * row + 1] * q1 + p[3 * row + 2] * q2;
-    }
-    last_block_finish(d, 0.0, part, flags, [&](double t0, double) { sc[S_PQ] = t0; });
-}
-
-__global__ void k_pcg_update(int nV, double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ p,
-                             const double* __restrict__ q, const double* __restrict__ dinv, double* __restrict__ sc,
-                             double* __restrict__ part, int* __restrict__ flags) {
-    if (flags[F_DONE]) return;
-    double pq = sc[S_PQ];
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    double rz = 0, rr = 0;
-    if (pq > 0) {
-        double alpha = sc[S_RZ] / pq;
-        if (v < nV) {
-            double rv[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                x[3 * v + k] += alpha * p[3 * v + k];
-                rv[k] = r[3 * v + k] - alpha * q[3 * v + k];
-                r[3 * v + k] = rv[k];
-            }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                double zk = dinv[9 * v + 3 * k] * rv[0] + dinv[9 * v + 3 * k + 1] * rv[1] + dinv[9 * v + 3 * k + 2] * rv[2];
-                z[3 * v + k] = zk;
-                rz += rv[k] * zk;
-                rr += rv[k] * rv[k];
-            }
-        }
-    }
-    last_block_finish(rz, rr, part, flags, [&](double t0, double t1) {
-        int it = flags[F_ITERS] + 1;
-        flags[F_ITERS] = it;
-        if (!(pq > 0)) { flags[F_FAIL] = 1; flags[F_DONE] = 1; return; }
-        sc[S_BETA] = t0 / sc[S_RZ];
-        sc[S_RZ] = t0;
-        sc[S_RR] = t1;
-        if (sqrt(t1) <= sc[S_TOL] * sc[S_BNORM] || it >= flags[F_MAXIT]) flags[F_DONE] = 1;
-    });
-}
-__global__ void k_pcg_p(int n, double* __restrict__ p, const double* __restrict__ z, const double* __restrict__ sc, const int* __restrict__ flags) {
-    if (flags[F_DONE]) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + sc[S_BETA] * p[i];
-}
-
-// r = b, x = 0, z = Dinv r, p = z; partial sums of r.z and b.b
-__global__ void k_pcg_init(int nV, const double* __restrict__ b, double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
-                           double* __restrict__ p, const double* __restrict__ dinv, double* __restrict__ sc, double* __restrict__ part,
-                           int* __restrict__ flags) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    double rz = 0, bb = 0;
-    if (v < nV) {
-        double rv[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { rv[k] = b[3 * v + k]; r[3 * v + k] = rv[k]; x[3 * v + k] = 0; bb += rv[k] * rv[k]; }
+    for (int o = LANES / 2; o > 0; o >>= 1)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            double zk = dinv[9 * v + 3 * k] * rv[0] + dinv[9 * v + 3 * k + 1] * rv[1] + dinv[9 * v + 3 * k + 2] * rv[2];
-            z[3 * v + k] = zk; p[3 * v + k] = zk;
+            az[k] += __shfl_xor(az[k], o, LANES);
+            ap[k] += __shfl_xor(ap[k], o, LANES);
+        }
+
+    const double rz = block_allsum<BS>(arz, lds);
+    const double rr = block_allsum<BS>(arr, lds);
+    const double bb = it == 0 ? rr : sc[S_BB];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (rr <= tol2 * bb || it >= maxIt) {
+        if (lead) {
+            if (it == 0) sc[S_BB] = rr;
+            sc[S_RR] = rr;
+            flags[F_ITERS] = it;
+            flags[F_DONE] = 1;
+        }
+        return;
+    }
+    const double beta = withP ? rz / sc[S_RZ0 + ((it - 1) & 1)] : 0.0;
+    double pq = 0;
+    if (row < nrow && lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double zk = z[3 * row + k];
+            const double pk = withP ? zk + beta * pold[3 * row + k] : zk;
+            const double qk = withP ? az[k] + beta * ap[k] : az[k];
+            pnew[3 * row + k] = pk;
+            q[3 * row + k] = qk;
+            pq += pk * qk;
+        }
+    }
+    const double bpq = block_allsum<BS>(pq, lds);
+    if (threadIdx.x == 0) partS[blockIdx.x] = bpq;
+    if (lead) {
+        sc[S_RZ0 + (it & 1)] = rz;
+        if (it == 0) sc[S_BB] = rr;
+    }
+}
+
+// Iteration `it`: alpha = r.z / p.q, x += alpha p, r -= alpha q, z = Dinv r, partials
+// of r.z and r.r (written as partU[block], partU[nU + block]).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPerXcd, double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ z, const double* __restrict__ p,
+                                                   const double* __restrict__ q, const double* __restrict__ dinv,
+                                                   const double* __restrict__ partS, int nS, double* __restrict__ partU,
+                                                   double* __restrict__ sc, int* __restrict__ flags) {
+    if (flags[F_DONE]) return;
+    __shared__ double lds[BS / 64];
+    double a = 0;
+    for (int i = threadIdx.x; i < nS; i += BS) a += partS[i];
+    const int row = xcd_row_base(rowsPerXcd, BS) + (int)threadIdx.x;
+    const bool valid = row < nrow;
+    double pv[3] = {0, 0, 0}, qv[3] = {0, 0, 0}, xv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, D[9];
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            pv[k] = p[3 * row + k]; qv[k] = q[3 * row + k];
+            xv[k] = x[3 * row + k]; rv[k] = r[3 * row + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) D[k] = dinv[9l * row + k];
+    }
+    const double pq = block_allsum<BS>(a, lds);
+    if (!(pq > 0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            flags[F_FAIL] = 1;
+            flags[F_ITERS] = it;
+            flags[F_DONE] = 1;
+        }
+        return;
+    }
+    const double alpha = sc[S_RZ0 + (it & 1)] / pq;
+    double rz = 0, rr = 0;
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            x[3 * row + k] = xv[k] + alpha * pv[k];
+            rv[k] = rv[k] - alpha * qv[k];
+            r[3 * row + k] = rv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double zk = D[3 * k] * rv[0] + D[3 * k + 1] * rv[1] + D[3 * k + 2] * rv[2];
+            z[3 * row + k] = zk;
+            rz += rv[k] * zk;
+            rr += rv[k] * rv[k];
+        }
+    }
+    const double brz = block_allsum<BS>(rz, lds);
+    const double brr = block_allsum<BS>(rr, lds);
+    if (threadIdx.x == 0) {
+        partU[blockIdx.x] = brz;
+        partU[gridDim.x + blockIdx.x] = brr;
+    }
+}
+
+// r = b, x = 0, z = Dinv r; partials of r.z and r.r in the update kernel's layout;
+// the solver flags are cleared.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pcg_init(int nrow, int rowsPerXcd, const double* __restrict__ b, double* __restrict__ x,
+                                                 double* __restrict__ r, double* __restrict__ z, const double* __restrict__ dinv,
+                                                 double* __restrict__ partU, int* __restrict__ flags) {
+    __shared__ double lds[BS / 64];
+    const int row = xcd_row_base(rowsPerXcd, BS) + (int)threadIdx.x;
+    double rz = 0, rr = 0;
+    if (row < nrow) {
+        double rv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            rv[k] = b[3 * row + k];
+            r[3 * row + k] = rv[k];
+            x[3 * row + k] = 0;
+            rr += rv[k] * rv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double zk = dinv[9l * row + 3 * k] * rv[0] + dinv[9l * row + 3 * k + 1] * rv[1] + dinv[9l * row + 3 * k + 2] * rv[2];
+            z[3 * row + k] = zk;
             rz += rv[k] * zk;
         }
     }
-    last_block_finish(rz, bb, part, flags, [&](double t0, double t1) {
-        sc[S_RZ] = t0;
-        sc[S_RR] = t1;
-        sc[S_BNORM] = sqrt(t1);
+    const double brz = block_allsum<BS>(rz, lds);
+    const double brr = block_allsum<BS>(rr, lds);
+    if (threadIdx.x == 0) {
+        partU[blockIdx.x] = brz;
+        partU[gridDim.x + blockIdx.x] = brr;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        flags[F_DONE] = 0;
         flags[F_ITERS] = 0;
         flags[F_FAIL] = 0;
-        flags[F_DONE] = (t1 == 0.0) ? 1 : 0;
-    });
+    }
 }
 
 // diagonal 3x3 blocks of A_el + A_con, inverted (symmetric adjugate)
@@ -220,19 +283,6 @@ __global__ void k_dinv(int nV, const int* __restrict__ rpe, const int* __restric
     for (int k = 0; k < 9; ++k) dinv[9 * v + k] = r[k];
 }
 
-void spmv_full(Ctx& c, const double* p, double* q) {
-    BSR& E = c.Ael;
-    BSR& K = c.Acon;
-    hipLaunchKernelGGL(k_spmv<8>, dim3(nblocks(8l * E.nrow)), dim3(kBlock), 0, c.s, E.rowptr.p, E.col.p, E.val.p, E.nnzb, E.nrow, p, q, 0,
-                       (const int*)nullptr);
-    CIPC_LAUNCH_CHECK();
-    if (K.nnzb > 0) {
-        hipLaunchKernelGGL(k_spmv<4>, dim3(nblocks(4l * K.nrow)), dim3(kBlock), 0, c.s, K.rowptr.p, K.col.p, K.val.p, K.nnzb, K.nrow, p, q,
-                           1, (const int*)nullptr);
-        CIPC_LAUNCH_CHECK();
-    }
-}
-
 // returns 0 converged, 1 not SPD (p^T A p <= 0), 2 iteration limit
 int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
     const int nV = c.nV, n = 3 * nV;
@@ -241,45 +291,78 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     if (K.nnzb == 0) { K.rowptr.ensure(nV + 1); }
     double* r = c.pr.ensure(n);
     double* z = c.pz.ensure(n);
-    double* p = c.pp.ensure(n);
+    double* pbuf = c.pp.ensure(2l * n);       // p_old / p_new, alternating by iteration
     double* q = c.pq.ensure(n);
     double* dinv = c.pdinv.ensure(9l * nV);
-    double* sc = c.scal.p + 32;               // 16 solver scalars
-    int* flags = (int*)(c.counters.p + 32);   // 16 solver flags
-    int nbv = nblocks(nV);
-    int nbr = nblocks(4l * nV);
-    double* part = c.dscr.ensure(2l * std::max(nbv, nbr) + 2);
-    hipLaunchKernelGGL(k_dinv, dim3(nbv), dim3(kBlock), 0, c.s, nV, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p, K.val.p, K.nnzb,
-                       dinv);
+    double* sc = c.scal.p + 32;               // solver scalars
+    int* flags = (int*)(c.counters.p + 32);   // solver flags
+
+    constexpr int RPB = kSpmvBlock / kSpmvLanes;
+    const int rpxS = rows_per_xcd(nV, RPB), nS = kXcd * (rpxS / RPB);
+    const int rpxU = rows_per_xcd(nV, kUpdBlock), nU = kXcd * (rpxU / kUpdBlock);
+    double* partS = c.dscr.ensure((size_t)nS + 2l * nU);
+    double* partU = partS + nS;
+
+    hipLaunchKernelGGL(k_dinv, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p,
+                       K.val.p, K.nnzb, dinv);
     CIPC_LAUNCH_CHECK();
-    int hflags[5] = {0, 0, 0, 0, maxIter};
-    CIPC_HIP(hipMemcpyAsync(flags, hflags, sizeof(hflags), hipMemcpyHostToDevice, c.s));
-    double htol = tol;
-    CIPC_HIP(hipMemcpyAsync(sc + S_TOL, &htol, sizeof(double), hipMemcpyHostToDevice, c.s));
-    hipLaunchKernelGGL(k_pcg_init, dim3(nbv), dim3(kBlock), 0, c.s, nV, b, xsol, r, z, p, dinv, sc, part, flags);
+    hipLaunchKernelGGL(k_pcg_init<kUpdBlock>, dim3(nU), dim3(kUpdBlock), 0, c.s, nV, rpxU, b, xsol, r, z, dinv, partU, flags);
     CIPC_LAUNCH_CHECK();
+
     const int chunk = 16;
     int* hf = (int*)(c.hCnt + 32);
+    if (c.profiling && c.evPool.size() < 2 * chunk) {
+        for (size_t i = c.evPool.size(); i < 2 * chunk; ++i) {
+            hipEvent_t e;
+            CIPC_HIP(hipEventCreate(&e));
+            c.evPool.push_back(e);
+        }
+    }
+    const double tol2 = tol * tol;
+    const double spmvBytes = 76.0 * E.nnzb + 4.0 * (nV + 1) + (K.nnzb > 0 ? 76.0 * K.nnzb + 4.0 * (nV + 1) : 0.0);
+    int it = 0;
     for (;;) {
-        for (int k = 0; k < chunk; ++k) {
-            hipLaunchKernelGGL(k_spmv<8>, dim3(nblocks(8l * nV)), dim3(kBlock), 0, c.s, E.rowptr.p, E.col.p, E.val.p, E.nnzb, nV, p, q, 0,
-                               (const int*)flags);
-            hipLaunchKernelGGL(k_spmv_con_dot, dim3(nbr), dim3(kBlock), 0, c.s, K.rowptr.p, K.col.p, K.val.p, K.nnzb, nV, p, q, sc, part, flags);
-            hipLaunchKernelGGL(k_pcg_update, dim3(nbv), dim3(kBlock), 0, c.s, nV, xsol, r, z, p, q, dinv, sc, part, flags);
-            hipLaunchKernelGGL(k_pcg_p, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, p, z, sc, flags);
+        int launched = 0;
+        for (int k = 0; k < chunk && it <= maxIter; ++k, ++it, ++launched) {
+            const double* pold = pbuf + (size_t)((it + 1) & 1) * n;
+            double* pnew = pbuf + (size_t)(it & 1) * n;
+            if (c.profiling)
+                hipExtLaunchKernelGGL(k_pcg_spmv<kSpmvLanes, kSpmvBlock>, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.evPool[2 * k],
+                                      c.evPool[2 * k + 1], 0, it, maxIter, tol2, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p,
+                                      K.val.p, K.nnzb, nV, rpxS, z, pold, pnew, q, partU, nU, partS, sc, flags);
+            else
+                hipLaunchKernelGGL((k_pcg_spmv<kSpmvLanes, kSpmvBlock>), dim3(nS), dim3(kSpmvBlock), 0, c.s, it, maxIter, tol2, E.rowptr.p,
+                                   E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p, K.val.p, K.nnzb, nV, rpxS, z, pold, pnew, q, partU, nU,
+                                   partS, sc, flags);
+            hipLaunchKernelGGL(k_pcg_update<kUpdBlock>, dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z, (const double*)pnew, q,
+                               dinv, partS, nS, partU, sc, flags);
         }
         CIPC_LAUNCH_CHECK();
-        CIPC_HIP(hipMemcpyAsync(hf, flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipMemcpyAsync(hf, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, c.s));
         CIPC_HIP(hipStreamSynchronize(c.s));
-        if (hf[F_DONE]) break;
+        if (c.profiling) {
+            // SpMV launches of iterations before the one that detected convergence did
+            // the full work; the detecting launch and later ones return early
+            const int done = hf[F_DONE] ? hf[F_ITERS] : it;
+            const int base = it - launched;
+            for (int k = 0; k < launched && base + k < done; ++k) {
+                float ms = 0;
+                CIPC_HIP(hipEventElapsedTime(&ms, c.evPool[2 * k], c.evPool[2 * k + 1]));
+                c.profMs += ms;
+                c.profLaunches++;
+                c.profBytes += spmvBytes + 96.0 * nV;
+                c.profFlops += 18.0 * (E.nnzb + K.nnzb) + 8.0 * n;
+            }
+        }
+        if (hf[F_DONE] || it > maxIter) break;
     }
-    CIPC_HIP(hipMemcpyAsync(c.hScal + 32, sc, 8 * sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipMemcpyAsync(c.hScal + 32, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, c.s));
     CIPC_HIP(hipStreamSynchronize(c.s));
     if (iters) *iters = hf[F_ITERS];
-    double bn = c.hScal[32 + S_BNORM];
-    if (relres) *relres = bn > 0 ? std::sqrt(c.hScal[32 + S_RR]) / bn : 0.0;
+    const double bb = c.hScal[32 + S_BB], rr = c.hScal[32 + S_RR];
+    if (relres) *relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     if (hf[F_FAIL]) return 1;
-    if (bn > 0 && std::sqrt(c.hScal[32 + S_RR]) > tol * bn) return 2;
+    if (!hf[F_DONE] || rr > tol2 * bb) return 2;
     return 0;
 }
 

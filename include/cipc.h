@@ -114,6 +114,17 @@ int cipc_ccd_stepsize(cipc_ctx* ctx, const double* x, const double* p, double al
 int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, const double* b,
                       double* sol, double tol, int32_t* iters, double* relres);
 
+/* ---- instrumentation (no counterpart in the reference) ------------------ */
+/* Kernel timing of the dominant kernel of the step, the PCG SpMV (q = A p with
+ * p = z + beta p_old formed on the fly), from the dispatch's own start/end
+ * timestamps: enable != 0 starts recording (and resets the counters).
+ * out[0..5] = launches timed, total ms, block rows, stored 3x3 blocks of the
+ * elastic matrix, mean algorithmic bytes per launch (matrix values + column indices
+ * + row pointers of the elastic and contact matrices, z and p_old read, p_new and q
+ * written, each touched once), mean algorithmic flops per launch. */
+int cipc_set_profiling(cipc_ctx* ctx, int32_t enable);
+int cipc_get_profile(const cipc_ctx* ctx, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -194,7 +194,7 @@ int orc_solve(int n, const int* rp, const int* ci, const double* val, const doub
     A.ci.assign(ci, ci + rp[n]);
     A.val.assign(val, val + rp[n]);
     if (solver == 0) return solve_direct(A, b, x);
-    return solve_pcg(A, b, x, tol, 1000000, iters);
+    return solve_pcg(A, b, x, tol, 1000000, iters, 4);
 }
 // stats: newtonIters, nContacts, pcgIters, useGD, lsTrials, kappa, lastRes, E, tHess, tSolve, tCS, tCCD, tEnergy, tGrad, tTotal
 int orc_step(void* h, double* stats) {

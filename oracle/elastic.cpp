@@ -72,9 +72,14 @@ void elastic_grad(const Scene& S, const double* x, double* g) {
     }
 }
 
+// Local blocks are written straight into their final slots of `out` (the reference's
+// per-element triplet offsets, MEMBRANE.h:170-180, BENDING.h:443-455).
 void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
     const double h2 = S.P.dt * S.P.dt;
-    std::vector<std::vector<Block>> perT(S.nT);
+    std::vector<long> offT(S.nT + 1, 0);
+    for (int t = 0; t < S.nT; ++t) offT[t + 1] = offT[t] + (tri_all_dbc(S, t) ? 0 : 9);
+    const long base = (long)out.size();
+    out.resize(base + offT[S.nT]);
 #pragma omp parallel for schedule(static) num_threads(S.P.threads)
     for (int t = 0; t < S.nT; ++t) {
         if (tri_all_dbc(S, t)) continue;
@@ -83,18 +88,21 @@ void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
         membrane_hess(X3(x, v[0]), X3(x, v[1]), X3(x, v[2]), &S.Binv[3 * t], S.vol[t], S.lam, S.mu, H);
         for (int i = 0; i < 81; ++i) H[i] *= h2;
         make_pd(9, H);
+        Block* dst = &out[base + offT[t]];
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) {
-                Block B; B.r = v[a]; B.c = v[b];
+                Block& B = dst[3 * a + b];
+                B.r = v[a]; B.c = v[b];
                 for (int k = 0; k < 3; ++k)
                     for (int l = 0; l < 3; ++l) B.b[3 * k + l] = H[(3 * a + k) * 9 + 3 * b + l];
-                perT[t].push_back(B);
             }
     }
-    for (auto& v : perT) out.insert(out.end(), v.begin(), v.end());
     if (S.P.bendMult == 0) return;
     int nH = (int)S.hinge.size() / 4;
-    std::vector<std::vector<Block>> perH(nH);
+    std::vector<long> offH(nH + 1, 0);
+    for (int h = 0; h < nH; ++h) offH[h + 1] = offH[h] + (hinge_all_dbc(S, h) ? 0 : 16);
+    const long baseH = (long)out.size();
+    out.resize(baseH + offH[nH]);
 #pragma omp parallel for schedule(static) num_threads(S.P.threads)
     for (int h = 0; h < nH; ++h) {
         if (hinge_all_dbc(S, h)) continue;
@@ -110,15 +118,15 @@ void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
         for (int i = 0; i < 12; ++i)
             for (int j = 0; j < 12; ++j) H[i * 12 + j] = s1 * H[i * 12 + j] + s2 * gl[i] * gl[j];
         make_pd(12, H);
+        Block* dst = &out[baseH + offH[h]];
         for (int a = 0; a < 4; ++a)
             for (int b = 0; b < 4; ++b) {
-                Block B; B.r = v[a]; B.c = v[b];
+                Block& B = dst[4 * a + b];
+                B.r = v[a]; B.c = v[b];
                 for (int k = 0; k < 3; ++k)
                     for (int l = 0; l < 3; ++l) B.b[3 * k + l] = H[(3 * a + k) * 12 + 3 * b + l];
-                perH[h].push_back(B);
             }
     }
-    for (auto& v : perH) out.insert(out.end(), v.begin(), v.end());
 }
 
 double inertia_energy(const Scene& S, const double* x, const double* xt) {

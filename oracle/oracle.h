@@ -141,7 +141,7 @@ double inertia_energy(const Scene& S, const double* x, const double* xt);
 struct CSR { int n = 0; std::vector<int> rp, ci; std::vector<double> val; };
 void assemble(const Scene& S, const std::vector<Block>& blocks, CSR& A);   // + M, DBC projection
 int solve_direct(const CSR& A, const double* b, double* x);                // 0 ok, 1 not SPD
-int solve_pcg(const CSR& A, const double* b, double* x, double tol, int maxIter, int* iters);
+int solve_pcg(const CSR& A, const double* b, double* x, double tol, int maxIter, int* iters, int threads = 1);
 
 struct StepStats {
     int newtonIters = 0, nContacts = 0, pcgIters = 0, useGDCount = 0, lsTrials = 0;

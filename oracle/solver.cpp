@@ -15,38 +15,67 @@
 // on committed fixtures (tests/test_oracle_solver.py).
 #include "oracle.h"
 #include <algorithm>
+#include <parallel/algorithm>
 #include <queue>
 #include <numeric>
 
 namespace orc {
 
+// Blocks are summed per (row, col) in their input order (stable sort), the mass is
+// added to the diagonal blocks, then the block matrix is expanded to scalar CSR.
 void assemble(const Scene& S, const std::vector<Block>& blocks, CSR& A) {
     const int n = 3 * S.nV;
-    struct T { int r, c; double v; };
-    std::vector<T> trip;
-    trip.reserve(blocks.size() * 9 + n);
-    for (const Block& B : blocks)
-        for (int k = 0; k < 3; ++k)
-            for (int l = 0; l < 3; ++l) trip.push_back({3 * B.r + k, 3 * B.c + l, B.b[3 * k + l]});
-    for (int i = 0; i < S.nV; ++i)
-        for (int k = 0; k < 3; ++k) trip.push_back({3 * i + k, 3 * i + k, S.mass[i]});
-    std::stable_sort(trip.begin(), trip.end(), [](const T& a, const T& b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+    const long nb = (long)blocks.size();
+    std::vector<std::pair<uint64_t, long>> key(nb + S.nV);
+#pragma omp parallel for schedule(static) num_threads(S.P.threads)
+    for (long i = 0; i < nb; ++i) key[i] = {(uint64_t)blocks[i].r * S.nV + blocks[i].c, i};
+    for (int v = 0; v < S.nV; ++v) key[nb + v] = {(uint64_t)v * S.nV + v, -1 - (long)v};   // mass entries
+    __gnu_parallel::stable_sort(key.begin(), key.end(), [](const std::pair<uint64_t, long>& a, const std::pair<uint64_t, long>& b) {
+        return a.first < b.first;
+    });
+    // unique blocks
+    std::vector<long> start;
+    start.reserve(key.size() / 4 + 1);
+    for (size_t i = 0; i < key.size(); ++i)
+        if (i == 0 || key[i].first != key[i - 1].first) start.push_back((long)i);
+    const long nu = (long)start.size();
+    start.push_back((long)key.size());
+    std::vector<double> bval(9 * nu);
+    std::vector<int> brow(nu), bcol(nu);
+#pragma omp parallel for schedule(static) num_threads(S.P.threads)
+    for (long u = 0; u < nu; ++u) {
+        double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (long j = start[u]; j < start[u + 1]; ++j) {
+            long id = key[j].second;
+            if (id >= 0) for (int k = 0; k < 9; ++k) s[k] += blocks[id].b[k];
+            else { double m = S.mass[-1 - id]; s[0] += m; s[4] += m; s[8] += m; }
+        }
+        int r = (int)(key[start[u]].first / S.nV), c = (int)(key[start[u]].first % S.nV);
+        brow[u] = r; bcol[u] = c;
+        bool z = S.isDBC[r] || S.isDBC[c];
+        for (int k = 0; k < 9; ++k) bval[9 * u + k] = z ? ((r == c && k % 4 == 0) ? 1.0 : 0.0) : s[k];
+    }
+    // expand block rows to scalar rows
+    std::vector<long> rowBlock(S.nV + 1, 0);
+    for (long u = 0; u < nu; ++u) rowBlock[brow[u] + 1]++;
+    for (int v = 0; v < S.nV; ++v) rowBlock[v + 1] += rowBlock[v];
     A.n = n;
     A.rp.assign(n + 1, 0);
-    A.ci.clear(); A.val.clear();
-    for (size_t i = 0; i < trip.size();) {
-        size_t j = i;
-        double s = 0;
-        while (j < trip.size() && trip[j].r == trip[i].r && trip[j].c == trip[i].c) s += trip[j++].v;
-        A.ci.push_back(trip[i].c); A.val.push_back(s);
-        A.rp[trip[i].r + 1]++;
-        i = j;
-    }
+    for (int v = 0; v < S.nV; ++v)
+        for (int k = 0; k < 3; ++k) A.rp[3 * v + k + 1] = (int)(3 * (rowBlock[v + 1] - rowBlock[v]));
     for (int r = 0; r < n; ++r) A.rp[r + 1] += A.rp[r];
-    for (int r = 0; r < n; ++r)
-        for (int k = A.rp[r]; k < A.rp[r + 1]; ++k) {
-            int c = A.ci[k];
-            if (S.isDBC[r / 3] || S.isDBC[c / 3]) A.val[k] = (r == c) ? 1.0 : 0.0;
+    A.ci.resize(A.rp[n]);
+    A.val.resize(A.rp[n]);
+#pragma omp parallel for schedule(static) num_threads(S.P.threads)
+    for (int v = 0; v < S.nV; ++v)
+        for (int k = 0; k < 3; ++k) {
+            long o = A.rp[3 * v + k];
+            for (long u = rowBlock[v]; u < rowBlock[v + 1]; ++u)
+                for (int l = 0; l < 3; ++l) {
+                    A.ci[o] = 3 * bcol[u] + l;
+                    A.val[o] = bval[9 * u + 3 * k + l];
+                    ++o;
+                }
         }
 }
 
@@ -155,14 +184,6 @@ int solve_direct(const CSR& A, const double* b, double* x) {
     return 0;
 }
 
-static void spmv(const CSR& A, const double* x, double* y) {
-    for (int r = 0; r < A.n; ++r) {
-        double s = 0;
-        for (int k = A.rp[r]; k < A.rp[r + 1]; ++k) s += A.val[k] * x[A.ci[k]];
-        y[r] = s;
-    }
-}
-
 // 3x3 symmetric block inverse via the adjugate
 static void inv3(const double* a, double* r) {
     double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
@@ -173,9 +194,13 @@ static void inv3(const double* a, double* r) {
     r[6] = c02 * id; r[7] = (a[1] * a[6] - a[0] * a[7]) * id; r[8] = (a[0] * a[4] - a[1] * a[3]) * id;
 }
 
-int solve_pcg(const CSR& A, const double* b, double* x, double tol, int maxIter, int* iters) {
+// Block-Jacobi PCG with the same stopping rule as the GPU solver; OpenMP parallel
+// loops (the CPU baseline runs it on all host cores).
+int solve_pcg(const CSR& A, const double* b, double* x, double tol, int maxIter, int* iters, int threads) {
     const int n = A.n, nb = n / 3;
-    std::vector<double> Dinv(9 * nb, 0.0);
+    if (threads < 1) threads = 1;
+    std::vector<double> Dinv(9 * (size_t)nb, 0.0);
+#pragma omp parallel for schedule(static) num_threads(threads)
     for (int i = 0; i < nb; ++i) {
         double D[9] = {0};
         for (int k = 0; k < 3; ++k) {
@@ -185,39 +210,56 @@ int solve_pcg(const CSR& A, const double* b, double* x, double tol, int maxIter,
                 if (c / 3 == i) D[3 * k + c % 3] = A.val[q];
             }
         }
-        inv3(D, &Dinv[9 * i]);
+        inv3(D, &Dinv[9 * (size_t)i]);
     }
     std::vector<double> r(b, b + n), z(n), p(n), Ap(n);
-    for (int i = 0; i < n; ++i) x[i] = 0;
-    double bn = 0;
-    for (int i = 0; i < n; ++i) bn += b[i] * b[i];
+    double bn = 0, rz = 0;
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : bn, rz)
+    for (int i = 0; i < nb; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            x[3 * i + k] = 0;
+            z[3 * i + k] = Dinv[9 * (size_t)i + 3 * k] * r[3 * i] + Dinv[9 * (size_t)i + 3 * k + 1] * r[3 * i + 1] +
+                           Dinv[9 * (size_t)i + 3 * k + 2] * r[3 * i + 2];
+            p[3 * i + k] = z[3 * i + k];
+            bn += b[3 * i + k] * b[3 * i + k];
+            rz += r[3 * i + k] * z[3 * i + k];
+        }
+    }
     bn = std::sqrt(bn);
     if (bn == 0) { if (iters) *iters = 0; return 0; }
-    auto prec = [&](const std::vector<double>& in, std::vector<double>& out) {
-        for (int i = 0; i < nb; ++i)
-            for (int k = 0; k < 3; ++k)
-                out[3 * i + k] = Dinv[9 * i + 3 * k] * in[3 * i] + Dinv[9 * i + 3 * k + 1] * in[3 * i + 1] + Dinv[9 * i + 3 * k + 2] * in[3 * i + 2];
-    };
-    prec(r, z);
-    p = z;
-    double rz = 0;
-    for (int i = 0; i < n; ++i) rz += r[i] * z[i];
+    double rn = bn * bn;
     int it = 0;
     for (; it < maxIter; ++it) {
-        double rn = 0;
-        for (int i = 0; i < n; ++i) rn += r[i] * r[i];
         if (std::sqrt(rn) <= tol * bn) break;
-        spmv(A, p.data(), Ap.data());
         double pAp = 0;
-        for (int i = 0; i < n; ++i) pAp += p[i] * Ap[i];
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : pAp)
+        for (int row = 0; row < n; ++row) {
+            double s = 0;
+            for (int k = A.rp[row]; k < A.rp[row + 1]; ++k) s += A.val[k] * p[A.ci[k]];
+            Ap[row] = s;
+            pAp += p[row] * s;
+        }
         if (!(pAp > 0)) { if (iters) *iters = it; return 1; }
-        double al = rz / pAp;
-        for (int i = 0; i < n; ++i) { x[i] += al * p[i]; r[i] -= al * Ap[i]; }
-        prec(r, z);
-        double rz2 = 0;
-        for (int i = 0; i < n; ++i) rz2 += r[i] * z[i];
+        double al = rz / pAp, rz2 = 0, rn2 = 0;
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : rz2, rn2)
+        for (int i = 0; i < nb; ++i) {
+            double rv[3];
+            for (int k = 0; k < 3; ++k) {
+                x[3 * i + k] += al * p[3 * i + k];
+                rv[k] = r[3 * i + k] - al * Ap[3 * i + k];
+                r[3 * i + k] = rv[k];
+            }
+            for (int k = 0; k < 3; ++k) {
+                double zk = Dinv[9 * (size_t)i + 3 * k] * rv[0] + Dinv[9 * (size_t)i + 3 * k + 1] * rv[1] + Dinv[9 * (size_t)i + 3 * k + 2] * rv[2];
+                z[3 * i + k] = zk;
+                rz2 += rv[k] * zk;
+                rn2 += rv[k] * rv[k];
+            }
+        }
         double be = rz2 / rz;
         rz = rz2;
+        rn = rn2;
+#pragma omp parallel for schedule(static) num_threads(threads)
         for (int i = 0; i < n; ++i) p[i] = z[i] + be * p[i];
     }
     if (iters) *iters = it;

@@ -185,7 +185,7 @@ int advance_one_step(Scene& S, StepStats& st) {
             ta = now();
             int rc;
             if (P.solver == 0) rc = solve_direct(A, rhs.data(), sol.data());
-            else { int it = 0; rc = solve_pcg(A, rhs.data(), sol.data(), P.pcgTol, P.pcgMaxIter, &it); st.pcgIters += it; }
+            else { int it = 0; rc = solve_pcg(A, rhs.data(), sol.data(), P.pcgTol, P.pcgMaxIter, &it, P.threads); st.pcgIters += it; }
             st.tSolve += now() - ta;
             solved = (rc == 0);
             if (!solved) { useGD = true; st.useGDCount++; }
