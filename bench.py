@@ -3,14 +3,21 @@
 Workload (default): four 201x201 sheets 5 mm apart (161,604 cloth vertices / 320,000
 triangles) falling onto a Dirichlet icosphere (r 0.2 m, level 4) and a 2-triangle ground,
 rho 200, E 1e6, nu 0.3, t 1 mm, d_hat 1 mm, dt 0.01 s, Newton tol 1e-2 * bbox diagonal,
-FP64 everywhere, block-Jacobi PCG to relative residual 1e-5.  A "step" is one call of the
-implicit-Euler step (the reference's Advance_One_Step_IE_Hinge).  Inputs are generated on
-the host and uploaded before the timed region; positions stay resident in HBM.
+Newton linear solve to relative residual 1e-5.  A "step" is one call of the implicit-Euler
+step (the reference's Advance_One_Step_IE_Hinge).  Inputs are generated on the host and
+uploaded before the timed region; positions stay resident in HBM.
 
-The JSON line carries the roofline of the dominant kernel (elastic-block SpMV of the
-Newton PCG, timed with HIP events on the library's stream during the timed steps) and a
-CPU baseline: the FP64 oracle (a C++ restatement of the reference path, same PCG) on a
-bounded sample -- the first step(s) of the same scene -- on the host's cores.
+Two solver forms are timed on the same scene, each from the initial state:
+  fp64   block-Jacobi PCG in FP64;
+  mixed  FP32 Hessian blocks / Krylov vectors / SpMV inside FP64 iterative refinement,
+         stopped on the FP64 residual (same 1e-5 relative bar).
+`value` is the steps/s of the headline form (mixed when both run); the other form's
+figures are reported beside it.
+
+The JSON line carries the roofline of the dominant kernel of the headline form (the PCG
+SpMV, timed from its own dispatch timestamps on the library's stream during the timed
+steps) and a CPU baseline: the FP64 oracle (a C++ restatement of the reference path, same
+PCG) on a bounded sample -- the first step of the same scene -- on the host's cores.
 """
 import argparse
 import json
@@ -23,11 +30,14 @@ sys.path.insert(0, os.path.join(ROOT, "codim-ipc_amd"))
 
 METRIC = "sim steps/s + ms/Newton iter, 320k-tri cloth; plain FP64 vs mixed+learned PCG"
 HBM_PEAK_GBS = 8000.0
+KERNEL = {"fp64": "k_pcg_spmv<double,16,256> (PCG SpMV, FP64 blocks)",
+          "mixed": "k_pcg_spmv<float,16,256> (PCG SpMV, FP32 blocks)"}
 
 
-def gpu_run(args, sd):
+def gpu_run(args, sd, precision):
     import cipc
-    sim = cipc.Simulator(sd, pcg_tol=args.pcg_tol)
+    import numpy as np
+    sim = cipc.Simulator(sd, pcg_tol=args.pcg_tol, precision=precision)
     for _ in range(args.warmup):
         sim.step()
     sim.set_profiling(True)
@@ -38,7 +48,22 @@ def gpu_run(args, sd):
     elapsed = time.perf_counter() - t0
     prof = sim.profile()
     sim.set_profiling(False)
-    return elapsed, stats, prof
+    x, _ = sim.get_state()
+    C = sim.constraint_set(x)
+    min_d = float(np.sqrt(sim.min_dist2(x, C)[1])) if len(C) else None
+    newton = sum(s["newton_iters"] for s in stats)
+    pcg = sum(s["pcg_iters"] for s in stats)
+    roof = None
+    if prof["launches"]:
+        avg_s = prof["total_ms"] / prof["launches"] / 1e3
+        ach = prof["bytes_per_launch"] / avg_s / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+                    traffic=None, kernel=KERNEL[precision], launches=prof["launches"], avg_us=round(avg_s * 1e6, 2),
+                    bytes_per_launch=int(prof["bytes_per_launch"]), nnzb=prof["nnzb"], block_rows=prof["block_rows"])
+    return dict(precision=precision, steps_per_s=args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps,
+                ms_per_newton_iter=1e3 * elapsed / max(newton, 1), newton_iters_per_step=newton / args.steps,
+                pcg_iters_per_newton=pcg / max(newton, 1), contacts_last_step=stats[-1]["contacts"],
+                statuses=sorted(set(s["status"] for s in stats)), min_dist_final=min_d, roofline=roof)
 
 
 def cpu_baseline(args, sd):
@@ -62,9 +87,10 @@ def cpu_baseline(args, sd):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--pcg-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -77,45 +103,39 @@ def main():
     from cipc import scenes
     sd = scenes.scene_c2() if args.workload == "c2" else scenes.scene_c1()
 
-    elapsed, stats, prof = gpu_run(args, sd)
-    newton = sum(s["newton_iters"] for s in stats)
-    pcg = sum(s["pcg_iters"] for s in stats)
-    steps_per_s = args.steps / elapsed
-    roof = None
-    if prof["launches"]:
-        avg_s = prof["total_ms"] / prof["launches"] / 1e3
-        ach = prof["bytes_per_launch"] / avg_s / 1e9
-        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=None, kernel="k_spmv<8> (elastic BSR SpMV in PCG)", launches=prof["launches"],
-                    avg_us=round(avg_s * 1e6, 2), bytes_per_launch=int(prof["bytes_per_launch"]),
-                    nnzb=prof["nnzb"], block_rows=prof["block_rows"])
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args, sd)
+    modes = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    runs = {m: gpu_run(args, sd, m) for m in modes}
+    head = runs[modes[-1]]
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, sd)
     out = {
         "metric": METRIC,
-        "value": round(steps_per_s, 4),
+        "value": round(head["steps_per_s"], 4),
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
-        "ms_per_newton_iter": round(1e3 * elapsed / max(newton, 1), 3),
-        "newton_iters_per_step": round(newton / args.steps, 3),
-        "pcg_iters_per_newton": round(pcg / max(newton, 1), 1),
-        "contacts_last_step": stats[-1]["contacts"],
+        "ms_per_step": round(head["ms_per_step"], 3),
+        "ms_per_newton_iter": round(head["ms_per_newton_iter"], 3),
+        "newton_iters_per_step": round(head["newton_iters_per_step"], 3),
+        "pcg_iters_per_newton": round(head["pcg_iters_per_newton"], 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head["precision"] == "mixed" else "f64",
         "data": "synthetic",
-        "mode": "plain FP64, block-Jacobi PCG",
+        "mode": {"fp64": "plain FP64, block-Jacobi PCG",
+                 "mixed": "FP32 blocks/vectors/SpMV in FP64 iterative refinement, block-Jacobi PCG"}[head["precision"]],
         "config": {"workload": sd.name, "vertices": sd.nV, "triangles": sd.nT, "cloth_vertices": sd.n_cloth,
-                   "steps_simulated": args.warmup + args.steps, "dt": sd.params["dt"], "dhat": sd.params["dHat"],
+                   "steps_simulated_per_form": args.warmup + args.steps, "dt": sd.params["dt"], "dhat": sd.params["dHat"],
                    "pcg_tol": args.pcg_tol},
-        "roofline": roof,
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
+        "forms": {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items() if k != "roofline"}
+                  for m, r in runs.items()},
     }
+    if "fp64" in runs and len(runs) > 1:
+        out["speedup_vs_fp64"] = round(head["steps_per_s"] / runs["fp64"]["steps_per_s"], 3)
+        out["roofline_fp64"] = runs["fp64"]["roofline"]
     print(json.dumps(out))
 
 

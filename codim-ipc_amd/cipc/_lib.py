@@ -97,6 +97,8 @@ PROTOTYPES = {
     "cipc_min_dist2": (ctypes.c_int, [P, D, I32, ctypes.c_int32, D, D]),
     "cipc_ccd_stepsize": (ctypes.c_int, [P, D, D, ctypes.c_double, D, D]),
     "cipc_linear_solve": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, D, D, ctypes.c_double, I32, D]),
+    "cipc_set_precision": (ctypes.c_int, [P, ctypes.c_int32]),
+    "cipc_get_solver_info": (ctypes.c_int, [P, I32]),
     "cipc_set_profiling": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_get_profile": (ctypes.c_int, [P, D]),
 }

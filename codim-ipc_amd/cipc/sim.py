@@ -26,8 +26,13 @@ def _i(a):
 
 
 class Simulator:
-    def __init__(self, sd, pcg_tol=1e-5, pcg_max_iter=100000):
-        """Initialize_Shell_Hinge_EIPC + Initialize_OIPC (DISCRETE_SHELL.h:218-357, 555-577)."""
+    PRECISION = {"fp64": 0, "mixed": 1}
+
+    def __init__(self, sd, pcg_tol=1e-5, pcg_max_iter=100000, precision="fp64"):
+        """Initialize_Shell_Hinge_EIPC + Initialize_OIPC (DISCRETE_SHELL.h:218-357, 555-577).
+
+        precision: "fp64" (FP64 block-Jacobi PCG) or "mixed" (FP32 blocks/vectors/SpMV
+        inside FP64 iterative refinement on the true residual)."""
         self.lib = _lib.load()
         P = sd.params
         self._keep = [sd.X, sd.tris, sd.dbc_idx, sd.dbc_target]
@@ -64,6 +69,8 @@ class Simulator:
         s = np.zeros(6, dtype=np.int32)
         self.lib.cipc_get_sizes(self.h, _i(s))
         self.nT, self.nBN, self.nBE, self.nH = int(s[1]), int(s[2]), int(s[3]), int(s[4])
+        if precision != "fp64":
+            self.set_precision(precision)
 
     def close(self):
         if getattr(self, "h", None):
@@ -116,11 +123,19 @@ class Simulator:
         self._check(rc)
         return st.as_dict()
 
+    def set_precision(self, precision):
+        self._check(self.lib.cipc_set_precision(self.h, self.PRECISION[precision]))
+
+    def solver_info(self):
+        out = np.zeros(4, np.int32)
+        self._check(self.lib.cipc_get_solver_info(self.h, _i(out)))
+        return dict(refinements=int(out[0]))
+
     def set_profiling(self, enable=True):
         self._check(self.lib.cipc_set_profiling(self.h, 1 if enable else 0))
 
     def profile(self):
-        """Event timing of the elastic SpMV inside the Newton PCG (the step's dominant kernel)."""
+        """Dispatch timing of the PCG SpMV (the step's dominant kernel) since set_profiling."""
         out = np.zeros(6)
         self._check(self.lib.cipc_get_profile(self.h, _d(out)))
         return dict(launches=int(out[0]), total_ms=out[1], block_rows=int(out[2]), nnzb=int(out[3]),

@@ -330,6 +330,21 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
     });
 }
 
+int cipc_set_precision(cipc_ctx* ctx, int32_t mode) {
+    if (!ctx) return CIPC_E_INVALID;
+    if (mode != 0 && mode != 1) {
+        ctx->c.lastError = "precision mode must be 0 (FP64) or 1 (mixed)";
+        return CIPC_E_INVALID;
+    }
+    ctx->c.P.precision = mode;
+    return CIPC_OK;
+}
+int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out) {
+    if (!ctx || !out) return CIPC_E_INVALID;
+    out[0] = ctx->c.lastRefinements;
+    return CIPC_OK;
+}
+
 int cipc_set_profiling(cipc_ctx* ctx, int32_t enable) {
     if (!ctx) return CIPC_E_INVALID;
     ctx->c.profiling = enable ? 1 : 0;

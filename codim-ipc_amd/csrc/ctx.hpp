@@ -19,6 +19,7 @@ struct Params {
     double pcgTol = 1e-5;
     int pcgMaxIter = 100000;
     int collision = 1;
+    int precision = 0;            // 0: FP64 PCG; 1: FP32 PCG inside FP64 iterative refinement
 };
 
 // spatial grid used by the constraint-set broad phase and the CCD candidate search
@@ -103,6 +104,10 @@ struct Ctx {
 
     // PCG
     DevBuf<double> pr, pz, pp, pq, pdinv, psol;
+    // mixed precision: FP32 copies of the matrix values, FP32 Krylov vectors, FP64 residual
+    DevBuf<float> valE32, valK32, fr, fz, fp, fq, fdinv, fx, fb;
+    DevBuf<double> rres, xacc;
+    int lastRefinements = 0;
 
     // host pinned scalars
     double* hScal = nullptr;           // 64 doubles

@@ -5,22 +5,28 @@
 // rule ||r|| <= tol ||b|| follows the reference's iterative-solver precedent
 // (AMGCL_SOLVER.h:53-65, rel-tol 1e-5).
 //
-// One iteration is two kernels and no atomics:
-//   k_pcg_spmv   q = A (z + beta p_old) for every block row, p_new written, partial
-//                sums of p_new . q per workgroup;
-//   k_pcg_update x += alpha p, r -= alpha q, z = Dinv r, partial sums of r.z, r.r.
+// Two precisions share one set of kernels (template on the value type V):
+//   FP64      V = double throughout;
+//   mixed     matrix blocks, Krylov vectors and SpMV accumulation in FP32, dot
+//             products summed in FP64; an outer FP64 iterative refinement
+//             (x += d, r = b - H x with the FP64 blocks) repeats the FP32 solve on the
+//             FP64 residual until ||r|| <= tol ||b|| holds in FP64.
+//
+// One CG iteration is two kernels and no atomics:
+//   k_pcg_spmv   beta from the previous update's partial sums, p = z + beta p_old formed
+//                on the fly, q = A p, p written, partial sums of p.q per workgroup;
+//   k_pcg_update x += alpha p, r -= alpha q, z = Dinv r, partial sums of r.z and r.r.
 // Each kernel starts by summing the previous kernel's per-workgroup partials itself
-// (every workgroup, same fixed order, so every workgroup holds the identical scalar);
-// kernel boundaries on the one stream order the partials' writes before those reads,
-// so no fence or ticket counter is needed.  The SpMV keeps A z and A p_old in separate
-// accumulators so the scalar reduction's latency overlaps the matrix stream, and beta
-// is applied when the row is finished.
+// (every workgroup in the same fixed order, so all hold the identical scalar); kernel
+// boundaries on the one stream order those writes before the reads, so no fence or
+// ticket counter is needed.  The SpMV runs a fixed grid (kSpmvGrid workgroups) so the
+// update kernel has few partials to sum.
 //
 // Matrix blocks are stored structure-of-arrays (entry k of all blocks contiguous);
-// LANES lanes share a block row and stride over its blocks, so the lanes of a wave
-// read consecutive addresses.  Rows are mapped XCD-aware: workgroup b runs on XCD
-// b % 8 and XCD x owns one contiguous eighth of the rows, so the gathered neighbour
-// values stay in that XCD's L2.
+// LANES lanes share a block row and stride over its blocks, so the lanes of a wave read
+// consecutive addresses.  Rows are mapped XCD-aware: workgroup b runs on XCD b % 8 and
+// XCD x owns one contiguous eighth of the rows, so gathered neighbour values stay in
+// that XCD's L2.
 //
 // The host launches iterations in chunks and polls an on-device done flag between
 // chunks; kernels launched after convergence return at once.
@@ -34,8 +40,9 @@ namespace cipc {
 enum { S_RZ0 = 0, S_RZ1, S_BB, S_RR };
 enum { F_DONE = 0, F_ITERS, F_FAIL };
 
-constexpr int kSpmvLanes = 8;
-constexpr int kSpmvBlock = 512;
+constexpr int kSpmvLanes = 16;
+constexpr int kSpmvBlock = 256;
+constexpr int kSpmvGrid = 1024;
 constexpr int kUpdBlock = 256;
 constexpr int kXcd = 8;
 
@@ -53,71 +60,48 @@ __device__ __forceinline__ double block_allsum(double v, double* lds) {
     return r;
 }
 
-// rows per XCD, rounded up to whole workgroups of `rowsPerBlock`
-inline int rows_per_xcd(int nrow, int rowsPerBlock) {
+// rows per XCD, rounded up to whole groups of `rowsPerGroup`
+inline int rows_per_xcd(int nrow, int rowsPerGroup) {
     int r = (nrow + kXcd - 1) / kXcd;
-    return (r + rowsPerBlock - 1) / rowsPerBlock * rowsPerBlock;
-}
-__device__ __forceinline__ int xcd_row_base(int rowsPerXcd, int rowsPerBlock) {
-    return (int)(blockIdx.x % kXcd) * rowsPerXcd + (int)(blockIdx.x / kXcd) * rowsPerBlock;
+    return (r + rowsPerGroup - 1) / rowsPerGroup * rowsPerGroup;
 }
 
-template <int LANES>
-__device__ __forceinline__ void bsr_row_acc(const int* __restrict__ rp, const int* __restrict__ col, const double* __restrict__ val,
-                                            long nnzb, int row, int lane, const double* __restrict__ z,
-                                            const double* __restrict__ p, bool withP, double* az, double* ap) {
+template <class V, int LANES>
+__device__ __forceinline__ void bsr_row(const int* __restrict__ rp, const int* __restrict__ col, const V* __restrict__ val, long nnzb,
+                                        int row, int lane, const V* __restrict__ z, const V* __restrict__ p, V beta, bool withP,
+                                        V* s) {
     const int e = rp[row + 1];
     for (int b = rp[row] + lane; b < e; b += LANES) {
         const int c = col[b];
-        double a[9];
+        V a[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) a[k] = val[k * nnzb + b];
-        const double z0 = z[3 * c], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
-        az[0] += a[0] * z0 + a[1] * z1 + a[2] * z2;
-        az[1] += a[3] * z0 + a[4] * z1 + a[5] * z2;
-        az[2] += a[6] * z0 + a[7] * z1 + a[8] * z2;
+        V x0 = z[3 * c], x1 = z[3 * c + 1], x2 = z[3 * c + 2];
         if (withP) {
-            const double p0 = p[3 * c], p1 = p[3 * c + 1], p2 = p[3 * c + 2];
-            ap[0] += a[0] * p0 + a[1] * p1 + a[2] * p2;
-            ap[1] += a[3] * p0 + a[4] * p1 + a[5] * p2;
-            ap[2] += a[6] * p0 + a[7] * p1 + a[8] * p2;
+            x0 += beta * p[3 * c];
+            x1 += beta * p[3 * c + 1];
+            x2 += beta * p[3 * c + 2];
         }
+        s[0] += a[0] * x0 + a[1] * x1 + a[2] * x2;
+        s[1] += a[3] * x0 + a[4] * x1 + a[5] * x2;
+        s[2] += a[6] * x0 + a[7] * x1 + a[8] * x2;
     }
 }
 
-// Iteration `it`: p_new = z + beta p_old (p_new = z at it = 0), q = A p_new, partial
-// p_new . q.  Convergence of the residual left by the previous update is decided here.
-template <int LANES, int BS>
+// Iteration `it`: convergence of the residual left by the previous update is decided
+// first; then p_new = z + beta p_old (p_new = z at it = 0), q = A p_new, partial p.q.
+template <class V, int LANES, int BS>
 __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2, const int* __restrict__ erp,
-                                                 const int* __restrict__ ecol, const double* __restrict__ ev, int enz,
-                                                 const int* __restrict__ crp, const int* __restrict__ ccol,
-                                                 const double* __restrict__ cv, int cnz, int nrow, int rowsPerXcd,
-                                                 const double* __restrict__ z, const double* __restrict__ pold,
-                                                 double* __restrict__ pnew, double* __restrict__ q,
+                                                 const int* __restrict__ ecol, const V* __restrict__ ev, int enz,
+                                                 const int* __restrict__ crp, const int* __restrict__ ccol, const V* __restrict__ cv,
+                                                 int cnz, int nrow, int rowsPerXcd, const V* __restrict__ z,
+                                                 const V* __restrict__ pold, V* __restrict__ pnew, V* __restrict__ q,
                                                  const double* __restrict__ partU, int nU, double* __restrict__ partS,
                                                  double* __restrict__ sc, int* __restrict__ flags) {
     if (flags[F_DONE]) return;
     __shared__ double lds[BS / 64];
     double arz = 0, arr = 0;
     for (int i = threadIdx.x; i < nU; i += BS) { arz += partU[i]; arr += partU[nU + i]; }
-
-    constexpr int RPB = BS / LANES;
-    const int row = xcd_row_base(rowsPerXcd, RPB) + (int)threadIdx.x / LANES;
-    const int lane = (int)threadIdx.x % LANES;
-    const bool withP = it > 0;
-    double az[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
-    if (row < nrow) {
-        bsr_row_acc<LANES>(erp, ecol, ev, enz, row, lane, z, pold, withP, az, ap);
-        if (cnz > 0) bsr_row_acc<LANES>(crp, ccol, cv, cnz, row, lane, z, pold, withP, az, ap);
-    }
-#pragma unroll
-    for (int o = LANES / 2; o > 0; o >>= 1)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            az[k] += __shfl_xor(az[k], o, LANES);
-            ap[k] += __shfl_xor(ap[k], o, LANES);
-        }
-
     const double rz = block_allsum<BS>(arz, lds);
     const double rr = block_allsum<BS>(arr, lds);
     const double bb = it == 0 ? rr : sc[S_BB];
@@ -131,17 +115,33 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
         }
         return;
     }
-    const double beta = withP ? rz / sc[S_RZ0 + ((it - 1) & 1)] : 0.0;
+    const bool withP = it > 0;
+    const V beta = withP ? (V)(rz / sc[S_RZ0 + ((it - 1) & 1)]) : (V)0;
+
+    constexpr int RPG = BS / LANES;   // rows per group
+    const int xb = blockIdx.x % kXcd, jb = blockIdx.x / kXcd, nj = gridDim.x / kXcd;
+    const int lane = threadIdx.x % LANES;
     double pq = 0;
-    if (row < nrow && lane == 0) {
+    for (int g = jb; g * RPG < rowsPerXcd; g += nj) {
+        const int row = xb * rowsPerXcd + g * RPG + (int)threadIdx.x / LANES;
+        V s[3] = {0, 0, 0};
+        if (row < nrow) {
+            bsr_row<V, LANES>(erp, ecol, ev, enz, row, lane, z, pold, beta, withP, s);
+            if (cnz > 0) bsr_row<V, LANES>(crp, ccol, cv, cnz, row, lane, z, pold, beta, withP, s);
+        }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double zk = z[3 * row + k];
-            const double pk = withP ? zk + beta * pold[3 * row + k] : zk;
-            const double qk = withP ? az[k] + beta * ap[k] : az[k];
-            pnew[3 * row + k] = pk;
-            q[3 * row + k] = qk;
-            pq += pk * qk;
+        for (int o = LANES / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s[k] += __shfl_xor(s[k], o, LANES);
+        if (row < nrow && lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                V pk = z[3 * row + k];
+                if (withP) pk += beta * pold[3 * row + k];
+                pnew[3 * row + k] = pk;
+                q[3 * row + k] = s[k];
+                pq += (double)pk * (double)s[k];
+            }
         }
     }
     const double bpq = block_allsum<BS>(pq, lds);
@@ -154,19 +154,18 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
 
 // Iteration `it`: alpha = r.z / p.q, x += alpha p, r -= alpha q, z = Dinv r, partials
 // of r.z and r.r (written as partU[block], partU[nU + block]).
-template <int BS>
-__global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPerXcd, double* __restrict__ x, double* __restrict__ r,
-                                                   double* __restrict__ z, const double* __restrict__ p,
-                                                   const double* __restrict__ q, const double* __restrict__ dinv,
-                                                   const double* __restrict__ partS, int nS, double* __restrict__ partU,
-                                                   double* __restrict__ sc, int* __restrict__ flags) {
+template <class V, int BS>
+__global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPerXcd, V* __restrict__ x, V* __restrict__ r,
+                                                   V* __restrict__ z, const V* __restrict__ p, const V* __restrict__ q,
+                                                   const V* __restrict__ dinv, const double* __restrict__ partS, int nS,
+                                                   double* __restrict__ partU, double* __restrict__ sc, int* __restrict__ flags) {
     if (flags[F_DONE]) return;
     __shared__ double lds[BS / 64];
     double a = 0;
     for (int i = threadIdx.x; i < nS; i += BS) a += partS[i];
-    const int row = xcd_row_base(rowsPerXcd, BS) + (int)threadIdx.x;
+    const int row = (int)(blockIdx.x % kXcd) * rowsPerXcd + (int)(blockIdx.x / kXcd) * BS + (int)threadIdx.x;
     const bool valid = row < nrow;
-    double pv[3] = {0, 0, 0}, qv[3] = {0, 0, 0}, xv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, D[9];
+    V pv[3] = {0, 0, 0}, qv[3] = {0, 0, 0}, xv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, D[9];
     if (valid) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -185,7 +184,7 @@ __global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPer
         }
         return;
     }
-    const double alpha = sc[S_RZ0 + (it & 1)] / pq;
+    const V alpha = (V)(sc[S_RZ0 + (it & 1)] / pq);
     double rz = 0, rr = 0;
     if (valid) {
 #pragma unroll
@@ -196,10 +195,10 @@ __global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPer
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double zk = D[3 * k] * rv[0] + D[3 * k + 1] * rv[1] + D[3 * k + 2] * rv[2];
+            const V zk = D[3 * k] * rv[0] + D[3 * k + 1] * rv[1] + D[3 * k + 2] * rv[2];
             z[3 * row + k] = zk;
-            rz += rv[k] * zk;
-            rr += rv[k] * rv[k];
+            rz += (double)rv[k] * (double)zk;
+            rr += (double)rv[k] * (double)rv[k];
         }
     }
     const double brz = block_allsum<BS>(rz, lds);
@@ -212,27 +211,27 @@ __global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPer
 
 // r = b, x = 0, z = Dinv r; partials of r.z and r.r in the update kernel's layout;
 // the solver flags are cleared.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_pcg_init(int nrow, int rowsPerXcd, const double* __restrict__ b, double* __restrict__ x,
-                                                 double* __restrict__ r, double* __restrict__ z, const double* __restrict__ dinv,
-                                                 double* __restrict__ partU, int* __restrict__ flags) {
+template <class V, int BS>
+__global__ __launch_bounds__(BS) void k_pcg_init(int nrow, int rowsPerXcd, const V* __restrict__ b, V* __restrict__ x, V* __restrict__ r,
+                                                 V* __restrict__ z, const V* __restrict__ dinv, double* __restrict__ partU,
+                                                 int* __restrict__ flags) {
     __shared__ double lds[BS / 64];
-    const int row = xcd_row_base(rowsPerXcd, BS) + (int)threadIdx.x;
+    const int row = (int)(blockIdx.x % kXcd) * rowsPerXcd + (int)(blockIdx.x / kXcd) * BS + (int)threadIdx.x;
     double rz = 0, rr = 0;
     if (row < nrow) {
-        double rv[3];
+        V rv[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             rv[k] = b[3 * row + k];
             r[3 * row + k] = rv[k];
             x[3 * row + k] = 0;
-            rr += rv[k] * rv[k];
+            rr += (double)rv[k] * (double)rv[k];
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double zk = dinv[9l * row + 3 * k] * rv[0] + dinv[9l * row + 3 * k + 1] * rv[1] + dinv[9l * row + 3 * k + 2] * rv[2];
+            const V zk = dinv[9l * row + 3 * k] * rv[0] + dinv[9l * row + 3 * k + 1] * rv[1] + dinv[9l * row + 3 * k + 2] * rv[2];
             z[3 * row + k] = zk;
-            rz += rv[k] * zk;
+            rz += (double)rv[k] * (double)zk;
         }
     }
     const double brz = block_allsum<BS>(rz, lds);
@@ -248,10 +247,11 @@ __global__ __launch_bounds__(BS) void k_pcg_init(int nrow, int rowsPerXcd, const
     }
 }
 
-// diagonal 3x3 blocks of A_el + A_con, inverted (symmetric adjugate)
+// diagonal 3x3 blocks of A_el + A_con (FP64), inverted (symmetric adjugate), stored as V
+template <class V>
 __global__ void k_dinv(int nV, const int* __restrict__ rpe, const int* __restrict__ ce, const double* __restrict__ ve, int nze,
                        const int* __restrict__ rpc, const int* __restrict__ cc, const double* __restrict__ vc, int nzc,
-                       double* __restrict__ dinv) {
+                       V* __restrict__ dinv) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nV) return;
     double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -280,33 +280,66 @@ __global__ void k_dinv(int nV, const int* __restrict__ rpe, const int* __restric
         r[0] = a[0] > 0 ? 1.0 / a[0] : 1.0; r[4] = a[4] > 0 ? 1.0 / a[4] : 1.0; r[8] = a[8] > 0 ? 1.0 / a[8] : 1.0;
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) dinv[9 * v + k] = r[k];
+    for (int k = 0; k < 9; ++k) dinv[9l * v + k] = (V)r[k];
 }
 
-// returns 0 converged, 1 not SPD (p^T A p <= 0), 2 iteration limit
-int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
+// ---- mixed-precision helpers
+__global__ void k_f64_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_acc_f32(long n, const float* __restrict__ d, double* __restrict__ x) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += (double)d[i];
+}
+// r = b - (A_el + A_con) x in FP64, rows of LANES lanes; per-workgroup partial of r.r
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void k_residual(const int* __restrict__ erp, const int* __restrict__ ecol, const double* __restrict__ ev,
+                                                     int enz, const int* __restrict__ crp, const int* __restrict__ ccol,
+                                                     const double* __restrict__ cv, int cnz, int nrow, const double* __restrict__ b,
+                                                     const double* __restrict__ x, double* __restrict__ r, double* __restrict__ part) {
+    __shared__ double lds[kBlock / 64];
+    const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int row = (int)(gid / LANES), lane = (int)(gid % LANES);
+    double s[3] = {0, 0, 0};
+    if (row < nrow) {
+        bsr_row<double, LANES>(erp, ecol, ev, enz, row, lane, x, x, 0.0, false, s);
+        if (cnz > 0) bsr_row<double, LANES>(crp, ccol, cv, cnz, row, lane, x, x, 0.0, false, s);
+    }
+#pragma unroll
+    for (int o = LANES / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s[k] += __shfl_xor(s[k], o, LANES);
+    double rr = 0;
+    if (row < nrow && lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double rk = b[3 * row + k] - s[k];
+            r[3 * row + k] = rk;
+            rr += rk * rk;
+        }
+    }
+    const double br = block_allsum<kBlock>(rr, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = br;
+}
+
+// ---- one preconditioned CG solve in precision V; returns 0 converged, 1 not SPD
+// (p^T A p <= 0), 2 iteration limit
+template <class V>
+static int pcg_run(Ctx& c, const V* valE, const V* valK, const V* b, V* xsol, V* r, V* z, V* pbuf, V* q, const V* dinv, double tol, int maxIter,
+                   int* iters, double* relres) {
     const int nV = c.nV, n = 3 * nV;
     BSR& E = c.Ael;
     BSR& K = c.Acon;
-    if (K.nnzb == 0) { K.rowptr.ensure(nV + 1); }
-    double* r = c.pr.ensure(n);
-    double* z = c.pz.ensure(n);
-    double* pbuf = c.pp.ensure(2l * n);       // p_old / p_new, alternating by iteration
-    double* q = c.pq.ensure(n);
-    double* dinv = c.pdinv.ensure(9l * nV);
     double* sc = c.scal.p + 32;               // solver scalars
     int* flags = (int*)(c.counters.p + 32);   // solver flags
-
-    constexpr int RPB = kSpmvBlock / kSpmvLanes;
-    const int rpxS = rows_per_xcd(nV, RPB), nS = kXcd * (rpxS / RPB);
+    constexpr int RPG = kSpmvBlock / kSpmvLanes;
+    const int rpxS = rows_per_xcd(nV, RPG), nS = kSpmvGrid;
     const int rpxU = rows_per_xcd(nV, kUpdBlock), nU = kXcd * (rpxU / kUpdBlock);
     double* partS = c.dscr.ensure((size_t)nS + 2l * nU);
     double* partU = partS + nS;
 
-    hipLaunchKernelGGL(k_dinv, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p,
-                       K.val.p, K.nnzb, dinv);
-    CIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_pcg_init<kUpdBlock>, dim3(nU), dim3(kUpdBlock), 0, c.s, nV, rpxU, b, xsol, r, z, dinv, partU, flags);
+    hipLaunchKernelGGL((k_pcg_init<V, kUpdBlock>), dim3(nU), dim3(kUpdBlock), 0, c.s, nV, rpxU, b, xsol, r, z, dinv, partU, flags);
     CIPC_LAUNCH_CHECK();
 
     const int chunk = 16;
@@ -319,23 +352,24 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
         }
     }
     const double tol2 = tol * tol;
-    const double spmvBytes = 76.0 * E.nnzb + 4.0 * (nV + 1) + (K.nnzb > 0 ? 76.0 * K.nnzb + 4.0 * (nV + 1) : 0.0);
+    const double vb = (double)sizeof(V);
+    const double spmvBytes = (9 * vb + 4) * ((double)E.nnzb + K.nnzb) + 4.0 * (nV + 1) * (K.nnzb > 0 ? 2 : 1) + 4 * 3 * vb * nV;
     int it = 0;
     for (;;) {
         int launched = 0;
         for (int k = 0; k < chunk && it <= maxIter; ++k, ++it, ++launched) {
-            const double* pold = pbuf + (size_t)((it + 1) & 1) * n;
-            double* pnew = pbuf + (size_t)(it & 1) * n;
+            const V* pold = pbuf + (size_t)((it + 1) & 1) * n;
+            V* pnew = pbuf + (size_t)(it & 1) * n;
             if (c.profiling)
-                hipExtLaunchKernelGGL(k_pcg_spmv<kSpmvLanes, kSpmvBlock>, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.evPool[2 * k],
-                                      c.evPool[2 * k + 1], 0, it, maxIter, tol2, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p,
-                                      K.val.p, K.nnzb, nV, rpxS, z, pold, pnew, q, partU, nU, partS, sc, flags);
+                hipExtLaunchKernelGGL(k_pcg_spmv<V, kSpmvLanes, kSpmvBlock>, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.evPool[2 * k],
+                                      c.evPool[2 * k + 1], 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb, K.rowptr.p, K.col.p,
+                                      valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, (const double*)partU, nU, partS, sc, flags);
             else
-                hipLaunchKernelGGL((k_pcg_spmv<kSpmvLanes, kSpmvBlock>), dim3(nS), dim3(kSpmvBlock), 0, c.s, it, maxIter, tol2, E.rowptr.p,
-                                   E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p, K.val.p, K.nnzb, nV, rpxS, z, pold, pnew, q, partU, nU,
-                                   partS, sc, flags);
-            hipLaunchKernelGGL(k_pcg_update<kUpdBlock>, dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z, (const double*)pnew, q,
-                               dinv, partS, nS, partU, sc, flags);
+                hipLaunchKernelGGL((k_pcg_spmv<V, kSpmvLanes, kSpmvBlock>), dim3(nS), dim3(kSpmvBlock), 0, c.s, it, maxIter, tol2, E.rowptr.p,
+                                   E.col.p, valE, E.nnzb, K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q,
+                                   (const double*)partU, nU, partS, sc, flags);
+            hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z, (const V*)pnew,
+                               (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
         }
         CIPC_LAUNCH_CHECK();
         CIPC_HIP(hipMemcpyAsync(hf, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, c.s));
@@ -350,8 +384,8 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
                 CIPC_HIP(hipEventElapsedTime(&ms, c.evPool[2 * k], c.evPool[2 * k + 1]));
                 c.profMs += ms;
                 c.profLaunches++;
-                c.profBytes += spmvBytes + 96.0 * nV;
-                c.profFlops += 18.0 * (E.nnzb + K.nnzb) + 8.0 * n;
+                c.profBytes += spmvBytes;
+                c.profFlops += 18.0 * ((double)E.nnzb + K.nnzb) + 8.0 * n;
             }
         }
         if (hf[F_DONE] || it > maxIter) break;
@@ -364,6 +398,83 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     if (hf[F_FAIL]) return 1;
     if (!hf[F_DONE] || rr > tol2 * bb) return 2;
     return 0;
+}
+
+// ||b - H x||^2 in FP64 (r written)
+static double residual_norm2(Ctx& c, const double* b, const double* x, double* r) {
+    BSR& E = c.Ael;
+    BSR& K = c.Acon;
+    const int nV = c.nV;
+    const int nb = nblocks(4l * nV);
+    double* part = c.dscr2.ensure((size_t)nb + 1);
+    hipLaunchKernelGGL(k_residual<4>, dim3(nb), dim3(kBlock), 0, c.s, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p, K.val.p, K.nnzb,
+                       nV, b, x, r, part);
+    CIPC_LAUNCH_CHECK();
+    sum_partials(part, nb, c.scal.p + 40, c.s);
+    CIPC_HIP(hipMemcpyAsync(c.hScal + 40, c.scal.p + 40, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    return c.hScal[40];
+}
+
+int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
+    const int nV = c.nV;
+    const long n = 3l * nV;
+    BSR& E = c.Ael;
+    BSR& K = c.Acon;
+    if (K.nnzb == 0) { K.rowptr.ensure(nV + 1); }
+    c.lastRefinements = 0;
+    if (c.P.precision == 0) {
+        double* dinv = c.pdinv.ensure(9l * nV);
+        hipLaunchKernelGGL(k_dinv<double>, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p,
+                           K.col.p, K.val.p, K.nnzb, dinv);
+        CIPC_LAUNCH_CHECK();
+        return pcg_run<double>(c, E.val.p, K.val.p, b, xsol, c.pr.ensure(n), c.pz.ensure(n), c.pp.ensure(2 * n), c.pq.ensure(n), dinv, tol,
+                               maxIter, iters, relres);
+    }
+    // mixed: FP32 copies of the blocks, FP32 Jacobi blocks from the FP64 diagonal
+    float* vE = c.valE32.ensure(9l * E.nnzb + 1);
+    float* vK = c.valK32.ensure(9l * K.nnzb + 1);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(nblocks(9l * E.nnzb)), dim3(kBlock), 0, c.s, 9l * E.nnzb, E.val.p, vE);
+    if (K.nnzb > 0) hipLaunchKernelGGL(k_f64_to_f32, dim3(nblocks(9l * K.nnzb)), dim3(kBlock), 0, c.s, 9l * K.nnzb, K.val.p, vK);
+    float* dinv = c.fdinv.ensure(9l * nV);
+    hipLaunchKernelGGL(k_dinv<float>, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p,
+                       K.val.p, K.nnzb, dinv);
+    CIPC_LAUNCH_CHECK();
+    float* fb = c.fb.ensure(n);
+    float* fx = c.fx.ensure(n);
+    double* r = c.rres.ensure(n);
+    CIPC_HIP(hipMemsetAsync(xsol, 0, n * sizeof(double), c.s));
+    const double bnorm = std::sqrt(residual_norm2(c, b, xsol, r));
+    if (relres) *relres = 0;
+    if (iters) *iters = 0;
+    if (bnorm == 0) return 0;
+    double rnorm = bnorm;
+    int total = 0;
+    constexpr int kMaxRefine = 8;
+    constexpr double kInnerFloor = 1e-6;   // FP32 recurrences stall near this relative residual
+    for (int k = 0; k < kMaxRefine; ++k) {
+        const double inner = std::max(kInnerFloor, std::min(0.5, 0.9 * tol * bnorm / rnorm));
+        hipLaunchKernelGGL(k_f64_to_f32, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const double*)r, fb);
+        CIPC_LAUNCH_CHECK();
+        int it = 0;
+        double rr = 0;
+        int rc = pcg_run<float>(c, vE, vK, fb, fx, c.fr.ensure(n), c.fz.ensure(n), c.fp.ensure(2 * n), c.fq.ensure(n), dinv, inner,
+                                std::max(1, maxIter - total), &it, &rr);
+        total += it;
+        c.lastRefinements = k + 1;
+        if (rc == 1) {
+            if (iters) *iters = total;
+            return 1;
+        }
+        hipLaunchKernelGGL(k_acc_f32, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const float*)fx, xsol);
+        CIPC_LAUNCH_CHECK();
+        rnorm = std::sqrt(residual_norm2(c, b, xsol, r));
+        if (relres) *relres = rnorm / bnorm;
+        if (iters) *iters = total;
+        if (rnorm <= tol * bnorm) return 0;
+        if (total >= maxIter) return 2;
+    }
+    return 2;
 }
 
 }  // namespace cipc

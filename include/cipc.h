@@ -114,6 +114,16 @@ int cipc_ccd_stepsize(cipc_ctx* ctx, const double* x, const double* p, double al
 int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, const double* b,
                       double* sol, double tol, int32_t* iters, double* relres);
 
+/* ---- solver precision (no counterpart in the reference, which factorises in FP64
+ * with CHOLMOD, DIRECT_SOLVER.h:14-66) -----------------------------------------
+ * mode 0: FP64 block-Jacobi PCG.  mode 1: mixed -- FP32 Hessian blocks, FP32 Krylov
+ * vectors and FP32 SpMV accumulation inside an FP64 iterative refinement that stops
+ * when the FP64 residual ||b - H x|| <= pcg_tol ||b||.  Affects cipc_step and
+ * cipc_linear_solve; out[0] of cipc_get_solver_info = refinement sweeps of the last
+ * solve (0 in mode 0). */
+int cipc_set_precision(cipc_ctx* ctx, int32_t mode);
+int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out);
+
 /* ---- instrumentation (no counterpart in the reference) ------------------ */
 /* Kernel timing of the dominant kernel of the step, the PCG SpMV (q = A p with
  * p = z + beta p_old formed on the fly), from the dispatch's own start/end

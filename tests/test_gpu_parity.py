@@ -160,11 +160,35 @@ def test_linear_solve_residual(stack):
     assert rel(sol, ref) <= 1e-5
 
 
-def test_step_trajectory(require_gpu):
-    """Five steps of the contact stack, tight Newton tolerance, FP64 on both sides."""
+def test_linear_solve_mixed_refinement(stack):
+    """FP32 PCG inside FP64 iterative refinement: the FP64 residual of the oracle's own
+    assembled matrix meets the requested 1e-8, several sweeps below the FP32 floor."""
+    sd, gpu, orc, states = stack
+    kappa = orc.scalars()["kappa"]
+    x = states[0]
+    C = orc.constraint_set(x)
+    H = orc.hessian(x, C, kappa, which=3)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(x.size)
+    b.reshape(-1, 3)[sd.dbc_idx] = 0
+    gpu.set_precision("mixed")
+    try:
+        sol, it, rr = gpu.linear_solve(x, C, kappa, b, tol=1e-8)
+        sweeps = gpu.solver_info()["refinements"]
+    finally:
+        gpu.set_precision("fp64")
+    assert rr <= 1e-8
+    assert np.linalg.norm(H @ sol - b) <= 2e-8 * np.linalg.norm(b)
+    assert sweeps >= 2
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_step_trajectory(require_gpu, precision):
+    """Five steps of the contact stack, tight Newton tolerance; FP64 oracle.  The mixed
+    solver refines to the same 1e-10 linear residual, so the same bar applies."""
     sd = scenes.scene_contact_stack()
     sd.params["PNTol"] = 1e-6 * sd.bbox_diag()
-    gpu = Simulator(sd, pcg_tol=1e-10)
+    gpu = Simulator(sd, pcg_tol=1e-10, precision=precision)
     orc = O.OracleScene(sd, threads=4)
     for s in range(5):
         st = gpu.step()
