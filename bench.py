@@ -7,12 +7,13 @@ Newton linear solve to relative residual 1e-5.  A "step" is one call of the impl
 step (the reference's Advance_One_Step_IE_Hinge).  Inputs are generated on the host and
 uploaded before the timed region; positions stay resident in HBM.
 
-Two solver forms are timed on the same scene, each from the initial state:
+Three solver forms are timed on the same scene, each from the initial state:
   fp64   block-Jacobi PCG in FP64;
-  mixed  FP32 Hessian blocks / Krylov vectors / SpMV inside FP64 iterative refinement,
-         stopped on the FP64 residual (same 1e-5 relative bar).
-`value` is the steps/s of the headline form (mixed when both run); the other form's
-figures are reported beside it.
+  mixed  FP32 Hessian blocks, FP64 Krylov vectors / SpMV accumulation;
+  fp32   FP32 Hessian blocks, Krylov vectors and SpMV accumulation;
+the last two inside FP64 iterative refinement stopped on the FP64 residual (same 1e-5
+relative bar).  `value` is the steps/s of the faster reduced-precision form (named in
+`mode`); every form's figures are reported under `forms`.
 
 The JSON line carries the roofline of the dominant kernel of the headline form (the PCG
 SpMV, timed from its own dispatch timestamps on the library's stream during the timed
@@ -30,8 +31,12 @@ sys.path.insert(0, os.path.join(ROOT, "codim-ipc_amd"))
 
 METRIC = "sim steps/s + ms/Newton iter, 320k-tri cloth; plain FP64 vs mixed+learned PCG"
 HBM_PEAK_GBS = 8000.0
-KERNEL = {"fp64": "k_pcg_spmv<double,16,256> (PCG SpMV, FP64 blocks)",
-          "mixed": "k_pcg_spmv<float,16,256> (PCG SpMV, FP32 blocks)"}
+KERNEL = {"fp64": "k_pcg_spmv<double,double,16,256> (PCG SpMV, FP64 blocks, FP64 vectors)",
+          "mixed": "k_pcg_spmv<float,double,16,256> (PCG SpMV, FP32 blocks, FP64 vectors)",
+          "fp32": "k_pcg_spmv<float,float,16,256> (PCG SpMV, FP32 blocks, FP32 vectors)"}
+MODE = {"fp64": "plain FP64, block-Jacobi PCG",
+        "mixed": "FP32 Hessian blocks, FP64 vectors, FP64 iterative refinement, block-Jacobi PCG",
+        "fp32": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, block-Jacobi PCG"}
 
 
 def gpu_run(args, sd, precision):
@@ -63,7 +68,8 @@ def gpu_run(args, sd, precision):
     return dict(precision=precision, steps_per_s=args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps,
                 ms_per_newton_iter=1e3 * elapsed / max(newton, 1), newton_iters_per_step=newton / args.steps,
                 pcg_iters_per_newton=pcg / max(newton, 1), contacts_last_step=stats[-1]["contacts"],
-                statuses=sorted(set(s["status"] for s in stats)), min_dist_final=min_d, roofline=roof)
+                statuses=sorted(set(s["status"] for s in stats)), min_dist_final=min_d,
+                spmv_avg_us=roof["avg_us"] if roof else None, spmv_frac=roof["frac"] if roof else None, roofline=roof)
 
 
 def cpu_baseline(args, sd):
@@ -90,7 +96,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
-    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--precision", default="all", choices=["fp64", "mixed", "fp32", "all"])
     ap.add_argument("--pcg-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -103,9 +109,10 @@ def main():
     from cipc import scenes
     sd = scenes.scene_c2() if args.workload == "c2" else scenes.scene_c1()
 
-    modes = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    modes = ["fp64", "mixed", "fp32"] if args.precision == "all" else [args.precision]
     runs = {m: gpu_run(args, sd, m) for m in modes}
-    head = runs[modes[-1]]
+    reduced = [m for m in modes if m != "fp64"] or modes
+    head = runs[max(reduced, key=lambda m: runs[m]["steps_per_s"])]
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, sd)
     out = {
         "metric": METRIC,
@@ -121,10 +128,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head["precision"] == "mixed" else "f64",
+        "dtype": {"fp64": "f64", "mixed": "f32 blocks + f64", "fp32": "f32 + f64 refinement"}[head["precision"]],
         "data": "synthetic",
-        "mode": {"fp64": "plain FP64, block-Jacobi PCG",
-                 "mixed": "FP32 blocks/vectors/SpMV in FP64 iterative refinement, block-Jacobi PCG"}[head["precision"]],
+        "mode": MODE[head["precision"]],
         "config": {"workload": sd.name, "vertices": sd.nV, "triangles": sd.nT, "cloth_vertices": sd.n_cloth,
                    "steps_simulated_per_form": args.warmup + args.steps, "dt": sd.params["dt"], "dhat": sd.params["dHat"],
                    "pcg_tol": args.pcg_tol},

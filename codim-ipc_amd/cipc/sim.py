@@ -26,13 +26,14 @@ def _i(a):
 
 
 class Simulator:
-    PRECISION = {"fp64": 0, "mixed": 1}
+    PRECISION = {"fp64": 0, "mixed": 1, "fp32": 2}
 
     def __init__(self, sd, pcg_tol=1e-5, pcg_max_iter=100000, precision="fp64"):
         """Initialize_Shell_Hinge_EIPC + Initialize_OIPC (DISCRETE_SHELL.h:218-357, 555-577).
 
-        precision: "fp64" (FP64 block-Jacobi PCG) or "mixed" (FP32 blocks/vectors/SpMV
-        inside FP64 iterative refinement on the true residual)."""
+        precision: "fp64" (FP64 block-Jacobi PCG), "mixed" (FP32 Hessian blocks, FP64
+        vectors) or "fp32" (FP32 blocks, vectors and SpMV accumulation); the last two run
+        inside FP64 iterative refinement on the true residual."""
         self.lib = _lib.load()
         P = sd.params
         self._keep = [sd.X, sd.tris, sd.dbc_idx, sd.dbc_target]

@@ -332,8 +332,8 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
 
 int cipc_set_precision(cipc_ctx* ctx, int32_t mode) {
     if (!ctx) return CIPC_E_INVALID;
-    if (mode != 0 && mode != 1) {
-        ctx->c.lastError = "precision mode must be 0 (FP64) or 1 (mixed)";
+    if (mode < 0 || mode > 2) {
+        ctx->c.lastError = "precision mode must be 0 (fp64), 1 (mixed) or 2 (fp32)";
         return CIPC_E_INVALID;
     }
     ctx->c.P.precision = mode;

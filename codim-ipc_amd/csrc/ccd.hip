@@ -165,17 +165,6 @@ __global__ void k_swept_partial(const double* __restrict__ x, const double* __re
     if (t == 0)
         for (int d = 0; d < 3; ++d) { part[6 * blockIdx.x + d] = lo[d][0]; part[6 * blockIdx.x + 3 + d] = hi[d][0]; }
 }
-__global__ void k_minmax_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
-    if (threadIdx.x != 0) return;
-    double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
-    for (int b = 0; b < nb; ++b)
-        for (int d = 0; d < 3; ++d) {
-            l[d] = part[6 * b + d] < l[d] ? part[6 * b + d] : l[d];
-            h[d] = part[6 * b + 3 + d] > h[d] ? part[6 * b + 3 + d] : h[d];
-        }
-    for (int d = 0; d < 3; ++d) { out[d] = l[d]; out[3 + d] = h[d]; }
-}
-
 struct VoxGrid { double lb[3]; double inv; double halfXi; double alpha; int f; int edim[3]; };
 
 // per boundary node: voxel index range of its swept, offset-padded box (SPATIAL_HASH.h:546-553)
@@ -446,8 +435,7 @@ double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, doub
         double* part = c.dscr2.ensure(6l * nb);
         hipLaunchKernelGGL(k_swept_partial, dim3(nb), dim3(kBlock), 0, c.s, x, p, alpha, c.bNode.p, c.nBN, part);
         CIPC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, c.s, part, nb, c.scal.p + 10);
-        CIPC_LAUNCH_CHECK();
+        minmax6_final(part, nb, c.scal.p + 10, c.s);
         CIPC_HIP(hipMemcpyAsync(c.hScal + 10, c.scal.p + 10, 6 * sizeof(double), hipMemcpyDeviceToHost, c.s));
         CIPC_HIP(hipStreamSynchronize(c.s));
         double lb[3], range[3];

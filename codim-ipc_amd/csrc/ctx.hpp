@@ -19,7 +19,7 @@ struct Params {
     double pcgTol = 1e-5;
     int pcgMaxIter = 100000;
     int collision = 1;
-    int precision = 0;            // 0: FP64 PCG; 1: FP32 PCG inside FP64 iterative refinement
+    int precision = 0;            // 0: FP64 PCG; 1: FP32 blocks + FP64 vectors; 2: all-FP32 PCG (1, 2 inside FP64 refinement)
 };
 
 // spatial grid used by the constraint-set broad phase and the CCD candidate search
@@ -104,7 +104,7 @@ struct Ctx {
 
     // PCG
     DevBuf<double> pr, pz, pp, pq, pdinv, psol;
-    // mixed precision: FP32 copies of the matrix values, FP32 Krylov vectors, FP64 residual
+    // mixed forms: FP32 copies of the matrix values, FP32 Krylov vectors (fp32 form), FP64 residual
     DevBuf<float> valE32, valK32, fr, fz, fp, fq, fdinv, fx, fb;
     DevBuf<double> rres, xacc;
     int lastRefinements = 0;
@@ -127,6 +127,7 @@ struct Ctx {
 // ---- utilities (util.hip)
 void fill_iota(uint32_t* a, long n, hipStream_t s);
 void bbox_minmax(Ctx& c, const double* x, double* out6);   // min xyz, max xyz into device memory
+void minmax6_final(const double* part, int nb, double* out6, hipStream_t s);   // reduce (lo, hi) records
 
 // ---- stages (implemented in the .hip files)
 void ctx_init_device(Ctx& c);                                      // upload + derived data

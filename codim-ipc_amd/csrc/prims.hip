@@ -199,12 +199,22 @@ __global__ void k_chunk_sum(const double* __restrict__ in, long n, double* __res
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = a[0];
 }
+// Sequential sum in index order (the order the deterministic reductions define); the
+// partials are staged through LDS by the whole workgroup so the single summing thread
+// reads LDS instead of waiting on one global load per term.
 __global__ void k_serial_sum(const double* __restrict__ partial, int n, double* __restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double t = 0;
-        for (int i = 0; i < n; ++i) t += partial[i];
-        *out = t;
+    constexpr int kTile = 4096;
+    __shared__ double buf[kTile];
+    double t = 0;
+    for (int base = 0; base < n; base += kTile) {
+        const int m = n - base < kTile ? n - base : kTile;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) buf[i] = partial[base + i];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int i = 0; i < m; ++i) t += buf[i];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *out = t;
 }
 void chunk_sum(const double* in, long n, double* out, PrimWork& w, hipStream_t s) {
     int nb = n > 0 ? nblocks(n) : 0;
@@ -213,11 +223,11 @@ void chunk_sum(const double* in, long n, double* out, PrimWork& w, hipStream_t s
         hipLaunchKernelGGL(k_chunk_sum, dim3(nb), dim3(kBlock), 0, s, in, n, part);
         CIPC_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(64), 0, s, part, nb, out);
+    hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(kBlock), 0, s, part, nb, out);
     CIPC_LAUNCH_CHECK();
 }
 void sum_partials(const double* partials, int n, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(64), 0, s, partials, n, out);
+    hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(kBlock), 0, s, partials, n, out);
     CIPC_LAUNCH_CHECK();
 }
 

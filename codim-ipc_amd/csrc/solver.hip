@@ -5,12 +5,15 @@
 // rule ||r|| <= tol ||b|| follows the reference's iterative-solver precedent
 // (AMGCL_SOLVER.h:53-65, rel-tol 1e-5).
 //
-// Two precisions share one set of kernels (template on the value type V):
-//   FP64      V = double throughout;
-//   mixed     matrix blocks, Krylov vectors and SpMV accumulation in FP32, dot
-//             products summed in FP64; an outer FP64 iterative refinement
-//             (x += d, r = b - H x with the FP64 blocks) repeats the FP32 solve on the
-//             FP64 residual until ||r|| <= tol ||b|| holds in FP64.
+// Three forms share one set of kernels (templates on the matrix value type M and the
+// vector type V):
+//   fp64   M = V = double;
+//   mixed  M = float (FP32 copies of the Hessian blocks, half the SpMV bytes), V = double
+//          (Krylov vectors, SpMV accumulation), inside FP64 iterative refinement;
+//   fp32   M = V = float (blocks, vectors and SpMV accumulation in FP32, dot products
+//          summed in FP64), inside FP64 iterative refinement.
+// The refinement (x += d, r = b - H x with the FP64 blocks) repeats the inner solve on
+// the FP64 residual until ||r|| <= tol ||b|| holds in FP64.
 //
 // One CG iteration is two kernels and no atomics:
 //   k_pcg_spmv   beta from the previous update's partial sums, p = z + beta p_old formed
@@ -66,21 +69,28 @@ inline int rows_per_xcd(int nrow, int rowsPerGroup) {
     return (r + rowsPerGroup - 1) / rowsPerGroup * rowsPerGroup;
 }
 
-template <class V, int LANES>
-__device__ __forceinline__ void bsr_row(const int* __restrict__ rp, const int* __restrict__ col, const V* __restrict__ val, long nnzb,
-                                        int row, int lane, const V* __restrict__ z, const V* __restrict__ p, V beta, bool withP,
-                                        V* s) {
-    const int e = rp[row + 1];
-    for (int b = rp[row] + lane; b < e; b += LANES) {
+// One block row's share of lane `lane`: s += A_row,c (z_c + beta p_c) over the row's
+// blocks [beg, end).  The lane that meets the diagonal block keeps z_row + beta p_row
+// in `own` (every row stores its diagonal block), so the row's new direction needs no
+// separate load.
+template <class M, class V, int LANES, bool WITHP>
+__device__ __forceinline__ void bsr_row(int beg, int end, const int* __restrict__ col, const M* __restrict__ val, long nnzb, int row,
+                                        const V* __restrict__ z, const V* __restrict__ p, V beta, V* s, V* own, bool& hasOwn) {
+    for (int b = beg; b < end; b += LANES) {
         const int c = col[b];
         V a[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) a[k] = val[k * nnzb + b];
+        for (int k = 0; k < 9; ++k) a[k] = (V)val[k * nnzb + b];
         V x0 = z[3 * c], x1 = z[3 * c + 1], x2 = z[3 * c + 2];
-        if (withP) {
-            x0 += beta * p[3 * c];
-            x1 += beta * p[3 * c + 1];
-            x2 += beta * p[3 * c + 2];
+        if (WITHP) {
+            const V p0 = p[3 * c], p1 = p[3 * c + 1], p2 = p[3 * c + 2];
+            x0 += beta * p0;
+            x1 += beta * p1;
+            x2 += beta * p2;
+        }
+        if (c == row) {
+            own[0] = x0; own[1] = x1; own[2] = x2;
+            hasOwn = true;
         }
         s[0] += a[0] * x0 + a[1] * x1 + a[2] * x2;
         s[1] += a[3] * x0 + a[4] * x1 + a[5] * x2;
@@ -89,18 +99,30 @@ __device__ __forceinline__ void bsr_row(const int* __restrict__ rp, const int* _
 }
 
 // Iteration `it`: convergence of the residual left by the previous update is decided
-// first; then p_new = z + beta p_old (p_new = z at it = 0), q = A p_new, partial p.q.
-template <class V, int LANES, int BS>
+// first; then p_new = z + beta p_old (p_new = z at it = 0, WITHP = false), q = A p_new,
+// partial p.q.  Row bounds of the next row group are loaded one group ahead.
+template <class M, class V, int LANES, int BS, bool WITHP>
 __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2, const int* __restrict__ erp,
-                                                 const int* __restrict__ ecol, const V* __restrict__ ev, int enz,
-                                                 const int* __restrict__ crp, const int* __restrict__ ccol, const V* __restrict__ cv,
+                                                 const int* __restrict__ ecol, const M* __restrict__ ev, int enz,
+                                                 const int* __restrict__ crp, const int* __restrict__ ccol, const M* __restrict__ cv,
                                                  int cnz, int nrow, int rowsPerXcd, const V* __restrict__ z,
                                                  const V* __restrict__ pold, V* __restrict__ pnew, V* __restrict__ q,
                                                  const double* __restrict__ partU, int nU, double* __restrict__ partS,
                                                  double* __restrict__ sc, int* __restrict__ flags) {
     if (flags[F_DONE]) return;
     __shared__ double lds[BS / 64];
+    constexpr int RPG = BS / LANES;   // rows per group
+    const int xb = blockIdx.x % kXcd, jb = blockIdx.x / kXcd, nj = gridDim.x / kXcd;
+    const int lane = threadIdx.x % LANES;
+    const int rowBase = xb * rowsPerXcd + (int)threadIdx.x / LANES;
+    // first group's row bounds, issued before the scalar reduction
+    int g = jb;
+    int row = rowBase + g * RPG;
+    bool live = g * RPG < rowsPerXcd && row < nrow;
+    int beg = live ? erp[row] : 0, end = live ? erp[row + 1] : 0;
+
     double arz = 0, arr = 0;
+#pragma unroll 4
     for (int i = threadIdx.x; i < nU; i += BS) { arz += partU[i]; arr += partU[nU + i]; }
     const double rz = block_allsum<BS>(arz, lds);
     const double rr = block_allsum<BS>(arr, lds);
@@ -115,34 +137,39 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
         }
         return;
     }
-    const bool withP = it > 0;
-    const V beta = withP ? (V)(rz / sc[S_RZ0 + ((it - 1) & 1)]) : (V)0;
+    const V beta = WITHP ? (V)(rz / sc[S_RZ0 + ((it - 1) & 1)]) : (V)0;
 
-    constexpr int RPG = BS / LANES;   // rows per group
-    const int xb = blockIdx.x % kXcd, jb = blockIdx.x / kXcd, nj = gridDim.x / kXcd;
-    const int lane = threadIdx.x % LANES;
     double pq = 0;
-    for (int g = jb; g * RPG < rowsPerXcd; g += nj) {
-        const int row = xb * rowsPerXcd + g * RPG + (int)threadIdx.x / LANES;
-        V s[3] = {0, 0, 0};
+    for (; g * RPG < rowsPerXcd; g += nj) {
+        const int gn = g + nj;
+        const int rowN = rowBase + gn * RPG;
+        const bool liveN = gn * RPG < rowsPerXcd && rowN < nrow;
+        const int begN = liveN ? erp[rowN] : 0, endN = liveN ? erp[rowN + 1] : 0;
+        V s[3] = {0, 0, 0}, own[3] = {0, 0, 0};
+        bool hasOwn = false;
         if (row < nrow) {
-            bsr_row<V, LANES>(erp, ecol, ev, enz, row, lane, z, pold, beta, withP, s);
-            if (cnz > 0) bsr_row<V, LANES>(crp, ccol, cv, cnz, row, lane, z, pold, beta, withP, s);
+            bsr_row<M, V, LANES, WITHP>(beg + lane, end, ecol, ev, enz, row, z, pold, beta, s, own, hasOwn);
+            if (cnz > 0) {
+                bool unused = false;
+                V dummy[3];
+                bsr_row<M, V, LANES, WITHP>(crp[row] + lane, crp[row + 1], ccol, cv, cnz, -1, z, pold, beta, s, dummy, unused);
+            }
         }
 #pragma unroll
         for (int o = LANES / 2; o > 0; o >>= 1)
 #pragma unroll
             for (int k = 0; k < 3; ++k) s[k] += __shfl_xor(s[k], o, LANES);
-        if (row < nrow && lane == 0) {
+        if (row < nrow && hasOwn) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                V pk = z[3 * row + k];
-                if (withP) pk += beta * pold[3 * row + k];
-                pnew[3 * row + k] = pk;
+                pnew[3 * row + k] = own[k];
                 q[3 * row + k] = s[k];
-                pq += (double)pk * (double)s[k];
+                pq += (double)own[k] * (double)s[k];
             }
         }
+        row = rowN;
+        beg = begN;
+        end = endN;
     }
     const double bpq = block_allsum<BS>(pq, lds);
     if (threadIdx.x == 0) partS[blockIdx.x] = bpq;
@@ -303,8 +330,10 @@ __global__ __launch_bounds__(kBlock) void k_residual(const int* __restrict__ erp
     const int row = (int)(gid / LANES), lane = (int)(gid % LANES);
     double s[3] = {0, 0, 0};
     if (row < nrow) {
-        bsr_row<double, LANES>(erp, ecol, ev, enz, row, lane, x, x, 0.0, false, s);
-        if (cnz > 0) bsr_row<double, LANES>(crp, ccol, cv, cnz, row, lane, x, x, 0.0, false, s);
+        double own[3];
+        bool hasOwn = false;
+        bsr_row<double, double, LANES, false>(erp[row] + lane, erp[row + 1], ecol, ev, enz, row, x, x, 0.0, s, own, hasOwn);
+        if (cnz > 0) bsr_row<double, double, LANES, false>(crp[row] + lane, crp[row + 1], ccol, cv, cnz, -1, x, x, 0.0, s, own, hasOwn);
     }
 #pragma unroll
     for (int o = LANES / 2; o > 0; o >>= 1)
@@ -325,8 +354,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(const int* __restrict__ erp
 
 // ---- one preconditioned CG solve in precision V; returns 0 converged, 1 not SPD
 // (p^T A p <= 0), 2 iteration limit
-template <class V>
-static int pcg_run(Ctx& c, const V* valE, const V* valK, const V* b, V* xsol, V* r, V* z, V* pbuf, V* q, const V* dinv, double tol, int maxIter,
+template <class M, class V>
+static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V* r, V* z, V* pbuf, V* q, const V* dinv, double tol, int maxIter,
                    int* iters, double* relres) {
     const int nV = c.nV, n = 3 * nV;
     BSR& E = c.Ael;
@@ -352,22 +381,19 @@ static int pcg_run(Ctx& c, const V* valE, const V* valK, const V* b, V* xsol, V*
         }
     }
     const double tol2 = tol * tol;
-    const double vb = (double)sizeof(V);
-    const double spmvBytes = (9 * vb + 4) * ((double)E.nnzb + K.nnzb) + 4.0 * (nV + 1) * (K.nnzb > 0 ? 2 : 1) + 4 * 3 * vb * nV;
+    const double mb = (double)sizeof(M), vb = (double)sizeof(V);
+    const double spmvBytes = (9 * mb + 4) * ((double)E.nnzb + K.nnzb) + 4.0 * (nV + 1) * (K.nnzb > 0 ? 2 : 1) + 4 * 3 * vb * nV;
     int it = 0;
     for (;;) {
         int launched = 0;
         for (int k = 0; k < chunk && it <= maxIter; ++k, ++it, ++launched) {
             const V* pold = pbuf + (size_t)((it + 1) & 1) * n;
             V* pnew = pbuf + (size_t)(it & 1) * n;
-            if (c.profiling)
-                hipExtLaunchKernelGGL(k_pcg_spmv<V, kSpmvLanes, kSpmvBlock>, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.evPool[2 * k],
-                                      c.evPool[2 * k + 1], 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb, K.rowptr.p, K.col.p,
-                                      valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, (const double*)partU, nU, partS, sc, flags);
-            else
-                hipLaunchKernelGGL((k_pcg_spmv<V, kSpmvLanes, kSpmvBlock>), dim3(nS), dim3(kSpmvBlock), 0, c.s, it, maxIter, tol2, E.rowptr.p,
-                                   E.col.p, valE, E.nnzb, K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q,
-                                   (const double*)partU, nU, partS, sc, flags);
+            auto kern = it > 0 ? k_pcg_spmv<M, V, kSpmvLanes, kSpmvBlock, true> : k_pcg_spmv<M, V, kSpmvLanes, kSpmvBlock, false>;
+            hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.profiling ? c.evPool[2 * k] : nullptr,
+                                  c.profiling ? c.evPool[2 * k + 1] : nullptr, 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
+                                  K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, (const double*)partU, nU,
+                                  partS, sc, flags);
             hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z, (const V*)pnew,
                                (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
         }
@@ -416,6 +442,67 @@ static double residual_norm2(Ctx& c, const double* b, const double* x, double* r
     return c.hScal[40];
 }
 
+__global__ void k_acc_f64(long n, const double* __restrict__ d, double* __restrict__ x) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
+// FP64 iterative refinement: x = 0, r = b; repeat { solve A_M d = r in precision V to a
+// relative tolerance chosen from the remaining gap; x += d; r = b - A x in FP64 } until
+// ||r|| <= tol ||b||.  rb is the precision-V copy of r (unused when V = double).
+template <class M, class V>
+static int refine(Ctx& c, const M* vE, const M* vK, const V* dinv, const double* b, double* xsol, double tol, int maxIter, int* iters,
+                  double* relres, V* wr, V* wz, V* wp, V* wq, V* d, V* rb) {
+    const long n = 3l * c.nV;
+    double* r = c.rres.ensure(n);
+    CIPC_HIP(hipMemsetAsync(xsol, 0, n * sizeof(double), c.s));
+    const double bnorm = std::sqrt(residual_norm2(c, b, xsol, r));
+    if (relres) *relres = 0;
+    if (iters) *iters = 0;
+    if (bnorm == 0) return 0;
+    double rnorm = bnorm;
+    int total = 0;
+    constexpr int kMaxRefine = 8;
+    // FP32 Krylov recurrences stall near 1e-6 relative; FP64 ones do not
+    const double floorTol = sizeof(V) == sizeof(float) ? 1e-6 : 1e-12;
+    // aim below the target: the inner solve's residual drifts from the true FP64 residual
+    // by the rounding of the FP32 blocks, and a restarted sweep costs far more iterations
+    // than overshooting.  FP64 Krylov vectors reach a tighter inner target cheaply; FP32
+    // ones slow down near their floor, so they aim just below the target.
+    const double kInnerMargin = sizeof(V) == sizeof(double) ? 0.3 : 0.9;
+    for (int k = 0; k < kMaxRefine; ++k) {
+        const double inner = std::max(floorTol, std::min(0.5, kInnerMargin * tol * bnorm / rnorm));
+        const V* rhs;
+        if constexpr (sizeof(V) == sizeof(double)) {
+            rhs = (const V*)r;
+        } else {
+            hipLaunchKernelGGL(k_f64_to_f32, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const double*)r, rb);
+            CIPC_LAUNCH_CHECK();
+            rhs = rb;
+        }
+        int it = 0;
+        double rr = 0;
+        int rc = pcg_run<M, V>(c, vE, vK, rhs, d, wr, wz, wp, wq, dinv, inner, std::max(1, maxIter - total), &it, &rr);
+        total += it;
+        c.lastRefinements = k + 1;
+        if (rc == 1) {
+            if (iters) *iters = total;
+            return 1;
+        }
+        if constexpr (sizeof(V) == sizeof(double))
+            hipLaunchKernelGGL(k_acc_f64, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const double*)d, xsol);
+        else
+            hipLaunchKernelGGL(k_acc_f32, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const float*)d, xsol);
+        CIPC_LAUNCH_CHECK();
+        rnorm = std::sqrt(residual_norm2(c, b, xsol, r));
+        if (relres) *relres = rnorm / bnorm;
+        if (iters) *iters = total;
+        if (rnorm <= tol * bnorm) return 0;
+        if (total >= maxIter) return 2;
+    }
+    return 2;
+}
+
 int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
     const int nV = c.nV;
     const long n = 3l * nV;
@@ -428,53 +515,29 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
         hipLaunchKernelGGL(k_dinv<double>, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p,
                            K.col.p, K.val.p, K.nnzb, dinv);
         CIPC_LAUNCH_CHECK();
-        return pcg_run<double>(c, E.val.p, K.val.p, b, xsol, c.pr.ensure(n), c.pz.ensure(n), c.pp.ensure(2 * n), c.pq.ensure(n), dinv, tol,
+        return pcg_run<double, double>(c, E.val.p, K.val.p, b, xsol, c.pr.ensure(n), c.pz.ensure(n), c.pp.ensure(2 * n), c.pq.ensure(n), dinv, tol,
                                maxIter, iters, relres);
     }
-    // mixed: FP32 copies of the blocks, FP32 Jacobi blocks from the FP64 diagonal
+    // mixed forms: FP32 copies of the blocks; FP64 iterative refinement around the solve
     float* vE = c.valE32.ensure(9l * E.nnzb + 1);
     float* vK = c.valK32.ensure(9l * K.nnzb + 1);
     hipLaunchKernelGGL(k_f64_to_f32, dim3(nblocks(9l * E.nnzb)), dim3(kBlock), 0, c.s, 9l * E.nnzb, E.val.p, vE);
     if (K.nnzb > 0) hipLaunchKernelGGL(k_f64_to_f32, dim3(nblocks(9l * K.nnzb)), dim3(kBlock), 0, c.s, 9l * K.nnzb, K.val.p, vK);
+    CIPC_LAUNCH_CHECK();
+    if (c.P.precision == 1) {
+        double* dinv = c.pdinv.ensure(9l * nV);
+        hipLaunchKernelGGL(k_dinv<double>, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p,
+                           K.col.p, K.val.p, K.nnzb, dinv);
+        CIPC_LAUNCH_CHECK();
+        return refine<float, double>(c, vE, vK, dinv, b, xsol, tol, maxIter, iters, relres, c.pr.ensure(n), c.pz.ensure(n),
+                                     c.pp.ensure(2 * n), c.pq.ensure(n), c.xacc.ensure(n), nullptr);
+    }
     float* dinv = c.fdinv.ensure(9l * nV);
     hipLaunchKernelGGL(k_dinv<float>, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p,
                        K.val.p, K.nnzb, dinv);
     CIPC_LAUNCH_CHECK();
-    float* fb = c.fb.ensure(n);
-    float* fx = c.fx.ensure(n);
-    double* r = c.rres.ensure(n);
-    CIPC_HIP(hipMemsetAsync(xsol, 0, n * sizeof(double), c.s));
-    const double bnorm = std::sqrt(residual_norm2(c, b, xsol, r));
-    if (relres) *relres = 0;
-    if (iters) *iters = 0;
-    if (bnorm == 0) return 0;
-    double rnorm = bnorm;
-    int total = 0;
-    constexpr int kMaxRefine = 8;
-    constexpr double kInnerFloor = 1e-6;   // FP32 recurrences stall near this relative residual
-    for (int k = 0; k < kMaxRefine; ++k) {
-        const double inner = std::max(kInnerFloor, std::min(0.5, 0.9 * tol * bnorm / rnorm));
-        hipLaunchKernelGGL(k_f64_to_f32, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const double*)r, fb);
-        CIPC_LAUNCH_CHECK();
-        int it = 0;
-        double rr = 0;
-        int rc = pcg_run<float>(c, vE, vK, fb, fx, c.fr.ensure(n), c.fz.ensure(n), c.fp.ensure(2 * n), c.fq.ensure(n), dinv, inner,
-                                std::max(1, maxIter - total), &it, &rr);
-        total += it;
-        c.lastRefinements = k + 1;
-        if (rc == 1) {
-            if (iters) *iters = total;
-            return 1;
-        }
-        hipLaunchKernelGGL(k_acc_f32, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const float*)fx, xsol);
-        CIPC_LAUNCH_CHECK();
-        rnorm = std::sqrt(residual_norm2(c, b, xsol, r));
-        if (relres) *relres = rnorm / bnorm;
-        if (iters) *iters = total;
-        if (rnorm <= tol * bnorm) return 0;
-        if (total >= maxIter) return 2;
-    }
-    return 2;
+    return refine<float, float>(c, vE, vK, dinv, b, xsol, tol, maxIter, iters, relres, c.fr.ensure(n), c.fz.ensure(n), c.fp.ensure(2 * n),
+                                c.fq.ensure(n), c.fx.ensure(n), c.fb.ensure(n));
 }
 
 }  // namespace cipc

@@ -116,11 +116,12 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
 
 /* ---- solver precision (no counterpart in the reference, which factorises in FP64
  * with CHOLMOD, DIRECT_SOLVER.h:14-66) -----------------------------------------
- * mode 0: FP64 block-Jacobi PCG.  mode 1: mixed -- FP32 Hessian blocks, FP32 Krylov
- * vectors and FP32 SpMV accumulation inside an FP64 iterative refinement that stops
- * when the FP64 residual ||b - H x|| <= pcg_tol ||b||.  Affects cipc_step and
- * cipc_linear_solve; out[0] of cipc_get_solver_info = refinement sweeps of the last
- * solve (0 in mode 0). */
+ * mode 0 (fp64): FP64 block-Jacobi PCG.
+ * mode 1 (mixed): FP32 Hessian blocks, FP64 Krylov vectors and SpMV accumulation.
+ * mode 2 (fp32): FP32 blocks, FP32 Krylov vectors, FP32 SpMV accumulation.
+ * Modes 1 and 2 run inside an FP64 iterative refinement that stops when the FP64
+ * residual ||b - H x|| <= pcg_tol ||b||.  Affects cipc_step and cipc_linear_solve;
+ * out[0] of cipc_get_solver_info = refinement sweeps of the last solve (0 in mode 0). */
 int cipc_set_precision(cipc_ctx* ctx, int32_t mode);
 int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out);
 
