@@ -11,6 +11,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libcipc.so"))
 
 D = ctypes.POINTER(ctypes.c_double)
 I32 = ctypes.POINTER(ctypes.c_int32)
+F32 = ctypes.POINTER(ctypes.c_float)
 
 CIPC_OK = 0
 ERRORS = {
@@ -99,6 +100,17 @@ PROTOTYPES = {
     "cipc_linear_solve": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, D, D, ctypes.c_double, I32, D]),
     "cipc_set_precision": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_get_solver_info": (ctypes.c_int, [P, I32]),
+    "cipc_precond_num_params": (ctypes.c_int32, []),
+    "cipc_precond_init": (ctypes.c_int, [P, ctypes.c_uint64]),
+    "cipc_precond_set_weights": (ctypes.c_int, [P, F32, ctypes.c_int32]),
+    "cipc_precond_get_weights": (ctypes.c_int, [P, F32, ctypes.c_int32]),
+    "cipc_set_preconditioner": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32]),
+    "cipc_precond_loss_grad": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_uint64, D, F32]),
+    "cipc_precond_train_step": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_uint64,
+                                               ctypes.c_double, D]),
+    "cipc_precond_get_adam": (ctypes.c_int, [P, F32, F32, I32, ctypes.c_int32]),
+    "cipc_precond_set_adam": (ctypes.c_int, [P, F32, F32, ctypes.c_int32, ctypes.c_int32]),
+    "cipc_precond_info": (ctypes.c_int, [P, I32]),
     "cipc_set_profiling": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_get_profile": (ctypes.c_int, [P, D]),
 }

@@ -132,6 +132,58 @@ class Simulator:
         self._check(self.lib.cipc_get_solver_info(self.h, _i(out)))
         return dict(refinements=int(out[0]))
 
+    # ------------------------------------------------------------ learned preconditioner
+    def precond_init(self, seed=1):
+        self._check(self.lib.cipc_precond_init(self.h, ctypes.c_uint64(seed)))
+
+    def precond_weights(self):
+        n = self.lib.cipc_precond_num_params()
+        w = np.zeros(n, np.float32)
+        self._check(self.lib.cipc_precond_get_weights(self.h, w.ctypes.data_as(_lib.F32), n))
+        return w
+
+    def precond_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self.lib.cipc_precond_set_weights(self.h, w.ctypes.data_as(_lib.F32), len(w)))
+
+    def set_preconditioner(self, mode, budget=0):
+        """mode "jacobi" (block Jacobi) or "learned" (falls back to block Jacobi after `budget` iterations)."""
+        self._check(self.lib.cipc_set_preconditioner(self.h, {"jacobi": 0, "learned": 1}[mode], int(budget)))
+
+    def precond_loss_grad(self, x, C, kappa, n_probes=8, seed=1, want_grad=True):
+        x, C = self._x(x), self._cs(C)
+        loss = np.zeros(1)
+        g = np.zeros(self.lib.cipc_precond_num_params(), np.float32) if want_grad else None
+        self._check(self.lib.cipc_precond_loss_grad(self.h, _d(x), _i(C), len(C), float(kappa), int(n_probes), ctypes.c_uint64(seed),
+                                                    _d(loss), None if g is None else g.ctypes.data_as(_lib.F32)))
+        return loss[0], g
+
+    def precond_train_step(self, x, C, kappa, lr=1e-3, n_probes=8, seed=1):
+        x, C = self._x(x), self._cs(C)
+        loss = np.zeros(1)
+        self._check(self.lib.cipc_precond_train_step(self.h, _d(x), _i(C), len(C), float(kappa), int(n_probes), ctypes.c_uint64(seed),
+                                                     float(lr), _d(loss)))
+        return loss[0]
+
+    def precond_get_state(self):
+        """(weights, Adam m, Adam v, Adam step) -- to continue training in another context"""
+        n = self.lib.cipc_precond_num_params()
+        m, v = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        t = ctypes.c_int32(0)
+        self._check(self.lib.cipc_precond_get_adam(self.h, m.ctypes.data_as(_lib.F32), v.ctypes.data_as(_lib.F32), ctypes.byref(t), n))
+        return self.precond_weights(), m, v, t.value
+
+    def precond_set_state(self, state):
+        w, m, v, t = state
+        self.precond_set_weights(w)
+        m, v = np.ascontiguousarray(m, np.float32), np.ascontiguousarray(v, np.float32)
+        self._check(self.lib.cipc_precond_set_adam(self.h, m.ctypes.data_as(_lib.F32), v.ctypes.data_as(_lib.F32), int(t), len(m)))
+
+    def precond_info(self):
+        out = np.zeros(4, np.int32)
+        self._check(self.lib.cipc_precond_info(self.h, _i(out)))
+        return dict(fallback=int(out[0]))
+
     def set_profiling(self, enable=True):
         self._check(self.lib.cipc_set_profiling(self.h, 1 if enable else 0))
 

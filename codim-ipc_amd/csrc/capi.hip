@@ -1,6 +1,7 @@
 // C ABI (include/cipc.h).  Every entry point converts HIP failures and invalid
 // arguments into negative status codes and records a message for cipc_last_error().
 #include "ctx.hpp"
+#include "learn.hpp"
 #include <cstring>
 #include <exception>
 
@@ -342,6 +343,92 @@ int cipc_set_precision(cipc_ctx* ctx, int32_t mode) {
 int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out) {
     if (!ctx || !out) return CIPC_E_INVALID;
     out[0] = ctx->c.lastRefinements;
+    return CIPC_OK;
+}
+
+int32_t cipc_precond_num_params(void) { return lp_num_params(); }
+int cipc_precond_init(cipc_ctx* ctx, uint64_t seed) {
+    if (!ctx) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        lp_init_weights(ctx->c, seed);
+        return CIPC_OK;
+    });
+}
+int cipc_precond_set_weights(cipc_ctx* ctx, const float* w, int32_t n) {
+    if (!ctx || !w || n != lp_num_params()) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        set_weights(ctx->c, w, n);
+        return CIPC_OK;
+    });
+}
+int cipc_precond_get_weights(cipc_ctx* ctx, float* w, int32_t n) {
+    if (!ctx || !w || n != lp_num_params()) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        get_weights(ctx->c, w, n);
+        return CIPC_OK;
+    });
+}
+int cipc_set_preconditioner(cipc_ctx* ctx, int32_t mode, int32_t budget) {
+    if (!ctx || mode < 0 || mode > 1 || budget < 0) return CIPC_E_INVALID;
+    if (mode == 1 && (!ctx->c.lp || !ctx->c.lp->weightsReady)) {
+        ctx->c.lastError = "learned preconditioner has no weights (cipc_precond_init / cipc_precond_set_weights)";
+        return CIPC_E_INVALID;
+    }
+    ctx->c.precond = mode;
+    if (budget > 0) ctx->c.lpBudget = budget;
+    return CIPC_OK;
+}
+static int precond_common(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa) {
+    Ctx& c = ctx->c;
+    if (!c.lp || !c.lp->weightsReady) {
+        c.lastError = "learned preconditioner has no weights";
+        return CIPC_E_INVALID;
+    }
+    size_t n = 3 * (size_t)c.nV;
+    upload_x(ctx, ctx->qx, x, n);
+    upload_cs(ctx, cs, n_cs);
+    elastic_hessian(c, ctx->qx.p);
+    barrier_hessian(c, ctx->qx.p, ctx->qC.p, n_cs, kappa);
+    return CIPC_OK;
+}
+int cipc_precond_loss_grad(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t n_probes, uint64_t seed,
+                           double* loss, float* grad) {
+    if (!ctx || !x || !loss || n_cs < 0 || (n_cs && !cs) || n_probes < 1) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        int rc = precond_common(ctx, x, cs, n_cs, kappa);
+        if (rc != CIPC_OK) return rc;
+        *loss = lp_loss_grad(ctx->c, n_probes, seed, grad);
+        return CIPC_OK;
+    });
+}
+int cipc_precond_train_step(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t n_probes, uint64_t seed,
+                            double lr, double* loss) {
+    if (!ctx || !x || n_cs < 0 || (n_cs && !cs) || n_probes < 1 || !(lr > 0)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        int rc = precond_common(ctx, x, cs, n_cs, kappa);
+        if (rc != CIPC_OK) return rc;
+        const double l = lp_train_step(ctx->c, n_probes, seed, lr);
+        if (loss) *loss = l;
+        return CIPC_OK;
+    });
+}
+int cipc_precond_get_adam(cipc_ctx* ctx, float* m, float* v, int32_t* t, int32_t n) {
+    if (!ctx || !m || !v || !t || n != lp_num_params()) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        get_adam(ctx->c, m, v, t, n);
+        return CIPC_OK;
+    });
+}
+int cipc_precond_set_adam(cipc_ctx* ctx, const float* m, const float* v, int32_t t, int32_t n) {
+    if (!ctx || !m || !v || t < 0 || n != lp_num_params()) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        set_adam(ctx->c, m, v, t, n);
+        return CIPC_OK;
+    });
+}
+int cipc_precond_info(const cipc_ctx* ctx, int32_t* out) {
+    if (!ctx || !out) return CIPC_E_INVALID;
+    out[0] = ctx->c.lastPrecondFallback;
     return CIPC_OK;
 }
 

@@ -7,6 +7,7 @@
 // component k contiguous, ...), so stencil kernels read them coalesced.
 #pragma once
 #include "common.hpp"
+#include <memory>
 #include "../../include/cipc.h"
 
 namespace cipc {
@@ -26,6 +27,8 @@ struct Params {
 // A primitive whose box covers more grid cells than this is not entered in the cell
 // lists; queries test it directly from a compact list instead.
 constexpr uint32_t kLargeCells = 512;
+
+struct LearnedPrecond;
 
 struct GridParams {
     double lb[3];
@@ -108,6 +111,12 @@ struct Ctx {
     DevBuf<float> valE32, valK32, fr, fz, fp, fq, fdinv, fx, fb;
     DevBuf<double> rres, xacc;
     int lastRefinements = 0;
+    // learned preconditioner (created on first use); 0 = block Jacobi, 1 = learned
+    std::shared_ptr<LearnedPrecond> lp;
+    DevBuf<double> lpTmp;
+    int precond = 0;
+    int lpBudget = 3000;              // PCG iterations before the learned solve falls back
+    int lastPrecondFallback = 0;
 
     // host pinned scalars
     double* hScal = nullptr;           // 64 doubles

@@ -34,6 +34,7 @@
 // The host launches iterations in chunks and polls an on-device done flag between
 // chunks; kernels launched after convergence return at once.
 #include "ctx.hpp"
+#include "learn.hpp"
 #include <hip/hip_ext.h>
 #include <cmath>
 
@@ -107,8 +108,8 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
                                                  const int* __restrict__ crp, const int* __restrict__ ccol, const M* __restrict__ cv,
                                                  int cnz, int nrow, int rowsPerXcd, const V* __restrict__ z,
                                                  const V* __restrict__ pold, V* __restrict__ pnew, V* __restrict__ q,
-                                                 const double* __restrict__ partU, int nU, double* __restrict__ partS,
-                                                 double* __restrict__ sc, int* __restrict__ flags) {
+                                                 const double* __restrict__ partRZ, int nRZ, const double* __restrict__ partRR, int nRR,
+                                                 double* __restrict__ partS, double* __restrict__ sc, int* __restrict__ flags) {
     if (flags[F_DONE]) return;
     __shared__ double lds[BS / 64];
     constexpr int RPG = BS / LANES;   // rows per group
@@ -123,7 +124,9 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
 
     double arz = 0, arr = 0;
 #pragma unroll 4
-    for (int i = threadIdx.x; i < nU; i += BS) { arz += partU[i]; arr += partU[nU + i]; }
+    for (int i = threadIdx.x; i < nRZ; i += BS) arz += partRZ[i];
+#pragma unroll 4
+    for (int i = threadIdx.x; i < nRR; i += BS) arr += partRR[i];
     const double rz = block_allsum<BS>(arz, lds);
     const double rr = block_allsum<BS>(arr, lds);
     const double bb = it == 0 ? rr : sc[S_BB];
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
 
 // Iteration `it`: alpha = r.z / p.q, x += alpha p, r -= alpha q, z = Dinv r, partials
 // of r.z and r.r (written as partU[block], partU[nU + block]).
-template <class V, int BS>
+template <class V, int BS, bool JAC>
 __global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPerXcd, V* __restrict__ x, V* __restrict__ r,
                                                    V* __restrict__ z, const V* __restrict__ p, const V* __restrict__ q,
                                                    const V* __restrict__ dinv, const double* __restrict__ partS, int nS,
@@ -199,8 +202,9 @@ __global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPer
             pv[k] = p[3 * row + k]; qv[k] = q[3 * row + k];
             xv[k] = x[3 * row + k]; rv[k] = r[3 * row + k];
         }
+        if (JAC)
 #pragma unroll
-        for (int k = 0; k < 9; ++k) D[k] = dinv[9l * row + k];
+            for (int k = 0; k < 9; ++k) D[k] = dinv[9l * row + k];
     }
     const double pq = block_allsum<BS>(a, lds);
     if (!(pq > 0)) {
@@ -222,9 +226,11 @@ __global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPer
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const V zk = D[3 * k] * rv[0] + D[3 * k + 1] * rv[1] + D[3 * k + 2] * rv[2];
-            z[3 * row + k] = zk;
-            rz += (double)rv[k] * (double)zk;
+            if (JAC) {
+                const V zk = D[3 * k] * rv[0] + D[3 * k + 1] * rv[1] + D[3 * k + 2] * rv[2];
+                z[3 * row + k] = zk;
+                rz += (double)rv[k] * (double)zk;
+            }
             rr += (double)rv[k] * (double)rv[k];
         }
     }
@@ -356,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const int* __restrict__ erp
 // (p^T A p <= 0), 2 iteration limit
 template <class M, class V>
 static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V* r, V* z, V* pbuf, V* q, const V* dinv, double tol, int maxIter,
-                   int* iters, double* relres) {
+                   int* iters, double* relres, bool learned = false) {
     const int nV = c.nV, n = 3 * nV;
     BSR& E = c.Ael;
     BSR& K = c.Acon;
@@ -365,11 +371,23 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     constexpr int RPG = kSpmvBlock / kSpmvLanes;
     const int rpxS = rows_per_xcd(nV, RPG), nS = kSpmvGrid;
     const int rpxU = rows_per_xcd(nV, kUpdBlock), nU = kXcd * (rpxU / kUpdBlock);
-    double* partS = c.dscr.ensure((size_t)nS + 2l * nU);
+    const int nP = lp_apply_partials(nV);
+    double* partS = c.dscr.ensure((size_t)nS + 2l * nU + nP);
     double* partU = partS + nS;
+    double* partP = partU + 2l * nU;
+    // r.z partials: the update kernel's (block Jacobi) or the preconditioner product's
+    const double* partRZ = learned ? partP : partU;
+    const int nRZ = learned ? nP : nU;
+    double* ltmp = nullptr;
 
     hipLaunchKernelGGL((k_pcg_init<V, kUpdBlock>), dim3(nU), dim3(kUpdBlock), 0, c.s, nV, rpxU, b, xsol, r, z, dinv, partU, flags);
     CIPC_LAUNCH_CHECK();
+    if constexpr (sizeof(V) == sizeof(double)) {
+        if (learned) {
+            ltmp = c.lpTmp.ensure((size_t)n);
+            lp_apply(c, (const double*)r, (double*)z, ltmp, partP);
+        }
+    }
 
     const int chunk = 16;
     int* hf = (int*)(c.hCnt + 32);
@@ -392,10 +410,16 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
             auto kern = it > 0 ? k_pcg_spmv<M, V, kSpmvLanes, kSpmvBlock, true> : k_pcg_spmv<M, V, kSpmvLanes, kSpmvBlock, false>;
             hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.profiling ? c.evPool[2 * k] : nullptr,
                                   c.profiling ? c.evPool[2 * k + 1] : nullptr, 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
-                                  K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, (const double*)partU, nU,
-                                  partS, sc, flags);
-            hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z, (const V*)pnew,
-                               (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
+                                  K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, partRZ, nRZ,
+                                  (const double*)(partU + nU), nU, partS, sc, flags);
+            if (!learned) {
+                hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock, true>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z,
+                                   (const V*)pnew, (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
+            } else if constexpr (sizeof(V) == sizeof(double)) {
+                hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock, false>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z,
+                                   (const V*)pnew, (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
+                lp_apply(c, (const double*)r, (double*)z, ltmp, partP);
+            }
         }
         CIPC_LAUNCH_CHECK();
         CIPC_HIP(hipMemcpyAsync(hf, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, c.s));
@@ -452,7 +476,7 @@ __global__ void k_acc_f64(long n, const double* __restrict__ d, double* __restri
 // ||r|| <= tol ||b||.  rb is the precision-V copy of r (unused when V = double).
 template <class M, class V>
 static int refine(Ctx& c, const M* vE, const M* vK, const V* dinv, const double* b, double* xsol, double tol, int maxIter, int* iters,
-                  double* relres, V* wr, V* wz, V* wp, V* wq, V* d, V* rb) {
+                  double* relres, V* wr, V* wz, V* wp, V* wq, V* d, V* rb, bool learned) {
     const long n = 3l * c.nV;
     double* r = c.rres.ensure(n);
     CIPC_HIP(hipMemsetAsync(xsol, 0, n * sizeof(double), c.s));
@@ -482,7 +506,7 @@ static int refine(Ctx& c, const M* vE, const M* vK, const V* dinv, const double*
         }
         int it = 0;
         double rr = 0;
-        int rc = pcg_run<M, V>(c, vE, vK, rhs, d, wr, wz, wp, wq, dinv, inner, std::max(1, maxIter - total), &it, &rr);
+        int rc = pcg_run<M, V>(c, vE, vK, rhs, d, wr, wz, wp, wq, dinv, inner, std::max(1, maxIter - total), &it, &rr, learned);
         total += it;
         c.lastRefinements = k + 1;
         if (rc == 1) {
@@ -503,7 +527,7 @@ static int refine(Ctx& c, const M* vE, const M* vK, const V* dinv, const double*
     return 2;
 }
 
-int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
+static int solve_form(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres, bool learned) {
     const int nV = c.nV;
     const long n = 3l * nV;
     BSR& E = c.Ael;
@@ -516,7 +540,7 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
                            K.col.p, K.val.p, K.nnzb, dinv);
         CIPC_LAUNCH_CHECK();
         return pcg_run<double, double>(c, E.val.p, K.val.p, b, xsol, c.pr.ensure(n), c.pz.ensure(n), c.pp.ensure(2 * n), c.pq.ensure(n), dinv, tol,
-                               maxIter, iters, relres);
+                                       maxIter, iters, relres, learned);
     }
     // mixed forms: FP32 copies of the blocks; FP64 iterative refinement around the solve
     float* vE = c.valE32.ensure(9l * E.nnzb + 1);
@@ -530,14 +554,37 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
                            K.col.p, K.val.p, K.nnzb, dinv);
         CIPC_LAUNCH_CHECK();
         return refine<float, double>(c, vE, vK, dinv, b, xsol, tol, maxIter, iters, relres, c.pr.ensure(n), c.pz.ensure(n),
-                                     c.pp.ensure(2 * n), c.pq.ensure(n), c.xacc.ensure(n), nullptr);
+                                     c.pp.ensure(2 * n), c.pq.ensure(n), c.xacc.ensure(n), nullptr, learned);
     }
     float* dinv = c.fdinv.ensure(9l * nV);
     hipLaunchKernelGGL(k_dinv<float>, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, E.rowptr.p, E.col.p, E.val.p, E.nnzb, K.rowptr.p, K.col.p,
                        K.val.p, K.nnzb, dinv);
     CIPC_LAUNCH_CHECK();
     return refine<float, float>(c, vE, vK, dinv, b, xsol, tol, maxIter, iters, relres, c.fr.ensure(n), c.fz.ensure(n), c.fp.ensure(2 * n),
-                                c.fq.ensure(n), c.fx.ensure(n), c.fb.ensure(n));
+                                c.fq.ensure(n), c.fx.ensure(n), c.fb.ensure(n), false);
+}
+
+// The learned preconditioner runs with an iteration budget; when it does not converge
+// within it (or meets a non-positive curvature) the solve restarts with block Jacobi.
+// The all-FP32 form has no learned variant (the preconditioner product is FP64).
+int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
+    c.lastPrecondFallback = 0;
+    const bool learned = c.precond == 1 && c.P.precision != 2;
+    if (!learned) return solve_form(c, b, xsol, tol, maxIter, iters, relres, false);
+    lp_prepare(c);
+    int it = 0;
+    double rr = 0;
+    int rc = solve_form(c, b, xsol, tol, std::min(maxIter, c.lpBudget), &it, &rr, true);
+    if (rc == 0) {
+        if (iters) *iters = it;
+        if (relres) *relres = rr;
+        return 0;
+    }
+    c.lastPrecondFallback = 1;
+    int it2 = 0;
+    rc = solve_form(c, b, xsol, tol, maxIter, &it2, relres, false);
+    if (iters) *iters = it + it2;
+    return rc;
 }
 
 }  // namespace cipc

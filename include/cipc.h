@@ -125,6 +125,31 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
 int cipc_set_precision(cipc_ctx* ctx, int32_t mode);
 int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out);
 
+/* ---- learned preconditioner (no counterpart in the reference) ------------
+ * A 4-layer message-passing network (hidden 64, BF16 weights, FP32 accumulation) over
+ * the block graph of the Hessian predicts a block lower-triangular factor G on the
+ * Hessian's sparsity pattern; the PCG uses M^-1 = S G^T G S with S = diag(H)^-1/2.
+ * Training minimises the Kaporin condition number of G S H S G^T (Rademacher trace
+ * estimate with n_probes probes, exact log-det).  mode 1 of cipc_set_preconditioner
+ * uses it with `budget` PCG iterations, after which the solve restarts with block Jacobi
+ * (mode 0, the default).  Weights: cipc_precond_num_params() FP32 values. */
+int32_t cipc_precond_num_params(void);
+int cipc_precond_init(cipc_ctx* ctx, uint64_t seed);
+int cipc_precond_set_weights(cipc_ctx* ctx, const float* w, int32_t n);
+int cipc_precond_get_weights(cipc_ctx* ctx, float* w, int32_t n);
+int cipc_set_preconditioner(cipc_ctx* ctx, int32_t mode, int32_t budget);
+/* assemble H(x, constraint set) as cipc_linear_solve does, then: loss and the flat
+ * parameter gradient (grad may be NULL) / one Adam step at learning rate lr */
+int cipc_precond_loss_grad(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t n_probes, uint64_t seed,
+                           double* loss, float* grad);
+int cipc_precond_train_step(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t n_probes, uint64_t seed,
+                            double lr, double* loss);
+/* Adam moments and step count, so that training can move between scenes (contexts) */
+int cipc_precond_get_adam(cipc_ctx* ctx, float* m, float* v, int32_t* t, int32_t n);
+int cipc_precond_set_adam(cipc_ctx* ctx, const float* m, const float* v, int32_t t, int32_t n);
+/* out[0] = 1 if the last solve fell back to block Jacobi */
+int cipc_precond_info(const cipc_ctx* ctx, int32_t* out);
+
 /* ---- instrumentation (no counterpart in the reference) ------------------ */
 /* Kernel timing of the dominant kernel of the step, the PCG SpMV (q = A p with
  * p = z + beta p_old formed on the fly), from the dispatch's own start/end
