@@ -108,6 +108,7 @@ PROTOTYPES = {
     "cipc_precond_loss_grad": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_uint64, D, F32]),
     "cipc_precond_train_step": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_uint64,
                                                ctypes.c_double, D]),
+    "cipc_precond_factor": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, F32, I32]),
     "cipc_precond_get_adam": (ctypes.c_int, [P, F32, F32, I32, ctypes.c_int32]),
     "cipc_precond_set_adam": (ctypes.c_int, [P, F32, F32, ctypes.c_int32, ctypes.c_int32]),
     "cipc_precond_info": (ctypes.c_int, [P, I32]),

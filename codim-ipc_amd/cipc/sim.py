@@ -165,6 +165,16 @@ class Simulator:
                                                      float(lr), _d(loss)))
         return loss[0]
 
+    def precond_factor(self, x, C, kappa, mfma=True):
+        """factor G (SoA [9][nnzb] float32) at H(x, C); mfma selects the BF16 matrix-core layers"""
+        x, C = self._x(x), self._cs(C)
+        nnzb = ctypes.c_int32(0)
+        self._check(self.lib.cipc_precond_factor(self.h, _d(x), _i(C), len(C), float(kappa), 1 if mfma else 0, None, ctypes.byref(nnzb)))
+        g = np.zeros(9 * nnzb.value, np.float32)
+        self._check(self.lib.cipc_precond_factor(self.h, _d(x), _i(C), len(C), float(kappa), 1 if mfma else 0, g.ctypes.data_as(_lib.F32),
+                                                 ctypes.byref(nnzb)))
+        return g.reshape(9, -1)
+
     def precond_get_state(self):
         """(weights, Adam m, Adam v, Adam step) -- to continue training in another context"""
         n = self.lib.cipc_precond_num_params()

@@ -412,6 +412,18 @@ int cipc_precond_train_step(cipc_ctx* ctx, const double* x, const int32_t* cs, i
         return CIPC_OK;
     });
 }
+int cipc_precond_factor(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t use_mfma, float* g,
+                        int32_t* nnzb) {
+    if (!ctx || !x || !nnzb || n_cs < 0 || (n_cs && !cs)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        *nnzb = ctx->c.Ael.nnzb;
+        if (!g) return CIPC_OK;
+        int rc = precond_common(ctx, x, cs, n_cs, kappa);
+        if (rc != CIPC_OK) return rc;
+        lp_factor(ctx->c, use_mfma != 0, g, 9 * ctx->c.Ael.nnzb);
+        return CIPC_OK;
+    });
+}
 int cipc_precond_get_adam(cipc_ctx* ctx, float* m, float* v, int32_t* t, int32_t n) {
     if (!ctx || !m || !v || !t || n != lp_num_params()) return CIPC_E_INVALID;
     return guarded(ctx, [&]() {

@@ -87,15 +87,32 @@ __global__ void k_lp_node_features(int nV, const int* __restrict__ diagBlk, cons
     f[10] = logf(1.0f + (float)ncon);
     f[11] = dbc[i] ? 1.f : 0.f;
 }
+// edge features: S_i (H_el + H_con)_ij S_j; the contact block of (i, j), if any, is found
+// by binary search in row i of the contact matrix (columns sorted)
 __global__ void k_lp_edge_features(int nE, const int* __restrict__ eBlock, const int* __restrict__ eSrc, const int* __restrict__ eDst,
-                                   const double* __restrict__ ev, int enz, const double* __restrict__ S, float* __restrict__ edgeF) {
+                                   const double* __restrict__ ev, int enz, const int* __restrict__ crp, const int* __restrict__ ccol,
+                                   const double* __restrict__ cv, int cnz, const double* __restrict__ S, float* __restrict__ edgeF) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nE) return;
     const int b = eBlock[e], i = eSrc[e], j = eDst[e];
+    double a[9];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 9; ++k) a[k] = ev[(long)k * enz + b];
+    if (cnz > 0) {
+        int lo = crp[i], hi = crp[i + 1];
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ccol[mid] < j) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < crp[i + 1] && ccol[lo] == j)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) edgeF[(long)kLpEdgeIn * e + 3 * a + c] = (float)(S[3 * i + a] * ev[(long)(3 * a + c) * enz + b] * S[3 * j + c]);
+            for (int k = 0; k < 9; ++k) a[k] += cv[(long)k * cnz + lo];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) edgeF[(long)kLpEdgeIn * e + 3 * r + c] = (float)(S[3 * i + r] * a[3 * r + c] * S[3 * j + c]);
 }
 
 // ------------------------------------------------------------------ dense layers
@@ -286,6 +303,75 @@ __global__ void k_lp_scatter_lower(int nL, const int* __restrict__ lEdge, const 
     const long l = t / kLpHidden;
     const int f = (int)(t % kLpHidden);
     de[(long)lEdge[l] * kLpHidden + f] = dE[t];
+}
+
+
+// ------------------------------------------------------------------ MFMA message layer
+// Inference form of the 192 -> 64 message layer on the matrix cores: inputs and weights
+// in BF16, FP32 accumulation (v_mfma_f32_16x16x16_bf16).  A wave computes 16 rows x 64
+// outputs: for every 16-wide k step it feeds one 16x16 A tile (lane l: row l % 16,
+// k = 4 (l / 16) .. +3, from the gathered input row) and four 16x16 B tiles (lane l:
+// column 16 j + l % 16, the same 4 k's) held transposed in LDS; the accumulators hold
+// rows 4 (l / 16) + t, column 16 j + l % 16.
+typedef short lp_v4i16 __attribute__((ext_vector_type(4)));
+typedef float lp_v4f32 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_lp_dense_mfma(int nrows, LpSeg s0, LpSeg s1, LpSeg s2, const float* __restrict__ W,
+                                                       const float* __restrict__ bias, int relu, float* __restrict__ out) {
+    static_assert(K % 64 == 0 && K <= kLpMaxK, "segments are 64 wide");
+    constexpr int PITCH = K + 4;   // shorts; keeps 8-byte alignment of every 4-k group
+    __shared__ short Wt[kLpHidden * PITCH];
+    for (int t = threadIdx.x; t < K * kLpHidden; t += 256) {
+        const int k = t / kLpHidden, n = t % kLpHidden;
+        Wt[n * PITCH + k] = bf16_bits(W[t]);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int i = lane % 16, kg = lane / 16;
+    const int rbase = blockIdx.x * 64 + wave * 16;
+    const int r = rbase + i;
+    const bool valid = r < nrows;
+    long rowIn[3] = {0, 0, 0};
+    if (valid) {
+        rowIn[0] = s0.idx ? s0.idx[r] : r;
+        rowIn[1] = s1.idx ? s1.idx[r] : r;
+        rowIn[2] = s2.idx ? s2.idx[r] : r;
+    }
+    lp_v4f32 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = (lp_v4f32){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < K / 16; ++ks) {
+        const int k0 = ks * 16 + 4 * kg;
+        const int seg = k0 / kLpHidden, kk = k0 % kLpHidden;
+        const float* src = seg == 0 ? s0.x : (seg == 1 ? s1.x : s2.x);
+        float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) xv = *(const float4*)(src + rowIn[seg] * kLpHidden + kk);
+        const lp_v4i16 a = {bf16_bits(xv.x), bf16_bits(xv.y), bf16_bits(xv.z), bf16_bits(xv.w)};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const lp_v4i16 b = *(const lp_v4i16*)&Wt[(16 * j + i) * PITCH + k0];
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int row = rbase + 4 * kg + t, col = 16 * j + i;
+            if (row < nrows) {
+                float y = acc[j][t] + bias[col];
+                if (relu) y = y > 0.f ? y : 0.f;
+                out[(long)row * kLpHidden + col] = y;
+            }
+        }
 }
 
 // ------------------------------------------------------------------ G assembly / products
@@ -595,6 +681,17 @@ void lp_init_weights(Ctx& c, uint64_t seed) {
 
 int lp_num_params() { return LpLayout().total; }
 
+void lp_factor(Ctx& c, bool useMfma, float* G, int n) {
+    LearnedPrecond& lp = LP(c);
+    const bool mf = lp.mfma;
+    lp.mfma = useMfma;
+    lp_prepare(c);
+    lp.mfma = mf;
+    if (n != 9 * lp.nnzb) throw HipError("learned preconditioner: factor buffer size", -1);
+    CIPC_HIP(hipMemcpyAsync(G, lp.Gblk.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+}
+
 void set_weights(Ctx& c, const float* w, int n) {
     LearnedPrecond& lp = LP(c);
     if (n != lp.L.total) throw HipError("learned preconditioner: wrong parameter count", -1);
@@ -682,7 +779,8 @@ static void features(Ctx& c) {
                        (const uint8_t*)c.dbc.p, lp.S.p, lp.nodeF.p);
     if (lp.nE > 0)
         hipLaunchKernelGGL(k_lp_edge_features, dim3(nblocks(lp.nE)), dim3(kBlock), 0, c.s, lp.nE, (const int*)lp.eBlock.p, (const int*)lp.eSrc.p,
-                           (const int*)lp.eDst.p, (const double*)E.val.p, E.nnzb, (const double*)lp.S.p, lp.edgeF.p);
+                           (const int*)lp.eDst.p, (const double*)E.val.p, E.nnzb, (const int*)K.rowptr.p, (const int*)K.col.p,
+                           (const double*)K.val.p, K.nnzb, (const double*)lp.S.p, lp.edgeF.p);
     CIPC_LAUNCH_CHECK();
 }
 
@@ -695,8 +793,14 @@ void lp_forward(Ctx& c) {
     dense<kLpHidden>(c, nV, {lp.nodeF.p, nullptr, kLpNodeIn}, kNoSeg, kNoSeg, Wq + L.we, Wq + L.be, 1, nullptr, lp.h[0].p);
     dense<kLpHidden>(c, nE, {lp.edgeF.p, nullptr, kLpEdgeIn}, kNoSeg, kNoSeg, Wq + L.wx, Wq + L.bx, 1, nullptr, lp.e[0].p);
     for (int l = 0; l < kLpLayers; ++l) {
-        dense<kLpHidden>(c, nE, {lp.h[l].p, lp.eSrc.p, kLpHidden}, {lp.h[l].p, lp.eDst.p, kLpHidden}, {lp.e[l].p, nullptr, kLpHidden},
-                         Wq + L.wm[l], Wq + L.bm[l], 1, nullptr, lp.m[l].p);
+        const LpSeg ms0 = {lp.h[l].p, lp.eSrc.p, kLpHidden}, ms1 = {lp.h[l].p, lp.eDst.p, kLpHidden}, ms2 = {lp.e[l].p, nullptr, kLpHidden};
+        if (lp.mfma && nE > 0) {
+            hipLaunchKernelGGL(k_lp_dense_mfma<3 * kLpHidden>, dim3((nE + 63) / 64), dim3(256), 0, c.s, nE, ms0, ms1, ms2, Wq + L.wm[l],
+                               Wq + L.bm[l], 1, lp.m[l].p);
+            CIPC_LAUNCH_CHECK();
+        } else {
+            dense<kLpHidden>(c, nE, ms0, ms1, ms2, Wq + L.wm[l], Wq + L.bm[l], 1, nullptr, lp.m[l].p);
+        }
         add(c, H * nE, lp.e[l].p, lp.m[l].p, lp.e[l + 1].p);
         hipLaunchKernelGGL(k_lp_agg, dim3((nV + 3) / 4), dim3(256), 0, c.s, nV, (const int*)lp.erp.p, (const float*)lp.m[l].p, lp.agg[l].p);
         CIPC_LAUNCH_CHECK();
@@ -738,7 +842,10 @@ int lp_apply(Ctx& c, const double* r, double* z, double* tmp, double* part) {
 
 double lp_loss_grad(Ctx& c, int nProbes, uint64_t seed, float* gradOut) {
     LearnedPrecond& lp = LP(c);
+    const bool mf = lp.mfma;
+    lp.mfma = false;   // training differentiates the FP32 forward
     lp_prepare(c);
+    lp.mfma = mf;
     const LpLayout& L = lp.L;
     BSR& E = c.Ael;
     BSR& K = c.Acon;

@@ -8,6 +8,8 @@
 //     L = log( tr(G H^ G^T) / n ) - (2/n) log det G,
 // with tr() estimated by Rademacher probes and log det G exact (G is triangular with
 // exponentiated diagonal).  With the decoder at zero, G = I and M^-1 is point Jacobi.
+// Node features: H^_ii, m_i mean(S_i^2), log(1 + contact blocks in row i), Dirichlet
+// flag; edge features: H^_ij (elastic plus contact part) on the elastic block pattern.
 //
 // Network (4 message-passing layers, hidden 64):
 //   h0 = relu(We f_i + be)            f_i: 12 node features
@@ -37,6 +39,7 @@ struct LpLayout {
 struct LearnedPrecond {
     LpLayout L;
     bool graphReady = false, weightsReady = false;
+    bool mfma = true;                 // inference forward: message layers on the matrix cores
     int nV = 0, nnzb = 0, nE = 0, nL = 0;
     // static graph of the elastic BSR pattern (directed edges = off-diagonal blocks)
     DevBuf<int> eBlock, eSrc, eDst, erp, eT, lEdge, blkT, diagBlk, lSrcOff, lSrcIdx, lDstOff, lDstIdx, lowSrc, lowDst;
@@ -63,6 +66,8 @@ void get_weights(Ctx& c, float* w, int n);
 void get_adam(Ctx& c, float* m, float* v, int* t, int n);
 void set_adam(Ctx& c, const float* m, const float* v, int t, int n);
 int lp_num_params();
+// forward on the current Hessian with or without the MFMA message layers; G = SoA [9][nnzb]
+void lp_factor(Ctx& c, bool useMfma, float* G, int n);
 void lp_forward(Ctx& c);
 // one training step on the current Hessian: loss, gradients, Adam update; returns loss
 double lp_train_step(Ctx& c, int nProbes, uint64_t seed, double lr);

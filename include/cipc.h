@@ -144,6 +144,12 @@ int cipc_precond_loss_grad(cipc_ctx* ctx, const double* x, const int32_t* cs, in
                            double* loss, float* grad);
 int cipc_precond_train_step(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t n_probes, uint64_t seed,
                             double lr, double* loss);
+/* the factor G at H(x, constraint set): 9 * nnzb floats (SoA [9][nnzb] over the elastic
+ * block pattern, lower blocks, their transposes at the upper positions, G_ii on the
+ * diagonal); use_mfma selects the BF16 matrix-core message layers (inference form) or
+ * the FP32 ones (training form); *nnzb receives the pattern size when g is NULL */
+int cipc_precond_factor(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t use_mfma, float* g,
+                        int32_t* nnzb);
 /* Adam moments and step count, so that training can move between scenes (contexts) */
 int cipc_precond_get_adam(cipc_ctx* ctx, float* m, float* v, int32_t* t, int32_t n);
 int cipc_precond_set_adam(cipc_ctx* ctx, const float* m, const float* v, int32_t t, int32_t n);

@@ -7,7 +7,8 @@ TEST INFRASTRUCTURE ONLY: the checker for the HIP kernels in codim-ipc_amd/csrc/
   projection); graph = the elastic block pattern (all vertex pairs of a triangle or hinge,
   plus the diagonal).
   node features (12): H^_ii (9), m_i mean(S_i^2), log(1 + contact blocks in row i),
-  Dirichlet flag;  edge features (9): H^_ij restricted to the elastic part of H.
+  Dirichlet flag;  edge features (9): H^_ij (all of H: elastic and contact) on the
+  elastic block pattern.
   h0 = relu(We f + be), e0 = relu(Wx a + bx)
   4 x [ m = relu(Wm [h_i, h_j, e_ij] + bm); h += relu(Wu [h, sum_j m] + bu); e += m ]
   G_ij (j < i) = Wo [h_i, h_j, e_ij] + bo;  G_ii = tril, diag exp(r_0..2), r_3..5 below
@@ -121,9 +122,6 @@ def loss_and_grad(H, mass, dbc, tri, hinge, weights, nprobes, seed, contact_rows
     node[:, 11] = dbc.astype(float)
     off = row != col
     er, ec = row[off], col[off]
-    # edge features from the elastic part: H minus contact blocks is not separable from the
-    # assembled CSR, so callers pass an H whose off-diagonal elastic blocks are what the
-    # library reads (states without contact, or contact confined to blocks off the pattern)
     ef = np.zeros((len(er), 9))
     for t, (i, j) in enumerate(zip(er, ec)):
         k = lut.get((int(i), int(j)))

@@ -29,7 +29,7 @@ def setup(require_gpu):
     return sd, gpu, orc, states
 
 
-def random_weights(gpu, seed):
+def random_weights(gpu, seed, scale=0.02):
     """library initialisation, then the decoder moved off zero so every layer gets a gradient"""
     gpu.precond_init(seed)
     w = gpu.precond_weights()
@@ -37,7 +37,7 @@ def random_weights(gpu, seed):
     L, _ = R.layout()
     for name in ("wo", "bo", "wd", "bd"):
         o, n = L[name]
-        w[o:o + n] = (rng.standard_normal(n) * 0.02).astype(np.float32)
+        w[o:o + n] = (rng.standard_normal(n) * scale).astype(np.float32)
     gpu.precond_set_weights(w)
     return w
 
@@ -61,6 +61,26 @@ def test_loss_and_gradient_match_cpu(setup, state):
     for name, (o, n) in L.items():
         gc, gg = grad_c[o:o + n], grad_g[o:o + n].astype(np.float64)
         assert np.linalg.norm(gg - gc) <= 1e-3 * np.linalg.norm(gc) + 1e-9, name
+
+
+def test_mfma_forward_matches_fp32_forward(setup):
+    """inference form (BF16 activations into the matrix cores, FP32 accumulation) against
+    the FP32 training form of the same network: BF16 keeps 8 mantissa bits, so the factor
+    agrees to a few 1e-3 relative (bar 2e-2)"""
+    sd, gpu, orc, states = setup
+    x = states[2]
+    C = orc.constraint_set(x)
+    kappa = orc.scalars()["kappa"]
+    random_weights(gpu, 9, scale=0.1)
+    g32 = gpu.precond_factor(x, C, kappa, mfma=False)
+    g16 = gpu.precond_factor(x, C, kappa, mfma=True)
+    assert np.isfinite(g16).all()
+    assert np.array_equal(g16 == 0, g32 == 0)
+    err = np.linalg.norm(g16 - g32) / np.linalg.norm(g32 - np.round(g32 * 0))
+    assert err <= 2e-2, err
+    # the deviation from identity is what the network contributes; compare it too
+    dev32 = g32 - (np.abs(g32 - 1) < 0.3) * 1.0
+    assert np.linalg.norm(g16 - g32) <= 5e-2 * np.linalg.norm(dev32)
 
 
 def test_untrained_learned_pcg_is_point_jacobi_and_converges(setup):
