@@ -7,13 +7,16 @@ Newton linear solve to relative residual 1e-5.  A "step" is one call of the impl
 step (the reference's Advance_One_Step_IE_Hinge).  Inputs are generated on the host and
 uploaded before the timed region; positions stay resident in HBM.
 
-Three solver forms are timed on the same scene, each from the initial state:
-  fp64   block-Jacobi PCG in FP64;
-  mixed  FP32 Hessian blocks, FP64 Krylov vectors / SpMV accumulation;
-  fp32   FP32 Hessian blocks, Krylov vectors and SpMV accumulation;
-the last two inside FP64 iterative refinement stopped on the FP64 residual (same 1e-5
-relative bar).  `value` is the steps/s of the faster reduced-precision form (named in
-`mode`); every form's figures are reported under `forms`.
+Solver forms timed on the same scene, each from the initial state:
+  fp64           block-Jacobi PCG in FP64;
+  mixed          FP32 Hessian blocks, FP64 Krylov vectors / SpMV accumulation;
+  fp32           FP32 Hessian blocks, Krylov vectors and SpMV accumulation;
+  mixed_learned  the mixed form preconditioned by the learned factor (weights trained by
+  fp32_learned   cipc.precond, codim-ipc_amd/cipc/weights/precond.npz), falling back to
+                 block Jacobi when its iteration budget runs out; the same for fp32;
+the reduced ones inside FP64 iterative refinement stopped on the FP64 residual (same
+1e-5 relative bar).  `value` is the steps/s of the fastest reduced-precision form (named
+in `mode`); every form's figures are reported under `forms`.
 
 The JSON line carries the roofline of the dominant kernel of the headline form (the PCG
 SpMV, timed from its own dispatch timestamps on the library's stream during the timed
@@ -33,24 +36,37 @@ METRIC = "sim steps/s + ms/Newton iter, 320k-tri cloth; plain FP64 vs mixed+lear
 HBM_PEAK_GBS = 8000.0
 KERNEL = {"fp64": "k_pcg_spmv<double,double,16,256> (PCG SpMV, FP64 blocks, FP64 vectors)",
           "mixed": "k_pcg_spmv<float,double,16,256> (PCG SpMV, FP32 blocks, FP64 vectors)",
-          "fp32": "k_pcg_spmv<float,float,16,256> (PCG SpMV, FP32 blocks, FP32 vectors)"}
+          "fp32": "k_pcg_spmv<float,float,16,256> (PCG SpMV, FP32 blocks, FP32 vectors)",
+          "mixed_learned": "k_pcg_spmv<float,double,16,256> (PCG SpMV, FP32 blocks, FP64 vectors)",
+          "fp32_learned": "k_pcg_spmv<float,float,16,256> (PCG SpMV, FP32 blocks, FP32 vectors)"}
 MODE = {"fp64": "plain FP64, block-Jacobi PCG",
         "mixed": "FP32 Hessian blocks, FP64 vectors, FP64 iterative refinement, block-Jacobi PCG",
-        "fp32": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, block-Jacobi PCG"}
+        "fp32": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, block-Jacobi PCG",
+        "mixed_learned": "FP32 Hessian blocks, FP64 vectors, FP64 iterative refinement, learned-factor PCG (block-Jacobi fallback)",
+        "fp32_learned": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, learned-factor PCG (block-Jacobi fallback)"}
+PRECISION = {"fp64": "fp64", "mixed": "mixed", "fp32": "fp32", "mixed_learned": "mixed", "fp32_learned": "fp32"}
 
 
-def gpu_run(args, sd, precision):
+def gpu_run(args, sd, form):
     import cipc
     import numpy as np
-    sim = cipc.Simulator(sd, pcg_tol=args.pcg_tol, precision=precision)
+    precision = form
+    sim = cipc.Simulator(sd, pcg_tol=args.pcg_tol, precision=PRECISION[form])
+    if form.endswith("_learned"):
+        from cipc import precond
+        sim.precond_set_weights(precond.load())
+        sim.set_preconditioner("learned", budget=args.learned_budget)
     for _ in range(args.warmup):
         sim.step()
     sim.set_profiling(True)
     stats = []
+    learned = form.endswith("_learned")
+    info0 = sim.precond_info() if learned else None
     t0 = time.perf_counter()
     for _ in range(args.steps):
         stats.append(sim.step())
     elapsed = time.perf_counter() - t0
+    info1 = sim.precond_info() if learned else None
     prof = sim.profile()
     sim.set_profiling(False)
     x, _ = sim.get_state()
@@ -69,6 +85,8 @@ def gpu_run(args, sd, precision):
                 ms_per_newton_iter=1e3 * elapsed / max(newton, 1), newton_iters_per_step=newton / args.steps,
                 pcg_iters_per_newton=pcg / max(newton, 1), contacts_last_step=stats[-1]["contacts"],
                 statuses=sorted(set(s["status"] for s in stats)), min_dist_final=min_d,
+                learned_solves=(info1["solves"] - info0["solves"]) if learned else None,
+                learned_fallbacks=(info1["fallbacks"] - info0["fallbacks"]) if learned else None,
                 spmv_avg_us=roof["avg_us"] if roof else None, spmv_frac=roof["frac"] if roof else None, roofline=roof)
 
 
@@ -96,7 +114,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
-    ap.add_argument("--precision", default="all", choices=["fp64", "mixed", "fp32", "all"])
+    ap.add_argument("--precision", default="all", choices=["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned", "all"])
+    ap.add_argument("--learned-budget", type=int, default=2500)
     ap.add_argument("--pcg-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -110,6 +129,10 @@ def main():
     sd = scenes.scene_c2() if args.workload == "c2" else scenes.scene_c1()
 
     modes = ["fp64", "mixed", "fp32"] if args.precision == "all" else [args.precision]
+    if args.precision == "all":
+        from cipc import precond
+        if os.path.exists(precond.WEIGHTS):
+            modes += ["mixed_learned", "fp32_learned"]
     runs = {m: gpu_run(args, sd, m) for m in modes}
     reduced = [m for m in modes if m != "fp64"] or modes
     head = runs[max(reduced, key=lambda m: runs[m]["steps_per_s"])]
@@ -128,7 +151,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "mixed": "f32 blocks + f64", "fp32": "f32 + f64 refinement"}[head["precision"]],
+        "dtype": {"fp64": "f64", "mixed": "f32 blocks + f64", "fp32": "f32 + f64 refinement",
+                  "mixed_learned": "f32 blocks + f64, bf16 network", "fp32_learned": "f32 + f64 refinement, bf16 network"}[head["precision"]],
         "data": "synthetic",
         "mode": MODE[head["precision"]],
         "config": {"workload": sd.name, "vertices": sd.nV, "triangles": sd.nT, "cloth_vertices": sd.n_cloth,

@@ -190,9 +190,9 @@ class Simulator:
         self._check(self.lib.cipc_precond_set_adam(self.h, m.ctypes.data_as(_lib.F32), v.ctypes.data_as(_lib.F32), int(t), len(m)))
 
     def precond_info(self):
-        out = np.zeros(4, np.int32)
+        out = np.zeros(8, np.int32)
         self._check(self.lib.cipc_precond_info(self.h, _i(out)))
-        return dict(fallback=int(out[0]))
+        return dict(fallback=int(out[0]), solves=int(out[1]), fallbacks=int(out[2]), last_status=int(out[3]), last_iters=int(out[4]))
 
     def set_profiling(self, enable=True):
         self._check(self.lib.cipc_set_profiling(self.h, 1 if enable else 0))

@@ -441,6 +441,10 @@ int cipc_precond_set_adam(cipc_ctx* ctx, const float* m, const float* v, int32_t
 int cipc_precond_info(const cipc_ctx* ctx, int32_t* out) {
     if (!ctx || !out) return CIPC_E_INVALID;
     out[0] = ctx->c.lastPrecondFallback;
+    out[1] = (int32_t)ctx->c.precondSolves;
+    out[2] = (int32_t)ctx->c.precondFallbacks;
+    out[3] = ctx->c.lastLearnedRc;
+    out[4] = ctx->c.lastLearnedIters;
     return CIPC_OK;
 }
 

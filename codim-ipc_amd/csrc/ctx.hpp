@@ -117,6 +117,8 @@ struct Ctx {
     int precond = 0;
     int lpBudget = 3000;              // PCG iterations before the learned solve falls back
     int lastPrecondFallback = 0;
+    long precondSolves = 0, precondFallbacks = 0;   // learned solves and how many fell back
+    int lastLearnedRc = 0, lastLearnedIters = 0;     // outcome of the last learned attempt
 
     // host pinned scalars
     double* hScal = nullptr;           // 64 doubles

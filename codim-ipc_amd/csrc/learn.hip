@@ -322,9 +322,11 @@ __device__ __forceinline__ short bf16_bits(float f) {
     return (short)(u >> 16);
 }
 
+// eNext (optional): eNext = eIn + out, the edge-state update fused into the epilogue
 template <int K>
 __global__ __launch_bounds__(256) void k_lp_dense_mfma(int nrows, LpSeg s0, LpSeg s1, LpSeg s2, const float* __restrict__ W,
-                                                       const float* __restrict__ bias, int relu, float* __restrict__ out) {
+                                                       const float* __restrict__ bias, int relu, float* __restrict__ out,
+                                                       const float* __restrict__ eIn, float* __restrict__ eNext) {
     static_assert(K % 64 == 0 && K <= kLpMaxK, "segments are 64 wide");
     constexpr int PITCH = K + 4;   // shorts; keeps 8-byte alignment of every 4-k group
     __shared__ short Wt[kLpHidden * PITCH];
@@ -370,6 +372,7 @@ __global__ __launch_bounds__(256) void k_lp_dense_mfma(int nrows, LpSeg s0, LpSe
                 float y = acc[j][t] + bias[col];
                 if (relu) y = y > 0.f ? y : 0.f;
                 out[(long)row * kLpHidden + col] = y;
+                if (eNext) eNext[(long)row * kLpHidden + col] = eIn[(long)row * kLpHidden + col] + y;
             }
         }
 }
@@ -377,10 +380,12 @@ __global__ __launch_bounds__(256) void k_lp_dense_mfma(int nrows, LpSeg s0, LpSe
 // ------------------------------------------------------------------ G assembly / products
 __global__ void k_lp_assemble_G(int nL, int nV, int nnzb, const int* __restrict__ lEdge, const int* __restrict__ eBlock,
                                 const int* __restrict__ blkT, const int* __restrict__ diagBlk, const float* __restrict__ Gl,
-                                const float* __restrict__ raw, float* __restrict__ G) {
+                                const float* __restrict__ raw, float* __restrict__ G, int nLow, const int* __restrict__ posLE,
+                                const int* __restrict__ posLD, const int* __restrict__ posUE, const int* __restrict__ posUD,
+                                float* __restrict__ GL, float* __restrict__ GU) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nL) {
-        const int b = eBlock[lEdge[t]], tb = blkT[b];
+        const int b = eBlock[lEdge[t]], tb = blkT[b], pl = posLE[t], pu = posUE[t];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -388,44 +393,90 @@ __global__ void k_lp_assemble_G(int nL, int nV, int nnzb, const int* __restrict_
                 const float v = Gl[9l * t + 3 * a + c];
                 G[(long)(3 * a + c) * nnzb + b] = v;
                 G[(long)(3 * c + a) * nnzb + tb] = v;
+                GL[(long)(3 * a + c) * nLow + pl] = v;
+                GU[(long)(3 * c + a) * nLow + pu] = v;
             }
     } else if (t < nL + nV) {
-        const int i = t - nL, b = diagBlk[i];
+        const int i = t - nL, b = diagBlk[i], pl = posLD[i], pu = posUD[i];
         const float* r = raw + 6l * i;
         const float g[9] = {expf(r[0]), 0.f, 0.f, r[3], expf(r[1]), 0.f, r[4], r[5], expf(r[2])};
 #pragma unroll
-        for (int k = 0; k < 9; ++k) G[(long)k * nnzb + b] = g[k];
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                G[(long)(3 * a + c) * nnzb + b] = g[3 * a + c];
+                GL[(long)(3 * a + c) * nLow + pl] = g[3 * a + c];
+                GU[(long)(3 * c + a) * nLow + pu] = g[3 * a + c];
+            }
     }
 }
 
-// y_i = sum_{blocks b of row i, col <= i} G_b (s . x)_col   (s = nullptr: no scaling)
-__global__ void k_lp_apply_lower(int nV, const int* __restrict__ rp, const int* __restrict__ col, const float* __restrict__ G, int nnzb,
-                                 const double* __restrict__ s, const double* __restrict__ x, double* __restrict__ y) {
-    constexpr int LANES = 16;
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int row = (int)(gid / LANES), lane = (int)(gid % LANES);
-    double acc[3] = {0, 0, 0};
-    if (row < nV) {
-        for (int b = rp[row] + lane; b < rp[row + 1]; b += LANES) {
+// y_i = so_i . sum_b V_b (si . x)_col(b) over a compact block CSR (FP32 blocks, FP64
+// vectors); si / so optional diagonal scalings; when r != nullptr, per-workgroup partial
+// sums of r.y.  Fixed grid, rows mapped XCD-aware, next row group's bounds loaded ahead.
+template <int LANES, int BS, class V>
+__global__ __launch_bounds__(BS) void k_lp_spmv(int nrow, int rowsPerXcd, const int* __restrict__ rp, const int* __restrict__ col,
+                                                const float* __restrict__ val, int nnz, const double* __restrict__ si,
+                                                const V* __restrict__ x, const double* __restrict__ so, V* __restrict__ y,
+                                                const V* __restrict__ r, double* __restrict__ part) {
+    __shared__ double lds[BS / 64];
+    constexpr int RPG = BS / LANES;
+    const int xb = blockIdx.x % 8, jb = blockIdx.x / 8, nj = gridDim.x / 8;
+    const int lane = threadIdx.x % LANES;
+    const int rowBase = xb * rowsPerXcd + (int)threadIdx.x / LANES;
+    int g = jb;
+    int row = rowBase + g * RPG;
+    bool live = g * RPG < rowsPerXcd && row < nrow;
+    int beg = live ? rp[row] : 0, end = live ? rp[row + 1] : 0;
+    double dot = 0;
+    for (; g * RPG < rowsPerXcd; g += nj) {
+        const int gn = g + nj, rowN = rowBase + gn * RPG;
+        const bool liveN = gn * RPG < rowsPerXcd && rowN < nrow;
+        const int begN = liveN ? rp[rowN] : 0, endN = liveN ? rp[rowN + 1] : 0;
+        V acc[3] = {0, 0, 0};
+        for (int b = beg + lane; b < end; b += LANES) {
             const int c = col[b];
-            if (c > row) continue;
-            double xv[3];
+            V x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
+            if (si) {
+                x0 *= (V)si[3 * c];
+                x1 *= (V)si[3 * c + 1];
+                x2 *= (V)si[3 * c + 2];
+            }
+            V a[9];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) xv[k] = s ? s[3 * c + k] * x[3 * c + k] : x[3 * c + k];
+            for (int k = 0; k < 9; ++k) a[k] = (V)val[(long)k * nnz + b];
+            acc[0] += a[0] * x0 + a[1] * x1 + a[2] * x2;
+            acc[1] += a[3] * x0 + a[4] * x1 + a[5] * x2;
+            acc[2] += a[6] * x0 + a[7] * x1 + a[8] * x2;
+        }
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+        for (int o = LANES / 2; o > 0; o >>= 1)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) acc[a] += (double)G[(long)(3 * a + k) * nnzb + b] * xv[k];
+            for (int k = 0; k < 3; ++k) acc[k] += __shfl_xor(acc[k], o, LANES);
+        if (row < nrow && lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const V v = so ? (V)so[3 * row + k] * acc[k] : acc[k];
+                y[3 * row + k] = v;
+                if (r) dot += (double)r[3 * row + k] * (double)v;
+            }
+        }
+        row = rowN;
+        beg = begN;
+        end = endN;
+    }
+    if (part) {
+        double v = wave_sum(dot);
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0;
+            for (int w = 0; w < BS / 64; ++w) t += lds[w];
+            part[blockIdx.x] = t;
         }
     }
-#pragma unroll
-    for (int o = LANES / 2; o > 0; o >>= 1)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) acc[k] += __shfl_xor(acc[k], o, LANES);
-    if (row < nV && lane == 0)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) y[3 * row + k] = acc[k];
 }
+
 // z_j = s_j . sum_{blocks b of row j, col >= j} T_b y_col, T_b = G_b at upper positions
 // (they hold transposed lower blocks) and G_b^T on the diagonal.  Grid-stride over row
 // groups so that the optional per-workgroup partial sums of r.z stay few.
@@ -637,6 +688,33 @@ void lp_build_graph(Ctx& c) {
     std::vector<int> lS(nL), lD(nL);
     for (int l = 0; l < nL; ++l) { lS[l] = eSrc[lEdge[l]]; lD[l] = eDst[lEdge[l]]; }
     upload(lp.lowSrc, lS, c.s); upload(lp.lowDst, lD, c.s);
+    // compact lower (G) and upper (G^T) block CSRs over the same rows, columns ascending
+    std::vector<int> lowIdxOfEdge(nE, -1);
+    for (int l = 0; l < nL; ++l) lowIdxOfEdge[lEdge[l]] = l;
+    std::vector<int> Lrp(nV + 1, 0), Lcol, Urp(nV + 1, 0), Ucol, pLE(nL), pLD(nV), pUE(nL), pUD(nV);
+    for (int i = 0; i < nV; ++i) {
+        Lrp[i] = (int)Lcol.size();
+        for (int b = rp[i]; b < rp[i + 1] && col[b] <= i; ++b) {
+            if (col[b] == i) pLD[i] = (int)Lcol.size();
+            else pLE[lowIdxOfEdge[blkEdge[b]]] = (int)Lcol.size();
+            Lcol.push_back(col[b]);
+        }
+        Urp[i] = (int)Ucol.size();
+        for (int b = rp[i]; b < rp[i + 1]; ++b) {
+            if (col[b] < i) continue;
+            if (col[b] == i) pUD[i] = (int)Ucol.size();
+            else pUE[lowIdxOfEdge[eT[blkEdge[b]]]] = (int)Ucol.size();   // (i, j>i) holds G_ji^T
+            Ucol.push_back(col[b]);
+        }
+    }
+    Lrp[nV] = (int)Lcol.size();
+    Urp[nV] = (int)Ucol.size();
+    if (Lcol.size() != Ucol.size()) throw HipError("learned preconditioner: lower/upper pattern mismatch", -1);
+    lp.nLow = (int)Lcol.size();
+    upload(lp.gLrp, Lrp, c.s); upload(lp.gLcol, Lcol, c.s); upload(lp.gUrp, Urp, c.s); upload(lp.gUcol, Ucol, c.s);
+    upload(lp.posLE, pLE, c.s); upload(lp.posLD, pLD, c.s); upload(lp.posUE, pUE, c.s); upload(lp.posUD, pUD, c.s);
+    lp.GL.ensure_exact(9 * (size_t)lp.nLow);
+    lp.GU.ensure_exact(9 * (size_t)lp.nLow);
     CIPC_HIP(hipStreamSynchronize(c.s));
     lp.nV = nV; lp.nnzb = nnzb; lp.nE = nE; lp.nL = nL;
     const size_t H = kLpHidden;
@@ -796,12 +874,12 @@ void lp_forward(Ctx& c) {
         const LpSeg ms0 = {lp.h[l].p, lp.eSrc.p, kLpHidden}, ms1 = {lp.h[l].p, lp.eDst.p, kLpHidden}, ms2 = {lp.e[l].p, nullptr, kLpHidden};
         if (lp.mfma && nE > 0) {
             hipLaunchKernelGGL(k_lp_dense_mfma<3 * kLpHidden>, dim3((nE + 63) / 64), dim3(256), 0, c.s, nE, ms0, ms1, ms2, Wq + L.wm[l],
-                               Wq + L.bm[l], 1, lp.m[l].p);
+                               Wq + L.bm[l], 1, lp.m[l].p, (const float*)lp.e[l].p, lp.e[l + 1].p);
             CIPC_LAUNCH_CHECK();
         } else {
             dense<kLpHidden>(c, nE, ms0, ms1, ms2, Wq + L.wm[l], Wq + L.bm[l], 1, nullptr, lp.m[l].p);
+            add(c, H * nE, lp.e[l].p, lp.m[l].p, lp.e[l + 1].p);
         }
-        add(c, H * nE, lp.e[l].p, lp.m[l].p, lp.e[l + 1].p);
         hipLaunchKernelGGL(k_lp_agg, dim3((nV + 3) / 4), dim3(256), 0, c.s, nV, (const int*)lp.erp.p, (const float*)lp.m[l].p, lp.agg[l].p);
         CIPC_LAUNCH_CHECK();
         dense<kLpHidden>(c, nV, {lp.h[l].p, nullptr, kLpHidden}, {lp.agg[l].p, nullptr, kLpHidden}, kNoSeg, Wq + L.wu[l], Wq + L.bu[l], 1,
@@ -814,7 +892,8 @@ void lp_forward(Ctx& c) {
     dense<6>(c, nV, {hL, nullptr, kLpHidden}, kNoSeg, kNoSeg, Wq + L.wd, Wq + L.bd, 0, nullptr, lp.raw.p);
     hipLaunchKernelGGL(k_lp_assemble_G, dim3(nblocks(nL + nV)), dim3(kBlock), 0, c.s, nL, nV, lp.nnzb, (const int*)lp.lEdge.p,
                        (const int*)lp.eBlock.p, (const int*)lp.blkT.p, (const int*)lp.diagBlk.p, (const float*)lp.Gl.p,
-                       (const float*)lp.raw.p, lp.Gblk.p);
+                       (const float*)lp.raw.p, lp.Gblk.p, lp.nLow, (const int*)lp.posLE.p, (const int*)lp.posLD.p, (const int*)lp.posUE.p,
+                       (const int*)lp.posUD.p, lp.GL.p, lp.GU.p);
     CIPC_LAUNCH_CHECK();
 }
 
@@ -826,19 +905,27 @@ void lp_prepare(Ctx& c) {
     lp_forward(c);
 }
 
-int lp_apply_partials(int nV) { return std::min(nblocks(16l * nV), 512); }
+constexpr int kLpSpmvLanes = 8, kLpSpmvBlock = 256, kLpSpmvGrid = 1024;
+int lp_apply_partials(int) { return kLpSpmvGrid; }
 
-int lp_apply(Ctx& c, const double* r, double* z, double* tmp, double* part) {
+template <class V>
+int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part) {
     LearnedPrecond& lp = LP(c);
-    BSR& E = c.Ael;
-    const int nb = nblocks(16l * lp.nV), np = lp_apply_partials(lp.nV);
-    hipLaunchKernelGGL(k_lp_apply_lower, dim3(nb), dim3(kBlock), 0, c.s, lp.nV, (const int*)E.rowptr.p, (const int*)E.col.p,
-                       (const float*)lp.Gblk.p, lp.nnzb, (const double*)lp.S.p, r, tmp);
-    hipLaunchKernelGGL(k_lp_apply_upper, dim3(np), dim3(kBlock), 0, c.s, lp.nV, (const int*)E.rowptr.p, (const int*)E.col.p,
-                       (const float*)lp.Gblk.p, lp.nnzb, (const double*)lp.S.p, (const double*)tmp, z, r, part);
+    constexpr int RPG = kLpSpmvBlock / kLpSpmvLanes;
+    int rpx = (lp.nV + 7) / 8;
+    rpx = (rpx + RPG - 1) / RPG * RPG;
+    // tmp = G (S r);  z = S G^T tmp  (+ partial sums of r.z)
+    hipLaunchKernelGGL((k_lp_spmv<kLpSpmvLanes, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, lp.nV, rpx,
+                       (const int*)lp.gLrp.p, (const int*)lp.gLcol.p, (const float*)lp.GL.p, lp.nLow, (const double*)lp.S.p, r,
+                       (const double*)nullptr, tmp, (const V*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL((k_lp_spmv<kLpSpmvLanes, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, lp.nV, rpx,
+                       (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const float*)lp.GU.p, lp.nLow, (const double*)nullptr,
+                       (const V*)tmp, (const double*)lp.S.p, z, r, part);
     CIPC_LAUNCH_CHECK();
-    return np;
+    return kLpSpmvGrid;
 }
+template int lp_apply<double>(Ctx&, const double*, double*, double*, double*);
+template int lp_apply<float>(Ctx&, const float*, float*, float*, double*);
 
 double lp_loss_grad(Ctx& c, int nProbes, uint64_t seed, float* gradOut) {
     LearnedPrecond& lp = LP(c);

@@ -43,6 +43,11 @@ struct LearnedPrecond {
     int nV = 0, nnzb = 0, nE = 0, nL = 0;
     // static graph of the elastic BSR pattern (directed edges = off-diagonal blocks)
     DevBuf<int> eBlock, eSrc, eDst, erp, eT, lEdge, blkT, diagBlk, lSrcOff, lSrcIdx, lDstOff, lDstIdx, lowSrc, lowDst;
+    // compact block-CSR of G (lower triangle incl. diagonal) and of G^T (upper), for the
+    // two products of the preconditioner; pos* map lower edges / diagonals to their slots
+    int nLow = 0;
+    DevBuf<int> gLrp, gLcol, gUrp, gUcol, posLE, posLD, posUE, posUD;
+    DevBuf<float> GL, GU;             // SoA [9][nLow]
     // parameters
     DevBuf<float> W, Wq, gW, mW, vW;
     int adamT = 0;
@@ -75,9 +80,10 @@ double lp_train_step(Ctx& c, int nProbes, uint64_t seed, double lr);
 double lp_loss_grad(Ctx& c, int nProbes, uint64_t seed, float* gradOut);
 // features of the current Hessian + forward pass (fills Gblk)
 void lp_prepare(Ctx& c);
-// z = S G^T G S r (FP64 vectors); tmp holds 3 nV doubles; when part != nullptr the
-// per-workgroup partial sums of r.z are written there (returns the partial count)
-int lp_apply(Ctx& c, const double* r, double* z, double* tmp, double* part);
+// z = S G^T G S r (V = double or float vectors); tmp holds 3 nV values; when part !=
+// nullptr the per-workgroup partial sums of r.z are written there (returns their count)
+template <class V>
+int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part);
 int lp_apply_partials(int nV);
 
 }  // namespace cipc

@@ -378,15 +378,13 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     // r.z partials: the update kernel's (block Jacobi) or the preconditioner product's
     const double* partRZ = learned ? partP : partU;
     const int nRZ = learned ? nP : nU;
-    double* ltmp = nullptr;
+    V* ltmp = nullptr;
 
     hipLaunchKernelGGL((k_pcg_init<V, kUpdBlock>), dim3(nU), dim3(kUpdBlock), 0, c.s, nV, rpxU, b, xsol, r, z, dinv, partU, flags);
     CIPC_LAUNCH_CHECK();
-    if constexpr (sizeof(V) == sizeof(double)) {
-        if (learned) {
-            ltmp = c.lpTmp.ensure((size_t)n);
-            lp_apply(c, (const double*)r, (double*)z, ltmp, partP);
-        }
+    if (learned) {
+        ltmp = (V*)c.lpTmp.ensure((size_t)n);
+        lp_apply<V>(c, (const V*)r, z, ltmp, partP);
     }
 
     const int chunk = 16;
@@ -415,10 +413,10 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
             if (!learned) {
                 hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock, true>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z,
                                    (const V*)pnew, (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
-            } else if constexpr (sizeof(V) == sizeof(double)) {
+            } else {
                 hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock, false>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z,
                                    (const V*)pnew, (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
-                lp_apply(c, (const double*)r, (double*)z, ltmp, partP);
+                lp_apply<V>(c, (const V*)r, z, ltmp, partP);
             }
         }
         CIPC_LAUNCH_CHECK();
@@ -489,11 +487,10 @@ static int refine(Ctx& c, const M* vE, const M* vK, const V* dinv, const double*
     constexpr int kMaxRefine = 8;
     // FP32 Krylov recurrences stall near 1e-6 relative; FP64 ones do not
     const double floorTol = sizeof(V) == sizeof(float) ? 1e-6 : 1e-12;
-    // aim below the target: the inner solve's residual drifts from the true FP64 residual
-    // by the rounding of the FP32 blocks, and a restarted sweep costs far more iterations
-    // than overshooting.  FP64 Krylov vectors reach a tighter inner target cheaply; FP32
-    // ones slow down near their floor, so they aim just below the target.
-    const double kInnerMargin = sizeof(V) == sizeof(double) ? 0.3 : 0.9;
+    // aim just below the target: the inner residual drifts from the true FP64 residual by
+    // the rounding of the FP32 blocks, and pushing the inner solve further costs more
+    // iterations than the short second sweep it would save (measured on the 320k stack)
+    constexpr double kInnerMargin = 0.9;
     for (int k = 0; k < kMaxRefine; ++k) {
         const double inner = std::max(floorTol, std::min(0.5, kInnerMargin * tol * bnorm / rnorm));
         const V* rhs;
@@ -561,26 +558,29 @@ static int solve_form(Ctx& c, const double* b, double* xsol, double tol, int max
                        K.val.p, K.nnzb, dinv);
     CIPC_LAUNCH_CHECK();
     return refine<float, float>(c, vE, vK, dinv, b, xsol, tol, maxIter, iters, relres, c.fr.ensure(n), c.fz.ensure(n), c.fp.ensure(2 * n),
-                                c.fq.ensure(n), c.fx.ensure(n), c.fb.ensure(n), false);
+                                c.fq.ensure(n), c.fx.ensure(n), c.fb.ensure(n), learned);
 }
 
 // The learned preconditioner runs with an iteration budget; when it does not converge
 // within it (or meets a non-positive curvature) the solve restarts with block Jacobi.
-// The all-FP32 form has no learned variant (the preconditioner product is FP64).
 int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
     c.lastPrecondFallback = 0;
-    const bool learned = c.precond == 1 && c.P.precision != 2;
+    const bool learned = c.precond == 1;
     if (!learned) return solve_form(c, b, xsol, tol, maxIter, iters, relres, false);
     lp_prepare(c);
+    c.precondSolves++;
     int it = 0;
     double rr = 0;
     int rc = solve_form(c, b, xsol, tol, std::min(maxIter, c.lpBudget), &it, &rr, true);
+    c.lastLearnedRc = rc;
+    c.lastLearnedIters = it;
     if (rc == 0) {
         if (iters) *iters = it;
         if (relres) *relres = rr;
         return 0;
     }
     c.lastPrecondFallback = 1;
+    c.precondFallbacks++;
     int it2 = 0;
     rc = solve_form(c, b, xsol, tol, maxIter, &it2, relres, false);
     if (iters) *iters = it + it2;

@@ -153,7 +153,10 @@ int cipc_precond_factor(cipc_ctx* ctx, const double* x, const int32_t* cs, int32
 /* Adam moments and step count, so that training can move between scenes (contexts) */
 int cipc_precond_get_adam(cipc_ctx* ctx, float* m, float* v, int32_t* t, int32_t n);
 int cipc_precond_set_adam(cipc_ctx* ctx, const float* m, const float* v, int32_t t, int32_t n);
-/* out[0] = 1 if the last solve fell back to block Jacobi */
+/* out[0] = 1 if the last solve fell back to block Jacobi; out[1], out[2] = learned
+ * solves so far and how many of them fell back; out[3], out[4] = status (0 converged,
+ * 1 non-positive curvature, 2 budget/refinement limit) and iterations of the last
+ * learned attempt */
 int cipc_precond_info(const cipc_ctx* ctx, int32_t* out);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------ */

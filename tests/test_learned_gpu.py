@@ -102,6 +102,32 @@ def test_untrained_learned_pcg_is_point_jacobi_and_converges(setup):
     assert np.linalg.norm(H @ sol - b) <= 2e-8 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp32"])
+def test_learned_refinement_forms(setup, precision):
+    """the learned preconditioner inside the FP64 refinement of the reduced-precision
+    forms meets the FP64 residual bar of the oracle's matrix"""
+    sd, gpu, orc, states = setup
+    x = states[1]
+    C = orc.constraint_set(x)
+    kappa = orc.scalars()["kappa"]
+    H = orc.hessian(x, C, kappa, which=3)
+    b = np.random.default_rng(6).standard_normal(x.size)
+    b.reshape(-1, 3)[sd.dbc_idx] = 0
+    random_weights(gpu, 13, scale=1e-3)   # near point Jacobi: the test is about the plumbing
+    gpu.set_precision(precision)
+    gpu.set_preconditioner("learned", budget=20000)
+    try:
+        sol, it, rr = gpu.linear_solve(x, C, kappa, b, tol=1e-8)
+        info = gpu.precond_info()
+        print(precision, "iterations", it, "relres", rr, info, gpu.solver_info())
+        assert info["fallback"] == 0, info
+    finally:
+        gpu.set_preconditioner("jacobi")
+        gpu.set_precision("fp64")
+    assert rr <= 1e-8
+    assert np.linalg.norm(H @ sol - b) <= 2e-8 * np.linalg.norm(b)
+
+
 def test_training_cuts_pcg_iterations(setup):
     """train on three states, evaluate on a fourth: fewer PCG iterations than block Jacobi"""
     sd, gpu, orc, states = setup
