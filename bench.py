@@ -47,6 +47,19 @@ MODE = {"fp64": "plain FP64, block-Jacobi PCG",
 PRECISION = {"fp64": "fp64", "mixed": "mixed", "fp32": "fp32", "mixed_learned": "mixed", "fp32_learned": "fp32"}
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed FETCH_SIZE /
+    WRITE_SIZE summary (profiles/), or None when no measurement matches."""
+    path = os.path.join(ROOT, "profiles", "k_pcg_spmv_c2_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    if not kernel.startswith(t["kernel"]) or t["workload"] != workload:
+        return None, None
+    return int(t["traffic_bytes_per_launch"]), t["summary"]
+
+
 def gpu_run(args, sd, form):
     import cipc
     import numpy as np
@@ -78,8 +91,10 @@ def gpu_run(args, sd, form):
     if prof["launches"]:
         avg_s = prof["total_ms"] / prof["launches"] / 1e3
         ach = prof["bytes_per_launch"] / avg_s / 1e9
+        traffic, source = measured_traffic(KERNEL[precision], sd.name)
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=None, kernel=KERNEL[precision], launches=prof["launches"], avg_us=round(avg_s * 1e6, 2),
+                    traffic=traffic, traffic_source=source, kernel=KERNEL[precision], launches=prof["launches"],
+                    avg_us=round(avg_s * 1e6, 2),
                     bytes_per_launch=int(prof["bytes_per_launch"]), nnzb=prof["nnzb"], block_rows=prof["block_rows"])
     return dict(precision=precision, steps_per_s=args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps,
                 ms_per_newton_iter=1e3 * elapsed / max(newton, 1), newton_iters_per_step=newton / args.steps,

@@ -49,6 +49,16 @@ class SceneDesc(ctypes.Structure):
         ("pcg_tol", ctypes.c_double),
         ("pcg_max_iter", ctypes.c_int32),
         ("with_collision", ctypes.c_int32),
+        ("n_rod_segments", ctypes.c_int32),
+        ("rod_segments", I32),
+        ("rod_density", ctypes.c_double),
+        ("rod_youngs", ctypes.c_double),
+        ("rod_bend_stiff_mult", ctypes.c_double),
+        ("rod_thickness", ctypes.c_double),
+        ("n_particles", ctypes.c_int32),
+        ("particles", I32),
+        ("particle_density", ctypes.c_double),
+        ("particle_thickness", ctypes.c_double),
     ]
 
 
@@ -85,6 +95,8 @@ PROTOTYPES = {
     "cipc_get_sizes": (ctypes.c_int, [P, I32]),
     "cipc_get_scalars": (ctypes.c_int, [P, D]),
     "cipc_get_setup": (ctypes.c_int, [P, D, I32, D, I32]),
+    "cipc_get_codim_sizes": (ctypes.c_int, [P, I32]),
+    "cipc_get_codim_setup": (ctypes.c_int, [P, D, I32, D, I32]),
     "cipc_set_state": (ctypes.c_int, [P, D, D]),
     "cipc_get_state": (ctypes.c_int, [P, D, D]),
     "cipc_set_kappa": (ctypes.c_int, [P, ctypes.c_double]),

@@ -100,6 +100,8 @@ class SceneData:
         self.dbc_target = np.ascontiguousarray(dbc_target, dtype=np.float64)
         self.params = dict(params)
         self.name = name
+        self.rods = None        # (n, 2) int32 rod segments or None
+        self.particles = None   # (n,) int32 particle vertex ids or None
 
     @property
     def nV(self):
@@ -196,3 +198,133 @@ def scene_cloth_on_ground(n=40, gap=6e-4, seed=9, twist_deg=0.0, ground_half=1.0
     sd.params["PNTol"] = 1e-2 * sd.bbox_diag()
     sd.n_cloth = n_cloth
     return sd
+
+
+# ---------------------------------------------------------------- codimensional scenes
+ROD_PARAMS = dict(rodRho=1200.0, rodE=1e7, rodBendMult=1.0, rodThick=1e-3)
+PARTICLE_MASS = 1e-5
+PARTICLE_PARAMS = dict(partThick=1e-3, partRho=PARTICLE_MASS / (math.pi * 1e-3 ** 3 / 6))
+
+
+def make_rod(p0, p1, nseg):
+    """Make_Rod (Rod/DISCRETE_ROD.h:6-28) between two points: nseg + 1 nodes, nseg segments."""
+    t = np.linspace(0.0, 1.0, nseg + 1)[:, None]
+    return (1 - t) * np.asarray(p0, float) + t * np.asarray(p1, float)
+
+
+def with_codim(sd, rod_nodes, particles, xi=1e-3, name=None, rod_params=None, particle_params=None):
+    """Append rods (list of node arrays, each a polyline) and particles ((n, 3) array) to a
+    scene: initialize_rod / initialize_particle of the reference's driver, offset xi."""
+    X = [sd.X]
+    nv = sd.nV
+    segs = []
+    for nodes in rod_nodes:
+        k = len(nodes)
+        X.append(np.asarray(nodes, float))
+        segs.append(np.stack([np.arange(nv, nv + k - 1), np.arange(nv + 1, nv + k)], 1))
+        nv += k
+    parts = np.arange(nv, nv + len(particles), dtype=np.int32)
+    X.append(np.asarray(particles, float).reshape(-1, 3))
+    out = SceneData(np.concatenate(X), sd.tris, sd.dbc_idx, sd.dbc_target, sd.params, name or sd.name)
+    out.n_cloth = getattr(sd, "n_cloth", 0)
+    out.rods = np.concatenate(segs).astype(np.int32) if segs else None
+    out.particles = parts if len(parts) else None
+    out.params.update(ROD_PARAMS if rod_params is None else rod_params)
+    out.params.update(PARTICLE_PARAMS if particle_params is None else particle_params)
+    out.params["xi"] = xi
+    out.params["PNTol"] = 1e-2 * out.bbox_diag()
+    return out
+
+
+def scene_codim_contact(seed=11):
+    """Every codimensional contact kind active in the first state (offset xi = 1 mm, so the
+    activation distance is 2 mm): a 12x12 sheet 1.5 mm above the sphere top, four rods of
+    eight segments lying 1.5 mm above the sheet and 1.5 mm apart, particles 1.5 mm above
+    the rods (point-edge) in pairs 1.5 mm apart (point-point)."""
+    base = cloth_on_sphere(sheets=1, n=12, y0=0.2025, seed=seed, sphere_level=3, name="codim_contact")
+    base.X[:base.n_cloth, 0] *= 0.1
+    base.X[:base.n_cloth, 2] *= 0.1
+    rng = np.random.default_rng(seed)
+    rods = []
+    for k in range(4):
+        z = -0.00225 + 0.0015 * k
+        y = 0.2040
+        rods.append(make_rod((-0.03, y, z), (0.03, y, z), 8) + rng.uniform(-5e-5, 5e-5, 3))
+    parts = []
+    for k in range(4):
+        z = -0.00225 + 0.0015 * k
+        for xk in (-0.012, 0.009):
+            parts.append((xk, 0.2055, z))
+            parts.append((xk + 0.0015, 0.2055, z))
+    parts = np.array(parts) + rng.uniform(-5e-5, 5e-5, (len(parts), 3))
+    return with_codim(base, rods, parts, name="codim_contact")
+
+
+def scene_codim_small(seed=13):
+    """A falling codimensional mix: a 12x12 sheet 3 mm above the sphere, 6 rods of 5 segments
+    4 mm above it, 30 particles 4 mm above the rods."""
+    base = cloth_on_sphere(sheets=1, n=12, y0=0.203, seed=seed, sphere_level=3, name="codim_small")
+    base.X[:base.n_cloth, 0] *= 0.2
+    base.X[:base.n_cloth, 2] *= 0.2
+    rng = np.random.default_rng(seed)
+    rods = [make_rod((-0.04, 0.207, -0.025 + 0.01 * k), (0.04, 0.207, -0.025 + 0.01 * k), 5) + rng.uniform(-1e-4, 1e-4, 3)
+            for k in range(6)]
+    g = np.stack(np.meshgrid(np.linspace(-0.04, 0.04, 6), [0.211], np.linspace(-0.02, 0.02, 5), indexing="ij"), -1).reshape(-1, 3)
+    parts = g + rng.uniform(-2e-4, 2e-4, g.shape)
+    return with_codim(base, rods, parts, name="codim_small")
+
+
+def scene_c3(seed=3, n_rods=20000, n_seg=10, n_particles=100000):
+    """C1 (41x41 cloth over the sphere and ground) + 20,000 rods (10 segments, 0.1 m, E 1e7)
+    + 100,000 particles (1e-5 kg) in a 0.3 m cube above the cloth, offset xi = 1 mm.
+    Rods run along x in two columns (x in [-0.145, -0.045] and [0.045, 0.145]) on a
+    100 x 100 (y, z) lattice of 3 mm pitch; particles fill the 9 cm gap between the
+    columns on a jittered 4.2 x 4.3 x 4.3 mm lattice.  Seeded jitter; the initial
+    separation of every pair of codimensional nodes / segments exceeds xi + d_hat = 2 mm
+    (checked by cipc.scenes.min_codim_separation)."""
+    base = scene_c1(seed=1)
+    rng = np.random.default_rng(seed)
+    lines = n_rods // 2
+    m = int(math.ceil(math.sqrt(lines)))
+    pitch = 0.3 / m
+    rods = []
+    for i in range(lines):
+        iy, iz = divmod(i, m)
+        y = 0.55 + (iy + 0.5) * pitch
+        z = -0.15 + (iz + 0.5) * pitch
+        for x0 in (-0.145, 0.045):
+            j = rng.uniform(-2e-4, 2e-4, 3)
+            rods.append(make_rod((x0, y, z), (x0 + 0.1, y, z), n_seg) + j)
+    nx, ny, nz = 20, 71, 71
+    gx = np.linspace(-0.04, 0.04, nx)
+    gy = 0.55 + (np.arange(ny) + 0.5) * (0.3 / ny)
+    gz = -0.15 + (np.arange(nz) + 0.5) * (0.3 / nz)
+    g = np.stack(np.meshgrid(gx, gy, gz, indexing="ij"), -1).reshape(-1, 3)[:n_particles]
+    parts = g + rng.uniform(-4e-4, 4e-4, g.shape)
+    return with_codim(base, rods, parts, name="c3_codim_mix")
+
+
+def min_codim_separation(sd):
+    """Smallest distance between a particle and any other vertex and between nodes of
+    different rods (a lower bound check of the initial separation)."""
+    from scipy.spatial import cKDTree
+    n0 = sd.nV - (0 if sd.particles is None else len(sd.particles))
+    tree = cKDTree(sd.X)
+    best = np.inf
+    if sd.particles is not None:
+        d, _ = tree.query(sd.X[sd.particles], k=2)
+        best = min(best, float(d[:, 1].min()))
+    if sd.rods is not None:
+        rod_of = -np.ones(sd.nV, dtype=np.int64)
+        rid = np.cumsum(np.r_[1, (sd.rods[1:, 0] != sd.rods[:-1, 1]).astype(int)]) - 1
+        rod_of[sd.rods[:, 0]] = rid
+        rod_of[sd.rods[:, 1]] = rid
+        nodes = np.nonzero(rod_of >= 0)[0]
+        t2 = cKDTree(sd.X[nodes])
+        pairs = t2.query_pairs(0.0025, output_type="ndarray")
+        if len(pairs):
+            other = rod_of[nodes[pairs[:, 0]]] != rod_of[nodes[pairs[:, 1]]]
+            if other.any():
+                dd = np.linalg.norm(sd.X[nodes[pairs[other, 0]]] - sd.X[nodes[pairs[other, 1]]], axis=1)
+                best = min(best, float(dd.min()))
+    return best

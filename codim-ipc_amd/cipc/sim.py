@@ -61,6 +61,24 @@ class Simulator:
         desc.pcg_tol = pcg_tol
         desc.pcg_max_iter = pcg_max_iter
         desc.with_collision = int(P.get("collision", 1))
+        rods = getattr(sd, "rods", None)
+        parts = getattr(sd, "particles", None)
+        if rods is not None and len(rods):
+            rods = np.ascontiguousarray(rods, dtype=np.int32)
+            self._keep.append(rods)
+            desc.n_rod_segments = len(rods)
+            desc.rod_segments = _i(rods)
+        if parts is not None and len(parts):
+            parts = np.ascontiguousarray(parts, dtype=np.int32)
+            self._keep.append(parts)
+            desc.n_particles = len(parts)
+            desc.particles = _i(parts)
+        desc.rod_density = P.get("rodRho", 0.0)
+        desc.rod_youngs = P.get("rodE", 0.0)
+        desc.rod_bend_stiff_mult = P.get("rodBendMult", 1.0)
+        desc.rod_thickness = P.get("rodThick", 1e-3)
+        desc.particle_density = P.get("partRho", 0.0)
+        desc.particle_thickness = P.get("partThick", 1e-3)
         h = ctypes.c_void_p()
         rc = self.lib.cipc_create(ctypes.byref(desc), ctypes.byref(h))
         if rc != 0:
@@ -70,6 +88,8 @@ class Simulator:
         s = np.zeros(6, dtype=np.int32)
         self.lib.cipc_get_sizes(self.h, _i(s))
         self.nT, self.nBN, self.nBE, self.nH = int(s[1]), int(s[2]), int(s[3]), int(s[4])
+        self.lib.cipc_get_codim_sizes(self.h, _i(s))
+        self.nRod, self.nRodHinge, self.nParticle, self.codimBN0, self.codimBN1 = [int(v) for v in s[:5]]
         if precision != "fp64":
             self.set_precision(precision)
 
@@ -101,6 +121,16 @@ class Simulator:
         bedge = np.zeros(2 * self.nBE, dtype=np.int32)
         self._check(self.lib.cipc_get_setup(self.h, _d(mass), _i(hinge), _d(hinfo), _i(bedge)))
         return dict(mass=mass, hinge=hinge.reshape(-1, 4), hinge_info=hinfo.reshape(-1, 3), bedge=bedge.reshape(-1, 2))
+
+    def codim_setup(self):
+        """Rod segment data, rod hinges and the boundary-node list (DISCRETE_ROD.h:87-165,
+        IMPLICIT_EULER.h:243-277)."""
+        ri = np.zeros(3 * self.nRod)
+        rh = np.zeros(3 * self.nRodHinge, dtype=np.int32)
+        rhi = np.zeros(3 * self.nRodHinge)
+        bn = np.zeros(self.nBN, dtype=np.int32)
+        self._check(self.lib.cipc_get_codim_setup(self.h, _d(ri), _i(rh), _d(rhi), _i(bn)))
+        return dict(rod_info=ri.reshape(-1, 3), rod_hinge=rh.reshape(-1, 3), rod_hinge_info=rhi.reshape(-1, 3), bnode=bn)
 
     def get_state(self):
         x = np.zeros(3 * self.nV)

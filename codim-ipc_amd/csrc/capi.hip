@@ -55,6 +55,15 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
         if (d->triangles[i] < 0 || d->triangles[i] >= d->n_vertices) return CIPC_E_INVALID;
     for (int i = 0; i < d->n_dbc; ++i)
         if (d->dbc_index[i] < 0 || d->dbc_index[i] >= d->n_vertices) return CIPC_E_INVALID;
+    if (d->n_rod_segments < 0 || (d->n_rod_segments && !d->rod_segments) || d->n_particles < 0 ||
+        (d->n_particles && !d->particles))
+        return CIPC_E_INVALID;
+    for (int i = 0; i < 2 * d->n_rod_segments; ++i)
+        if (d->rod_segments[i] < 0 || d->rod_segments[i] >= d->n_vertices) return CIPC_E_INVALID;
+    for (int i = 0; i < d->n_rod_segments; ++i)
+        if (d->rod_segments[2 * i] == d->rod_segments[2 * i + 1]) return CIPC_E_INVALID;
+    for (int i = 0; i < d->n_particles; ++i)
+        if (d->particles[i] < 0 || d->particles[i] >= d->n_vertices) return CIPC_E_INVALID;
     if (d->n_vertices >= (1 << 21) - 1) return CIPC_E_INVALID;   // contact-key packing limit
     cipc_ctx* ctx = new cipc_ctx();
     int rc = guarded(ctx, [&]() {
@@ -67,6 +76,10 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
         P.pcgTol = d->pcg_tol > 0 ? d->pcg_tol : 1e-5;
         P.pcgMaxIter = d->pcg_max_iter > 0 ? d->pcg_max_iter : 100000;
         P.collision = d->with_collision;
+        P.rodRho = d->rod_density; P.rodE = d->rod_youngs; P.rodBendMult = d->rod_bend_stiff_mult; P.rodThick = d->rod_thickness;
+        P.partRho = d->particle_density; P.partThick = d->particle_thickness;
+        c.hRod.assign(d->rod_segments, d->rod_segments + 2 * (size_t)d->n_rod_segments);
+        c.hPart.assign(d->particles, d->particles + (size_t)d->n_particles);
         CIPC_HIP(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
         CIPC_HIP(hipHostMalloc((void**)&c.hScal, 64 * sizeof(double), hipHostMallocDefault));
         CIPC_HIP(hipHostMalloc((void**)&c.hCnt, 64 * sizeof(uint32_t), hipHostMallocDefault));
@@ -125,6 +138,21 @@ int cipc_get_setup(const cipc_ctx* ctx, double* mass, int32_t* hinges, double* h
     if (hinges) std::memcpy(hinges, c.hHinge.data(), c.hHinge.size() * sizeof(int));
     if (hinge_info) std::memcpy(hinge_info, c.hHingeInfo.data(), c.hHingeInfo.size() * sizeof(double));
     if (bedges) std::memcpy(bedges, c.hBEdge.data(), c.hBEdge.size() * sizeof(int));
+    return CIPC_OK;
+}
+int cipc_get_codim_sizes(const cipc_ctx* ctx, int32_t* out) {
+    if (!ctx || !out) return CIPC_E_INVALID;
+    const Ctx& c = ctx->c;
+    out[0] = c.nRod; out[1] = c.nRH; out[2] = c.nPart; out[3] = c.codimBN0; out[4] = c.codimBN1;
+    return CIPC_OK;
+}
+int cipc_get_codim_setup(const cipc_ctx* ctx, double* rod_info, int32_t* rod_hinges, double* rod_hinge_info, int32_t* bnodes) {
+    if (!ctx) return CIPC_E_INVALID;
+    const Ctx& c = ctx->c;
+    if (rod_info) std::memcpy(rod_info, c.hRodInfo.data(), c.hRodInfo.size() * sizeof(double));
+    if (rod_hinges) std::memcpy(rod_hinges, c.hRodHinge.data(), c.hRodHinge.size() * sizeof(int));
+    if (rod_hinge_info) std::memcpy(rod_hinge_info, c.hRodHingeInfo.data(), c.hRodHingeInfo.size() * sizeof(double));
+    if (bnodes) std::memcpy(bnodes, c.hBNode.data(), c.hBNode.size() * sizeof(int));
     return CIPC_OK;
 }
 int cipc_set_state(cipc_ctx* ctx, const double* x, const double* v) {

@@ -123,6 +123,77 @@ __device__ bool ee_accd(d3 ea0, d3 ea1, d3 eb0, d3 eb1, d3 dea0, d3 dea1, d3 deb
     return true;
 }
 
+// CCD.h:237-277
+__device__ __forceinline__ bool pe_ccd_broadphase(d3 p, d3 e0, d3 e1, d3 dp, d3 de0, d3 de1, double dist) {
+    d3 pe = p + dp, q0 = e0 + de0, q1 = e1 + de1;
+    double maxp, minp, mx, mn;
+    maxp = dmax(p.x, pe.x); minp = dmin(p.x, pe.x);
+    mx = dmax(dmax(dmax(e0.x, e1.x), q0.x), q1.x); mn = dmin(dmin(dmin(e0.x, e1.x), q0.x), q1.x);
+    if (minp - mx > dist || mn - maxp > dist) return false;
+    maxp = dmax(p.y, pe.y); minp = dmin(p.y, pe.y);
+    mx = dmax(dmax(dmax(e0.y, e1.y), q0.y), q1.y); mn = dmin(dmin(dmin(e0.y, e1.y), q0.y), q1.y);
+    if (minp - mx > dist || mn - maxp > dist) return false;
+    maxp = dmax(p.z, pe.z); minp = dmin(p.z, pe.z);
+    mx = dmax(dmax(dmax(e0.z, e1.z), q0.z), q1.z); mn = dmin(dmin(dmin(e0.z, e1.z), q0.z), q1.z);
+    if (minp - mx > dist || mn - maxp > dist) return false;
+    return true;
+}
+__device__ __forceinline__ bool pp_ccd_broadphase(d3 p0, d3 p1, d3 dp0, d3 dp1, double dist) {
+    d3 q0 = p0 + dp0, q1 = p1 + dp1;
+    if (dmin(p0.x, q0.x) - dmax(p1.x, q1.x) > dist || dmin(p1.x, q1.x) - dmax(p0.x, q0.x) > dist) return false;
+    if (dmin(p0.y, q0.y) - dmax(p1.y, q1.y) > dist || dmin(p1.y, q1.y) - dmax(p0.y, q0.y) > dist) return false;
+    if (dmin(p0.z, q0.z) - dmax(p1.z, q1.z) > dist || dmin(p1.z, q1.z) - dmax(p0.z, q0.z) > dist) return false;
+    return true;
+}
+// CCD.h:397-441
+__device__ bool pe_accd(d3 p, d3 e0, d3 e1, d3 dp, d3 de0, d3 de1, double eta, double th, double& toc) {
+    d3 mov = {(dp.x + de0.x + de1.x) / 3, (dp.y + de0.y + de1.y) / 3, (dp.z + de0.z + de1.z) / 3};
+    de0 = de0 - mov; de1 = de1 - mov; dp = dp - mov;
+    double maxDispMag = sqrt(sq(dp)) + sqrt(dmax(sq(de0), sq(de1)));
+    if (maxDispMag == 0) return false;
+    double dist2_cur = dist2_pe_unclassified(p, e0, e1);
+    double dist_cur = sqrt(dist2_cur);
+    double gap = eta * (dist2_cur - th * th) / (dist_cur + th);
+    double toc_prev = toc;
+    double t = 0;
+    for (long it = 0;; ++it) {
+        if (it >= ACCD_MAX_ITERS) { toc = t; return true; }
+        double lb = (1 - eta) * (dist2_cur - th * th) / ((dist_cur + th) * maxDispMag);
+        p = p + lb * dp; e0 = e0 + lb * de0; e1 = e1 + lb * de1;
+        dist2_cur = dist2_pe_unclassified(p, e0, e1);
+        dist_cur = sqrt(dist2_cur);
+        if (t != 0 && (dist2_cur - th * th) / (dist_cur + th) < gap) break;
+        t += lb;
+        if (t > toc_prev) { toc = t; return false; }
+    }
+    toc = t;
+    return true;
+}
+// CCD.h:443-483
+__device__ bool pp_accd(d3 p0, d3 p1, d3 dp0, d3 dp1, double eta, double th, double& toc) {
+    d3 mov = {(dp0.x + dp1.x) / 2, (dp0.y + dp1.y) / 2, (dp0.z + dp1.z) / 2};
+    dp1 = dp1 - mov; dp0 = dp0 - mov;
+    double maxDispMag = sqrt(sq(dp0)) + sqrt(sq(dp1));
+    if (maxDispMag == 0) return false;
+    double dist2_cur = dist2_pp(p0, p1);
+    double dist_cur = sqrt(dist2_cur);
+    double gap = eta * (dist2_cur - th * th) / (dist_cur + th);
+    double toc_prev = toc;
+    double t = 0;
+    for (long it = 0;; ++it) {
+        if (it >= ACCD_MAX_ITERS) { toc = t; return true; }
+        double lb = (1 - eta) * (dist2_cur - th * th) / ((dist_cur + th) * maxDispMag);
+        p0 = p0 + lb * dp0; p1 = p1 + lb * dp1;
+        dist2_cur = dist2_pp(p0, p1);
+        dist_cur = sqrt(dist2_cur);
+        if (t != 0 && (dist2_cur - th * th) / (dist_cur + th) < gap) break;
+        t += lb;
+        if (t > toc_prev) { toc = t; return false; }
+    }
+    toc = t;
+    return true;
+}
+
 // ------------------------------------------------------------------ clamp inputs
 __global__ void k_edge_len(const double* __restrict__ x, const int* __restrict__ bEdge, int nBE, double* __restrict__ len) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,6 +435,105 @@ __global__ void k_ccd_ee(const double* __restrict__ x, const double* __restrict_
     if (largest < alpha0) atomic_min_pos(gmin, largest);
 }
 
+// particle (boundary-node slot s >= codimBN1) against rod segments, IPC.h:2095-2133,
+// and particle pairs, IPC.h:2135-2162 (a pair is run with the lower slot first, as the
+// reference does); same voxel-overlap candidate rule and cell coverage as k_ccd_ee
+__device__ __forceinline__ void pe_ccd_candidate(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bEdge,
+                                                 int nBE, const uint8_t* __restrict__ dbc, const int* __restrict__ elo,
+                                                 const int* __restrict__ ehi, int vI, const int* nlo, const int* nhi, d3 P, d3 dP, int e,
+                                                 double xi, double& largest) {
+    if (nlo[0] > ehi[e] || elo[e] > nhi[0] || nlo[1] > ehi[nBE + e] || elo[nBE + e] > nhi[1] || nlo[2] > ehi[2l * nBE + e] ||
+        elo[2l * nBE + e] > nhi[2])
+        return;
+    int a = bEdge[e], b = bEdge[nBE + e];
+    if (vI == a || vI == b) return;
+    if (dbc[vI] && dbc[a] && dbc[b]) return;
+    d3 e0 = ld3(x, a), e1 = ld3(x, b), de0 = ld3(p, a), de1 = ld3(p, b);
+    if (!pe_ccd_broadphase(P, e0, e1, dP, de0, de1, xi)) return;
+    double la = largest;
+    if (pe_accd(P, e0, e1, dP, de0, de1, 0.1, xi, la)) {
+        if (largest > la) largest = la;
+    }
+}
+__device__ __forceinline__ void pp_ccd_candidate(const double* __restrict__ x, const double* __restrict__ p, const uint8_t* __restrict__ dbc,
+                                                 const int* __restrict__ vlo, const int* __restrict__ vhi, int nV, int vA, int vB,
+                                                 double xi, double& largest) {
+    if (vlo[vA] > vhi[vB] || vlo[vB] > vhi[vA] || vlo[nV + vA] > vhi[nV + vB] || vlo[nV + vB] > vhi[nV + vA] ||
+        vlo[2 * nV + vA] > vhi[2 * nV + vB] || vlo[2 * nV + vB] > vhi[2 * nV + vA])
+        return;
+    if (dbc[vA] && dbc[vB]) return;
+    d3 A = ld3(x, vA), B = ld3(x, vB), dA = ld3(p, vA), dB = ld3(p, vB);
+    if (!pp_ccd_broadphase(A, B, dA, dB, xi)) return;
+    double la = largest;
+    if (pp_accd(A, B, dA, dB, 0.1, xi, la)) {
+        if (largest > la) largest = la;
+    }
+}
+__global__ void k_ccd_codim(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bNode, int nBN,
+                            int codimBN1, const int* __restrict__ bEdge, int nBE, int nRod, const uint8_t* __restrict__ dbc,
+                            const int* __restrict__ vlo, const int* __restrict__ vhi, int nV, const int* __restrict__ elo,
+                            const int* __restrict__ ehi, VoxGrid G, const uint32_t* __restrict__ estart, const uint32_t* __restrict__ eend,
+                            const uint32_t* __restrict__ evals, const int* __restrict__ lgListE, const uint32_t* __restrict__ lgCountE,
+                            const int* __restrict__ plo, const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ pend,
+                            const uint32_t* __restrict__ pvals, const uint8_t* __restrict__ lgFlagP, const int* __restrict__ lgListP,
+                            const uint32_t* __restrict__ lgCountP, double alpha0, double xi, unsigned long long* __restrict__ gmin) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = codimBN1 + j;
+    if (s >= nBN) return;
+    const int nPart = nBN - codimBN1, firstRod = nBE - nRod;
+    int vI = bNode[s];
+    int nlo[3], nhi[3], qlo[3], qhi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        nlo[d] = vlo[d * nV + vI]; nhi[d] = vhi[d * nV + vI];
+        qlo[d] = ecell(nlo[d], G.f, G.edim[d]); qhi[d] = ecell(nhi[d], G.f, G.edim[d]);
+    }
+    d3 P = ld3(x, vI), dP = ld3(p, vI);
+    double largest = alpha0;
+    const bool large = lgFlagP[j] != 0;
+    if (!large) {
+        for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
+            for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
+                for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
+                    uint64_t cell = (uint64_t)ix + (uint64_t)G.edim[0] * ((uint64_t)iy + (uint64_t)G.edim[1] * (uint64_t)iz);
+                    for (uint32_t k = estart[cell], en = eend[cell]; k < en; ++k) {
+                        int e = (int)evals[k];
+                        if (e < firstRod) continue;
+                        if (ix != max(qlo[0], ecell(elo[e], G.f, G.edim[0])) || iy != max(qlo[1], ecell(elo[nBE + e], G.f, G.edim[1])) ||
+                            iz != max(qlo[2], ecell(elo[2l * nBE + e], G.f, G.edim[2])))
+                            continue;
+                        pe_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, vI, nlo, nhi, P, dP, e, xi, largest);
+                    }
+                    for (uint32_t k = pstart[cell], en = pend[cell]; k < en; ++k) {
+                        int jJ = (int)pvals[k];
+                        if (jJ <= j) continue;
+                        if (ix != max(qlo[0], ecell(plo[jJ], G.f, G.edim[0])) || iy != max(qlo[1], ecell(plo[nPart + jJ], G.f, G.edim[1])) ||
+                            iz != max(qlo[2], ecell(plo[2 * nPart + jJ], G.f, G.edim[2])))
+                            continue;
+                        pp_ccd_candidate(x, p, dbc, vlo, vhi, nV, vI, bNode[codimBN1 + jJ], xi, largest);
+                    }
+                }
+    } else {
+        // a large particle meets the grid-listed rod segments only through the large-edge
+        // rule below; test every rod segment directly
+        for (int e = firstRod; e < nBE; ++e) pe_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, vI, nlo, nhi, P, dP, e, xi, largest);
+    }
+    if (!large) {
+        const uint32_t nle = *lgCountE;
+        for (uint32_t k = 0; k < nle; ++k) {
+            int e = lgListE[k];
+            if (e >= firstRod) pe_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, vI, nlo, nhi, P, dP, e, xi, largest);
+        }
+    }
+    const uint32_t nlp = *lgCountP;
+    for (uint32_t k = 0; k < nlp; ++k) {
+        int jJ = lgListP[k];
+        if (jJ > j) pp_ccd_candidate(x, p, dbc, vlo, vhi, nV, vI, bNode[codimBN1 + jJ], xi, largest);
+        else if (!large) pp_ccd_candidate(x, p, dbc, vlo, vhi, nV, bNode[codimBN1 + jJ], vI, xi, largest);
+    }
+    if (largest < alpha0) atomic_min_pos(gmin, largest);
+}
+
 static int bits_for2(uint64_t n) {
     int b = 1;
     while (b < 64 && (1ull << b) < n) ++b;
@@ -490,6 +660,17 @@ double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, doub
         hipLaunchKernelGGL(k_ccd_ee, dim3(nblocks(c.nBE)), dim3(kBlock), 0, c.s, x, p, c.bEdge.p, c.nBE, c.dbc.p, c.ccdElo.p, c.ccdEhi.p, G,
                            c.ecellStart.p, c.ecellEnd.p, c.u32d.p, c.lgFlagE.p, c.lgListE.p, lgCountE, alpha, xi, gmin,
                            c.counters.p + 13);
+        CIPC_LAUNCH_CHECK();
+    }
+    // particles against rod segments and particle pairs
+    if (c.nPart) {
+        uint32_t* lgCountP = c.counters.p + 21;
+        build_vox_cells(c, c.partSlot.p, c.nPart, 1, vlo, vhi, G, c.ccdPlo, c.ccdPhi, c.pKeys, c.pVals, c.pcellStart, c.pcellEnd,
+                        c.lgFlagP, c.lgListP, lgCountP);
+        hipLaunchKernelGGL(k_ccd_codim, dim3(nblocks(c.nPart)), dim3(kBlock), 0, c.s, x, p, c.bNode.p, c.nBN, c.codimBN1, c.bEdge.p, c.nBE,
+                           c.nRod, c.dbc.p, vlo, vhi, c.nV, c.ccdElo.p, c.ccdEhi.p, G, c.ecellStart.p, c.ecellEnd.p, c.u32d.p,
+                           c.lgListE.p, lgCountE, c.ccdPlo.p, c.pcellStart.p, c.pcellEnd.p, c.pVals.p, c.lgFlagP.p, c.lgListP.p,
+                           lgCountP, alpha, xi, gmin);
         CIPC_LAUNCH_CHECK();
     }
     CIPC_HIP(hipMemcpyAsync(c.hScal + 24, gmin, sizeof(double), hipMemcpyDeviceToHost, c.s));

@@ -313,6 +313,80 @@ __global__ void k_query_ee(const double* __restrict__ x, const double* __restric
         }
     }
 }
+// rod nodes and particles (boundary-node slots >= codimBN0) against rod segments (the
+// last nRod boundary edges), IPC.h:271-317; particles (slots >= codimBN1) against the
+// particles after them, IPC.h:319-349.  All results are PP/PE entries for the
+// multiplicity merge.
+__device__ __forceinline__ bool pe_cd_broadphase(d3 p, d3 e0, d3 e1, double dist) {
+    if (p.x - dmax(e0.x, e1.x) > dist || dmin(e0.x, e1.x) - p.x > dist) return false;
+    if (p.y - dmax(e0.y, e1.y) > dist || dmin(e0.y, e1.y) - p.y > dist) return false;
+    if (p.z - dmax(e0.z, e1.z) > dist || dmin(e0.z, e1.z) - p.z > dist) return false;
+    return true;
+}
+__device__ __forceinline__ void pe_codim_candidate(const CSOut& o, const double* __restrict__ x, const int* __restrict__ bEdge, int nBE,
+                                                   const uint8_t* __restrict__ dbc, int vI, d3 p, int eJ, double dHat, double dHat2,
+                                                   int nV) {
+    int a = bEdge[eJ], b = bEdge[nBE + eJ];
+    if (vI == a || vI == b) return;
+    if (dbc[vI] && dbc[a] && dbc[b]) return;
+    d3 e0 = ld3(x, a), e1 = ld3(x, b);
+    if (!pe_cd_broadphase(p, e0, e1, dHat)) return;
+    double d;
+    int c1, c2 = -1;
+    switch (pe_type(p, e0, e1)) {
+        case 0: d = dist2_pp(p, e0); c1 = a; break;
+        case 1: d = dist2_pp(p, e1); c1 = b; break;
+        default: d = dist2_pe(p, e0, e1); c1 = a; c2 = b; break;
+    }
+    if (d < dHat2) emit_dup(o, dup_key(-vI - 1, c1, c2, nV));
+}
+__global__ void k_query_codim(const double* __restrict__ x, const int* __restrict__ bNode, int nBN, int codimBN0, int codimBN1,
+                              const int* __restrict__ bEdge, int nBE, int nRod, const uint8_t* __restrict__ dbc, GridParams G,
+                              const uint32_t* __restrict__ estart, const uint32_t* __restrict__ eend, const uint32_t* __restrict__ evals,
+                              const int* __restrict__ edgeLo, const int* __restrict__ lgListE, const uint32_t* __restrict__ lgCountE,
+                              const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ pend, const uint32_t* __restrict__ pvals,
+                              double dHat, double dHat2, int nV, CSOut o) {
+    int s = codimBN0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nBN) return;
+    int vI = bNode[s];
+    d3 p = ld3(x, vI);
+    int qlo[3], qhi[3];
+    double pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { qlo[d] = cell_of(pc[d] - dHat, G.lb[d], G.inv, G.dim[d]); qhi[d] = cell_of(pc[d] + dHat, G.lb[d], G.inv, G.dim[d]); }
+    const int firstRod = nBE - nRod;
+    for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
+        for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
+            for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
+                uint64_t cell = (uint64_t)ix + (uint64_t)G.dim[0] * ((uint64_t)iy + (uint64_t)G.dim[1] * (uint64_t)iz);
+                for (uint32_t k = estart[cell], e = eend[cell]; k < e; ++k) {
+                    int eJ = (int)evals[k];
+                    if (eJ < firstRod) continue;
+                    if (ix != max(qlo[0], edgeLo[eJ]) || iy != max(qlo[1], edgeLo[nBE + eJ]) || iz != max(qlo[2], edgeLo[2 * nBE + eJ])) continue;
+                    pe_codim_candidate(o, x, bEdge, nBE, dbc, vI, p, eJ, dHat, dHat2, nV);
+                }
+            }
+    const uint32_t nl = *lgCountE;
+    for (uint32_t k = 0; k < nl; ++k) {
+        int eJ = lgListE[k];
+        if (eJ >= firstRod) pe_codim_candidate(o, x, bEdge, nBE, dbc, vI, p, eJ, dHat, dHat2, nV);
+    }
+    if (s < codimBN1) return;
+    // a point lies in exactly one cell, so each particle pair is met once
+    for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
+        for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
+            for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
+                uint64_t cell = (uint64_t)ix + (uint64_t)G.dim[0] * ((uint64_t)iy + (uint64_t)G.dim[1] * (uint64_t)iz);
+                for (uint32_t k = pstart[cell], e = pend[cell]; k < e; ++k) {
+                    int sJ = codimBN1 + (int)pvals[k];
+                    if (sJ <= s) continue;
+                    int vJ = bNode[sJ];
+                    if (dbc[vI] && dbc[vJ]) continue;
+                    double d = dist2_pp(p, ld3(x, vJ));
+                    if (d < dHat2) emit_dup(o, dup_key(-vI - 1, vJ, -1, nV));
+                }
+            }
+}
 #pragma clang fp contract(on)
 
 // gather sorted kept entries into the SoA constraint array
@@ -374,6 +448,9 @@ void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
     uint32_t* lgCountE = c.counters.p + 17;
     build_cells(c, x, c.tri.p, c.nT, 3, G, c.triLo, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT, lgCountT);
     build_cells(c, x, c.bEdge.p, c.nBE, 2, G, c.edgeLo, c.k64b, c.u32d, c.ecellStart, c.ecellEnd, c.lgFlagE, c.lgListE, lgCountE);
+    if (c.nPart)
+        build_cells(c, x, c.partSlot.p, c.nPart, 1, G, c.pLo, c.pKeys, c.pVals, c.pcellStart, c.pcellEnd, c.lgFlagP, c.lgListP,
+                    c.counters.p + 20);
 
     // narrow phase with capacity retry
     CSOut o;
@@ -396,6 +473,13 @@ void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
                            c.ecellStart.p, c.ecellEnd.p, c.u32d.p, c.edgeLo.p, c.lgFlagE.p, c.lgListE.p, lgCountE, dHat, dHat2, c.nV,
                            oEE);
         CIPC_LAUNCH_CHECK();
+        if (c.nBN > c.codimBN0) {
+            hipLaunchKernelGGL(k_query_codim, dim3(nblocks(c.nBN - c.codimBN0)), dim3(kBlock), 0, c.s, x, c.bNode.p, c.nBN, c.codimBN0,
+                               c.codimBN1, c.bEdge.p, c.nBE, c.nRod, c.dbc.p, G, c.ecellStart.p, c.ecellEnd.p, c.u32d.p, c.edgeLo.p,
+                               c.lgListE.p, lgCountE, c.nPart ? c.pcellStart.p : c.ecellStart.p, c.nPart ? c.pcellEnd.p : c.ecellEnd.p,
+                               c.nPart ? c.pVals.p : c.u32d.p, dHat, dHat2, c.nV, o);
+            CIPC_LAUNCH_CHECK();
+        }
         CIPC_HIP(hipMemcpyAsync(c.hCnt, c.counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
         CIPC_HIP(hipStreamSynchronize(c.s));
         uint32_t nPT = c.hCnt[0], nDup = c.hCnt[1], nEE = c.hCnt[2];

@@ -21,6 +21,9 @@ struct Params {
     int pcgMaxIter = 100000;
     int collision = 1;
     int precision = 0;            // 0: FP64 PCG; 1: FP32 blocks + FP64 vectors; 2: all-FP32 PCG (1, 2 inside FP64 refinement)
+    // rods and particles
+    double rodRho = 0, rodE = 0, rodBendMult = 1, rodThick = 1e-3;
+    double partRho = 0, partThick = 1e-3;
 };
 
 // spatial grid used by the constraint-set broad phase and the CCD candidate search
@@ -56,6 +59,22 @@ struct Ctx {
     // host mirrors of the topology
     std::vector<int> hTri, hHinge, hBNode, hBEdge, hDbcIdx;
     std::vector<double> hDbcTarget, hMass, hHingeInfo, hBinv, hVol;
+    // codimensional primitives: rod segments are the last nRod boundary edges, rod nodes
+    // occupy boundary-node slots [codimBN0, codimBN1), particles [codimBN1, nBN)
+    int nRod = 0, nRH = 0, nPart = 0, codimBN0 = 0, codimBN1 = 0;
+    std::vector<int> hRod, hRodHinge, hPart;
+    std::vector<double> hRodInfo, hRodHingeInfo;
+    DevBuf<int> rod;          // SoA [2][nRod]
+    DevBuf<double> rodPar;    // SoA [3][nRod]: E, rest length, weight h^2 pi t^2 / 4
+    DevBuf<int> rodHinge;     // SoA [3][nRH]
+    DevBuf<double> rodHW;     // bending weight h^2 E t^4 pi / (64 lsum)
+    DevBuf<int> partSlot;     // particles as point primitives (vertex ids, [1][nPart])
+    DevBuf<int> pLo;
+    DevBuf<uint64_t> pKeys;
+    DevBuf<uint32_t> pVals, pcellStart, pcellEnd;
+    DevBuf<uint8_t> lgFlagP;
+    DevBuf<int> lgListP;
+    DevBuf<int> ccdPlo, ccdPhi;
 
     // device state
     DevBuf<double> x, xn, xt, v, xprev, X0, mass, p, g, rhs;

@@ -119,6 +119,16 @@ CIPC_DEV int ee_type(d3 a0, d3 a1, d3 b0, d3 b1) {
     }
     return defaultCase;
 }
+// DISTANCE_UNCLASSIFIED.h:125-148
+CIPC_DEV double dist2_pe_unclassified(d3 p, d3 e0, d3 e1) {
+    d3 v = e1 - e0, w = p - e0;
+    double c1 = dot(w, v);
+    if (c1 <= 0.0) return dist2_pp(p, e0);
+    double c2 = sq(v);
+    if (c2 <= c1) return dist2_pp(p, e1);
+    double b = c1 / c2;
+    return dist2_pp(p, e0 + b * v);
+}
 CIPC_DEV double dist2_pt_unclassified(d3 p, d3 t0, d3 t1, d3 t2) {
     switch (pt_type(p, t0, t1, t2)) {
         case 0: return dist2_pp(p, t0);
@@ -622,6 +632,97 @@ CIPC_DEV void dihedral_angle_hess(d3 x0, d3 x1, d3 x2, d3 x3, double* H) {
         for (int j = i + 1; j < 12; ++j) {
             double s = 0.5 * (H[i * 12 + j] + H[j * 12 + i]);
             H[i * 12 + j] = s; H[j * 12 + i] = s;
+        }
+}
+
+// ------------------------------------------------------------- rods
+// Segment spring, Rod/ROD_STRETCHING.h:10-107: E = Er / (2 lbar) (l - lbar)^2 with
+// l = |x0 - x1| (the caller multiplies by h^2 pi t^2 / 4).  Gradient, and the Hessian
+// already projected to PSD in closed form: [[K, -K], [-K, K]] with
+// K = Er / lbar (n n^T + max(0, 1 - lbar / l) (I - n n^T)) -- the block structure has
+// eigenvalues 2 Er / lbar, 2 Er / lbar (1 - lbar / l) (twice) and 0 (three times), so
+// clamping the negative ones (makePD, Math/UTILS.h:10-27) is exactly this.
+CIPC_DEV double rod_spring_energy(d3 x0, d3 x1, double Er, double lbar) {
+    double l = sqrt(sq(x0 - x1));
+    double r = l / lbar - 1;
+    return lbar * Er / 2 * r * r;
+}
+CIPC_DEV void rod_spring_grad(d3 x0, d3 x1, double Er, double lbar, double* g) {
+    d3 d = x0 - x1;
+    double l = sqrt(sq(d));
+    double s = Er / lbar * (l - lbar) / l;
+    g[0] = s * d.x; g[1] = s * d.y; g[2] = s * d.z;
+    g[3] = -g[0]; g[4] = -g[1]; g[5] = -g[2];
+}
+// K (3x3, row-major) of the projected Hessian
+CIPC_DEV void rod_spring_hess_pd(d3 x0, d3 x1, double Er, double lbar, double* K) {
+    d3 d = x0 - x1;
+    double l = sqrt(sq(d));
+    double n[3] = {d.x / l, d.y / l, d.z / l};
+    double k = Er / lbar, perp = dmax(Er / lbar * (l - lbar) / l, 0.0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) K[3 * i + j] = (k - perp) * n[i] * n[j] + (i == j ? perp : 0.0);
+}
+
+// Rod hinge, Rod/ROD_BENDING.h:10-117: |kappa|^2 with kappa = 2 e0 x e1 / (|e0||e1| + e0.e1),
+// e0 = x1 - x0, e1 = x2 - x1.  Derivatives through |kappa|^2 = 4 (u - v) / (u + v),
+// u = |e0||e1|, v = e0.e1.
+CIPC_DEV double rod_kappa2(d3 x0, d3 x1, d3 x2) {
+    d3 e0 = x1 - x0, e1 = x2 - x1;
+    double den = sqrt(sq(e0) * sq(e1)) + dot(e0, e1);
+    d3 k = (2 / den) * cross(e0, e1);
+    return sq(k);
+}
+CIPC_DEV void rod_kappa2_grad(d3 x0, d3 x1, d3 x2, double* g) {
+    d3 e0 = x1 - x0, e1 = x2 - x1;
+    double a = sqrt(sq(e0)), b = sqrt(sq(e1)), c = dot(e0, e1);
+    double s = a * b + c, s2 = s * s;
+    double fu = 8 * c / s2, fv = -8 * (a * b) / s2;
+    d3 g0 = (fu * b / a) * e0 + fv * e1;
+    d3 g1 = (fu * a / b) * e1 + fv * e0;
+    g[0] = -g0.x; g[1] = -g0.y; g[2] = -g0.z;
+    g[3] = g0.x - g1.x; g[4] = g0.y - g1.y; g[5] = g0.z - g1.z;
+    g[6] = g1.x; g[7] = g1.y; g[8] = g1.z;
+}
+CIPC_DEV void rod_kappa2_hess(d3 x0, d3 x1, d3 x2, double* H) {
+    d3 e0v = x1 - x0, e1v = x2 - x1;
+    double e[2][3] = {{e0v.x, e0v.y, e0v.z}, {e1v.x, e1v.y, e1v.z}};
+    double a = sqrt(sq(e0v)), b = sqrt(sq(e1v)), c = dot(e0v, e1v);
+    double u = a * b, s = u + c, s2 = s * s, s3 = s2 * s;
+    double fu = 8 * c / s2, fv = -8 * u / s2, fuu = -16 * c / s3, fvv = 16 * u / s3, fuv = 8 * (u - c) / s3;
+    double len[2] = {a, b};
+    double gu[6], gv[6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        gu[i] = b / a * e[0][i]; gu[3 + i] = a / b * e[1][i];
+        gv[i] = e[1][i]; gv[3 + i] = e[0][i];
+    }
+    double He[36];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int bi = i / 3, bj = j / 3, k = i % 3, l = j % 3;
+            double d2u;
+            if (bi == bj) {
+                double lo = len[1 - bi], li = len[bi];
+                d2u = lo * ((k == l ? 1.0 : 0.0) / li - e[bi][k] * e[bi][l] / (li * li * li));
+            } else {
+                d2u = e[bi][k] / len[bi] * e[bj][l] / len[bj];
+            }
+            double d2v = (bi != bj && k == l) ? 1.0 : 0.0;
+            He[6 * i + j] = fu * d2u + fv * d2v + fuu * gu[i] * gu[j] + fvv * gv[i] * gv[j] + fuv * (gu[i] * gv[j] + gv[i] * gu[j]);
+        }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            double H00 = He[6 * k + l], H01 = He[6 * k + 3 + l], H10 = He[6 * (3 + k) + l], H11 = He[6 * (3 + k) + 3 + l];
+            H[9 * k + l] = H00;                  H[9 * k + 3 + l] = -H00 + H01;                   H[9 * k + 6 + l] = -H01;
+            H[9 * (3 + k) + l] = -H00 + H10;     H[9 * (3 + k) + 3 + l] = H00 - H01 - H10 + H11;  H[9 * (3 + k) + 6 + l] = H01 - H11;
+            H[9 * (6 + k) + l] = -H10;           H[9 * (6 + k) + 3 + l] = H10 - H11;              H[9 * (6 + k) + 6 + l] = H11;
         }
 }
 

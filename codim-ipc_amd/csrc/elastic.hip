@@ -1,5 +1,6 @@
 // Shell elasticity and inertia: Neo-Hookean membrane (FEM/Shell/MEMBRANE.h:7-290),
-// hinge bending (FEM/Shell/BENDING.h:52-75, 177-208, 441-496), inertia
+// hinge bending (FEM/Shell/BENDING.h:52-75, 177-208, 441-496), rod springs and rod
+// hinges (FEM/Shell/Rod/ROD_STRETCHING.h:10-107, ROD_BENDING.h:10-117), inertia
 // (FEM/Shell/INC_POTENTIAL.h:132-144, 250-269).  All carry the h^2 of the incremental
 // potential.  Triangles / hinges whose nodes are all Dirichlet contribute nothing.
 //
@@ -48,6 +49,26 @@ __global__ void k_bend_energy(const double* __restrict__ x, const int* __restric
     double s = block_sum_256(e, lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
+// rod springs (ROD_STRETCHING.h:10-28) and rod hinges (ROD_BENDING.h:10-34); par:
+// SoA [3][nR] = E, rest length, h^2 pi t^2 / 4; hw: h^2 E t^4 pi / (64 lsum)
+__global__ void k_rod_spring_energy(const double* __restrict__ x, const int* __restrict__ rv, int nR, const double* __restrict__ par,
+                                    double* __restrict__ partial) {
+    __shared__ double lds[4];
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (r < nR) e = par[2 * nR + r] * rod_spring_energy(ld3(x, rv[r]), ld3(x, rv[nR + r]), par[r], par[nR + r]);
+    double s = block_sum_256(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+__global__ void k_rod_bend_energy(const double* __restrict__ x, const int* __restrict__ hv3, int nRH, const double* __restrict__ hw,
+                                  double* __restrict__ partial) {
+    __shared__ double lds[4];
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (h < nRH) e = hw[h] * rod_kappa2(ld3(x, hv3[h]), ld3(x, hv3[nRH + h]), ld3(x, hv3[2 * nRH + h]));
+    double s = block_sum_256(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
 __global__ void k_inertia_energy(const double* __restrict__ x, const double* __restrict__ xt, const double* __restrict__ m, int nV,
                                  double* __restrict__ partial) {
     __shared__ double lds[4];
@@ -87,6 +108,21 @@ double elastic_energy(Ctx& c, const double* x, double* memb, double* bend) {
         hipLaunchKernelGGL(k_bend_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.hinge.p, c.nH, c.hInfo.p, c.kBend, h2, c.dbc.p, part);
         CIPC_LAUNCH_CHECK();
         Eb = finish_sum(c, part, nb, 4);
+    }
+    // rods: springs join the stretching energy, rod hinges the bending energy
+    if (c.nRod) {
+        int nb = nblocks(c.nRod);
+        double* part = c.red.ensure(nb);
+        hipLaunchKernelGGL(k_rod_spring_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.rod.p, c.nRod, c.rodPar.p, part);
+        CIPC_LAUNCH_CHECK();
+        Em += finish_sum(c, part, nb, 3);
+    }
+    if (c.nRH) {
+        int nb = nblocks(c.nRH);
+        double* part = c.red.ensure(nb);
+        hipLaunchKernelGGL(k_rod_bend_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.rodHinge.p, c.nRH, c.rodHW.p, part);
+        CIPC_LAUNCH_CHECK();
+        Eb += finish_sum(c, part, nb, 4);
     }
     if (memb) *memb = Em;
     if (bend) *bend = Eb;
@@ -141,6 +177,30 @@ __global__ void k_bend_grad_local(const double* __restrict__ x, const int* __res
 #pragma unroll
         for (int k = 0; k < 3; ++k) gl[k * ns + off + 4l * h + q] = g[3 * q + k];
 }
+__global__ void k_rod_grad_local(const double* __restrict__ x, const int* __restrict__ rv, int nR, const double* __restrict__ par,
+                                 double* __restrict__ gl, long ns, long off) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nR) return;
+    double g[6];
+    rod_spring_grad(ld3(x, rv[r]), ld3(x, rv[nR + r]), par[r], par[nR + r], g);
+    const double w = par[2 * nR + r];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gl[k * ns + off + 2l * r + q] = w * g[3 * q + k];
+}
+__global__ void k_rod_bend_grad_local(const double* __restrict__ x, const int* __restrict__ hv3, int nRH, const double* __restrict__ hw,
+                                      double* __restrict__ gl, long ns, long off) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nRH) return;
+    double g[9];
+    rod_kappa2_grad(ld3(x, hv3[h]), ld3(x, hv3[nRH + h]), ld3(x, hv3[2 * nRH + h]), g);
+    const double w = hw[h];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gl[k * ns + off + 3l * h + q] = w * g[3 * q + k];
+}
 // g[v] = sum of local slots + m (x - xt)
 __global__ void k_el_grad_gather(const double* __restrict__ gl, long ns, const int* __restrict__ off, const int* __restrict__ slot,
                                  const double* __restrict__ x, const double* __restrict__ xt, const double* __restrict__ m, int nV,
@@ -173,6 +233,16 @@ static void elastic_grad_impl(Ctx& c, const double* x, double* g, int withInerti
         double kb = c.P.bendMult != 0 ? c.kBend : 0.0;
         hipLaunchKernelGGL(k_bend_grad_local, dim3(nblocks(c.nH)), dim3(kBlock), 0, c.s, x, c.hinge.p, c.nH, c.hInfo.p, kb, h2, c.dbc.p,
                            gl, ns, 3l * c.nT);
+        CIPC_LAUNCH_CHECK();
+    }
+    if (c.nRod) {
+        hipLaunchKernelGGL(k_rod_grad_local, dim3(nblocks(c.nRod)), dim3(kBlock), 0, c.s, x, c.rod.p, c.nRod, c.rodPar.p, gl, ns,
+                           3l * c.nT + 4l * c.nH);
+        CIPC_LAUNCH_CHECK();
+    }
+    if (c.nRH) {
+        hipLaunchKernelGGL(k_rod_bend_grad_local, dim3(nblocks(c.nRH)), dim3(kBlock), 0, c.s, x, c.rodHinge.p, c.nRH, c.rodHW.p, gl, ns,
+                           3l * c.nT + 4l * c.nH + 2l * c.nRod);
         CIPC_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_el_grad_gather, dim3(nblocks(c.nV)), dim3(kBlock), 0, c.s, gl, ns, c.gplanOff.p, c.gplanSlot.p, x, c.xt.p,
@@ -244,6 +314,41 @@ __global__ void __launch_bounds__(64) k_bend_hess_local(const double* __restrict
 #pragma unroll
                 for (int l = 0; l < 3; ++l) hl[(long)(3 * k + l) * ns + off + 16l * h + 4 * p + q] = H[(3 * p + k) * 12 + 3 * q + l];
 }
+// rod segment s block (a, b) at off + 4 s + 2 a + b; rod hinge h block (a, b) at
+// off2 + 9 h + 3 a + b
+__global__ void k_rod_hess_local(const double* __restrict__ x, const int* __restrict__ rv, int nR, const double* __restrict__ par,
+                                 double* __restrict__ hl, long ns, long off) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nR) return;
+    double K[9];
+    rod_spring_hess_pd(ld3(x, rv[r]), ld3(x, rv[nR + r]), par[r], par[nR + r], K);
+    const double w = par[2 * nR + r];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const double sg = (p == q) ? w : -w;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) hl[(long)e * ns + off + 4l * r + 2 * p + q] = sg * K[e];
+        }
+}
+__global__ void __launch_bounds__(64) k_rod_bend_hess_local(const double* __restrict__ x, const int* __restrict__ hv3, int nRH,
+                                                            const double* __restrict__ hw, double* __restrict__ hl, long ns, long off) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nRH) return;
+    double H[81];
+    rod_kappa2_hess(ld3(x, hv3[h]), ld3(x, hv3[nRH + h]), ld3(x, hv3[2 * nRH + h]), H);
+    make_pd<9>(H);
+    const double w = hw[h];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) hl[(long)(3 * k + l) * ns + off + 9l * h + 3 * p + q] = w * H[(3 * p + k) * 9 + 3 * q + l];
+}
 __global__ void k_el_assemble(const double* __restrict__ hl, long ns, const int* __restrict__ off, const int* __restrict__ slot,
                               const int* __restrict__ rowOf, const int* __restrict__ col, int nnzb, const double* __restrict__ m,
                               const uint8_t* __restrict__ dbc, double* __restrict__ val) {
@@ -280,6 +385,16 @@ void elastic_hessian(Ctx& c, const double* x) {
                            hl, ns, 9l * c.nT);
         CIPC_LAUNCH_CHECK();
     }
+    if (c.nRod) {
+        hipLaunchKernelGGL(k_rod_hess_local, dim3(nblocks(c.nRod)), dim3(kBlock), 0, c.s, x, c.rod.p, c.nRod, c.rodPar.p, hl, ns,
+                           9l * c.nT + 16l * c.nH);
+        CIPC_LAUNCH_CHECK();
+    }
+    if (c.nRH) {
+        hipLaunchKernelGGL(k_rod_bend_hess_local, dim3(nblocks(c.nRH, 64)), dim3(64), 0, c.s, x, c.rodHinge.p, c.nRH, c.rodHW.p, hl, ns,
+                           9l * c.nT + 16l * c.nH + 4l * c.nRod);
+        CIPC_LAUNCH_CHECK();
+    }
     BSR& A = c.Ael;
     hipLaunchKernelGGL(k_el_assemble, dim3(nblocks(A.nnzb)), dim3(kBlock), 0, c.s, hl, ns, c.planOff.p, c.planSlot.p, A.rowOf.p,
                        A.col.p, A.nnzb, c.mass.p, c.dbc.p, A.val.p);
@@ -291,7 +406,8 @@ void build_elastic_plans(Ctx& c) {
     const int nT = c.nT, nH = c.nH, nV = c.nV;
     // gradient plan: slot -> vertex
     {
-        long ns = 3l * nT + 4l * nH;
+        const int nR = c.nRod, nRH = c.nRH;
+        long ns = 3l * nT + 4l * nH + 2l * nR + 3l * nRH;
         c.nGSlots = (int)ns;
         std::vector<std::pair<int, int>> vs;
         vs.reserve(ns);
@@ -299,6 +415,11 @@ void build_elastic_plans(Ctx& c) {
             for (int a = 0; a < 3; ++a) vs.push_back({c.hTri[3 * t + a], (int)(3l * t + a)});
         for (int h = 0; h < nH; ++h)
             for (int a = 0; a < 4; ++a) vs.push_back({c.hHinge[4 * h + a], (int)(3l * nT + 4l * h + a)});
+        const long oR = 3l * nT + 4l * nH, oRH = oR + 2l * nR;
+        for (int r = 0; r < nR; ++r)
+            for (int a = 0; a < 2; ++a) vs.push_back({c.hRod[2 * r + a], (int)(oR + 2l * r + a)});
+        for (int h = 0; h < nRH; ++h)
+            for (int a = 0; a < 3; ++a) vs.push_back({c.hRodHinge[3 * h + a], (int)(oRH + 3l * h + a)});
         std::sort(vs.begin(), vs.end());
         std::vector<int> off(nV + 1, 0), slot(ns);
         for (auto& p : vs) off[p.first + 1]++;
@@ -309,10 +430,20 @@ void build_elastic_plans(Ctx& c) {
     }
     // Hessian pattern and plan
     {
-        long ns = 9l * nT + 16l * nH;
+        const int nR = c.nRod, nRH = c.nRH;
+        long ns = 9l * nT + 16l * nH + 4l * nR + 9l * nRH;
         c.nElSlots = (int)ns;
         std::vector<std::pair<uint64_t, int>> ks;
         ks.reserve(ns + nV);
+        const long oR = 9l * nT + 16l * nH, oRH = oR + 4l * nR;
+        for (int r = 0; r < nR; ++r)
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b)
+                    ks.push_back({(uint64_t)c.hRod[2 * r + a] * nV + c.hRod[2 * r + b], (int)(oR + 4l * r + 2 * a + b)});
+        for (int h = 0; h < nRH; ++h)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    ks.push_back({(uint64_t)c.hRodHinge[3 * h + a] * nV + c.hRodHinge[3 * h + b], (int)(oRH + 9l * h + 3 * a + b)});
         for (int t = 0; t < nT; ++t)
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b)

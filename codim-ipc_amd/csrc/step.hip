@@ -132,6 +132,55 @@ void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri
     c.lam = P.E * P.nu / (1.0 - P.nu * P.nu);
     c.mu = P.E / (2.0 * (1.0 + P.nu));
     c.kBend = P.bendMult * (P.E * std::pow(P.thick, 3) / (24 * (1.0 - P.nu * P.nu)));
+    // rods, DISCRETE_ROD.h:87-165: per-segment E, rest length, thickness and lumped mass;
+    // a bending hinge per pair of segments sharing a node (std::map node order)
+    c.nRod = (int)c.hRod.size() / 2;
+    c.hRodInfo.assign(3 * (size_t)c.nRod, 0.0);
+    std::map<int, std::vector<int>> nbr;
+    for (int r = 0; r < c.nRod; ++r) {
+        int a = c.hRod[2 * r], b = c.hRod[2 * r + 1];
+        nbr[a].push_back(b);
+        nbr[b].push_back(a);
+        double pa[3], pb[3], d[3];
+        hv(X, a, pa); hv(X, b, pb); hsub(pa, pb, d);
+        double l = std::sqrt(hdot(d, d));
+        c.hRodInfo[3 * r] = P.rodE;
+        c.hRodInfo[3 * r + 1] = l;
+        c.hRodInfo[3 * r + 2] = P.rodThick;
+        double mp = l * M_PI * P.rodThick * P.rodThick / 4 * P.rodRho / 2;
+        c.hMass[a] += mp;
+        c.hMass[b] += mp;
+    }
+    c.hRodHinge.clear(); c.hRodHingeInfo.clear();
+    for (auto& kv : nbr) {
+        const std::vector<int>& n = kv.second;
+        for (size_t i = 0; i < n.size(); ++i)
+            for (size_t j = i + 1; j < n.size(); ++j) {
+                c.hRodHinge.push_back(n[i]); c.hRodHinge.push_back(kv.first); c.hRodHinge.push_back(n[j]);
+                double p0[3], p1[3], p2[3], d0[3], d1[3];
+                hv(X, n[i], p0); hv(X, kv.first, p1); hv(X, n[j], p2);
+                hsub(p1, p0, d0); hsub(p2, p1, d1);
+                c.hRodHingeInfo.push_back(P.rodE * P.rodBendMult);
+                c.hRodHingeInfo.push_back(std::sqrt(hdot(d0, d0)) + std::sqrt(hdot(d1, d1)));
+                c.hRodHingeInfo.push_back(P.rodThick);
+            }
+    }
+    c.nRH = (int)c.hRodHinge.size() / 3;
+    // particles, DISCRETE_PARTICLE.h:52-75
+    c.nPart = (int)c.hPart.size();
+    {
+        const double pm = 4.0 / 3.0 * M_PI * P.partThick * P.partThick * P.partThick / 8 * P.partRho;
+        for (int v : c.hPart) c.hMass[v] += pm;
+    }
+    // contact primitives, IMPLICIT_EULER.h:243-277: rod segments after the surface edges,
+    // rod nodes (ascending) then particles after the surface nodes
+    for (int r = 0; r < c.nRod; ++r) { c.hBEdge.push_back(c.hRod[2 * r]); c.hBEdge.push_back(c.hRod[2 * r + 1]); }
+    c.codimBN0 = (int)c.hBNode.size();
+    for (auto& kv : nbr) c.hBNode.push_back(kv.first);
+    c.codimBN1 = (int)c.hBNode.size();
+    for (int v : c.hPart) c.hBNode.push_back(v);
+    c.nBN = (int)c.hBNode.size();
+    c.nBE = (int)c.hBEdge.size() / 2;
     // Initialize_EIPC<double,false>, DISCRETE_SHELL.h:563-574
     c.dHat2 = P.dHat * P.dHat;
     double t2 = 1.0e-16 - c.dHat2;
@@ -175,6 +224,29 @@ void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri
     upload(c.hInfo, soa_d(c.hHingeInfo, c.nH, 3), s);
     upload(c.bNode, c.hBNode, s);
     upload(c.bEdge, soa_i(c.hBEdge, c.nBE, 2), s);
+    if (c.nRod) {
+        // spring weight h^2 pi t^2 / 4 (ROD_STRETCHING.h:22-24), bending weight
+        // h^2 / lsum * E t^4 pi / 64 (ROD_BENDING.h:55-56)
+        std::vector<double> par(3 * (size_t)c.nRod);
+        for (int r = 0; r < c.nRod; ++r) {
+            double t = c.hRodInfo[3 * r + 2];
+            par[r] = c.hRodInfo[3 * r];
+            par[(size_t)c.nRod + r] = c.hRodInfo[3 * r + 1];
+            par[2 * (size_t)c.nRod + r] = P.dt * P.dt * M_PI * t * t / 4;
+        }
+        upload(c.rod, soa_i(c.hRod, c.nRod, 2), s);
+        upload(c.rodPar, par, s);
+    }
+    if (c.nRH) {
+        std::vector<double> w(c.nRH);
+        for (int h = 0; h < c.nRH; ++h) {
+            const double* I = &c.hRodHingeInfo[3 * h];
+            w[h] = P.dt * P.dt / I[1] * I[0] * std::pow(I[2], 4) * M_PI / 64;
+        }
+        upload(c.rodHinge, soa_i(c.hRodHinge, c.nRH, 3), s);
+        upload(c.rodHW, w, s);
+    }
+    if (c.nPart) upload(c.partSlot, c.hPart, s);
     const size_t n3 = 3 * (size_t)c.nV;
     for (DevBuf<double>* b : {&c.x, &c.xn, &c.xt, &c.v, &c.xprev, &c.p, &c.g, &c.rhs}) b->ensure_exact(n3);
     c.scal.ensure_exact(64);

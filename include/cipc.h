@@ -55,6 +55,18 @@ typedef struct cipc_scene_desc {
     double pcg_tol;            /* relative residual of the Newton linear solve */
     int32_t pcg_max_iter;
     int32_t with_collision;
+    /* codimensional primitives; zero counts for a pure shell scene.  Rod segments are
+     * Initialize_Discrete_Rod (Rod/DISCRETE_ROD.h:87-165, per-segment springs, one
+     * bending hinge per pair of segments sharing a node); particles are
+     * Initialize_Discrete_Particle (DISCRETE_PARTICLE.h:52-75).  Both join the contact
+     * primitives as in IMPLICIT_EULER.h:243-277 and the codimensional contact and CCD
+     * branches of IPC.h:271-353, 2095-2162. */
+    int32_t n_rod_segments;
+    const int32_t* rod_segments;     /* 2 n_rod_segments vertex ids */
+    double rod_density, rod_youngs, rod_bend_stiff_mult, rod_thickness;
+    int32_t n_particles;
+    const int32_t* particles;        /* n_particles vertex ids */
+    double particle_density, particle_thickness;
 } cipc_scene_desc;
 
 typedef struct cipc_step_stats {
@@ -75,6 +87,14 @@ int cipc_get_scalars(const cipc_ctx* ctx, double* out);
 /* derived set-up arrays: lumped mass (n_vertices), hinge stencils (4 n_hinges) and
  * rest data (3 n_hinges: theta_bar, |e_bar|, h_bar), boundary edges (2 n_boundary_edges) */
 int cipc_get_setup(const cipc_ctx* ctx, double* mass, int32_t* hinges, double* hinge_info, int32_t* bedges);
+/* codimensional set-up (DISCRETE_ROD.h:87-165, IMPLICIT_EULER.h:243-277):
+ * out[0..4] = n_rod_segments, n_rod_hinges, n_particles, first rod-node slot and first
+ * particle slot among the boundary nodes */
+int cipc_get_codim_sizes(const cipc_ctx* ctx, int32_t* out);
+/* rod_info 3 per segment (E, rest length, thickness), rod_hinges 3 per hinge (x0, shared
+ * node, x2), rod_hinge_info 3 per hinge (E * bend mult, rest |e0| + |e1|, thickness),
+ * bnodes (n_boundary_nodes) */
+int cipc_get_codim_setup(const cipc_ctx* ctx, double* rod_info, int32_t* rod_hinges, double* rod_hinge_info, int32_t* bnodes);
 int cipc_set_state(cipc_ctx* ctx, const double* x, const double* v);
 int cipc_get_state(const cipc_ctx* ctx, double* x, double* v);
 int cipc_set_kappa(cipc_ctx* ctx, double kappa);

@@ -64,12 +64,31 @@ void orc_dihedral(const double* x, double* th, double* g, double* H) {
     dihedral_angle_hess(P3(x, 0), P3(x, 1), P3(x, 2), P3(x, 3), H);
 }
 
+void orc_rod_spring(const double* x, double Er, double lbar, double* E, double* g, double* H) {
+    *E = rod_spring_energy(P3(x, 0), P3(x, 1), Er, lbar);
+    rod_spring_grad(P3(x, 0), P3(x, 1), Er, lbar, g);
+    rod_spring_hess(P3(x, 0), P3(x, 1), Er, lbar, H);
+}
+void orc_rod_kappa2(const double* x, double* k2, double* g, double* H) {
+    *k2 = rod_kappa2(P3(x, 0), P3(x, 1), P3(x, 2));
+    rod_kappa2_grad(P3(x, 0), P3(x, 1), P3(x, 2), g);
+    rod_kappa2_hess(P3(x, 0), P3(x, 1), P3(x, 2), H);
+}
+
 // ---------------------------------------------------------------- scenes
 // params: rho, E, nu, thick, bendMult, dt, dHat, xi, PNTol, gx, gy, gz, kappaStiffMult, pcgTol
 // iparams: collision, solver, pcgMaxIter, threads
-void* orc_scene_create(int nV, const double* X0, int nT, const int* tri, int nDBC, const int* dbcIdx,
-                       const double* dbcTarget, const double* params, const int* iparams) {
+// codim (may be null when nRod = nPart = 0): rodRho, rodE, rodBendMult, rodThick, partRho, partThick
+void* orc_scene_create_codim(int nV, const double* X0, int nT, const int* tri, int nDBC, const int* dbcIdx,
+                             const double* dbcTarget, const double* params, const int* iparams, int nRod, const int* rods,
+                             int nPart, const int* parts, const double* codim) {
     Scene* S = new Scene();
+    if (codim) {
+        S->P.rodRho = codim[0]; S->P.rodE = codim[1]; S->P.rodBendMult = codim[2]; S->P.rodThick = codim[3];
+        S->P.partRho = codim[4]; S->P.partThick = codim[5];
+    }
+    if (nRod) S->rod.assign(rods, rods + 2 * nRod);
+    if (nPart) S->particle.assign(parts, parts + nPart);
     Params& P = S->P;
     P.rho = params[0]; P.E = params[1]; P.nu = params[2]; P.thick = params[3]; P.bendMult = params[4];
     P.dt = params[5]; P.dHat = params[6]; P.xi = params[7]; P.PNTol = params[8];
@@ -83,6 +102,22 @@ void* orc_scene_create(int nV, const double* X0, int nT, const int* tri, int nDB
     S->dbcTarget.assign(dbcTarget, dbcTarget + 3 * nDBC);
     scene_init(*S);
     return S;
+}
+void* orc_scene_create(int nV, const double* X0, int nT, const int* tri, int nDBC, const int* dbcIdx,
+                       const double* dbcTarget, const double* params, const int* iparams) {
+    return orc_scene_create_codim(nV, X0, nT, tri, nDBC, dbcIdx, dbcTarget, params, iparams, 0, nullptr, 0, nullptr, nullptr);
+}
+// codim sizes: nRod, nRodHinge, nParticle, codimBN0, codimBN1
+void orc_codim_sizes(void* h, int* out) {
+    Scene* S = (Scene*)h;
+    out[0] = (int)S->rod.size() / 2; out[1] = (int)S->rodHinge.size() / 3; out[2] = (int)S->particle.size();
+    out[3] = S->codimBN0; out[4] = S->codimBN1;
+}
+void orc_codim_arrays(void* h, double* rodInfo, int* rodHinge, double* rodHingeInfo) {
+    Scene* S = (Scene*)h;
+    if (rodInfo) std::memcpy(rodInfo, S->rodInfo.data(), sizeof(double) * S->rodInfo.size());
+    if (rodHinge) std::memcpy(rodHinge, S->rodHinge.data(), sizeof(int) * S->rodHinge.size());
+    if (rodHingeInfo) std::memcpy(rodHingeInfo, S->rodHingeInfo.data(), sizeof(double) * S->rodHingeInfo.size());
 }
 void orc_scene_destroy(void* h) { delete (Scene*)h; }
 void orc_set_threads(void* h, int t) { ((Scene*)h)->P.threads = t > 0 ? t : 1; }

@@ -3,8 +3,8 @@
 // minimum distance, and the CCD step size with the spatial-hash step clamp.
 // TEST INFRASTRUCTURE ONLY (see oracle.h).
 //
-// Restates FEM/IPC.h:19-740 (Compute_Constraint_Set, shell path: no rods/particles
-// yet), :742-941 (Compute_Barrier), :943-1256 (Compute_Barrier_Gradient),
+// Restates FEM/IPC.h:19-740 (Compute_Constraint_Set, shell path with the rod and
+// particle branches :271-353), :742-941 (Compute_Barrier), :943-1256 (Compute_Barrier_Gradient),
 // :1258-1731 (Compute_Barrier_Hessian), :1879-2244 (Compute_Intersection_Free_StepSize),
 // :2246-2388 (Compute_Min_Dist2), Grid/SPATIAL_HASH.h:432-622 (CCD hash incl. the
 // step clamp) and Math/Distance/CCD.h:150-483.
@@ -35,6 +35,13 @@ static bool pt_cd_broadphase(V3 p, V3 t0, V3 t1, V3 t2, double dist) {
     double mn[3] = {std::min(std::min(t0.x, t1.x), t2.x), std::min(std::min(t0.y, t1.y), t2.y), std::min(std::min(t0.z, t1.z), t2.z)};
     double pp[3] = {p.x, p.y, p.z};
     for (int k = 0; k < 3; ++k) if (pp[k] - mx[k] > dist || mn[k] - pp[k] > dist) return false;
+    return true;
+}
+// CCD.h:15-30
+static bool pe_cd_broadphase(V3 p, V3 e0, V3 e1, double dist) {
+    if (p.x - std::max(e0.x, e1.x) > dist || std::min(e0.x, e1.x) - p.x > dist) return false;
+    if (p.y - std::max(e0.y, e1.y) > dist || std::min(e0.y, e1.y) - p.y > dist) return false;
+    if (p.z - std::max(e0.z, e1.z) > dist || std::min(e0.z, e1.z) - p.z > dist) return false;
     return true;
 }
 // CCD.h:168-185
@@ -87,13 +94,18 @@ void constraint_set(const Scene& S, const double* x, std::vector<Constraint>& ou
     const double* X0 = S.X0.data();
     const char* dbc = S.isDBC.data();
 
-    Grid gT, gE;
+    const int nRod = (int)S.rod.size() / 2;
+    Grid gT, gE, gP;
     if (!bruteForce) {
         double cs = mean_bedge_len(S, x);
         if (!(cs > 0)) cs = 1;
         double lo[3] = {1e300, 1e300, 1e300};
         for (int i = 0; i < S.nV; ++i) for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], x[3 * i + d]);
-        for (Grid* g : {&gT, &gE}) { for (int d = 0; d < 3; ++d) g->org[d] = lo[d]; g->inv = 1.0 / cs; }
+        for (Grid* g : {&gT, &gE, &gP}) { for (int d = 0; d < 3; ++d) g->org[d] = lo[d]; g->inv = 1.0 / cs; }
+        for (int s = S.codimBN1; s < nBN; ++s) {
+            double q[3] = {x[3 * S.bNode[s]], x[3 * S.bNode[s] + 1], x[3 * S.bNode[s] + 2]};
+            gP.insert(q, q, s);
+        }
         for (int t = 0; t < nBT; ++t) {
             double l[3], h[3];
             for (int d = 0; d < 3; ++d) {
@@ -143,6 +155,45 @@ void constraint_set(const Scene& S, const double* x, std::vector<Constraint>& ou
                 default: d = dist2_pt(p, t0, t1, t2); c.c[1] = t[0]; c.c[2] = t[1]; c.c[3] = t[2]; isPT = true; break;
             }
             if (d < dHat2) (isPT ? keptPT : dupPT)[svI].push_back(c);
+        }
+        if (svI < S.codimBN0) continue;
+        // rod nodes and particles against rod segments, IPC.h:271-317
+        std::vector<int> ecand;
+        if (bruteForce) { for (int e = nBE - nRod; e < nBE; ++e) ecand.push_back(e); }
+        else {
+            double l[3] = {p.x - dHat, p.y - dHat, p.z - dHat}, h[3] = {p.x + dHat, p.y + dHat, p.z + dHat};
+            gE.query(l, h, ecand);
+        }
+        for (int eI : ecand) {
+            if (eI < nBE - nRod) continue;
+            const int* e = &S.bEdge[2 * eI];
+            if (vI == e[0] || vI == e[1]) continue;
+            if (dbc[vI] && dbc[e[0]] && dbc[e[1]]) continue;
+            V3 e0 = X3(x, e[0]), e1 = X3(x, e[1]);
+            if (!pe_cd_broadphase(p, e0, e1, dHat)) continue;
+            double d;
+            Constraint c{{-vI - 1, -1, -1, -1}};
+            switch (pe_type(p, e0, e1)) {
+                case 0: d = dist2_pp(p, e0); c.c[1] = e[0]; break;
+                case 1: d = dist2_pp(p, e1); c.c[1] = e[1]; break;
+                default: d = dist2_pe(p, e0, e1); c.c[1] = e[0]; c.c[2] = e[1]; break;
+            }
+            if (d < dHat2) dupPT[svI].push_back(c);
+        }
+        if (svI < S.codimBN1) continue;
+        // particle against the boundary nodes after it (all particles), IPC.h:319-349
+        std::vector<int> pcand;
+        if (bruteForce) { for (int j = svI + 1; j < nBN; ++j) pcand.push_back(j); }
+        else {
+            double l[3] = {p.x - dHat, p.y - dHat, p.z - dHat}, h[3] = {p.x + dHat, p.y + dHat, p.z + dHat};
+            gP.query(l, h, pcand);
+        }
+        for (int svJ : pcand) {
+            if (svJ <= svI) continue;
+            int vJ = S.bNode[svJ];
+            if (dbc[vI] && dbc[vJ]) continue;
+            double d = dist2_pp(p, X3(x, vJ));
+            if (d < dHat2) dupPT[svI].push_back(Constraint{{-vI - 1, vJ, -1, -1}});
         }
     }
 
@@ -402,7 +453,79 @@ static bool ee_ccd_broadphase(V3 a0, V3 a1, V3 b0, V3 b1, V3 da0, V3 da1, V3 db0
     }
     return true;
 }
+// CCD.h:237-277
+static bool pe_ccd_broadphase(V3 p, V3 e0, V3 e1, V3 dp, V3 de0, V3 de1, double dist) {
+    V3 pe = p + dp, q0 = e0 + de0, q1 = e1 + de1;
+    double P[2][3] = {{p.x, p.y, p.z}, {pe.x, pe.y, pe.z}};
+    V3 E[4] = {e0, e1, q0, q1};
+    for (int k = 0; k < 3; ++k) {
+        double maxp = std::max(P[0][k], P[1][k]), minp = std::min(P[0][k], P[1][k]);
+        double mx = -1e300, mn = 1e300;
+        for (auto& e : E) { double v = k == 0 ? e.x : (k == 1 ? e.y : e.z); mx = std::max(mx, v); mn = std::min(mn, v); }
+        if (minp - mx > dist || mn - maxp > dist) return false;
+    }
+    return true;
+}
+static bool pp_ccd_broadphase(V3 p0, V3 p1, V3 dp0, V3 dp1, double dist) {
+    V3 q0 = p0 + dp0, q1 = p1 + dp1;
+    double A[2][3] = {{p0.x, p0.y, p0.z}, {q0.x, q0.y, q0.z}}, B[2][3] = {{p1.x, p1.y, p1.z}, {q1.x, q1.y, q1.z}};
+    for (int k = 0; k < 3; ++k) {
+        double mxa = std::max(A[0][k], A[1][k]), mna = std::min(A[0][k], A[1][k]);
+        double mxb = std::max(B[0][k], B[1][k]), mnb = std::min(B[0][k], B[1][k]);
+        if (mna - mxb > dist || mnb - mxa > dist) return false;
+    }
+    return true;
+}
 static const long ACCD_MAX_ITERS = 1000000;
+
+// CCD.h:397-441 (additive CCD, point-edge)
+static bool pe_accd(V3 p, V3 e0, V3 e1, V3 dp, V3 de0, V3 de1, double eta, double thickness, double* toc) {
+    V3 mov = V3{(dp.x + de0.x + de1.x) / 3, (dp.y + de0.y + de1.y) / 3, (dp.z + de0.z + de1.z) / 3};
+    de0 = de0 - mov; de1 = de1 - mov; dp = dp - mov;
+    double maxDispMag = std::sqrt(sq(dp)) + std::sqrt(std::max(sq(de0), sq(de1)));
+    if (maxDispMag == 0) return false;
+    double dist2_cur = dist2_pe_unclassified(p, e0, e1);
+    double dist_cur = std::sqrt(dist2_cur);
+    double gap = eta * (dist2_cur - thickness * thickness) / (dist_cur + thickness);
+    double toc_prev = *toc;
+    double t = 0;
+    for (long it = 0;; ++it) {
+        if (it >= ACCD_MAX_ITERS) { *toc = t; return true; }
+        double tocLowerBound = (1 - eta) * (dist2_cur - thickness * thickness) / ((dist_cur + thickness) * maxDispMag);
+        p = p + tocLowerBound * dp; e0 = e0 + tocLowerBound * de0; e1 = e1 + tocLowerBound * de1;
+        dist2_cur = dist2_pe_unclassified(p, e0, e1);
+        dist_cur = std::sqrt(dist2_cur);
+        if (t != 0 && (dist2_cur - thickness * thickness) / (dist_cur + thickness) < gap) break;
+        t += tocLowerBound;
+        if (t > toc_prev) { *toc = t; return false; }
+    }
+    *toc = t;
+    return true;
+}
+// CCD.h:443-483 (additive CCD, point-point)
+static bool pp_accd(V3 p0, V3 p1, V3 dp0, V3 dp1, double eta, double thickness, double* toc) {
+    V3 mov = V3{(dp0.x + dp1.x) / 2, (dp0.y + dp1.y) / 2, (dp0.z + dp1.z) / 2};
+    dp1 = dp1 - mov; dp0 = dp0 - mov;
+    double maxDispMag = std::sqrt(sq(dp0)) + std::sqrt(sq(dp1));
+    if (maxDispMag == 0) return false;
+    double dist2_cur = dist2_pp(p0, p1);
+    double dist_cur = std::sqrt(dist2_cur);
+    double gap = eta * (dist2_cur - thickness * thickness) / (dist_cur + thickness);
+    double toc_prev = *toc;
+    double t = 0;
+    for (long it = 0;; ++it) {
+        if (it >= ACCD_MAX_ITERS) { *toc = t; return true; }
+        double tocLowerBound = (1 - eta) * (dist2_cur - thickness * thickness) / ((dist_cur + thickness) * maxDispMag);
+        p0 = p0 + tocLowerBound * dp0; p1 = p1 + tocLowerBound * dp1;
+        dist2_cur = dist2_pp(p0, p1);
+        dist_cur = std::sqrt(dist2_cur);
+        if (t != 0 && (dist2_cur - thickness * thickness) / (dist_cur + thickness) < gap) break;
+        t += tocLowerBound;
+        if (t > toc_prev) { *toc = t; return false; }
+    }
+    *toc = t;
+    return true;
+}
 
 // CCD.h:280-328 (additive CCD, point-triangle); toc in/out as in the reference
 static bool pt_accd(V3 p, V3 t0, V3 t1, V3 t2, V3 dp, V3 dt0, V3 dt1, V3 dt2, double eta, double thickness, double* toc) {
@@ -556,8 +679,14 @@ double ccd_stepsize(const Scene& S, const double* x, const double* p, double alp
     (void)bruteForce;
 
     // index grid over voxel ranges to enumerate overlapping pairs quickly
-    Grid gT, gE;
-    for (Grid* g : {&gT, &gE}) { g->org[0] = g->org[1] = g->org[2] = 0; g->inv = 1.0 / 4.0; }
+    const int nRod = (int)S.rod.size() / 2;
+    Grid gT, gE, gP;
+    for (Grid* g : {&gT, &gE, &gP}) { g->org[0] = g->org[1] = g->org[2] = 0; g->inv = 1.0 / 4.0; }
+    for (int s = S.codimBN1; s < nBN; ++s) {
+        double l[3] = {(double)vmin[3 * s], (double)vmin[3 * s + 1], (double)vmin[3 * s + 2]};
+        double h[3] = {(double)vmax[3 * s], (double)vmax[3 * s + 1], (double)vmax[3 * s + 2]};
+        gP.insert(l, h, s);
+    }
     for (int t = 0; t < nBT; ++t) {
         double l[3] = {(double)tlo[3 * t], (double)tlo[3 * t + 1], (double)tlo[3 * t + 2]};
         double h[3] = {(double)thi[3 * t], (double)thi[3 * t + 1], (double)thi[3 * t + 2]};
@@ -595,6 +724,36 @@ double ccd_stepsize(const Scene& S, const double* x, const double* p, double alp
 #pragma omp atomic write
                     zeroToi = 1;
                 }
+            }
+        }
+        if (s < S.codimBN1) continue;
+        // particle against rod segments, IPC.h:2095-2133
+        gE.query(l, h, cand);
+        for (int e : cand) {
+            if (e < nBE - nRod) continue;
+            if (!overlap(&vmin[3 * s], &vmax[3 * s], &elo[3 * e], &ehi[3 * e])) continue;
+            const int* ev = &S.bEdge[2 * e];
+            if (vI == ev[0] || vI == ev[1]) continue;
+            if (dbc[vI] && dbc[ev[0]] && dbc[ev[1]]) continue;
+            V3 e0 = X3(x, ev[0]), e1 = X3(x, ev[1]), de0 = X3(p, ev[0]), de1 = X3(p, ev[1]);
+            if (!pe_ccd_broadphase(P, e0, e1, dP, de0, de1, xi)) continue;
+            double la = lpt[s];
+            if (pe_accd(P, e0, e1, dP, de0, de1, 0.1, xi, &la)) {
+                if (lpt[s] > la) lpt[s] = la;
+            }
+        }
+        // particle against the particles after it, IPC.h:2135-2162
+        gP.query(l, h, cand);
+        for (int sJ : cand) {
+            if (sJ <= s) continue;
+            if (!overlap(&vmin[3 * s], &vmax[3 * s], &vmin[3 * sJ], &vmax[3 * sJ])) continue;
+            int vJ = S.bNode[sJ];
+            if (dbc[vI] && dbc[vJ]) continue;
+            V3 Q = X3(x, vJ), dQ = X3(p, vJ);
+            if (!pp_ccd_broadphase(P, Q, dP, dQ, xi)) continue;
+            double la = lpt[s];
+            if (pp_accd(P, Q, dP, dQ, 0.1, xi, &la)) {
+                if (lpt[s] > la) lpt[s] = la;
             }
         }
     }

@@ -1,6 +1,8 @@
 // Oracle: shell elasticity and inertia summed over the mesh.
 // TEST INFRASTRUCTURE ONLY (see oracle.h).
 //
+// Rods (springs, hinges) are added by codim.cpp: stretching energy with the membrane,
+// rod bending with the hinge bending.
 // Restates FEM/Shell/MEMBRANE.h:7-290 (Neo-Hookean membrane, triangles whose three
 // nodes are all Dirichlet are skipped), FEM/Shell/BENDING.h:52-75, 177-208, 441-496
 // (hinge bending, KL = false), and the inertia term of FEM/Shell/INC_POTENTIAL.h:
@@ -40,6 +42,7 @@ double elastic_energy(const Scene& S, const double* x, double* membOut, double* 
             Eb += h2 * S.kBend * (th - I[0]) * (th - I[0]) * I[1] / I[2];
         }
     }
+    codim_energy(S, x, &Em, &Eb);
     if (membOut) *membOut = Em;
     if (bendOut) *bendOut = Eb;
     return Em + Eb;
@@ -70,6 +73,7 @@ void elastic_grad(const Scene& S, const double* x, double* g) {
                 for (int k = 0; k < 3; ++k) g[3 * v[a] + k] += s * gl[3 * a + k];
         }
     }
+    codim_grad(S, x, g);
 }
 
 // Local blocks are written straight into their final slots of `out` (the reference's
@@ -97,6 +101,7 @@ void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
                     for (int l = 0; l < 3; ++l) B.b[3 * k + l] = H[(3 * a + k) * 9 + 3 * b + l];
             }
     }
+    codim_hess(S, x, out);
     if (S.P.bendMult == 0) return;
     int nH = (int)S.hinge.size() / 4;
     std::vector<long> offH(nH + 1, 0);

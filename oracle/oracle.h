@@ -80,12 +80,28 @@ double dihedral_angle(V3 x0, V3 x1, V3 x2, V3 x3);
 void dihedral_angle_grad(V3 x0, V3 x1, V3 x2, V3 x3, double* g);
 void dihedral_angle_hess(V3 x0, V3 x1, V3 x2, V3 x3, double* H);
 
+// ---------------------------------------------------------------- codimensional
+// rod segment spring, FEM/Shell/Rod/ROD_STRETCHING.h:10-107 (without the h^2 pi t^2/4
+// weight): E = Er / (2 lbar) (|x0 - x1| - lbar)^2, g 6, H 6x6 (not projected)
+double rod_spring_energy(V3 x0, V3 x1, double Er, double lbar);
+void rod_spring_grad(V3 x0, V3 x1, double Er, double lbar, double* g);
+void rod_spring_hess(V3 x0, V3 x1, double Er, double lbar, double* H);
+// squared discrete curvature of a rod hinge, FEM/Shell/Rod/ROD_BENDING.h:10-117:
+// kappa = 2 (e0 x e1) / (|e0| |e1| + e0.e1), e0 = x1 - x0, e1 = x2 - x1; returns |kappa|^2,
+// g 9, H 9x9 (not projected)
+double rod_kappa2(V3 x0, V3 x1, V3 x2);
+void rod_kappa2_grad(V3 x0, V3 x1, V3 x2, double* g);
+void rod_kappa2_hess(V3 x0, V3 x1, V3 x2, double* H);
+
 // ---------------------------------------------------------------- scene
 struct Params {
     double rho = 200, E = 1e6, nu = 0.3, thick = 1e-3, bendMult = 1;
     double dt = 0.01, dHat = 1e-3, xi = 0, PNTol = 1e-2;
     double grav[3] = {0, -9.81, 0};
     double kappaStiffMult = 1;
+    // rods (Initialize_Discrete_Rod) and particles (Initialize_Discrete_Particle)
+    double rodRho = 0, rodE = 0, rodBendMult = 1, rodThick = 1e-3;
+    double partRho = 0, partThick = 1e-3;
     int collision = 1;
     int solver = 0;            // 0 = direct (sparse Cholesky), 1 = block-Jacobi PCG
     double pcgTol = 1e-5;
@@ -110,22 +126,36 @@ struct Scene {
     std::vector<int> hinge;            // 4 per hinge
     std::vector<double> hingeInfo;     // theta_bar, |e_bar|, h_bar
     std::vector<int> bNode, bEdge, bTri;
+    // codimensional primitives (DISCRETE_ROD.h:87-165, DISCRETE_PARTICLE.h:52-75)
+    std::vector<int> rod;              // 2 per segment
+    std::vector<double> rodInfo;       // E, rest length, thickness per segment
+    std::vector<int> rodHinge;         // 3 per rod hinge (x0, x1 = shared node, x2)
+    std::vector<double> rodHingeInfo;  // E * bend mult, rest |e0| + |e1|, thickness
+    std::vector<int> particle;
+    int codimBN0 = 0, codimBN1 = 0;    // boundary-node slots where rod nodes / particles start
     double kappa = 0, kappaMax = 0, dHat2 = 0;
     double bboxDiag = 0;
 };
 
 // constraint: 4 ints in the reference encoding (FEM/IPC.h:144-661)
 struct Constraint { int c[4]; };
+// local Hessian blocks, appended as (row vertex, col vertex, 3x3 block)
+struct Block { int r, c; double b[9]; };
 
 void scene_init(Scene& S);                       // A22
+// rods and particles: mass, rod hinges, boundary primitives (called by scene_init)
+void codim_init(Scene& S);
+// rod springs added to *stretch, rod hinges to *bend; gradient accumulated into g;
+// local blocks appended to out
+void codim_energy(const Scene& S, const double* x, double* stretch, double* bend);
+void codim_grad(const Scene& S, const double* x, double* g);
+void codim_hess(const Scene& S, const double* x, std::vector<Block>& out);
 // contact
 void constraint_set(const Scene& S, const double* x, std::vector<Constraint>& out,
                     bool bruteForce = false);
 double barrier_energy(const Scene& S, const double* x, const std::vector<Constraint>& C, double kappa,
                       int* bad = nullptr);
 void barrier_grad(const Scene& S, const double* x, const std::vector<Constraint>& C, double kappa, double* g);
-// local Hessian blocks per constraint, appended as (row vertex, col vertex, 3x3 block)
-struct Block { int r, c; double b[9]; };
 void barrier_hess(const Scene& S, const double* x, const std::vector<Constraint>& C, double kappa,
                   std::vector<Block>& out);
 void constraint_dist2(const Scene& S, const double* x, const std::vector<Constraint>& C, std::vector<double>& d2);

@@ -33,6 +33,7 @@ def lib():
         L.orc_dist2.restype = ctypes.c_double
         L.orc_dist2_unclassified.restype = ctypes.c_double
         L.orc_scene_create.restype = ctypes.c_void_p
+        L.orc_scene_create_codim.restype = ctypes.c_void_p
         L.orc_ccd.restype = ctypes.c_double
         for name in ("orc_scene_destroy", "orc_scene_sizes", "orc_scene_scalars", "orc_scene_arrays", "orc_get_state",
                      "orc_set_state", "orc_set_kappa", "orc_constraint_set", "orc_energy", "orc_gradient", "orc_hessian",
@@ -131,6 +132,24 @@ def dihedral(x):
     return th[0], g, H
 
 
+def rod_spring(x, Er, lbar):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    E = np.zeros(1)
+    g = np.zeros(6)
+    H = np.zeros((6, 6))
+    lib().orc_rod_spring(_dp(x), ctypes.c_double(Er), ctypes.c_double(lbar), _dp(E), _dp(g), _dp(H))
+    return E[0], g, H
+
+
+def rod_kappa2(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    k2 = np.zeros(1)
+    g = np.zeros(9)
+    H = np.zeros((9, 9))
+    lib().orc_rod_kappa2(_dp(x), _dp(k2), _dp(g), _dp(H))
+    return k2[0], g, H
+
+
 def solve(A_csr, b, solver=0, tol=1e-5):
     import scipy.sparse as sp
     A = sp.csr_matrix(A_csr)
@@ -157,12 +176,22 @@ class OracleScene:
         self._iparams = np.array([P.get("collision", 1), solver, 1000000, threads], dtype=np.int32)
         L = lib()
         self.sd = sd
-        self.h = ctypes.c_void_p(L.orc_scene_create(
+        rods = getattr(sd, "rods", None)
+        parts = getattr(sd, "particles", None)
+        rods = np.zeros((0, 2), np.int32) if rods is None else np.ascontiguousarray(rods, dtype=np.int32)
+        parts = np.zeros(0, np.int32) if parts is None else np.ascontiguousarray(parts, dtype=np.int32)
+        self._codim = np.array([P.get("rodRho", 0.0), P.get("rodE", 0.0), P.get("rodBendMult", 1.0), P.get("rodThick", 1e-3),
+                                P.get("partRho", 0.0), P.get("partThick", 1e-3)], dtype=np.float64)
+        self._rods, self._parts = rods, parts
+        self.h = ctypes.c_void_p(L.orc_scene_create_codim(
             ctypes.c_int(sd.nV), _dp(sd.X), ctypes.c_int(sd.nT), _ip(sd.tris), ctypes.c_int(len(sd.dbc_idx)),
-            _ip(sd.dbc_idx), _dp(sd.dbc_target), _dp(self._params), _ip(self._iparams)))
+            _ip(sd.dbc_idx), _dp(sd.dbc_target), _dp(self._params), _ip(self._iparams), ctypes.c_int(len(rods)), _ip(rods),
+            ctypes.c_int(len(parts)), _ip(parts), _dp(self._codim)))
         s = np.zeros(5, dtype=np.int32)
         L.orc_scene_sizes(self.h, _ip(s))
         self.nT, self.nBN, self.nBE, self.nBT, self.nH = [int(v) for v in s]
+        L.orc_codim_sizes(self.h, _ip(s))
+        self.nRod, self.nRodHinge, self.nParticle, self.codimBN0, self.codimBN1 = [int(v) for v in s]
         self.nV = sd.nV
 
     def __del__(self):
@@ -195,6 +224,13 @@ class OracleScene:
                                _ip(bedge))
         return dict(mass=mass, Binv=Binv.reshape(-1, 3), vol=vol, tri=tri.reshape(-1, 3), hinge=hinge.reshape(-1, 4),
                     hinge_info=hinfo.reshape(-1, 3), bnode=bnode, bedge=bedge.reshape(-1, 2))
+
+    def codim_arrays(self):
+        ri = np.zeros(3 * self.nRod)
+        rh = np.zeros(3 * self.nRodHinge, dtype=np.int32)
+        rhi = np.zeros(3 * self.nRodHinge)
+        lib().orc_codim_arrays(self.h, _dp(ri), _ip(rh), _dp(rhi))
+        return dict(rod_info=ri.reshape(-1, 3), rod_hinge=rh.reshape(-1, 3), rod_hinge_info=rhi.reshape(-1, 3))
 
     def get_state(self):
         x = np.zeros(3 * self.nV)

@@ -106,6 +106,8 @@ void scene_init(Scene& S) {
     S.lam = P.E * P.nu / (1.0 - P.nu * P.nu);
     S.mu = P.E / (2.0 * (1.0 + P.nu));
     S.kBend = P.bendMult * (P.E * std::pow(P.thick, 3) / (24 * (1.0 - P.nu * P.nu)));
+    // rods and particles (initialize_rod / initialize_particle run before initialize_OIPC)
+    codim_init(S);
     // Initialize_EIPC<double,false>, DISCRETE_SHELL.h:563-574
     S.dHat2 = P.dHat * P.dHat;
     double Hb = barrier_h(1.0e-16, S.dHat2, 1.0);

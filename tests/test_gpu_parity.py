@@ -26,6 +26,9 @@ SCENES = {
     "contact_stack": scenes.scene_contact_stack,
     # ground triangles/edges cover thousands of cells: large-primitive path of both broad phases
     "cloth_on_ground": scenes.scene_cloth_on_ground,
+    # rods and particles on the sheet (offset 1 mm): codimensional point-edge / point-point
+    # entries, rod springs and hinges, particle CCD
+    "codim_contact": scenes.scene_codim_contact,
 }
 
 
@@ -55,6 +58,12 @@ def test_setup_matches_oracle(stack):
     so, sg = orc.scalars(), gpu.scalars()
     for k in ("kappa", "kappaMax", "dHat2", "lam", "mu", "kBend"):
         assert sg[k] == so[k], k
+    assert (gpu.nRod, gpu.nRodHinge, gpu.nParticle, gpu.codimBN0, gpu.codimBN1) == \
+        (orc.nRod, orc.nRodHinge, orc.nParticle, orc.codimBN0, orc.codimBN1)
+    cg, co = gpu.codim_setup(), orc.codim_arrays()
+    assert np.array_equal(cg["bnode"], a["bnode"])
+    for k in ("rod_info", "rod_hinge", "rod_hinge_info"):
+        assert np.array_equal(cg[k], co[k]), k
 
 
 def test_constraint_set_bit_exact(stack):
@@ -66,7 +75,7 @@ def test_constraint_set_bit_exact(stack):
         assert np.array_equal(Co, Cb)
         assert Cg.shape == Co.shape
         assert np.array_equal(Cg, Co)
-    assert len(gpu.constraint_set(states[0])) > 1000
+    assert len(gpu.constraint_set(states[0])) > 200
 
 
 def test_min_dist2_bit_exact(stack):
@@ -202,3 +211,45 @@ def test_step_trajectory(require_gpu, precision):
         assert np.abs(xg - xo).max() <= 1e-6 * max(disp, 1e-3), s
         d2, m = gpu.min_dist2(xg, gpu.constraint_set(xg)) if len(gpu.constraint_set(xg)) else (None, 1.0)
         assert m > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_codim_step_trajectory(require_gpu, precision):
+    """Six steps of the falling rods / particles / sheet mix (offset 1 mm) against the
+    FP64 oracle, tight Newton tolerance; every state intersection free."""
+    sd = scenes.scene_codim_small()
+    sd.params["PNTol"] = 1e-6 * sd.bbox_diag()
+    gpu = Simulator(sd, pcg_tol=1e-10, precision=precision)
+    orc = O.OracleScene(sd, threads=4)
+    for s in range(6):
+        st = gpu.step()
+        rc, so = orc.step()
+        assert rc > 0 and st["status"] == 0
+        xg, _ = gpu.get_state()
+        xo, _ = orc.get_state()
+        disp = np.abs(xo - sd.X.ravel()).max()
+        assert np.abs(xg - xo).max() <= 1e-6 * max(disp, 1e-3), s
+        C = gpu.constraint_set(xg)
+        if len(C):
+            assert gpu.min_dist2(xg, C)[1] > 0          # min d^2 - offset^2
+    assert st["contacts"] > 0
+
+
+def test_c3_one_step(require_gpu):
+    """The 324k-vertex codimensional mix (C1 + 20,000 rods + 100,000 particles): one step in
+    FP64 and in the FP32 form at a tight Newton / linear tolerance; they agree to 1e-6 of
+    the step's displacement."""
+    sd = scenes.scene_c3()
+    sd.params["PNTol"] = 1e-6 * sd.bbox_diag()
+    out = {}
+    for prec in ("fp64", "fp32"):
+        gpu = Simulator(sd, pcg_tol=1e-10, precision=prec)
+        st = gpu.step()
+        assert st["status"] == 0
+        x, _ = gpu.get_state()
+        out[prec] = x
+        print(prec, {k: st[k] for k in ("newton_iters", "pcg_iters", "contacts", "t_total_ms")}, flush=True)
+        gpu.close()
+    disp = np.abs(out["fp64"] - sd.X.ravel()).max()
+    assert disp > 0
+    assert np.abs(out["fp32"] - out["fp64"]).max() <= 1e-6 * disp + 1e-9
