@@ -111,10 +111,10 @@ struct Ctx {
     // elastic Hessian: static BSR pattern + gather plan
     BSR Ael;
     DevBuf<int> planOff, planSlot;     // per BSR block: list of local block slots
-    DevBuf<double> elLocal;            // local 3x3 blocks, SoA [9][nSlots]
+    DevBuf<double> elLocal;            // local 3x3 blocks, [nSlots][9]
     int nElSlots = 0, nTriSlots = 0;
     DevBuf<int> gplanOff, gplanSlot;   // per vertex: gradient slots (tri 3 per, hinge 4 per)
-    DevBuf<double> gLocal;             // local gradient 3-vectors, SoA [3][nGSlots]
+    DevBuf<double> gLocal;             // local gradient 3-vectors, [nGSlots][3]
     int nGSlots = 0;
 
     // contact Hessian (rebuilt per Newton iteration)

@@ -153,7 +153,7 @@ __global__ void k_memb_grad_local(const double* __restrict__ x, const int* __res
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) gl[k * ns + 3l * t + q] = h2 * g[3 * q + k];
+        for (int k = 0; k < 3; ++k) gl[3 * (3l * t + q) + k] = h2 * g[3 * q + k];
 }
 __global__ void k_bend_grad_local(const double* __restrict__ x, const int* __restrict__ hg, int nH, const double* __restrict__ hi,
                                   double kb, double h2, const uint8_t* __restrict__ dbc, double* __restrict__ gl, long ns, long off) {
@@ -175,7 +175,7 @@ __global__ void k_bend_grad_local(const double* __restrict__ x, const int* __res
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) gl[k * ns + off + 4l * h + q] = g[3 * q + k];
+        for (int k = 0; k < 3; ++k) gl[3 * (off + 4l * h + q) + k] = g[3 * q + k];
 }
 __global__ void k_rod_grad_local(const double* __restrict__ x, const int* __restrict__ rv, int nR, const double* __restrict__ par,
                                  double* __restrict__ gl, long ns, long off) {
@@ -187,7 +187,7 @@ __global__ void k_rod_grad_local(const double* __restrict__ x, const int* __rest
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) gl[k * ns + off + 2l * r + q] = w * g[3 * q + k];
+        for (int k = 0; k < 3; ++k) gl[3 * (off + 2l * r + q) + k] = w * g[3 * q + k];
 }
 __global__ void k_rod_bend_grad_local(const double* __restrict__ x, const int* __restrict__ hv3, int nRH, const double* __restrict__ hw,
                                       double* __restrict__ gl, long ns, long off) {
@@ -199,7 +199,7 @@ __global__ void k_rod_bend_grad_local(const double* __restrict__ x, const int* _
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) gl[k * ns + off + 3l * h + q] = w * g[3 * q + k];
+        for (int k = 0; k < 3; ++k) gl[3 * (off + 3l * h + q) + k] = w * g[3 * q + k];
 }
 // g[v] = sum of local slots + m (x - xt)
 __global__ void k_el_grad_gather(const double* __restrict__ gl, long ns, const int* __restrict__ off, const int* __restrict__ slot,
@@ -210,7 +210,7 @@ __global__ void k_el_grad_gather(const double* __restrict__ gl, long ns, const i
     double a = 0, b = 0, d = 0;
     for (int k = off[v]; k < off[v + 1]; ++k) {
         long q = slot[k];
-        a += gl[q]; b += gl[ns + q]; d += gl[2 * ns + q];
+        a += gl[3 * q]; b += gl[3 * q + 1]; d += gl[3 * q + 2];
     }
     if (withInertia) {
         a += m[v] * (x[3 * v] - xt[3 * v]);
@@ -253,8 +253,9 @@ void elastic_gradient(Ctx& c, const double* x, double* g) { elastic_grad_impl(c,
 void elastic_gradient_only(Ctx& c, const double* x, double* g) { elastic_grad_impl(c, x, g, 0); }
 
 // ------------------------------------------------------------------ Hessian
-// local blocks SoA [9][nSlots]; triangle t block (a, b) at 9 t + 3 a + b, hinge h
-// block (a, b) at 9 nT + 16 h + 4 a + b
+// local blocks, one 3x3 block (9 contiguous doubles, row-major) per slot, so that the
+// assembly gathers each slot with one or two cache lines; triangle t block (a, b) at slot
+// 9 t + 3 a + b, hinge h block (a, b) at 9 nT + 16 h + 4 a + b
 __global__ void __launch_bounds__(64) k_memb_hess_local(const double* __restrict__ x, const int* __restrict__ tri, int nT,
                                                         const double* __restrict__ Binv, const double* __restrict__ vol, double lam,
                                                         double mu, double h2, const uint8_t* __restrict__ dbc, double* __restrict__ hl,
@@ -279,7 +280,7 @@ __global__ void __launch_bounds__(64) k_memb_hess_local(const double* __restrict
 #pragma unroll
             for (int k = 0; k < 3; ++k)
 #pragma unroll
-                for (int l = 0; l < 3; ++l) hl[(long)(3 * k + l) * ns + 9l * t + 3 * p + q] = H[(3 * p + k) * 9 + 3 * q + l];
+                for (int l = 0; l < 3; ++l) hl[9 * (9l * t + 3 * p + q) + 3 * k + l] = H[(3 * p + k) * 9 + 3 * q + l];
 }
 __global__ void __launch_bounds__(64) k_bend_hess_local(const double* __restrict__ x, const int* __restrict__ hg, int nH,
                                                         const double* __restrict__ hi, double kb, double h2,
@@ -312,7 +313,7 @@ __global__ void __launch_bounds__(64) k_bend_hess_local(const double* __restrict
 #pragma unroll
             for (int k = 0; k < 3; ++k)
 #pragma unroll
-                for (int l = 0; l < 3; ++l) hl[(long)(3 * k + l) * ns + off + 16l * h + 4 * p + q] = H[(3 * p + k) * 12 + 3 * q + l];
+                for (int l = 0; l < 3; ++l) hl[9 * (off + 16l * h + 4 * p + q) + 3 * k + l] = H[(3 * p + k) * 12 + 3 * q + l];
 }
 // rod segment s block (a, b) at off + 4 s + 2 a + b; rod hinge h block (a, b) at
 // off2 + 9 h + 3 a + b
@@ -329,7 +330,7 @@ __global__ void k_rod_hess_local(const double* __restrict__ x, const int* __rest
         for (int q = 0; q < 2; ++q) {
             const double sg = (p == q) ? w : -w;
 #pragma unroll
-            for (int e = 0; e < 9; ++e) hl[(long)e * ns + off + 4l * r + 2 * p + q] = sg * K[e];
+            for (int e = 0; e < 9; ++e) hl[9 * (off + 4l * r + 2 * p + q) + e] = sg * K[e];
         }
 }
 __global__ void __launch_bounds__(64) k_rod_bend_hess_local(const double* __restrict__ x, const int* __restrict__ hv3, int nRH,
@@ -347,7 +348,7 @@ __global__ void __launch_bounds__(64) k_rod_bend_hess_local(const double* __rest
 #pragma unroll
             for (int k = 0; k < 3; ++k)
 #pragma unroll
-                for (int l = 0; l < 3; ++l) hl[(long)(3 * k + l) * ns + off + 9l * h + 3 * p + q] = w * H[(3 * p + k) * 9 + 3 * q + l];
+                for (int l = 0; l < 3; ++l) hl[9 * (off + 9l * h + 3 * p + q) + 3 * k + l] = w * H[(3 * p + k) * 9 + 3 * q + l];
 }
 __global__ void k_el_assemble(const double* __restrict__ hl, long ns, const int* __restrict__ off, const int* __restrict__ slot,
                               const int* __restrict__ rowOf, const int* __restrict__ col, int nnzb, const double* __restrict__ m,
@@ -358,7 +359,7 @@ __global__ void k_el_assemble(const double* __restrict__ hl, long ns, const int*
     for (int k = off[b]; k < off[b + 1]; ++k) {
         long q = slot[k];
 #pragma unroll
-        for (int e = 0; e < 9; ++e) acc[e] += hl[e * ns + q];
+        for (int e = 0; e < 9; ++e) acc[e] += hl[9 * q + e];
     }
     int r = rowOf[b], cl = col[b];
     if (r == cl) { acc[0] += m[r]; acc[4] += m[r]; acc[8] += m[r]; }

@@ -378,47 +378,76 @@ __global__ __launch_bounds__(256) void k_lp_dense_mfma(int nrows, LpSeg s0, LpSe
 }
 
 // ------------------------------------------------------------------ G assembly / products
+// BF16 (round to nearest even) of a finite float, as the high half of its bit pattern
+__device__ __forceinline__ uint32_t lp_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+// 9 BF16 values of a 3x3 block packed in five 32-bit words (entries 2w, 2w+1 in word w;
+// entry 8 alone in word 4), stored SoA [5][n]
+__device__ __forceinline__ void lp_store_packed(uint32_t* __restrict__ dst, long n, long pos, const uint32_t* h) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) dst[w * n + pos] = h[2 * w] | (h[2 * w + 1] << 16);
+    dst[4 * n + pos] = h[8];
+}
+
+// Writes G (FP32, SoA over the elastic pattern: the training / inspection form) and the
+// two compact BF16 block CSRs of F = G S used by the PCG: FL = F (lower incl. diagonal)
+// and FU = F^T, so that M^-1 r = S G^T G S r = FU (FL r).  F's column scaling by
+// S = diag(H)^-1/2 removes the scaling vectors from both products; the same rounded F
+// sits on both sides, so M^-1 stays symmetric positive definite.
 __global__ void k_lp_assemble_G(int nL, int nV, int nnzb, const int* __restrict__ lEdge, const int* __restrict__ eBlock,
-                                const int* __restrict__ blkT, const int* __restrict__ diagBlk, const float* __restrict__ Gl,
-                                const float* __restrict__ raw, float* __restrict__ G, int nLow, const int* __restrict__ posLE,
-                                const int* __restrict__ posLD, const int* __restrict__ posUE, const int* __restrict__ posUD,
-                                float* __restrict__ GL, float* __restrict__ GU) {
+                                const int* __restrict__ blkT, const int* __restrict__ diagBlk, const int* __restrict__ bcol,
+                                const double* __restrict__ S, const float* __restrict__ Gl, const float* __restrict__ raw,
+                                float* __restrict__ G, int nLow, const int* __restrict__ posLE, const int* __restrict__ posLD,
+                                const int* __restrict__ posUE, const int* __restrict__ posUD, uint32_t* __restrict__ FL,
+                                uint32_t* __restrict__ FU) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float g[9];
+    int col, pl, pu;
     if (t < nL) {
-        const int b = eBlock[lEdge[t]], tb = blkT[b], pl = posLE[t], pu = posUE[t];
+        const int b = eBlock[lEdge[t]], tb = blkT[b];
+        pl = posLE[t]; pu = posUE[t]; col = bcol[b];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float v = Gl[9l * t + 3 * a + c];
+                g[3 * a + c] = v;
                 G[(long)(3 * a + c) * nnzb + b] = v;
                 G[(long)(3 * c + a) * nnzb + tb] = v;
-                GL[(long)(3 * a + c) * nLow + pl] = v;
-                GU[(long)(3 * c + a) * nLow + pu] = v;
             }
     } else if (t < nL + nV) {
-        const int i = t - nL, b = diagBlk[i], pl = posLD[i], pu = posUD[i];
+        const int i = t - nL, b = diagBlk[i];
+        pl = posLD[i]; pu = posUD[i]; col = i;
         const float* r = raw + 6l * i;
-        const float g[9] = {expf(r[0]), 0.f, 0.f, r[3], expf(r[1]), 0.f, r[4], r[5], expf(r[2])};
+        g[0] = expf(r[0]); g[1] = 0.f; g[2] = 0.f; g[3] = r[3]; g[4] = expf(r[1]); g[5] = 0.f; g[6] = r[4]; g[7] = r[5]; g[8] = expf(r[2]);
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                G[(long)(3 * a + c) * nnzb + b] = g[3 * a + c];
-                GL[(long)(3 * a + c) * nLow + pl] = g[3 * a + c];
-                GU[(long)(3 * c + a) * nLow + pu] = g[3 * a + c];
-            }
+        for (int k = 0; k < 9; ++k) G[(long)k * nnzb + b] = g[k];
+    } else {
+        return;
     }
+    uint32_t hl[9], hu[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t h = lp_bf16(g[3 * a + c] * (float)S[3 * col + c]);
+            hl[3 * a + c] = h;
+            hu[3 * c + a] = h;
+        }
+    lp_store_packed(FL, nLow, pl, hl);
+    lp_store_packed(FU, nLow, pu, hu);
 }
 
-// y_i = so_i . sum_b V_b (si . x)_col(b) over a compact block CSR (FP32 blocks, FP64
-// vectors); si / so optional diagonal scalings; when r != nullptr, per-workgroup partial
-// sums of r.y.  Fixed grid, rows mapped XCD-aware, next row group's bounds loaded ahead.
+// y_i = sum_b F_b x_col(b) over a compact block CSR of BF16 blocks packed five words per
+// block (SoA [5][nnz]); FP32 or FP64 arithmetic by V.  When r != nullptr, per-workgroup
+// partial sums of r.y.  Fixed grid, rows mapped XCD-aware, next row group's bounds
+// loaded ahead.
 template <int LANES, int BS, class V>
-__global__ __launch_bounds__(BS) void k_lp_spmv(int nrow, int rowsPerXcd, const int* __restrict__ rp, const int* __restrict__ col,
-                                                const float* __restrict__ val, int nnz, const double* __restrict__ si,
-                                                const V* __restrict__ x, const double* __restrict__ so, V* __restrict__ y,
-                                                const V* __restrict__ r, double* __restrict__ part) {
+__global__ __launch_bounds__(BS) void k_lp_spmv_bf16(int nrow, int rowsPerXcd, const int* __restrict__ rp, const int* __restrict__ col,
+                                                     const uint32_t* __restrict__ val, int nnz, const V* __restrict__ x,
+                                                     V* __restrict__ y, const V* __restrict__ r, double* __restrict__ part) {
     __shared__ double lds[BS / 64];
     constexpr int RPG = BS / LANES;
     const int xb = blockIdx.x % 8, jb = blockIdx.x / 8, nj = gridDim.x / 8;
@@ -436,15 +465,17 @@ __global__ __launch_bounds__(BS) void k_lp_spmv(int nrow, int rowsPerXcd, const 
         V acc[3] = {0, 0, 0};
         for (int b = beg + lane; b < end; b += LANES) {
             const int c = col[b];
-            V x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
-            if (si) {
-                x0 *= (V)si[3 * c];
-                x1 *= (V)si[3 * c + 1];
-                x2 *= (V)si[3 * c + 2];
-            }
+            const V x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
+            uint32_t w[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) w[k] = val[(long)k * nnz + b];
             V a[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) a[k] = (V)val[(long)k * nnz + b];
+            for (int k = 0; k < 4; ++k) {
+                a[2 * k] = (V)__uint_as_float(w[k] << 16);
+                a[2 * k + 1] = (V)__uint_as_float(w[k] & 0xFFFF0000u);
+            }
+            a[8] = (V)__uint_as_float(w[4] << 16);
             acc[0] += a[0] * x0 + a[1] * x1 + a[2] * x2;
             acc[1] += a[3] * x0 + a[4] * x1 + a[5] * x2;
             acc[2] += a[6] * x0 + a[7] * x1 + a[8] * x2;
@@ -456,9 +487,8 @@ __global__ __launch_bounds__(BS) void k_lp_spmv(int nrow, int rowsPerXcd, const 
         if (row < nrow && lane == 0) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const V v = so ? (V)so[3 * row + k] * acc[k] : acc[k];
-                y[3 * row + k] = v;
-                if (r) dot += (double)r[3 * row + k] * (double)v;
+                y[3 * row + k] = acc[k];
+                if (r) dot += (double)r[3 * row + k] * (double)acc[k];
             }
         }
         row = rowN;
@@ -713,8 +743,8 @@ void lp_build_graph(Ctx& c) {
     lp.nLow = (int)Lcol.size();
     upload(lp.gLrp, Lrp, c.s); upload(lp.gLcol, Lcol, c.s); upload(lp.gUrp, Urp, c.s); upload(lp.gUcol, Ucol, c.s);
     upload(lp.posLE, pLE, c.s); upload(lp.posLD, pLD, c.s); upload(lp.posUE, pUE, c.s); upload(lp.posUD, pUD, c.s);
-    lp.GL.ensure_exact(9 * (size_t)lp.nLow);
-    lp.GU.ensure_exact(9 * (size_t)lp.nLow);
+    lp.FL.ensure_exact(5 * (size_t)lp.nLow);
+    lp.FU.ensure_exact(5 * (size_t)lp.nLow);
     CIPC_HIP(hipStreamSynchronize(c.s));
     lp.nV = nV; lp.nnzb = nnzb; lp.nE = nE; lp.nL = nL;
     const size_t H = kLpHidden;
@@ -891,9 +921,10 @@ void lp_forward(Ctx& c) {
              Wq + L.bo, 0, nullptr, lp.Gl.p);
     dense<6>(c, nV, {hL, nullptr, kLpHidden}, kNoSeg, kNoSeg, Wq + L.wd, Wq + L.bd, 0, nullptr, lp.raw.p);
     hipLaunchKernelGGL(k_lp_assemble_G, dim3(nblocks(nL + nV)), dim3(kBlock), 0, c.s, nL, nV, lp.nnzb, (const int*)lp.lEdge.p,
-                       (const int*)lp.eBlock.p, (const int*)lp.blkT.p, (const int*)lp.diagBlk.p, (const float*)lp.Gl.p,
-                       (const float*)lp.raw.p, lp.Gblk.p, lp.nLow, (const int*)lp.posLE.p, (const int*)lp.posLD.p, (const int*)lp.posUE.p,
-                       (const int*)lp.posUD.p, lp.GL.p, lp.GU.p);
+                       (const int*)lp.eBlock.p, (const int*)lp.blkT.p, (const int*)lp.diagBlk.p, (const int*)c.Ael.col.p,
+                       (const double*)lp.S.p, (const float*)lp.Gl.p, (const float*)lp.raw.p, lp.Gblk.p, lp.nLow,
+                       (const int*)lp.posLE.p, (const int*)lp.posLD.p, (const int*)lp.posUE.p, (const int*)lp.posUD.p, lp.FL.p,
+                       lp.FU.p);
     CIPC_LAUNCH_CHECK();
 }
 
@@ -914,13 +945,12 @@ int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part) {
     constexpr int RPG = kLpSpmvBlock / kLpSpmvLanes;
     int rpx = (lp.nV + 7) / 8;
     rpx = (rpx + RPG - 1) / RPG * RPG;
-    // tmp = G (S r);  z = S G^T tmp  (+ partial sums of r.z)
-    hipLaunchKernelGGL((k_lp_spmv<kLpSpmvLanes, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, lp.nV, rpx,
-                       (const int*)lp.gLrp.p, (const int*)lp.gLcol.p, (const float*)lp.GL.p, lp.nLow, (const double*)lp.S.p, r,
-                       (const double*)nullptr, tmp, (const V*)nullptr, (double*)nullptr);
-    hipLaunchKernelGGL((k_lp_spmv<kLpSpmvLanes, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, lp.nV, rpx,
-                       (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const float*)lp.GU.p, lp.nLow, (const double*)nullptr,
-                       (const V*)tmp, (const double*)lp.S.p, z, r, part);
+    // tmp = F r;  z = F^T tmp  (+ partial sums of r.z), F = G S
+    hipLaunchKernelGGL((k_lp_spmv_bf16<kLpSpmvLanes, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, lp.nV, rpx,
+                       (const int*)lp.gLrp.p, (const int*)lp.gLcol.p, (const uint32_t*)lp.FL.p, lp.nLow, r, tmp, (const V*)nullptr,
+                       (double*)nullptr);
+    hipLaunchKernelGGL((k_lp_spmv_bf16<kLpSpmvLanes, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, lp.nV, rpx,
+                       (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, (const V*)tmp, z, r, part);
     CIPC_LAUNCH_CHECK();
     return kLpSpmvGrid;
 }

@@ -43,11 +43,12 @@ struct LearnedPrecond {
     int nV = 0, nnzb = 0, nE = 0, nL = 0;
     // static graph of the elastic BSR pattern (directed edges = off-diagonal blocks)
     DevBuf<int> eBlock, eSrc, eDst, erp, eT, lEdge, blkT, diagBlk, lSrcOff, lSrcIdx, lDstOff, lDstIdx, lowSrc, lowDst;
-    // compact block-CSR of G (lower triangle incl. diagonal) and of G^T (upper), for the
-    // two products of the preconditioner; pos* map lower edges / diagonals to their slots
+    // compact block CSRs of F = G S (lower triangle incl. diagonal) and of F^T (upper),
+    // for the two products of the preconditioner; pos* map lower edges / diagonals to
+    // their slots
     int nLow = 0;
     DevBuf<int> gLrp, gLcol, gUrp, gUcol, posLE, posLD, posUE, posUD;
-    DevBuf<float> GL, GU;             // SoA [9][nLow]
+    DevBuf<uint32_t> FL, FU;          // BF16 blocks, five packed words per block, SoA [5][nLow]
     // parameters
     DevBuf<float> W, Wq, gW, mW, vW;
     int adamT = 0;
