@@ -635,6 +635,76 @@ CIPC_DEV void dihedral_angle_hess(d3 x0, d3 x1, d3 x2, d3 x3, double* H) {
         }
 }
 
+// Row r (vertex r / 3, component r % 3) of the dihedral angle's Hessian before the
+// symmetrisation done by dihedral_angle_hess: the same closed form, evaluated for one
+// vertex of the stencil (used by the lane-cooperative hinge kernel, one row per lane).
+CIPC_DEV void dihedral_hess_row(d3 x0, d3 x1, d3 x2, d3 x3, int r, double* row) {
+    const int i = r / 3, a = r % 3;
+    d3 e = x2 - x1;
+    double L = sqrt(sq(e)), iL = 1.0 / L;
+    d3 nA = cross(x1 - x0, x2 - x0), nB = cross(x2 - x3, x1 - x3);
+    double AA = sq(nA), AB = sq(nB);
+    d3 zero = {0, 0, 0};
+    d3 vA[4] = {x2 - x1, x0 - x2, x1 - x0, zero};
+    d3 vB[4] = {zero, x2 - x3, x3 - x1, x1 - x2};
+    d3 gL[4] = {zero, (-iL) * e, iL * e, zero};
+    d3 gAA[4], gAB[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { gAA[j] = 2.0 * cross(nA, vA[j]); gAB[j] = 2.0 * cross(nB, vB[j]); }
+    double sA = 0, sB = 0;
+    d3 gsA[4] = {zero, zero, zero, zero}, gsB[4] = {zero, zero, zero, zero};
+    if (i == 0) {
+        sA = -L / AA;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gsA[j] = (-1.0 / AA) * gL[j] + (L / (AA * AA)) * gAA[j];
+    } else if (i == 3) {
+        sB = -L / AB;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gsB[j] = (-1.0 / AB) * gL[j] + (L / (AB * AB)) * gAB[j];
+    } else {
+        const double LA = L * AA, iLA = 1.0 / LA, LB = L * AB, iLB = 1.0 / LB;
+        if (i == 1) {
+            double PA1 = dot(e, x0 - x2), PB1 = dot(e, x3 - x2);
+            d3 gPA1[4] = {e, -1.0 * (x0 - x2), (x0 - x2) - e, zero};
+            d3 gPB1[4] = {zero, -1.0 * (x3 - x2), (x3 - x2) - e, e};
+            double cA = PA1 * iLA * iLA, cB = PB1 * iLB * iLB;
+            sA = -PA1 * iLA;
+            sB = -PB1 * iLB;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                gsA[j] = -1.0 * (iLA * gPA1[j] - cA * (AA * gL[j] + L * gAA[j]));
+                gsB[j] = -1.0 * (iLB * gPB1[j] - cB * (AB * gL[j] + L * gAB[j]));
+            }
+        } else {
+            double PA2 = dot(e, x0 - x1), PB2 = dot(e, x3 - x1);
+            d3 gPA2[4] = {e, -1.0 * (x0 - x1) - e, x0 - x1, zero};
+            d3 gPB2[4] = {zero, -1.0 * (x3 - x1) - e, x3 - x1, e};
+            double cA = PA2 * iLA * iLA, cB = PB2 * iLB * iLB;
+            sA = PA2 * iLA;
+            sB = PB2 * iLB;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                gsA[j] = iLA * gPA2[j] - cA * (AA * gL[j] + L * gAA[j]);
+                gsB[j] = iLB * gPB2[j] - cB * (AB * gL[j] + L * gAB[j]);
+            }
+        }
+    }
+    const double na = a == 0 ? nA.x : (a == 1 ? nA.y : nA.z);
+    const double nb = a == 0 ? nB.x : (a == 1 ? nB.y : nB.z);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        double CA[9], CB[9];
+        cpm(vA[j], CA); cpm(vB[j], CB);
+        const double ga[3] = {gsA[j].x, gsA[j].y, gsA[j].z}, gb[3] = {gsB[j].x, gsB[j].y, gsB[j].z};
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const double ca = a == 0 ? CA[b] : (a == 1 ? CA[3 + b] : CA[6 + b]);
+            const double cb = a == 0 ? CB[b] : (a == 1 ? CB[3 + b] : CB[6 + b]);
+            row[j * 3 + b] = na * ga[b] + sA * ca + nb * gb[b] + sB * cb;
+        }
+    }
+}
+
 // ------------------------------------------------------------- rods
 // Segment spring, Rod/ROD_STRETCHING.h:10-107: E = Er / (2 lbar) (l - lbar)^2 with
 // l = |x0 - x1| (the caller multiplies by h^2 pi t^2 / 4).  Gradient, and the Hessian

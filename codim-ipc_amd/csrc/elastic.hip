@@ -282,38 +282,151 @@ __global__ void __launch_bounds__(64) k_memb_hess_local(const double* __restrict
 #pragma unroll
                 for (int l = 0; l < 3; ++l) hl[9 * (9l * t + 3 * p + q) + 3 * k + l] = H[(3 * p + k) * 9 + 3 * q + l];
 }
-__global__ void __launch_bounds__(64) k_bend_hess_local(const double* __restrict__ x, const int* __restrict__ hg, int nH,
-                                                        const double* __restrict__ hi, double kb, double h2,
-                                                        const uint8_t* __restrict__ dbc, double* __restrict__ hl, long ns, long off) {
-    int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nH) return;
-    int v[4] = {hg[h], hg[nH + h], hg[2 * nH + h], hg[3 * nH + h]};
-    double H[144];
-    if (kb == 0.0 || (dbc[v[0]] && dbc[v[1]] && dbc[v[2]] && dbc[v[3]])) {
+// Hinge Hessians with 16 lanes per hinge (lanes 0..11 own one row each, 16 hinges per
+// workgroup): row r of the dihedral Hessian is evaluated by lane r, symmetrised through
+// LDS, combined with the gradient term, and projected to PSD by a parallel cyclic Jacobi
+// eigensolver -- each round rotates 6 disjoint index pairs of a round-robin schedule
+// (12 players: 11 rounds per sweep), rows exchanged between partner lanes and rotation
+// parameters broadcast by shuffles -- then rebuilt as V max(L, 0) V^T (Math/UTILS.h:
+// 10-27).  Registers stay at one row of A and one of V per lane (no spills).
+__device__ __forceinline__ int rr_partner(int r, int t) {
+    if (r >= 12) return r;
+    if (r == 11) return t;
+    if (r == t) return 11;
+    return (2 * t - r + 22) % 11;
+}
+__global__ void __launch_bounds__(256) k_bend_hess_local(const double* __restrict__ x, const int* __restrict__ hg, int nH,
+                                                         const double* __restrict__ hi, double kb, double h2,
+                                                         const uint8_t* __restrict__ dbc, double* __restrict__ hl, long ns, long off) {
+    __shared__ double sh[16][168];
+    const int grp = threadIdx.x >> 4, r = threadIdx.x & 15;
+    const int h = blockIdx.x * 16 + grp;
+    const bool live = h < nH;
+    double* M = sh[grp];
+    int v[4] = {0, 0, 0, 0};
+    if (live) { v[0] = hg[h]; v[1] = hg[nH + h]; v[2] = hg[2 * nH + h]; v[3] = hg[3 * nH + h]; }
+    const bool active = live && kb != 0.0 && !(dbc[v[0]] && dbc[v[1]] && dbc[v[2]] && dbc[v[3]]);
+    double A[12], g[12];
 #pragma unroll
-        for (int k = 0; k < 144; ++k) H[k] = 0;
-    } else {
+    for (int c = 0; c < 12; ++c) { A[c] = 0; g[c] = 0; }
+    double s1 = 0, s2 = 0;
+    if (active) {
         d3 p0 = ld3(x, v[0]), p1 = ld3(x, v[1]), p2 = ld3(x, v[2]), p3 = ld3(x, v[3]);
-        double th = dihedral_angle(p0, p1, p2, p3);
-        double g[12];
+        const double th = dihedral_angle(p0, p1, p2, p3);
         dihedral_angle_grad(p0, p1, p2, p3, g);
-        dihedral_angle_hess(p0, p1, p2, p3, H);
-        double s1 = h2 * kb * 2 * (th - hi[h]) * hi[nH + h] / hi[2 * nH + h];
-        double s2 = h2 * kb * 2 * hi[nH + h] / hi[2 * nH + h];
-#pragma unroll
-        for (int i = 0; i < 12; ++i)
-#pragma unroll
-            for (int j = 0; j < 12; ++j) H[i * 12 + j] = s1 * H[i * 12 + j] + s2 * g[i] * g[j];
-        make_pd<12>(H);
+        s1 = h2 * kb * 2 * (th - hi[h]) * hi[nH + h] / hi[2 * nH + h];
+        s2 = h2 * kb * 2 * hi[nH + h] / hi[2 * nH + h];
+        if (r < 12) dihedral_hess_row(p0, p1, p2, p3, r, A);
     }
+    if (r < 12)
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+        for (int c = 0; c < 12; ++c) M[r * 12 + c] = A[c];
+    __syncthreads();
+    double gr = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+    for (int c = 0; c < 12; ++c) gr = (c == r) ? g[c] : gr;
+    if (r < 12)
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 12; ++c) A[c] = s1 * (0.5 * (M[r * 12 + c] + M[c * 12 + r])) + s2 * gr * g[c];
+    __syncthreads();
+    // Jacobi: V = I; stop when the squared off-diagonal mass is <= 1e-24 of ||A||_F^2
+    // (entries of the rebuilt matrix then carry relative errors of ~1e-12).  Partner rows
+    // and rotation parameters go through the hinge's LDS rows; a hinge's 16 lanes lie in
+    // one wave, whose LDS operations complete in issue order.
+    double* P = sh[grp] + 144;   // rotation (c, s) of each pair, at its lower index
+    double V[12];
 #pragma unroll
-                for (int l = 0; l < 3; ++l) hl[9 * (off + 16l * h + 4 * p + q) + 3 * k + l] = H[(3 * p + k) * 12 + 3 * q + l];
+    for (int c = 0; c < 12; ++c) V[c] = (c == r) ? 1.0 : 0.0;
+    double fro = 0;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) fro += A[c] * A[c];
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) fro += __shfl_xor(fro, o, 16);
+    const double tol = 1e-24 * fro;
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        double off = 0;
+#pragma unroll
+        for (int c = 0; c < 12; ++c) off += (c > r) ? A[c] * A[c] : 0.0;
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) off += __shfl_xor(off, o, 16);
+        if (off <= tol) break;
+#pragma unroll
+        for (int t = 0; t < 11; ++t) {
+            const int pr = rr_partner(r, t);
+            if (r < 12)
+#pragma unroll
+                for (int c = 0; c < 12; ++c) M[r * 12 + c] = A[c];
+            __builtin_amdgcn_wave_barrier();
+            double app = 0, apq = 0, aqq = 0;
+            if (r < 12) {
+                app = M[r * 13];
+                apq = M[r * 12 + pr];
+                aqq = M[pr * 13];
+            }
+            // rotation of the pair, computed by its lower-index lane
+            if (r < pr) {
+                double cl = 1, sl = 0;
+                if (apq != 0.0) {
+                    const double theta = (aqq - app) / (2 * apq);
+                    const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                    cl = 1 / sqrt(tt * tt + 1);
+                    sl = tt * cl;
+                }
+                P[2 * r] = cl;
+                P[2 * r + 1] = sl;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // row rotation with the partner's row (P^T A)
+            if (r < 12) {
+                const int ld = min(r, pr);
+                const double cs = P[2 * ld], sn = r < pr ? -P[2 * ld + 1] : P[2 * ld + 1];
+                double o[12];
+#pragma unroll
+                for (int c = 0; c < 12; ++c) o[c] = M[pr * 12 + c];
+#pragma unroll
+                for (int c = 0; c < 12; ++c) A[c] = cs * A[c] + sn * o[c];
+            }
+            // column rotations of all six pairs (A P, V P); the round's pairing is known
+            // at compile time once the round loop is unrolled
+            double An[12], Vn[12];
+#pragma unroll
+            for (int c = 0; c < 12; ++c) {
+                const int q = rr_partner(c, t), L = c < q ? c : q;
+                const double cc = P[2 * L], ss = P[2 * L + 1];
+                if (c < q) {
+                    An[c] = cc * A[c] - ss * A[q];
+                    Vn[c] = cc * V[c] - ss * V[q];
+                } else {
+                    An[c] = ss * A[q] + cc * A[c];
+                    Vn[c] = ss * V[q] + cc * V[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 12; ++c) { A[c] = An[c]; V[c] = Vn[c]; }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    // H+ = V diag(max(lambda, 0)) V^T, lambda_r = A[r][r]
+    double lam = 0;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) lam = (c == r) ? A[c] : lam;
+    __syncthreads();
+    if (r < 12) {
+#pragma unroll
+        for (int c = 0; c < 12; ++c) M[r * 12 + c] = V[c];
+        M[144 + r] = lam > 0 ? lam : 0.0;
+    }
+    __syncthreads();
+    if (!live || r >= 12) return;
+    const int p = r / 3, k = r % 3;
+    double* dst = hl + 9 * (off + 16l * h + 4 * p) + 3 * k;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+        double sum = 0;
+#pragma unroll
+        for (int m = 0; m < 12; ++m) sum += V[m] * M[144 + m] * M[c * 12 + m];
+        dst[9 * (c / 3) + c % 3] = active ? sum : 0.0;
+    }
 }
 // rod segment s block (a, b) at off + 4 s + 2 a + b; rod hinge h block (a, b) at
 // off2 + 9 h + 3 a + b
@@ -382,7 +495,7 @@ void elastic_hessian(Ctx& c, const double* x) {
     }
     if (c.nH) {
         double kb = c.P.bendMult != 0 ? c.kBend : 0.0;
-        hipLaunchKernelGGL(k_bend_hess_local, dim3(nblocks(c.nH, 64)), dim3(64), 0, c.s, x, c.hinge.p, c.nH, c.hInfo.p, kb, h2, c.dbc.p,
+        hipLaunchKernelGGL(k_bend_hess_local, dim3((c.nH + 15) / 16), dim3(256), 0, c.s, x, c.hinge.p, c.nH, c.hInfo.p, kb, h2, c.dbc.p,
                            hl, ns, 9l * c.nT);
         CIPC_LAUNCH_CHECK();
     }

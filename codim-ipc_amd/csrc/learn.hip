@@ -7,6 +7,7 @@
 // weight gradients are summed per workgroup over a fixed row range and the workgroup
 // partials are added in index order.
 #include "learn.hpp"
+#include <cstdlib>
 #include <algorithm>
 #include <cmath>
 #include <random>
@@ -656,7 +657,14 @@ __global__ void k_lp_sum_rawdiag(int nV, const float* __restrict__ raw, double* 
 
 // ------------------------------------------------------------------ host side
 static LearnedPrecond& LP(Ctx& c) {
-    if (!c.lp) c.lp.reset(new LearnedPrecond());
+    if (!c.lp) {
+        c.lp.reset(new LearnedPrecond());
+        // lanes per block row of the factor products (2, 4, 8 or 16); CIPC_LP_LANES overrides
+        if (const char* e = std::getenv("CIPC_LP_LANES")) {
+            const int v = std::atoi(e);
+            if (v == 2 || v == 4 || v == 8 || v == 16) c.lp->lanes = v;
+        }
+    }
     return *c.lp;
 }
 static double host_sum(Ctx& c, const double* part, int n) {
@@ -939,23 +947,45 @@ void lp_prepare(Ctx& c) {
 constexpr int kLpSpmvLanes = 8, kLpSpmvBlock = 256, kLpSpmvGrid = 1024;
 int lp_apply_partials(int) { return kLpSpmvGrid; }
 
+template <int LANES, class V>
+static void lp_spmv_go(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part) {
+    constexpr int RPG = kLpSpmvBlock / LANES;
+    const int nV = c.lp->nV;
+    int rpx = (nV + 7) / 8;
+    rpx = (rpx + RPG - 1) / RPG * RPG;
+    hipLaunchKernelGGL((k_lp_spmv_bf16<LANES, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, nV, rpx, rp, col, val, nnz,
+                       x, y, r, part);
+}
+template <class V>
+static void lp_spmv(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part) {
+    switch (c.lp->lanes) {
+        case 2: lp_spmv_go<2, V>(c, rp, col, val, nnz, x, y, r, part); break;
+        case 4: lp_spmv_go<4, V>(c, rp, col, val, nnz, x, y, r, part); break;
+        case 16: lp_spmv_go<16, V>(c, rp, col, val, nnz, x, y, r, part); break;
+        default: lp_spmv_go<8, V>(c, rp, col, val, nnz, x, y, r, part); break;
+    }
+}
+
 template <class V>
 int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part) {
     LearnedPrecond& lp = LP(c);
-    constexpr int RPG = kLpSpmvBlock / kLpSpmvLanes;
-    int rpx = (lp.nV + 7) / 8;
-    rpx = (rpx + RPG - 1) / RPG * RPG;
     // tmp = F r;  z = F^T tmp  (+ partial sums of r.z), F = G S
-    hipLaunchKernelGGL((k_lp_spmv_bf16<kLpSpmvLanes, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, lp.nV, rpx,
-                       (const int*)lp.gLrp.p, (const int*)lp.gLcol.p, (const uint32_t*)lp.FL.p, lp.nLow, r, tmp, (const V*)nullptr,
-                       (double*)nullptr);
-    hipLaunchKernelGGL((k_lp_spmv_bf16<kLpSpmvLanes, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, lp.nV, rpx,
-                       (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, (const V*)tmp, z, r, part);
+    lp_spmv<V>(c, (const int*)lp.gLrp.p, (const int*)lp.gLcol.p, (const uint32_t*)lp.FL.p, lp.nLow, r, tmp, (const V*)nullptr, nullptr);
+    lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, (const V*)tmp, z, r, part);
     CIPC_LAUNCH_CHECK();
     return kLpSpmvGrid;
 }
 template int lp_apply<double>(Ctx&, const double*, double*, double*, double*);
 template int lp_apply<float>(Ctx&, const float*, float*, float*, double*);
+
+template <class V>
+void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part) {
+    LearnedPrecond& lp = LP(c);
+    lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, y, z, r, part);
+    CIPC_LAUNCH_CHECK();
+}
+template void lp_apply_upper<double>(Ctx&, const double*, double*, const double*, double*);
+template void lp_apply_upper<float>(Ctx&, const float*, float*, const float*, double*);
 
 double lp_loss_grad(Ctx& c, int nProbes, uint64_t seed, float* gradOut) {
     LearnedPrecond& lp = LP(c);

@@ -40,6 +40,7 @@ struct LearnedPrecond {
     LpLayout L;
     bool graphReady = false, weightsReady = false;
     bool mfma = true;                 // inference forward: message layers on the matrix cores
+    int lanes = 4;                    // lanes per block row in the factor products
     int nV = 0, nnzb = 0, nE = 0, nL = 0;
     // static graph of the elastic BSR pattern (directed edges = off-diagonal blocks)
     DevBuf<int> eBlock, eSrc, eDst, erp, eT, lEdge, blkT, diagBlk, lSrcOff, lSrcIdx, lDstOff, lDstIdx, lowSrc, lowDst;
@@ -86,5 +87,21 @@ void lp_prepare(Ctx& c);
 template <class V>
 int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part);
 int lp_apply_partials(int nV);
+// z = F^T y (the second product of lp_apply) with partial sums of r.z
+template <class V>
+void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part);
+// unpack the 9 BF16 entries of a block stored as five words (SoA [5][n]) at position b
+template <class V>
+__device__ __forceinline__ void lp_unpack9(const uint32_t* __restrict__ val, long n, long b, V* a) {
+    uint32_t w[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) w[k] = val[k * n + b];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a[2 * k] = (V)__uint_as_float(w[k] << 16);
+        a[2 * k + 1] = (V)__uint_as_float(w[k] & 0xFFFF0000u);
+    }
+    a[8] = (V)__uint_as_float(w[4] << 16);
+}
 
 }  // namespace cipc

@@ -242,6 +242,81 @@ __global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPer
     }
 }
 
+// Learned form, iteration `it`, fused with the first factor product: alpha = r.z / p.q,
+// r' = r - alpha q (written to the other r buffer: rows are read by their neighbours
+// while the owner updates them), x += alpha p, y = F r' over the rows of the lower BF16
+// factor F, partial sums of r'.r'.  Every row of F stores its diagonal block; the lane
+// that meets it updates the row's r and x.  Fixed grid, rows mapped XCD-aware.
+template <int LANES, int BS, class V>
+__global__ __launch_bounds__(BS) void k_pcg_update_lower(int it, int nrow, int rowsPerXcd, const int* __restrict__ rp,
+                                                         const int* __restrict__ col, const uint32_t* __restrict__ val, int nnz,
+                                                         V* __restrict__ x, const V* __restrict__ p, const V* __restrict__ q,
+                                                         const V* __restrict__ rOld, V* __restrict__ rNew, V* __restrict__ y,
+                                                         const double* __restrict__ partS, int nS, double* __restrict__ partRR,
+                                                         double* __restrict__ sc, int* __restrict__ flags) {
+    if (flags[F_DONE]) return;
+    __shared__ double lds[BS / 64];
+    constexpr int RPG = BS / LANES;
+    const int xb = blockIdx.x % kXcd, jb = blockIdx.x / kXcd, nj = gridDim.x / kXcd;
+    const int lane = threadIdx.x % LANES;
+    const int rowBase = xb * rowsPerXcd + (int)threadIdx.x / LANES;
+    int g = jb;
+    int row = rowBase + g * RPG;
+    bool live = g * RPG < rowsPerXcd && row < nrow;
+    int beg = live ? rp[row] : 0, end = live ? rp[row + 1] : 0;
+    double a = 0;
+    for (int i = threadIdx.x; i < nS; i += BS) a += partS[i];
+    const double pq = block_allsum<BS>(a, lds);
+    if (!(pq > 0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            flags[F_FAIL] = 1;
+            flags[F_ITERS] = it;
+            flags[F_DONE] = 1;
+        }
+        return;
+    }
+    const V alpha = (V)(sc[S_RZ0 + (it & 1)] / pq);
+    double rr = 0;
+    for (; g * RPG < rowsPerXcd; g += nj) {
+        const int gn = g + nj, rowN = rowBase + gn * RPG;
+        const bool liveN = gn * RPG < rowsPerXcd && rowN < nrow;
+        const int begN = liveN ? rp[rowN] : 0, endN = liveN ? rp[rowN + 1] : 0;
+        V acc[3] = {0, 0, 0};
+        for (int b = beg + lane; b < end; b += LANES) {
+            const int c = col[b];
+            V rv[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) rv[k] = rOld[3 * c + k] - alpha * q[3 * c + k];
+            V f[9];
+            lp_unpack9<V>(val, nnz, b, f);
+            acc[0] += f[0] * rv[0] + f[1] * rv[1] + f[2] * rv[2];
+            acc[1] += f[3] * rv[0] + f[4] * rv[1] + f[5] * rv[2];
+            acc[2] += f[6] * rv[0] + f[7] * rv[1] + f[8] * rv[2];
+            if (c == row) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    rNew[3 * row + k] = rv[k];
+                    x[3 * row + k] += alpha * p[3 * row + k];
+                    rr += (double)rv[k] * (double)rv[k];
+                }
+            }
+        }
+#pragma unroll
+        for (int o = LANES / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[k] += __shfl_xor(acc[k], o, LANES);
+        if (row < nrow && lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) y[3 * row + k] = acc[k];
+        }
+        row = rowN;
+        beg = begN;
+        end = endN;
+    }
+    const double brr = block_allsum<BS>(rr, lds);
+    if (threadIdx.x == 0) partRR[blockIdx.x] = brr;
+}
+
 // r = b, x = 0, z = Dinv r; partials of r.z and r.r in the update kernel's layout;
 // the solver flags are cleared.
 template <class V, int BS>
@@ -372,18 +447,22 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     const int rpxS = rows_per_xcd(nV, RPG), nS = kSpmvGrid;
     const int rpxU = rows_per_xcd(nV, kUpdBlock), nU = kXcd * (rpxU / kUpdBlock);
     const int nP = lp_apply_partials(nV);
-    double* partS = c.dscr.ensure((size_t)nS + 2l * nU + nP);
+    double* partS = c.dscr.ensure((size_t)nS + 2l * nU + 2l * nP);
     double* partU = partS + nS;
     double* partP = partU + 2l * nU;
+    double* partF = partP + nP;               // r.r partials of the fused learned update
     // r.z partials: the update kernel's (block Jacobi) or the preconditioner product's
     const double* partRZ = learned ? partP : partU;
     const int nRZ = learned ? nP : nU;
     V* ltmp = nullptr;
+    V* rb[2] = {r, r};                        // learned form: r ping-pongs between r and a second buffer
 
     hipLaunchKernelGGL((k_pcg_init<V, kUpdBlock>), dim3(nU), dim3(kUpdBlock), 0, c.s, nV, rpxU, b, xsol, r, z, dinv, partU, flags);
     CIPC_LAUNCH_CHECK();
     if (learned) {
-        ltmp = (V*)c.lpTmp.ensure((size_t)n);
+        static_assert(sizeof(double) >= sizeof(V), "");
+        ltmp = (V*)c.lpTmp.ensure((size_t)2 * n);
+        rb[1] = ltmp + n;
         lp_apply<V>(c, (const V*)r, z, ltmp, partP);
     }
 
@@ -409,14 +488,23 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
             hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.profiling ? c.evPool[2 * k] : nullptr,
                                   c.profiling ? c.evPool[2 * k + 1] : nullptr, 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
                                   K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, partRZ, nRZ,
-                                  (const double*)(partU + nU), nU, partS, sc, flags);
+                                  (const double*)(learned && it > 0 ? partF : partU + nU), learned && it > 0 ? nP : nU, partS, sc,
+                                  flags);
             if (!learned) {
                 hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock, true>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z,
                                    (const V*)pnew, (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
             } else {
-                hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock, false>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z,
-                                   (const V*)pnew, (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
-                lp_apply<V>(c, (const V*)r, z, ltmp, partP);
+                LearnedPrecond& lp = *c.lp;
+                V* rNew = rb[(it + 1) & 1];
+                auto fused = lp.lanes == 2    ? k_pcg_update_lower<2, kSpmvBlock, V>
+                             : lp.lanes == 4  ? k_pcg_update_lower<4, kSpmvBlock, V>
+                             : lp.lanes == 16 ? k_pcg_update_lower<16, kSpmvBlock, V>
+                                              : k_pcg_update_lower<8, kSpmvBlock, V>;
+                const int rpxL = rows_per_xcd(nV, kSpmvBlock / lp.lanes);
+                hipLaunchKernelGGL(fused, dim3(nP), dim3(kSpmvBlock), 0, c.s, it, nV, rpxL, (const int*)lp.gLrp.p, (const int*)lp.gLcol.p,
+                                   (const uint32_t*)lp.FL.p, lp.nLow, xsol, (const V*)pnew, (const V*)q, (const V*)rb[it & 1], rNew, ltmp,
+                                   (const double*)partS, nS, partF, sc, flags);
+                lp_apply_upper<V>(c, (const V*)ltmp, z, (const V*)rNew, partP);
             }
         }
         CIPC_LAUNCH_CHECK();
