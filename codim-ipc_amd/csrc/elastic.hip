@@ -357,11 +357,17 @@ __global__ void __launch_bounds__(256) k_bend_hess_local(const double* __restric
 #pragma unroll
                 for (int c = 0; c < 12; ++c) M[r * 12 + c] = A[c];
             __builtin_amdgcn_wave_barrier();
+            double o[12];
             double app = 0, apq = 0, aqq = 0;
             if (r < 12) {
+#pragma unroll
+                for (int c = 0; c < 12; ++c) o[c] = M[pr * 12 + c];
                 app = M[r * 13];
                 apq = M[r * 12 + pr];
                 aqq = M[pr * 13];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 12; ++c) o[c] = 0.0;
             }
             // rotation of the pair, computed by its lower-index lane
             if (r < pr) {
@@ -380,9 +386,6 @@ __global__ void __launch_bounds__(256) k_bend_hess_local(const double* __restric
             if (r < 12) {
                 const int ld = min(r, pr);
                 const double cs = P[2 * ld], sn = r < pr ? -P[2 * ld + 1] : P[2 * ld + 1];
-                double o[12];
-#pragma unroll
-                for (int c = 0; c < 12; ++c) o[c] = M[pr * 12 + c];
 #pragma unroll
                 for (int c = 0; c < 12; ++c) A[c] = cs * A[c] + sn * o[c];
             }
