@@ -68,7 +68,7 @@ def gpu_run(args, sd, form):
     if form.endswith("_learned"):
         from cipc import precond
         sim.precond_set_weights(precond.load())
-        sim.set_preconditioner("learned", budget=args.learned_budget)
+        sim.set_preconditioner("learned", budget=args.learned_budget, reuse=args.learned_reuse)
     for _ in range(args.warmup):
         sim.step()
     sim.set_profiling(True)
@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--precision", default="all", choices=["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned", "all"])
     ap.add_argument("--learned-budget", type=int, default=2500)
+    ap.add_argument("--learned-reuse", type=int, default=None,
+                    help="Newton solves per network evaluation (default: once per step, at its first solve)")
     ap.add_argument("--pcg-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)

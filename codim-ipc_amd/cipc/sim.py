@@ -176,9 +176,13 @@ class Simulator:
         w = np.ascontiguousarray(w, dtype=np.float32)
         self._check(self.lib.cipc_precond_set_weights(self.h, w.ctypes.data_as(_lib.F32), len(w)))
 
-    def set_preconditioner(self, mode, budget=0):
-        """mode "jacobi" (block Jacobi) or "learned" (falls back to block Jacobi after `budget` iterations)."""
+    def set_preconditioner(self, mode, budget=0, reuse=None):
+        """mode "jacobi" (block Jacobi) or "learned" (falls back to block Jacobi after `budget`
+        iterations).  reuse: run the network at the first Newton solve of a step and then
+        every `reuse` solves (the others rescale its last factor)."""
         self._check(self.lib.cipc_set_preconditioner(self.h, {"jacobi": 0, "learned": 1}[mode], int(budget)))
+        if reuse is not None:
+            self._check(self.lib.cipc_set_precond_reuse(self.h, int(reuse)))
 
     def precond_loss_grad(self, x, C, kappa, n_probes=8, seed=1, want_grad=True):
         x, C = self._x(x), self._cs(C)

@@ -406,6 +406,11 @@ int cipc_set_preconditioner(cipc_ctx* ctx, int32_t mode, int32_t budget) {
     if (budget > 0) ctx->c.lpBudget = budget;
     return CIPC_OK;
 }
+int cipc_set_precond_reuse(cipc_ctx* ctx, int32_t every) {
+    if (!ctx || every < 1) return CIPC_E_INVALID;
+    ctx->c.lpEvery = every;
+    return CIPC_OK;
+}
 static int precond_common(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa) {
     Ctx& c = ctx->c;
     if (!c.lp || !c.lp->weightsReady) {

@@ -138,6 +138,11 @@ struct Ctx {
     int lastPrecondFallback = 0;
     long precondSolves = 0, precondFallbacks = 0;   // learned solves and how many fell back
     int lastLearnedRc = 0, lastLearnedIters = 0;     // outcome of the last learned attempt
+    // the network runs at the first Newton solve of a step and then every lpEvery solves
+    // (default: once per step); in between, the last factor is reused with the diagonal
+    // scaling of the new Hessian
+    int lpEvery = 1 << 30, lpSinceForward = 0;
+    bool lpStepStart = true;
 
     // host pinned scalars
     double* hScal = nullptr;           // 64 doubles

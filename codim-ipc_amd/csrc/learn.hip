@@ -770,6 +770,7 @@ void lp_build_graph(Ctx& c) {
     lp.Gblk.ensure_exact(9 * (size_t)nnzb);
     lp.wpartial.ensure_exact((size_t)256 * (kLpMaxK * kLpHidden + kLpHidden));
     lp.graphReady = true;
+    lp.forwardReady = false;
 }
 
 void lp_init_weights(Ctx& c, uint64_t seed) {
@@ -823,6 +824,8 @@ void set_weights(Ctx& c, const float* w, int n) {
     CIPC_HIP(hipStreamSynchronize(c.s));
     lp.adamT = 0;
     lp.weightsReady = true;
+    lp.forwardReady = false;
+    lp.forwardReady = false;
 }
 void get_weights(Ctx& c, float* w, int n) {
     LearnedPrecond& lp = LP(c);
@@ -934,6 +937,26 @@ void lp_forward(Ctx& c) {
                        (const int*)lp.posLE.p, (const int*)lp.posLD.p, (const int*)lp.posUE.p, (const int*)lp.posUD.p, lp.FL.p,
                        lp.FU.p);
     CIPC_LAUNCH_CHECK();
+    lp.forwardReady = true;
+}
+
+bool lp_refresh(Ctx& c) {
+    LearnedPrecond& lp = LP(c);
+    if (!lp.forwardReady || !lp.graphReady || lp.nV != c.nV || lp.nnzb != c.Ael.nnzb) return false;
+    BSR& E = c.Ael;
+    BSR& K = c.Acon;
+    if (K.nnzb == 0) K.rowptr.ensure(c.nV + 1);
+    const int nL = lp.nL, nV = lp.nV;
+    hipLaunchKernelGGL(k_lp_node_features, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, (const int*)lp.diagBlk.p, (const double*)E.val.p,
+                       E.nnzb, (const int*)K.rowptr.p, (const int*)K.col.p, (const double*)K.val.p, K.nnzb, (const double*)c.mass.p,
+                       (const uint8_t*)c.dbc.p, lp.S.p, lp.nodeF.p);
+    hipLaunchKernelGGL(k_lp_assemble_G, dim3(nblocks(nL + nV)), dim3(kBlock), 0, c.s, nL, nV, lp.nnzb, (const int*)lp.lEdge.p,
+                       (const int*)lp.eBlock.p, (const int*)lp.blkT.p, (const int*)lp.diagBlk.p, (const int*)c.Ael.col.p,
+                       (const double*)lp.S.p, (const float*)lp.Gl.p, (const float*)lp.raw.p, lp.Gblk.p, lp.nLow,
+                       (const int*)lp.posLE.p, (const int*)lp.posLD.p, (const int*)lp.posUE.p, (const int*)lp.posUD.p, lp.FL.p,
+                       lp.FU.p);
+    CIPC_LAUNCH_CHECK();
+    return true;
 }
 
 void lp_prepare(Ctx& c) {
@@ -944,8 +967,18 @@ void lp_prepare(Ctx& c) {
     lp_forward(c);
 }
 
-constexpr int kLpSpmvLanes = 8, kLpSpmvBlock = 256, kLpSpmvGrid = 1024;
-int lp_apply_partials(int) { return kLpSpmvGrid; }
+constexpr int kLpSpmvBlock = 256;
+// workgroups of the factor products (a multiple of 8: rows are split by XCD);
+// CIPC_LP_GRID overrides the default
+static int lp_grid() {
+    static const int g = [] {
+        const char* e = std::getenv("CIPC_LP_GRID");
+        const int v = e ? std::atoi(e) : 0;
+        return (v >= 8 && v % 8 == 0 && v <= 65536) ? v : 2048;
+    }();
+    return g;
+}
+int lp_apply_partials(int) { return lp_grid(); }
 
 template <int LANES, class V>
 static void lp_spmv_go(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part) {
@@ -953,7 +986,7 @@ static void lp_spmv_go(Ctx& c, const int* rp, const int* col, const uint32_t* va
     const int nV = c.lp->nV;
     int rpx = (nV + 7) / 8;
     rpx = (rpx + RPG - 1) / RPG * RPG;
-    hipLaunchKernelGGL((k_lp_spmv_bf16<LANES, kLpSpmvBlock, V>), dim3(kLpSpmvGrid), dim3(kLpSpmvBlock), 0, c.s, nV, rpx, rp, col, val, nnz,
+    hipLaunchKernelGGL((k_lp_spmv_bf16<LANES, kLpSpmvBlock, V>), dim3(lp_grid()), dim3(kLpSpmvBlock), 0, c.s, nV, rpx, rp, col, val, nnz,
                        x, y, r, part);
 }
 template <class V>
@@ -973,7 +1006,7 @@ int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part) {
     lp_spmv<V>(c, (const int*)lp.gLrp.p, (const int*)lp.gLcol.p, (const uint32_t*)lp.FL.p, lp.nLow, r, tmp, (const V*)nullptr, nullptr);
     lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, (const V*)tmp, z, r, part);
     CIPC_LAUNCH_CHECK();
-    return kLpSpmvGrid;
+    return lp_grid();
 }
 template int lp_apply<double>(Ctx&, const double*, double*, double*, double*);
 template int lp_apply<float>(Ctx&, const float*, float*, float*, double*);
@@ -1110,6 +1143,7 @@ double lp_train_step(Ctx& c, int nProbes, uint64_t seed, double lr) {
                        c2, 1e-8f);
     hipLaunchKernelGGL(k_lp_quantize, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const float*)lp.W.p, lp.Wq.p);
     CIPC_LAUNCH_CHECK();
+    lp.forwardReady = false;
     return loss;
 }
 

@@ -38,7 +38,7 @@ struct LpLayout {
 
 struct LearnedPrecond {
     LpLayout L;
-    bool graphReady = false, weightsReady = false;
+    bool graphReady = false, weightsReady = false, forwardReady = false;
     bool mfma = true;                 // inference forward: message layers on the matrix cores
     int lanes = 4;                    // lanes per block row in the factor products
     int nV = 0, nnzb = 0, nE = 0, nL = 0;
@@ -82,6 +82,9 @@ double lp_train_step(Ctx& c, int nProbes, uint64_t seed, double lr);
 double lp_loss_grad(Ctx& c, int nProbes, uint64_t seed, float* gradOut);
 // features of the current Hessian + forward pass (fills Gblk)
 void lp_prepare(Ctx& c);
+// reuse the last forward pass's factor with the diagonal scaling S of the current
+// Hessian (F = G S); false when there is no forward pass to reuse for this pattern
+bool lp_refresh(Ctx& c);
 // z = S G^T G S r (V = double or float vectors); tmp holds 3 nV values; when part !=
 // nullptr the per-workgroup partial sums of r.z are written there (returns their count)
 template <class V>

@@ -22,7 +22,7 @@
 // Each kernel starts by summing the previous kernel's per-workgroup partials itself
 // (every workgroup in the same fixed order, so all hold the identical scalar); kernel
 // boundaries on the one stream order those writes before the reads, so no fence or
-// ticket counter is needed.  The SpMV runs a fixed grid (kSpmvGrid workgroups) so the
+// ticket counter is needed.  The SpMV runs a fixed grid (spmv_grid() workgroups) so the
 // update kernel has few partials to sum.
 //
 // Matrix blocks are stored structure-of-arrays (entry k of all blocks contiguous);
@@ -35,6 +35,7 @@
 // chunks; kernels launched after convergence return at once.
 #include "ctx.hpp"
 #include "learn.hpp"
+#include <cstdlib>
 #include <hip/hip_ext.h>
 #include <cmath>
 
@@ -46,7 +47,16 @@ enum { F_DONE = 0, F_ITERS, F_FAIL };
 
 constexpr int kSpmvLanes = 16;
 constexpr int kSpmvBlock = 256;
-constexpr int kSpmvGrid = 1024;
+constexpr int kSpmvGridDefault = 1024;
+// workgroups of the PCG SpMV (a multiple of 8); CIPC_SPMV_GRID overrides the default
+static int spmv_grid() {
+    static const int g = [] {
+        const char* e = std::getenv("CIPC_SPMV_GRID");
+        const int v = e ? std::atoi(e) : 0;
+        return (v >= 8 && v % 8 == 0 && v <= 65536) ? v : kSpmvGridDefault;
+    }();
+    return g;
+}
 constexpr int kUpdBlock = 256;
 constexpr int kXcd = 8;
 
@@ -444,7 +454,7 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     double* sc = c.scal.p + 32;               // solver scalars
     int* flags = (int*)(c.counters.p + 32);   // solver flags
     constexpr int RPG = kSpmvBlock / kSpmvLanes;
-    const int rpxS = rows_per_xcd(nV, RPG), nS = kSpmvGrid;
+    const int rpxS = rows_per_xcd(nV, RPG), nS = spmv_grid();
     const int rpxU = rows_per_xcd(nV, kUpdBlock), nU = kXcd * (rpxU / kUpdBlock);
     const int nP = lp_apply_partials(nV);
     double* partS = c.dscr.ensure((size_t)nS + 2l * nU + 2l * nP);
@@ -655,7 +665,12 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     c.lastPrecondFallback = 0;
     const bool learned = c.precond == 1;
     if (!learned) return solve_form(c, b, xsol, tol, maxIter, iters, relres, false);
-    lp_prepare(c);
+    if (c.lpStepStart || c.lpSinceForward >= c.lpEvery || !lp_refresh(c)) {
+        lp_prepare(c);
+        c.lpSinceForward = 0;
+        c.lpStepStart = false;
+    }
+    c.lpSinceForward++;
     c.precondSolves++;
     int it = 0;
     double rr = 0;

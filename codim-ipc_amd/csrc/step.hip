@@ -340,6 +340,7 @@ int advance_one_step(Ctx& c) {
     const double h = P.dt;
     cipc_step_stats& st = c.stats;
     st = cipc_step_stats{};
+    c.lpStepStart = true;
     double t0 = now_ms(), ta;
     hipStream_t s = c.s;
     DevBuf<int>& dbcIdxD = c.i32c;

@@ -158,6 +158,10 @@ int cipc_precond_init(cipc_ctx* ctx, uint64_t seed);
 int cipc_precond_set_weights(cipc_ctx* ctx, const float* w, int32_t n);
 int cipc_precond_get_weights(cipc_ctx* ctx, float* w, int32_t n);
 int cipc_set_preconditioner(cipc_ctx* ctx, int32_t mode, int32_t budget);
+/* run the network at the first Newton solve of each step and then every `every` solves
+ * (>= 1; default: once per step); the solves in between reuse its factor G with the
+ * diagonal scaling S of their own Hessian */
+int cipc_set_precond_reuse(cipc_ctx* ctx, int32_t every);
 /* assemble H(x, constraint set) as cipc_linear_solve does, then: loss and the flat
  * parameter gradient (grad may be NULL) / one Adam step at learning rate lr */
 int cipc_precond_loss_grad(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t n_probes, uint64_t seed,
