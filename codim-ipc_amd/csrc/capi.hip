@@ -349,6 +349,7 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
         int it = 0;
         double rr = 0;
         c.lpStepStart = true;   // a standalone solve evaluates the learned network afresh
+        c.lpOffThisStep = false;
         int rc = pcg_solve(c, ctx->qb.p, sd, tol > 0 ? tol : c.P.pcgTol, c.P.pcgMaxIter, &it, &rr);
         CIPC_HIP(hipMemcpyAsync(sol, sd, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
         CIPC_HIP(hipStreamSynchronize(c.s));

@@ -868,14 +868,18 @@ void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kapp
     CIPC_LAUNCH_CHECK();
 }
 
-__global__ void k_hess_keys(const int* __restrict__ C, int nC, int nV, uint64_t* __restrict__ key) {
+// Blocks in a Dirichlet row or column are zero after Project_DBC (Math/CSR_MATRIX.h:
+// 130-141) and are not stored: a Dirichlet vertex of a static obstacle (a ground corner)
+// touches every stencil near it, and its row would otherwise hold a block per contact.
+__global__ void k_hess_keys(const int* __restrict__ C, int nC, int nV, const uint8_t* __restrict__ dbc, uint64_t* __restrict__ key) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= 16l * nC) return;
     int i = (int)(q / 16), a = (int)((q / 4) % 4), b = (int)(q % 4);
     int cc[4], vid[4], kind, nv;
     double mult;
     decode(C, nC, i, cc, kind, vid, nv, mult);
-    key[q] = (a < nv && b < nv) ? (uint64_t)vid[a] * (uint64_t)nV + (uint64_t)vid[b] : (uint64_t)nV * (uint64_t)nV;
+    const bool keep = a < nv && b < nv && !dbc[vid[a]] && !dbc[vid[b]];
+    key[q] = keep ? (uint64_t)vid[a] * (uint64_t)nV + (uint64_t)vid[b] : (uint64_t)nV * (uint64_t)nV;
 }
 __global__ void k_bsr_from_sorted(const uint64_t* __restrict__ key, long n, int nV, uint32_t* __restrict__ head) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -930,7 +934,7 @@ void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa
     CIPC_LAUNCH_CHECK();
     uint64_t* key = c.k64b.ensure(ns);
     uint32_t* slot = c.u32d.ensure(ns);
-    hipLaunchKernelGGL(k_hess_keys, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, C, nC, c.nV, key);
+    hipLaunchKernelGGL(k_hess_keys, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, C, nC, c.nV, c.dbc.p, key);
     CIPC_LAUNCH_CHECK();
     fill_iota(slot, ns, c.s);
     radix_sort_pairs(key, slot, ns, bits_for((uint64_t)c.nV * c.nV + 1), c.pw, c.s);

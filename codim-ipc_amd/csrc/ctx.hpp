@@ -143,6 +143,10 @@ struct Ctx {
     // scaling of the new Hessian
     int lpEvery = 1 << 30, lpSinceForward = 0;
     bool lpStepStart = true;
+    // after a fallback the rest of the step's solves use block Jacobi; a learned attempt
+    // is also cut off at 1.25x the iterations of the last block-Jacobi solve
+    bool lpOffThisStep = false;
+    int lastJacobiIters = 0;
 
     // host pinned scalars
     double* hScal = nullptr;           // 64 doubles

@@ -663,8 +663,14 @@ static int solve_form(Ctx& c, const double* b, double* xsol, double tol, int max
 // within it (or meets a non-positive curvature) the solve restarts with block Jacobi.
 int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
     c.lastPrecondFallback = 0;
-    const bool learned = c.precond == 1;
-    if (!learned) return solve_form(c, b, xsol, tol, maxIter, iters, relres, false);
+    const bool learned = c.precond == 1 && !c.lpOffThisStep;
+    if (!learned) {
+        int it = 0;
+        const int rc = solve_form(c, b, xsol, tol, maxIter, &it, relres, false);
+        c.lastJacobiIters = it;
+        if (iters) *iters = it;
+        return rc;
+    }
     if (c.lpStepStart || c.lpSinceForward >= c.lpEvery || !lp_refresh(c)) {
         lp_prepare(c);
         c.lpSinceForward = 0;
@@ -674,7 +680,9 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     c.precondSolves++;
     int it = 0;
     double rr = 0;
-    int rc = solve_form(c, b, xsol, tol, std::min(maxIter, c.lpBudget), &it, &rr, true);
+    int budget = std::min(maxIter, c.lpBudget);
+    if (c.lastJacobiIters > 0) budget = std::min(budget, std::max(200, c.lastJacobiIters + c.lastJacobiIters / 4));
+    int rc = solve_form(c, b, xsol, tol, budget, &it, &rr, true);
     c.lastLearnedRc = rc;
     c.lastLearnedIters = it;
     if (rc == 0) {
@@ -684,8 +692,10 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     }
     c.lastPrecondFallback = 1;
     c.precondFallbacks++;
+    c.lpOffThisStep = true;
     int it2 = 0;
     rc = solve_form(c, b, xsol, tol, maxIter, &it2, relres, false);
+    c.lastJacobiIters = it2;
     if (iters) *iters = it + it2;
     return rc;
 }

@@ -341,6 +341,7 @@ int advance_one_step(Ctx& c) {
     cipc_step_stats& st = c.stats;
     st = cipc_step_stats{};
     c.lpStepStart = true;
+    c.lpOffThisStep = false;
     double t0 = now_ms(), ta;
     hipStream_t s = c.s;
     DevBuf<int>& dbcIdxD = c.i32c;

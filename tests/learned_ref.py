@@ -88,13 +88,16 @@ def stencil(c):
     return [p, c1]
 
 
-def contact_row_counts(nV, C):
-    """blocks per row of the contact Hessian: distinct stencil partners (incl. itself)"""
+def contact_row_counts(nV, C, dbc=None):
+    """blocks per row of the contact Hessian: distinct stencil partners (incl. itself);
+    blocks in a Dirichlet row or column are not stored (they are zero after projection)"""
     rows = [set() for _ in range(nV)]
     for c in C:
         v = stencil(c)
         for a in v:
-            rows[a].update(v)
+            if dbc is not None and dbc[a]:
+                continue
+            rows[a].update(b for b in v if dbc is None or not dbc[b])
     return np.array([len(r) for r in rows], dtype=float)
 
 
