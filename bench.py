@@ -118,6 +118,52 @@ def gpu_run(args, sd, form):
                 spmv_avg_us=roof["avg_us"] if roof else None, spmv_frac=roof["frac"] if roof else None, roofline=roof)
 
 
+def batch_run(args):
+    """C5: `--batch-count` independent scenes stepped together; aggregate steps/s and
+    per-scene step latency percentiles over the timed batch steps."""
+    import numpy as np
+    import cipc
+    from cipc import precond, scenes
+    sds = scenes.scene_c5(count=args.batch_count)
+    t0 = time.perf_counter()
+    sims = []
+    for sd in sds:
+        sim = cipc.Simulator(sd, pcg_tol=args.pcg_tol, precision=PRECISION[args.batch_form])
+        if args.batch_form.endswith("_learned"):
+            sim.precond_set_weights(precond.load())
+            sim.set_preconditioner("learned", budget=args.learned_budget)
+        sims.append(sim)
+    setup_s = time.perf_counter() - t0
+    b = cipc.BatchStepper(sims, threads=args.batch_threads)
+    for k in range(args.warmup):
+        b.step()
+        print("[batch] warm-up step %d/%d" % (k + 1, args.warmup), file=sys.stderr, flush=True)
+    lat = []
+    newton = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st, ms = b.step()
+        lat.append(ms)
+        newton += sum(s["newton_iters"] for s in st)
+        print("[batch] step %d/%d" % (k + 1, args.steps), file=sys.stderr, flush=True)
+    el = time.perf_counter() - t0
+    lat = np.concatenate(lat)
+    n = len(sims)
+    tris = [sd.nT for sd in sds]
+    return {
+        "metric": "aggregate sim steps/s, %d-scene batch (per-scene p50/p99 step latency)" % n,
+        "value": round(n * args.steps / el, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el / args.steps, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32 + f64 refinement", "fp32_learned": "f32 + f64 refinement, bf16 network"}[args.batch_form],
+        "data": "synthetic",
+        "config": {"workload": "c5_batch", "scenes": n, "triangles_min": int(min(tris)), "triangles_max": int(max(tris)),
+                   "triangles_total": int(sum(tris)), "threads": args.batch_threads, "form": args.batch_form},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
+                       "max": round(float(lat.max()), 2)},
+        "newton_iters_per_scene_step": round(newton / (n * args.steps), 2), "setup_s": round(setup_s, 1),
+    }
+
+
 def cpu_baseline(args, sd):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -143,7 +189,10 @@ def main():
     # other (the first 26 steps are the free fall, run untimed)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=26)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c5"])
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-threads", type=int, default=16)
+    ap.add_argument("--batch-form", default="fp32_learned", choices=["fp64", "fp32", "fp32_learned"])
     ap.add_argument("--precision", default="all", choices=["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned", "all"])
     ap.add_argument("--learned-budget", type=int, default=2500)
     ap.add_argument("--learned-reuse", type=int, default=None,
@@ -157,6 +206,9 @@ def main():
         print("bench.py: this build runs one scene on one GPU; --gpus %d is not supported (use --gpus 1)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c5":
+        print(json.dumps(batch_run(args)))
+        return
     from cipc import scenes
     sd = scenes.scene_c2() if args.workload == "c2" else scenes.scene_c1()
 

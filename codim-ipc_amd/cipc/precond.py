@@ -24,7 +24,21 @@ from .sim import Simulator
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "precond.npz")
 
 
-def training_scenes(seed=0, n_coarse=6, n_fine=3):
+def _crop(sd, rng):
+    """a 0.3 m crop of a 61x61 sheet stack: the 320k stack's 5 mm spacing"""
+    X = sd.X.copy()
+    X[:sd.n_cloth, 0] *= 0.3
+    X[:sd.n_cloth, 2] *= 0.3
+    sd2 = scenes.SceneData(X, sd.tris, sd.dbc_idx, sd.dbc_target, sd.params, sd.name)
+    sd2.params["E"] = float(10 ** rng.uniform(5, 7))
+    sd2.params["PNTol"] = 1e-2 * sd2.bbox_diag()
+    sd2.n_cloth = sd.n_cloth
+    return sd2
+
+
+def training_scenes(seed=0, n_coarse=6, n_fine=3, n_stack=0, stack_steps=24):
+    """n_coarse 41x41 sheets, n_fine two-sheet 61x61 crops, n_stack four-sheet 61x61 crops
+    stepped `stack_steps` steps (deep self-contact, as in the 320k stack once it drapes)"""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n_coarse):
@@ -33,15 +47,12 @@ def training_scenes(seed=0, n_coarse=6, n_fine=3):
         out.append(sd)
     for k in range(n_fine):
         sd = scenes.cloth_on_sphere(sheets=2, n=61, y0=float(rng.uniform(0.203, 0.23)), dy=0.005, seed=2000 + k, name="train_c61_%d" % k)
-        # a 0.3 m crop at the 320k stack's 5 mm spacing
-        X = sd.X.copy()
-        X[:sd.n_cloth, 0] *= 0.3
-        X[:sd.n_cloth, 2] *= 0.3
-        sd2 = scenes.SceneData(X, sd.tris, sd.dbc_idx, sd.dbc_target, sd.params, sd.name)
-        sd2.params["E"] = float(10 ** rng.uniform(5, 7))
-        sd2.params["PNTol"] = 1e-2 * sd2.bbox_diag()
-        sd2.n_cloth = sd.n_cloth
-        out.append(sd2)
+        out.append(_crop(sd, rng))
+    for k in range(n_stack):
+        sd = scenes.cloth_on_sphere(sheets=4, n=61, y0=float(rng.uniform(0.201, 0.21)), dy=0.005, seed=3000 + k, name="train_s61_%d" % k)
+        sd = _crop(sd, rng)
+        sd.train_steps = stack_steps
+        out.append(sd)
     return out
 
 
@@ -57,14 +68,16 @@ def collect(sd, steps, log):
     return sim, systems
 
 
-def train(steps=1500, lr=2e-3, lr_final=2e-4, n_probes=4, seed=0, sim_steps=12, log=print):
+def train(steps=1500, lr=2e-3, lr_final=2e-4, n_probes=4, seed=0, sim_steps=12, log=print, n_stack=0, init=None):
     t0 = time.time()
     data = []
-    for sd in training_scenes(seed):
-        sim, systems = collect(sd, sim_steps, log)
+    for sd in training_scenes(seed, n_stack=n_stack):
+        sim, systems = collect(sd, getattr(sd, "train_steps", sim_steps), log)
         data.append((sim, systems))
     log("collected %d systems from %d scenes in %.1f s" % (sum(len(s) for _, s in data), len(data), time.time() - t0))
     data[0][0].precond_init(seed + 1)
+    if init is not None:
+        data[0][0].precond_set_weights(init)
     state = data[0][0].precond_get_state()
     active = None
     losses = []
@@ -105,11 +118,14 @@ def main():
     ap.add_argument("--probes", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--sim-steps", type=int, default=12)
+    ap.add_argument("--stack", type=int, default=0, help="four-sheet crops stepped into deep contact")
+    ap.add_argument("--init", default=None, help="start from these weights (npz)")
     ap.add_argument("--out", default=WEIGHTS)
     args = ap.parse_args()
     w, losses, _ = train(args.steps, args.lr, n_probes=args.probes, seed=args.seed, sim_steps=args.sim_steps,
-                         log=lambda m: print(m, flush=True))
-    save(args.out, w, losses, dict(steps=args.steps, lr=args.lr, probes=args.probes, seed=args.seed, sim_steps=args.sim_steps))
+                         log=lambda m: print(m, flush=True), n_stack=args.stack, init=load(args.init) if args.init else None)
+    save(args.out, w, losses, dict(steps=args.steps, lr=args.lr, probes=args.probes, seed=args.seed, sim_steps=args.sim_steps,
+                                   stack=args.stack, init=args.init))
     print("saved", args.out, flush=True)
 
 

@@ -328,3 +328,20 @@ def min_codim_separation(sd):
                 dd = np.linalg.norm(sd.X[nodes[pairs[other, 0]]] - sd.X[nodes[pairs[other, 1]]], axis=1)
                 best = min(best, float(dd.min()))
     return best
+
+
+def scene_c5(seed=5, count=256, tri_min=1e3, tri_max=1e5):
+    """`count` independent scenes: an N x N sheet (1 m square) whose triangle count 2 (N-1)^2
+    is drawn log-uniform in [tri_min, tri_max], its own icosphere (radius uniform in
+    [0.1, 0.3] m, level 3) and ground, dropped from a height uniform in [0.2, 0.8] m
+    above the sphere top.  Returns a list of SceneData."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        tris = 10 ** rng.uniform(np.log10(tri_min), np.log10(tri_max))
+        n = max(3, int(round(math.sqrt(tris / 2.0))) + 1)
+        r = float(rng.uniform(0.1, 0.3))
+        y0 = r + float(rng.uniform(0.2, 0.8))
+        sd = cloth_on_sphere(sheets=1, n=n, y0=y0, seed=10000 + k, sphere_level=3, sphere_radius=r, name="c5_%03d" % k)
+        out.append(sd)
+    return out

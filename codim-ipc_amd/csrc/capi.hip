@@ -2,8 +2,13 @@
 // arguments into negative status codes and records a message for cipc_last_error().
 #include "ctx.hpp"
 #include "learn.hpp"
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
 #include <exception>
+#include <thread>
+#include <vector>
 
 using namespace cipc;
 
@@ -190,6 +195,40 @@ int cipc_step(cipc_ctx* ctx, cipc_step_stats* stats) {
     ctx->c.stats.status = rc;
     if (stats) *stats = ctx->c.stats;
     return rc;
+}
+
+int cipc_batch_step(cipc_ctx* const* ctxs, int32_t n, int32_t threads, cipc_step_stats* stats, double* step_ms) {
+    if (!ctxs || n < 0) return CIPC_E_INVALID;
+    for (int i = 0; i < n; ++i)
+        if (!ctxs[i]) return CIPC_E_INVALID;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j)
+            if (ctxs[i] == ctxs[j]) return CIPC_E_INVALID;   // one scene, one stepping thread
+    if (n == 0) return CIPC_OK;
+    int T = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    T = std::min(T, (int)n);
+    std::atomic<int> next(0), firstErr(0);
+    auto work = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n) return;
+            const auto t0 = std::chrono::steady_clock::now();
+            cipc_step_stats st{};
+            const int rc = cipc_step(ctxs[i], &st);
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (stats) stats[i] = st;
+            if (step_ms) step_ms[i] = ms;
+            if (rc != CIPC_OK) {
+                int z = 0;
+                firstErr.compare_exchange_strong(z, rc);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    return firstErr.load();
 }
 
 int cipc_constraint_set(cipc_ctx* ctx, const double* x, int32_t* out, int32_t cap, int32_t* n) {

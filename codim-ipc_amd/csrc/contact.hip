@@ -910,12 +910,18 @@ __global__ void k_bsr_sum(const uint64_t* __restrict__ key, long n, int nV, cons
 #pragma unroll
     for (int m = 0; m < 9; ++m) val[(long)m * nnzb + o] = z ? 0.0 : acc[m];
 }
+// rowptr[r] = first block of a row >= r (rowOf sorted): one binary search per row, so
+// long runs of empty rows (Dirichlet obstacles) cost no serial loop
 __global__ void k_rowptr(const int* __restrict__ rowOf, int nnzb, int nrow, int* __restrict__ rowptr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > nnzb) return;
-    int prev = (i == 0) ? -1 : rowOf[i - 1];
-    int cur = (i == nnzb) ? nrow : rowOf[i];
-    for (int r = prev + 1; r <= cur; ++r) rowptr[r] = i;
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nrow) return;
+    int lo = 0, hi = nnzb;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rowOf[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    rowptr[r] = lo;
 }
 
 void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa) {
@@ -954,7 +960,7 @@ void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa
     hipLaunchKernelGGL(k_bsr_sum, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, key, ns, c.nV, pos, slot, hl, ns, c.dbc.p, nnzb,
                        A.rowOf.p, A.col.p, A.val.p);
     CIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_rowptr, dim3(nblocks(nnzb + 1)), dim3(kBlock), 0, c.s, A.rowOf.p, nnzb, c.nV, A.rowptr.p);
+    hipLaunchKernelGGL(k_rowptr, dim3(nblocks(c.nV + 1)), dim3(kBlock), 0, c.s, A.rowOf.p, nnzb, c.nV, A.rowptr.p);
     CIPC_LAUNCH_CHECK();
 }
 

@@ -103,6 +103,13 @@ int cipc_set_kappa(cipc_ctx* ctx, double kappa);
 /* Advance_One_Step_IE_Discrete_Shell<double,3,false,false> (IMPLICIT_EULER.h:151-902):
  * one implicit-Euler step; mutates positions, velocities and kappa. */
 int cipc_step(cipc_ctx* ctx, cipc_step_stats* stats);
+/* Many independent scenes stepped together (no counterpart in the reference, which
+ * steps one scene per process): one cipc_step per context, run by `threads` host
+ * threads (<= 0: one per hardware thread) taking contexts from a shared queue; each
+ * context keeps its own HIP stream, so the scenes' kernels overlap on the GPU.  stats
+ * and step_ms (may be NULL) receive each scene's statistics and step wall time; returns
+ * the first error (every scene is still attempted).  Contexts must be distinct. */
+int cipc_batch_step(cipc_ctx* const* ctxs, int32_t n, int32_t threads, cipc_step_stats* stats, double* step_ms);
 
 /* ---- components at caller positions x (parity) ---------------------------- */
 /* Compute_Constraint_Set (IPC.h:19-740): int4 encoding of the reference, order:
