@@ -74,6 +74,8 @@ def gpu_run(args, sd, form):
         from cipc import precond
         sim.precond_set_weights(precond.load())
         sim.set_preconditioner("learned", budget=args.learned_budget, reuse=args.learned_reuse)
+    if args.autotune:
+        sim.autotune()          # settles during the warm-up steps
     tw = time.perf_counter()
     wstats = []
     for k in range(args.warmup):
@@ -108,7 +110,7 @@ def gpu_run(args, sd, form):
                     bytes_per_launch=int(prof["bytes_per_launch"]), nnzb=prof["nnzb"], block_rows=prof["block_rows"])
     wn = sum(s["newton_iters"] for s in wstats[1:])
     return dict(precision=precision, steps_per_s=args.steps / elapsed,
-                warmup_steps_per_s=(args.warmup / tw) if args.warmup else None,
+                warmup_steps_per_s=(args.warmup / tw) if args.warmup else None, tuning=sim.tuning(),
                 warmup_pcg_iters_per_newton=(sum(s["pcg_iters"] for s in wstats[1:]) / wn) if wn else None, ms_per_step=1e3 * elapsed / args.steps,
                 ms_per_newton_iter=1e3 * elapsed / max(newton, 1), newton_iters_per_step=newton / args.steps,
                 pcg_iters_per_newton=pcg / max(newton, 1), contacts_last_step=stats[-1]["contacts"],
@@ -190,6 +192,8 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=26)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c5"])
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shapes instead of tuning them during the warm-up")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--batch-form", default="fp32_learned", choices=["fp64", "fp32", "fp32_learned"])

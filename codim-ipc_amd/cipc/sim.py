@@ -122,6 +122,20 @@ class Simulator:
         self._check(self.lib.cipc_get_setup(self.h, _d(mass), _i(hinge), _d(hinfo), _i(bedge)))
         return dict(mass=mass, hinge=hinge.reshape(-1, 4), hinge_info=hinfo.reshape(-1, 3), bedge=bedge.reshape(-1, 2))
 
+    def autotune(self, pcg=True, cell=True):
+        """Tune the PCG launch shapes over the next Newton solves and the broad-phase cell
+        size over the next constraint-set builds (cipc_autotune)."""
+        self._check(self.lib.cipc_autotune(self.h, (1 if pcg else 0) | (2 if cell else 0)))
+
+    def tuning(self):
+        out = np.zeros(6)
+        self._check(self.lib.cipc_get_tuning(self.h, _d(out)))
+        return dict(spmv_grid=int(out[0]), factor_grid=int(out[1]), factor_lanes=int(out[2]), cell_mult=float(out[3]),
+                    tuning_pcg=bool(out[4]), tuning_cell=bool(out[5]))
+
+    def set_tuning(self, spmv_grid=1024, factor_grid=2048, factor_lanes=4, cell_mult=1.0):
+        self._check(self.lib.cipc_set_tuning(self.h, int(spmv_grid), int(factor_grid), int(factor_lanes), float(cell_mult)))
+
     def codim_setup(self):
         """Rod segment data, rod hinges and the boundary-node list (DISCRETE_ROD.h:87-165,
         IMPLICIT_EULER.h:243-277)."""

@@ -21,6 +21,7 @@
 #include "ctx.hpp"
 #include "dev_math.hpp"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 
 namespace cipc {
@@ -421,7 +422,7 @@ __global__ void k_run_emit(const uint64_t* __restrict__ key, int n, const uint32
     C[3l * nC + o] = -(j - i);
 }
 
-void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
+static void constraint_set_impl(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
     const double dHat = std::sqrt(c.dHat2) + c.P.xi;     // IPC.h:53-54
     const double dHat2 = dHat * dHat;
     // grid over the current bounding box of all vertices
@@ -432,7 +433,7 @@ void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
         CIPC_HIP(hipStreamSynchronize(c.s));
     }
     GridParams G;
-    double cs = c.cellSize;
+    double cs = c.cellSize * c.cellMult;
     double rng[3];
     for (int d = 0; d < 3; ++d) { G.lb[d] = c.hScal[16 + d]; rng[d] = c.hScal[19 + d] - c.hScal[16 + d]; }
     for (;;) {
@@ -525,6 +526,30 @@ void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
         }
         nOut = nC;
         return;
+    }
+}
+
+// Autotuner of the grid cell size: while it runs, each call uses the next candidate
+// multiplier of the mean rest edge length and its wall time is recorded (stream drained
+// before and after; two rounds over the candidates); then the fastest multiplier is
+// kept.  The constraint set itself does not depend on the cell size.
+void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut) {
+    Ctx::Tuner& T = c.tune;
+    if (!T.cell) return constraint_set_impl(c, x, out, nOut);
+    c.cellMult = T.cellCand[T.cellIdx];
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    const auto t0 = std::chrono::steady_clock::now();
+    constraint_set_impl(c, x, out, nOut);
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    T.cellCost[T.cellIdx] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (++T.cellIdx == (int)T.cellCand.size()) {
+        // sum the rounds per multiplier
+        const int m = (int)T.cellCand.size() / 2;
+        int best = 0;
+        for (int i = 1; i < m; ++i)
+            if (T.cellCost[i] + T.cellCost[i + m] < T.cellCost[best] + T.cellCost[best + m]) best = i;
+        c.cellMult = T.cellCand[best];
+        T.cell = false;
     }
 }
 

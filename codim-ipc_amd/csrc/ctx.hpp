@@ -7,7 +7,9 @@
 // component k contiguous, ...), so stencil kernels read them coalesced.
 #pragma once
 #include "common.hpp"
+#include <array>
 #include <memory>
+#include <vector>
 #include "../../include/cipc.h"
 
 namespace cipc {
@@ -148,6 +150,17 @@ struct Ctx {
     bool lpOffThisStep = false;
     int lastJacobiIters = 0;
 
+    // launch shapes and broad-phase cell size (set by the autotuner or the caller)
+    int spmvGrid = 1024;               // workgroups of the PCG SpMV (multiple of 8)
+    int lpGrid = 2048, lpLanes = 4;    // workgroups / lanes per row of the factor products
+    double cellMult = 1.0;             // constraint-set grid cell = cellMult x mean rest edge length
+    struct Tuner {
+        bool pcg = false, cell = false;
+        int pcgIdx = 0, cellIdx = 0;
+        std::vector<std::array<int, 3>> pcgCand;   // (spmvGrid, lpGrid, lpLanes)
+        std::vector<double> pcgCost, cellCand, cellCost;
+    } tune;
+
     // host pinned scalars
     double* hScal = nullptr;           // 64 doubles
     uint32_t* hCnt = nullptr;          // 64 counters
@@ -181,6 +194,8 @@ double inertia_energy(Ctx& c, const double* x);
 void elastic_gradient(Ctx& c, const double* x, double* g);        // g = elastic + inertia (overwrites)
 void elastic_hessian(Ctx& c, const double* x);                    // fills Ael (incl. mass, DBC projection)
 int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres);
+// start the autotuner: bit 1 PCG launch shapes, bit 2 broad-phase cell size
+void autotune_start(Ctx& c, int what);
 int advance_one_step(Ctx& c);
 
 }  // namespace cipc

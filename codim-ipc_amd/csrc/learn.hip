@@ -657,14 +657,7 @@ __global__ void k_lp_sum_rawdiag(int nV, const float* __restrict__ raw, double* 
 
 // ------------------------------------------------------------------ host side
 static LearnedPrecond& LP(Ctx& c) {
-    if (!c.lp) {
-        c.lp.reset(new LearnedPrecond());
-        // lanes per block row of the factor products (2, 4, 8 or 16); CIPC_LP_LANES overrides
-        if (const char* e = std::getenv("CIPC_LP_LANES")) {
-            const int v = std::atoi(e);
-            if (v == 2 || v == 4 || v == 8 || v == 16) c.lp->lanes = v;
-        }
-    }
+    if (!c.lp) c.lp.reset(new LearnedPrecond());
     return *c.lp;
 }
 static double host_sum(Ctx& c, const double* part, int n) {
@@ -968,17 +961,7 @@ void lp_prepare(Ctx& c) {
 }
 
 constexpr int kLpSpmvBlock = 256;
-// workgroups of the factor products (a multiple of 8: rows are split by XCD);
-// CIPC_LP_GRID overrides the default
-static int lp_grid() {
-    static const int g = [] {
-        const char* e = std::getenv("CIPC_LP_GRID");
-        const int v = e ? std::atoi(e) : 0;
-        return (v >= 8 && v % 8 == 0 && v <= 65536) ? v : 2048;
-    }();
-    return g;
-}
-int lp_apply_partials(int) { return lp_grid(); }
+int lp_apply_partials(const Ctx& c) { return c.lpGrid; }
 
 template <int LANES, class V>
 static void lp_spmv_go(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part) {
@@ -986,12 +969,12 @@ static void lp_spmv_go(Ctx& c, const int* rp, const int* col, const uint32_t* va
     const int nV = c.lp->nV;
     int rpx = (nV + 7) / 8;
     rpx = (rpx + RPG - 1) / RPG * RPG;
-    hipLaunchKernelGGL((k_lp_spmv_bf16<LANES, kLpSpmvBlock, V>), dim3(lp_grid()), dim3(kLpSpmvBlock), 0, c.s, nV, rpx, rp, col, val, nnz,
+    hipLaunchKernelGGL((k_lp_spmv_bf16<LANES, kLpSpmvBlock, V>), dim3(c.lpGrid), dim3(kLpSpmvBlock), 0, c.s, nV, rpx, rp, col, val, nnz,
                        x, y, r, part);
 }
 template <class V>
 static void lp_spmv(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part) {
-    switch (c.lp->lanes) {
+    switch (c.lpLanes) {
         case 2: lp_spmv_go<2, V>(c, rp, col, val, nnz, x, y, r, part); break;
         case 4: lp_spmv_go<4, V>(c, rp, col, val, nnz, x, y, r, part); break;
         case 16: lp_spmv_go<16, V>(c, rp, col, val, nnz, x, y, r, part); break;
@@ -1006,7 +989,7 @@ int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part) {
     lp_spmv<V>(c, (const int*)lp.gLrp.p, (const int*)lp.gLcol.p, (const uint32_t*)lp.FL.p, lp.nLow, r, tmp, (const V*)nullptr, nullptr);
     lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, (const V*)tmp, z, r, part);
     CIPC_LAUNCH_CHECK();
-    return lp_grid();
+    return c.lpGrid;
 }
 template int lp_apply<double>(Ctx&, const double*, double*, double*, double*);
 template int lp_apply<float>(Ctx&, const float*, float*, float*, double*);

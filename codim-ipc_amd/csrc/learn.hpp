@@ -40,7 +40,6 @@ struct LearnedPrecond {
     LpLayout L;
     bool graphReady = false, weightsReady = false, forwardReady = false;
     bool mfma = true;                 // inference forward: message layers on the matrix cores
-    int lanes = 4;                    // lanes per block row in the factor products
     int nV = 0, nnzb = 0, nE = 0, nL = 0;
     // static graph of the elastic BSR pattern (directed edges = off-diagonal blocks)
     DevBuf<int> eBlock, eSrc, eDst, erp, eT, lEdge, blkT, diagBlk, lSrcOff, lSrcIdx, lDstOff, lDstIdx, lowSrc, lowDst;
@@ -89,7 +88,7 @@ bool lp_refresh(Ctx& c);
 // nullptr the per-workgroup partial sums of r.z are written there (returns their count)
 template <class V>
 int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part);
-int lp_apply_partials(int nV);
+int lp_apply_partials(const Ctx& c);
 // z = F^T y (the second product of lp_apply) with partial sums of r.z
 template <class V>
 void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part);

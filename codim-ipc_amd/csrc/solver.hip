@@ -22,7 +22,7 @@
 // Each kernel starts by summing the previous kernel's per-workgroup partials itself
 // (every workgroup in the same fixed order, so all hold the identical scalar); kernel
 // boundaries on the one stream order those writes before the reads, so no fence or
-// ticket counter is needed.  The SpMV runs a fixed grid (spmv_grid() workgroups) so the
+// ticket counter is needed.  The SpMV runs a fixed grid (Ctx::spmvGrid workgroups) so the
 // update kernel has few partials to sum.
 //
 // Matrix blocks are stored structure-of-arrays (entry k of all blocks contiguous);
@@ -35,6 +35,7 @@
 // chunks; kernels launched after convergence return at once.
 #include "ctx.hpp"
 #include "learn.hpp"
+#include <chrono>
 #include <cstdlib>
 #include <hip/hip_ext.h>
 #include <cmath>
@@ -47,16 +48,6 @@ enum { F_DONE = 0, F_ITERS, F_FAIL };
 
 constexpr int kSpmvLanes = 16;
 constexpr int kSpmvBlock = 256;
-constexpr int kSpmvGridDefault = 1024;
-// workgroups of the PCG SpMV (a multiple of 8); CIPC_SPMV_GRID overrides the default
-static int spmv_grid() {
-    static const int g = [] {
-        const char* e = std::getenv("CIPC_SPMV_GRID");
-        const int v = e ? std::atoi(e) : 0;
-        return (v >= 8 && v % 8 == 0 && v <= 65536) ? v : kSpmvGridDefault;
-    }();
-    return g;
-}
 constexpr int kUpdBlock = 256;
 constexpr int kXcd = 8;
 
@@ -454,9 +445,9 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     double* sc = c.scal.p + 32;               // solver scalars
     int* flags = (int*)(c.counters.p + 32);   // solver flags
     constexpr int RPG = kSpmvBlock / kSpmvLanes;
-    const int rpxS = rows_per_xcd(nV, RPG), nS = spmv_grid();
+    const int rpxS = rows_per_xcd(nV, RPG), nS = c.spmvGrid;
     const int rpxU = rows_per_xcd(nV, kUpdBlock), nU = kXcd * (rpxU / kUpdBlock);
-    const int nP = lp_apply_partials(nV);
+    const int nP = lp_apply_partials(c);
     double* partS = c.dscr.ensure((size_t)nS + 2l * nU + 2l * nP);
     double* partU = partS + nS;
     double* partP = partU + 2l * nU;
@@ -506,11 +497,11 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
             } else {
                 LearnedPrecond& lp = *c.lp;
                 V* rNew = rb[(it + 1) & 1];
-                auto fused = lp.lanes == 2    ? k_pcg_update_lower<2, kSpmvBlock, V>
-                             : lp.lanes == 4  ? k_pcg_update_lower<4, kSpmvBlock, V>
-                             : lp.lanes == 16 ? k_pcg_update_lower<16, kSpmvBlock, V>
-                                              : k_pcg_update_lower<8, kSpmvBlock, V>;
-                const int rpxL = rows_per_xcd(nV, kSpmvBlock / lp.lanes);
+                auto fused = c.lpLanes == 2    ? k_pcg_update_lower<2, kSpmvBlock, V>
+                             : c.lpLanes == 4  ? k_pcg_update_lower<4, kSpmvBlock, V>
+                             : c.lpLanes == 16 ? k_pcg_update_lower<16, kSpmvBlock, V>
+                                               : k_pcg_update_lower<8, kSpmvBlock, V>;
+                const int rpxL = rows_per_xcd(nV, kSpmvBlock / c.lpLanes);
                 hipLaunchKernelGGL(fused, dim3(nP), dim3(kSpmvBlock), 0, c.s, it, nV, rpxL, (const int*)lp.gLrp.p, (const int*)lp.gLcol.p,
                                    (const uint32_t*)lp.FL.p, lp.nLow, xsol, (const V*)pnew, (const V*)q, (const V*)rb[it & 1], rNew, ltmp,
                                    (const double*)partS, nS, partF, sc, flags);
@@ -661,7 +652,7 @@ static int solve_form(Ctx& c, const double* b, double* xsol, double tol, int max
 
 // The learned preconditioner runs with an iteration budget; when it does not converge
 // within it (or meets a non-positive curvature) the solve restarts with block Jacobi.
-int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
+static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
     c.lastPrecondFallback = 0;
     const bool learned = c.precond == 1 && !c.lpOffThisStep;
     if (!learned) {
@@ -698,6 +689,56 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     c.lastJacobiIters = it2;
     if (iters) *iters = it + it2;
     return rc;
+}
+
+// Autotuner of the PCG launch shapes: while it runs, each Newton solve uses the next
+// candidate (SpMV grid, factor-product grid, lanes per factor row) and its wall time per
+// PCG iteration is recorded (stream drained before and after); once every candidate has
+// run, the fastest is kept.
+int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
+    Ctx::Tuner& T = c.tune;
+    if (!T.pcg) return pcg_solve_impl(c, b, xsol, tol, maxIter, iters, relres);
+    const auto& k = T.pcgCand[T.pcgIdx];
+    c.spmvGrid = k[0]; c.lpGrid = k[1]; c.lpLanes = k[2];
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    const auto t0 = std::chrono::steady_clock::now();
+    int it = 0;
+    const int rc = pcg_solve_impl(c, b, xsol, tol, maxIter, &it, relres);
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (iters) *iters = it;
+    T.pcgCost[T.pcgIdx] = it > 0 ? ms / it : 1e300;
+    if (++T.pcgIdx == (int)T.pcgCand.size()) {
+        int best = 0;
+        for (int i = 1; i < (int)T.pcgCand.size(); ++i)
+            if (T.pcgCost[i] < T.pcgCost[best]) best = i;
+        c.spmvGrid = T.pcgCand[best][0]; c.lpGrid = T.pcgCand[best][1]; c.lpLanes = T.pcgCand[best][2];
+        T.pcg = false;
+    }
+    return rc;
+}
+
+void autotune_start(Ctx& c, int what) {
+    Ctx::Tuner& T = c.tune;
+    if (what & 1) {
+        T.pcgCand.clear();
+        for (int sg : {1024, 2048})
+            if (c.precond == 1) {
+                for (int lg : {1024, 2048})
+                    for (int ln : {2, 4, 8}) T.pcgCand.push_back({sg, lg, ln});
+            } else {
+                T.pcgCand.push_back({sg, c.lpGrid, c.lpLanes});
+            }
+        T.pcgCost.assign(T.pcgCand.size(), 0.0);
+        T.pcgIdx = 0;
+        T.pcg = true;
+    }
+    if (what & 2) {
+        T.cellCand = {0.5, 0.7, 1.0, 1.4, 2.0, 0.5, 0.7, 1.0, 1.4, 2.0};
+        T.cellCost.assign(T.cellCand.size(), 0.0);
+        T.cellIdx = 0;
+        T.cell = true;
+    }
 }
 
 }  // namespace cipc
