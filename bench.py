@@ -39,11 +39,18 @@ sys.path.insert(0, os.path.join(ROOT, "codim-ipc_amd"))
 
 METRIC = "sim steps/s + ms/Newton iter, 320k-tri cloth; plain FP64 vs mixed+learned PCG"
 HBM_PEAK_GBS = 8000.0
-KERNEL = {"fp64": "k_pcg_spmv<double,double,16,256> (PCG SpMV, FP64 blocks, FP64 vectors)",
-          "mixed": "k_pcg_spmv<float,double,16,256> (PCG SpMV, FP32 blocks, FP64 vectors)",
-          "fp32": "k_pcg_spmv<float,float,16,256> (PCG SpMV, FP32 blocks, FP32 vectors)",
-          "mixed_learned": "k_pcg_spmv<float,double,16,256> (PCG SpMV, FP32 blocks, FP64 vectors)",
-          "fp32_learned": "k_pcg_spmv<float,float,16,256> (PCG SpMV, FP32 blocks, FP32 vectors)"}
+KERNEL = {"fp64": ("k_pcg_spmv<double,double,%d,256>", "PCG SpMV, FP64 blocks, FP64 vectors"),
+          "mixed": ("k_pcg_spmv<float,double,%d,256>", "PCG SpMV, FP32 blocks, FP64 vectors"),
+          "fp32": ("k_pcg_spmv<float,float,%d,256>", "PCG SpMV, FP32 blocks, FP32 vectors"),
+          "mixed_learned": ("k_pcg_spmv<float,double,%d,256>", "PCG SpMV, FP32 blocks, FP64 vectors"),
+          "fp32_learned": ("k_pcg_spmv<float,float,%d,256>", "PCG SpMV, FP32 blocks, FP32 vectors")}
+
+
+def kernel_name(precision, lanes):
+    """the PCG SpMV instance a run launches (template: blocks, vectors, lanes per block row,
+    workgroup size)"""
+    name, what = KERNEL[precision]
+    return "%s (%s)" % (name % lanes, what)
 MODE = {"fp64": "plain FP64, block-Jacobi PCG",
         "mixed": "FP32 Hessian blocks, FP64 vectors, FP64 iterative refinement, block-Jacobi PCG",
         "fp32": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, block-Jacobi PCG",
@@ -60,7 +67,7 @@ def measured_traffic(kernel, workload):
         return None, None
     with open(path) as f:
         t = json.load(f)
-    if not kernel.startswith(t["kernel"]) or t["workload"] != workload:
+    if not kernel.startswith(t["kernel"] + " ") or t["workload"] != workload:
         return None, None
     return int(t["traffic_bytes_per_launch"]), t["summary"]
 
@@ -103,9 +110,10 @@ def gpu_run(args, sd, form):
     if prof["launches"]:
         avg_s = prof["total_ms"] / prof["launches"] / 1e3
         ach = prof["bytes_per_launch"] / avg_s / 1e9
-        traffic, source = measured_traffic(KERNEL[precision], sd.name)
+        kname = kernel_name(precision, sim.tuning()["spmv_lanes"])
+        traffic, source = measured_traffic(kname, sd.name)
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=traffic, traffic_source=source, kernel=KERNEL[precision], launches=prof["launches"],
+                    traffic=traffic, traffic_source=source, kernel=kname, launches=prof["launches"],
                     avg_us=round(avg_s * 1e6, 2),
                     bytes_per_launch=int(prof["bytes_per_launch"]), nnzb=prof["nnzb"], block_rows=prof["block_rows"])
     wn = sum(s["newton_iters"] for s in wstats[1:])

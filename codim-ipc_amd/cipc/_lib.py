@@ -99,7 +99,7 @@ PROTOTYPES = {
     "cipc_set_precond_reuse": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_autotune": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_get_tuning": (ctypes.c_int, [P, D]),
-    "cipc_set_tuning": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
+    "cipc_set_tuning": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
     "cipc_batch_step": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, D]),
     "cipc_get_codim_setup": (ctypes.c_int, [P, D, I32, D, I32]),
     "cipc_set_state": (ctypes.c_int, [P, D, D]),

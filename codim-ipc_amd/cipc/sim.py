@@ -128,13 +128,14 @@ class Simulator:
         self._check(self.lib.cipc_autotune(self.h, (1 if pcg else 0) | (2 if cell else 0)))
 
     def tuning(self):
-        out = np.zeros(6)
+        out = np.zeros(7)
         self._check(self.lib.cipc_get_tuning(self.h, _d(out)))
-        return dict(spmv_grid=int(out[0]), factor_grid=int(out[1]), factor_lanes=int(out[2]), cell_mult=float(out[3]),
-                    tuning_pcg=bool(out[4]), tuning_cell=bool(out[5]))
+        return dict(spmv_grid=int(out[0]), spmv_lanes=int(out[6]), factor_grid=int(out[1]), factor_lanes=int(out[2]),
+                    cell_mult=float(out[3]), tuning_pcg=bool(out[4]), tuning_cell=bool(out[5]))
 
-    def set_tuning(self, spmv_grid=1024, factor_grid=2048, factor_lanes=4, cell_mult=1.0):
-        self._check(self.lib.cipc_set_tuning(self.h, int(spmv_grid), int(factor_grid), int(factor_lanes), float(cell_mult)))
+    def set_tuning(self, spmv_grid=1024, spmv_lanes=16, factor_grid=2048, factor_lanes=4, cell_mult=1.0):
+        self._check(self.lib.cipc_set_tuning(self.h, int(spmv_grid), int(spmv_lanes), int(factor_grid), int(factor_lanes),
+                                             float(cell_mult)))
 
     def codim_setup(self):
         """Rod segment data, rod hinges and the boundary-node list (DISCRETE_ROD.h:87-165,

@@ -456,17 +456,17 @@ int cipc_get_tuning(const cipc_ctx* ctx, double* out) {
     if (!ctx || !out) return CIPC_E_INVALID;
     const Ctx& c = ctx->c;
     out[0] = c.spmvGrid; out[1] = c.lpGrid; out[2] = c.lpLanes; out[3] = c.cellMult;
-    out[4] = c.tune.pcg ? 1 : 0; out[5] = c.tune.cell ? 1 : 0;
+    out[4] = c.tune.pcg ? 1 : 0; out[5] = c.tune.cell ? 1 : 0; out[6] = c.spmvLanes;
     return CIPC_OK;
 }
-int cipc_set_tuning(cipc_ctx* ctx, int32_t spmv_grid, int32_t factor_grid, int32_t factor_lanes, double cell_mult) {
-    if (!ctx) return CIPC_E_INVALID;
+int cipc_set_tuning(cipc_ctx* ctx, int32_t spmv_grid, int32_t spmv_lanes, int32_t factor_grid, int32_t factor_lanes, double cell_mult) {
+    if (!ctx || !(spmv_lanes == 8 || spmv_lanes == 16)) return CIPC_E_INVALID;
     auto okGrid = [](int g) { return g >= 8 && g % 8 == 0 && g <= 65536; };
     if (!okGrid(spmv_grid) || !okGrid(factor_grid) || !(factor_lanes == 2 || factor_lanes == 4 || factor_lanes == 8 || factor_lanes == 16) ||
         !(cell_mult >= 0.125 && cell_mult <= 16))
         return CIPC_E_INVALID;
     Ctx& c = ctx->c;
-    c.spmvGrid = spmv_grid; c.lpGrid = factor_grid; c.lpLanes = factor_lanes; c.cellMult = cell_mult;
+    c.spmvGrid = spmv_grid; c.spmvLanes = spmv_lanes; c.lpGrid = factor_grid; c.lpLanes = factor_lanes; c.cellMult = cell_mult;
     c.tune.pcg = c.tune.cell = false;
     return CIPC_OK;
 }

@@ -152,12 +152,13 @@ struct Ctx {
 
     // launch shapes and broad-phase cell size (set by the autotuner or the caller)
     int spmvGrid = 1024;               // workgroups of the PCG SpMV (multiple of 8)
+    int spmvLanes = 16;                // lanes per block row of the PCG SpMV (8 or 16)
     int lpGrid = 2048, lpLanes = 4;    // workgroups / lanes per row of the factor products
     double cellMult = 1.0;             // constraint-set grid cell = cellMult x mean rest edge length
     struct Tuner {
         bool pcg = false, cell = false;
         int pcgIdx = 0, cellIdx = 0;
-        std::vector<std::array<int, 3>> pcgCand;   // (spmvGrid, lpGrid, lpLanes)
+        std::vector<std::array<int, 4>> pcgCand;   // (spmvGrid, lpGrid, lpLanes, spmvLanes)
         std::vector<double> pcgCost, cellCand, cellCost;
     } tune;
 

@@ -46,7 +46,6 @@ namespace cipc {
 enum { S_RZ0 = 0, S_RZ1, S_BB, S_RR };
 enum { F_DONE = 0, F_ITERS, F_FAIL };
 
-constexpr int kSpmvLanes = 16;
 constexpr int kSpmvBlock = 256;
 constexpr int kUpdBlock = 256;
 constexpr int kXcd = 8;
@@ -444,7 +443,7 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     BSR& K = c.Acon;
     double* sc = c.scal.p + 32;               // solver scalars
     int* flags = (int*)(c.counters.p + 32);   // solver flags
-    constexpr int RPG = kSpmvBlock / kSpmvLanes;
+    const int RPG = kSpmvBlock / c.spmvLanes;
     const int rpxS = rows_per_xcd(nV, RPG), nS = c.spmvGrid;
     const int rpxU = rows_per_xcd(nV, kUpdBlock), nU = kXcd * (rpxU / kUpdBlock);
     const int nP = lp_apply_partials(c);
@@ -485,7 +484,8 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
         for (int k = 0; k < chunk && it <= maxIter; ++k, ++it, ++launched) {
             const V* pold = pbuf + (size_t)((it + 1) & 1) * n;
             V* pnew = pbuf + (size_t)(it & 1) * n;
-            auto kern = it > 0 ? k_pcg_spmv<M, V, kSpmvLanes, kSpmvBlock, true> : k_pcg_spmv<M, V, kSpmvLanes, kSpmvBlock, false>;
+            auto kern = c.spmvLanes == 8 ? (it > 0 ? k_pcg_spmv<M, V, 8, kSpmvBlock, true> : k_pcg_spmv<M, V, 8, kSpmvBlock, false>)
+                                         : (it > 0 ? k_pcg_spmv<M, V, 16, kSpmvBlock, true> : k_pcg_spmv<M, V, 16, kSpmvBlock, false>);
             hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.profiling ? c.evPool[2 * k] : nullptr,
                                   c.profiling ? c.evPool[2 * k + 1] : nullptr, 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
                                   K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, partRZ, nRZ,
@@ -699,7 +699,7 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     Ctx::Tuner& T = c.tune;
     if (!T.pcg) return pcg_solve_impl(c, b, xsol, tol, maxIter, iters, relres);
     const auto& k = T.pcgCand[T.pcgIdx];
-    c.spmvGrid = k[0]; c.lpGrid = k[1]; c.lpLanes = k[2];
+    c.spmvGrid = k[0]; c.lpGrid = k[1]; c.lpLanes = k[2]; c.spmvLanes = k[3];
     CIPC_HIP(hipStreamSynchronize(c.s));
     const auto t0 = std::chrono::steady_clock::now();
     int it = 0;
@@ -713,6 +713,7 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
         for (int i = 1; i < (int)T.pcgCand.size(); ++i)
             if (T.pcgCost[i] < T.pcgCost[best]) best = i;
         c.spmvGrid = T.pcgCand[best][0]; c.lpGrid = T.pcgCand[best][1]; c.lpLanes = T.pcgCand[best][2];
+        c.spmvLanes = T.pcgCand[best][3];
         T.pcg = false;
     }
     return rc;
@@ -722,13 +723,14 @@ void autotune_start(Ctx& c, int what) {
     Ctx::Tuner& T = c.tune;
     if (what & 1) {
         T.pcgCand.clear();
-        for (int sg : {1024, 2048})
-            if (c.precond == 1) {
-                for (int lg : {1024, 2048})
-                    for (int ln : {2, 4, 8}) T.pcgCand.push_back({sg, lg, ln});
-            } else {
-                T.pcgCand.push_back({sg, c.lpGrid, c.lpLanes});
-            }
+        for (int sl : {16, 8})
+            for (int sg : {1024, 2048})
+                if (c.precond == 1) {
+                    for (int lg : {1024, 2048})
+                        for (int ln : {2, 4, 8}) T.pcgCand.push_back({sg, lg, ln, sl});
+                } else {
+                    T.pcgCand.push_back({sg, c.lpGrid, c.lpLanes, sl});
+                }
         T.pcgCost.assign(T.pcgCand.size(), 0.0);
         T.pcgIdx = 0;
         T.pcg = true;

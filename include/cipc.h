@@ -192,15 +192,16 @@ int cipc_precond_info(const cipc_ctx* ctx, int32_t* out);
 
 /* ---- launch shapes and autotuning (no counterpart in the reference) -------
  * cipc_autotune(what): bit 1 -- the next Newton solves cycle through PCG launch shapes
- * (SpMV grid 1024/2048; with the learned preconditioner also factor-product grid
- * 1024/2048 and 2/4/8 lanes per factor row), timing each per PCG iteration, and keep the
- * fastest; bit 2 -- the next ten constraint-set builds cycle twice through broad-phase
- * cell sizes of 0.5-2x the mean rest edge length and keep the fastest (the constraint
- * set does not depend on it).  out[0..5] of cipc_get_tuning = SpMV grid, factor grid,
- * factor lanes, cell multiplier, PCG tuning running, cell tuning running. */
+ * (SpMV grid 1024/2048 x 8/16 lanes per block row; with the learned preconditioner also
+ * factor-product grid 1024/2048 and 2/4/8 lanes per factor row), timing each per PCG
+ * iteration, and keep the fastest; bit 2 -- the next ten constraint-set builds cycle
+ * twice through broad-phase cell sizes of 0.5-2x the mean rest edge length and keep the
+ * fastest (the constraint set does not depend on it).  out[0..6] of cipc_get_tuning =
+ * SpMV grid, factor grid, factor lanes, cell multiplier, PCG tuning running, cell tuning
+ * running, SpMV lanes. */
 int cipc_autotune(cipc_ctx* ctx, int32_t what);
 int cipc_get_tuning(const cipc_ctx* ctx, double* out);
-int cipc_set_tuning(cipc_ctx* ctx, int32_t spmv_grid, int32_t factor_grid, int32_t factor_lanes, double cell_mult);
+int cipc_set_tuning(cipc_ctx* ctx, int32_t spmv_grid, int32_t spmv_lanes, int32_t factor_grid, int32_t factor_lanes, double cell_mult);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------ */
 /* Kernel timing of the dominant kernel of the step, the PCG SpMV (q = A p with
