@@ -39,18 +39,18 @@ sys.path.insert(0, os.path.join(ROOT, "codim-ipc_amd"))
 
 METRIC = "sim steps/s + ms/Newton iter, 320k-tri cloth; plain FP64 vs mixed+learned PCG"
 HBM_PEAK_GBS = 8000.0
-KERNEL = {"fp64": ("k_pcg_spmv<double,double,%d,256>", "PCG SpMV, FP64 blocks, FP64 vectors"),
-          "mixed": ("k_pcg_spmv<float,double,%d,256>", "PCG SpMV, FP32 blocks, FP64 vectors"),
-          "fp32": ("k_pcg_spmv<float,float,%d,256>", "PCG SpMV, FP32 blocks, FP32 vectors"),
-          "mixed_learned": ("k_pcg_spmv<float,double,%d,256>", "PCG SpMV, FP32 blocks, FP64 vectors"),
-          "fp32_learned": ("k_pcg_spmv<float,float,%d,256>", "PCG SpMV, FP32 blocks, FP32 vectors")}
+KERNEL = {"fp64": ("k_pcg_spmv<double,double,%d,256,true,%d>", "PCG SpMV, FP64 blocks, FP64 vectors"),
+          "mixed": ("k_pcg_spmv<float,double,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP64 vectors"),
+          "fp32": ("k_pcg_spmv<float,float,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP32 vectors"),
+          "mixed_learned": ("k_pcg_spmv<float,double,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP64 vectors"),
+          "fp32_learned": ("k_pcg_spmv<float,float,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP32 vectors")}
 
 
-def kernel_name(precision, lanes):
+def kernel_name(precision, tuning):
     """the PCG SpMV instance a run launches (template: blocks, vectors, lanes per block row,
-    workgroup size)"""
+    workgroup size, direction update, blocks per lane pass)"""
     name, what = KERNEL[precision]
-    return "%s (%s)" % (name % lanes, what)
+    return "%s (%s)" % (name % (tuning["spmv_lanes"], tuning["spmv_unroll"]), what)
 MODE = {"fp64": "plain FP64, block-Jacobi PCG",
         "mixed": "FP32 Hessian blocks, FP64 vectors, FP64 iterative refinement, block-Jacobi PCG",
         "fp32": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, block-Jacobi PCG",
@@ -106,14 +106,17 @@ def gpu_run(args, sd, form):
     min_d = float(np.sqrt(sim.min_dist2(x, C)[1])) if len(C) else None
     newton = sum(s["newton_iters"] for s in stats)
     pcg = sum(s["pcg_iters"] for s in stats)
+    stage_ms = {k[2:-3]: round(sum(s[k] for s in stats) / args.steps, 2)
+                for k in ("t_total_ms", "t_constraint_set_ms", "t_ccd_ms", "t_hessian_ms", "t_solve_ms", "t_energy_ms", "t_gradient_ms")}
     roof = None
     if prof["launches"]:
         avg_s = prof["total_ms"] / prof["launches"] / 1e3
         ach = prof["bytes_per_launch"] / avg_s / 1e9
-        kname = kernel_name(precision, sim.tuning()["spmv_lanes"])
+        kname = kernel_name(precision, sim.tuning())
         traffic, source = measured_traffic(kname, sd.name)
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=traffic, traffic_source=source, kernel=kname, launches=prof["launches"],
+                    traffic=traffic, traffic_source=source, kernel=kname, launches_timed=prof["launches"],
+                    pcg_iterations=pcg,
                     avg_us=round(avg_s * 1e6, 2),
                     bytes_per_launch=int(prof["bytes_per_launch"]), nnzb=prof["nnzb"], block_rows=prof["block_rows"])
     wn = sum(s["newton_iters"] for s in wstats[1:])
@@ -121,7 +124,7 @@ def gpu_run(args, sd, form):
                 warmup_steps_per_s=(args.warmup / tw) if args.warmup else None, tuning=sim.tuning(),
                 warmup_pcg_iters_per_newton=(sum(s["pcg_iters"] for s in wstats[1:]) / wn) if wn else None, ms_per_step=1e3 * elapsed / args.steps,
                 ms_per_newton_iter=1e3 * elapsed / max(newton, 1), newton_iters_per_step=newton / args.steps,
-                pcg_iters_per_newton=pcg / max(newton, 1), contacts_last_step=stats[-1]["contacts"],
+                pcg_iters_per_newton=pcg / max(newton, 1), stage_ms_per_step=stage_ms, contacts_last_step=stats[-1]["contacts"],
                 statuses=sorted(set(s["status"] for s in stats)), min_dist_final=min_d,
                 learned_solves=(info1["solves"] - info0["solves"]) if learned else None,
                 learned_fallbacks=(info1["fallbacks"] - info0["fallbacks"]) if learned else None,
@@ -208,7 +211,7 @@ def main():
     ap.add_argument("--precision", default="all", choices=["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned", "all"])
     ap.add_argument("--learned-budget", type=int, default=2500)
     ap.add_argument("--learned-reuse", type=int, default=None,
-                    help="Newton solves per network evaluation (default: once per step, at its first solve)")
+                    help="Newton solves per network evaluation (default 1: every solve)")
     ap.add_argument("--pcg-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)

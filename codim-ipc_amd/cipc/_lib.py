@@ -86,6 +86,20 @@ class StepStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("spmv_grid", ctypes.c_int32),
+        ("spmv_lanes", ctypes.c_int32),
+        ("spmv_unroll", ctypes.c_int32),
+        ("factor_grid", ctypes.c_int32),
+        ("factor_lanes", ctypes.c_int32),
+        ("tuning_pcg", ctypes.c_int32),
+        ("tuning_cell", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("cell_mult", ctypes.c_double),
+    ]
+
+
 # exported symbols and their prototypes (restype, argtypes)
 P = ctypes.c_void_p
 PROTOTYPES = {
@@ -98,8 +112,8 @@ PROTOTYPES = {
     "cipc_get_codim_sizes": (ctypes.c_int, [P, I32]),
     "cipc_set_precond_reuse": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_autotune": (ctypes.c_int, [P, ctypes.c_int32]),
-    "cipc_get_tuning": (ctypes.c_int, [P, D]),
-    "cipc_set_tuning": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
+    "cipc_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
+    "cipc_set_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
     "cipc_batch_step": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, D]),
     "cipc_get_codim_setup": (ctypes.c_int, [P, D, I32, D, I32]),
     "cipc_set_state": (ctypes.c_int, [P, D, D]),

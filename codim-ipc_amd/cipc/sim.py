@@ -128,14 +128,16 @@ class Simulator:
         self._check(self.lib.cipc_autotune(self.h, (1 if pcg else 0) | (2 if cell else 0)))
 
     def tuning(self):
-        out = np.zeros(7)
-        self._check(self.lib.cipc_get_tuning(self.h, _d(out)))
-        return dict(spmv_grid=int(out[0]), spmv_lanes=int(out[6]), factor_grid=int(out[1]), factor_lanes=int(out[2]),
-                    cell_mult=float(out[3]), tuning_pcg=bool(out[4]), tuning_cell=bool(out[5]))
+        t = _lib.Tuning()
+        self._check(self.lib.cipc_get_tuning(self.h, ctypes.byref(t)))
+        return dict(spmv_grid=t.spmv_grid, spmv_lanes=t.spmv_lanes, spmv_unroll=t.spmv_unroll, factor_grid=t.factor_grid,
+                    factor_lanes=t.factor_lanes, cell_mult=t.cell_mult, tuning_pcg=bool(t.tuning_pcg),
+                    tuning_cell=bool(t.tuning_cell))
 
-    def set_tuning(self, spmv_grid=1024, spmv_lanes=16, factor_grid=2048, factor_lanes=4, cell_mult=1.0):
-        self._check(self.lib.cipc_set_tuning(self.h, int(spmv_grid), int(spmv_lanes), int(factor_grid), int(factor_lanes),
-                                             float(cell_mult)))
+    def set_tuning(self, spmv_grid=1024, spmv_lanes=16, spmv_unroll=1, factor_grid=2048, factor_lanes=4, cell_mult=1.0):
+        t = _lib.Tuning(spmv_grid=spmv_grid, spmv_lanes=spmv_lanes, spmv_unroll=spmv_unroll, factor_grid=factor_grid,
+                        factor_lanes=factor_lanes, cell_mult=cell_mult)
+        self._check(self.lib.cipc_set_tuning(self.h, ctypes.byref(t)))
 
     def codim_setup(self):
         """Rod segment data, rod hinges and the boundary-node list (DISCRETE_ROD.h:87-165,

@@ -452,22 +452,28 @@ int cipc_autotune(cipc_ctx* ctx, int32_t what) {
     autotune_start(ctx->c, what);
     return CIPC_OK;
 }
-int cipc_get_tuning(const cipc_ctx* ctx, double* out) {
+int cipc_get_tuning(const cipc_ctx* ctx, cipc_tuning* out) {
     if (!ctx || !out) return CIPC_E_INVALID;
     const Ctx& c = ctx->c;
-    out[0] = c.spmvGrid; out[1] = c.lpGrid; out[2] = c.lpLanes; out[3] = c.cellMult;
-    out[4] = c.tune.pcg ? 1 : 0; out[5] = c.tune.cell ? 1 : 0; out[6] = c.spmvLanes;
+    *out = cipc_tuning{};
+    out->spmv_grid = c.spmvGrid; out->spmv_lanes = c.spmvLanes; out->spmv_unroll = c.spmvUnroll;
+    out->factor_grid = c.lpGrid; out->factor_lanes = c.lpLanes;
+    out->tuning_pcg = c.tune.pcg ? 1 : 0; out->tuning_cell = c.tune.cell ? 1 : 0;
+    out->cell_mult = c.cellMult;
     return CIPC_OK;
 }
-int cipc_set_tuning(cipc_ctx* ctx, int32_t spmv_grid, int32_t spmv_lanes, int32_t factor_grid, int32_t factor_lanes, double cell_mult) {
-    if (!ctx || !(spmv_lanes == 8 || spmv_lanes == 16)) return CIPC_E_INVALID;
+int cipc_set_tuning(cipc_ctx* ctx, const cipc_tuning* t) {
+    if (!ctx || !t) return CIPC_E_INVALID;
     auto okGrid = [](int g) { return g >= 8 && g % 8 == 0 && g <= 65536; };
-    if (!okGrid(spmv_grid) || !okGrid(factor_grid) || !(factor_lanes == 2 || factor_lanes == 4 || factor_lanes == 8 || factor_lanes == 16) ||
-        !(cell_mult >= 0.125 && cell_mult <= 16))
+    const int fl = t->factor_lanes, sl = t->spmv_lanes;
+    if (!okGrid(t->spmv_grid) || !okGrid(t->factor_grid) || !(sl == 4 || sl == 8 || sl == 16) ||
+        !(t->spmv_unroll == 1 || t->spmv_unroll == 2) || !(fl == 2 || fl == 4 || fl == 8 || fl == 16) ||
+        !(t->cell_mult >= 0.125 && t->cell_mult <= 16))
         return CIPC_E_INVALID;
     Ctx& c = ctx->c;
-    c.spmvGrid = spmv_grid; c.spmvLanes = spmv_lanes; c.lpGrid = factor_grid; c.lpLanes = factor_lanes; c.cellMult = cell_mult;
-    c.tune.pcg = c.tune.cell = false;
+    c.spmvGrid = t->spmv_grid; c.spmvLanes = sl; c.spmvUnroll = t->spmv_unroll;
+    c.lpGrid = t->factor_grid; c.lpLanes = fl; c.cellMult = t->cell_mult;
+    c.tune.pcg = c.tune.cell = c.tune.enabled = false;
     return CIPC_OK;
 }
 int cipc_set_precond_reuse(cipc_ctx* ctx, int32_t every) {

@@ -141,9 +141,11 @@ struct Ctx {
     long precondSolves = 0, precondFallbacks = 0;   // learned solves and how many fell back
     int lastLearnedRc = 0, lastLearnedIters = 0;     // outcome of the last learned attempt
     // the network runs at the first Newton solve of a step and then every lpEvery solves
-    // (default: once per step); in between, the last factor is reused with the diagonal
-    // scaling of the new Hessian
-    int lpEvery = 1 << 30, lpSinceForward = 0;
+    // (default: every solve -- in contact the Hessian changes enough between Newton
+    // iterations that a fresh factor saves far more PCG iterations than the forward pass
+    // costs); in between, the last factor is reused with the diagonal scaling of the new
+    // Hessian
+    int lpEvery = 1, lpSinceForward = 0;
     bool lpStepStart = true;
     // after a fallback the rest of the step's solves use block Jacobi; a learned attempt
     // is also cut off at 1.25x the iterations of the last block-Jacobi solve
@@ -152,13 +154,19 @@ struct Ctx {
 
     // launch shapes and broad-phase cell size (set by the autotuner or the caller)
     int spmvGrid = 1024;               // workgroups of the PCG SpMV (multiple of 8)
-    int spmvLanes = 16;                // lanes per block row of the PCG SpMV (8 or 16)
+    int spmvLanes = 16;                // lanes per block row of the PCG SpMV (4, 8 or 16)
+    int spmvUnroll = 1;                // blocks per lane pass of the PCG SpMV (1 or 2)
     int lpGrid = 2048, lpLanes = 4;    // workgroups / lanes per row of the factor products
     double cellMult = 1.0;             // constraint-set grid cell = cellMult x mean rest edge length
     struct Tuner {
         bool pcg = false, cell = false;
-        int pcgIdx = 0, cellIdx = 0;
-        std::vector<std::array<int, 4>> pcgCand;   // (spmvGrid, lpGrid, lpLanes, spmvLanes)
+        bool enabled = false;                      // PCG tuning requested (re-tunes allowed)
+        int retunes = 0;
+        long tunedNnzb = 0;                        // stored blocks of the matrix tuned on
+        double spmvMs = 0;                         // sampled SpMV time of the current solve
+        long spmvLaunches = 0;
+        int pcgIdx = 0, cellIdx = 0, pcgPhase = 0;
+        std::vector<std::array<int, 5>> pcgCand;   // (spmvGrid, lpGrid, lpLanes, spmvLanes, spmvUnroll)
         std::vector<double> pcgCost, cellCand, cellCost;
     } tune;
 

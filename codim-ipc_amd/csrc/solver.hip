@@ -49,6 +49,11 @@ enum { F_DONE = 0, F_ITERS, F_FAIL };
 constexpr int kSpmvBlock = 256;
 constexpr int kUpdBlock = 256;
 constexpr int kXcd = 8;
+// with profiling on, every kProfEvery-th SpMV launch is bracketed by timing events (events
+// on every launch slow the solve by about 10%), starting at launch kProfFirst of each
+// chunk: the chunk's first launch follows the host's poll of the done flag with the GPU
+// idle, which is not representative
+constexpr int kProfEvery = 8, kProfFirst = 3;
 
 // Sum over the workgroup; every thread receives the same value (butterfly shuffles
 // give bit-identical results in all lanes, waves are added in a fixed order).
@@ -74,35 +79,114 @@ inline int rows_per_xcd(int nrow, int rowsPerGroup) {
 // blocks [beg, end).  The lane that meets the diagonal block keeps z_row + beta p_row
 // in `own` (every row stores its diagonal block), so the row's new direction needs no
 // separate load.
-template <class M, class V, int LANES, bool WITHP>
+// UNR = 2: each pass takes the lane's next two blocks (b and b + LANES) and issues all of
+// their loads before any product, doubling the loads in flight per lane; a missing second
+// block re-reads the first and contributes zero.
+template <class M, class V, int LANES, bool WITHP, int UNR = 1>
 __device__ __forceinline__ void bsr_row(int beg, int end, const int* __restrict__ col, const M* __restrict__ val, long nnzb, int row,
                                         const V* __restrict__ z, const V* __restrict__ p, V beta, V* s, V* own, bool& hasOwn) {
-    for (int b = beg; b < end; b += LANES) {
-        const int c = col[b];
-        V a[9];
+    static_assert(UNR == 1 || UNR == 2, "");
+    for (int b = beg; b < end; b += UNR * LANES) {
+        int bu[UNR];
+        bool on[UNR];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) a[k] = (V)val[k * nnzb + b];
-        V x0 = z[3 * c], x1 = z[3 * c + 1], x2 = z[3 * c + 2];
-        if (WITHP) {
-            const V p0 = p[3 * c], p1 = p[3 * c + 1], p2 = p[3 * c + 2];
-            x0 += beta * p0;
-            x1 += beta * p1;
-            x2 += beta * p2;
+        for (int u = 0; u < UNR; ++u) {
+            on[u] = u == 0 || b + u * LANES < end;
+            bu[u] = on[u] ? b + u * LANES : b;
         }
-        if (c == row) {
-            own[0] = x0; own[1] = x1; own[2] = x2;
-            hasOwn = true;
+        int c[UNR];
+        V a[UNR][9], x[UNR][3];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            c[u] = col[bu[u]];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) a[u][k] = (V)val[k * nnzb + bu[u]];
         }
-        s[0] += a[0] * x0 + a[1] * x1 + a[2] * x2;
-        s[1] += a[3] * x0 + a[4] * x1 + a[5] * x2;
-        s[2] += a[6] * x0 + a[7] * x1 + a[8] * x2;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) x[u][k] = z[3 * c[u] + k];
+            if (WITHP) {
+                V pv[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pv[k] = p[3 * c[u] + k];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) x[u][k] += beta * pv[k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (on[u] && c[u] == row) {
+                own[0] = x[u][0]; own[1] = x[u][1]; own[2] = x[u][2];
+                hasOwn = true;
+            }
+            if (!on[u]) x[u][0] = x[u][1] = x[u][2] = 0;
+            s[0] += a[u][0] * x[u][0] + a[u][1] * x[u][1] + a[u][2] * x[u][2];
+            s[1] += a[u][3] * x[u][0] + a[u][4] * x[u][1] + a[u][5] * x[u][2];
+            s[2] += a[u][6] * x[u][0] + a[u][7] * x[u][1] + a[u][8] * x[u][2];
+        }
+    }
+}
+
+// One block row of E + K in a single lane loop: positions j < nE of the row walk the
+// elastic blocks [eb, ee), the rest the contact blocks [cb, ce) (each lane picks its
+// pattern's arrays per position), so the lanes of a row fill up across both patterns and
+// no second dependent row-bound load follows the elastic pass.  The diagonal block that
+// fills `own` is the elastic one (contact rows also hold diagonal blocks).
+template <class M, class V, int LANES, bool WITHP, int UNR>
+__device__ __forceinline__ void bsr_row_ek(int lane, int eb, int ee, int cb, int ce, const int* __restrict__ ecol,
+                                           const M* __restrict__ ev, long enz, const int* __restrict__ ccol, const M* __restrict__ cv,
+                                           long cnz, int row, const V* __restrict__ z, const V* __restrict__ p, V beta, V* s, V* own,
+                                           bool& hasOwn) {
+    static_assert(UNR == 1 || UNR == 2, "");
+    const int nE = ee - eb, nT = nE + (ce - cb);
+    for (int j = lane; j < nT; j += UNR * LANES) {
+        int c[UNR];
+        bool on[UNR], el[UNR];
+        V a[UNR][9], x[UNR][3];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int ju = u == 0 || j + u * LANES < nT ? j + u * LANES : j;
+            on[u] = u == 0 || j + u * LANES < nT;
+            el[u] = ju < nE;
+            const int b = el[u] ? eb + ju : cb + (ju - nE);
+            const int* cp = el[u] ? ecol : ccol;
+            const M* vp = el[u] ? ev : cv;
+            const long nz = el[u] ? enz : cnz;
+            c[u] = cp[b];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) a[u][k] = (V)vp[k * nz + b];
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) x[u][k] = z[3 * c[u] + k];
+            if (WITHP) {
+                V pv[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pv[k] = p[3 * c[u] + k];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) x[u][k] += beta * pv[k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (on[u] && el[u] && c[u] == row) {
+                own[0] = x[u][0]; own[1] = x[u][1]; own[2] = x[u][2];
+                hasOwn = true;
+            }
+            if (!on[u]) x[u][0] = x[u][1] = x[u][2] = 0;
+            s[0] += a[u][0] * x[u][0] + a[u][1] * x[u][1] + a[u][2] * x[u][2];
+            s[1] += a[u][3] * x[u][0] + a[u][4] * x[u][1] + a[u][5] * x[u][2];
+            s[2] += a[u][6] * x[u][0] + a[u][7] * x[u][1] + a[u][8] * x[u][2];
+        }
     }
 }
 
 // Iteration `it`: convergence of the residual left by the previous update is decided
 // first; then p_new = z + beta p_old (p_new = z at it = 0, WITHP = false), q = A p_new,
 // partial p.q.  Row bounds of the next row group are loaded one group ahead.
-template <class M, class V, int LANES, int BS, bool WITHP>
+template <class M, class V, int LANES, int BS, bool WITHP, int UNR = 1>
 __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2, const int* __restrict__ erp,
                                                  const int* __restrict__ ecol, const M* __restrict__ ev, int enz,
                                                  const int* __restrict__ crp, const int* __restrict__ ccol, const M* __restrict__ cv,
@@ -110,17 +194,19 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
                                                  const V* __restrict__ pold, V* __restrict__ pnew, V* __restrict__ q,
                                                  const double* __restrict__ partRZ, int nRZ, const double* __restrict__ partRR, int nRR,
                                                  double* __restrict__ partS, double* __restrict__ sc, int* __restrict__ flags) {
-    if (flags[F_DONE]) return;
     __shared__ double lds[BS / 64];
     constexpr int RPG = BS / LANES;   // rows per group
     const int xb = blockIdx.x % kXcd, jb = blockIdx.x / kXcd, nj = gridDim.x / kXcd;
     const int lane = threadIdx.x % LANES;
     const int rowBase = xb * rowsPerXcd + (int)threadIdx.x / LANES;
-    // first group's row bounds, issued before the scalar reduction
+    // first group's row bounds (elastic and contact), issued before the done flag and the
+    // scalar reduction are read
     int g = jb;
     int row = rowBase + g * RPG;
     bool live = g * RPG < rowsPerXcd && row < nrow;
     int beg = live ? erp[row] : 0, end = live ? erp[row + 1] : 0;
+    int cbeg = live && cnz > 0 ? crp[row] : 0, cend = live && cnz > 0 ? crp[row + 1] : 0;
+    if (flags[F_DONE]) return;
 
     double arz = 0, arr = 0;
 #pragma unroll 4
@@ -148,16 +234,12 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
         const int rowN = rowBase + gn * RPG;
         const bool liveN = gn * RPG < rowsPerXcd && rowN < nrow;
         const int begN = liveN ? erp[rowN] : 0, endN = liveN ? erp[rowN + 1] : 0;
+        const int cbegN = liveN && cnz > 0 ? crp[rowN] : 0, cendN = liveN && cnz > 0 ? crp[rowN + 1] : 0;
         V s[3] = {0, 0, 0}, own[3] = {0, 0, 0};
         bool hasOwn = false;
-        if (row < nrow) {
-            bsr_row<M, V, LANES, WITHP>(beg + lane, end, ecol, ev, enz, row, z, pold, beta, s, own, hasOwn);
-            if (cnz > 0) {
-                bool unused = false;
-                V dummy[3];
-                bsr_row<M, V, LANES, WITHP>(crp[row] + lane, crp[row + 1], ccol, cv, cnz, -1, z, pold, beta, s, dummy, unused);
-            }
-        }
+        if (row < nrow)
+            bsr_row_ek<M, V, LANES, WITHP, UNR>(lane, beg, end, cbeg, cend, ecol, ev, enz, ccol, cv, cnz, row, z, pold, beta, s, own,
+                                                hasOwn);
 #pragma unroll
         for (int o = LANES / 2; o > 0; o >>= 1)
 #pragma unroll
@@ -173,6 +255,8 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
         row = rowN;
         beg = begN;
         end = endN;
+        cbeg = cbegN;
+        cend = cendN;
     }
     const double bpq = block_allsum<BS>(pq, lds);
     if (threadIdx.x == 0) partS[blockIdx.x] = bpq;
@@ -180,6 +264,21 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
         sc[S_RZ0 + (it & 1)] = rz;
         if (it == 0) sc[S_BB] = rr;
     }
+}
+
+// the SpMV instance for a launch shape: LANES lanes per block row, UNR blocks per lane pass
+template <class M, class V>
+static auto spmv_kernel(int lanes, int unr, bool withP) -> decltype(&k_pcg_spmv<M, V, 16, kSpmvBlock, true, 1>) {
+#define CIPC_SPMV_PICK(L, U) \
+    if (lanes == L && unr == U) return withP ? k_pcg_spmv<M, V, L, kSpmvBlock, true, U> : k_pcg_spmv<M, V, L, kSpmvBlock, false, U>;
+    CIPC_SPMV_PICK(8, 1)
+    CIPC_SPMV_PICK(8, 2)
+    CIPC_SPMV_PICK(4, 1)
+    CIPC_SPMV_PICK(4, 2)
+    CIPC_SPMV_PICK(16, 2)
+    CIPC_SPMV_PICK(16, 1)
+#undef CIPC_SPMV_PICK
+    return nullptr;
 }
 
 // Iteration `it`: alpha = r.z / p.q, x += alpha p, r -= alpha q, z = Dinv r, partials
@@ -468,7 +567,9 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
 
     const int chunk = 16;
     int* hf = (int*)(c.hCnt + 32);
-    if (c.profiling && c.evPool.size() < 2 * chunk) {
+    // sampled SpMV timing: for the caller's profile and for the autotuner's SpMV phase
+    const bool sampling = c.profiling || (c.tune.pcg && c.tune.pcgPhase == 1);
+    if (sampling && c.evPool.size() < 2 * chunk) {
         for (size_t i = c.evPool.size(); i < 2 * chunk; ++i) {
             hipEvent_t e;
             CIPC_HIP(hipEventCreate(&e));
@@ -484,10 +585,10 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
         for (int k = 0; k < chunk && it <= maxIter; ++k, ++it, ++launched) {
             const V* pold = pbuf + (size_t)((it + 1) & 1) * n;
             V* pnew = pbuf + (size_t)(it & 1) * n;
-            auto kern = c.spmvLanes == 8 ? (it > 0 ? k_pcg_spmv<M, V, 8, kSpmvBlock, true> : k_pcg_spmv<M, V, 8, kSpmvBlock, false>)
-                                         : (it > 0 ? k_pcg_spmv<M, V, 16, kSpmvBlock, true> : k_pcg_spmv<M, V, 16, kSpmvBlock, false>);
-            hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.profiling ? c.evPool[2 * k] : nullptr,
-                                  c.profiling ? c.evPool[2 * k + 1] : nullptr, 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
+            auto kern = spmv_kernel<M, V>(c.spmvLanes, c.spmvUnroll, it > 0);
+            const bool timed = sampling && k % kProfEvery == kProfFirst;
+            hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, timed ? c.evPool[2 * k] : nullptr,
+                                  timed ? c.evPool[2 * k + 1] : nullptr, 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
                                   K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, partRZ, nRZ,
                                   (const double*)(learned && it > 0 ? partF : partU + nU), learned && it > 0 ? nP : nU, partS, sc,
                                   flags);
@@ -511,18 +612,24 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
         CIPC_LAUNCH_CHECK();
         CIPC_HIP(hipMemcpyAsync(hf, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, c.s));
         CIPC_HIP(hipStreamSynchronize(c.s));
-        if (c.profiling) {
+        if (sampling) {
             // SpMV launches of iterations before the one that detected convergence did
             // the full work; the detecting launch and later ones return early
             const int done = hf[F_DONE] ? hf[F_ITERS] : it;
             const int base = it - launched;
-            for (int k = 0; k < launched && base + k < done; ++k) {
+            for (int k = kProfFirst; k < launched && base + k < done; k += kProfEvery) {
                 float ms = 0;
                 CIPC_HIP(hipEventElapsedTime(&ms, c.evPool[2 * k], c.evPool[2 * k + 1]));
-                c.profMs += ms;
-                c.profLaunches++;
-                c.profBytes += spmvBytes;
-                c.profFlops += 18.0 * ((double)E.nnzb + K.nnzb) + 8.0 * n;
+                if (c.profiling) {
+                    c.profMs += ms;
+                    c.profLaunches++;
+                    c.profBytes += spmvBytes;
+                    c.profFlops += 18.0 * ((double)E.nnzb + K.nnzb) + 8.0 * n;
+                }
+                if (c.tune.pcg) {
+                    c.tune.spmvMs += ms;
+                    c.tune.spmvLaunches++;
+                }
             }
         }
         if (hf[F_DONE] || it > maxIter) break;
@@ -692,14 +799,51 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
 }
 
 // Autotuner of the PCG launch shapes: while it runs, each Newton solve uses the next
-// candidate (SpMV grid, factor-product grid, lanes per factor row) and its wall time per
-// PCG iteration is recorded (stream drained before and after); once every candidate has
-// run, the fastest is kept.
+// candidate and records its cost; every candidate runs twice (the list is walked twice,
+// so slow drift of the matrix between solves affects all alike) and the lower cost
+// counts; once the list is done the cheapest is kept.  Phase 1 tries the SpMV shapes
+// (grid x lanes per row x blocks per lane pass) and costs them by the SpMV's own sampled
+// dispatch time; with the learned preconditioner a phase 2 then tries the factor-product
+// shapes (grid x lanes per row) with the chosen SpMV, costed by the solve's wall time per
+// PCG iteration (stream drained before and after).  When the number of stored Hessian
+// blocks later moves more than 5% from the tuned matrix's (contacts forming or
+// releasing), the tuning reruns, at most kRetunes times.
+constexpr int kRetunes = 2;
+
+static void tune_apply(Ctx& c, const std::array<int, 5>& k) {
+    c.spmvGrid = k[0]; c.lpGrid = k[1]; c.lpLanes = k[2]; c.spmvLanes = k[3]; c.spmvUnroll = k[4];
+}
+
+static void tune_phase(Ctx& c, int phase) {
+    Ctx::Tuner& T = c.tune;
+    T.pcgCand.clear();
+    for (int pass = 0; pass < 2; ++pass) {
+        if (phase == 1) {
+            for (int sg : {1024, 2048})
+                for (auto ls : {std::array<int, 2>{16, 1}, {8, 1}, {8, 2}, {16, 2}})
+                    T.pcgCand.push_back({sg, c.lpGrid, c.lpLanes, ls[0], ls[1]});
+        } else {
+            for (int lg : {1024, 2048})
+                for (int ln : {2, 4, 8}) T.pcgCand.push_back({c.spmvGrid, lg, ln, c.spmvLanes, c.spmvUnroll});
+        }
+    }
+    T.pcgCost.assign(T.pcgCand.size(), 0.0);
+    T.pcgIdx = 0;
+    T.pcgPhase = phase;
+    T.pcg = true;
+}
+
 int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
     Ctx::Tuner& T = c.tune;
+    const long nnzb = (long)c.Ael.nnzb + c.Acon.nnzb;
+    if (!T.pcg && T.enabled && T.retunes > 0 && std::fabs((double)(nnzb - T.tunedNnzb)) > 0.05 * (double)T.tunedNnzb) {
+        T.retunes--;
+        tune_phase(c, 1);
+    }
     if (!T.pcg) return pcg_solve_impl(c, b, xsol, tol, maxIter, iters, relres);
-    const auto& k = T.pcgCand[T.pcgIdx];
-    c.spmvGrid = k[0]; c.lpGrid = k[1]; c.lpLanes = k[2]; c.spmvLanes = k[3];
+    tune_apply(c, T.pcgCand[T.pcgIdx]);
+    T.spmvMs = 0;
+    T.spmvLaunches = 0;
     CIPC_HIP(hipStreamSynchronize(c.s));
     const auto t0 = std::chrono::steady_clock::now();
     int it = 0;
@@ -707,14 +851,18 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     CIPC_HIP(hipStreamSynchronize(c.s));
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (iters) *iters = it;
-    T.pcgCost[T.pcgIdx] = it > 0 ? ms / it : 1e300;
+    double cost = it > 0 ? ms / it : 1e300;
+    if (T.pcgPhase == 1 && T.spmvLaunches > 0) cost = T.spmvMs / (double)T.spmvLaunches;
+    T.pcgCost[T.pcgIdx] = cost;
     if (++T.pcgIdx == (int)T.pcgCand.size()) {
+        const int half = (int)T.pcgCand.size() / 2;
         int best = 0;
-        for (int i = 1; i < (int)T.pcgCand.size(); ++i)
-            if (T.pcgCost[i] < T.pcgCost[best]) best = i;
-        c.spmvGrid = T.pcgCand[best][0]; c.lpGrid = T.pcgCand[best][1]; c.lpLanes = T.pcgCand[best][2];
-        c.spmvLanes = T.pcgCand[best][3];
+        for (int i = 1; i < half; ++i)
+            if (std::min(T.pcgCost[i], T.pcgCost[i + half]) < std::min(T.pcgCost[best], T.pcgCost[best + half])) best = i;
+        tune_apply(c, T.pcgCand[best]);
         T.pcg = false;
+        T.tunedNnzb = nnzb;
+        if (T.pcgPhase == 1 && c.precond == 1) tune_phase(c, 2);
     }
     return rc;
 }
@@ -722,18 +870,9 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
 void autotune_start(Ctx& c, int what) {
     Ctx::Tuner& T = c.tune;
     if (what & 1) {
-        T.pcgCand.clear();
-        for (int sl : {16, 8})
-            for (int sg : {1024, 2048})
-                if (c.precond == 1) {
-                    for (int lg : {1024, 2048})
-                        for (int ln : {2, 4, 8}) T.pcgCand.push_back({sg, lg, ln, sl});
-                } else {
-                    T.pcgCand.push_back({sg, c.lpGrid, c.lpLanes, sl});
-                }
-        T.pcgCost.assign(T.pcgCand.size(), 0.0);
-        T.pcgIdx = 0;
-        T.pcg = true;
+        T.enabled = true;
+        T.retunes = kRetunes;
+        tune_phase(c, 1);
     }
     if (what & 2) {
         T.cellCand = {0.5, 0.7, 1.0, 1.4, 2.0, 0.5, 0.7, 1.0, 1.4, 2.0};

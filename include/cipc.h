@@ -166,7 +166,7 @@ int cipc_precond_set_weights(cipc_ctx* ctx, const float* w, int32_t n);
 int cipc_precond_get_weights(cipc_ctx* ctx, float* w, int32_t n);
 int cipc_set_preconditioner(cipc_ctx* ctx, int32_t mode, int32_t budget);
 /* run the network at the first Newton solve of each step and then every `every` solves
- * (>= 1; default: once per step); the solves in between reuse its factor G with the
+ * (>= 1; default 1: at every solve); the solves in between reuse its factor G with the
  * diagonal scaling S of their own Hessian */
 int cipc_set_precond_reuse(cipc_ctx* ctx, int32_t every);
 /* assemble H(x, constraint set) as cipc_linear_solve does, then: loss and the flat
@@ -191,22 +191,35 @@ int cipc_precond_set_adam(cipc_ctx* ctx, const float* m, const float* v, int32_t
 int cipc_precond_info(const cipc_ctx* ctx, int32_t* out);
 
 /* ---- launch shapes and autotuning (no counterpart in the reference) -------
- * cipc_autotune(what): bit 1 -- the next Newton solves cycle through PCG launch shapes
- * (SpMV grid 1024/2048 x 8/16 lanes per block row; with the learned preconditioner also
- * factor-product grid 1024/2048 and 2/4/8 lanes per factor row), timing each per PCG
- * iteration, and keep the fastest; bit 2 -- the next ten constraint-set builds cycle
- * twice through broad-phase cell sizes of 0.5-2x the mean rest edge length and keep the
- * fastest (the constraint set does not depend on it).  out[0..6] of cipc_get_tuning =
- * SpMV grid, factor grid, factor lanes, cell multiplier, PCG tuning running, cell tuning
- * running, SpMV lanes. */
+ * cipc_autotune(what): bit 1 -- the next Newton solves cycle twice through PCG launch
+ * shapes and keep the fastest: first the SpMV shapes (grid 1024/2048 x 16 or 8 lanes per
+ * block row x 1 or 2 blocks per lane pass), timed by the SpMV's sampled dispatch time,
+ * then, with the learned preconditioner, the factor-product shapes (grid 1024/2048 x
+ * 2/4/8 lanes per row), timed per PCG iteration; the tuning reruns (at most twice) when
+ * the number of stored Hessian blocks moves more than 5% from the tuned matrix's;
+ * bit 2 -- the next ten constraint-set builds cycle twice through broad-phase cell
+ * sizes of 0.5-2x the mean rest edge length and keep the fastest (the constraint set
+ * does not depend on it).  cipc_set_tuning fixes the shapes and stops any tuning. */
+typedef struct cipc_tuning {
+    int32_t spmv_grid;      /* workgroups of the PCG SpMV, multiple of 8 */
+    int32_t spmv_lanes;     /* lanes per block row: 4, 8 or 16 */
+    int32_t spmv_unroll;    /* blocks per lane pass: 1 or 2 */
+    int32_t factor_grid;    /* workgroups of the learned factor products, multiple of 8 */
+    int32_t factor_lanes;   /* lanes per factor row: 2, 4, 8 or 16 */
+    int32_t tuning_pcg;     /* out (get only): PCG tuning still running */
+    int32_t tuning_cell;    /* out (get only): cell-size tuning still running */
+    int32_t reserved;
+    double cell_mult;       /* broad-phase cell = cell_mult x mean rest edge length, 0.125-16 */
+} cipc_tuning;
 int cipc_autotune(cipc_ctx* ctx, int32_t what);
-int cipc_get_tuning(const cipc_ctx* ctx, double* out);
-int cipc_set_tuning(cipc_ctx* ctx, int32_t spmv_grid, int32_t spmv_lanes, int32_t factor_grid, int32_t factor_lanes, double cell_mult);
+int cipc_get_tuning(const cipc_ctx* ctx, cipc_tuning* out);
+int cipc_set_tuning(cipc_ctx* ctx, const cipc_tuning* t);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------ */
 /* Kernel timing of the dominant kernel of the step, the PCG SpMV (q = A p with
  * p = z + beta p_old formed on the fly), from the dispatch's own start/end
- * timestamps: enable != 0 starts recording (and resets the counters).
+ * timestamps, sampled on every 8th launch (timing every launch slows the solve by about
+ * 10%): enable != 0 starts recording (and resets the counters).
  * out[0..5] = launches timed, total ms, block rows, stored 3x3 blocks of the
  * elastic matrix, mean algorithmic bytes per launch (matrix values + column indices
  * + row pointers of the elastic and contact matrices, z and p_old read, p_new and q

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from bars import trajectory_bar
 from cipc import Simulator, scenes
 
 pytestmark = pytest.mark.gpu
@@ -195,20 +196,26 @@ def test_linear_solve_mixed_refinement(stack, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_step_trajectory(require_gpu, precision):
-    """Five steps of the contact stack, tight Newton tolerance; FP64 oracle.  The mixed
-    solver refines to the same 1e-10 linear residual, so the same bar applies."""
+    """Five steps of the contact stack, tight Newton tolerance; FP64 oracle.  Before each
+    step the oracle takes the GPU's state (positions, velocities, barrier stiffness), so
+    every step is compared from the same input (bars.trajectory_bar: 1e-6 of the step's
+    displacement or the Newton stopping resolution).  The mixed solver refines to the same
+    1e-10 linear residual, so the same bar applies."""
     sd = scenes.scene_contact_stack()
     sd.params["PNTol"] = 1e-6 * sd.bbox_diag()
     gpu = Simulator(sd, pcg_tol=1e-10, precision=precision)
     orc = O.OracleScene(sd, threads=4)
     for s in range(5):
+        x0, v0 = gpu.get_state()
+        orc.set_state(x0, v0)
+        orc.set_kappa(gpu.scalars()["kappa"])
         st = gpu.step()
         rc, so = orc.step()
         assert rc > 0 and st["status"] == 0
         xg, vg = gpu.get_state()
         xo, vo = orc.get_state()
-        disp = np.abs(xo - sd.X.ravel()).max()
-        assert np.abs(xg - xo).max() <= 1e-6 * max(disp, 1e-3), s
+        disp = np.abs(xo - x0).max()
+        assert np.abs(xg - xo).max() <= trajectory_bar(sd, disp), (s, st["newton_iters"], so)
         d2, m = gpu.min_dist2(xg, gpu.constraint_set(xg)) if len(gpu.constraint_set(xg)) else (None, 1.0)
         assert m > 0
 
@@ -216,19 +223,23 @@ def test_step_trajectory(require_gpu, precision):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_codim_step_trajectory(require_gpu, precision):
     """Six steps of the falling rods / particles / sheet mix (offset 1 mm) against the
-    FP64 oracle, tight Newton tolerance; every state intersection free."""
+    FP64 oracle, tight Newton tolerance, each from the GPU's state as in
+    test_step_trajectory; every state intersection free."""
     sd = scenes.scene_codim_small()
     sd.params["PNTol"] = 1e-6 * sd.bbox_diag()
     gpu = Simulator(sd, pcg_tol=1e-10, precision=precision)
     orc = O.OracleScene(sd, threads=4)
     for s in range(6):
+        x0, v0 = gpu.get_state()
+        orc.set_state(x0, v0)
+        orc.set_kappa(gpu.scalars()["kappa"])
         st = gpu.step()
         rc, so = orc.step()
         assert rc > 0 and st["status"] == 0
         xg, _ = gpu.get_state()
         xo, _ = orc.get_state()
-        disp = np.abs(xo - sd.X.ravel()).max()
-        assert np.abs(xg - xo).max() <= 1e-6 * max(disp, 1e-3), s
+        disp = np.abs(xo - x0).max()
+        assert np.abs(xg - xo).max() <= trajectory_bar(sd, disp), (s, st["newton_iters"], so)
         C = gpu.constraint_set(xg)
         if len(C):
             assert gpu.min_dist2(xg, C)[1] > 0          # min d^2 - offset^2
