@@ -307,11 +307,11 @@ __global__ void k_lp_scatter_lower(int nL, const int* __restrict__ lEdge, const 
 }
 
 
-// ------------------------------------------------------------------ MFMA message layer
-// Inference form of the 192 -> 64 message layer on the matrix cores: inputs and weights
-// in BF16, FP32 accumulation (v_mfma_f32_16x16x16_bf16).  A wave computes 16 rows x 64
+// ------------------------------------------------------------------ MFMA inference layers
+// Inference form of the dense layers on the matrix cores: inputs and weights in BF16,
+// FP32 accumulation (v_mfma_f32_16x16x16_bf16).  A wave computes 16 rows x 16 NT
 // outputs: for every 16-wide k step it feeds one 16x16 A tile (lane l: row l % 16,
-// k = 4 (l / 16) .. +3, from the gathered input row) and four 16x16 B tiles (lane l:
+// k = 4 (l / 16) .. +3, from the gathered input row) and NT 16x16 B tiles (lane l:
 // column 16 j + l % 16, the same 4 k's) held transposed in LDS; the accumulators hold
 // rows 4 (l / 16) + t, column 16 j + l % 16.
 typedef short lp_v4i16 __attribute__((ext_vector_type(4)));
@@ -323,7 +323,8 @@ __device__ __forceinline__ short bf16_bits(float f) {
     return (short)(u >> 16);
 }
 
-// eNext (optional): eNext = eIn + out, the edge-state update fused into the epilogue
+// Message layer (192 -> 64, rows gathered through the edge lists): one 16-row tile per
+// wave; eNext (optional): eNext = eIn + out, the edge-state update fused into the epilogue
 template <int K>
 __global__ __launch_bounds__(256) void k_lp_dense_mfma(int nrows, LpSeg s0, LpSeg s1, LpSeg s2, const float* __restrict__ W,
                                                        const float* __restrict__ bias, int relu, float* __restrict__ out,
@@ -376,6 +377,114 @@ __global__ __launch_bounds__(256) void k_lp_dense_mfma(int nrows, LpSeg s0, LpSe
                 if (eNext) eNext[(long)row * kLpHidden + col] = eIn[(long)row * kLpHidden + col] + y;
             }
         }
+}
+
+
+// General inference layer on the matrix cores (the node-level layers and the edge encoder):
+//   out[r][o] = act(b[o] + sum_seg x_seg[idx_seg(r)] . W[seg rows][o]) (+ resid[r][o]),
+//   o < nout <= 16 NT; optionally eNext = eIn + out.
+// The K axis is the concatenation of the segments, each padded with zeros to a multiple
+// of 16.  The weights are packed once per call into BF16 [16 NT][KP] (transposed,
+// zero-padded; k_lp_wt_pack) and staged in LDS; each wave then walks 16-row tiles
+// (grid-stride), so a workgroup stages them once for many tiles.
+__global__ void k_lp_wt_pack(const float* __restrict__ W, int k0, int k1, int k2, int nout, int NT, int KP, short* __restrict__ Wt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 16 * NT * KP) return;
+    const int n = t / KP, kp = t % KP;
+    const int P0 = (k0 + 15) / 16 * 16, P1 = P0 + (k1 + 15) / 16 * 16;
+    int seg = 0, kk = kp, base = 0;
+    if (kp >= P1) { seg = 2; kk = kp - P1; base = k0 + k1; }
+    else if (kp >= P0) { seg = 1; kk = kp - P0; base = k0; }
+    const int ks = seg == 0 ? k0 : (seg == 1 ? k1 : k2);
+    Wt[t] = (n < nout && kk < ks) ? bf16_bits(W[(long)(base + kk) * nout + n]) : (short)0;
+}
+
+template <int NT, int KP>
+__global__ __launch_bounds__(256) void k_lp_mfma(int nrows, LpSeg s0, LpSeg s1, LpSeg s2, const short* __restrict__ WtG,
+                                                 const float* __restrict__ bias, int nout, int relu, const float* __restrict__ resid,
+                                                 float* __restrict__ out, const float* __restrict__ eIn, float* __restrict__ eNext) {
+    static_assert(KP % 16 == 0 && KP <= kLpMaxK, "");
+    static_assert(NT == 1 || NT * 16 == kLpHidden, "64-wide outputs or one 16-wide tile");
+    constexpr int PITCH = KP + 4;   // shorts; keeps 8-byte alignment of every 4-k group
+    __shared__ short Wt[16 * NT * PITCH];
+    for (int t = threadIdx.x; t < 16 * NT * KP; t += 256) Wt[(t / KP) * PITCH + t % KP] = WtG[t];
+    __syncthreads();
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int i = lane % 16, kg = lane / 16;
+    const int P0 = (s0.k + 15) / 16 * 16, P1 = P0 + (s1.k + 15) / 16 * 16;
+    const int nTiles = (nrows + 15) / 16;
+    for (int tile = blockIdx.x * 4 + wave; tile < nTiles; tile += gridDim.x * 4) {
+        const int rbase = tile * 16;
+        const int r = rbase + i;
+        const bool valid = r < nrows;
+        long rowIn[3] = {0, 0, 0};
+        if (valid) {
+            rowIn[0] = s0.idx ? s0.idx[r] : r;
+            rowIn[1] = s1.idx ? s1.idx[r] : r;
+            rowIn[2] = s2.idx ? s2.idx[r] : r;
+        }
+        // the tile's A operand (this lane's 4 k of every 16-wide step), all loads first
+        lp_v4i16 a[KP / 16];
+        if (KP >= 64) {
+            // wide layers: every segment is 64 wide (one base address per segment)
+#pragma unroll
+            for (int sg = 0; sg < KP / 64; ++sg) {
+                const LpSeg& S = sg == 0 ? s0 : (sg == 1 ? s1 : s2);
+                const float* src = S.x + rowIn[sg] * kLpHidden + 4 * kg;
+                float4 xv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) xv[q] = valid ? *(const float4*)(src + 16 * q) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    a[4 * sg + q] = (lp_v4i16){bf16_bits(xv[q].x), bf16_bits(xv[q].y), bf16_bits(xv[q].z), bf16_bits(xv[q].w)};
+            }
+        } else
+#pragma unroll
+        for (int ks = 0; ks < KP / 16; ++ks) {
+            const int kp0 = ks * 16 + 4 * kg;
+            const int seg = kp0 < P0 ? 0 : (kp0 < P1 ? 1 : 2);
+            const int kk = kp0 - (seg == 0 ? 0 : (seg == 1 ? P0 : P1));
+            const LpSeg& S = seg == 0 ? s0 : (seg == 1 ? s1 : s2);
+            const float* src = S.x + rowIn[seg] * S.k;
+            float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (valid) {
+                if ((S.k & 3) == 0 && kk + 4 <= S.k) {
+                    xv = *(const float4*)(src + kk);
+                } else {
+                    if (kk < S.k) xv.x = src[kk];
+                    if (kk + 1 < S.k) xv.y = src[kk + 1];
+                    if (kk + 2 < S.k) xv.z = src[kk + 2];
+                    if (kk + 3 < S.k) xv.w = src[kk + 3];
+                }
+            }
+            a[ks] = (lp_v4i16){bf16_bits(xv.x), bf16_bits(xv.y), bf16_bits(xv.z), bf16_bits(xv.w)};
+        }
+        lp_v4f32 acc[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[j] = (lp_v4f32){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+        for (int ks = 0; ks < KP / 16; ++ks) {
+            const int kp0 = ks * 16 + 4 * kg;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const lp_v4i16 bv = *(const lp_v4i16*)&Wt[(16 * j + i) * PITCH + kp0];
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ks], bv, acc[j], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int row = rbase + 4 * kg + t, col = 16 * j + i;
+                if (row < nrows && col < nout) {
+                    float y = acc[j][t] + bias[col];
+                    if (relu) y = y > 0.f ? y : 0.f;
+                    if (resid) y += resid[(long)row * nout + col];
+                    out[(long)row * nout + col] = y;
+                    if (eNext) eNext[(long)row * nout + col] = eIn[(long)row * nout + col] + y;
+                }
+            }
+    }
 }
 
 // ------------------------------------------------------------------ G assembly / products
@@ -881,6 +990,28 @@ static void add(Ctx& c, long n, const float* a, const float* b, float* out) {
     CIPC_LAUNCH_CHECK();
 }
 
+// workgroups of the general MFMA layers (waves walk 16-row tiles grid-stride)
+constexpr int kLpMfmaGrid = 8192;
+
+static void mfma_dense(Ctx& c, int nrows, LpSeg s0, LpSeg s1, LpSeg s2, const float* W, const float* b, int nout, int relu,
+                       const float* resid, float* out, const float* eIn = nullptr, float* eNext = nullptr) {
+    if (nrows <= 0) return;
+    LearnedPrecond& lp = LP(c);
+    const int KP = (s0.k + 15) / 16 * 16 + (s1.k + 15) / 16 * 16 + (s2.k + 15) / 16 * 16;
+    const int NT = (nout + 15) / 16;
+    if (KP > kLpMaxK || !(NT == 1 || NT == 4)) throw HipError("learned preconditioner: MFMA layer shape", -1);
+    short* wt = lp.wt16.ensure_exact((size_t)64 * kLpMaxK);
+    const int nw = 16 * NT * KP;
+    hipLaunchKernelGGL(k_lp_wt_pack, dim3(nblocks(nw)), dim3(kBlock), 0, c.s, W, s0.k, s1.k, s2.k, nout, NT, KP, wt);
+    const int grid = std::min(kLpMfmaGrid, ((nrows + 15) / 16 + 3) / 4);
+    auto kern = NT == 4 ? (KP == 16 ? k_lp_mfma<4, 16> : KP == 64 ? k_lp_mfma<4, 64> : k_lp_mfma<4, 128>)
+                        : (KP == 16 ? k_lp_mfma<1, 16> : KP == 64 ? k_lp_mfma<1, 64> : k_lp_mfma<1, 128>);
+    const bool wide = (s0.k == 64) && (s1.k == 0 || s1.k == 64) && s2.k == 0;
+    if (!(KP == 16 || (wide && (KP == 64 || KP == 128)))) throw HipError("learned preconditioner: MFMA layer shape", -1);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c.s, nrows, s0, s1, s2, (const short*)wt, b, nout, relu, resid, out, eIn, eNext);
+    CIPC_LAUNCH_CHECK();
+}
+
 static void features(Ctx& c) {
     LearnedPrecond& lp = LP(c);
     BSR& E = c.Ael;
@@ -896,24 +1027,65 @@ static void features(Ctx& c) {
     CIPC_LAUNCH_CHECK();
 }
 
+// G, FL, FU from the decoder outputs Gl, raw and the scaling S
+static void lp_assemble(Ctx& c) {
+    LearnedPrecond& lp = LP(c);
+    const int nV = lp.nV, nL = lp.nL;
+    hipLaunchKernelGGL(k_lp_assemble_G, dim3(nblocks(nL + nV)), dim3(kBlock), 0, c.s, nL, nV, lp.nnzb, (const int*)lp.lEdge.p,
+                       (const int*)lp.eBlock.p, (const int*)lp.blkT.p, (const int*)lp.diagBlk.p, (const int*)c.Ael.col.p,
+                       (const double*)lp.S.p, (const float*)lp.Gl.p, (const float*)lp.raw.p, lp.Gblk.p, lp.nLow,
+                       (const int*)lp.posLE.p, (const int*)lp.posLD.p, (const int*)lp.posUE.p, (const int*)lp.posUD.p, lp.FL.p,
+                       lp.FU.p);
+    CIPC_LAUNCH_CHECK();
+    lp.forwardReady = true;
+}
+
+// inference form: the encoders, message layers, node updates and the diagonal decoder on
+// the matrix cores (BF16 inputs and weights, FP32 accumulation and FP32 activations); the
+// off-diagonal decoder (nine outputs per lower edge) in FP32
+static void lp_forward_mfma(Ctx& c) {
+    LearnedPrecond& lp = LP(c);
+    const LpLayout& L = lp.L;
+    const float* Wq = lp.Wq.p;
+    const int nV = lp.nV, nE = lp.nE, nL = lp.nL;
+    mfma_dense(c, nV, {lp.nodeF.p, nullptr, kLpNodeIn}, kNoSeg, kNoSeg, Wq + L.we, Wq + L.be, kLpHidden, 1, nullptr, lp.h[0].p);
+    mfma_dense(c, nE, {lp.edgeF.p, nullptr, kLpEdgeIn}, kNoSeg, kNoSeg, Wq + L.wx, Wq + L.bx, kLpHidden, 1, nullptr, lp.e[0].p);
+    for (int l = 0; l < kLpLayers; ++l) {
+        const LpSeg ms0 = {lp.h[l].p, lp.eSrc.p, kLpHidden}, ms1 = {lp.h[l].p, lp.eDst.p, kLpHidden}, ms2 = {lp.e[l].p, nullptr, kLpHidden};
+        if (nE > 0) {
+            hipLaunchKernelGGL(k_lp_dense_mfma<3 * kLpHidden>, dim3((nE + 63) / 64), dim3(256), 0, c.s, nE, ms0, ms1, ms2, Wq + L.wm[l],
+                               Wq + L.bm[l], 1, lp.m[l].p, (const float*)lp.e[l].p, lp.e[l + 1].p);
+            CIPC_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_lp_agg, dim3((nV + 3) / 4), dim3(256), 0, c.s, nV, (const int*)lp.erp.p, (const float*)lp.m[l].p, lp.agg[l].p);
+        CIPC_LAUNCH_CHECK();
+        // h_{l+1} = h_l + relu(Wu [h_l, agg] + bu)
+        mfma_dense(c, nV, {lp.h[l].p, nullptr, kLpHidden}, {lp.agg[l].p, nullptr, kLpHidden}, kNoSeg, Wq + L.wu[l], Wq + L.bu[l], kLpHidden,
+                   1, lp.h[l].p, lp.h[l + 1].p);
+    }
+    const float* hL = lp.h[kLpLayers].p;
+    dense<9>(c, nL, {hL, lp.lowSrc.p, kLpHidden}, {hL, lp.lowDst.p, kLpHidden}, {lp.e[kLpLayers].p, lp.lEdge.p, kLpHidden}, Wq + L.wo,
+             Wq + L.bo, 0, nullptr, lp.Gl.p);
+    mfma_dense(c, nV, {hL, nullptr, kLpHidden}, kNoSeg, kNoSeg, Wq + L.wd, Wq + L.bd, 6, 0, nullptr, lp.raw.p);
+}
+
 void lp_forward(Ctx& c) {
     LearnedPrecond& lp = LP(c);
     const LpLayout& L = lp.L;
     const float* Wq = lp.Wq.p;
     const int nV = lp.nV, nE = lp.nE, nL = lp.nL;
     const long H = kLpHidden;
+    if (lp.mfma) {
+        lp_forward_mfma(c);
+        lp_assemble(c);
+        return;
+    }
     dense<kLpHidden>(c, nV, {lp.nodeF.p, nullptr, kLpNodeIn}, kNoSeg, kNoSeg, Wq + L.we, Wq + L.be, 1, nullptr, lp.h[0].p);
     dense<kLpHidden>(c, nE, {lp.edgeF.p, nullptr, kLpEdgeIn}, kNoSeg, kNoSeg, Wq + L.wx, Wq + L.bx, 1, nullptr, lp.e[0].p);
     for (int l = 0; l < kLpLayers; ++l) {
         const LpSeg ms0 = {lp.h[l].p, lp.eSrc.p, kLpHidden}, ms1 = {lp.h[l].p, lp.eDst.p, kLpHidden}, ms2 = {lp.e[l].p, nullptr, kLpHidden};
-        if (lp.mfma && nE > 0) {
-            hipLaunchKernelGGL(k_lp_dense_mfma<3 * kLpHidden>, dim3((nE + 63) / 64), dim3(256), 0, c.s, nE, ms0, ms1, ms2, Wq + L.wm[l],
-                               Wq + L.bm[l], 1, lp.m[l].p, (const float*)lp.e[l].p, lp.e[l + 1].p);
-            CIPC_LAUNCH_CHECK();
-        } else {
-            dense<kLpHidden>(c, nE, ms0, ms1, ms2, Wq + L.wm[l], Wq + L.bm[l], 1, nullptr, lp.m[l].p);
-            add(c, H * nE, lp.e[l].p, lp.m[l].p, lp.e[l + 1].p);
-        }
+        dense<kLpHidden>(c, nE, ms0, ms1, ms2, Wq + L.wm[l], Wq + L.bm[l], 1, nullptr, lp.m[l].p);
+        add(c, H * nE, lp.e[l].p, lp.m[l].p, lp.e[l + 1].p);
         hipLaunchKernelGGL(k_lp_agg, dim3((nV + 3) / 4), dim3(256), 0, c.s, nV, (const int*)lp.erp.p, (const float*)lp.m[l].p, lp.agg[l].p);
         CIPC_LAUNCH_CHECK();
         dense<kLpHidden>(c, nV, {lp.h[l].p, nullptr, kLpHidden}, {lp.agg[l].p, nullptr, kLpHidden}, kNoSeg, Wq + L.wu[l], Wq + L.bu[l], 1,
@@ -924,13 +1096,7 @@ void lp_forward(Ctx& c) {
     dense<9>(c, nL, {hL, lp.lowSrc.p, kLpHidden}, {hL, lp.lowDst.p, kLpHidden}, {lp.e[kLpLayers].p, lp.lEdge.p, kLpHidden}, Wq + L.wo,
              Wq + L.bo, 0, nullptr, lp.Gl.p);
     dense<6>(c, nV, {hL, nullptr, kLpHidden}, kNoSeg, kNoSeg, Wq + L.wd, Wq + L.bd, 0, nullptr, lp.raw.p);
-    hipLaunchKernelGGL(k_lp_assemble_G, dim3(nblocks(nL + nV)), dim3(kBlock), 0, c.s, nL, nV, lp.nnzb, (const int*)lp.lEdge.p,
-                       (const int*)lp.eBlock.p, (const int*)lp.blkT.p, (const int*)lp.diagBlk.p, (const int*)c.Ael.col.p,
-                       (const double*)lp.S.p, (const float*)lp.Gl.p, (const float*)lp.raw.p, lp.Gblk.p, lp.nLow,
-                       (const int*)lp.posLE.p, (const int*)lp.posLD.p, (const int*)lp.posUE.p, (const int*)lp.posUD.p, lp.FL.p,
-                       lp.FU.p);
-    CIPC_LAUNCH_CHECK();
-    lp.forwardReady = true;
+    lp_assemble(c);
 }
 
 bool lp_refresh(Ctx& c) {

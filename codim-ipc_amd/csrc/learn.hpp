@@ -58,6 +58,7 @@ struct LearnedPrecond {
     DevBuf<float> h[kLpLayers + 1], e[kLpLayers + 1], m[kLpLayers], u[kLpLayers], agg[kLpLayers];
     DevBuf<float> Gl, raw;            // [nL][9], [nV][6]
     DevBuf<float> Gblk;               // SoA [9][nnzb]: lower blocks, transposes at upper positions, G_ii on the diagonal
+    DevBuf<short> wt16;               // one layer's weights packed for the MFMA inference layers (BF16, transposed)
     // training scratch
     DevBuf<float> dh, dh2, de, de2, dtmp, dtmp2, dtmp3, dagg, dY, dGl, draw, wpartial;
     DevBuf<double> probes, pu, pv, lossPart;
