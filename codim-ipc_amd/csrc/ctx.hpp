@@ -163,10 +163,12 @@ struct Ctx {
         bool enabled = false;                      // PCG tuning requested (re-tunes allowed)
         int retunes = 0;
         long tunedNnzb = 0;                        // stored blocks of the matrix tuned on
-        double spmvMs = 0;                         // sampled SpMV time of the current solve
-        long spmvLaunches = 0;
         int pcgIdx = 0, cellIdx = 0, pcgPhase = 0;
-        std::vector<std::array<int, 5>> pcgCand;   // (spmvGrid, lpGrid, lpLanes, spmvLanes, spmvUnroll)
+        std::vector<std::array<int, 3>> spmvCand;  // phase 1: (spmvGrid, spmvLanes, spmvUnroll)
+        std::vector<double> spmvMs;                // sampled SpMV time per shape
+        std::vector<long> spmvN;
+        long spmvTurn = 0;                         // chunk counter choosing the shape
+        std::vector<std::array<int, 2>> pcgCand;   // phase 2: (lpGrid, lpLanes)
         std::vector<double> pcgCost, cellCand, cellCost;
     } tune;
 

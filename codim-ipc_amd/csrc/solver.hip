@@ -542,12 +542,17 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     BSR& K = c.Acon;
     double* sc = c.scal.p + 32;               // solver scalars
     int* flags = (int*)(c.counters.p + 32);   // solver flags
-    const int RPG = kSpmvBlock / c.spmvLanes;
-    const int rpxS = rows_per_xcd(nV, RPG), nS = c.spmvGrid;
+    Ctx::Tuner& T = c.tune;
+    // the autotuner's SpMV phase changes the SpMV shape from chunk to chunk (all shapes
+    // compute the same q and p.q partials, so the iteration is unchanged)
+    const bool spmvTuning = T.pcg && T.pcgPhase == 1;
+    int nSmax = c.spmvGrid;
+    if (spmvTuning)
+        for (const auto& k : T.spmvCand) nSmax = std::max(nSmax, k[0]);
     const int rpxU = rows_per_xcd(nV, kUpdBlock), nU = kXcd * (rpxU / kUpdBlock);
     const int nP = lp_apply_partials(c);
-    double* partS = c.dscr.ensure((size_t)nS + 2l * nU + 2l * nP);
-    double* partU = partS + nS;
+    double* partS = c.dscr.ensure((size_t)nSmax + 2l * nU + 2l * nP);
+    double* partU = partS + nSmax;
     double* partP = partU + 2l * nU;
     double* partF = partP + nP;               // r.r partials of the fused learned update
     // r.z partials: the update kernel's (block Jacobi) or the preconditioner product's
@@ -568,7 +573,7 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     const int chunk = 16;
     int* hf = (int*)(c.hCnt + 32);
     // sampled SpMV timing: for the caller's profile and for the autotuner's SpMV phase
-    const bool sampling = c.profiling || (c.tune.pcg && c.tune.pcgPhase == 1);
+    const bool sampling = c.profiling || spmvTuning;
     if (sampling && c.evPool.size() < 2 * chunk) {
         for (size_t i = c.evPool.size(); i < 2 * chunk; ++i) {
             hipEvent_t e;
@@ -582,10 +587,16 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     int it = 0;
     for (;;) {
         int launched = 0;
+        int cand = -1, nS = c.spmvGrid, sLanes = c.spmvLanes, sUnroll = c.spmvUnroll;
+        if (spmvTuning) {
+            cand = (int)(T.spmvTurn++ % (long)T.spmvCand.size());
+            nS = T.spmvCand[cand][0]; sLanes = T.spmvCand[cand][1]; sUnroll = T.spmvCand[cand][2];
+        }
+        const int rpxS = rows_per_xcd(nV, kSpmvBlock / sLanes);
         for (int k = 0; k < chunk && it <= maxIter; ++k, ++it, ++launched) {
             const V* pold = pbuf + (size_t)((it + 1) & 1) * n;
             V* pnew = pbuf + (size_t)(it & 1) * n;
-            auto kern = spmv_kernel<M, V>(c.spmvLanes, c.spmvUnroll, it > 0);
+            auto kern = spmv_kernel<M, V>(sLanes, sUnroll, it > 0);
             const bool timed = sampling && k % kProfEvery == kProfFirst;
             hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, timed ? c.evPool[2 * k] : nullptr,
                                   timed ? c.evPool[2 * k + 1] : nullptr, 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
@@ -626,9 +637,9 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
                     c.profBytes += spmvBytes;
                     c.profFlops += 18.0 * ((double)E.nnzb + K.nnzb) + 8.0 * n;
                 }
-                if (c.tune.pcg) {
-                    c.tune.spmvMs += ms;
-                    c.tune.spmvLaunches++;
+                if (cand >= 0) {
+                    T.spmvMs[cand] += ms;
+                    T.spmvN[cand]++;
                 }
             }
         }
@@ -798,39 +809,45 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
     return rc;
 }
 
-// Autotuner of the PCG launch shapes: while it runs, each Newton solve uses the next
-// candidate and records its cost; every candidate runs twice (the list is walked twice,
-// so slow drift of the matrix between solves affects all alike) and the lower cost
-// counts; once the list is done the cheapest is kept.  Phase 1 tries the SpMV shapes
-// (grid x lanes per row x blocks per lane pass) and costs them by the SpMV's own sampled
-// dispatch time; with the learned preconditioner a phase 2 then tries the factor-product
-// shapes (grid x lanes per row) with the chosen SpMV, costed by the solve's wall time per
-// PCG iteration (stream drained before and after).  When the number of stored Hessian
-// blocks later moves more than 5% from the tuned matrix's (contacts forming or
-// releasing), the tuning reruns, at most kRetunes times.
+// Autotuner of the PCG launch shapes.  Phase 1 (SpMV grid x lanes per row x blocks per
+// lane pass): the Newton solves switch the SpMV shape every chunk of 16 iterations, in
+// turn, and time it on sampled launches, so every shape is timed on the same matrices;
+// once each has kTuneSamples samples the fastest mean is kept.  Phase 2 (learned
+// preconditioner only: factor-product grid x lanes per row): each Newton solve uses the
+// next candidate; every candidate runs twice (the list is walked twice, so slow drift of
+// the matrix between solves affects all alike), costed by the solve's wall time per PCG
+// iteration (stream drained before and after), and the lower of its two costs counts.
+// When the number of stored Hessian blocks later moves more than 5% from the tuned
+// matrix's (contacts forming or releasing), the tuning reruns, at most kRetunes times.
 constexpr int kRetunes = 2;
-
-static void tune_apply(Ctx& c, const std::array<int, 5>& k) {
-    c.spmvGrid = k[0]; c.lpGrid = k[1]; c.lpLanes = k[2]; c.spmvLanes = k[3]; c.spmvUnroll = k[4];
-}
+constexpr long kTuneSamples = 24;
 
 static void tune_phase(Ctx& c, int phase) {
     Ctx::Tuner& T = c.tune;
     T.pcgCand.clear();
-    for (int pass = 0; pass < 2; ++pass) {
-        if (phase == 1) {
-            for (int sg : {1024, 2048})
-                for (auto ls : {std::array<int, 2>{16, 1}, {8, 1}, {8, 2}, {16, 2}})
-                    T.pcgCand.push_back({sg, c.lpGrid, c.lpLanes, ls[0], ls[1]});
-        } else {
+    if (phase == 1) {
+        T.spmvCand.clear();
+        for (int sg : {1024, 2048})
+            for (auto ls : {std::array<int, 2>{16, 1}, {8, 1}, {8, 2}, {16, 2}}) T.spmvCand.push_back({sg, ls[0], ls[1]});
+        T.spmvMs.assign(T.spmvCand.size(), 0.0);
+        T.spmvN.assign(T.spmvCand.size(), 0);
+        T.spmvTurn = 0;
+    } else {
+        for (int pass = 0; pass < 2; ++pass)
             for (int lg : {1024, 2048})
-                for (int ln : {2, 4, 8}) T.pcgCand.push_back({c.spmvGrid, lg, ln, c.spmvLanes, c.spmvUnroll});
-        }
+                for (int ln : {2, 4, 8}) T.pcgCand.push_back({lg, ln});
+        T.pcgCost.assign(T.pcgCand.size(), 0.0);
+        T.pcgIdx = 0;
     }
-    T.pcgCost.assign(T.pcgCand.size(), 0.0);
-    T.pcgIdx = 0;
     T.pcgPhase = phase;
     T.pcg = true;
+}
+
+static void tune_done(Ctx& c, long nnzb) {
+    Ctx::Tuner& T = c.tune;
+    T.pcg = false;
+    T.tunedNnzb = nnzb;
+    if (T.pcgPhase == 1 && c.precond == 1) tune_phase(c, 2);
 }
 
 int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
@@ -841,9 +858,21 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
         tune_phase(c, 1);
     }
     if (!T.pcg) return pcg_solve_impl(c, b, xsol, tol, maxIter, iters, relres);
-    tune_apply(c, T.pcgCand[T.pcgIdx]);
-    T.spmvMs = 0;
-    T.spmvLaunches = 0;
+    if (T.pcgPhase == 1) {
+        const int rc = pcg_solve_impl(c, b, xsol, tol, maxIter, iters, relres);
+        long nMin = T.spmvN[0];
+        for (long v : T.spmvN) nMin = std::min(nMin, v);
+        if (nMin >= kTuneSamples) {
+            int best = 0;
+            for (int i = 1; i < (int)T.spmvCand.size(); ++i)
+                if (T.spmvMs[i] / (double)T.spmvN[i] < T.spmvMs[best] / (double)T.spmvN[best]) best = i;
+            c.spmvGrid = T.spmvCand[best][0]; c.spmvLanes = T.spmvCand[best][1]; c.spmvUnroll = T.spmvCand[best][2];
+            tune_done(c, nnzb);
+        }
+        return rc;
+    }
+    c.lpGrid = T.pcgCand[T.pcgIdx][0];
+    c.lpLanes = T.pcgCand[T.pcgIdx][1];
     CIPC_HIP(hipStreamSynchronize(c.s));
     const auto t0 = std::chrono::steady_clock::now();
     int it = 0;
@@ -851,18 +880,15 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     CIPC_HIP(hipStreamSynchronize(c.s));
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (iters) *iters = it;
-    double cost = it > 0 ? ms / it : 1e300;
-    if (T.pcgPhase == 1 && T.spmvLaunches > 0) cost = T.spmvMs / (double)T.spmvLaunches;
-    T.pcgCost[T.pcgIdx] = cost;
+    T.pcgCost[T.pcgIdx] = it > 0 ? ms / it : 1e300;
     if (++T.pcgIdx == (int)T.pcgCand.size()) {
         const int half = (int)T.pcgCand.size() / 2;
         int best = 0;
         for (int i = 1; i < half; ++i)
             if (std::min(T.pcgCost[i], T.pcgCost[i + half]) < std::min(T.pcgCost[best], T.pcgCost[best + half])) best = i;
-        tune_apply(c, T.pcgCand[best]);
-        T.pcg = false;
-        T.tunedNnzb = nnzb;
-        if (T.pcgPhase == 1 && c.precond == 1) tune_phase(c, 2);
+        c.lpGrid = T.pcgCand[best][0];
+        c.lpLanes = T.pcgCand[best][1];
+        tune_done(c, nnzb);
     }
     return rc;
 }
