@@ -114,6 +114,7 @@ struct Ctx {
     BSR Ael;
     DevBuf<int> planOff, planSlot;     // per BSR block: list of local block slots
     DevBuf<double> elLocal;            // local 3x3 blocks, [nSlots][9]
+    DevBuf<double> hingeH;             // hinge Hessians before projection, [nH][12][12]
     int nElSlots = 0, nTriSlots = 0;
     DevBuf<int> gplanOff, gplanSlot;   // per vertex: gradient slots (tri 3 per, hinge 4 per)
     DevBuf<double> gLocal;             // local gradient 3-vectors, [nGSlots][3]

@@ -282,23 +282,14 @@ __global__ void __launch_bounds__(64) k_memb_hess_local(const double* __restrict
 #pragma unroll
                 for (int l = 0; l < 3; ++l) hl[9 * (9l * t + 3 * p + q) + 3 * k + l] = H[(3 * p + k) * 9 + 3 * q + l];
 }
-// Hinge Hessians with 16 lanes per hinge (lanes 0..11 own one row each, 16 hinges per
-// workgroup): row r of the dihedral Hessian is evaluated by lane r, symmetrised through
-// LDS, combined with the gradient term, and projected to PSD by a parallel cyclic Jacobi
-// eigensolver -- each round rotates 6 disjoint index pairs of a round-robin schedule
-// (12 players: 11 rounds per sweep), rows exchanged between partner lanes and rotation
-// parameters broadcast by shuffles -- then rebuilt as V max(L, 0) V^T (Math/UTILS.h:
-// 10-27).  Registers stay at one row of A and one of V per lane (no spills).
-__device__ __forceinline__ int rr_partner(int r, int t) {
-    if (r >= 12) return r;
-    if (r == 11) return t;
-    if (r == t) return 11;
-    return (2 * t - r + 22) % 11;
-}
-__global__ void __launch_bounds__(256) k_bend_hess_local(const double* __restrict__ x, const int* __restrict__ hg, int nH,
-                                                         const double* __restrict__ hi, double kb, double h2,
-                                                         const uint8_t* __restrict__ dbc, double* __restrict__ hl, long ns, long off) {
-    __shared__ double sh[16][168];
+// Hinge Hessians in two kernels, so that the register-hungry row evaluation and the
+// eigensolver each run at their own occupancy (Math/UTILS.h:10-27 for the projection).
+// Stage 1: the symmetrised, scaled hinge Hessian, row r by lane r, into H[144 h + 12 r + c]
+// (zero for inactive hinges).
+__global__ void __launch_bounds__(256) k_bend_hess_rows(const double* __restrict__ x, const int* __restrict__ hg, int nH,
+                                                        const double* __restrict__ hi, double kb, double h2,
+                                                        const uint8_t* __restrict__ dbc, double* __restrict__ H) {
+    __shared__ double sh[16][144];
     const int grp = threadIdx.x >> 4, r = threadIdx.x & 15;
     const int h = blockIdx.x * 16 + grp;
     const bool live = h < nH;
@@ -325,112 +316,126 @@ __global__ void __launch_bounds__(256) k_bend_hess_local(const double* __restric
     double gr = 0;
 #pragma unroll
     for (int c = 0; c < 12; ++c) gr = (c == r) ? g[c] : gr;
-    if (r < 12)
+    if (live && r < 12)
 #pragma unroll
-        for (int c = 0; c < 12; ++c) A[c] = s1 * (0.5 * (M[r * 12 + c] + M[c * 12 + r])) + s2 * gr * g[c];
-    __syncthreads();
-    // Jacobi: V = I; stop when the squared off-diagonal mass is <= 1e-24 of ||A||_F^2
-    // (entries of the rebuilt matrix then carry relative errors of ~1e-12).  Partner rows
-    // and rotation parameters go through the hinge's LDS rows; a hinge's 16 lanes lie in
-    // one wave, whose LDS operations complete in issue order.
-    double* P = sh[grp] + 144;   // rotation (c, s) of each pair, at its lower index
-    double V[12];
+        for (int c = 0; c < 12; ++c) H[144l * h + 12 * r + c] = s1 * (0.5 * (M[r * 12 + c] + M[c * 12 + r])) + s2 * gr * g[c];
+}
+// Stage 2, one hinge per thread: the dihedral energy is invariant under translations, so
+// H (x + t) = 0 for every rigid translation t = (u, u, u, u), and with the orthonormal
+// basis Q = Wq (x) I3 of the translation complement (Wq: the three +-1/2 patterns
+// orthogonal to (1, 1, 1, 1)) the projection is H+ = Q proj(Q^T H Q) Q^T.  The 9x9
+// problem fits in registers (45 upper entries + 81 eigenvector entries): cyclic Jacobi,
+// fully unrolled over the 36 pairs of a sweep, stopping at off^2 <= 1e-24 ||H9||_F^2.
+__device__ __forceinline__ constexpr int u9(int i, int j) { return i <= j ? i * 9 - i * (i - 1) / 2 + (j - i) : j * 9 - j * (j - 1) / 2 + (i - j); }
+__device__ __forceinline__ constexpr int wsign(int k, int a) {
+    // Wq[k][a] = wsign / 2: columns (+,-,+,-), (+,+,-,-), (+,-,-,+)
+    return a == 0 ? ((k & 1) ? -1 : 1) : a == 1 ? ((k >> 1) ? -1 : 1) : (((k & 1) ^ (k >> 1)) ? -1 : 1);
+}
+__global__ void __launch_bounds__(64) k_bend_hess_proj9(const double* __restrict__ H, int nH, double* __restrict__ hl, long off) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nH) return;
+    const double* Hh = H + 144l * h;
+    // H9 = Q^T H Q, upper triangle: block (a, b), entry (i, j) = 1/4 sum_kl s_ka s_lb H[3k+i][3l+j]
+    double a9[45];
 #pragma unroll
-    for (int c = 0; c < 12; ++c) V[c] = (c == r) ? 1.0 : 0.0;
+    for (int e = 0; e < 45; ++e) a9[e] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double v = 0.25 * Hh[(3 * k + i) * 12 + 3 * l + j];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) {
+                            const int R = 3 * a + i, C = 3 * b + j;
+                            if (R <= C) a9[u9(R, C)] += (wsign(k, a) * wsign(l, b) > 0) ? v : -v;
+                        }
+                }
+    double V[81];
+#pragma unroll
+    for (int e = 0; e < 81; ++e) V[e] = (e % 10 == 0) ? 1.0 : 0.0;
     double fro = 0;
 #pragma unroll
-    for (int c = 0; c < 12; ++c) fro += A[c] * A[c];
+    for (int i = 0; i < 9; ++i)
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) fro += __shfl_xor(fro, o, 16);
+        for (int j = i; j < 9; ++j) fro += (i == j ? 1.0 : 2.0) * a9[u9(i, j)] * a9[u9(i, j)];
     const double tol = 1e-24 * fro;
     for (int sweep = 0; sweep < 16; ++sweep) {
-        double off = 0;
+        double offd = 0;
 #pragma unroll
-        for (int c = 0; c < 12; ++c) off += (c > r) ? A[c] * A[c] : 0.0;
+        for (int i = 0; i < 9; ++i)
 #pragma unroll
-        for (int o = 8; o > 0; o >>= 1) off += __shfl_xor(off, o, 16);
-        if (off <= tol) break;
+            for (int j = i + 1; j < 9; ++j) offd += a9[u9(i, j)] * a9[u9(i, j)];
+        if (offd <= 0.5 * tol) break;   // off^2 of the full matrix is twice the upper sum
 #pragma unroll
-        for (int t = 0; t < 11; ++t) {
-            const int pr = rr_partner(r, t);
-            if (r < 12)
+        for (int p = 0; p < 8; ++p)
 #pragma unroll
-                for (int c = 0; c < 12; ++c) M[r * 12 + c] = A[c];
-            __builtin_amdgcn_wave_barrier();
-            double o[12];
-            double app = 0, apq = 0, aqq = 0;
-            if (r < 12) {
+            for (int q = p + 1; q < 9; ++q) {
+                const double apq = a9[u9(p, q)];
+                if (apq == 0.0) continue;
+                const double app = a9[u9(p, p)], aqq = a9[u9(q, q)];
+                const double theta = (aqq - app) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                const double c = 1 / sqrt(t * t + 1), s = t * c;
+                a9[u9(p, p)] = app - t * apq;
+                a9[u9(q, q)] = aqq + t * apq;
+                a9[u9(p, q)] = 0.0;
 #pragma unroll
-                for (int c = 0; c < 12; ++c) o[c] = M[pr * 12 + c];
-                app = M[r * 13];
-                apq = M[r * 12 + pr];
-                aqq = M[pr * 13];
-            } else {
-#pragma unroll
-                for (int c = 0; c < 12; ++c) o[c] = 0.0;
-            }
-            // rotation of the pair, computed by its lower-index lane
-            if (r < pr) {
-                double cl = 1, sl = 0;
-                if (apq != 0.0) {
-                    const double theta = (aqq - app) / (2 * apq);
-                    const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
-                    cl = 1 / sqrt(tt * tt + 1);
-                    sl = tt * cl;
+                for (int k = 0; k < 9; ++k) {
+                    if (k == p || k == q) continue;
+                    const double akp = a9[u9(k, p)], akq = a9[u9(k, q)];
+                    a9[u9(k, p)] = c * akp - s * akq;
+                    a9[u9(k, q)] = s * akp + c * akq;
                 }
-                P[2 * r] = cl;
-                P[2 * r + 1] = sl;
-            }
-            __builtin_amdgcn_wave_barrier();
-            // row rotation with the partner's row (P^T A)
-            if (r < 12) {
-                const int ld = min(r, pr);
-                const double cs = P[2 * ld], sn = r < pr ? -P[2 * ld + 1] : P[2 * ld + 1];
 #pragma unroll
-                for (int c = 0; c < 12; ++c) A[c] = cs * A[c] + sn * o[c];
-            }
-            // column rotations of all six pairs (A P, V P); the round's pairing is known
-            // at compile time once the round loop is unrolled
-            double An[12], Vn[12];
-#pragma unroll
-            for (int c = 0; c < 12; ++c) {
-                const int q = rr_partner(c, t), L = c < q ? c : q;
-                const double cc = P[2 * L], ss = P[2 * L + 1];
-                if (c < q) {
-                    An[c] = cc * A[c] - ss * A[q];
-                    Vn[c] = cc * V[c] - ss * V[q];
-                } else {
-                    An[c] = ss * A[q] + cc * A[c];
-                    Vn[c] = ss * V[q] + cc * V[c];
+                for (int k = 0; k < 9; ++k) {
+                    const double vkp = V[9 * k + p], vkq = V[9 * k + q];
+                    V[9 * k + p] = c * vkp - s * vkq;
+                    V[9 * k + q] = s * vkp + c * vkq;
                 }
             }
+    }
+    // H9+ = V diag(max(lambda, 0)) V^T (upper), reusing a9
+    double lam[9];
 #pragma unroll
-            for (int c = 0; c < 12; ++c) { A[c] = An[c]; V[c] = Vn[c]; }
-            __builtin_amdgcn_wave_barrier();
+    for (int i = 0; i < 9; ++i) lam[i] = a9[u9(i, i)] > 0 ? a9[u9(i, i)] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int j = i; j < 9; ++j) {
+            double sum = 0;
+#pragma unroll
+            for (int m = 0; m < 9; ++m) sum += V[9 * i + m] * lam[m] * V[9 * j + m];
+            a9[u9(i, j)] = sum;
         }
-    }
-    // H+ = V diag(max(lambda, 0)) V^T, lambda_r = A[r][r]
-    double lam = 0;
+    // H+ = Q H9+ Q^T: block (k, l), entry (i, j) = 1/4 sum_ab s_ka s_lb H9+[3a+i][3b+j],
+    // written as the hinge's 16 local 3x3 blocks
+    double* dst = hl + 9 * (off + 16l * h);
 #pragma unroll
-    for (int c = 0; c < 12; ++c) lam = (c == r) ? A[c] : lam;
-    __syncthreads();
-    if (r < 12) {
+    for (int k = 0; k < 4; ++k)
 #pragma unroll
-        for (int c = 0; c < 12; ++c) M[r * 12 + c] = V[c];
-        M[144 + r] = lam > 0 ? lam : 0.0;
-    }
-    __syncthreads();
-    if (!live || r >= 12) return;
-    const int p = r / 3, k = r % 3;
-    double* dst = hl + 9 * (off + 16l * h + 4 * p) + 3 * k;
+        for (int l = 0; l < 4; ++l)
 #pragma unroll
-    for (int c = 0; c < 12; ++c) {
-        double sum = 0;
+            for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int m = 0; m < 12; ++m) sum += V[m] * M[144 + m] * M[c * 12 + m];
-        dst[9 * (c / 3) + c % 3] = active ? sum : 0.0;
-    }
+                for (int j = 0; j < 3; ++j) {
+                    double sum = 0;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) {
+                            const double v = a9[u9(3 * a + i, 3 * b + j)];
+                            sum += (wsign(k, a) * wsign(l, b) > 0) ? v : -v;
+                        }
+                    dst[9 * (4 * k + l) + 3 * i + j] = 0.25 * sum;
+                }
 }
+
 // rod segment s block (a, b) at off + 4 s + 2 a + b; rod hinge h block (a, b) at
 // off2 + 9 h + 3 a + b
 __global__ void k_rod_hess_local(const double* __restrict__ x, const int* __restrict__ rv, int nR, const double* __restrict__ par,
@@ -498,8 +503,9 @@ void elastic_hessian(Ctx& c, const double* x) {
     }
     if (c.nH) {
         double kb = c.P.bendMult != 0 ? c.kBend : 0.0;
-        hipLaunchKernelGGL(k_bend_hess_local, dim3((c.nH + 15) / 16), dim3(256), 0, c.s, x, c.hinge.p, c.nH, c.hInfo.p, kb, h2, c.dbc.p,
-                           hl, ns, 9l * c.nT);
+        double* Hh = c.hingeH.ensure(144 * (size_t)c.nH);
+        hipLaunchKernelGGL(k_bend_hess_rows, dim3((c.nH + 15) / 16), dim3(256), 0, c.s, x, c.hinge.p, c.nH, c.hInfo.p, kb, h2, c.dbc.p, Hh);
+        hipLaunchKernelGGL(k_bend_hess_proj9, dim3(nblocks(c.nH, 64)), dim3(64), 0, c.s, (const double*)Hh, c.nH, hl, 9l * c.nT);
         CIPC_LAUNCH_CHECK();
     }
     if (c.nRod) {
