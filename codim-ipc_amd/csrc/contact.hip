@@ -736,31 +736,11 @@ __global__ void k_barrier_grad_local(const double* __restrict__ x, const double*
     }
 }
 
-// local Hessian blocks: SoA [9][16 nC]; block (a, b) of constraint i at slot 16 i + 4 a + b
-template <int N>
-__device__ __forceinline__ void psd_store(double* Hf, double* __restrict__ hl, long ns, long base, int nvv) {
-    double H[N * N];
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = 0; j < N; ++j) H[i * N + j] = Hf[i * 12 + j];
-    make_pd<N>(H);
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int l = 0; l < 3; ++l) {
-                    double v = (a < nvv && b < nvv) ? H[(a * 3 + k) * N + b * 3 + l] : 0.0;
-                    hl[(long)(k * 3 + l) * ns + base + 4 * a + b] = v;
-                }
-}
-
-__global__ void __launch_bounds__(64) k_barrier_hess_local(const double* __restrict__ x, const double* __restrict__ X0,
+// Stage 1 of the barrier Hessian: each constraint's local Hessian (12 x 12 slot layout,
+// leading 3 nv x 3 nv block used) into Hc[144 i], and nvv[i] = nv (0: not evaluated)
+__global__ void __launch_bounds__(64) k_barrier_hess_build(const double* __restrict__ x, const double* __restrict__ X0,
                                                            const int* __restrict__ C, int nC, double kappa, double xi2, double shat,
-                                                           double* __restrict__ hl) {
+                                                           double* __restrict__ Hc, int* __restrict__ nvv) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nC) return;
     int cc[4], vid[4], kind, nv;
@@ -768,12 +748,9 @@ __global__ void __launch_bounds__(64) k_barrier_hess_local(const double* __restr
     decode(C, nC, i, cc, kind, vid, nv, mult);
     double d2 = con_dist2(x, kind, vid);
     double s = d2 - xi2;
-    long ns = 16l * nC;
-    double H[144];
+    double* H = Hc + 144l * i;
     if (!(s > 0)) {
-#pragma unroll
-        for (int k = 0; k < 144; ++k) H[k] = 0;
-        psd_store<12>(H, hl, ns, 16l * i, 0);
+        nvv[i] = 0;
         return;
     }
     double gd[12], Hd[144];
@@ -789,16 +766,39 @@ __global__ void __launch_bounds__(64) k_barrier_hess_local(const double* __restr
 #pragma unroll
             for (int q = 0; q < 12; ++q)
                 H[r * 12 + q] = bg * gd[r] * ge[q] + bg * ge[r] * gd[q] + b * He[r * 12 + q] + m * bh * gd[r] * gd[q] + m * bg * Hd[r * 12 + q];
-        psd_store<12>(H, hl, ns, 16l * i, 4);
+        nvv[i] = 4;
     } else {
+        const int n3 = 3 * nv;
 #pragma unroll
         for (int r = 0; r < 12; ++r)
 #pragma unroll
-            for (int q = 0; q < 12; ++q) H[r * 12 + q] = mult * (bh * gd[r] * gd[q] + bg * Hd[r * 12 + q]);
-        if (nv == 4) psd_store<12>(H, hl, ns, 16l * i, 4);
-        else if (nv == 3) psd_store<9>(H, hl, ns, 16l * i, 3);
-        else psd_store<6>(H, hl, ns, 16l * i, 2);
+            for (int q = 0; q < 12; ++q)
+                if (r < n3 && q < n3) H[r * 12 + q] = mult * (bh * gd[r] * gd[q] + bg * Hd[r * 12 + q]);
+        nvv[i] = nv;
     }
+}
+
+// Stage 2: PSD projection of each local Hessian (translation-reduced Jacobi in registers,
+// one constraint per thread) written as SoA [9][16 nC] blocks; block (a, b) of
+// constraint i at slot 16 i + 4 a + b, zero outside the constraint's nv x nv blocks.
+__global__ void __launch_bounds__(64) k_barrier_hess_proj(const double* __restrict__ Hc, const int* __restrict__ nvv, int nC,
+                                                          double* __restrict__ hl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nC) return;
+    const long ns = 16l * nC, base = 16l * i;
+    const int nv = nvv[i];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (a >= nv || b >= nv)
+#pragma unroll
+                for (int e = 0; e < 9; ++e) hl[(long)e * ns + base + 4 * a + b] = 0.0;
+    auto store = [&](int r, int c, double v) { hl[(long)((r % 3) * 3 + c % 3) * ns + base + 4 * (r / 3) + c / 3] = v; };
+    const double* H = Hc + 144l * i;
+    if (nv == 4) make_pd_translation<4>(H, 12, store);
+    else if (nv == 3) make_pd_translation<3>(H, 12, store);
+    else if (nv == 2) make_pd_translation<2>(H, 12, store);
 }
 
 __global__ void k_con_dist2(const double* __restrict__ x, const int* __restrict__ C, int nC, double* __restrict__ d2,
@@ -961,7 +961,10 @@ void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa
     const double xi2 = c.P.xi * c.P.xi, shat = c.dHat2 + 2 * std::sqrt(c.dHat2) * c.P.xi;
     long ns = 16l * nC;
     double* hl = c.conLocal.ensure(9 * ns);
-    hipLaunchKernelGGL(k_barrier_hess_local, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, hl);
+    double* Hc = c.conH.ensure(144 * (size_t)nC);
+    int* nvv = c.conNv.ensure((size_t)nC);
+    hipLaunchKernelGGL(k_barrier_hess_build, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, Hc, nvv);
+    hipLaunchKernelGGL(k_barrier_hess_proj, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, (const double*)Hc, (const int*)nvv, nC, hl);
     CIPC_LAUNCH_CHECK();
     uint64_t* key = c.k64b.ensure(ns);
     uint32_t* slot = c.u32d.ensure(ns);

@@ -123,6 +123,8 @@ struct Ctx {
     // contact Hessian (rebuilt per Newton iteration)
     BSR Acon;
     DevBuf<double> conLocal;           // SoA [9][16 nC]
+    DevBuf<double> conH;               // local barrier Hessians before projection, [nC][12][12]
+    DevBuf<int> conNv;                 // vertices of each constraint's Hessian (0: not evaluated)
     DevBuf<int> conPlanSlot;           // sorted slot order
     DevBuf<double> conGLocal;          // SoA [3][4 nC]
     DevBuf<int> conGVert;              // vertex of each gradient slot (4 nC; -1 unused)

@@ -388,6 +388,151 @@ CIPC_DEV double ee_mollifier(d3 a0, d3 a1, d3 b0, d3 b1, double eps, double* g, 
     return e;
 }
 
+// ------------------------------------------------------------- translation-reduced PSD projection
+// For the Hessian of a translation-invariant function of NV points (hinge bending,
+// every barrier: distances and the edge-edge mollifier depend only on differences),
+// H t = 0 for every rigid translation t, so with an orthonormal basis Q = Wq (x) I3 of the
+// translation complement (Wq: NV x (NV-1), columns orthogonal to (1, ..., 1)),
+// makePD(H) = Q makePD(Q^T H Q) Q^T: the eigenproblem shrinks from 3 NV to 3 (NV - 1).
+// Wq: NV = 2: (1, -1)/sqrt2; NV = 3: (1, -1, 0)/sqrt2, (1, 1, -2)/sqrt6; NV = 4: the
+// +-1/2 patterns (+,-,+,-), (+,+,-,-), (+,-,-,+).
+template <int NV>
+CIPC_DEV constexpr double tr_basis(int k, int a) {
+    if (NV == 2) return k == 0 ? 0.70710678118654752440 : -0.70710678118654752440;
+    if (NV == 3) {
+        if (a == 0) return k == 0 ? 0.70710678118654752440 : (k == 1 ? -0.70710678118654752440 : 0.0);
+        return k == 2 ? -0.81649658092772603273 : 0.40824829046386301637;
+    }
+    const int sg = a == 0 ? ((k & 1) ? -1 : 1) : a == 1 ? ((k >> 1) ? -1 : 1) : (((k & 1) ^ (k >> 1)) ? -1 : 1);
+    return 0.5 * sg;
+}
+CIPC_DEV constexpr int upk(int n, int i, int j) { return i <= j ? i * n - i * (i - 1) / 2 + (j - i) : j * n - j * (j - 1) / 2 + (i - j); }
+
+// H (row stride `ld`, leading 3 NV x 3 NV block) -> PSD projection into out(r, c) via
+// store(r, c, value).  Positive-definite reduced matrices are returned unchanged; the rest
+// go through cyclic Jacobi on the reduced matrix (off^2 <= 1e-24 of ||.||_F^2).
+template <int NV, class Store>
+CIPC_DEV void make_pd_translation(const double* __restrict__ Hm, int ld, Store store) {
+    constexpr int R = 3 * (NV - 1), NU = R * (R + 1) / 2;
+    double a[NU];
+#pragma unroll
+    for (int e = 0; e < NU; ++e) a[e] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int l = 0; l < NV; ++l)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double v = Hm[(3 * k + i) * ld + 3 * l + j];
+#pragma unroll
+                    for (int p = 0; p < NV - 1; ++p)
+#pragma unroll
+                        for (int q = 0; q < NV - 1; ++q) {
+                            const int rr = 3 * p + i, cc = 3 * q + j;
+                            if (rr <= cc) a[upk(R, rr, cc)] += tr_basis<NV>(k, p) * tr_basis<NV>(l, q) * v;
+                        }
+                }
+    // positive definite: unchanged
+    bool pd = true;
+    {
+        double L[NU];
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j) {
+                double sum = a[upk(R, j, i)];
+#pragma unroll
+                for (int k = 0; k < j; ++k) sum -= L[upk(R, k, i)] * L[upk(R, k, j)];
+                if (i == j) {
+                    pd = pd && sum > 0;
+                    L[upk(R, i, i)] = sqrt(sum > 0 ? sum : 1.0);
+                } else {
+                    L[upk(R, j, i)] = sum / L[upk(R, j, j)];
+                }
+            }
+    }
+    if (pd) {
+#pragma unroll
+        for (int r = 0; r < 3 * NV; ++r)
+#pragma unroll
+            for (int c = 0; c < 3 * NV; ++c) store(r, c, Hm[r * ld + c]);
+        return;
+    }
+    double V[R * R];
+#pragma unroll
+    for (int e = 0; e < R * R; ++e) V[e] = (e % (R + 1) == 0) ? 1.0 : 0.0;
+    double fro = 0;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = i; j < R; ++j) fro += (i == j ? 1.0 : 2.0) * a[upk(R, i, j)] * a[upk(R, i, j)];
+    const double tol = 1e-24 * fro;
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        double offd = 0;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = i + 1; j < R; ++j) offd += a[upk(R, i, j)] * a[upk(R, i, j)];
+        if (offd <= 0.5 * tol) break;
+#pragma unroll
+        for (int p = 0; p < R - 1; ++p)
+#pragma unroll
+            for (int q = p + 1; q < R; ++q) {
+                const double apq = a[upk(R, p, q)];
+                if (apq == 0.0) continue;
+                const double app = a[upk(R, p, p)], aqq = a[upk(R, q, q)];
+                const double theta = (aqq - app) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                const double c = 1 / sqrt(t * t + 1), s = t * c;
+                a[upk(R, p, p)] = app - t * apq;
+                a[upk(R, q, q)] = aqq + t * apq;
+                a[upk(R, p, q)] = 0.0;
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    if (k == p || k == q) continue;
+                    const double akp = a[upk(R, k, p)], akq = a[upk(R, k, q)];
+                    a[upk(R, k, p)] = c * akp - s * akq;
+                    a[upk(R, k, q)] = s * akp + c * akq;
+                }
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const double vkp = V[R * k + p], vkq = V[R * k + q];
+                    V[R * k + p] = c * vkp - s * vkq;
+                    V[R * k + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    double lam[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) lam[i] = a[upk(R, i, i)] > 0 ? a[upk(R, i, i)] : 0.0;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = i; j < R; ++j) {
+            double sum = 0;
+#pragma unroll
+            for (int m = 0; m < R; ++m) sum += V[R * i + m] * lam[m] * V[R * j + m];
+            a[upk(R, i, j)] = sum;
+        }
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int l = 0; l < NV; ++l)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    double sum = 0;
+#pragma unroll
+                    for (int p = 0; p < NV - 1; ++p)
+#pragma unroll
+                        for (int q = 0; q < NV - 1; ++q) sum += tr_basis<NV>(k, p) * tr_basis<NV>(l, q) * a[upk(R, 3 * p + i, 3 * q + j)];
+                    store(3 * k + i, 3 * l + j, sum);
+                }
+}
+
 // ------------------------------------------------------------- PSD projection
 // makePD (Math/UTILS.h:10-27): leave a positive-definite matrix unchanged, else clamp
 // negative eigenvalues to zero.  A Cholesky attempt filters the definite case; the
