@@ -263,24 +263,18 @@ __global__ void __launch_bounds__(64) k_memb_hess_local(const double* __restrict
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nT) return;
     int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
-    double H[81];
+    double* dst = hl + 81l * t;   // block (p, q) at 9 (3 p + q), entry (k, l) at 3 k + l
     if (all_dbc3(dbc, a, b, c)) {
 #pragma unroll
-        for (int k = 0; k < 81; ++k) H[k] = 0;
-    } else {
-        membrane_hess(ld3(x, a), ld3(x, b), ld3(x, c), Binv[t], Binv[nT + t], Binv[2 * nT + t], vol[t], lam, mu, H);
-#pragma unroll
-        for (int k = 0; k < 81; ++k) H[k] *= h2;
-        make_pd<9>(H);
+        for (int k = 0; k < 81; ++k) dst[k] = 0;
+        return;
     }
+    double H[81];
+    membrane_hess(ld3(x, a), ld3(x, b), ld3(x, c), Binv[t], Binv[nT + t], Binv[2 * nT + t], vol[t], lam, mu, H);
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int l = 0; l < 3; ++l) hl[9 * (9l * t + 3 * p + q) + 3 * k + l] = H[(3 * p + k) * 9 + 3 * q + l];
+    for (int k = 0; k < 81; ++k) H[k] *= h2;
+    // the membrane energy depends on edge vectors only: translation-reduced projection
+    make_pd_translation<3>(H, 9, [&](int r, int cc, double v) { dst[9 * (3 * (r / 3) + cc / 3) + 3 * (r % 3) + cc % 3] = v; });
 }
 // Hinge Hessians in two kernels, so that the register-hungry row evaluation and the
 // eigensolver each run at their own occupancy (Math/UTILS.h:10-27 for the projection).
@@ -460,16 +454,11 @@ __global__ void __launch_bounds__(64) k_rod_bend_hess_local(const double* __rest
     if (h >= nRH) return;
     double H[81];
     rod_kappa2_hess(ld3(x, hv3[h]), ld3(x, hv3[nRH + h]), ld3(x, hv3[2 * nRH + h]), H);
-    make_pd<9>(H);
+    // |kappa|^2 depends on the two edge vectors only: translation-reduced projection (the
+    // positive weight commutes with it)
     const double w = hw[h];
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int l = 0; l < 3; ++l) hl[9 * (off + 9l * h + 3 * p + q) + 3 * k + l] = w * H[(3 * p + k) * 9 + 3 * q + l];
+    double* dst = hl + 9 * (off + 9l * h);
+    make_pd_translation<3>(H, 9, [&](int r, int c, double v) { dst[9 * (3 * (r / 3) + c / 3) + 3 * (r % 3) + c % 3] = w * v; });
 }
 __global__ void k_el_assemble(const double* __restrict__ hl, long ns, const int* __restrict__ off, const int* __restrict__ slot,
                               const int* __restrict__ rowOf, const int* __restrict__ col, int nnzb, const double* __restrict__ m,
