@@ -111,6 +111,7 @@ PROTOTYPES = {
     "cipc_get_setup": (ctypes.c_int, [P, D, I32, D, I32]),
     "cipc_get_codim_sizes": (ctypes.c_int, [P, I32]),
     "cipc_set_precond_reuse": (ctypes.c_int, [P, ctypes.c_int32]),
+    "cipc_set_warm_start": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_autotune": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
     "cipc_set_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),

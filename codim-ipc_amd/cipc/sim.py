@@ -160,6 +160,11 @@ class Simulator:
         vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64).ravel()
         self._check(self.lib.cipc_set_state(self.h, _d(xx), _d(vv)))
 
+    def set_warm_start(self, enable=True):
+        """Seed the Newton solves that follow a cut line search with (1 - alpha) p_prev
+        (cipc_set_warm_start; on by default)."""
+        self._check(self.lib.cipc_set_warm_start(self.h, 1 if enable else 0))
+
     def set_kappa(self, k):
         self._check(self.lib.cipc_set_kappa(self.h, float(k)))
 

@@ -481,6 +481,11 @@ int cipc_set_precond_reuse(cipc_ctx* ctx, int32_t every) {
     ctx->c.lpEvery = every;
     return CIPC_OK;
 }
+int cipc_set_warm_start(cipc_ctx* ctx, int32_t enable) {
+    if (!ctx) return CIPC_E_INVALID;
+    ctx->c.warmStart = enable != 0;
+    return CIPC_OK;
+}
 static int precond_common(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa) {
     Ctx& c = ctx->c;
     if (!c.lp || !c.lp->weightsReady) {

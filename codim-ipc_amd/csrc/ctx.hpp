@@ -154,6 +154,11 @@ struct Ctx {
     // is also cut off at 1.25x the iterations of the last block-Jacobi solve
     bool lpOffThisStep = false;
     int lastJacobiIters = 0;
+    // Newton solves after a line search that cut the step to alpha < 1 start from
+    // (1 - alpha) p_prev, the quadratic model's prediction of the next direction
+    bool warmStart = true;
+    double lastWarmRatio = 1.0;       // ||b - H x0|| / ||b|| of the last warm-started solve
+    DevBuf<double> warmR, warmX;
 
     // launch shapes and broad-phase cell size (set by the autotuner or the caller)
     int spmvGrid = 1024;               // workgroups of the PCG SpMV (multiple of 8)
@@ -207,7 +212,8 @@ double elastic_energy(Ctx& c, const double* x, double* memb, double* bend);
 double inertia_energy(Ctx& c, const double* x);
 void elastic_gradient(Ctx& c, const double* x, double* g);        // g = elastic + inertia (overwrites)
 void elastic_hessian(Ctx& c, const double* x);                    // fills Ael (incl. mass, DBC projection)
-int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres);
+// x0: optional initial guess (nullptr: zero)
+int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres, const double* x0 = nullptr);
 // start the autotuner: bit 1 PCG launch shapes, bit 2 broad-phase cell size
 void autotune_start(Ctx& c, int what);
 int advance_one_step(Ctx& c);

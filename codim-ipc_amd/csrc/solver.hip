@@ -850,7 +850,7 @@ static void tune_done(Ctx& c, long nnzb) {
     if (T.pcgPhase == 1 && c.precond == 1) tune_phase(c, 2);
 }
 
-int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
+static int pcg_solve_tuned(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
     Ctx::Tuner& T = c.tune;
     const long nnzb = (long)c.Ael.nnzb + c.Acon.nnzb;
     if (!T.pcg && T.enabled && T.retunes > 0 && std::fabs((double)(nnzb - T.tunedNnzb)) > 0.05 * (double)T.tunedNnzb) {
@@ -890,6 +890,44 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
         c.lpLanes = T.pcgCand[best][1];
         tune_done(c, nnzb);
     }
+    return rc;
+}
+
+__global__ void k_sq_partial(long n, const double* __restrict__ a, double* __restrict__ part) {
+    __shared__ double lds[kBlock / 64];
+    double v = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v += a[i] * a[i];
+    const double s = block_allsum<kBlock>(v, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+static double norm2(Ctx& c, const double* a, long n) {
+    const int nb = std::min(nblocks(n), 1024);
+    double* part = c.dscr2.ensure((size_t)nb + 1);
+    hipLaunchKernelGGL(k_sq_partial, dim3(nb), dim3(kBlock), 0, c.s, n, a, part);
+    CIPC_LAUNCH_CHECK();
+    sum_partials(part, nb, c.scal.p + 41, c.s);
+    CIPC_HIP(hipMemcpyAsync(c.hScal + 41, c.scal.p + 41, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    return c.hScal[41];
+}
+
+// With an initial guess x0: r0 = b - H x0 in FP64; when ||r0|| < ||b|| / 2 the solve runs
+// on H d = r0 to the tolerance that gives ||b - H (x0 + d)|| <= tol ||b||, and x = x0 + d;
+// otherwise x0 is ignored.  The stopping rule is the same as from a zero start.
+int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres, const double* x0) {
+    if (!x0) return pcg_solve_tuned(c, b, xsol, tol, maxIter, iters, relres);
+    const long n = 3l * c.nV;
+    if (c.Acon.nnzb == 0) c.Acon.rowptr.ensure(c.nV + 1);
+    double* r0 = c.warmR.ensure(n);
+    const double bn2 = norm2(c, b, n);
+    const double rn2 = residual_norm2(c, b, x0, r0);
+    c.lastWarmRatio = bn2 > 0 ? std::sqrt(rn2 / bn2) : 1.0;
+    if (!(rn2 < 0.25 * bn2)) return pcg_solve_tuned(c, b, xsol, tol, maxIter, iters, relres);
+    double rr = 0;
+    const int rc = pcg_solve_tuned(c, r0, xsol, tol * std::sqrt(bn2 / rn2), maxIter, iters, &rr);
+    hipLaunchKernelGGL(k_acc_f64, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, x0, xsol);
+    CIPC_LAUNCH_CHECK();
+    if (relres) *relres = rr * std::sqrt(rn2 / bn2);
     return rc;
 }
 

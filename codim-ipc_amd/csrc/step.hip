@@ -293,6 +293,10 @@ __global__ void k_rhs(int nV, double* __restrict__ g, const uint8_t* __restrict_
         rhs[3 * i + k] = -gv;
     }
 }
+__global__ void k_scale_to(long n, double s, const double* __restrict__ a, double* __restrict__ out) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = s * a[i];
+}
 __global__ void k_axpy_to(long n, const double* __restrict__ a, double s, const double* __restrict__ b, double* __restrict__ out) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = a[i] + s * b[i];
@@ -381,6 +385,8 @@ int advance_one_step(Ctx& c) {
     bool useGD = false;
     double L2Norm = 0;
     int PNIter = 0;
+    bool havePrev = false;     // c.p holds the last Newton direction, accepted at alphaPrev
+    double alphaPrev = 1.0;
     do {
         // gradient (INC_POTENTIAL.h:154-275)
         ta = now_ms();
@@ -401,7 +407,17 @@ int advance_one_step(Ctx& c) {
             ta = now_ms();
             int it = 0;
             double rr = 0;
-            int rc = pcg_solve(c, c.rhs.p, c.p.p, P.pcgTol, P.pcgMaxIter, &it, &rr);
+            // after a line search that stopped at alpha < 1, the quadratic model predicts
+            // the next direction (1 - alpha) p (the gradient shrank to (1 - alpha) g); it
+            // seeds the solve, whose stopping rule stays ||b - H p|| <= tol ||b||
+            const double* x0 = nullptr;
+            if (c.warmStart && havePrev && alphaPrev < 1.0) {
+                double* wx = c.warmX.ensure(n);
+                hipLaunchKernelGGL(k_scale_to, dim3(nblocks(n)), dim3(kBlock), 0, s, n, 1.0 - alphaPrev, c.p.p, wx);
+                CIPC_LAUNCH_CHECK();
+                x0 = wx;
+            }
+            int rc = pcg_solve(c, c.rhs.p, c.p.p, P.pcgTol, P.pcgMaxIter, &it, &rr, x0);
             st.pcg_iters += it;
             st.t_solve_ms += now_ms() - ta;
             if (rc == 1) { useGD = true; st.gd_fallbacks++; }
@@ -439,6 +455,8 @@ int advance_one_step(Ctx& c) {
             alpha /= 2.0;
         } while (E > Eprev);
         Eprev = E;
+        alphaPrev = 2.0 * alpha;   // the accepted step
+        havePrev = !useGD;
         // kappa adaptation (IMPLICIT_EULER.h:566-595)
         if (c.nCprev > 0) {
             double* cur = c.dscr2.ensure(c.nCprev);

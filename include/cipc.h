@@ -169,6 +169,12 @@ int cipc_set_preconditioner(cipc_ctx* ctx, int32_t mode, int32_t budget);
  * (>= 1; default 1: at every solve); the solves in between reuse its factor G with the
  * diagonal scaling S of their own Hessian */
 int cipc_set_precond_reuse(cipc_ctx* ctx, int32_t every);
+/* Initial guess of the Newton solves (no counterpart in the reference, whose direct solve
+ * needs none): enable != 0 (the default) starts a solve that follows a line search cut
+ * to alpha < 1 from (1 - alpha) p_prev, the quadratic model's next direction, when that
+ * guess leaves less than half of ||b|| as residual; the stopping rule
+ * ||b - H p|| <= pcg_tol ||b|| is unchanged. */
+int cipc_set_warm_start(cipc_ctx* ctx, int32_t enable);
 /* assemble H(x, constraint set) as cipc_linear_solve does, then: loss and the flat
  * parameter gradient (grad may be NULL) / one Adam step at learning rate lr */
 int cipc_precond_loss_grad(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t n_probes, uint64_t seed,
