@@ -261,11 +261,12 @@ __device__ __forceinline__ int ecell(int vox, int f, int dim) {
 // primitive voxel range = union of its nodes' ranges; count of enumeration cells
 // (0 for primitives covering more than kLargeCells cells, which go to the large list)
 __global__ void k_prim_vox(const int* __restrict__ verts, int n, int nv, const int* __restrict__ vlo, const int* __restrict__ vhi, int nV,
-                           VoxGrid G, int* __restrict__ plo, int* __restrict__ phi, uint32_t* __restrict__ cnt,
+                           VoxGrid G, int* __restrict__ plo, int* __restrict__ phi, int4* __restrict__ box, uint32_t* __restrict__ cnt,
                            uint8_t* __restrict__ lgFlag, int* __restrict__ lgList, uint32_t* __restrict__ lgCount) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint64_t c = 1;
+    int blo[3], bhi[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         int lo = 1 << 30, hi = -(1 << 30);
@@ -276,8 +277,13 @@ __global__ void k_prim_vox(const int* __restrict__ verts, int n, int nv, const i
         }
         plo[(long)d * n + i] = lo;
         phi[(long)d * n + i] = hi;
+        blo[d] = lo;
+        bhi[d] = hi;
         c *= (uint64_t)(ecell(hi, G.f, G.edim[d]) - ecell(lo, G.f, G.edim[d]) + 1);
     }
+    // the same box as one 32-byte record (lo xyz, hi x | hi yz) for the candidate loops
+    box[2l * i] = make_int4(blo[0], blo[1], blo[2], bhi[0]);
+    box[2l * i + 1] = make_int4(bhi[1], bhi[2], 0, 0);
     const bool large = c > kLargeCells;
     lgFlag[i] = large ? 1 : 0;
     cnt[i] = large ? 0u : (uint32_t)c;
@@ -311,16 +317,23 @@ __device__ __forceinline__ void atomic_min_pos(unsigned long long* a, double v) 
     atomicMin(a, (unsigned long long)__double_as_longlong(v));
 }
 
-// point vI against triangle t (IPC.h:2120-2164): exact voxel-range overlap, shared
-// vertex and Dirichlet filters, swept-box broad phase, additive CCD with the running
-// step `largest` as cut-off
-__device__ __forceinline__ void pt_ccd_candidate(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ tri,
-                                                 int nT, const uint8_t* __restrict__ dbc, const int* __restrict__ tlo,
-                                                 const int* __restrict__ thi, int vI, const int* nlo, const int* nhi, d3 P, d3 dP,
-                                                 int t, double xi, double& largest, uint32_t* __restrict__ zero) {
-    int l0 = tlo[t], l1 = tlo[nT + t], l2 = tlo[2 * nT + t];
-    int h0 = thi[t], h1 = thi[nT + t], h2 = thi[2 * nT + t];
-    if (nlo[0] > h0 || l0 > nhi[0] || nlo[1] > h1 || l1 > nhi[1] || nlo[2] > h2 || l2 > nhi[2]) return;
+// voxel box of primitive i from its 32-byte record: lo = (a.x, a.y, a.z), hi = (a.w, b.x, b.y)
+struct VBox {
+    int4 a, b;
+};
+__device__ __forceinline__ VBox ld_box(const int4* __restrict__ box, int i) { return {box[2l * i], box[2l * i + 1]}; }
+__device__ __forceinline__ bool box_overlap(const VBox& A, const VBox& B) {
+    return !(A.a.x > B.a.w || B.a.x > A.a.w || A.a.y > B.b.x || B.a.y > A.b.x || A.a.z > B.b.y || B.a.z > A.b.y);
+}
+
+// point vI against triangle t (IPC.h:2120-2164): exact voxel-range overlap (the
+// triangle's box record), shared vertex and Dirichlet filters, swept-box broad phase,
+// additive CCD with the running step `largest` as cut-off
+__device__ __forceinline__ void pt_ccd_candidate_box(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ tri,
+                                                     int nT, const uint8_t* __restrict__ dbc, const VBox& T, int vI, const int* nlo,
+                                                     const int* nhi, d3 P, d3 dP, int t, double xi, double& largest,
+                                                     uint32_t* __restrict__ zero) {
+    if (nlo[0] > T.a.w || T.a.x > nhi[0] || nlo[1] > T.b.x || T.a.y > nhi[1] || nlo[2] > T.b.y || T.a.z > nhi[2]) return;
     int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
     if (vI == a || vI == b || vI == c) return;
     if (dbc[vI] && dbc[a] && dbc[b] && dbc[c]) return;
@@ -333,14 +346,12 @@ __device__ __forceinline__ void pt_ccd_candidate(const double* __restrict__ x, c
         if (la == 0) atomicOr(zero, 1u);
     }
 }
-// edge pair eA < eB (IPC.h:2166-2242), edge A first in the additive CCD
-__device__ __forceinline__ void ee_ccd_candidate(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bEdge,
-                                                 int nBE, const uint8_t* __restrict__ dbc, const int* __restrict__ elo,
-                                                 const int* __restrict__ ehi, int eA, int eB, double xi, double& largest,
-                                                 uint32_t* __restrict__ zero) {
-    if (elo[eA] > ehi[eB] || elo[eB] > ehi[eA] || elo[nBE + eA] > ehi[nBE + eB] || elo[nBE + eB] > ehi[nBE + eA] ||
-        elo[2l * nBE + eA] > ehi[2l * nBE + eB] || elo[2l * nBE + eB] > ehi[2l * nBE + eA])
-        return;
+// edge pair eA < eB (IPC.h:2166-2242) with both box records loaded; edge A first in
+// the additive CCD
+__device__ __forceinline__ void ee_ccd_candidate_box(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bEdge,
+                                                     int nBE, const uint8_t* __restrict__ dbc, const VBox& A, const VBox& B, int eA, int eB,
+                                                     double xi, double& largest, uint32_t* __restrict__ zero) {
+    if (!box_overlap(A, B)) return;
     int a0 = bEdge[eA], a1 = bEdge[nBE + eA], b0 = bEdge[eB], b1 = bEdge[nBE + eB];
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
     if (dbc[a0] && dbc[a1] && dbc[b0] && dbc[b1]) return;
@@ -353,10 +364,9 @@ __device__ __forceinline__ void ee_ccd_candidate(const double* __restrict__ x, c
         if (la == 0) atomicOr(zero, 1u);
     }
 }
-
 __global__ void k_ccd_pt(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bNode, int nBN,
                          const int* __restrict__ tri, int nT, const uint8_t* __restrict__ dbc, const int* __restrict__ vlo,
-                         const int* __restrict__ vhi, int nV, const int* __restrict__ tlo, const int* __restrict__ thi, VoxGrid G,
+                         const int* __restrict__ vhi, int nV, const int4* __restrict__ tbox, VoxGrid G,
                          const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals,
                          const int* __restrict__ lgList, const uint32_t* __restrict__ lgCount, double alpha0, double xi,
                          unsigned long long* __restrict__ gmin, uint32_t* __restrict__ zero) {
@@ -378,21 +388,25 @@ __global__ void k_ccd_pt(const double* __restrict__ x, const double* __restrict_
                 uint32_t st = start[cell], en = end[cell];
                 for (uint32_t k = st; k < en; ++k) {
                     int t = (int)vals[k];
-                    if (ix != max(qlo[0], ecell(tlo[t], G.f, G.edim[0])) || iy != max(qlo[1], ecell(tlo[nT + t], G.f, G.edim[1])) ||
-                        iz != max(qlo[2], ecell(tlo[2 * nT + t], G.f, G.edim[2])))
+                    const VBox T = ld_box(tbox, t);
+                    if (ix != max(qlo[0], ecell(T.a.x, G.f, G.edim[0])) || iy != max(qlo[1], ecell(T.a.y, G.f, G.edim[1])) ||
+                        iz != max(qlo[2], ecell(T.a.z, G.f, G.edim[2])))
                         continue;
-                    pt_ccd_candidate(x, p, tri, nT, dbc, tlo, thi, vI, nlo, nhi, P, dP, t, xi, largest, zero);
+                    pt_ccd_candidate_box(x, p, tri, nT, dbc, T, vI, nlo, nhi, P, dP, t, xi, largest, zero);
                 }
             }
     const uint32_t nl = *lgCount;
-    for (uint32_t k = 0; k < nl; ++k) pt_ccd_candidate(x, p, tri, nT, dbc, tlo, thi, vI, nlo, nhi, P, dP, lgList[k], xi, largest, zero);
+    for (uint32_t k = 0; k < nl; ++k) {
+        const int t = lgList[k];
+        pt_ccd_candidate_box(x, p, tri, nT, dbc, ld_box(tbox, t), vI, nlo, nhi, P, dP, t, xi, largest, zero);
+    }
     if (largest < alpha0) atomic_min_pos(gmin, largest);
 }
 
 // pair coverage as in k_query_ee: grid-listed edges take grid-listed partners of
 // larger index plus every large edge; large edges take large partners of larger index
 __global__ void k_ccd_ee(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bEdge, int nBE,
-                         const uint8_t* __restrict__ dbc, const int* __restrict__ elo, const int* __restrict__ ehi, VoxGrid G,
+                         const uint8_t* __restrict__ dbc, const int4* __restrict__ ebox, VoxGrid G,
                          const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals,
                          const uint8_t* __restrict__ lgFlag, const int* __restrict__ lgList, const uint32_t* __restrict__ lgCount,
                          double alpha0, double xi, unsigned long long* __restrict__ gmin, uint32_t* __restrict__ zero) {
@@ -400,18 +414,15 @@ __global__ void k_ccd_ee(const double* __restrict__ x, const double* __restrict_
     if (eI >= nBE) return;
     double largest = alpha0;
     const uint32_t nl = *lgCount;
+    const VBox A = ld_box(ebox, eI);
     if (lgFlag[eI]) {
         for (uint32_t k = 0; k < nl; ++k) {
             int eJ = lgList[k];
-            if (eJ > eI) ee_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, eI, eJ, xi, largest, zero);
+            if (eJ > eI) ee_ccd_candidate_box(x, p, bEdge, nBE, dbc, A, ld_box(ebox, eJ), eI, eJ, xi, largest, zero);
         }
     } else {
-        int qlo[3], qhi[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            qlo[d] = ecell(elo[(long)d * nBE + eI], G.f, G.edim[d]);
-            qhi[d] = ecell(ehi[(long)d * nBE + eI], G.f, G.edim[d]);
-        }
+        const int qlo[3] = {ecell(A.a.x, G.f, G.edim[0]), ecell(A.a.y, G.f, G.edim[1]), ecell(A.a.z, G.f, G.edim[2])};
+        const int qhi[3] = {ecell(A.a.w, G.f, G.edim[0]), ecell(A.b.x, G.f, G.edim[1]), ecell(A.b.y, G.f, G.edim[2])};
         for (int iz = qlo[2]; iz <= qhi[2]; ++iz)
             for (int iy = qlo[1]; iy <= qhi[1]; ++iy)
                 for (int ix = qlo[0]; ix <= qhi[0]; ++ix) {
@@ -420,16 +431,18 @@ __global__ void k_ccd_ee(const double* __restrict__ x, const double* __restrict_
                     for (uint32_t k = st; k < en; ++k) {
                         int eJ = (int)vals[k];
                         if (eJ <= eI) continue;
-                        if (ix != max(qlo[0], ecell(elo[eJ], G.f, G.edim[0])) || iy != max(qlo[1], ecell(elo[nBE + eJ], G.f, G.edim[1])) ||
-                            iz != max(qlo[2], ecell(elo[2l * nBE + eJ], G.f, G.edim[2])))
+                        const VBox B = ld_box(ebox, eJ);
+                        if (ix != max(qlo[0], ecell(B.a.x, G.f, G.edim[0])) || iy != max(qlo[1], ecell(B.a.y, G.f, G.edim[1])) ||
+                            iz != max(qlo[2], ecell(B.a.z, G.f, G.edim[2])))
                             continue;
-                        ee_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, eI, eJ, xi, largest, zero);
+                        ee_ccd_candidate_box(x, p, bEdge, nBE, dbc, A, B, eI, eJ, xi, largest, zero);
                     }
                 }
         for (uint32_t k = 0; k < nl; ++k) {
             int eJ = lgList[k];
-            if (eJ > eI) ee_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, eI, eJ, xi, largest, zero);
-            else ee_ccd_candidate(x, p, bEdge, nBE, dbc, elo, ehi, eJ, eI, xi, largest, zero);
+            const VBox B = ld_box(ebox, eJ);
+            if (eJ > eI) ee_ccd_candidate_box(x, p, bEdge, nBE, dbc, A, B, eI, eJ, xi, largest, zero);
+            else ee_ccd_candidate_box(x, p, bEdge, nBE, dbc, B, A, eJ, eI, xi, largest, zero);
         }
     }
     if (largest < alpha0) atomic_min_pos(gmin, largest);
@@ -541,7 +554,7 @@ static int bits_for2(uint64_t n) {
 }
 
 static long build_vox_cells(Ctx& c, const int* verts, int n, int nv, const int* vlo, const int* vhi, const VoxGrid& G, DevBuf<int>& plo,
-                            DevBuf<int>& phi, DevBuf<uint64_t>& keys, DevBuf<uint32_t>& vals, DevBuf<uint32_t>& start,
+                            DevBuf<int>& phi, DevBuf<int>& box, DevBuf<uint64_t>& keys, DevBuf<uint32_t>& vals, DevBuf<uint32_t>& start,
                             DevBuf<uint32_t>& end, DevBuf<uint8_t>& lgFlag, DevBuf<int>& lgList, uint32_t* lgCount) {
     long nCells = (long)G.edim[0] * G.edim[1] * G.edim[2];
     start.ensure(nCells);
@@ -554,9 +567,10 @@ static long build_vox_cells(Ctx& c, const int* verts, int n, int nv, const int* 
     if (n == 0) return 0;
     plo.ensure(3l * n);
     phi.ensure(3l * n);
+    box.ensure(8l * n);
     uint32_t* cnt = c.u32a.ensure(n + 1);
     uint32_t* off = c.u32b.ensure(n + 1);
-    hipLaunchKernelGGL(k_prim_vox, dim3(nblocks(n)), dim3(kBlock), 0, c.s, verts, n, nv, vlo, vhi, c.nV, G, plo.p, phi.p, cnt, lgFlag.p,
+    hipLaunchKernelGGL(k_prim_vox, dim3(nblocks(n)), dim3(kBlock), 0, c.s, verts, n, nv, vlo, vhi, c.nV, G, plo.p, phi.p, (int4*)box.p, cnt, lgFlag.p,
                        lgList.p, lgCount);
     CIPC_LAUNCH_CHECK();
     scan_exclusive(cnt, off, n, c.counters.p + 9, c.pw, c.s);
@@ -645,19 +659,19 @@ double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, doub
     // point-triangle
     uint32_t* lgCountT = c.counters.p + 18;
     uint32_t* lgCountE = c.counters.p + 19;
-    build_vox_cells(c, c.tri.p, c.nT, 3, vlo, vhi, G, c.ccdTlo, c.ccdThi, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT,
+    build_vox_cells(c, c.tri.p, c.nT, 3, vlo, vhi, G, c.ccdTlo, c.ccdThi, c.ccdTbox, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT,
                     lgCountT);
     if (c.nBN && c.nT) {
         hipLaunchKernelGGL(k_ccd_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, p, c.bNode.p, c.nBN, c.tri.p, c.nT, c.dbc.p, vlo, vhi, c.nV,
-                           c.ccdTlo.p, c.ccdThi.p, G, c.cellStart.p, c.cellEnd.p, c.u32c.p, c.lgListT.p, lgCountT, alpha, xi, gmin,
+                           (const int4*)c.ccdTbox.p, G, c.cellStart.p, c.cellEnd.p, c.u32c.p, c.lgListT.p, lgCountT, alpha, xi, gmin,
                            c.counters.p + 13);
         CIPC_LAUNCH_CHECK();
     }
     // edge-edge
-    build_vox_cells(c, c.bEdge.p, c.nBE, 2, vlo, vhi, G, c.ccdElo, c.ccdEhi, c.k64b, c.u32d, c.ecellStart, c.ecellEnd, c.lgFlagE,
+    build_vox_cells(c, c.bEdge.p, c.nBE, 2, vlo, vhi, G, c.ccdElo, c.ccdEhi, c.ccdEbox, c.k64b, c.u32d, c.ecellStart, c.ecellEnd, c.lgFlagE,
                     c.lgListE, lgCountE);
     if (c.nBE) {
-        hipLaunchKernelGGL(k_ccd_ee, dim3(nblocks(c.nBE)), dim3(kBlock), 0, c.s, x, p, c.bEdge.p, c.nBE, c.dbc.p, c.ccdElo.p, c.ccdEhi.p, G,
+        hipLaunchKernelGGL(k_ccd_ee, dim3(nblocks(c.nBE)), dim3(kBlock), 0, c.s, x, p, c.bEdge.p, c.nBE, c.dbc.p, (const int4*)c.ccdEbox.p, G,
                            c.ecellStart.p, c.ecellEnd.p, c.u32d.p, c.lgFlagE.p, c.lgListE.p, lgCountE, alpha, xi, gmin,
                            c.counters.p + 13);
         CIPC_LAUNCH_CHECK();
@@ -665,7 +679,7 @@ double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, doub
     // particles against rod segments and particle pairs
     if (c.nPart) {
         uint32_t* lgCountP = c.counters.p + 21;
-        build_vox_cells(c, c.partSlot.p, c.nPart, 1, vlo, vhi, G, c.ccdPlo, c.ccdPhi, c.pKeys, c.pVals, c.pcellStart, c.pcellEnd,
+        build_vox_cells(c, c.partSlot.p, c.nPart, 1, vlo, vhi, G, c.ccdPlo, c.ccdPhi, c.ccdPbox, c.pKeys, c.pVals, c.pcellStart, c.pcellEnd,
                         c.lgFlagP, c.lgListP, lgCountP);
         hipLaunchKernelGGL(k_ccd_codim, dim3(nblocks(c.nPart)), dim3(kBlock), 0, c.s, x, p, c.bNode.p, c.nBN, c.codimBN1, c.bEdge.p, c.nBE,
                            c.nRod, c.dbc.p, vlo, vhi, c.nV, c.ccdElo.p, c.ccdEhi.p, G, c.ecellStart.p, c.ecellEnd.p, c.u32d.p,

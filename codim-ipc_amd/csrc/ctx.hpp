@@ -77,6 +77,7 @@ struct Ctx {
     DevBuf<uint8_t> lgFlagP;
     DevBuf<int> lgListP;
     DevBuf<int> ccdPlo, ccdPhi;
+    DevBuf<int> ccdTbox, ccdEbox, ccdPbox;   // the same voxel boxes as 32-byte records (lo xyz, hi xyz, 2 pad)
 
     // device state
     DevBuf<double> x, xn, xt, v, xprev, X0, mass, p, g, rhs;

@@ -809,8 +809,8 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
     return rc;
 }
 
-// Autotuner of the PCG launch shapes.  Phase 1 (SpMV grid x lanes per row x blocks per
-// lane pass): the Newton solves switch the SpMV shape every chunk of 16 iterations, in
+// Autotuner of the PCG launch shapes.  Phase 1 (SpMV grid x {16 lanes x 1 block, 8 x 1,
+// 8 x 2 blocks per lane pass}; 16 x 2 was the slowest shape in every measurement): the Newton solves switch the SpMV shape every chunk of 16 iterations, in
 // turn, and time it on sampled launches, so every shape is timed on the same matrices;
 // once each has kTuneSamples samples the fastest mean is kept.  Phase 2 (learned
 // preconditioner only: factor-product grid x lanes per row): each Newton solve uses the
@@ -820,7 +820,7 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
 // When the number of stored Hessian blocks later moves more than 5% from the tuned
 // matrix's (contacts forming or releasing), the tuning reruns, at most kRetunes times.
 constexpr int kRetunes = 2;
-constexpr long kTuneSamples = 24;
+constexpr long kTuneSamples = 32;
 
 static void tune_phase(Ctx& c, int phase) {
     Ctx::Tuner& T = c.tune;
@@ -828,7 +828,7 @@ static void tune_phase(Ctx& c, int phase) {
     if (phase == 1) {
         T.spmvCand.clear();
         for (int sg : {1024, 2048})
-            for (auto ls : {std::array<int, 2>{16, 1}, {8, 1}, {8, 2}, {16, 2}}) T.spmvCand.push_back({sg, ls[0], ls[1]});
+            for (auto ls : {std::array<int, 2>{16, 1}, {8, 1}, {8, 2}}) T.spmvCand.push_back({sg, ls[0], ls[1]});
         T.spmvMs.assign(T.spmvCand.size(), 0.0);
         T.spmvN.assign(T.spmvCand.size(), 0);
         T.spmvTurn = 0;

@@ -198,8 +198,8 @@ int cipc_precond_info(const cipc_ctx* ctx, int32_t* out);
 
 /* ---- launch shapes and autotuning (no counterpart in the reference) -------
  * cipc_autotune(what): bit 1 -- the next Newton solves cycle twice through PCG launch
- * shapes and keep the fastest: first the SpMV shapes (grid 1024/2048 x 16 or 8 lanes per
- * block row x 1 or 2 blocks per lane pass), timed by the SpMV's sampled dispatch time,
+ * shapes and keep the fastest: first the SpMV shapes (grid 1024/2048 x {16 lanes per
+ * block row x 1 block per lane pass, 8 x 1, 8 x 2}), timed by the SpMV's sampled dispatch time,
  * then, with the learned preconditioner, the factor-product shapes (grid 1024/2048 x
  * 2/4/8 lanes per row), timed per PCG iteration; the tuning reruns (at most twice) when
  * the number of stored Hessian blocks moves more than 5% from the tuned matrix's;
