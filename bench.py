@@ -202,7 +202,9 @@ def main():
     # other (the first 26 steps are the free fall, run untimed)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=26)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"],
+                    help="c2 (default): 320k-triangle stack; c1: 20k-triangle stack; c3: the c1 stack with "
+                         "200,000 rod segments and 100,000 particles; c5: batch of independent scenes")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes instead of tuning them during the warm-up")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -225,7 +227,7 @@ def main():
         print(json.dumps(batch_run(args)))
         return
     from cipc import scenes
-    sd = scenes.scene_c2() if args.workload == "c2" else scenes.scene_c1()
+    sd = {"c1": scenes.scene_c1, "c2": scenes.scene_c2, "c3": scenes.scene_c3}[args.workload]()
 
     modes = ["fp64", "mixed", "fp32"] if args.precision == "all" else [args.precision]
     if args.precision == "all":
@@ -255,6 +257,7 @@ def main():
         "data": "synthetic",
         "mode": MODE[head["precision"]],
         "config": {"workload": sd.name, "vertices": sd.nV, "triangles": sd.nT, "cloth_vertices": sd.n_cloth,
+                   "rod_segments": 0 if sd.rods is None else len(sd.rods), "particles": 0 if sd.particles is None else len(sd.particles),
                    "steps_simulated_per_form": args.warmup + args.steps, "dt": sd.params["dt"], "dhat": sd.params["dHat"],
                    "pcg_tol": args.pcg_tol},
         "roofline": head["roofline"],
