@@ -533,93 +533,6 @@ CIPC_DEV void make_pd_translation(const double* __restrict__ Hm, int ld, Store s
                 }
 }
 
-// ------------------------------------------------------------- PSD projection
-// makePD (Math/UTILS.h:10-27): leave a positive-definite matrix unchanged, else clamp
-// negative eigenvalues to zero.  A Cholesky attempt filters the definite case; the
-// rest goes through cyclic Jacobi (eigenvectors accumulated) and V diag(max(l,0)) V^T.
-template <int N>
-CIPC_DEV bool chol_ok(const double* A) {
-    double L[N * (N + 1) / 2];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-#pragma unroll
-        for (int j = 0; j <= i; ++j) {
-            double s = A[i * N + j];
-#pragma unroll
-            for (int k = 0; k < j; ++k) s -= L[i * (i + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
-            if (i == j) {
-                if (!(s > 0)) return false;
-                L[i * (i + 1) / 2 + i] = sqrt(s);
-            } else {
-                L[i * (i + 1) / 2 + j] = s / L[j * (j + 1) / 2 + j];
-            }
-        }
-    }
-    return true;
-}
-
-template <int N>
-CIPC_DEV void make_pd(double* A) {
-    if (chol_ok<N>(A)) return;
-    double V[N * N];
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = 0; j < N; ++j) V[i * N + j] = (i == j) ? 1.0 : 0.0;
-    double fro = 0;
-#pragma unroll
-    for (int i = 0; i < N * N; ++i) fro += A[i] * A[i];
-    const double tol = 1e-30 * fro;
-    for (int sweep = 0; sweep < 16; ++sweep) {
-        double off = 0;
-#pragma unroll
-        for (int p = 0; p < N; ++p)
-#pragma unroll
-            for (int q = p + 1; q < N; ++q) off += A[p * N + q] * A[p * N + q];
-        if (off <= tol) break;
-#pragma unroll
-        for (int p = 0; p < N; ++p)
-#pragma unroll
-            for (int q = p + 1; q < N; ++q) {
-                double apq = A[p * N + q];
-                if (apq == 0.0) continue;
-                double th = (A[q * N + q] - A[p * N + p]) / (2.0 * apq);
-                double t = 1.0 / (fabs(th) + sqrt(th * th + 1.0));
-                if (th < 0) t = -t;
-                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-                A[p * N + p] -= t * apq;
-                A[q * N + q] += t * apq;
-                A[p * N + q] = A[q * N + p] = 0.0;
-#pragma unroll
-                for (int r = 0; r < N; ++r) {
-                    if (r == p || r == q) continue;
-                    double arp = A[r * N + p], arq = A[r * N + q];
-                    double np = c * arp - s * arq, nq = s * arp + c * arq;
-                    A[r * N + p] = A[p * N + r] = np;
-                    A[r * N + q] = A[q * N + r] = nq;
-                }
-#pragma unroll
-                for (int r = 0; r < N; ++r) {
-                    double vrp = V[r * N + p], vrq = V[r * N + q];
-                    V[r * N + p] = c * vrp - s * vrq;
-                    V[r * N + q] = s * vrp + c * vrq;
-                }
-            }
-    }
-    double w[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) w[i] = A[i * N + i] > 0 ? A[i * N + i] : 0.0;
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = i; j < N; ++j) {
-            double s = 0;
-#pragma unroll
-            for (int k = 0; k < N; ++k) s += V[i * N + k] * w[k] * V[j * N + k];
-            A[i * N + j] = s;
-            A[j * N + i] = s;
-        }
-}
 
 // ------------------------------------------------------------- membrane
 // FEM/Shell/MEMBRANE.h (useNH): E = vol (mu/2 (tr(B A) - 2 - 2 lnJ) + lam/2 lnJ^2),
