@@ -73,6 +73,37 @@ def test_ccd_point_triangle_closed_form():
             assert a >= 0.5 * toi
 
 
+@pytest.mark.parametrize("layout", ["crossing", "parallel"])
+def test_ccd_edge_edge_closed_form(layout):
+    """A moving edge approaching a static edge at speed v from distance d0 (offset 0):
+    exact time of impact d0 / v.  "crossing": perpendicular edges whose closest points are
+    both interior (edge-edge, every vertex at least 1 m from the other triangle);
+    "parallel": the moving edge lies above the static one and parallel to it (the
+    near-parallel fallback of the edge-edge distance and the point-edge case)."""
+    d0 = 0.01
+    A = [[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, -1.0, 0.0]]
+    if layout == "crossing":
+        B = [[0.0, d0, -1.0], [0.0, d0, 1.0], [0.0, d0 + 1.0, 0.0]]
+    else:
+        B = [[-0.5, d0, 0.0], [0.5, d0, 0.0], [0.0, d0 + 1.0, 1.0]]
+    X = np.array(A + B)
+    T = np.array([[0, 1, 2], [3, 4, 5]], dtype=np.int32)
+    fixed = np.array([0, 1, 2], dtype=np.int32)
+    sd = scenes.SceneData(X, T, fixed, X[fixed].copy(), dict(scenes.DEFAULT_PARAMS, PNTol=1e-3, xi=0.0))
+    orc = O.OracleScene(sd)
+    x = X.ravel().copy()
+    for v in (0.005, 0.02, 0.2):
+        p = np.zeros_like(x).reshape(-1, 3)
+        p[3:, 1] = -v
+        a, cl = orc.ccd(x, p.ravel(), 1.0)
+        toi = d0 / v
+        assert a <= min(cl, toi)
+        if toi < cl:
+            assert a >= 0.85 * toi
+        else:
+            assert a >= 0.85 * cl
+
+
 def seg_tri_intersect(e0, e1, t0, t1, t2):
     M = np.column_stack([t1 - t0, t2 - t0, e0 - e1])
     det = np.linalg.det(M)
