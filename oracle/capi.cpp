@@ -187,6 +187,49 @@ void orc_gradient(void* h, const double* x, const int* c, int nC, double kappa, 
     if (gel) { std::memset(gel, 0, sizeof(double) * 3 * S.nV); elastic_grad(S, x, gel); }
     if (gbar) { std::memset(gbar, 0, sizeof(double) * 3 * S.nV); barrier_grad(S, x, toC(c, nC), kappa, gbar); }
 }
+// ---------------------------------------------------------------- friction
+void orc_set_friction(void* h, double mu, double epsv2, int iterAmt) {
+    Params& P = ((Scene*)h)->P;
+    P.fricMu = mu; P.epsv2 = epsv2; P.fricIterAmt = iterAmt;
+}
+static FricSet toF(const int* c, int nF, const double* info) {
+    FricSet F;
+    F.C = toC(c, nF);
+    F.info.assign(info, info + 9 * nF);
+    return F;
+}
+// friction stencils of constraint set c at x: returns count, writes up to cap entries
+// (4 ints and 9 doubles each)
+int orc_friction_basis(void* h, const double* x, const int* c, int nC, double kappa, int* outC, double* outInfo, int cap) {
+    FricSet F;
+    friction_basis(*(Scene*)h, x, toC(c, nC), kappa, F);
+    int n = (int)F.C.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        std::memcpy(outC + 4 * i, F.C[i].c, 4 * sizeof(int));
+        std::memcpy(outInfo + 9 * i, &F.info[9 * i], 9 * sizeof(double));
+    }
+    return n;
+}
+double orc_friction_energy(void* h, const double* x, const double* xn, const int* c, int nF, const double* info) {
+    return friction_energy(*(Scene*)h, x, xn, toF(c, nF, info));
+}
+void orc_friction_grad(void* h, const double* x, const double* xn, const int* c, int nF, const double* info, double* g) {
+    Scene& S = *(Scene*)h;
+    std::memset(g, 0, sizeof(double) * 3 * S.nV);
+    friction_grad(S, x, xn, toF(c, nF, info), g);
+}
+// local blocks (row vertex, col vertex, 9 doubles); returns the count (<= 16 nF)
+int orc_friction_hess(void* h, const double* x, const double* xn, const int* c, int nF, const double* info, int* rows,
+                      int* cols, double* vals) {
+    std::vector<Block> B;
+    friction_hess(*(Scene*)h, x, xn, toF(c, nF, info), B);
+    for (size_t i = 0; i < B.size(); ++i) {
+        rows[i] = B[i].r; cols[i] = B[i].c;
+        std::memcpy(vals + 9 * i, B[i].b, 9 * sizeof(double));
+    }
+    return (int)B.size();
+}
+
 // assembled Hessian (elastic + barrier + M, DBC projected) in CSR; call with rp only to get nnz
 static CSR g_lastA;
 int orc_hessian(void* h, const double* x, const int* c, int nC, double kappa, int which) {

@@ -103,6 +103,9 @@ struct Params {
     double rodRho = 0, rodE = 0, rodBendMult = 1, rodThick = 1e-3;
     double partRho = 0, partThick = 1e-3;
     int collision = 1;
+    // lagged friction (Advance_One_Step_IE_Hinge's mu, epsv2, fricIterAmt)
+    double fricMu = 0, epsv2 = 1e-6;
+    int fricIterAmt = 1;
     int solver = 0;            // 0 = direct (sparse Cholesky), 1 = block-Jacobi PCG
     double pcgTol = 1e-5;
     int pcgMaxIter = 100000;
@@ -159,6 +162,16 @@ void barrier_grad(const Scene& S, const double* x, const std::vector<Constraint>
 void barrier_hess(const Scene& S, const double* x, const std::vector<Constraint>& C, double kappa,
                   std::vector<Block>& out);
 void constraint_dist2(const Scene& S, const double* x, const std::vector<Constraint>& C, std::vector<double>& d2);
+// lagged friction (FEM/FRICTION.h): stencils, and per stencil 9 doubles: closest-point
+// coordinates (2), tangent basis columns (2 x 3), normal force
+struct FricSet {
+    std::vector<Constraint> C;
+    std::vector<double> info;
+};
+void friction_basis(const Scene& S, const double* x, const std::vector<Constraint>& C, double kappa, FricSet& F);
+double friction_energy(const Scene& S, const double* x, const double* xn, const FricSet& F);
+void friction_grad(const Scene& S, const double* x, const double* xn, const FricSet& F, double* g);
+void friction_hess(const Scene& S, const double* x, const double* xn, const FricSet& F, std::vector<Block>& out);
 double ccd_stepsize(const Scene& S, const double* x, const double* p, double alpha, bool bruteForce = false,
                     double* clampedAlpha = nullptr);
 // elastic + inertia

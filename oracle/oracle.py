@@ -49,6 +49,12 @@ def lib():
         L.orc_barrier_blocks.argtypes = [ctypes.c_void_p, D, I, ctypes.c_int, ctypes.c_double, I, D, ctypes.c_int]
         L.orc_membrane.argtypes = [D, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D, D, D]
         L.orc_solve.argtypes = [ctypes.c_int, I, I, D, D, D, ctypes.c_int, ctypes.c_double, I]
+        L.orc_set_friction.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_friction_basis.argtypes = [ctypes.c_void_p, D, I, ctypes.c_int, ctypes.c_double, I, D, ctypes.c_int]
+        L.orc_friction_energy.restype = ctypes.c_double
+        L.orc_friction_energy.argtypes = [ctypes.c_void_p, D, D, I, ctypes.c_int, D]
+        L.orc_friction_grad.argtypes = [ctypes.c_void_p, D, D, I, ctypes.c_int, D, D]
+        L.orc_friction_hess.argtypes = [ctypes.c_void_p, D, D, I, ctypes.c_int, D, I, I, D]
         _LIB = L
     return _LIB
 
@@ -193,6 +199,7 @@ class OracleScene:
         L.orc_codim_sizes(self.h, _ip(s))
         self.nRod, self.nRodHinge, self.nParticle, self.codimBN0, self.codimBN1 = [int(v) for v in s]
         self.nV = sd.nV
+        L.orc_set_friction(self.h, float(P.get("mu", 0.0)), float(P.get("epsv2", 1e-6)), int(P.get("fricIterAmt", 1)))
 
     def __del__(self):
         try:
@@ -292,6 +299,47 @@ class OracleScene:
         blk = np.zeros(9 * cap)
         n = lib().orc_barrier_blocks(self.h, _dp(x), _ip(C), len(C), kappa, _ip(rc), _dp(blk), cap)
         return rc[:2 * n].reshape(-1, 2), blk[:9 * n].reshape(-1, 3, 3)
+
+    # ---- lagged friction (FEM/FRICTION.h)
+    def friction_basis(self, x, C, kappa):
+        """Compute_Friction_Basis: (stencils int (n, 4), info (n, 9): closest point (2), tangent
+        basis columns (2 x 3), normal force)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        C = np.ascontiguousarray(C, dtype=np.int32).reshape(-1, 4)
+        cap = max(len(C), 1)
+        oc = np.zeros(4 * cap, dtype=np.int32)
+        oi = np.zeros(9 * cap)
+        n = lib().orc_friction_basis(self.h, _dp(x), _ip(C), len(C), kappa, _ip(oc), _dp(oi), cap)
+        return oc[:4 * n].reshape(-1, 4), oi[:9 * n].reshape(-1, 9)
+
+    @staticmethod
+    def _f(x, xn, FC, FI):
+        return (np.ascontiguousarray(x, dtype=np.float64).ravel(), np.ascontiguousarray(xn, dtype=np.float64).ravel(),
+                np.ascontiguousarray(FC, dtype=np.int32).reshape(-1, 4), np.ascontiguousarray(FI, dtype=np.float64).reshape(-1, 9))
+
+    def friction_energy(self, x, xn, FC, FI):
+        x, xn, FC, FI = self._f(x, xn, FC, FI)
+        return lib().orc_friction_energy(self.h, _dp(x), _dp(xn), _ip(FC), len(FC), _dp(FI))
+
+    def friction_gradient(self, x, xn, FC, FI):
+        x, xn, FC, FI = self._f(x, xn, FC, FI)
+        g = np.zeros(3 * self.nV)
+        lib().orc_friction_grad(self.h, _dp(x), _dp(xn), _ip(FC), len(FC), _dp(FI), _dp(g))
+        return g
+
+    def friction_hessian(self, x, xn, FC, FI):
+        """Friction Hessian as a scipy CSR matrix (local blocks summed; no mass, no Dirichlet projection)."""
+        import scipy.sparse as sp
+        x, xn, FC, FI = self._f(x, xn, FC, FI)
+        cap = 16 * len(FC) + 1
+        r = np.zeros(cap, dtype=np.int32)
+        c = np.zeros(cap, dtype=np.int32)
+        v = np.zeros(9 * cap)
+        nb = lib().orc_friction_hess(self.h, _dp(x), _dp(xn), _ip(FC), len(FC), _dp(FI), _ip(r), _ip(c), _dp(v))
+        rows = (3 * r[:nb, None] + np.array([0, 0, 0, 1, 1, 1, 2, 2, 2])[None, :]).ravel()
+        cols = (3 * c[:nb, None] + np.array([0, 1, 2, 0, 1, 2, 0, 1, 2])[None, :]).ravel()
+        n = 3 * self.nV
+        return sp.csr_matrix((v[:9 * nb], (rows, cols)), shape=(n, n))
 
     def dist2_list(self, x, C):
         x = np.ascontiguousarray(x, dtype=np.float64).ravel()

@@ -9,8 +9,10 @@
 //
 // advance_one_step restates Advance_One_Step_IE_Discrete_Shell<double,3,false,false,
 // false> (FEM/Shell/IMPLICIT_EULER.h:151-902) for the configuration this build
-// covers: no strain limiting, no fibres, no tets, no friction, no stitches, static
-// or CCD-unblocked Dirichlet nodes, and Line_Search (IMPLICIT_EULER.h:9-149).
+// covers: no strain limiting, no fibres, no tets, no stitches, static or
+// CCD-unblocked Dirichlet nodes, lagged friction (basis at the step start, re-lagged
+// after convergence up to fricIterAmt times, IMPLICIT_EULER.h:412-437, 770-859), and
+// Line_Search (IMPLICIT_EULER.h:9-149).
 // Parity pins: energy non-increase per accepted step, intersection-free states,
 // stopping rule; trajectories are the reference for the HIP path.
 #include "oracle.h"
@@ -118,11 +120,13 @@ void scene_init(Scene& S) {
     S.kappaMax = 100 * S.kappa;
 }
 
-// full incremental potential at x for constraint set C
+// full incremental potential at x for constraint set C (+ friction with the lagged set F)
 static double inc_potential(const Scene& S, const double* x, const double* xt, const std::vector<Constraint>& C,
-                            double kappa, bool withContact, int* bad) {
+                            double kappa, bool withContact, int* bad, const double* xn = nullptr,
+                            const FricSet* F = nullptr) {
     double E = elastic_energy(S, x) + inertia_energy(S, x, xt);
     if (withContact) E += barrier_energy(S, x, C, kappa, bad);
+    if (F) E += friction_energy(S, x, xn, *F);
     return E;
 }
 
@@ -155,9 +159,15 @@ int advance_one_step(Scene& S, StepStats& st) {
     std::vector<Constraint> C, Cprev;
     std::vector<double> d2prev;
     if (P.collision) constraint_set(S, x.data(), C);
+    // lagged friction set at the step start, IMPLICIT_EULER.h:419-437
+    const bool fric = P.collision && P.fricMu > 0;
+    FricSet F;
+    const FricSet* Fp = fric ? &F : nullptr;
+    if (fric) friction_basis(S, x.data(), C, S.kappa, F);
     int bad = 0;
-    double Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad);
+    double Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp);
     if (bad) return -1;
+    int fricIterI = 0;
     std::deque<double> resRecord;
     bool useGD = false;
     double L2Norm = 0;
@@ -172,6 +182,7 @@ int advance_one_step(Scene& S, StepStats& st) {
         for (int i = 0; i < S.nV; ++i)
             for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
         if (P.collision) barrier_grad(S, x.data(), C, S.kappa, g.data());
+        if (fric) friction_grad(S, x.data(), xn.data(), F, g.data());
         for (int v : S.dbcIdx) for (int k = 0; k < 3; ++k) g[3 * v + k] = 0;
         for (int i = 0; i < n; ++i) rhs[i] = -g[i];
         st.tGrad += now() - ta;
@@ -182,6 +193,7 @@ int advance_one_step(Scene& S, StepStats& st) {
             std::vector<Block> blocks;
             elastic_hess(S, x.data(), blocks);
             if (P.collision) barrier_hess(S, x.data(), C, S.kappa, blocks);
+            if (fric) friction_hess(S, x.data(), xn.data(), F, blocks);
             assemble(S, blocks, A);
             st.tHessian += now() - ta;
             ta = now();
@@ -220,7 +232,7 @@ int advance_one_step(Scene& S, StepStats& st) {
                 }
             }
             ta = now();
-            E = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad);
+            E = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp);
             st.tEnergy += now() - ta;
             if (bad) return -1;
             alpha /= 2.0;
@@ -235,7 +247,7 @@ int advance_one_step(Scene& S, StepStats& st) {
                 if (d2prev[i] < 1e-18 && cur[i] < d2prev[i]) { upd = true; break; }
             if (upd && S.kappa < S.kappaMax) {
                 S.kappa *= 2;
-                Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad);
+                Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp);
             }
         }
         std::vector<double> d2;
@@ -261,6 +273,35 @@ int advance_one_step(Scene& S, StepStats& st) {
         }
         st.lastRes = cur;
         if (PNIter > 2000) return -4;
+        // friction re-lagged at a converged iterate, IMPLICIT_EULER.h:770-859: new basis and
+        // normal forces, then the Newton direction of the updated potential decides
+        if (fric && resRecord.size() == 3 && L2Norm <= P.PNTol) {
+            ++fricIterI;
+            if (fricIterI < P.fricIterAmt || P.fricIterAmt <= 0) {
+                friction_basis(S, x.data(), C, S.kappa, F);
+                std::fill(g.begin(), g.end(), 0.0);
+                elastic_grad(S, x.data(), g.data());
+                for (int i = 0; i < S.nV; ++i)
+                    for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
+                barrier_grad(S, x.data(), C, S.kappa, g.data());
+                friction_grad(S, x.data(), xn.data(), F, g.data());
+                for (int v : S.dbcIdx) for (int k = 0; k < 3; ++k) g[3 * v + k] = 0;
+                for (int i = 0; i < n; ++i) rhs[i] = -g[i];
+                std::vector<Block> blocks;
+                elastic_hess(S, x.data(), blocks);
+                barrier_hess(S, x.data(), C, S.kappa, blocks);
+                friction_hess(S, x.data(), xn.data(), F, blocks);
+                assemble(S, blocks, A);
+                int rc;
+                if (P.solver == 0) rc = solve_direct(A, rhs.data(), sol.data());
+                else { int it = 0; rc = solve_pcg(A, rhs.data(), sol.data(), P.pcgTol, P.pcgMaxIter, &it, P.threads); st.pcgIters += it; }
+                if (rc != 0) return -5;   // IMPLICIT_EULER.h:826-831 exits on a non-SPD matrix
+                double s2 = 0;
+                for (int i = 0; i < n; ++i) s2 += sol[i] * sol[i];
+                L2Norm = std::sqrt(s2 / (S.nV - (int)S.dbcIdx.size())) / h;
+                if (L2Norm > P.PNTol) Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp);
+            }
+        }
     } while (resRecord.size() < 3 || L2Norm > P.PNTol);
     for (int i = 0; i < n; ++i) S.v[i] = (x[i] - xn[i]) / h;
     st.newtonIters = PNIter;
