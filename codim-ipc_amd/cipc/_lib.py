@@ -59,6 +59,9 @@ class SceneDesc(ctypes.Structure):
         ("particles", I32),
         ("particle_density", ctypes.c_double),
         ("particle_thickness", ctypes.c_double),
+        ("friction_mu", ctypes.c_double),
+        ("friction_epsv2", ctypes.c_double),
+        ("friction_iter_amt", ctypes.c_int32),
     ]
 
 
@@ -129,6 +132,11 @@ PROTOTYPES = {
     "cipc_hessian": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, I32, I32, D, ctypes.c_int32, I32]),
     "cipc_min_dist2": (ctypes.c_int, [P, D, I32, ctypes.c_int32, D, D]),
     "cipc_ccd_stepsize": (ctypes.c_int, [P, D, D, ctypes.c_double, D, D]),
+    "cipc_set_friction": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32]),
+    "cipc_friction_basis": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, I32, D, ctypes.c_int32, I32]),
+    "cipc_friction_energy": (ctypes.c_int, [P, D, D, D]),
+    "cipc_friction_gradient": (ctypes.c_int, [P, D, D, D]),
+    "cipc_friction_hessian": (ctypes.c_int, [P, D, D, I32, I32, D, ctypes.c_int32, I32]),
     "cipc_linear_solve": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, D, D, ctypes.c_double, I32, D]),
     "cipc_set_precision": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_get_solver_info": (ctypes.c_int, [P, I32]),

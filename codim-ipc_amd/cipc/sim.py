@@ -79,6 +79,9 @@ class Simulator:
         desc.rod_thickness = P.get("rodThick", 1e-3)
         desc.particle_density = P.get("partRho", 0.0)
         desc.particle_thickness = P.get("partThick", 1e-3)
+        desc.friction_mu = P.get("mu", 0.0)
+        desc.friction_epsv2 = P.get("epsv2", 1e-6)
+        desc.friction_iter_amt = int(P.get("fricIterAmt", 1))
         h = ctypes.c_void_p()
         rc = self.lib.cipc_create(ctypes.byref(desc), ctypes.byref(h))
         if rc != 0:
@@ -334,6 +337,52 @@ class Simulator:
         m = np.zeros(1)
         self._check(self.lib.cipc_min_dist2(self.h, _d(x), _i(C), len(C), _d(d2), _d(m)))
         return d2[:len(C)], m[0]
+
+    # ---- lagged friction (FEM/FRICTION.h)
+    def set_friction(self, mu, epsv2=1e-6, iter_amt=1):
+        """The step's mu, epsv2 and fricIterAmt arguments (IMPLICIT_EULER.h:167)."""
+        self._check(self.lib.cipc_set_friction(self.h, float(mu), float(epsv2), int(iter_amt)))
+
+    def friction_basis(self, x, C, kappa):
+        """Compute_Friction_Basis (FRICTION.h:17-112): (stencils (n, 4), info (n, 9)); the
+        stencils stay in the context for friction_energy / _gradient / _hessian."""
+        x, C = self._x(x), self._cs(C)
+        cap = max(len(C), 1)
+        oc = np.zeros(4 * cap, dtype=np.int32)
+        oi = np.zeros(9 * cap)
+        n = ctypes.c_int32(0)
+        self._check(self.lib.cipc_friction_basis(self.h, _d(x), _i(C), len(C), float(kappa), _i(oc), _d(oi), cap, ctypes.byref(n)))
+        return oc[:4 * n.value].reshape(-1, 4), oi[:9 * n.value].reshape(-1, 9)
+
+    def friction_energy(self, x, xn):
+        """Compute_Friction_Potential (FRICTION.h:173-243) of the kept stencils."""
+        e = np.zeros(1)
+        self._check(self.lib.cipc_friction_energy(self.h, _d(self._x(x)), _d(self._x(xn)), _d(e)))
+        return e[0]
+
+    def friction_gradient(self, x, xn):
+        """Compute_Friction_Gradient (FRICTION.h:245-394)."""
+        g = np.zeros(3 * self.nV)
+        self._check(self.lib.cipc_friction_gradient(self.h, _d(self._x(x)), _d(self._x(xn)), _d(g)))
+        return g
+
+    def friction_hessian(self, x, xn):
+        """Compute_Friction_Hessian (FRICTION.h:396-661) as a scipy CSR matrix (Dirichlet rows/columns dropped)."""
+        import scipy.sparse as sp
+        x, xn = self._x(x), self._x(xn)
+        nb = ctypes.c_int32(0)
+        rc = self.lib.cipc_friction_hessian(self.h, _d(x), _d(xn), None, None, None, 0, ctypes.byref(nb))
+        if rc not in (0, -7):
+            self._check(rc)
+        cap = max(nb.value, 1)
+        br = np.zeros(cap, dtype=np.int32)
+        bc = np.zeros(cap, dtype=np.int32)
+        bv = np.zeros((cap, 9))
+        self._check(self.lib.cipc_friction_hessian(self.h, _d(x), _d(xn), _i(br), _i(bc), _d(bv), cap, ctypes.byref(nb)))
+        n = nb.value
+        rows = (3 * br[:n, None] + np.array([0, 0, 0, 1, 1, 1, 2, 2, 2])[None, :]).ravel()
+        cols = (3 * bc[:n, None] + np.array([0, 1, 2, 0, 1, 2, 0, 1, 2])[None, :]).ravel()
+        return sp.csr_matrix((bv[:n].ravel(), (rows, cols)), shape=(3 * self.nV, 3 * self.nV))
 
     def ccd_stepsize(self, x, p, alpha=1.0):
         """Compute_Intersection_Free_StepSize (IPC.h:1879-2244): (alpha, alpha after the hash clamp)."""

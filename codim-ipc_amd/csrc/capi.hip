@@ -70,6 +70,7 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
     for (int i = 0; i < d->n_particles; ++i)
         if (d->particles[i] < 0 || d->particles[i] >= d->n_vertices) return CIPC_E_INVALID;
     if (d->n_vertices >= (1 << 21) - 1) return CIPC_E_INVALID;   // contact-key packing limit
+    if (d->friction_mu < 0 || (d->friction_mu > 0 && !(d->friction_epsv2 > 0))) return CIPC_E_INVALID;
     cipc_ctx* ctx = new cipc_ctx();
     int rc = guarded(ctx, [&]() {
         Ctx& c = ctx->c;
@@ -83,6 +84,9 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
         P.collision = d->with_collision;
         P.rodRho = d->rod_density; P.rodE = d->rod_youngs; P.rodBendMult = d->rod_bend_stiff_mult; P.rodThick = d->rod_thickness;
         P.partRho = d->particle_density; P.partThick = d->particle_thickness;
+        P.fricMu = d->friction_mu;
+        P.epsv2 = d->friction_mu > 0 ? d->friction_epsv2 : 1e-6;
+        P.fricIterAmt = d->friction_iter_amt;
         c.hRod.assign(d->rod_segments, d->rod_segments + 2 * (size_t)d->n_rod_segments);
         c.hPart.assign(d->particles, d->particles + (size_t)d->n_particles);
         CIPC_HIP(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
@@ -353,6 +357,88 @@ int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_
             CIPC_HIP(hipMemcpyAsync(d2, dd, n_cs * sizeof(double), hipMemcpyDeviceToHost, c.s));
             CIPC_HIP(hipStreamSynchronize(c.s));
         }
+        return CIPC_OK;
+    });
+}
+
+// ---- lagged friction
+int cipc_set_friction(cipc_ctx* ctx, double mu, double epsv2, int32_t iter_amt) {
+    if (!ctx || mu < 0 || (mu > 0 && !(epsv2 > 0))) return CIPC_E_INVALID;
+    Params& P = ctx->c.P;
+    P.fricMu = mu;
+    if (mu > 0) P.epsv2 = epsv2;
+    P.fricIterAmt = iter_amt;
+    return CIPC_OK;
+}
+
+int cipc_friction_basis(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t* out_cs,
+                        double* out_info, int32_t cap, int32_t* n) {
+    if (!ctx || !x || !n || n_cs < 0 || (n_cs && !cs) || cap < 0) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        upload_x(ctx, ctx->qx, x, 3 * (size_t)c.nV);
+        upload_cs(ctx, cs, n_cs);
+        friction_basis(c, ctx->qx.p, ctx->qC.p, n_cs, kappa);
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        const int nF = c.nF;
+        *n = nF;
+        if (nF > cap) return CIPC_E_CAPACITY;
+        if (nF && out_cs && out_info) {
+            std::vector<int> fc(4 * (size_t)nF);
+            std::vector<double> fi(9 * (size_t)nF);
+            CIPC_HIP(hipMemcpyAsync(fc.data(), c.fC.p, fc.size() * sizeof(int), hipMemcpyDeviceToHost, c.s));
+            CIPC_HIP(hipMemcpyAsync(fi.data(), c.fI.p, fi.size() * sizeof(double), hipMemcpyDeviceToHost, c.s));
+            CIPC_HIP(hipStreamSynchronize(c.s));
+            for (int i = 0; i < nF; ++i) {
+                for (int k = 0; k < 4; ++k) out_cs[4 * (size_t)i + k] = fc[(size_t)k * nF + i];
+                for (int k = 0; k < 9; ++k) out_info[9 * (size_t)i + k] = fi[(size_t)k * nF + i];
+            }
+        }
+        return CIPC_OK;
+    });
+}
+
+int cipc_friction_energy(cipc_ctx* ctx, const double* x, const double* xn, double* e) {
+    if (!ctx || !x || !xn || !e) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        size_t n = 3 * (size_t)c.nV;
+        upload_x(ctx, ctx->qx, x, n);
+        upload_x(ctx, ctx->qp, xn, n);
+        *e = friction_energy(c, ctx->qx.p, ctx->qp.p);
+        return CIPC_OK;
+    });
+}
+
+int cipc_friction_gradient(cipc_ctx* ctx, const double* x, const double* xn, double* g) {
+    if (!ctx || !x || !xn || !g) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        size_t n = 3 * (size_t)c.nV;
+        upload_x(ctx, ctx->qx, x, n);
+        upload_x(ctx, ctx->qp, xn, n);
+        double* gd = ctx->qout.ensure(n);
+        CIPC_HIP(hipMemsetAsync(gd, 0, n * sizeof(double), c.s));
+        friction_gradient(c, ctx->qx.p, ctx->qp.p, gd);
+        CIPC_HIP(hipMemcpyAsync(g, gd, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        return CIPC_OK;
+    });
+}
+
+int cipc_friction_hessian(cipc_ctx* ctx, const double* x, const double* xn, int32_t* brow, int32_t* bcol, double* bval,
+                          int32_t cap, int32_t* nblocks_out) {
+    if (!ctx || !x || !xn || !nblocks_out) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        size_t n = 3 * (size_t)c.nV;
+        upload_x(ctx, ctx->qx, x, n);
+        upload_x(ctx, ctx->qp, xn, n);
+        barrier_hessian(c, ctx->qx.p, nullptr, 0, 0.0, ctx->qp.p);
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        *nblocks_out = c.Acon.nnzb;
+        if (c.Acon.nnzb > cap || !brow || !bcol || !bval) return (c.Acon.nnzb > cap) ? CIPC_E_CAPACITY : CIPC_OK;
+        download_bsr(c, c.Acon, brow, bcol, bval, 0);
         return CIPC_OK;
     });
 }

@@ -779,13 +779,13 @@ __global__ void __launch_bounds__(64) k_barrier_hess_build(const double* __restr
 }
 
 // Stage 2: PSD projection of each local Hessian (translation-reduced Jacobi in registers,
-// one constraint per thread) written as SoA [9][16 nC] blocks; block (a, b) of
+// one constraint per thread) written as SoA [9][ns] blocks (ns >= 16 nC); block (a, b) of
 // constraint i at slot 16 i + 4 a + b, zero outside the constraint's nv x nv blocks.
 __global__ void __launch_bounds__(64) k_barrier_hess_proj(const double* __restrict__ Hc, const int* __restrict__ nvv, int nC,
-                                                          double* __restrict__ hl) {
+                                                          double* __restrict__ hl, long ns) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nC) return;
-    const long ns = 16l * nC, base = 16l * i;
+    const long base = 16l * i;
     const int nv = nvv[i];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -877,6 +877,12 @@ void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kapp
     int* gv = c.conGVert.ensure(ns);
     hipLaunchKernelGGL(k_barrier_grad_local, dim3(nblocks(nC)), dim3(kBlock), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, gl, gv);
     CIPC_LAUNCH_CHECK();
+    scatter_local_gradient(c, gl, gv, ns, g);
+}
+
+// g += local gradient slots (SoA [3][ns], vertex gv per slot, -1 unused), summed per
+// vertex in slot order (radix sort by vertex, stable)
+void scatter_local_gradient(Ctx& c, const double* gl, const int* gv, long ns, double* g) {
     uint64_t* key = c.k64a.ensure(ns);
     uint32_t* slot = c.u32c.ensure(ns);
     hipLaunchKernelGGL(k_grad_keys, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, gv, ns, c.nV, key);
@@ -949,27 +955,36 @@ __global__ void k_rowptr(const int* __restrict__ rowOf, int nnzb, int nrow, int*
     rowptr[r] = lo;
 }
 
-void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa) {
+// contact BSR: barrier blocks of C and, when fricXn is given, the friction blocks of the
+// lagged friction stencils (slots 16 nC onwards), summed per (row, col)
+void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa, const double* fricXn) {
     BSR& A = c.Acon;
     A.nrow = c.nV;
-    if (nC == 0) {
+    const int nF = fricXn ? c.nF : 0;
+    if (nC + nF == 0) {
         A.nnzb = 0;
         A.rowptr.ensure(c.nV + 1);
         CIPC_HIP(hipMemsetAsync(A.rowptr.p, 0, (c.nV + 1) * sizeof(int), c.s));
         return;
     }
     const double xi2 = c.P.xi * c.P.xi, shat = c.dHat2 + 2 * std::sqrt(c.dHat2) * c.P.xi;
-    long ns = 16l * nC;
+    long ns = 16l * (nC + nF);
     double* hl = c.conLocal.ensure(9 * ns);
-    double* Hc = c.conH.ensure(144 * (size_t)nC);
-    int* nvv = c.conNv.ensure((size_t)nC);
-    hipLaunchKernelGGL(k_barrier_hess_build, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, Hc, nvv);
-    hipLaunchKernelGGL(k_barrier_hess_proj, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, (const double*)Hc, (const int*)nvv, nC, hl);
-    CIPC_LAUNCH_CHECK();
     uint64_t* key = c.k64b.ensure(ns);
     uint32_t* slot = c.u32d.ensure(ns);
-    hipLaunchKernelGGL(k_hess_keys, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, C, nC, c.nV, c.dbc.p, key);
-    CIPC_LAUNCH_CHECK();
+    if (nC) {
+        double* Hc = c.conH.ensure(144 * (size_t)nC);
+        int* nvv = c.conNv.ensure((size_t)nC);
+        hipLaunchKernelGGL(k_barrier_hess_build, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, Hc, nvv);
+        hipLaunchKernelGGL(k_barrier_hess_proj, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, (const double*)Hc, (const int*)nvv, nC, hl, ns);
+        CIPC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_hess_keys, dim3(nblocks(16l * nC)), dim3(kBlock), 0, c.s, C, nC, c.nV, c.dbc.p, key);
+        CIPC_LAUNCH_CHECK();
+    }
+    if (nF) {
+        friction_hess_local(c, x, fricXn, hl, ns, 16l * nC);
+        friction_hess_keys(c, key + 16l * nC);
+    }
     fill_iota(slot, ns, c.s);
     radix_sort_pairs(key, slot, ns, bits_for((uint64_t)c.nV * c.nV + 1), c.pw, c.s);
     uint32_t* head = c.u32a.ensure(ns);

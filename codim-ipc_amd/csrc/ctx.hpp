@@ -26,6 +26,9 @@ struct Params {
     // rods and particles
     double rodRho = 0, rodE = 0, rodBendMult = 1, rodThick = 1e-3;
     double partRho = 0, partThick = 1e-3;
+    // lagged friction (Advance_One_Step_IE_Hinge's mu, epsv2, fricIterAmt)
+    double fricMu = 0, epsv2 = 1e-6;
+    int fricIterAmt = 1;
 };
 
 // spatial grid used by the constraint-set broad phase and the CCD candidate search
@@ -130,6 +133,15 @@ struct Ctx {
     DevBuf<double> conGLocal;          // SoA [3][4 nC]
     DevBuf<int> conGVert;              // vertex of each gradient slot (4 nC; -1 unused)
 
+    // lagged friction stencils (FEM/FRICTION.h): SoA [4][nF] encoding, SoA [9][nF] data
+    // (closest point 2, tangent basis columns 2 x 3, normal force)
+    int nF = 0;
+    bool fricOn = false;               // inside a step with mu > 0: potentials include friction
+    DevBuf<int> fC;
+    DevBuf<double> fI;
+    DevBuf<double> fGLocal;            // SoA [3][4 nF]
+    DevBuf<int> fGVert;
+
     // PCG
     DevBuf<double> pr, pz, pp, pq, pdinv, psol;
     // mixed forms: FP32 copies of the matrix values, FP32 Krylov vectors (fp32 form), FP64 residual
@@ -206,7 +218,15 @@ void ctx_init_device(Ctx& c);                                      // upload + d
 void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut);
 double barrier_energy(Ctx& c, const double* x, const int* C, int nC, double kappa, int* bad);
 void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kappa, double* g);
-void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa);   // fills Acon
+// fills Acon; with fricXn the lagged friction blocks (displacement x - fricXn) are added
+void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa, const double* fricXn = nullptr);
+void scatter_local_gradient(Ctx& c, const double* gl, const int* gv, long ns, double* g);
+// lagged friction (friction.hip): basis at x for constraint set C into c.fC / c.fI / c.nF
+void friction_basis(Ctx& c, const double* x, const int* C, int nC, double kappa);
+double friction_energy(Ctx& c, const double* x, const double* xn);
+void friction_gradient(Ctx& c, const double* x, const double* xn, double* g);   // g +=
+void friction_hess_local(Ctx& c, const double* x, const double* xn, double* hl, long ns, long off);
+void friction_hess_keys(Ctx& c, uint64_t* key);
 double min_dist2(Ctx& c, const double* x, const int* C, int nC, double* d2out);
 double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, double* clamped, int* zeroToi);
 double elastic_energy(Ctx& c, const double* x, double* memb, double* bend);

@@ -5,7 +5,9 @@
 // (Utils/MESHIO.h:728-835) and Initialize_EIPC<double,false> (DISCRETE_SHELL.h:555-577).
 // advance_one_step restates Advance_One_Step_IE_Discrete_Shell<double,3,false,false>
 // (FEM/Shell/IMPLICIT_EULER.h:151-902) and Line_Search (:9-149) for static or
-// CCD-unblocked Dirichlet nodes without strain limiting, fibres, tets or friction;
+// CCD-unblocked Dirichlet nodes without strain limiting, fibres or tets, with lagged
+// friction (basis at the step start, re-lagged after convergence up to fricIterAmt
+// times, IMPLICIT_EULER.h:412-437, 770-859);
 // every array operation runs on the device, the host only sequences the stages and
 // reads back the scalars the control flow branches on (energies, step sizes,
 // residuals, counts).
@@ -334,6 +336,7 @@ static double inc_potential(Ctx& c, const double* x, const int* C, int nC, int* 
     double E = elastic_energy(c, x, nullptr, nullptr) + inertia_energy(c, x);
     if (c.P.collision) E += barrier_energy(c, x, C, nC, c.kappa, bad);
     else if (bad) *bad = 0;
+    if (c.fricOn) E += friction_energy(c, x, c.xn.p);   // IMPLICIT_EULER.h:436
     return E;
 }
 
@@ -378,6 +381,10 @@ int advance_one_step(Ctx& c) {
     c.nC = 0;
     c.nCprev = 0;
     if (P.collision) { ta = now_ms(); constraint_set(c, c.x.p, c.C, c.nC); st.t_constraint_set_ms += now_ms() - ta; }
+    // lagged friction stencils at the step start (IMPLICIT_EULER.h:419-427)
+    c.fricOn = P.collision && P.fricMu > 0;
+    c.nF = 0;
+    if (c.fricOn) friction_basis(c, c.x.p, c.C.p, c.nC, c.kappa);
     int bad = 0;
     double Eprev = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
     if (bad) { c.lastError = "non-positive distance in the initial state"; return CIPC_E_PENETRATION; }
@@ -387,11 +394,16 @@ int advance_one_step(Ctx& c) {
     int PNIter = 0;
     bool havePrev = false;     // c.p holds the last Newton direction, accepted at alphaPrev
     double alphaPrev = 1.0;
-    do {
+    int fricIterI = 0;
+    bool reuseDir = false;     // c.p already holds this iterate's direction (friction re-lag)
+    // gradient, Hessian and Newton direction into c.p; returns false when the solve
+    // found the matrix not positive definite
+    auto newton_direction = [&]() -> bool {
         // gradient (INC_POTENTIAL.h:154-275)
         ta = now_ms();
         elastic_gradient(c, c.x.p, c.g.p);
         if (P.collision) barrier_gradient(c, c.x.p, c.C.p, c.nC, c.kappa, c.g.p);
+        if (c.fricOn) friction_gradient(c, c.x.p, c.xn.p, c.g.p);
         hipLaunchKernelGGL(k_rhs, dim3(nblocks(c.nV)), dim3(kBlock), 0, s, c.nV, c.g.p, c.dbc.p, c.rhs.p);
         CIPC_LAUNCH_CHECK();
         CIPC_HIP(hipStreamSynchronize(s));
@@ -400,7 +412,7 @@ int advance_one_step(Ctx& c) {
         if (!useGD) {
             ta = now_ms();
             elastic_hessian(c, c.x.p);
-            if (P.collision) barrier_hessian(c, c.x.p, c.C.p, c.nC, c.kappa);
+            if (P.collision) barrier_hessian(c, c.x.p, c.C.p, c.nC, c.kappa, c.fricOn ? c.xn.p : nullptr);
             else c.Acon.nnzb = 0;
             CIPC_HIP(hipStreamSynchronize(s));
             st.t_hessian_ms += now_ms() - ta;
@@ -420,9 +432,16 @@ int advance_one_step(Ctx& c) {
             int rc = pcg_solve(c, c.rhs.p, c.p.p, P.pcgTol, P.pcgMaxIter, &it, &rr, x0);
             st.pcg_iters += it;
             st.t_solve_ms += now_ms() - ta;
-            if (rc == 1) { useGD = true; st.gd_fallbacks++; }
+            if (rc == 1) { useGD = true; st.gd_fallbacks++; return false; }
         }
-        if (useGD) CIPC_HIP(hipMemcpyAsync(c.p.p, c.rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        return true;
+    };
+    do {
+        if (!reuseDir) {
+            newton_direction();
+            if (useGD) CIPC_HIP(hipMemcpyAsync(c.p.p, c.rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        reuseDir = false;
         // line search (IMPLICIT_EULER.h:9-149)
         CIPC_HIP(hipMemcpyAsync(c.xprev.p, c.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         double alpha = 1.0;
@@ -498,7 +517,25 @@ int advance_one_step(Ctx& c) {
         }
         st.last_residual = cur;
         if (PNIter > 2000) { c.lastError = "Newton iteration limit"; return CIPC_E_NO_CONVERGENCE; }
+        // friction re-lagged at a converged iterate (IMPLICIT_EULER.h:770-859): new basis
+        // and normal forces; the Newton direction of the updated potential decides whether
+        // the loop goes on.  The reference recomputes that same direction at the top of
+        // the next iteration; here it is kept in c.p and used directly.
+        if (c.fricOn && resRecord.size() == 3 && L2Norm <= P.PNTol) {
+            ++fricIterI;
+            if (fricIterI < P.fricIterAmt || P.fricIterAmt <= 0) {
+                friction_basis(c, c.x.p, c.C.p, c.nC, c.kappa);
+                havePrev = false;
+                if (!newton_direction()) { c.lastError = "friction update: Newton matrix not positive definite"; return CIPC_E_NO_CONVERGENCE; }
+                L2Norm = std::sqrt(sumsq(c, c.p.p, n) / (c.nV - c.nDBC)) / h;
+                if (L2Norm > P.PNTol) {
+                    Eprev = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
+                    reuseDir = true;
+                }
+            }
+        }
     } while (resRecord.size() < 3 || L2Norm > P.PNTol);
+    c.fricOn = false;
     hipLaunchKernelGGL(k_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c.x.p, c.xn.p, h, c.v.p);
     CIPC_LAUNCH_CHECK();
     CIPC_HIP(hipStreamSynchronize(s));

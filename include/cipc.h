@@ -67,6 +67,13 @@ typedef struct cipc_scene_desc {
     int32_t n_particles;
     const int32_t* particles;        /* n_particles vertex ids */
     double particle_density, particle_thickness;
+    /* lagged friction: the step's mu, epsv2 and fricIterAmt arguments (IMPLICIT_EULER.h:
+     * 167, FEM/FRICTION.h).  mu = 0 disables friction; with mu > 0, epsv2 > 0 is required
+     * (the reference's default is 1e-6) and friction_iter_amt follows fricIterAmt: 1 =
+     * basis taken at the step start only, k > 1 = re-lagged up to k - 1 times after
+     * convergence, <= 0 = re-lagged until the updated Newton step is below tolerance. */
+    double friction_mu, friction_epsv2;
+    int32_t friction_iter_amt;
 } cipc_scene_desc;
 
 typedef struct cipc_step_stats {
@@ -131,6 +138,24 @@ int cipc_hessian(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs
                  int32_t* brow, int32_t* bcol, double* bval, int32_t cap, int32_t* nblocks);
 /* Compute_Min_Dist2 (IPC.h:2246-2388): per-constraint d^2 (no offset) and min(d^2) - offset^2 */
 int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double* d2, double* min_val);
+/* Friction parameters for the following steps (the mu, epsv2, fricIterAmt arguments of
+ * Advance_One_Step_IE_Hinge, IMPLICIT_EULER.h:167); same meaning as in cipc_scene_desc. */
+int cipc_set_friction(cipc_ctx* ctx, double mu, double epsv2, int32_t iter_amt);
+/* Compute_Friction_Basis (FEM/FRICTION.h:17-112) at x for constraint set cs: the friction
+ * stencils (cs without mollified entries, same order) are kept in the context for the
+ * friction entry points below (a step replaces them) and returned: *n stencils,
+ * out_cs 4 int32 each, out_info 9 doubles each (closest-point coordinates 2, tangent
+ * basis columns 2 x 3, normal force).  CIPC_E_CAPACITY with *n set when n > cap. */
+int cipc_friction_basis(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t* out_cs,
+                        double* out_info, int32_t cap, int32_t* n);
+/* Compute_Friction_Potential (FRICTION.h:173-243) of the kept stencils, displacement x - xn */
+int cipc_friction_energy(cipc_ctx* ctx, const double* x, const double* xn, double* e);
+/* Compute_Friction_Gradient (FRICTION.h:245-394); g overwritten (3 n_vertices) */
+int cipc_friction_gradient(cipc_ctx* ctx, const double* x, const double* xn, double* g);
+/* Compute_Friction_Hessian (FRICTION.h:396-661), blocks summed per (row, col) with
+ * Dirichlet rows/columns dropped; COO layout as cipc_hessian */
+int cipc_friction_hessian(cipc_ctx* ctx, const double* x, const double* xn, int32_t* brow, int32_t* bcol, double* bval,
+                          int32_t cap, int32_t* nblocks);
 /* Compute_Intersection_Free_StepSize (IPC.h:1879-2244) incl. the CCD hash step clamp
  * (SPATIAL_HASH.h:466-482); alpha_clamped = step after the clamp */
 int cipc_ccd_stepsize(cipc_ctx* ctx, const double* x, const double* p, double alpha_in, double* alpha_out,
