@@ -187,6 +187,35 @@ void orc_gradient(void* h, const double* x, const int* c, int nC, double kappa, 
     if (gel) { std::memset(gel, 0, sizeof(double) * 3 * S.nV); elastic_grad(S, x, gel); }
     if (gbar) { std::memset(gbar, 0, sizeof(double) * 3 * S.nV); barrier_grad(S, x, toC(c, nC), kappa, gbar); }
 }
+// ---------------------------------------------------------------- strain limiting
+// kappa_s, s, sHat as in the step call (index 0 upper, 1 lower)
+void orc_set_strain_limit(void* h, const double* kappa, const double* s, const double* sHat) {
+    Params& P = ((Scene*)h)->P;
+    for (int i = 0; i < 2; ++i) { P.slKappa[i] = kappa[i]; P.slS[i] = s[i]; P.slSHat[i] = sHat[i]; }
+}
+// one triangle: returns 1 if valid; E, g (9), H (81, projected if proj), singular values (2)
+int orc_strain_tri(void* h, const double* x, int t, int proj, double* E, double* g, double* H, double* sig) {
+    return strain_limit_tri(*(Scene*)h, x, t, E, g, H, proj != 0, sig) ? 1 : 0;
+}
+// whole mesh: returns 1 if valid; E, gradient (3 nV, overwritten)
+int orc_strain_energy_grad(void* h, const double* x, double* E, double* g) {
+    Scene& S = *(Scene*)h;
+    *E = 0;
+    bool ok = strain_limit_energy(S, x, E);
+    if (g) { std::memset(g, 0, sizeof(double) * 3 * S.nV); strain_limit_grad(S, x, g); }
+    return ok ? 1 : 0;
+}
+// local 3x3 blocks of the projected Hessian; returns the count (9 per triangle)
+int orc_strain_hess(void* h, const double* x, int* rows, int* cols, double* vals) {
+    std::vector<Block> B;
+    strain_limit_hess(*(Scene*)h, x, B);
+    for (size_t i = 0; i < B.size(); ++i) {
+        rows[i] = B[i].r; cols[i] = B[i].c;
+        std::memcpy(vals + 9 * i, B[i].b, 9 * sizeof(double));
+    }
+    return (int)B.size();
+}
+
 // ---------------------------------------------------------------- friction
 void orc_set_friction(void* h, double mu, double epsv2, int iterAmt) {
     Params& P = ((Scene*)h)->P;

@@ -106,6 +106,8 @@ struct Params {
     // lagged friction (Advance_One_Step_IE_Hinge's mu, epsv2, fricIterAmt)
     double fricMu = 0, epsv2 = 1e-6;
     int fricIterAmt = 1;
+    // strain limiting (the step's kappa_s, s, sHat; upper limit index 0, lower 1)
+    double slKappa[2] = {0, 0}, slS[2] = {1.01, 0}, slSHat[2] = {1, 1};
     int solver = 0;            // 0 = direct (sparse Cholesky), 1 = block-Jacobi PCG
     double pcgTol = 1e-5;
     int pcgMaxIter = 100000;
@@ -138,6 +140,7 @@ struct Scene {
     int codimBN0 = 0, codimBN1 = 0;    // boundary-node slots where rod nodes / particles start
     double kappa = 0, kappaMax = 0, dHat2 = 0;
     double bboxDiag = 0;
+    std::vector<double> slIB;          // strain limiting: B^-1 of the rest metric (00, 01, 11 per tri)
 };
 
 // constraint: 4 ints in the reference encoding (FEM/IPC.h:144-661)
@@ -179,6 +182,14 @@ double elastic_energy(const Scene& S, const double* x, double* memb = nullptr, d
 void elastic_grad(const Scene& S, const double* x, double* g);
 void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out);
 double inertia_energy(const Scene& S, const double* x, const double* xt);
+
+// strain limiting (FEM/Shell/INEXT.h); energy adds to *E and returns false when a
+// singular value reaches a limit
+void strain_limit_setup(Scene& S);
+bool strain_limit_energy(const Scene& S, const double* x, double* E);
+void strain_limit_grad(const Scene& S, const double* x, double* g);
+void strain_limit_hess(const Scene& S, const double* x, std::vector<Block>& out);
+bool strain_limit_tri(const Scene& S, const double* x, int t, double* E, double* g9, double* H81, bool proj, double* sig);
 
 // assembled system in CSR (scalar) form
 struct CSR { int n = 0; std::vector<int> rp, ci; std::vector<double> val; };

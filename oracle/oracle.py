@@ -50,6 +50,10 @@ def lib():
         L.orc_membrane.argtypes = [D, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D, D, D]
         L.orc_solve.argtypes = [ctypes.c_int, I, I, D, D, D, ctypes.c_int, ctypes.c_double, I]
         L.orc_set_friction.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_set_strain_limit.argtypes = [ctypes.c_void_p, D, D, D]
+        L.orc_strain_tri.argtypes = [ctypes.c_void_p, D, ctypes.c_int, ctypes.c_int, D, D, D, D]
+        L.orc_strain_energy_grad.argtypes = [ctypes.c_void_p, D, D, D]
+        L.orc_strain_hess.argtypes = [ctypes.c_void_p, D, I, I, D]
         L.orc_friction_basis.argtypes = [ctypes.c_void_p, D, I, ctypes.c_int, ctypes.c_double, I, D, ctypes.c_int]
         L.orc_friction_energy.restype = ctypes.c_double
         L.orc_friction_energy.argtypes = [ctypes.c_void_p, D, D, I, ctypes.c_int, D]
@@ -200,6 +204,9 @@ class OracleScene:
         self.nRod, self.nRodHinge, self.nParticle, self.codimBN0, self.codimBN1 = [int(v) for v in s]
         self.nV = sd.nV
         L.orc_set_friction(self.h, float(P.get("mu", 0.0)), float(P.get("epsv2", 1e-6)), int(P.get("fricIterAmt", 1)))
+        self._sl = [np.array(P.get(k, d), dtype=np.float64) for k, d in
+                    (("kappa_s", (0.0, 0.0)), ("s", (1.01, 0.0)), ("sHat", (1.0, 1.0)))]
+        L.orc_set_strain_limit(self.h, _dp(self._sl[0]), _dp(self._sl[1]), _dp(self._sl[2]))
 
     def __del__(self):
         try:
@@ -299,6 +306,31 @@ class OracleScene:
         blk = np.zeros(9 * cap)
         n = lib().orc_barrier_blocks(self.h, _dp(x), _ip(C), len(C), kappa, _ip(rc), _dp(blk), cap)
         return rc[:2 * n].reshape(-1, 2), blk[:9 * n].reshape(-1, 3, 3)
+
+    # ---- strain limiting (FEM/Shell/INEXT.h)
+    def strain_tri(self, x, t, proj=True):
+        """(valid, E, g (9), H (9, 9), singular values (2)) of triangle t."""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        E, g, H, sg = np.zeros(1), np.zeros(9), np.zeros(81), np.zeros(2)
+        ok = lib().orc_strain_tri(self.h, _dp(x), int(t), int(proj), _dp(E), _dp(g), _dp(H), _dp(sg))
+        return bool(ok), E[0], g, H.reshape(9, 9), sg
+
+    def strain_energy_grad(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        E, g = np.zeros(1), np.zeros(3 * self.nV)
+        ok = lib().orc_strain_energy_grad(self.h, _dp(x), _dp(E), _dp(g))
+        return bool(ok), E[0], g
+
+    def strain_hessian(self, x):
+        import scipy.sparse as sp
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        cap = 9 * self.nT + 1
+        r, c, v = np.zeros(cap, dtype=np.int32), np.zeros(cap, dtype=np.int32), np.zeros(9 * cap)
+        nb = lib().orc_strain_hess(self.h, _dp(x), _ip(r), _ip(c), _dp(v))
+        rows = (3 * r[:nb, None] + np.array([0, 0, 0, 1, 1, 1, 2, 2, 2])[None, :]).ravel()
+        cols = (3 * c[:nb, None] + np.array([0, 1, 2, 0, 1, 2, 0, 1, 2])[None, :]).ravel()
+        n = 3 * self.nV
+        return sp.csr_matrix((v[:9 * nb], (rows, cols)), shape=(n, n))
 
     # ---- lagged friction (FEM/FRICTION.h)
     def friction_basis(self, x, C, kappa):

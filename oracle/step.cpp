@@ -9,8 +9,8 @@
 //
 // advance_one_step restates Advance_One_Step_IE_Discrete_Shell<double,3,false,false,
 // false> (FEM/Shell/IMPLICIT_EULER.h:151-902) for the configuration this build
-// covers: no strain limiting, no fibres, no tets, no stitches, static or
-// CCD-unblocked Dirichlet nodes, lagged friction (basis at the step start, re-lagged
+// covers: isotropic strain limiting (INEXT.h), no fibres, no tets, no stitches, static
+// or CCD-unblocked Dirichlet nodes, lagged friction (basis at the step start, re-lagged
 // after convergence up to fricIterAmt times, IMPLICIT_EULER.h:412-437, 770-859), and
 // Line_Search (IMPLICIT_EULER.h:9-149).
 // Parity pins: energy non-increase per accepted step, intersection-free states,
@@ -110,6 +110,7 @@ void scene_init(Scene& S) {
     S.kBend = P.bendMult * (P.E * std::pow(P.thick, 3) / (24 * (1.0 - P.nu * P.nu)));
     // rods and particles (initialize_rod / initialize_particle run before initialize_OIPC)
     codim_init(S);
+    strain_limit_setup(S);
     // Initialize_EIPC<double,false>, DISCRETE_SHELL.h:563-574
     S.dHat2 = P.dHat * P.dHat;
     double Hb = barrier_h(1.0e-16, S.dHat2, 1.0);
@@ -121,10 +122,14 @@ void scene_init(Scene& S) {
 }
 
 // full incremental potential at x for constraint set C (+ friction with the lagged set F)
+// (*invalid set when a strain limit is reached, INC_POTENTIAL.h:75-80)
 static double inc_potential(const Scene& S, const double* x, const double* xt, const std::vector<Constraint>& C,
                             double kappa, bool withContact, int* bad, const double* xn = nullptr,
-                            const FricSet* F = nullptr) {
-    double E = elastic_energy(S, x) + inertia_energy(S, x, xt);
+                            const FricSet* F = nullptr, bool* invalid = nullptr) {
+    double E = 0;
+    bool ok = strain_limit_energy(S, x, &E);
+    if (invalid) *invalid = !ok;
+    E += elastic_energy(S, x) + inertia_energy(S, x, xt);
     if (withContact) E += barrier_energy(S, x, C, kappa, bad);
     if (F) E += friction_energy(S, x, xn, *F);
     return E;
@@ -165,8 +170,10 @@ int advance_one_step(Scene& S, StepStats& st) {
     const FricSet* Fp = fric ? &F : nullptr;
     if (fric) friction_basis(S, x.data(), C, S.kappa, F);
     int bad = 0;
-    double Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp);
+    bool invalid = false;
+    double Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, &invalid);
     if (bad) return -1;
+    if (invalid) return -6;   // strain limit violated by the initial state
     int fricIterI = 0;
     std::deque<double> resRecord;
     bool useGD = false;
@@ -179,6 +186,7 @@ int advance_one_step(Scene& S, StepStats& st) {
         double ta = now();
         std::fill(g.begin(), g.end(), 0.0);
         elastic_grad(S, x.data(), g.data());
+        strain_limit_grad(S, x.data(), g.data());
         for (int i = 0; i < S.nV; ++i)
             for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
         if (P.collision) barrier_grad(S, x.data(), C, S.kappa, g.data());
@@ -192,6 +200,7 @@ int advance_one_step(Scene& S, StepStats& st) {
             ta = now();
             std::vector<Block> blocks;
             elastic_hess(S, x.data(), blocks);
+            strain_limit_hess(S, x.data(), blocks);
             if (P.collision) barrier_hess(S, x.data(), C, S.kappa, blocks);
             if (fric) friction_hess(S, x.data(), xn.data(), F, blocks);
             assemble(S, blocks, A);
@@ -208,9 +217,19 @@ int advance_one_step(Scene& S, StepStats& st) {
         // line search, IMPLICIT_EULER.h:9-149
         xprev = x;
         double alpha = 1.0;
+        // strain-limit feasibility first, IMPLICIT_EULER.h:74-92
+        if (P.slKappa[0] > 0) {
+            for (;;) {
+                for (int i = 0; i < n; ++i) x[i] = xprev[i] + alpha * sol[i];
+                double Et = 0;
+                if (strain_limit_energy(S, x.data(), &Et)) break;
+                alpha /= 2;
+            }
+            x = xprev;
+        }
         if (P.collision) {
             ta = now();
-            alpha = ccd_stepsize(S, x.data(), sol.data(), alpha);
+            alpha = ccd_stepsize(S, x.data(), sol.data(), alpha);   // from the feasible alpha
             st.tCCD += now() - ta;
             if (alpha == 0) return -3;
         }
@@ -232,9 +251,10 @@ int advance_one_step(Scene& S, StepStats& st) {
                 }
             }
             ta = now();
-            E = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp);
+            E = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, &invalid);
+            valid = !invalid;
             st.tEnergy += now() - ta;
-            if (bad) return -1;
+            if (bad && valid) return -1;
             alpha /= 2.0;
         } while (E > Eprev || !valid);
         Eprev = E;
@@ -281,6 +301,7 @@ int advance_one_step(Scene& S, StepStats& st) {
                 friction_basis(S, x.data(), C, S.kappa, F);
                 std::fill(g.begin(), g.end(), 0.0);
                 elastic_grad(S, x.data(), g.data());
+                strain_limit_grad(S, x.data(), g.data());
                 for (int i = 0; i < S.nV; ++i)
                     for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
                 barrier_grad(S, x.data(), C, S.kappa, g.data());
@@ -289,6 +310,7 @@ int advance_one_step(Scene& S, StepStats& st) {
                 for (int i = 0; i < n; ++i) rhs[i] = -g[i];
                 std::vector<Block> blocks;
                 elastic_hess(S, x.data(), blocks);
+                strain_limit_hess(S, x.data(), blocks);
                 barrier_hess(S, x.data(), C, S.kappa, blocks);
                 friction_hess(S, x.data(), xn.data(), F, blocks);
                 assemble(S, blocks, A);
