@@ -62,6 +62,9 @@ class SceneDesc(ctypes.Structure):
         ("friction_mu", ctypes.c_double),
         ("friction_epsv2", ctypes.c_double),
         ("friction_iter_amt", ctypes.c_int32),
+        ("strain_limit_kappa", ctypes.c_double * 2),
+        ("strain_limit_s", ctypes.c_double * 2),
+        ("strain_limit_shat", ctypes.c_double * 2),
     ]
 
 
@@ -133,6 +136,8 @@ PROTOTYPES = {
     "cipc_min_dist2": (ctypes.c_int, [P, D, I32, ctypes.c_int32, D, D]),
     "cipc_ccd_stepsize": (ctypes.c_int, [P, D, D, ctypes.c_double, D, D]),
     "cipc_set_friction": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32]),
+    "cipc_set_strain_limit": (ctypes.c_int, [P, D, D, D]),
+    "cipc_strain_limit_energy": (ctypes.c_int, [P, D, D, I32]),
     "cipc_friction_basis": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, I32, D, ctypes.c_int32, I32]),
     "cipc_friction_energy": (ctypes.c_int, [P, D, D, D]),
     "cipc_friction_gradient": (ctypes.c_int, [P, D, D, D]),

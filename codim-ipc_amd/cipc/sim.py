@@ -82,6 +82,8 @@ class Simulator:
         desc.friction_mu = P.get("mu", 0.0)
         desc.friction_epsv2 = P.get("epsv2", 1e-6)
         desc.friction_iter_amt = int(P.get("fricIterAmt", 1))
+        for i, (k, s_, sh) in enumerate(zip(P.get("kappa_s", (0.0, 0.0)), P.get("s", (1.01, 0.0)), P.get("sHat", (1.0, 1.0)))):
+            desc.strain_limit_kappa[i], desc.strain_limit_s[i], desc.strain_limit_shat[i] = k, s_, sh
         h = ctypes.c_void_p()
         rc = self.lib.cipc_create(ctypes.byref(desc), ctypes.byref(h))
         if rc != 0:
@@ -337,6 +339,19 @@ class Simulator:
         m = np.zeros(1)
         self._check(self.lib.cipc_min_dist2(self.h, _d(x), _i(C), len(C), _d(d2), _d(m)))
         return d2[:len(C)], m[0]
+
+    # ---- strain limiting (FEM/Shell/INEXT.h)
+    def set_strain_limit(self, kappa, s=(1.01, 0.0), sHat=(1.0, 1.0)):
+        """The step's kappa_s, s, sHat arguments (index 0 upper limit, 1 lower)."""
+        k, s_, sh = (np.ascontiguousarray(v, dtype=np.float64) for v in (kappa, s, sHat))
+        self._check(self.lib.cipc_set_strain_limit(self.h, _d(k), _d(s_), _d(sh)))
+
+    def strain_limit_energy(self, x):
+        """Compute_Inextensibility_Energy (INEXT.h:120-178): (valid, energy)."""
+        e = np.zeros(1)
+        ok = np.zeros(1, dtype=np.int32)
+        self._check(self.lib.cipc_strain_limit_energy(self.h, _d(self._x(x)), _d(e), _i(ok)))
+        return bool(ok[0]), e[0]
 
     # ---- lagged friction (FEM/FRICTION.h)
     def set_friction(self, mu, epsv2=1e-6, iter_amt=1):

@@ -1,6 +1,7 @@
 // C ABI (include/cipc.h).  Every entry point converts HIP failures and invalid
 // arguments into negative status codes and records a message for cipc_last_error().
 #include "ctx.hpp"
+#include <cmath>
 #include "learn.hpp"
 #include <algorithm>
 #include <atomic>
@@ -21,6 +22,15 @@ struct cipc_ctx {
 namespace cipc {
 void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri);
 void elastic_gradient_only(Ctx& c, const double* x, double* g);
+}
+
+// kappa[0] > 0 turns the upper limit on (needs s[0] > shat[0] >= 0); kappa[1] > 0 the
+// lower one (needs s[1] < shat[1]); kappa[1] alone is ignored, as in INEXT.h:157-175
+static bool strain_limit_args_ok(const double* k, const double* s, const double* sh) {
+    if (k[0] < 0 || k[1] < 0) return false;
+    if (k[0] > 0 && !(s[0] > sh[0] && sh[0] >= 0)) return false;
+    if (k[0] > 0 && k[1] > 0 && !(s[1] < sh[1] && s[1] >= 0 && sh[1] < s[0])) return false;
+    return true;
 }
 
 template <class F>
@@ -71,6 +81,7 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
         if (d->particles[i] < 0 || d->particles[i] >= d->n_vertices) return CIPC_E_INVALID;
     if (d->n_vertices >= (1 << 21) - 1) return CIPC_E_INVALID;   // contact-key packing limit
     if (d->friction_mu < 0 || (d->friction_mu > 0 && !(d->friction_epsv2 > 0))) return CIPC_E_INVALID;
+    if (!strain_limit_args_ok(d->strain_limit_kappa, d->strain_limit_s, d->strain_limit_shat)) return CIPC_E_INVALID;
     cipc_ctx* ctx = new cipc_ctx();
     int rc = guarded(ctx, [&]() {
         Ctx& c = ctx->c;
@@ -87,6 +98,10 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
         P.fricMu = d->friction_mu;
         P.epsv2 = d->friction_mu > 0 ? d->friction_epsv2 : 1e-6;
         P.fricIterAmt = d->friction_iter_amt;
+        if (d->strain_limit_kappa[0] > 0)
+            for (int i = 0; i < 2; ++i) {
+                P.slKappa[i] = d->strain_limit_kappa[i]; P.slS[i] = d->strain_limit_s[i]; P.slSHat[i] = d->strain_limit_shat[i];
+            }
         c.hRod.assign(d->rod_segments, d->rod_segments + 2 * (size_t)d->n_rod_segments);
         c.hPart.assign(d->particles, d->particles + (size_t)d->n_particles);
         CIPC_HIP(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
@@ -357,6 +372,27 @@ int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_
             CIPC_HIP(hipMemcpyAsync(d2, dd, n_cs * sizeof(double), hipMemcpyDeviceToHost, c.s));
             CIPC_HIP(hipStreamSynchronize(c.s));
         }
+        return CIPC_OK;
+    });
+}
+
+// ---- strain limiting
+int cipc_set_strain_limit(cipc_ctx* ctx, const double* kappa, const double* s, const double* shat) {
+    if (!ctx || !kappa || !s || !shat || !strain_limit_args_ok(kappa, s, shat)) return CIPC_E_INVALID;
+    Params& P = ctx->c.P;
+    for (int i = 0; i < 2; ++i) { P.slKappa[i] = kappa[i]; P.slS[i] = s[i]; P.slSHat[i] = shat[i]; }
+    if (!(kappa[0] > 0)) P.slKappa[0] = P.slKappa[1] = 0;
+    return CIPC_OK;
+}
+
+int cipc_strain_limit_energy(cipc_ctx* ctx, const double* x, double* e, int32_t* valid) {
+    if (!ctx || !x || !e || !valid) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        upload_x(ctx, ctx->qx, x, 3 * (size_t)c.nV);
+        const double E = strain_limit_energy(c, ctx->qx.p);
+        *valid = std::isinf(E) ? 0 : 1;
+        *e = *valid ? E : 0.0;
         return CIPC_OK;
     });
 }

@@ -29,6 +29,8 @@ struct Params {
     // lagged friction (Advance_One_Step_IE_Hinge's mu, epsv2, fricIterAmt)
     double fricMu = 0, epsv2 = 1e-6;
     int fricIterAmt = 1;
+    // strain limiting (the step's kappa_s, s, sHat; index 0 upper limit, 1 lower)
+    double slKappa[2] = {0, 0}, slS[2] = {1.01, 0}, slSHat[2] = {1, 1};
 };
 
 // spatial grid used by the constraint-set broad phase and the CCD candidate search
@@ -63,7 +65,7 @@ struct Ctx {
 
     // host mirrors of the topology
     std::vector<int> hTri, hHinge, hBNode, hBEdge, hDbcIdx;
-    std::vector<double> hDbcTarget, hMass, hHingeInfo, hBinv, hVol;
+    std::vector<double> hDbcTarget, hMass, hHingeInfo, hBinv, hVol, hSlIB;
     // codimensional primitives: rod segments are the last nRod boundary edges, rod nodes
     // occupy boundary-node slots [codimBN0, codimBN1), particles [codimBN1, nBN)
     int nRod = 0, nRH = 0, nPart = 0, codimBN0 = 0, codimBN1 = 0;
@@ -87,6 +89,7 @@ struct Ctx {
     DevBuf<uint8_t> dbc;
     DevBuf<int> tri;          // SoA [3][nT]
     DevBuf<double> Binv;      // SoA [3][nT]
+    DevBuf<double> slIB;      // SoA [3][nT]: B^-1 of the rest metric (strain limiting)
     DevBuf<double> vol;
     DevBuf<int> hinge;        // SoA [4][nH]
     DevBuf<double> hInfo;     // SoA [3][nH]
@@ -227,6 +230,12 @@ double friction_energy(Ctx& c, const double* x, const double* xn);
 void friction_gradient(Ctx& c, const double* x, const double* xn, double* g);   // g +=
 void friction_hess_local(Ctx& c, const double* x, const double* xn, double* hl, long ns, long off);
 void friction_hess_keys(Ctx& c, uint64_t* key);
+// strain limiting (strain.hip): energy (+inf when a limit is reached), gradient and
+// projected Hessian added to the membrane's local triangle slots
+bool strain_limit_on(const Ctx& c);
+double strain_limit_energy(Ctx& c, const double* x);
+void strain_limit_grad_add(Ctx& c, const double* x, double* gl);
+void strain_limit_hess_add(Ctx& c, const double* x, double* hl);
 double min_dist2(Ctx& c, const double* x, const int* C, int nC, double* d2out);
 double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, double* clamped, int* zeroToi);
 double elastic_energy(Ctx& c, const double* x, double* memb, double* bend);

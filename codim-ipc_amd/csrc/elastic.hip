@@ -228,6 +228,7 @@ static void elastic_grad_impl(Ctx& c, const double* x, double* g, int withInerti
         hipLaunchKernelGGL(k_memb_grad_local, dim3(nblocks(c.nT)), dim3(kBlock), 0, c.s, x, c.tri.p, c.nT, c.Binv.p, c.vol.p, c.lam, c.mu,
                            h2, c.dbc.p, gl, ns);
         CIPC_LAUNCH_CHECK();
+        strain_limit_grad_add(c, x, gl);   // INC_POTENTIAL.h:220-222
     }
     if (c.nH) {
         double kb = c.P.bendMult != 0 ? c.kBend : 0.0;
@@ -489,6 +490,7 @@ void elastic_hessian(Ctx& c, const double* x) {
         hipLaunchKernelGGL(k_memb_hess_local, dim3(nblocks(c.nT, 64)), dim3(64), 0, c.s, x, c.tri.p, c.nT, c.Binv.p, c.vol.p, c.lam, c.mu,
                            h2, c.dbc.p, hl, ns);
         CIPC_LAUNCH_CHECK();
+        strain_limit_hess_add(c, x, hl);   // INC_POTENTIAL.h:347-349
     }
     if (c.nH) {
         double kb = c.P.bendMult != 0 ? c.kBend : 0.0;

@@ -5,7 +5,7 @@
 // (Utils/MESHIO.h:728-835) and Initialize_EIPC<double,false> (DISCRETE_SHELL.h:555-577).
 // advance_one_step restates Advance_One_Step_IE_Discrete_Shell<double,3,false,false>
 // (FEM/Shell/IMPLICIT_EULER.h:151-902) and Line_Search (:9-149) for static or
-// CCD-unblocked Dirichlet nodes without strain limiting, fibres or tets, with lagged
+// CCD-unblocked Dirichlet nodes with isotropic strain limiting (INEXT.h) and lagged
 // friction (basis at the step start, re-lagged after convergence up to fricIterAmt
 // times, IMPLICIT_EULER.h:412-437, 770-859);
 // every array operation runs on the device, the host only sequences the stages and
@@ -91,6 +91,18 @@ void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri
         double I00 = hdot(e1, e1), I01 = hdot(e1, e2), I11 = hdot(e2, e2);
         double det = I00 * I11 - I01 * I01;
         c.hBinv[3 * t] = I11 / det; c.hBinv[3 * t + 1] = -I01 / det; c.hBinv[3 * t + 2] = I00 / det;
+    }
+    // strain limiting: inverse of the upper-triangular factor B of the rest metric
+    // (B^T B = I_rest, INEXT.h:145-151): (B^-1)_00, (B^-1)_01, (B^-1)_11
+    c.hSlIB.resize(3 * c.nT);
+    for (int t = 0; t < c.nT; ++t) {
+        double a[3], b[3], cc[3], e1[3], e2[3];
+        hv(X, tri[3 * t], a); hv(X, tri[3 * t + 1], b); hv(X, tri[3 * t + 2], cc);
+        hsub(b, a, e1); hsub(cc, a, e2);
+        double I00 = hdot(e1, e1), I01 = hdot(e1, e2), I11 = hdot(e2, e2);
+        double B00 = std::sqrt(I00), B01 = I01 / B00, B11 = std::sqrt(I11 - I01 * I01 / I00);
+        double inv = 1.0 / (B00 * B11);
+        c.hSlIB[3 * t] = B11 * inv; c.hSlIB[3 * t + 1] = -B01 * inv; c.hSlIB[3 * t + 2] = B00 * inv;
     }
     c.hHinge.clear(); c.hHingeInfo.clear();
     for (auto& e : edgeSet) {
@@ -221,6 +233,7 @@ void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri
     upload(c.dbc, dbc, s);
     upload(c.tri, soa_i(tri, c.nT, 3), s);
     upload(c.Binv, soa_d(c.hBinv, c.nT, 3), s);
+    upload(c.slIB, soa_d(c.hSlIB, c.nT, 3), s);
     upload(c.vol, c.hVol, s);
     upload(c.hinge, soa_i(c.hHinge, c.nH, 4), s);
     upload(c.hInfo, soa_d(c.hHingeInfo, c.nH, 3), s);
@@ -332,8 +345,11 @@ static double sumsq(Ctx& c, const double* a, long n) {
 }
 
 // incremental potential at x with constraint set C (INC_POTENTIAL.h:15-152)
+// +inf when a strain limit is reached (INC_POTENTIAL.h:75-80: the potential is invalid)
 static double inc_potential(Ctx& c, const double* x, const int* C, int nC, int* bad) {
-    double E = elastic_energy(c, x, nullptr, nullptr) + inertia_energy(c, x);
+    const double Esl = strain_limit_energy(c, x);
+    if (std::isinf(Esl)) { if (bad) *bad = 0; return Esl; }
+    double E = Esl + elastic_energy(c, x, nullptr, nullptr) + inertia_energy(c, x);
     if (c.P.collision) E += barrier_energy(c, x, C, nC, c.kappa, bad);
     else if (bad) *bad = 0;
     if (c.fricOn) E += friction_energy(c, x, c.xn.p);   // IMPLICIT_EULER.h:436
@@ -388,6 +404,7 @@ int advance_one_step(Ctx& c) {
     int bad = 0;
     double Eprev = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
     if (bad) { c.lastError = "non-positive distance in the initial state"; return CIPC_E_PENETRATION; }
+    if (std::isinf(Eprev)) { c.lastError = "strain limit violated by the initial state"; return CIPC_E_UNSUPPORTED; }
     std::deque<double> resRecord;
     bool useGD = false;
     double L2Norm = 0;
@@ -445,6 +462,16 @@ int advance_one_step(Ctx& c) {
         // line search (IMPLICIT_EULER.h:9-149)
         CIPC_HIP(hipMemcpyAsync(c.xprev.p, c.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         double alpha = 1.0;
+        // strain-limit feasibility first (IMPLICIT_EULER.h:74-92), then the CCD from there
+        if (strain_limit_on(c)) {
+            for (;;) {
+                hipLaunchKernelGGL(k_axpy_to, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c.xprev.p, alpha, c.p.p, c.x.p);
+                CIPC_LAUNCH_CHECK();
+                if (!std::isinf(strain_limit_energy(c, c.x.p))) break;
+                alpha /= 2;
+            }
+            CIPC_HIP(hipMemcpyAsync(c.x.p, c.xprev.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
         if (P.collision) {
             ta = now_ms();
             int z = 0;

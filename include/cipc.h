@@ -74,6 +74,11 @@ typedef struct cipc_scene_desc {
      * convergence, <= 0 = re-lagged until the updated Newton step is below tolerance. */
     double friction_mu, friction_epsv2;
     int32_t friction_iter_amt;
+    /* isotropic strain limiting: the step's kappa_s, s, sHat arguments (FEM/Shell/INEXT.h,
+     * index 0 = upper limit, 1 = lower limit).  kappa[0] = 0 disables it; with kappa[0] > 0
+     * the singular values of each triangle's deformation gradient stay below s[0]
+     * (barrier from shat[0]); kappa[1] > 0 adds the lower limit s[1] < shat[1]. */
+    double strain_limit_kappa[2], strain_limit_s[2], strain_limit_shat[2];
 } cipc_scene_desc;
 
 typedef struct cipc_step_stats {
@@ -138,6 +143,13 @@ int cipc_hessian(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs
                  int32_t* brow, int32_t* bcol, double* bval, int32_t cap, int32_t* nblocks);
 /* Compute_Min_Dist2 (IPC.h:2246-2388): per-constraint d^2 (no offset) and min(d^2) - offset^2 */
 int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double* d2, double* min_val);
+/* Strain-limit parameters for the following steps (kappa_s, s, sHat of
+ * Advance_One_Step_IE_Hinge, IMPLICIT_EULER.h:162); same meaning as in cipc_scene_desc. */
+int cipc_set_strain_limit(cipc_ctx* ctx, const double* kappa, const double* s, const double* shat);
+/* Compute_Inextensibility_Energy (FEM/Shell/INEXT.h:120-178) at x: *valid = 0 when a
+ * singular value reached a limit (then *e = 0).  Its gradient and Hessian are part of
+ * cipc_elastic_gradient and cipc_hessian (which & 1) while strain limiting is on. */
+int cipc_strain_limit_energy(cipc_ctx* ctx, const double* x, double* e, int32_t* valid);
 /* Friction parameters for the following steps (the mu, epsv2, fricIterAmt arguments of
  * Advance_One_Step_IE_Hinge, IMPLICIT_EULER.h:167); same meaning as in cipc_scene_desc. */
 int cipc_set_friction(cipc_ctx* ctx, double mu, double epsv2, int32_t iter_amt);

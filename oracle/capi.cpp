@@ -264,7 +264,7 @@ static CSR g_lastA;
 int orc_hessian(void* h, const double* x, const int* c, int nC, double kappa, int which) {
     Scene& S = *(Scene*)h;
     std::vector<Block> blocks;
-    if (which & 1) elastic_hess(S, x, blocks);
+    if (which & 1) { elastic_hess(S, x, blocks); strain_limit_hess(S, x, blocks); }
     if (which & 2) barrier_hess(S, x, toC(c, nC), kappa, blocks);
     assemble(S, blocks, g_lastA);
     return (int)g_lastA.val.size();
