@@ -187,6 +187,25 @@ void orc_gradient(void* h, const double* x, const int* c, int nC, double kappa, 
     if (gel) { std::memset(gel, 0, sizeof(double) * 3 * S.nV); elastic_grad(S, x, gel); }
     if (gbar) { std::memset(gbar, 0, sizeof(double) * 3 * S.nV); barrier_grad(S, x, toC(c, nC), kappa, gbar); }
 }
+// ---------------------------------------------------------------- Dirichlet motion
+// DBC_MOTION entry over Dirichlet entries [begin, end) (BOUNDARY_CONDITION.h:10-16, 103)
+void orc_dirichlet_add_motion(void* h, int begin, int end, const double* v, const double* center, const double* axis,
+                              double angVelDeg) {
+    Scene::DBCMotion M;
+    M.begin = begin; M.end = end; M.angVelDeg = angVelDeg;
+    for (int k = 0; k < 3; ++k) { M.v[k] = v[k]; M.center[k] = center[k]; M.axis[k] = axis[k]; }
+    ((Scene*)h)->dbcMotion.push_back(M);
+}
+void orc_step_dirichlet(void* h) { step_dirichlet(*(Scene*)h); }
+void orc_get_dirichlet_targets(void* h, double* t) {
+    Scene* S = (Scene*)h;
+    std::memcpy(t, S->dbcTarget.data(), sizeof(double) * S->dbcTarget.size());
+}
+void orc_set_dirichlet_targets(void* h, const double* t) {
+    Scene* S = (Scene*)h;
+    std::memcpy(S->dbcTarget.data(), t, sizeof(double) * S->dbcTarget.size());
+}
+
 // ---------------------------------------------------------------- strain limiting
 // kappa_s, s, sHat as in the step call (index 0 upper, 1 lower)
 void orc_set_strain_limit(void* h, const double* kappa, const double* s, const double* sHat) {

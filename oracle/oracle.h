@@ -141,7 +141,12 @@ struct Scene {
     double kappa = 0, kappaMax = 0, dHat2 = 0;
     double bboxDiag = 0;
     std::vector<double> slIB;          // strain limiting: B^-1 of the rest metric (00, 01, 11 per tri)
+    // Dirichlet motion groups (FEM/BOUNDARY_CONDITION.h DBC_MOTION): entries [begin, end)
+    struct DBCMotion { int begin, end; double v[3], center[3], axis[3], angVelDeg; };
+    std::vector<DBCMotion> dbcMotion;
 };
+// Step_Dirichlet (BOUNDARY_CONDITION.h:107-155): rotate then translate every group's targets
+void step_dirichlet(Scene& S);
 
 // constraint: 4 ints in the reference encoding (FEM/IPC.h:144-661)
 struct Constraint { int c[4]; };
@@ -193,7 +198,8 @@ bool strain_limit_tri(const Scene& S, const double* x, int t, double* E, double*
 
 // assembled system in CSR (scalar) form
 struct CSR { int n = 0; std::vector<int> rp, ci; std::vector<double> val; };
-void assemble(const Scene& S, const std::vector<Block>& blocks, CSR& A);   // + M, DBC projection
+// + M, projection of the rows/columns flagged in proj (default: every Dirichlet node)
+void assemble(const Scene& S, const std::vector<Block>& blocks, CSR& A, const std::vector<char>* proj = nullptr);
 int solve_direct(const CSR& A, const double* b, double* x);                // 0 ok, 1 not SPD
 int solve_pcg(const CSR& A, const double* b, double* x, double tol, int maxIter, int* iters, int threads = 1);
 

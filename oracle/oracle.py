@@ -51,6 +51,10 @@ def lib():
         L.orc_solve.argtypes = [ctypes.c_int, I, I, D, D, D, ctypes.c_int, ctypes.c_double, I]
         L.orc_set_friction.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_set_strain_limit.argtypes = [ctypes.c_void_p, D, D, D]
+        L.orc_dirichlet_add_motion.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, D, D, D, ctypes.c_double]
+        L.orc_step_dirichlet.argtypes = [ctypes.c_void_p]
+        L.orc_get_dirichlet_targets.argtypes = [ctypes.c_void_p, D]
+        L.orc_set_dirichlet_targets.argtypes = [ctypes.c_void_p, D]
         L.orc_strain_tri.argtypes = [ctypes.c_void_p, D, ctypes.c_int, ctypes.c_int, D, D, D, D]
         L.orc_strain_energy_grad.argtypes = [ctypes.c_void_p, D, D, D]
         L.orc_strain_hess.argtypes = [ctypes.c_void_p, D, I, I, D]
@@ -306,6 +310,23 @@ class OracleScene:
         blk = np.zeros(9 * cap)
         n = lib().orc_barrier_blocks(self.h, _dp(x), _ip(C), len(C), kappa, _ip(rc), _dp(blk), cap)
         return rc[:2 * n].reshape(-1, 2), blk[:9 * n].reshape(-1, 3, 3)
+
+    # ---- Dirichlet motion (FEM/BOUNDARY_CONDITION.h)
+    def dirichlet_add_motion(self, begin, end, v=(0, 0, 0), center=(0, 0, 0), axis=(0, 1, 0), ang_vel_deg=0.0):
+        arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (v, center, axis)]
+        lib().orc_dirichlet_add_motion(self.h, int(begin), int(end), _dp(arr[0]), _dp(arr[1]), _dp(arr[2]), float(ang_vel_deg))
+
+    def step_dirichlet(self):
+        lib().orc_step_dirichlet(self.h)
+
+    def dirichlet_targets(self):
+        t = np.zeros(3 * len(self.sd.dbc_idx))
+        lib().orc_get_dirichlet_targets(self.h, _dp(t))
+        return t.reshape(-1, 3)
+
+    def set_dirichlet_targets(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float64).ravel()
+        lib().orc_set_dirichlet_targets(self.h, _dp(t))
 
     # ---- strain limiting (FEM/Shell/INEXT.h)
     def strain_tri(self, x, t, proj=True):

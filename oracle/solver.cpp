@@ -23,7 +23,8 @@ namespace orc {
 
 // Blocks are summed per (row, col) in their input order (stable sort), the mass is
 // added to the diagonal blocks, then the block matrix is expanded to scalar CSR.
-void assemble(const Scene& S, const std::vector<Block>& blocks, CSR& A) {
+void assemble(const Scene& S, const std::vector<Block>& blocks, CSR& A, const std::vector<char>* proj) {
+    const std::vector<char>& pm = proj ? *proj : S.isDBC;
     const int n = 3 * S.nV;
     const long nb = (long)blocks.size();
     std::vector<std::pair<uint64_t, long>> key(nb + S.nV);
@@ -52,7 +53,7 @@ void assemble(const Scene& S, const std::vector<Block>& blocks, CSR& A) {
         }
         int r = (int)(key[start[u]].first / S.nV), c = (int)(key[start[u]].first % S.nV);
         brow[u] = r; bcol[u] = c;
-        bool z = S.isDBC[r] || S.isDBC[c];
+        bool z = pm[r] || pm[c];
         for (int k = 0; k < 9; ++k) bval[9 * u + k] = z ? ((r == c && k % 4 == 0) ? 1.0 : 0.0) : s[k];
     }
     // expand block rows to scalar rows

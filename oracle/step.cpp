@@ -121,17 +121,57 @@ void scene_init(Scene& S) {
     S.kappaMax = 100 * S.kappa;
 }
 
+void step_dirichlet(Scene& S) {
+    const double h = S.P.dt;
+    for (const Scene::DBCMotion& M : S.dbcMotion) {
+        const double th = M.angVelDeg / 180 * M_PI * h;
+        const double c = std::cos(th), sn = std::sin(th);
+        const double* a = M.axis;
+        // rotation by th about axis a (Rodrigues, axis taken as given)
+        const double R[9] = {c + (1 - c) * a[0] * a[0], (1 - c) * a[0] * a[1] - sn * a[2], (1 - c) * a[0] * a[2] + sn * a[1],
+                             (1 - c) * a[1] * a[0] + sn * a[2], c + (1 - c) * a[1] * a[1], (1 - c) * a[1] * a[2] - sn * a[0],
+                             (1 - c) * a[2] * a[0] - sn * a[1], (1 - c) * a[2] * a[1] + sn * a[0], c + (1 - c) * a[2] * a[2]};
+        for (int i = M.begin; i < M.end; ++i) {
+            double* t = &S.dbcTarget[3 * i];
+            if (M.angVelDeg != 0) {
+                const double d[3] = {t[0] - M.center[0], t[1] - M.center[1], t[2] - M.center[2]};
+                for (int k = 0; k < 3; ++k) t[k] = R[3 * k] * d[0] + R[3 * k + 1] * d[1] + R[3 * k + 2] * d[2] + M.center[k];
+            }
+            for (int k = 0; k < 3; ++k) t[k] += M.v[k] * h;
+        }
+    }
+}
+
+// 1/2 k sum_DBC m |x - target|^2 (Shell/DIRICHLET.h:26-48)
+static double dbc_energy(const Scene& S, const double* x, double k) {
+    double E = 0;
+    for (size_t i = 0; i < S.dbcIdx.size(); ++i) {
+        const int v = S.dbcIdx[i];
+        double d2 = 0;
+        for (int c = 0; c < 3; ++c) d2 += (S.dbcTarget[3 * i + c] - x[3 * v + c]) * (S.dbcTarget[3 * i + c] - x[3 * v + c]);
+        E += d2 * S.mass[v];
+    }
+    return 0.5 * k * E;
+}
+static double dbc_dist2(const Scene& S, const double* x) {
+    double E = 0;
+    for (size_t i = 0; i < S.dbcIdx.size(); ++i)
+        for (int c = 0; c < 3; ++c) E += (S.dbcTarget[3 * i + c] - x[3 * S.dbcIdx[i] + c]) * (S.dbcTarget[3 * i + c] - x[3 * S.dbcIdx[i] + c]);
+    return E;
+}
+
 // full incremental potential at x for constraint set C (+ friction with the lagged set F)
 // (*invalid set when a strain limit is reached, INC_POTENTIAL.h:75-80)
 static double inc_potential(const Scene& S, const double* x, const double* xt, const std::vector<Constraint>& C,
                             double kappa, bool withContact, int* bad, const double* xn = nullptr,
-                            const FricSet* F = nullptr, bool* invalid = nullptr) {
+                            const FricSet* F = nullptr, bool* invalid = nullptr, double dbcStiff = 0) {
     double E = 0;
     bool ok = strain_limit_energy(S, x, &E);
     if (invalid) *invalid = !ok;
     E += elastic_energy(S, x) + inertia_energy(S, x, xt);
     if (withContact) E += barrier_energy(S, x, C, kappa, bad);
     if (F) E += friction_energy(S, x, xn, *F);
+    if (dbcStiff) E += dbc_energy(S, x, dbcStiff);
     return E;
 }
 
@@ -149,17 +189,26 @@ int advance_one_step(Scene& S, StepStats& st) {
             double a = S.mass[i] > 0 ? P.grav[k] : 0.0;
             xt[3 * i + k] = x[3 * i + k] + h * S.v[3 * i + k] + h * h * a;
         }
-    // Dirichlet targets, IMPLICIT_EULER.h:304-401
+    // Dirichlet targets, IMPLICIT_EULER.h:304-401: move the nodes if the CCD allows the
+    // whole motion, else pull them with a mass-weighted penalty (DBCStiff) that is raised
+    // until 99% of the distance is covered; fixed nodes (zero motion) stay projected
     std::vector<double> disp(n, 0.0);
+    std::vector<char> fixedDBC(S.nV, 0);
     for (size_t k = 0; k < S.dbcIdx.size(); ++k) {
         int v = S.dbcIdx[k];
         for (int d = 0; d < 3; ++d) disp[3 * v + d] = S.dbcTarget[3 * k + d] - x[3 * v + d];
+        if (!(disp[3 * v] || disp[3 * v + 1] || disp[3 * v + 2])) fixedDBC[v] = 1;
     }
     double dbcAlpha = 1;
     if (P.collision) dbcAlpha = ccd_stepsize(S, x.data(), disp.data(), 1.0);
-    if (dbcAlpha != 1) return -2;   // augmented-Lagrangian Dirichlet path is not covered
-    for (size_t k = 0; k < S.dbcIdx.size(); ++k)
-        for (int d = 0; d < 3; ++d) x[3 * S.dbcIdx[k] + d] = S.dbcTarget[3 * k + d];
+    double dbcStiff = 0, dbcPenXn = 0;
+    if (dbcAlpha == 1) {
+        for (size_t k = 0; k < S.dbcIdx.size(); ++k)
+            for (int d = 0; d < 3; ++d) x[3 * S.dbcIdx[k] + d] = S.dbcTarget[3 * k + d];
+    } else {
+        dbcStiff = 1e6;
+        dbcPenXn = dbc_dist2(S, xn.data());
+    }
 
     std::vector<Constraint> C, Cprev;
     std::vector<double> d2prev;
@@ -171,7 +220,7 @@ int advance_one_step(Scene& S, StepStats& st) {
     if (fric) friction_basis(S, x.data(), C, S.kappa, F);
     int bad = 0;
     bool invalid = false;
-    double Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, &invalid);
+    double Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, &invalid, dbcStiff);
     if (bad) return -1;
     if (invalid) return -6;   // strain limit violated by the initial state
     int fricIterI = 0;
@@ -191,7 +240,15 @@ int advance_one_step(Scene& S, StepStats& st) {
             for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
         if (P.collision) barrier_grad(S, x.data(), C, S.kappa, g.data());
         if (fric) friction_grad(S, x.data(), xn.data(), F, g.data());
-        for (int v : S.dbcIdx) for (int k = 0; k < 3; ++k) g[3 * v + k] = 0;
+        if (dbcStiff) {   // Shell/DIRICHLET.h:50-70; only fixed nodes projected
+            for (size_t i = 0; i < S.dbcIdx.size(); ++i) {
+                const int v = S.dbcIdx[i];
+                for (int k = 0; k < 3; ++k) g[3 * v + k] += dbcStiff * S.mass[v] * (x[3 * v + k] - S.dbcTarget[3 * i + k]);
+            }
+            for (int v : S.dbcIdx) if (fixedDBC[v]) for (int k = 0; k < 3; ++k) g[3 * v + k] = 0;
+        } else {
+            for (int v : S.dbcIdx) for (int k = 0; k < 3; ++k) g[3 * v + k] = 0;
+        }
         for (int i = 0; i < n; ++i) rhs[i] = -g[i];
         st.tGrad += now() - ta;
         // Hessian + solve, IMPLICIT_EULER.h:494-554
@@ -203,7 +260,13 @@ int advance_one_step(Scene& S, StepStats& st) {
             strain_limit_hess(S, x.data(), blocks);
             if (P.collision) barrier_hess(S, x.data(), C, S.kappa, blocks);
             if (fric) friction_hess(S, x.data(), xn.data(), F, blocks);
-            assemble(S, blocks, A);
+            if (dbcStiff)
+                for (int v : S.dbcIdx) {
+                    Block B{v, v, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
+                    B.b[0] = B.b[4] = B.b[8] = dbcStiff * S.mass[v];
+                    blocks.push_back(B);
+                }
+            assemble(S, blocks, A, dbcStiff ? &fixedDBC : nullptr);
             st.tHessian += now() - ta;
             ta = now();
             int rc;
@@ -251,7 +314,7 @@ int advance_one_step(Scene& S, StepStats& st) {
                 }
             }
             ta = now();
-            E = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, &invalid);
+            E = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, &invalid, dbcStiff);
             valid = !invalid;
             st.tEnergy += now() - ta;
             if (bad && valid) return -1;
@@ -267,7 +330,7 @@ int advance_one_step(Scene& S, StepStats& st) {
                 if (d2prev[i] < 1e-18 && cur[i] < d2prev[i]) { upd = true; break; }
             if (upd && S.kappa < S.kappaMax) {
                 S.kappa *= 2;
-                Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp);
+                Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, nullptr, dbcStiff);
             }
         }
         std::vector<double> d2;
@@ -293,6 +356,20 @@ int advance_one_step(Scene& S, StepStats& st) {
         }
         st.lastRes = cur;
         if (PNIter > 2000) return -4;
+        // penalty continuation, IMPLICIT_EULER.h:712-768
+        if (dbcStiff) {
+            const double progress = 1 - std::sqrt(dbc_dist2(S, x.data()) / dbcPenXn);
+            if (progress < 0.99) {
+                if (L2Norm < P.PNTol * 10 && dbcStiff < 1e8) {
+                    dbcStiff *= 2;
+                    Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, nullptr, dbcStiff);
+                }
+                L2Norm = P.PNTol * 10;
+            } else {
+                dbcStiff = 0;
+                Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, nullptr, 0.0);
+            }
+        }
         // friction re-lagged at a converged iterate, IMPLICIT_EULER.h:770-859: new basis and
         // normal forces, then the Newton direction of the updated potential decides
         if (fric && resRecord.size() == 3 && L2Norm <= P.PNTol) {
@@ -306,14 +383,28 @@ int advance_one_step(Scene& S, StepStats& st) {
                     for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
                 barrier_grad(S, x.data(), C, S.kappa, g.data());
                 friction_grad(S, x.data(), xn.data(), F, g.data());
-                for (int v : S.dbcIdx) for (int k = 0; k < 3; ++k) g[3 * v + k] = 0;
+                if (dbcStiff) {
+                    for (size_t i = 0; i < S.dbcIdx.size(); ++i) {
+                        const int v = S.dbcIdx[i];
+                        for (int k = 0; k < 3; ++k) g[3 * v + k] += dbcStiff * S.mass[v] * (x[3 * v + k] - S.dbcTarget[3 * i + k]);
+                    }
+                    for (int v : S.dbcIdx) if (fixedDBC[v]) for (int k = 0; k < 3; ++k) g[3 * v + k] = 0;
+                } else {
+                    for (int v : S.dbcIdx) for (int k = 0; k < 3; ++k) g[3 * v + k] = 0;
+                }
                 for (int i = 0; i < n; ++i) rhs[i] = -g[i];
                 std::vector<Block> blocks;
                 elastic_hess(S, x.data(), blocks);
                 strain_limit_hess(S, x.data(), blocks);
                 barrier_hess(S, x.data(), C, S.kappa, blocks);
                 friction_hess(S, x.data(), xn.data(), F, blocks);
-                assemble(S, blocks, A);
+                if (dbcStiff)
+                    for (int v : S.dbcIdx) {
+                        Block B{v, v, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
+                        B.b[0] = B.b[4] = B.b[8] = dbcStiff * S.mass[v];
+                        blocks.push_back(B);
+                    }
+                assemble(S, blocks, A, dbcStiff ? &fixedDBC : nullptr);
                 int rc;
                 if (P.solver == 0) rc = solve_direct(A, rhs.data(), sol.data());
                 else { int it = 0; rc = solve_pcg(A, rhs.data(), sol.data(), P.pcgTol, P.pcgMaxIter, &it, P.threads); st.pcgIters += it; }
@@ -321,7 +412,7 @@ int advance_one_step(Scene& S, StepStats& st) {
                 double s2 = 0;
                 for (int i = 0; i < n; ++i) s2 += sol[i] * sol[i];
                 L2Norm = std::sqrt(s2 / (S.nV - (int)S.dbcIdx.size())) / h;
-                if (L2Norm > P.PNTol) Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp);
+                if (L2Norm > P.PNTol) Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, nullptr, dbcStiff);
             }
         }
     } while (resRecord.size() < 3 || L2Norm > P.PNTol);
