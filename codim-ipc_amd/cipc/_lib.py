@@ -137,6 +137,11 @@ PROTOTYPES = {
     "cipc_ccd_stepsize": (ctypes.c_int, [P, D, D, ctypes.c_double, D, D]),
     "cipc_set_friction": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32]),
     "cipc_set_strain_limit": (ctypes.c_int, [P, D, D, D]),
+    "cipc_dirichlet_add_motion": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, D, D, D, ctypes.c_double]),
+    "cipc_step_dirichlet": (ctypes.c_int, [P]),
+    "cipc_turn_dirichlet": (ctypes.c_int, [P]),
+    "cipc_get_dirichlet_targets": (ctypes.c_int, [P, D]),
+    "cipc_set_dirichlet_targets": (ctypes.c_int, [P, D]),
     "cipc_strain_limit_energy": (ctypes.c_int, [P, D, D, I32]),
     "cipc_friction_basis": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, I32, D, ctypes.c_int32, I32]),
     "cipc_friction_energy": (ctypes.c_int, [P, D, D, D]),

@@ -90,6 +90,7 @@ class Simulator:
             raise CipcError(rc, "cipc_create failed")
         self.h = h
         self.nV = sd.nV
+        self.nDBC = len(sd.dbc_idx)
         s = np.zeros(6, dtype=np.int32)
         self.lib.cipc_get_sizes(self.h, _i(s))
         self.nT, self.nBN, self.nBE, self.nH = int(s[1]), int(s[2]), int(s[3]), int(s[4])
@@ -339,6 +340,28 @@ class Simulator:
         m = np.zeros(1)
         self._check(self.lib.cipc_min_dist2(self.h, _d(x), _i(C), len(C), _d(d2), _d(m)))
         return d2[:len(C)], m[0]
+
+    # ---- moving Dirichlet nodes (FEM/BOUNDARY_CONDITION.h)
+    def dirichlet_add_motion(self, begin, end, v=(0, 0, 0), center=(0, 0, 0), axis=(0, 1, 0), ang_vel_deg=0.0):
+        """A DBC_MOTION group over Dirichlet entries [begin, end)."""
+        arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (v, center, axis)]
+        self._check(self.lib.cipc_dirichlet_add_motion(self.h, int(begin), int(end), _d(arr[0]), _d(arr[1]), _d(arr[2]),
+                                                       float(ang_vel_deg)))
+
+    def step_dirichlet(self):
+        """Step_Dirichlet (BOUNDARY_CONDITION.h:107-155)."""
+        self._check(self.lib.cipc_step_dirichlet(self.h))
+
+    def turn_dirichlet(self):
+        self._check(self.lib.cipc_turn_dirichlet(self.h))
+
+    def dirichlet_targets(self):
+        t = np.zeros(3 * self.nDBC)
+        self._check(self.lib.cipc_get_dirichlet_targets(self.h, _d(t)))
+        return t.reshape(-1, 3)
+
+    def set_dirichlet_targets(self, t):
+        self._check(self.lib.cipc_set_dirichlet_targets(self.h, _d(np.ascontiguousarray(t, dtype=np.float64).ravel())))
 
     # ---- strain limiting (FEM/Shell/INEXT.h)
     def set_strain_limit(self, kappa, s=(1.01, 0.0), sHat=(1.0, 1.0)):

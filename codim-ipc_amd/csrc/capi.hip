@@ -376,6 +376,53 @@ int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_
     });
 }
 
+// ---- moving Dirichlet nodes
+int cipc_dirichlet_add_motion(cipc_ctx* ctx, int32_t begin, int32_t end, const double* v, const double* rot_center,
+                              const double* rot_axis, double ang_vel_deg) {
+    if (!ctx || !v || !rot_center || !rot_axis || begin < 0 || end < begin || end > ctx->c.nDBC) return CIPC_E_INVALID;
+    DBCMotion M;
+    M.begin = begin; M.end = end; M.angVelDeg = ang_vel_deg;
+    for (int k = 0; k < 3; ++k) { M.v[k] = v[k]; M.center[k] = rot_center[k]; M.axis[k] = rot_axis[k]; }
+    ctx->c.dbcMotion.push_back(M);
+    return CIPC_OK;
+}
+
+int cipc_step_dirichlet(cipc_ctx* ctx) {
+    if (!ctx) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() { step_dirichlet(ctx->c); return CIPC_OK; });
+}
+
+int cipc_turn_dirichlet(cipc_ctx* ctx) {
+    if (!ctx) return CIPC_E_INVALID;
+    for (DBCMotion& M : ctx->c.dbcMotion)
+        for (int k = 0; k < 3; ++k) M.v[k] = -M.v[k];
+    return CIPC_OK;
+}
+
+int cipc_get_dirichlet_targets(cipc_ctx* ctx, double* targets) {
+    if (!ctx || (!targets && ctx->c.nDBC)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        if (!c.nDBC) return CIPC_OK;
+        dirichlet_device_init(c);
+        CIPC_HIP(hipMemcpyAsync(targets, c.dbcTgt.p, 3l * c.nDBC * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        return CIPC_OK;
+    });
+}
+
+int cipc_set_dirichlet_targets(cipc_ctx* ctx, const double* targets) {
+    if (!ctx || (!targets && ctx->c.nDBC)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        if (!c.nDBC) return CIPC_OK;
+        dirichlet_device_init(c);
+        CIPC_HIP(hipMemcpyAsync(c.dbcTgt.p, targets, 3l * c.nDBC * sizeof(double), hipMemcpyHostToDevice, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        return CIPC_OK;
+    });
+}
+
 // ---- strain limiting
 int cipc_set_strain_limit(cipc_ctx* ctx, const double* kappa, const double* s, const double* shat) {
     if (!ctx || !kappa || !s || !shat || !strain_limit_args_ok(kappa, s, shat)) return CIPC_E_INVALID;

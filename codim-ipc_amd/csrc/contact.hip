@@ -978,7 +978,7 @@ void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa
         hipLaunchKernelGGL(k_barrier_hess_build, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, Hc, nvv);
         hipLaunchKernelGGL(k_barrier_hess_proj, dim3(nblocks(nC, 64)), dim3(64), 0, c.s, (const double*)Hc, (const int*)nvv, nC, hl, ns);
         CIPC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_hess_keys, dim3(nblocks(16l * nC)), dim3(kBlock), 0, c.s, C, nC, c.nV, c.dbc.p, key);
+        hipLaunchKernelGGL(k_hess_keys, dim3(nblocks(16l * nC)), dim3(kBlock), 0, c.s, C, nC, c.nV, proj_mask(c), key);
         CIPC_LAUNCH_CHECK();
     }
     if (nF) {
@@ -1000,7 +1000,7 @@ void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa
     A.col.ensure(std::max(nnzb, 1));
     A.val.ensure(9l * std::max(nnzb, 1));
     A.rowptr.ensure(c.nV + 1);
-    hipLaunchKernelGGL(k_bsr_sum, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, key, ns, c.nV, pos, slot, hl, ns, c.dbc.p, nnzb,
+    hipLaunchKernelGGL(k_bsr_sum, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, key, ns, c.nV, pos, slot, hl, ns, proj_mask(c), nnzb,
                        A.rowOf.p, A.col.p, A.val.p);
     CIPC_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_rowptr, dim3(nblocks(c.nV + 1)), dim3(kBlock), 0, c.s, A.rowOf.p, nnzb, c.nV, A.rowptr.p);

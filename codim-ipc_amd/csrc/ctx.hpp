@@ -54,6 +54,12 @@ struct BSR {
     DevBuf<double> val;
 };
 
+// Dirichlet motion group (FEM/BOUNDARY_CONDITION.h DBC_MOTION): entries [begin, end)
+struct DBCMotion {
+    int begin, end;
+    double v[3], center[3], axis[3], angVelDeg;
+};
+
 struct Ctx {
     hipStream_t s = nullptr;
     PrimWork pw;
@@ -135,6 +141,15 @@ struct Ctx {
     DevBuf<int> conPlanSlot;           // sorted slot order
     DevBuf<double> conGLocal;          // SoA [3][4 nC]
     DevBuf<int> conGVert;              // vertex of each gradient slot (4 nC; -1 unused)
+
+    // Dirichlet targets resident on the device, motion groups, and the penalty state of a
+    // step whose Dirichlet motion the CCD blocked (dbcStiff > 0: only dbcFixed rows projected)
+    bool dbcDevReady = false;
+    DevBuf<int> dbcIdxD;
+    DevBuf<double> dbcTgt;             // 3 per Dirichlet entry
+    std::vector<DBCMotion> dbcMotion;
+    DevBuf<uint8_t> dbcFixed;          // per vertex: Dirichlet with zero prescribed motion
+    double dbcStiff = 0;
 
     // lagged friction stencils (FEM/FRICTION.h): SoA [4][nF] encoding, SoA [9][nF] data
     // (closest point 2, tangent basis columns 2 x 3, normal force)
@@ -230,6 +245,17 @@ double friction_energy(Ctx& c, const double* x, const double* xn);
 void friction_gradient(Ctx& c, const double* x, const double* xn, double* g);   // g +=
 void friction_hess_local(Ctx& c, const double* x, const double* xn, double* hl, long ns, long off);
 void friction_hess_keys(Ctx& c, uint64_t* key);
+// rows/columns projected out of the Newton system: every Dirichlet node, or only the
+// fixed ones while the Dirichlet penalty is on
+inline const uint8_t* proj_mask(const Ctx& c) { return c.dbcStiff ? c.dbcFixed.p : c.dbc.p; }
+// moving Dirichlet nodes (dirichlet.hip)
+void dirichlet_device_init(Ctx& c);
+void step_dirichlet(Ctx& c);
+double dbc_dist2(Ctx& c, const double* x);
+double dbc_penalty_energy(Ctx& c, const double* x);
+void dbc_penalty_grad(Ctx& c, const double* x, double* g);   // g +=
+void dbc_penalty_hess(Ctx& c);                               // diagonal blocks of Ael +=
+void dbc_fixed_mask(Ctx& c, const double* x);
 // strain limiting (strain.hip): energy (+inf when a limit is reached), gradient and
 // projected Hessian added to the membrane's local triangle slots
 bool strain_limit_on(const Ctx& c);

@@ -511,7 +511,7 @@ void elastic_hessian(Ctx& c, const double* x) {
     }
     BSR& A = c.Ael;
     hipLaunchKernelGGL(k_el_assemble, dim3(nblocks(A.nnzb)), dim3(kBlock), 0, c.s, hl, ns, c.planOff.p, c.planSlot.p, A.rowOf.p,
-                       A.col.p, A.nnzb, c.mass.p, c.dbc.p, A.val.p);
+                       A.col.p, A.nnzb, c.mass.p, proj_mask(c), A.val.p);
     CIPC_LAUNCH_CHECK();
 }
 

@@ -318,7 +318,7 @@ void friction_hess_local(Ctx& c, const double* x, const double* xn, double* hl, 
 
 void friction_hess_keys(Ctx& c, uint64_t* key) {
     if (c.nF == 0) return;
-    hipLaunchKernelGGL(k_fric_hess_keys, dim3(nblocks(16l * c.nF)), dim3(kBlock), 0, c.s, c.fC.p, c.nF, c.nV, c.dbc.p, key);
+    hipLaunchKernelGGL(k_fric_hess_keys, dim3(nblocks(16l * c.nF)), dim3(kBlock), 0, c.s, c.fC.p, c.nF, c.nV, proj_mask(c), key);
     CIPC_LAUNCH_CHECK();
 }
 

@@ -1019,7 +1019,7 @@ static void features(Ctx& c) {
     if (K.nnzb == 0) K.rowptr.ensure(c.nV + 1);
     hipLaunchKernelGGL(k_lp_node_features, dim3(nblocks(lp.nV)), dim3(kBlock), 0, c.s, lp.nV, (const int*)lp.diagBlk.p, (const double*)E.val.p,
                        E.nnzb, (const int*)K.rowptr.p, (const int*)K.col.p, (const double*)K.val.p, K.nnzb, (const double*)c.mass.p,
-                       (const uint8_t*)c.dbc.p, lp.S.p, lp.nodeF.p);
+                       proj_mask(c), lp.S.p, lp.nodeF.p);
     if (lp.nE > 0)
         hipLaunchKernelGGL(k_lp_edge_features, dim3(nblocks(lp.nE)), dim3(kBlock), 0, c.s, lp.nE, (const int*)lp.eBlock.p, (const int*)lp.eSrc.p,
                            (const int*)lp.eDst.p, (const double*)E.val.p, E.nnzb, (const int*)K.rowptr.p, (const int*)K.col.p,
@@ -1108,7 +1108,7 @@ bool lp_refresh(Ctx& c) {
     const int nL = lp.nL, nV = lp.nV;
     hipLaunchKernelGGL(k_lp_node_features, dim3(nblocks(nV)), dim3(kBlock), 0, c.s, nV, (const int*)lp.diagBlk.p, (const double*)E.val.p,
                        E.nnzb, (const int*)K.rowptr.p, (const int*)K.col.p, (const double*)K.val.p, K.nnzb, (const double*)c.mass.p,
-                       (const uint8_t*)c.dbc.p, lp.S.p, lp.nodeF.p);
+                       proj_mask(c), lp.S.p, lp.nodeF.p);
     hipLaunchKernelGGL(k_lp_assemble_G, dim3(nblocks(nL + nV)), dim3(kBlock), 0, c.s, nL, nV, lp.nnzb, (const int*)lp.lEdge.p,
                        (const int*)lp.eBlock.p, (const int*)lp.blkT.p, (const int*)lp.diagBlk.p, (const int*)c.Ael.col.p,
                        (const double*)lp.S.p, (const float*)lp.Gl.p, (const float*)lp.raw.p, lp.Gblk.p, lp.nLow,

@@ -4,8 +4,9 @@
 // (FEM/Shell/DISCRETE_SHELL.h:218-357), surface primitive extraction
 // (Utils/MESHIO.h:728-835) and Initialize_EIPC<double,false> (DISCRETE_SHELL.h:555-577).
 // advance_one_step restates Advance_One_Step_IE_Discrete_Shell<double,3,false,false>
-// (FEM/Shell/IMPLICIT_EULER.h:151-902) and Line_Search (:9-149) for static or
-// CCD-unblocked Dirichlet nodes with isotropic strain limiting (INEXT.h) and lagged
+// (FEM/Shell/IMPLICIT_EULER.h:151-902) and Line_Search (:9-149) with moving Dirichlet
+// nodes (the penalty continuation when the CCD blocks their motion), isotropic strain
+// limiting (INEXT.h) and lagged
 // friction (basis at the step start, re-lagged after convergence up to fricIterAmt
 // times, IMPLICIT_EULER.h:412-437, 770-859);
 // every array operation runs on the device, the host only sequences the stages and
@@ -353,11 +354,16 @@ static double inc_potential(Ctx& c, const double* x, const int* C, int nC, int* 
     if (c.P.collision) E += barrier_energy(c, x, C, nC, c.kappa, bad);
     else if (bad) *bad = 0;
     if (c.fricOn) E += friction_energy(c, x, c.xn.p);   // IMPLICIT_EULER.h:436
+    if (c.dbcStiff) E += dbc_penalty_energy(c, x);      // IMPLICIT_EULER.h:438-440
     return E;
 }
 
 // ------------------------------------------------------------------ the step
 int advance_one_step(Ctx& c) {
+    struct PenaltyOff {   // the Dirichlet penalty never outlives a step (early returns included)
+        Ctx& c;
+        ~PenaltyOff() { c.dbcStiff = 0; }
+    } penaltyOff{c};
     const Params& P = c.P;
     const long n = 3l * c.nV;
     const double h = P.dt;
@@ -367,32 +373,34 @@ int advance_one_step(Ctx& c) {
     c.lpOffThisStep = false;
     double t0 = now_ms(), ta;
     hipStream_t s = c.s;
-    DevBuf<int>& dbcIdxD = c.i32c;
     static_assert(sizeof(int) == 4, "");
     // x_n, x_tilde (IMPLICIT_EULER.h:195-217)
     CIPC_HIP(hipMemcpyAsync(c.xn.p, c.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_predict, dim3(nblocks(c.nV)), dim3(kBlock), 0, s, c.nV, c.x.p, c.v.p, c.mass.p, h, P.grav[0], P.grav[1], P.grav[2],
                        c.xt.p);
     CIPC_LAUNCH_CHECK();
-    // Dirichlet motion and its CCD (IMPLICIT_EULER.h:304-401)
+    // Dirichlet motion and its CCD (IMPLICIT_EULER.h:304-401): move the nodes when the CCD
+    // allows the whole displacement, else keep them and pull them with the penalty
+    c.dbcStiff = 0;
+    double dbcPenXn = 0;
     if (c.nDBC) {
-        dbcIdxD.ensure(c.nDBC);
-        CIPC_HIP(hipMemcpyAsync(dbcIdxD.p, c.hDbcIdx.data(), c.nDBC * sizeof(int), hipMemcpyHostToDevice, s));
-        double* tgt = c.dscr.ensure(3l * c.nDBC);
-        CIPC_HIP(hipMemcpyAsync(tgt, c.hDbcTarget.data(), 3l * c.nDBC * sizeof(double), hipMemcpyHostToDevice, s));
+        dirichlet_device_init(c);
         CIPC_HIP(hipMemsetAsync(c.p.p, 0, n * sizeof(double), s));
-        hipLaunchKernelGGL(k_dbc_disp, dim3(nblocks(c.nDBC)), dim3(kBlock), 0, s, c.nDBC, dbcIdxD.p, tgt, c.x.p, c.p.p);
+        hipLaunchKernelGGL(k_dbc_disp, dim3(nblocks(c.nDBC)), dim3(kBlock), 0, s, c.nDBC, c.dbcIdxD.p, c.dbcTgt.p, c.x.p, c.p.p);
         CIPC_LAUNCH_CHECK();
+        dbc_fixed_mask(c, c.x.p);
         double dbcAlpha = 1;
         if (P.collision) {
             int z = 0;
             dbcAlpha = ccd_stepsize(c, c.x.p, c.p.p, 1.0, nullptr, &z);
         }
-        if (dbcAlpha != 1) { c.lastError = "Dirichlet motion blocked by contact (augmented Lagrangian path not covered)"; return CIPC_E_UNSUPPORTED; }
-        tgt = c.dscr.ensure(3l * c.nDBC);
-        CIPC_HIP(hipMemcpyAsync(tgt, c.hDbcTarget.data(), 3l * c.nDBC * sizeof(double), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_dbc_set, dim3(nblocks(c.nDBC)), dim3(kBlock), 0, s, c.nDBC, dbcIdxD.p, tgt, c.x.p);
-        CIPC_LAUNCH_CHECK();
+        if (dbcAlpha == 1) {
+            hipLaunchKernelGGL(k_dbc_set, dim3(nblocks(c.nDBC)), dim3(kBlock), 0, s, c.nDBC, c.dbcIdxD.p, c.dbcTgt.p, c.x.p);
+            CIPC_LAUNCH_CHECK();
+        } else {
+            c.dbcStiff = 1e6;
+            dbcPenXn = dbc_dist2(c, c.xn.p);
+        }
     }
     c.nC = 0;
     c.nCprev = 0;
@@ -421,7 +429,8 @@ int advance_one_step(Ctx& c) {
         elastic_gradient(c, c.x.p, c.g.p);
         if (P.collision) barrier_gradient(c, c.x.p, c.C.p, c.nC, c.kappa, c.g.p);
         if (c.fricOn) friction_gradient(c, c.x.p, c.xn.p, c.g.p);
-        hipLaunchKernelGGL(k_rhs, dim3(nblocks(c.nV)), dim3(kBlock), 0, s, c.nV, c.g.p, c.dbc.p, c.rhs.p);
+        dbc_penalty_grad(c, c.x.p, c.g.p);   // IMPLICIT_EULER.h:467-475
+        hipLaunchKernelGGL(k_rhs, dim3(nblocks(c.nV)), dim3(kBlock), 0, s, c.nV, c.g.p, proj_mask(c), c.rhs.p);
         CIPC_LAUNCH_CHECK();
         CIPC_HIP(hipStreamSynchronize(s));
         st.t_gradient_ms += now_ms() - ta;
@@ -429,6 +438,7 @@ int advance_one_step(Ctx& c) {
         if (!useGD) {
             ta = now_ms();
             elastic_hessian(c, c.x.p);
+            dbc_penalty_hess(c);   // INC_POTENTIAL.h:379-381
             if (P.collision) barrier_hessian(c, c.x.p, c.C.p, c.nC, c.kappa, c.fricOn ? c.xn.p : nullptr);
             else c.Acon.nnzb = 0;
             CIPC_HIP(hipStreamSynchronize(s));
@@ -544,6 +554,20 @@ int advance_one_step(Ctx& c) {
         }
         st.last_residual = cur;
         if (PNIter > 2000) { c.lastError = "Newton iteration limit"; return CIPC_E_NO_CONVERGENCE; }
+        // Dirichlet penalty continuation (IMPLICIT_EULER.h:712-768)
+        if (c.dbcStiff) {
+            const double progress = 1 - std::sqrt(dbc_dist2(c, c.x.p) / dbcPenXn);
+            if (progress < 0.99) {
+                if (L2Norm < P.PNTol * 10 && c.dbcStiff < 1e8) {
+                    c.dbcStiff *= 2;
+                    Eprev = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
+                }
+                L2Norm = P.PNTol * 10;
+            } else {
+                c.dbcStiff = 0;
+                Eprev = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
+            }
+        }
         // friction re-lagged at a converged iterate (IMPLICIT_EULER.h:770-859): new basis
         // and normal forces; the Newton direction of the updated potential decides whether
         // the loop goes on.  The reference recomputes that same direction at the top of

@@ -143,6 +143,22 @@ int cipc_hessian(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs
                  int32_t* brow, int32_t* bcol, double* bval, int32_t cap, int32_t* nblocks);
 /* Compute_Min_Dist2 (IPC.h:2246-2388): per-constraint d^2 (no offset) and min(d^2) - offset^2 */
 int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double* d2, double* min_val);
+/* Moving Dirichlet nodes (FEM/BOUNDARY_CONDITION.h).  Targets live in the context and
+ * start at cipc_scene_desc.dbc_target; a step moves each Dirichlet node to its target
+ * when the CCD allows the whole motion, else it pulls the nodes there with the
+ * mass-weighted penalty of IMPLICIT_EULER.h:384-401, 712-768 (raised until 99% of the way
+ * is covered).  cipc_dirichlet_add_motion appends a DBC_MOTION group over Dirichlet
+ * entries [begin, end) (Init_Dirichlet's range, BOUNDARY_CONDITION.h:21-104);
+ * cipc_step_dirichlet is Step_Dirichlet (:107-155: rotation by ang_vel_deg * dt about
+ * rot_axis through rot_center, then translation by v * dt); cipc_turn_dirichlet is
+ * Turn_Dirichlet (:211-220); get/set read or replace the 3 n_dbc targets (Load_Dirichlet,
+ * Reset_Dirichlet with the caller's arrays). */
+int cipc_dirichlet_add_motion(cipc_ctx* ctx, int32_t begin, int32_t end, const double* v, const double* rot_center,
+                              const double* rot_axis, double ang_vel_deg);
+int cipc_step_dirichlet(cipc_ctx* ctx);
+int cipc_turn_dirichlet(cipc_ctx* ctx);
+int cipc_get_dirichlet_targets(cipc_ctx* ctx, double* targets);
+int cipc_set_dirichlet_targets(cipc_ctx* ctx, const double* targets);
 /* Strain-limit parameters for the following steps (kappa_s, s, sHat of
  * Advance_One_Step_IE_Hinge, IMPLICIT_EULER.h:162); same meaning as in cipc_scene_desc. */
 int cipc_set_strain_limit(cipc_ctx* ctx, const double* kappa, const double* s, const double* shat);
