@@ -139,6 +139,7 @@ PROTOTYPES = {
     "cipc_set_strain_limit": (ctypes.c_int, [P, D, D, D]),
     "cipc_dirichlet_add_motion": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, D, D, D, ctypes.c_double]),
     "cipc_step_dirichlet": (ctypes.c_int, [P]),
+    "cipc_write_obj": (ctypes.c_int, [P, ctypes.c_char_p]),
     "cipc_turn_dirichlet": (ctypes.c_int, [P]),
     "cipc_get_dirichlet_targets": (ctypes.c_int, [P, D]),
     "cipc_set_dirichlet_targets": (ctypes.c_int, [P, D]),

@@ -341,6 +341,10 @@ class Simulator:
         self._check(self.lib.cipc_min_dist2(self.h, _d(x), _i(C), len(C), _d(d2), _d(m)))
         return d2[:len(C)], m[0]
 
+    def write_obj(self, path):
+        """Write_TriMesh_Obj (MESHIO.h:184-213) of the current positions."""
+        self._check(self.lib.cipc_write_obj(self.h, str(path).encode()))
+
     # ---- moving Dirichlet nodes (FEM/BOUNDARY_CONDITION.h)
     def dirichlet_add_motion(self, begin, end, v=(0, 0, 0), center=(0, 0, 0), axis=(0, 1, 0), ang_vel_deg=0.0):
         """A DBC_MOTION group over Dirichlet entries [begin, end)."""

@@ -1,6 +1,7 @@
 // C ABI (include/cipc.h).  Every entry point converts HIP failures and invalid
 // arguments into negative status codes and records a message for cipc_last_error().
 #include "ctx.hpp"
+#include <cstdio>
 #include <cmath>
 #include "learn.hpp"
 #include <algorithm>
@@ -372,6 +373,24 @@ int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_
             CIPC_HIP(hipMemcpyAsync(d2, dd, n_cs * sizeof(double), hipMemcpyDeviceToHost, c.s));
             CIPC_HIP(hipStreamSynchronize(c.s));
         }
+        return CIPC_OK;
+    });
+}
+
+// ---- frame output
+int cipc_write_obj(cipc_ctx* ctx, const char* path) {
+    if (!ctx || !path) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        std::vector<double> x(3 * (size_t)c.nV);
+        CIPC_HIP(hipMemcpyAsync(x.data(), c.x.p, x.size() * sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        FILE* f = std::fopen(path, "w");
+        if (!f) { c.lastError = std::string("cannot open ") + path; return CIPC_E_INVALID; }
+        for (int i = 0; i < c.nV; ++i) std::fprintf(f, "v %le %le %le\n", x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        for (int t = 0; t < c.nT; ++t) std::fprintf(f, "f %d %d %d\n", c.hTri[3 * t] + 1, c.hTri[3 * t + 1] + 1, c.hTri[3 * t + 2] + 1);
+        const bool ok = std::fclose(f) == 0;
+        if (!ok) { c.lastError = std::string("write failed: ") + path; return CIPC_E_INVALID; }
         return CIPC_OK;
     });
 }

@@ -143,6 +143,9 @@ int cipc_hessian(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs
                  int32_t* brow, int32_t* bcol, double* bval, int32_t cap, int32_t* nblocks);
 /* Compute_Min_Dist2 (IPC.h:2246-2388): per-constraint d^2 (no offset) and min(d^2) - offset^2 */
 int cipc_min_dist2(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double* d2, double* min_val);
+/* Write_TriMesh_Obj (Utils/MESHIO.h:184-213): the current positions ("v %le %le %le") and
+ * the shell triangles kept at set-up ("f i j k", 1-based) to an OBJ file. */
+int cipc_write_obj(cipc_ctx* ctx, const char* path);
 /* Moving Dirichlet nodes (FEM/BOUNDARY_CONDITION.h).  Targets live in the context and
  * start at cipc_scene_desc.dbc_target; a step moves each Dirichlet node to its target
  * when the CCD allows the whole motion, else it pulls the nodes there with the

@@ -86,3 +86,20 @@ def test_blocked_motion_penalty(require_gpu):
 
     x = _run(sd, (4, 7, (0.0, -0.3, 0.0)), 3, check)
     assert x.reshape(-1, 3)[: sd.n_cloth, 1].min() < -1e-3
+
+
+def test_write_obj(require_gpu, tmp_path):
+    """Write_TriMesh_Obj of the current state: %le vertices, 1-based faces"""
+    sd = pinned_sheet(pusher_gap=0.01)
+    gpu = Simulator(sd)
+    gpu.step()
+    x, _ = gpu.get_state()
+    path = tmp_path / "frame.obj"
+    gpu.write_obj(path)
+    V, F = [], []
+    for line in path.read_text().splitlines():
+        tag, *vals = line.split()
+        (V if tag == "v" else F).append([float(v) for v in vals] if tag == "v" else [int(v) for v in vals])
+    assert np.allclose(np.array(V), x.reshape(-1, 3), rtol=1e-6, atol=1e-12)
+    assert np.array_equal(np.array(F) - 1, sd.tris)
+    gpu.close()
