@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--pcg-tol", type=float, default=1e-5)
     ap.add_argument("--mu", type=float, default=0.0,
                     help="friction coefficient (lagged friction, epsv2 = 1e-6, basis once per step); 0: frictionless")
+    ap.add_argument("--strain-limit", type=float, default=0.0,
+                    help="isotropic strain limit s (kappa_s = 1e3, sHat = 1); 0: off")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -232,6 +234,8 @@ def main():
     sd = {"c1": scenes.scene_c1, "c2": scenes.scene_c2, "c3": scenes.scene_c3}[args.workload]()
     if args.mu > 0:
         sd.params.update(mu=args.mu, epsv2=1e-6, fricIterAmt=1)
+    if args.strain_limit > 0:
+        sd.params.update(kappa_s=(1e3, 0.0), s=(args.strain_limit, 0.0), sHat=(1.0, 1.0))
 
     modes = ["fp64", "mixed", "fp32"] if args.precision == "all" else [args.precision]
     if args.precision == "all":
@@ -263,7 +267,7 @@ def main():
         "config": {"workload": sd.name, "vertices": sd.nV, "triangles": sd.nT, "cloth_vertices": sd.n_cloth,
                    "rod_segments": 0 if sd.rods is None else len(sd.rods), "particles": 0 if sd.particles is None else len(sd.particles),
                    "steps_simulated_per_form": args.warmup + args.steps, "dt": sd.params["dt"], "dhat": sd.params["dHat"],
-                   "pcg_tol": args.pcg_tol, "friction_mu": args.mu},
+                   "pcg_tol": args.pcg_tol, "friction_mu": args.mu, "strain_limit": args.strain_limit},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
         "forms": {m: {k: (round(v, 4) if isinstance(v, float) and k != "min_dist_final" else v) for k, v in r.items() if k != "roofline"}
