@@ -134,3 +134,31 @@ def test_sliding_sheet_momentum_gpu(require_gpu):
     pred = -F / h
     assert pred[0] < 0
     assert abs(dP[0] - pred[0]) <= 1e-4 * abs(pred[0])
+
+
+def test_friction_codim_multiplicity(require_gpu):
+    """rods and particles on the sheet (offset 1 mm): point-edge and point-point friction
+    stencils with multiplicity; basis, energy and gradient against the oracle, then
+    three steps of the falling codimensional mix with mu = 0.3"""
+    sd = scenes.scene_codim_contact()
+    sd.params.update(mu=0.3, epsv2=1e-6)
+    gpu = Simulator(sd)
+    orc = O.OracleScene(sd, threads=4)
+    x, _ = orc.get_state()
+    C = orc.constraint_set(x)
+    kappa = orc.scalars()["kappa"]
+    FCg, FIg = gpu.friction_basis(x, C, kappa)
+    FCo, FIo = orc.friction_basis(x, C, kappa)
+    assert np.array_equal(FCg, FCo)
+    assert ((FCo[:, 0] < 0) & (FCo[:, 3] < -1)).any()        # multiplicity > 1 present
+    for k in range(9):
+        assert rel(FIg[:, k], FIo[:, k]) <= 1e-12, k
+    ev = np.sqrt(1e-6) * sd.params["dt"]
+    xx = x + np.random.default_rng(3).standard_normal(x.size) * ev * 3.0
+    assert abs(gpu.friction_energy(xx, x) - orc.friction_energy(xx, x, FCo, FIo)) <= 1e-11 * abs(orc.friction_energy(xx, x, FCo, FIo))
+    assert rel(gpu.friction_gradient(xx, x), orc.friction_gradient(xx, x, FCo, FIo)) <= 1e-11
+    gpu.close()
+    sd2 = scenes.scene_codim_small()
+    sd2.params.update(mu=0.3, epsv2=1e-6)
+    sd2.params["PNTol"] = 1e-6 * sd2.bbox_diag()
+    _trajectory(sd2, 3)
