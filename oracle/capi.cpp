@@ -212,6 +212,15 @@ void orc_set_strain_limit(void* h, const double* kappa, const double* s, const d
     Params& P = ((Scene*)h)->P;
     for (int i = 0; i < 2; ++i) { P.slKappa[i] = kappa[i]; P.slS[i] = s[i]; P.slSHat[i] = sHat[i]; }
 }
+// fibre model: k (4) = fiberStiffMult, lim (3) = inextLimit
+void orc_set_fiber(void* h, const double* k, const double* lim) {
+    Params& P = ((Scene*)h)->P;
+    for (int i = 0; i < 4; ++i) P.fiberK[i] = k[i];
+    for (int i = 0; i < 3; ++i) P.fiberLimit[i] = lim[i];
+}
+int orc_fiber_tri(void* h, const double* x, int t, int proj, double* E, double* g, double* H, double* E3) {
+    return fiber_tri(*(Scene*)h, x, t, E, g, H, proj != 0, E3) ? 1 : 0;
+}
 // one triangle: returns 1 if valid; E, g (9), H (81, projected if proj), singular values (2)
 int orc_strain_tri(void* h, const double* x, int t, int proj, double* E, double* g, double* H, double* sig) {
     return strain_limit_tri(*(Scene*)h, x, t, E, g, H, proj != 0, sig) ? 1 : 0;
@@ -283,7 +292,11 @@ static CSR g_lastA;
 int orc_hessian(void* h, const double* x, const int* c, int nC, double kappa, int which) {
     Scene& S = *(Scene*)h;
     std::vector<Block> blocks;
-    if (which & 1) { elastic_hess(S, x, blocks); strain_limit_hess(S, x, blocks); }
+    if (which & 1) {
+        elastic_hess(S, x, blocks);
+        strain_limit_hess(S, x, blocks);
+        if (fiber_active(S)) fiber_hess(S, x, blocks);
+    }
     if (which & 2) barrier_hess(S, x, toC(c, nC), kappa, blocks);
     assemble(S, blocks, g_lastA);
     return (int)g_lastA.val.size();

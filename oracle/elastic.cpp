@@ -27,7 +27,8 @@ static inline bool hinge_all_dbc(const Scene& S, int h) {
 double elastic_energy(const Scene& S, const double* x, double* membOut, double* bendOut) {
     const double h2 = S.P.dt * S.P.dt;
     double Em = 0, Eb = 0;
-    for (int t = 0; t < S.nT; ++t) {
+    const bool memb = !fiber_active(S);   // the fibre model replaces the membrane (INC_POTENTIAL.h:81-92)
+    for (int t = 0; t < S.nT && memb; ++t) {
         if (tri_all_dbc(S, t)) continue;
         const int* v = &S.tri[3 * t];
         Em += h2 * membrane_energy(X3(x, v[0]), X3(x, v[1]), X3(x, v[2]), &S.Binv[3 * t], S.vol[t], S.lam, S.mu);
@@ -50,7 +51,8 @@ double elastic_energy(const Scene& S, const double* x, double* membOut, double* 
 
 void elastic_grad(const Scene& S, const double* x, double* g) {
     const double h2 = S.P.dt * S.P.dt;
-    for (int t = 0; t < S.nT; ++t) {
+    const bool memb = !fiber_active(S);
+    for (int t = 0; t < S.nT && memb; ++t) {
         if (tri_all_dbc(S, t)) continue;
         const int* v = &S.tri[3 * t];
         double gl[9];
@@ -81,12 +83,13 @@ void elastic_grad(const Scene& S, const double* x, double* g) {
 void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
     const double h2 = S.P.dt * S.P.dt;
     std::vector<long> offT(S.nT + 1, 0);
-    for (int t = 0; t < S.nT; ++t) offT[t + 1] = offT[t] + (tri_all_dbc(S, t) ? 0 : 9);
+    const bool memb = !fiber_active(S);
+    for (int t = 0; t < S.nT; ++t) offT[t + 1] = offT[t] + ((tri_all_dbc(S, t) || !memb) ? 0 : 9);
     const long base = (long)out.size();
     out.resize(base + offT[S.nT]);
 #pragma omp parallel for schedule(static) num_threads(S.P.threads)
     for (int t = 0; t < S.nT; ++t) {
-        if (tri_all_dbc(S, t)) continue;
+        if (tri_all_dbc(S, t) || !memb) continue;
         const int* v = &S.tri[3 * t];
         double H[81];
         membrane_hess(X3(x, v[0]), X3(x, v[1]), X3(x, v[2]), &S.Binv[3 * t], S.vol[t], S.lam, S.mu, H);

@@ -108,6 +108,8 @@ struct Params {
     int fricIterAmt = 1;
     // strain limiting (the step's kappa_s, s, sHat; upper limit index 0, lower 1)
     double slKappa[2] = {0, 0}, slS[2] = {1.01, 0}, slSHat[2] = {1, 1};
+    // anisotropic (fibre) membrane: the step's fiberStiffMult (4) and inextLimit (3)
+    double fiberK[4] = {0, 0, 0, 0}, fiberLimit[3] = {1, 1, 1};
     int solver = 0;            // 0 = direct (sparse Cholesky), 1 = block-Jacobi PCG
     double pcgTol = 1e-5;
     int pcgMaxIter = 100000;
@@ -141,6 +143,7 @@ struct Scene {
     double kappa = 0, kappaMax = 0, dHat2 = 0;
     double bboxDiag = 0;
     std::vector<double> slIB;          // strain limiting: B^-1 of the rest metric (00, 01, 11 per tri)
+    std::vector<double> fiberIB;       // fibre model: B^-1 of the rest x-z coordinates (00, 01, 10, 11 per tri)
     // Dirichlet motion groups (FEM/BOUNDARY_CONDITION.h DBC_MOTION): entries [begin, end)
     struct DBCMotion { int begin, end; double v[3], center[3], axis[3], angVelDeg; };
     std::vector<DBCMotion> dbcMotion;
@@ -195,6 +198,14 @@ bool strain_limit_energy(const Scene& S, const double* x, double* E);
 void strain_limit_grad(const Scene& S, const double* x, double* g);
 void strain_limit_hess(const Scene& S, const double* x, std::vector<Block>& out);
 bool strain_limit_tri(const Scene& S, const double* x, int t, double* E, double* g9, double* H81, bool proj, double* sig);
+// anisotropic (fibre) membrane (FEM/Shell/ANISO_INEXT.h), replaces the Neo-Hookean membrane
+// when fiberK[0] > 0 or fiberK[1] > 0; energy adds to *E and returns false when infeasible
+void fiber_setup(Scene& S);
+bool fiber_active(const Scene& S);
+bool fiber_energy(const Scene& S, const double* x, double* E);
+void fiber_grad(const Scene& S, const double* x, double* g);
+void fiber_hess(const Scene& S, const double* x, std::vector<Block>& out);
+bool fiber_tri(const Scene& S, const double* x, int t, double* E, double* g9, double* H81, bool proj, double* E3);
 
 // assembled system in CSR (scalar) form
 struct CSR { int n = 0; std::vector<int> rp, ci; std::vector<double> val; };

@@ -51,6 +51,8 @@ def lib():
         L.orc_solve.argtypes = [ctypes.c_int, I, I, D, D, D, ctypes.c_int, ctypes.c_double, I]
         L.orc_set_friction.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_set_strain_limit.argtypes = [ctypes.c_void_p, D, D, D]
+        L.orc_set_fiber.argtypes = [ctypes.c_void_p, D, D]
+        L.orc_fiber_tri.argtypes = [ctypes.c_void_p, D, ctypes.c_int, ctypes.c_int, D, D, D, D]
         L.orc_dirichlet_add_motion.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, D, D, D, ctypes.c_double]
         L.orc_step_dirichlet.argtypes = [ctypes.c_void_p]
         L.orc_get_dirichlet_targets.argtypes = [ctypes.c_void_p, D]
@@ -211,6 +213,9 @@ class OracleScene:
         self._sl = [np.array(P.get(k, d), dtype=np.float64) for k, d in
                     (("kappa_s", (0.0, 0.0)), ("s", (1.01, 0.0)), ("sHat", (1.0, 1.0)))]
         L.orc_set_strain_limit(self.h, _dp(self._sl[0]), _dp(self._sl[1]), _dp(self._sl[2]))
+        self._fb = [np.array(P.get("fiberStiffMult", (0.0, 0.0, 0.0, 0.0)), dtype=np.float64),
+                    np.array(P.get("inextLimit", (1.0, 1.0, 1.0)), dtype=np.float64)]
+        L.orc_set_fiber(self.h, _dp(self._fb[0]), _dp(self._fb[1]))
 
     def __del__(self):
         try:
@@ -335,6 +340,13 @@ class OracleScene:
         E, g, H, sg = np.zeros(1), np.zeros(9), np.zeros(81), np.zeros(2)
         ok = lib().orc_strain_tri(self.h, _dp(x), int(t), int(proj), _dp(E), _dp(g), _dp(H), _dp(sg))
         return bool(ok), E[0], g, H.reshape(9, 9), sg
+
+    def fiber_tri(self, x, t, proj=True):
+        """(feasible, E, g (9), H (9, 9), Green strain (E00, E11, E01)) of triangle t."""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        E, g, H, e3 = np.zeros(1), np.zeros(9), np.zeros(81), np.zeros(3)
+        ok = lib().orc_fiber_tri(self.h, _dp(x), int(t), int(proj), _dp(E), _dp(g), _dp(H), _dp(e3))
+        return bool(ok), E[0], g, H.reshape(9, 9), e3
 
     def strain_energy_grad(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).ravel()

@@ -111,6 +111,7 @@ void scene_init(Scene& S) {
     // rods and particles (initialize_rod / initialize_particle run before initialize_OIPC)
     codim_init(S);
     strain_limit_setup(S);
+    fiber_setup(S);
     // Initialize_EIPC<double,false>, DISCRETE_SHELL.h:563-574
     S.dHat2 = P.dHat * P.dHat;
     double Hb = barrier_h(1.0e-16, S.dHat2, 1.0);
@@ -167,6 +168,7 @@ static double inc_potential(const Scene& S, const double* x, const double* xt, c
                             const FricSet* F = nullptr, bool* invalid = nullptr, double dbcStiff = 0) {
     double E = 0;
     bool ok = strain_limit_energy(S, x, &E);
+    if (fiber_active(S)) ok = fiber_energy(S, x, &E) && ok;   // Check_Fiber_Feasibility, INC_POTENTIAL.h:81-88
     if (invalid) *invalid = !ok;
     E += elastic_energy(S, x) + inertia_energy(S, x, xt);
     if (withContact) E += barrier_energy(S, x, C, kappa, bad);
@@ -236,6 +238,7 @@ int advance_one_step(Scene& S, StepStats& st) {
         std::fill(g.begin(), g.end(), 0.0);
         elastic_grad(S, x.data(), g.data());
         strain_limit_grad(S, x.data(), g.data());
+        if (fiber_active(S)) fiber_grad(S, x.data(), g.data());
         for (int i = 0; i < S.nV; ++i)
             for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
         if (P.collision) barrier_grad(S, x.data(), C, S.kappa, g.data());
@@ -258,6 +261,7 @@ int advance_one_step(Scene& S, StepStats& st) {
             std::vector<Block> blocks;
             elastic_hess(S, x.data(), blocks);
             strain_limit_hess(S, x.data(), blocks);
+            if (fiber_active(S)) fiber_hess(S, x.data(), blocks);
             if (P.collision) barrier_hess(S, x.data(), C, S.kappa, blocks);
             if (fric) friction_hess(S, x.data(), xn.data(), F, blocks);
             if (dbcStiff)
@@ -281,11 +285,13 @@ int advance_one_step(Scene& S, StepStats& st) {
         xprev = x;
         double alpha = 1.0;
         // strain-limit feasibility first, IMPLICIT_EULER.h:74-92
-        if (P.slKappa[0] > 0) {
+        if (P.slKappa[0] > 0 || P.fiberK[0] > 0) {
             for (;;) {
                 for (int i = 0; i < n; ++i) x[i] = xprev[i] + alpha * sol[i];
                 double Et = 0;
-                if (strain_limit_energy(S, x.data(), &Et)) break;
+                bool ok = strain_limit_energy(S, x.data(), &Et);
+                if (fiber_active(S)) ok = fiber_energy(S, x.data(), &Et) && ok;
+                if (ok) break;
                 alpha /= 2;
             }
             x = xprev;
@@ -379,6 +385,7 @@ int advance_one_step(Scene& S, StepStats& st) {
                 std::fill(g.begin(), g.end(), 0.0);
                 elastic_grad(S, x.data(), g.data());
                 strain_limit_grad(S, x.data(), g.data());
+                if (fiber_active(S)) fiber_grad(S, x.data(), g.data());
                 for (int i = 0; i < S.nV; ++i)
                     for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
                 barrier_grad(S, x.data(), C, S.kappa, g.data());
@@ -396,6 +403,7 @@ int advance_one_step(Scene& S, StepStats& st) {
                 std::vector<Block> blocks;
                 elastic_hess(S, x.data(), blocks);
                 strain_limit_hess(S, x.data(), blocks);
+                if (fiber_active(S)) fiber_hess(S, x.data(), blocks);
                 barrier_hess(S, x.data(), C, S.kappa, blocks);
                 friction_hess(S, x.data(), xn.data(), F, blocks);
                 if (dbcStiff)

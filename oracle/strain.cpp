@@ -212,4 +212,134 @@ void strain_limit_hess(const Scene& S, const double* x, std::vector<Block>& out)
     }
 }
 
+// ---------------------------------------------------------------- anisotropic (fibre) model
+// Restates Check_Fiber_Feasibility / Compute_Fiber_Energy / _Gradient / _Hessian
+// (FEM/Shell/ANISO_INEXT.h:33-440; eta of :5-31 without ORTHO_STVK).  It replaces the
+// membrane when fiberStiffMult[0] > 0 or [1] > 0 (INC_POTENTIAL.h:81-92, 213-219,
+// 340-346).  Green strain E = (F^T F - I)/2 of F = A B^-1, B from the rest positions'
+// x and z coordinates (the sheet's rest plane, ANISO_INEXT.h:64-68);
+// psi = h^2 vol [k0/2 eta(E00^2, L0^2) + k1 eta(E00 E11, L0 L1) + k2/2 eta(E11^2, L1^2)
+//                + k3 eta(E01^2, L2^2)],  eta(s, m) = -m ln((m - s)/m);
+// infeasible when |E00| >= L0, |E11| >= L1 or |E01| >= L2.  The F-space Hessian is
+// sum_ab d2psi/dEa dEb dEa dEb^T + [[S00 I, S01/2 I], [S01/2 I, S11 I]] (own derivation),
+// projected to PSD in F space (ANISO_INEXT.h:322-324) and mapped to x.
+// Parity pins (tests/test_oracle_strain.py): closed-form energies of uniaxial stretch and
+// shear, central differences of energy and gradient, feasibility at the limits.
+
+static inline bool fiber_on(const Scene& S) { return S.P.fiberK[0] > 0 || S.P.fiberK[1] > 0; }
+
+void fiber_setup(Scene& S) {
+    S.fiberIB.resize(4 * S.nT);
+    for (int t = 0; t < S.nT; ++t) {
+        const int* v = &S.tri[3 * t];
+        const double* X = S.X0.data();
+        const double b00 = X[3 * v[1]] - X[3 * v[0]], b01 = X[3 * v[2]] - X[3 * v[0]];
+        const double b10 = X[3 * v[1] + 2] - X[3 * v[0] + 2], b11 = X[3 * v[2] + 2] - X[3 * v[0] + 2];
+        const double inv = 1.0 / (b00 * b11 - b01 * b10);
+        S.fiberIB[4 * t] = b11 * inv; S.fiberIB[4 * t + 1] = -b01 * inv;     // (B^-1)_00, (B^-1)_01
+        S.fiberIB[4 * t + 2] = -b10 * inv; S.fiberIB[4 * t + 3] = b00 * inv; // (B^-1)_10, (B^-1)_11
+    }
+}
+
+static inline double eta0(double s, double m) { return -m * std::log((m - s) / m); }
+static inline double eta1(double s, double m) { return m / (m - s); }
+static inline double eta2(double s, double m) { return m / ((m - s) * (m - s)); }
+
+// one triangle: false when infeasible; E, g (9), H (81, projected if proj)
+bool fiber_tri(const Scene& S, const double* x, int t, double* E, double* g9, double* H81, bool proj, double* Eout3) {
+    const Params& P = S.P;
+    const int* v = &S.tri[3 * t];
+    const double* ib = &S.fiberIB[4 * t];
+    V3 A0 = X3(x, v[1]) - X3(x, v[0]), A1 = X3(x, v[2]) - X3(x, v[0]);
+    V3 F0 = ib[0] * A0 + ib[2] * A1, F1 = ib[1] * A0 + ib[3] * A1;
+    const double E00 = 0.5 * (sq(F0) - 1), E11 = 0.5 * (sq(F1) - 1), E01 = 0.5 * dot(F0, F1);
+    if (Eout3) { Eout3[0] = E00; Eout3[1] = E11; Eout3[2] = E01; }
+    const double L0 = P.fiberLimit[0], L1 = P.fiberLimit[1], L2 = P.fiberLimit[2];
+    if (std::fabs(E00) >= L0 || std::fabs(E11) >= L1 || std::fabs(E01) >= L2) return false;
+    const double* k = P.fiberK;
+    const double w = P.dt * P.dt * S.vol[t];
+    const double m00 = L0 * L0, m11 = L1 * L1, m01 = L0 * L1, m2 = L2 * L2;
+    const double s00 = E00 * E00, s11 = E11 * E11, sx = E00 * E11, s2 = E01 * E01;
+    if (E) *E = w * (k[0] / 2 * eta0(s00, m00) + k[1] * eta0(sx, m01) + k[2] / 2 * eta0(s11, m11) + k[3] * eta0(s2, m2));
+    const double S00 = w * (k[0] * eta1(s00, m00) * E00 + k[1] * eta1(sx, m01) * E11);
+    const double S11 = w * (k[2] * eta1(s11, m11) * E11 + k[1] * eta1(sx, m01) * E00);
+    const double S01 = w * 2 * k[3] * eta1(s2, m2) * E01;
+    const double c0[3] = {-(ib[0] + ib[2]), ib[0], ib[2]}, c1[3] = {-(ib[1] + ib[3]), ib[1], ib[3]};
+    if (g9) {
+        V3 G0 = S00 * F0 + (0.5 * S01) * F1, G1 = S11 * F1 + (0.5 * S01) * F0;
+        for (int a = 0; a < 3; ++a) {
+            g9[3 * a] = c0[a] * G0.x + c1[a] * G1.x;
+            g9[3 * a + 1] = c0[a] * G0.y + c1[a] * G1.y;
+            g9[3 * a + 2] = c0[a] * G0.z + c1[a] * G1.z;
+        }
+    }
+    if (H81) {
+        const double H00 = w * (k[0] * (2 * eta2(s00, m00) * s00 + eta1(s00, m00)) + k[1] * eta2(sx, m01) * s11);
+        const double H11 = w * (k[2] * (2 * eta2(s11, m11) * s11 + eta1(s11, m11)) + k[1] * eta2(sx, m01) * s00);
+        const double H0x = w * k[1] * (eta2(sx, m01) * sx + eta1(sx, m01));
+        const double H22 = w * 2 * k[3] * (2 * eta2(s2, m2) * s2 + eta1(s2, m2));
+        double dA[6] = {F0.x, F0.y, F0.z, 0, 0, 0}, dB[6] = {0, 0, 0, F1.x, F1.y, F1.z};
+        double dC[6] = {0.5 * F1.x, 0.5 * F1.y, 0.5 * F1.z, 0.5 * F0.x, 0.5 * F0.y, 0.5 * F0.z};
+        double HF[36];
+        for (int r = 0; r < 6; ++r)
+            for (int q = 0; q < 6; ++q)
+                HF[r * 6 + q] = H00 * dA[r] * dA[q] + H11 * dB[r] * dB[q] + H0x * (dA[r] * dB[q] + dB[r] * dA[q]) + H22 * dC[r] * dC[q];
+        for (int i = 0; i < 3; ++i) {
+            HF[i * 6 + i] += S00;
+            HF[(3 + i) * 6 + 3 + i] += S11;
+            HF[i * 6 + 3 + i] += 0.5 * S01;
+            HF[(3 + i) * 6 + i] += 0.5 * S01;
+        }
+        if (proj) make_pd(6, HF);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int kk = 0; kk < 3; ++kk)
+                    for (int l = 0; l < 3; ++l)
+                        H81[(3 * a + kk) * 9 + 3 * b + l] = c0[a] * c0[b] * HF[kk * 6 + l] + c0[a] * c1[b] * HF[kk * 6 + 3 + l] +
+                                                            c1[a] * c0[b] * HF[(3 + kk) * 6 + l] + c1[a] * c1[b] * HF[(3 + kk) * 6 + 3 + l];
+    }
+    return true;
+}
+
+bool fiber_active(const Scene& S) { return fiber_on(S); }
+
+bool fiber_energy(const Scene& S, const double* x, double* E) {
+    bool ok = true;
+    double sum = 0;
+    for (int t = 0; t < S.nT; ++t) {
+        if (tri_all_dbc(S, t)) continue;
+        double e = 0;
+        if (!fiber_tri(S, x, t, &e, nullptr, nullptr, false, nullptr)) { ok = false; continue; }
+        sum += e;
+    }
+    *E += sum;
+    return ok;
+}
+void fiber_grad(const Scene& S, const double* x, double* g) {
+    for (int t = 0; t < S.nT; ++t) {
+        if (tri_all_dbc(S, t)) continue;
+        double gl[9];
+        if (!fiber_tri(S, x, t, nullptr, gl, nullptr, false, nullptr)) continue;
+        const int* v = &S.tri[3 * t];
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 3; ++k) g[3 * v[a] + k] += gl[3 * a + k];
+    }
+}
+void fiber_hess(const Scene& S, const double* x, std::vector<Block>& out) {
+    for (int t = 0; t < S.nT; ++t) {
+        if (tri_all_dbc(S, t)) continue;
+        double H[81];
+        if (!fiber_tri(S, x, t, nullptr, nullptr, H, true, nullptr)) continue;
+        const int* v = &S.tri[3 * t];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                Block B;
+                B.r = v[a]; B.c = v[b];
+                for (int k = 0; k < 3; ++k)
+                    for (int l = 0; l < 3; ++l) B.b[3 * k + l] = H[(3 * a + k) * 9 + 3 * b + l];
+                out.push_back(B);
+            }
+    }
+}
+
 }  // namespace orc

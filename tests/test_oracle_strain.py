@@ -120,3 +120,76 @@ def test_step_keeps_strain_below_limit():
         out[kappa] = max_sigma(probe, x, sd.nT)
     assert out[0.0] > 1.01          # the soft sheet stretches past the limit without it
     assert 1.0 < out[1e3] < 1.01    # and stays inside it with it
+
+
+# ---------------------------------------------------------------- anisotropic (fibre) model
+def fiber_tri_scene(k=(1e3, 5e2, 1e3, 2e2), lim=(0.05, 0.05, 0.05)):
+    sd, _ = tri_scene()
+    sd.params.update(fiberStiffMult=k, inextLimit=lim)
+    return sd, O.OracleScene(sd)
+
+
+def eta(s, m):
+    return -m * np.log((m - s) / m)
+
+
+def test_fiber_closed_forms():
+    k, lim = (1e3, 5e2, 1e3, 2e2), (0.05, 0.04, 0.03)
+    sd, orc = fiber_tri_scene(k, lim)
+    w = sd.params["dt"] ** 2 * orc.arrays()["vol"][0]
+    X = sd.X
+    for l in (1.01, 1.03):                       # stretch along x: E00 = (l^2 - 1)/2
+        ok, E, _, _, e3 = orc.fiber_tri((X * np.array([l, 1.0, 1.0])).ravel(), 0)
+        E00 = 0.5 * (l * l - 1)
+        assert ok and np.allclose(e3, [E00, 0, 0], atol=1e-15)
+        assert abs(E - w * k[0] / 2 * eta(E00 ** 2, lim[0] ** 2)) <= 1e-12 * E
+    for gmm in (0.02, 0.05):                     # shear x += g z: E11 = g^2/2, E01 = g/2
+        Xs = X.copy()
+        Xs[:, 0] += gmm * X[:, 2]
+        ok, E, _, _, e3 = orc.fiber_tri(Xs.ravel(), 0)
+        assert ok and np.allclose(e3, [0, gmm * gmm / 2, gmm / 2], atol=1e-15)
+        ref = w * (k[2] / 2 * eta((gmm * gmm / 2) ** 2, lim[1] ** 2) + k[3] * eta((gmm / 2) ** 2, lim[2] ** 2))
+        assert abs(E - ref) <= 1e-12 * ref
+    assert not orc.fiber_tri((X * np.array([1.06, 1.0, 1.0])).ravel(), 0)[0]   # E00 = 0.0618 >= 0.05
+    ok, E, *_ = orc.fiber_tri(X.ravel(), 0)
+    assert ok and E == 0.0
+
+
+@pytest.mark.parametrize("case", ["stretch", "shear", "mixed"])
+def test_fiber_derivatives_fd(case):
+    sd, orc = fiber_tri_scene()
+    rng = np.random.default_rng(7)
+    X = sd.X.copy()
+    if case == "stretch":
+        X = X * np.array([1.02, 1.0, 0.99])
+    elif case == "shear":
+        X[:, 0] += 0.03 * X[:, 2]
+    else:
+        X = X @ np.array([[1.02, 0.01, 0.015], [0.0, 1.0, 0.0], [-0.01, 0.02, 0.985]])
+    X = X + rng.standard_normal(X.shape) * 1e-5
+    x = X.ravel()
+    ok, E, g, H, _ = orc.fiber_tri(x, 0, proj=False)
+    assert ok and E > 0
+    for _ in range(4):
+        v = rng.standard_normal(9)
+        t = 1e-7
+        fd = (orc.fiber_tri(x + t * v, 0)[1] - orc.fiber_tri(x - t * v, 0)[1]) / (2 * t)
+        assert abs(fd - g @ v) <= 1e-6 * np.abs(g).sum() * np.abs(v).max()
+        gd = (orc.fiber_tri(x + t * v, 0, proj=False)[2] - orc.fiber_tri(x - t * v, 0, proj=False)[2]) / (2 * t)
+        assert np.abs(gd - H @ v).max() <= 1e-5 * np.abs(H).max() * np.abs(v).max()
+    Hp = orc.fiber_tri(x, 0, proj=True)[3]
+    assert np.linalg.eigvalsh(0.5 * (Hp + Hp.T)).min() >= -1e-10 * np.abs(Hp).max()
+
+
+def test_fiber_step_stays_feasible():
+    """the pinned sheet under gravity with the fibre model (limits 0.02): every step ends
+    with each Green strain component inside its limit"""
+    sd = hammock(0.0)
+    sd.params.update(fiberStiffMult=(1e2, 0.0, 1e2, 1e2), inextLimit=(0.02, 0.02, 0.02))
+    orc = O.OracleScene(sd)
+    for _ in range(6):
+        rc, _ = orc.step()
+        assert rc > 0
+    x, _ = orc.get_state()
+    e = np.array([orc.fiber_tri(x, t)[4] for t in range(sd.nT)])
+    assert np.abs(e).max() < 0.02 and np.abs(e).max() > 1e-4
