@@ -65,6 +65,8 @@ class SceneDesc(ctypes.Structure):
         ("strain_limit_kappa", ctypes.c_double * 2),
         ("strain_limit_s", ctypes.c_double * 2),
         ("strain_limit_shat", ctypes.c_double * 2),
+        ("fiber_stiff_mult", ctypes.c_double * 4),
+        ("fiber_limit", ctypes.c_double * 3),
     ]
 
 
@@ -137,6 +139,7 @@ PROTOTYPES = {
     "cipc_ccd_stepsize": (ctypes.c_int, [P, D, D, ctypes.c_double, D, D]),
     "cipc_set_friction": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32]),
     "cipc_set_strain_limit": (ctypes.c_int, [P, D, D, D]),
+    "cipc_set_fiber": (ctypes.c_int, [P, D, D]),
     "cipc_dirichlet_add_motion": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, D, D, D, ctypes.c_double]),
     "cipc_step_dirichlet": (ctypes.c_int, [P]),
     "cipc_write_obj": (ctypes.c_int, [P, ctypes.c_char_p]),

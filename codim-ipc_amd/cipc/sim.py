@@ -84,6 +84,10 @@ class Simulator:
         desc.friction_iter_amt = int(P.get("fricIterAmt", 1))
         for i, (k, s_, sh) in enumerate(zip(P.get("kappa_s", (0.0, 0.0)), P.get("s", (1.01, 0.0)), P.get("sHat", (1.0, 1.0)))):
             desc.strain_limit_kappa[i], desc.strain_limit_s[i], desc.strain_limit_shat[i] = k, s_, sh
+        for i, k in enumerate(P.get("fiberStiffMult", (0.0, 0.0, 0.0, 0.0))):
+            desc.fiber_stiff_mult[i] = k
+        for i, l in enumerate(P.get("inextLimit", (1.0, 1.0, 1.0))):
+            desc.fiber_limit[i] = l
         h = ctypes.c_void_p()
         rc = self.lib.cipc_create(ctypes.byref(desc), ctypes.byref(h))
         if rc != 0:

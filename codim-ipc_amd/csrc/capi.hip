@@ -34,6 +34,14 @@ static bool strain_limit_args_ok(const double* k, const double* s, const double*
     return true;
 }
 
+// fibre model: stiffness multipliers >= 0; when on, every limit > 0
+static bool fiber_args_ok(const double* k, const double* lim) {
+    for (int i = 0; i < 4; ++i) if (k[i] < 0) return false;
+    if (k[0] > 0 || k[1] > 0)
+        for (int i = 0; i < 3; ++i) if (!(lim[i] > 0)) return false;
+    return true;
+}
+
 template <class F>
 static int guarded(cipc_ctx* ctx, F f) {
     try {
@@ -83,6 +91,7 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
     if (d->n_vertices >= (1 << 21) - 1) return CIPC_E_INVALID;   // contact-key packing limit
     if (d->friction_mu < 0 || (d->friction_mu > 0 && !(d->friction_epsv2 > 0))) return CIPC_E_INVALID;
     if (!strain_limit_args_ok(d->strain_limit_kappa, d->strain_limit_s, d->strain_limit_shat)) return CIPC_E_INVALID;
+    if (!fiber_args_ok(d->fiber_stiff_mult, d->fiber_limit)) return CIPC_E_INVALID;
     cipc_ctx* ctx = new cipc_ctx();
     int rc = guarded(ctx, [&]() {
         Ctx& c = ctx->c;
@@ -99,6 +108,8 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
         P.fricMu = d->friction_mu;
         P.epsv2 = d->friction_mu > 0 ? d->friction_epsv2 : 1e-6;
         P.fricIterAmt = d->friction_iter_amt;
+        for (int i = 0; i < 4; ++i) P.fiberK[i] = d->fiber_stiff_mult[i];
+        for (int i = 0; i < 3; ++i) P.fiberLimit[i] = d->fiber_limit[i];
         if (d->strain_limit_kappa[0] > 0)
             for (int i = 0; i < 2; ++i) {
                 P.slKappa[i] = d->strain_limit_kappa[i]; P.slS[i] = d->strain_limit_s[i]; P.slSHat[i] = d->strain_limit_shat[i];
@@ -461,6 +472,14 @@ int cipc_strain_limit_energy(cipc_ctx* ctx, const double* x, double* e, int32_t*
         *e = *valid ? E : 0.0;
         return CIPC_OK;
     });
+}
+
+int cipc_set_fiber(cipc_ctx* ctx, const double* stiff_mult, const double* limit) {
+    if (!ctx || !stiff_mult || !limit || !fiber_args_ok(stiff_mult, limit)) return CIPC_E_INVALID;
+    Params& P = ctx->c.P;
+    for (int i = 0; i < 4; ++i) P.fiberK[i] = stiff_mult[i];
+    for (int i = 0; i < 3; ++i) P.fiberLimit[i] = limit[i];
+    return CIPC_OK;
 }
 
 // ---- lagged friction

@@ -31,6 +31,8 @@ struct Params {
     int fricIterAmt = 1;
     // strain limiting (the step's kappa_s, s, sHat; index 0 upper limit, 1 lower)
     double slKappa[2] = {0, 0}, slS[2] = {1.01, 0}, slSHat[2] = {1, 1};
+    // anisotropic (fibre) membrane: the step's fiberStiffMult and inextLimit
+    double fiberK[4] = {0, 0, 0, 0}, fiberLimit[3] = {1, 1, 1};
 };
 
 // spatial grid used by the constraint-set broad phase and the CCD candidate search
@@ -71,7 +73,7 @@ struct Ctx {
 
     // host mirrors of the topology
     std::vector<int> hTri, hHinge, hBNode, hBEdge, hDbcIdx;
-    std::vector<double> hDbcTarget, hMass, hHingeInfo, hBinv, hVol, hSlIB;
+    std::vector<double> hDbcTarget, hMass, hHingeInfo, hBinv, hVol, hSlIB, hFbIB;
     // codimensional primitives: rod segments are the last nRod boundary edges, rod nodes
     // occupy boundary-node slots [codimBN0, codimBN1), particles [codimBN1, nBN)
     int nRod = 0, nRH = 0, nPart = 0, codimBN0 = 0, codimBN1 = 0;
@@ -96,6 +98,7 @@ struct Ctx {
     DevBuf<int> tri;          // SoA [3][nT]
     DevBuf<double> Binv;      // SoA [3][nT]
     DevBuf<double> slIB;      // SoA [3][nT]: B^-1 of the rest metric (strain limiting)
+    DevBuf<double> fbIB;      // SoA [4][nT]: B^-1 of the rest x-z coordinates (fibre model)
     DevBuf<double> vol;
     DevBuf<int> hinge;        // SoA [4][nH]
     DevBuf<double> hInfo;     // SoA [3][nH]
@@ -262,6 +265,11 @@ bool strain_limit_on(const Ctx& c);
 double strain_limit_energy(Ctx& c, const double* x);
 void strain_limit_grad_add(Ctx& c, const double* x, double* gl);
 void strain_limit_hess_add(Ctx& c, const double* x, double* hl);
+// anisotropic (fibre) membrane (strain.hip): replaces the membrane's energy and local slots
+bool fiber_on(const Ctx& c);
+double fiber_energy(Ctx& c, const double* x);               // +inf when infeasible
+void fiber_grad_local(Ctx& c, const double* x, double* gl);
+void fiber_hess_local(Ctx& c, const double* x, double* hl);
 double min_dist2(Ctx& c, const double* x, const int* C, int nC, double* d2out);
 double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, double* clamped, int* zeroToi);
 double elastic_energy(Ctx& c, const double* x, double* memb, double* bend);

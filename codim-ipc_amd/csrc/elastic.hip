@@ -95,7 +95,7 @@ static double finish_sum(Ctx& c, const double* part, int nb, int slot) {
 double elastic_energy(Ctx& c, const double* x, double* memb, double* bend) {
     const double h2 = c.P.dt * c.P.dt;
     double Em = 0, Eb = 0;
-    if (c.nT) {
+    if (c.nT && !fiber_on(c)) {   // the fibre model replaces the membrane (INC_POTENTIAL.h:81-92)
         int nb = nblocks(c.nT);
         double* part = c.red.ensure(nb);
         hipLaunchKernelGGL(k_memb_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.tri.p, c.nT, c.Binv.p, c.vol.p, c.lam, c.mu, h2, c.dbc.p, part);
@@ -224,7 +224,10 @@ static void elastic_grad_impl(Ctx& c, const double* x, double* g, int withInerti
     const double h2 = c.P.dt * c.P.dt;
     long ns = c.nGSlots;
     double* gl = c.gLocal.ensure(3 * std::max<long>(ns, 1));
-    if (c.nT) {
+    if (c.nT && fiber_on(c)) {
+        fiber_grad_local(c, x, gl);        // INC_POTENTIAL.h:213-216
+        strain_limit_grad_add(c, x, gl);
+    } else if (c.nT) {
         hipLaunchKernelGGL(k_memb_grad_local, dim3(nblocks(c.nT)), dim3(kBlock), 0, c.s, x, c.tri.p, c.nT, c.Binv.p, c.vol.p, c.lam, c.mu,
                            h2, c.dbc.p, gl, ns);
         CIPC_LAUNCH_CHECK();
@@ -486,7 +489,10 @@ void elastic_hessian(Ctx& c, const double* x) {
     const double h2 = c.P.dt * c.P.dt;
     long ns = c.nElSlots;
     double* hl = c.elLocal.ensure(9 * std::max<long>(ns, 1));
-    if (c.nT) {
+    if (c.nT && fiber_on(c)) {
+        fiber_hess_local(c, x, hl);        // INC_POTENTIAL.h:340-343
+        strain_limit_hess_add(c, x, hl);
+    } else if (c.nT) {
         hipLaunchKernelGGL(k_memb_hess_local, dim3(nblocks(c.nT, 64)), dim3(64), 0, c.s, x, c.tri.p, c.nT, c.Binv.p, c.vol.p, c.lam, c.mu,
                            h2, c.dbc.p, hl, ns);
         CIPC_LAUNCH_CHECK();

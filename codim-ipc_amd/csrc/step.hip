@@ -105,6 +105,15 @@ void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri
         double inv = 1.0 / (B00 * B11);
         c.hSlIB[3 * t] = B11 * inv; c.hSlIB[3 * t + 1] = -B01 * inv; c.hSlIB[3 * t + 2] = B00 * inv;
     }
+    // fibre model: inverse of the rest x-z coordinate matrix (ANISO_INEXT.h:64-68)
+    c.hFbIB.resize(4 * c.nT);
+    for (int t = 0; t < c.nT; ++t) {
+        const int* v = &tri[3 * t];
+        const double b00 = X[3 * v[1]] - X[3 * v[0]], b01 = X[3 * v[2]] - X[3 * v[0]];
+        const double b10 = X[3 * v[1] + 2] - X[3 * v[0] + 2], b11 = X[3 * v[2] + 2] - X[3 * v[0] + 2];
+        const double inv = 1.0 / (b00 * b11 - b01 * b10);
+        c.hFbIB[4 * t] = b11 * inv; c.hFbIB[4 * t + 1] = -b01 * inv; c.hFbIB[4 * t + 2] = -b10 * inv; c.hFbIB[4 * t + 3] = b00 * inv;
+    }
     c.hHinge.clear(); c.hHingeInfo.clear();
     for (auto& e : edgeSet) {
         auto fa = edge2tri.find({e.first, e.second});
@@ -235,6 +244,7 @@ void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri
     upload(c.tri, soa_i(tri, c.nT, 3), s);
     upload(c.Binv, soa_d(c.hBinv, c.nT, 3), s);
     upload(c.slIB, soa_d(c.hSlIB, c.nT, 3), s);
+    upload(c.fbIB, soa_d(c.hFbIB, c.nT, 4), s);
     upload(c.vol, c.hVol, s);
     upload(c.hinge, soa_i(c.hHinge, c.nH, 4), s);
     upload(c.hInfo, soa_d(c.hHingeInfo, c.nH, 3), s);
@@ -348,7 +358,8 @@ static double sumsq(Ctx& c, const double* a, long n) {
 // incremental potential at x with constraint set C (INC_POTENTIAL.h:15-152)
 // +inf when a strain limit is reached (INC_POTENTIAL.h:75-80: the potential is invalid)
 static double inc_potential(Ctx& c, const double* x, const int* C, int nC, int* bad) {
-    const double Esl = strain_limit_energy(c, x);
+    double Esl = strain_limit_energy(c, x);
+    if (fiber_on(c)) Esl += fiber_energy(c, x);   // Check_Fiber_Feasibility + Compute_Fiber_Energy
     if (std::isinf(Esl)) { if (bad) *bad = 0; return Esl; }
     double E = Esl + elastic_energy(c, x, nullptr, nullptr) + inertia_energy(c, x);
     if (c.P.collision) E += barrier_energy(c, x, C, nC, c.kappa, bad);
@@ -473,11 +484,13 @@ int advance_one_step(Ctx& c) {
         CIPC_HIP(hipMemcpyAsync(c.xprev.p, c.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         double alpha = 1.0;
         // strain-limit feasibility first (IMPLICIT_EULER.h:74-92), then the CCD from there
-        if (strain_limit_on(c)) {
+        if (strain_limit_on(c) || (c.P.fiberK[0] > 0 && c.nT)) {
             for (;;) {
                 hipLaunchKernelGGL(k_axpy_to, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c.xprev.p, alpha, c.p.p, c.x.p);
                 CIPC_LAUNCH_CHECK();
-                if (!std::isinf(strain_limit_energy(c, c.x.p))) break;
+                double Ef = strain_limit_energy(c, c.x.p);
+                if (fiber_on(c)) Ef += fiber_energy(c, c.x.p);
+                if (!std::isinf(Ef)) break;
                 alpha /= 2;
             }
             CIPC_HIP(hipMemcpyAsync(c.x.p, c.xprev.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));

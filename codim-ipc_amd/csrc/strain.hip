@@ -195,6 +195,177 @@ __global__ void __launch_bounds__(64) k_sl_hess_add(const double* __restrict__ x
                                                         c1[p] * c0[q] * HF[(3 + k) * 6 + l] + c1[p] * c1[q] * HF[(3 + k) * 6 + 3 + l];
 }
 
+// ------------------------------------------------------------------ anisotropic (fibre) model
+// Check_Fiber_Feasibility / Compute_Fiber_Energy / _Gradient / _Hessian (FEM/Shell/
+// ANISO_INEXT.h:33-440): Green strain E = (F^T F - I)/2 of F = A B^-1, B from the rest
+// x and z coordinates; psi = h^2 vol [k0/2 eta(E00^2, L0^2) + k1 eta(E00 E11, L0 L1) +
+// k2/2 eta(E11^2, L1^2) + k3 eta(E01^2, L2^2)], eta(s, m) = -m ln((m - s)/m); +inf when
+// |E00| >= L0, |E11| >= L1 or |E01| >= L2.  It replaces the membrane (same local slots).
+// The 6x6 F-space Hessian is projected (ANISO_INEXT.h:322-324) by embedding it in the
+// translation-free subspace of a 9x9 matrix and running the in-register projection
+// make_pd_translation<3> there: for orthonormal W (3x2, columns orthogonal to (1,1,1)),
+// (W x I) H_F (W x I)^T has reduced matrix H_F, so the projected 9x9 is
+// (W x I) makePD(H_F) (W x I)^T.
+struct FBPar {
+    double k[4], L[3];
+    double h2;
+};
+CIPC_DEV double fb_eta0(double s, double m) { return -m * log((m - s) / m); }
+CIPC_DEV double fb_eta1(double s, double m) { return m / (m - s); }
+CIPC_DEV double fb_eta2(double s, double m) { return m / ((m - s) * (m - s)); }
+
+struct FBTri {
+    d3 F0, F1;
+    double E00, E11, E01, c0[3], c1[3];
+};
+CIPC_DEV bool fb_eval(const double* __restrict__ x, int a, int b, int c, const double* __restrict__ IB, int nT, int t, const FBPar& P,
+                      FBTri& T) {
+    const double i00 = IB[t], i01 = IB[nT + t], i10 = IB[2 * nT + t], i11 = IB[3 * nT + t];
+    const d3 A0 = ld3(x, b) - ld3(x, a), A1 = ld3(x, c) - ld3(x, a);
+    T.F0 = i00 * A0 + i10 * A1;
+    T.F1 = i01 * A0 + i11 * A1;
+    T.E00 = 0.5 * (sq(T.F0) - 1);
+    T.E11 = 0.5 * (sq(T.F1) - 1);
+    T.E01 = 0.5 * dot(T.F0, T.F1);
+    T.c0[0] = -(i00 + i10); T.c0[1] = i00; T.c0[2] = i10;
+    T.c1[0] = -(i01 + i11); T.c1[1] = i01; T.c1[2] = i11;
+    return fabs(T.E00) < P.L[0] && fabs(T.E11) < P.L[1] && fabs(T.E01) < P.L[2];
+}
+
+__global__ void k_fb_energy(const double* __restrict__ x, const int* __restrict__ tri, int nT, const double* __restrict__ IB,
+                            const double* __restrict__ vol, FBPar P, const uint8_t* __restrict__ dbc, double* __restrict__ partial) {
+    __shared__ double lds[4];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (t < nT) {
+        const int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+        if (!sl_all_dbc(dbc, a, b, c)) {
+            FBTri T;
+            if (fb_eval(x, a, b, c, IB, nT, t, P, T)) {
+                const double m00 = P.L[0] * P.L[0], m11 = P.L[1] * P.L[1], m01 = P.L[0] * P.L[1], m2 = P.L[2] * P.L[2];
+                e = P.h2 * vol[t] * (P.k[0] / 2 * fb_eta0(T.E00 * T.E00, m00) + P.k[1] * fb_eta0(T.E00 * T.E11, m01) +
+                                     P.k[2] / 2 * fb_eta0(T.E11 * T.E11, m11) + P.k[3] * fb_eta0(T.E01 * T.E01, m2));
+            } else {
+                e = __builtin_inf();
+            }
+        }
+    }
+    const double s = block_sum_256(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// dpsi/dE (S00, S11, and S01 for the single variable E01)
+CIPC_DEV void fb_stress(const FBTri& T, const FBPar& P, double w, double& S00, double& S11, double& S01) {
+    const double m00 = P.L[0] * P.L[0], m11 = P.L[1] * P.L[1], m01 = P.L[0] * P.L[1], m2 = P.L[2] * P.L[2];
+    const double ex = fb_eta1(T.E00 * T.E11, m01);
+    S00 = w * (P.k[0] * fb_eta1(T.E00 * T.E00, m00) * T.E00 + P.k[1] * ex * T.E11);
+    S11 = w * (P.k[2] * fb_eta1(T.E11 * T.E11, m11) * T.E11 + P.k[1] * ex * T.E00);
+    S01 = w * 2 * P.k[3] * fb_eta1(T.E01 * T.E01, m2) * T.E01;
+}
+
+// local gradient into the membrane's triangle slots (overwrites; zero when inactive)
+__global__ void k_fb_grad_local(const double* __restrict__ x, const int* __restrict__ tri, int nT, const double* __restrict__ IB,
+                                const double* __restrict__ vol, FBPar P, const uint8_t* __restrict__ dbc, double* __restrict__ gl) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    FBTri T;
+    if (!sl_all_dbc(dbc, a, b, c) && fb_eval(x, a, b, c, IB, nT, t, P, T)) {
+        double S00, S11, S01;
+        fb_stress(T, P, P.h2 * vol[t], S00, S11, S01);
+        const d3 G0 = S00 * T.F0 + (0.5 * S01) * T.F1, G1 = S11 * T.F1 + (0.5 * S01) * T.F0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            g[3 * q] = T.c0[q] * G0.x + T.c1[q] * G1.x;
+            g[3 * q + 1] = T.c0[q] * G0.y + T.c1[q] * G1.y;
+            g[3 * q + 2] = T.c0[q] * G0.z + T.c1[q] * G1.z;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) gl[9l * t + q] = g[q];
+}
+
+// projected local Hessian into the membrane's triangle blocks (overwrites)
+__global__ void __launch_bounds__(64) k_fb_hess_local(const double* __restrict__ x, const int* __restrict__ tri, int nT,
+                                                      const double* __restrict__ IB, const double* __restrict__ vol, FBPar P,
+                                                      const uint8_t* __restrict__ dbc, double* __restrict__ hl) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+    double* dst = hl + 81l * t;
+    FBTri T;
+    if (sl_all_dbc(dbc, a, b, c) || !fb_eval(x, a, b, c, IB, nT, t, P, T)) {
+#pragma unroll
+        for (int k = 0; k < 81; ++k) dst[k] = 0;
+        return;
+    }
+    const double w = P.h2 * vol[t];
+    double S00, S11, S01;
+    fb_stress(T, P, w, S00, S11, S01);
+    const double m00 = P.L[0] * P.L[0], m11 = P.L[1] * P.L[1], m01 = P.L[0] * P.L[1], m2 = P.L[2] * P.L[2];
+    const double s00 = T.E00 * T.E00, s11 = T.E11 * T.E11, sx = T.E00 * T.E11, s2 = T.E01 * T.E01;
+    const double H00 = w * (P.k[0] * (2 * fb_eta2(s00, m00) * s00 + fb_eta1(s00, m00)) + P.k[1] * fb_eta2(sx, m01) * s11);
+    const double H11 = w * (P.k[2] * (2 * fb_eta2(s11, m11) * s11 + fb_eta1(s11, m11)) + P.k[1] * fb_eta2(sx, m01) * s00);
+    const double H0x = w * P.k[1] * (fb_eta2(sx, m01) * sx + fb_eta1(sx, m01));
+    const double H22 = w * 2 * P.k[3] * (2 * fb_eta2(s2, m2) * s2 + fb_eta1(s2, m2));
+    const double dA[6] = {T.F0.x, T.F0.y, T.F0.z, 0, 0, 0}, dB[6] = {0, 0, 0, T.F1.x, T.F1.y, T.F1.z};
+    const double dC[6] = {0.5 * T.F1.x, 0.5 * T.F1.y, 0.5 * T.F1.z, 0.5 * T.F0.x, 0.5 * T.F0.y, 0.5 * T.F0.z};
+    double HF[36];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            HF[r * 6 + q] = H00 * dA[r] * dA[q] + H11 * dB[r] * dB[q] + H0x * (dA[r] * dB[q] + dB[r] * dA[q]) + H22 * dC[r] * dC[q];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        HF[i * 6 + i] += S00;
+        HF[(3 + i) * 6 + 3 + i] += S11;
+        HF[i * 6 + 3 + i] += 0.5 * S01;
+        HF[(3 + i) * 6 + i] += 0.5 * S01;
+    }
+    // embed: H9 = (W x I) H_F (W x I)^T, then project in the translation-free subspace
+    double H9[81];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    double v = 0;
+#pragma unroll
+                    for (int p = 0; p < 2; ++p)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) v += tr_basis<3>(k, p) * tr_basis<3>(l, q) * HF[(3 * p + i) * 6 + 3 * q + j];
+                    H9[(3 * k + i) * 9 + 3 * l + j] = v;
+                }
+    double P9[81];
+    make_pd_translation<3>(H9, 9, [&](int r, int cc, double v) { P9[r * 9 + cc] = v; });
+    // H_x = (Wh x I)^T P9 (Wh x I), Wh(k, a) = sum_p W(k, p) c_p[a]
+    double Wh[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) Wh[k * 3 + q] = tr_basis<3>(k, 0) * T.c0[q] + tr_basis<3>(k, 1) * T.c1[q];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    double v = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+#pragma unroll
+                        for (int l = 0; l < 3; ++l) v += Wh[k * 3 + p] * Wh[l * 3 + q] * P9[(3 * k + i) * 9 + 3 * l + j];
+                    dst[9 * (3 * p + q) + 3 * i + j] = v;
+                }
+}
+
 // ------------------------------------------------------------------ host side
 static SLPar sl_par(const Ctx& c) {
     SLPar P;
@@ -224,6 +395,33 @@ void strain_limit_grad_add(Ctx& c, const double* x, double* gl) {
 void strain_limit_hess_add(Ctx& c, const double* x, double* hl) {
     if (!strain_limit_on(c)) return;
     hipLaunchKernelGGL(k_sl_hess_add, dim3(nblocks(c.nT, 64)), dim3(64), 0, c.s, x, c.tri.p, c.nT, c.slIB.p, c.vol.p, sl_par(c), c.dbc.p, hl);
+    CIPC_LAUNCH_CHECK();
+}
+
+static FBPar fb_par(const Ctx& c) {
+    FBPar P;
+    for (int i = 0; i < 4; ++i) P.k[i] = c.P.fiberK[i];
+    for (int i = 0; i < 3; ++i) P.L[i] = c.P.fiberLimit[i];
+    P.h2 = c.P.dt * c.P.dt;
+    return P;
+}
+bool fiber_on(const Ctx& c) { return (c.P.fiberK[0] > 0 || c.P.fiberK[1] > 0) && c.nT > 0; }
+double fiber_energy(Ctx& c, const double* x) {
+    const int nb = nblocks(c.nT);
+    double* part = c.red.ensure(nb);
+    hipLaunchKernelGGL(k_fb_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.tri.p, c.nT, c.fbIB.p, c.vol.p, fb_par(c), c.dbc.p, part);
+    CIPC_LAUNCH_CHECK();
+    sum_partials(part, nb, c.scal.p + 51, c.s);
+    CIPC_HIP(hipMemcpyAsync(c.hScal + 51, c.scal.p + 51, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    return c.hScal[51];
+}
+void fiber_grad_local(Ctx& c, const double* x, double* gl) {
+    hipLaunchKernelGGL(k_fb_grad_local, dim3(nblocks(c.nT)), dim3(kBlock), 0, c.s, x, c.tri.p, c.nT, c.fbIB.p, c.vol.p, fb_par(c), c.dbc.p, gl);
+    CIPC_LAUNCH_CHECK();
+}
+void fiber_hess_local(Ctx& c, const double* x, double* hl) {
+    hipLaunchKernelGGL(k_fb_hess_local, dim3(nblocks(c.nT, 64)), dim3(64), 0, c.s, x, c.tri.p, c.nT, c.fbIB.p, c.vol.p, fb_par(c), c.dbc.p, hl);
     CIPC_LAUNCH_CHECK();
 }
 

@@ -79,6 +79,11 @@ typedef struct cipc_scene_desc {
      * the singular values of each triangle's deformation gradient stay below s[0]
      * (barrier from shat[0]); kappa[1] > 0 adds the lower limit s[1] < shat[1]. */
     double strain_limit_kappa[2], strain_limit_s[2], strain_limit_shat[2];
+    /* anisotropic (fibre) membrane: the step's fiberStiffMult and inextLimit arguments
+     * (FEM/Shell/ANISO_INEXT.h).  With fiber_stiff_mult[0] > 0 or [1] > 0 it replaces the
+     * Neo-Hookean membrane: log barriers on the Green strain components E00, E00 E11, E11,
+     * E01 of the rest x-z frame, limits |E00| < fiber_limit[0], |E11| < [1], |E01| < [2]. */
+    double fiber_stiff_mult[4], fiber_limit[3];
 } cipc_scene_desc;
 
 typedef struct cipc_step_stats {
@@ -165,6 +170,9 @@ int cipc_set_dirichlet_targets(cipc_ctx* ctx, const double* targets);
 /* Strain-limit parameters for the following steps (kappa_s, s, sHat of
  * Advance_One_Step_IE_Hinge, IMPLICIT_EULER.h:162); same meaning as in cipc_scene_desc. */
 int cipc_set_strain_limit(cipc_ctx* ctx, const double* kappa, const double* s, const double* shat);
+/* fiberStiffMult (4) and inextLimit (3) for the following steps; same meaning as in
+ * cipc_scene_desc. */
+int cipc_set_fiber(cipc_ctx* ctx, const double* stiff_mult, const double* limit);
 /* Compute_Inextensibility_Energy (FEM/Shell/INEXT.h:120-178) at x: *valid = 0 when a
  * singular value reached a limit (then *e = 0).  Its gradient and Hessian are part of
  * cipc_elastic_gradient and cipc_hessian (which & 1) while strain limiting is on. */

@@ -182,14 +182,14 @@ def test_fiber_derivatives_fd(case):
 
 
 def test_fiber_step_stays_feasible():
-    """the pinned sheet under gravity with the fibre model (limits 0.02): every step ends
-    with each Green strain component inside its limit"""
+    """the pinned sheet under gravity with the fibre model (limits 0.03): every step ends
+    with each Green strain component inside its limit, and the limit is reached closely"""
     sd = hammock(0.0)
-    sd.params.update(fiberStiffMult=(1e2, 0.0, 1e2, 1e2), inextLimit=(0.02, 0.02, 0.02))
+    sd.params.update(fiberStiffMult=(5e3, 1e3, 5e3, 5e3), inextLimit=(0.03, 0.03, 0.03))
     orc = O.OracleScene(sd)
     for _ in range(6):
         rc, _ = orc.step()
         assert rc > 0
     x, _ = orc.get_state()
     e = np.array([orc.fiber_tri(x, t)[4] for t in range(sd.nT)])
-    assert np.abs(e).max() < 0.02 and np.abs(e).max() > 1e-4
+    assert 0.02 < np.abs(e).max() < 0.03

@@ -81,3 +81,55 @@ def test_strain_limited_trajectory(require_gpu, precision):
         assert max_sigma(probe, xg, sd.nT) < 1.01
     assert max_sigma(probe, xg, sd.nT) > 1.0
     gpu.close()
+
+
+def fiber_hammock():
+    sd = hammock(0.0)
+    sd.params.update(fiberStiffMult=(5e3, 1e3, 5e3, 5e3), inextLimit=(0.03, 0.03, 0.03))
+    return sd
+
+
+def test_fiber_terms_match_oracle(require_gpu):
+    """anisotropic (fibre) membrane: elastic gradient and Hessian (fibre replaces the
+    membrane) against the oracle at a sagged state"""
+    sd = fiber_hammock()
+    orc = O.OracleScene(sd)
+    for _ in range(4):
+        rc, _ = orc.step()
+        assert rc > 0
+    x, _ = orc.get_state()
+    gpu = Simulator(sd)
+    gel, _ = orc.gradient(x, np.zeros((0, 4), np.int32), 1.0)      # bending only (membrane replaced)
+    gf = np.zeros_like(x)
+    for t in range(sd.nT):
+        ok, _, g9, _, _ = orc.fiber_tri(x, t)
+        assert ok
+        for a, v in enumerate(sd.tris[t]):
+            gf[3 * v:3 * v + 3] += g9[3 * a:3 * a + 3]
+    assert np.abs(gf).max() > 0
+    assert rel(gpu.elastic_gradient(x), gel + gf) <= 1e-10
+    none = np.zeros((0, 4), np.int32)
+    assert rel(gpu.hessian(x, none, 1.0, which=1).toarray(), orc.hessian(x, none, 1.0, which=1).toarray()) <= 1e-9
+    gpu.close()
+
+
+def test_fiber_trajectory(require_gpu):
+    """five fibre-membrane steps of the pinned sheet, each from the GPU's state, every
+    Green strain component inside its limit"""
+    sd = fiber_hammock()
+    gpu = Simulator(sd, pcg_tol=1e-10)
+    orc = O.OracleScene(sd)
+    for s in range(5):
+        x0, v0 = gpu.get_state()
+        orc.set_state(x0, v0)
+        st = gpu.step()
+        rc, so = orc.step()
+        assert rc > 0 and st["status"] == 0, (s, st, rc)
+        xg, _ = gpu.get_state()
+        xo, _ = orc.get_state()
+        disp = np.abs(xo - x0).max()
+        assert np.abs(xg - xo).max() <= trajectory_bar(sd, disp), (s, st["newton_iters"], so)
+        e = np.array([orc.fiber_tri(xg, t)[4] for t in range(sd.nT)])
+        assert np.abs(e).max() < 0.03
+    assert np.abs(e).max() > 0.02          # the limit is active
+    gpu.close()
