@@ -214,7 +214,7 @@ def make_rod(p0, p1, nseg):
 
 def with_codim(sd, rod_nodes, particles, xi=1e-3, name=None, rod_params=None, particle_params=None):
     """Append rods (list of node arrays, each a polyline) and particles ((n, 3) array) to a
-    scene: initialize_rod / initialize_particle of the reference's driver, offset xi."""
+    scene: initialize_rod / initialize_particle of Python/Drivers/FEMDiscreteShellBase.py, offset xi."""
     X = [sd.X]
     nv = sd.nV
     segs = []
