@@ -138,6 +138,7 @@ PROTOTYPES = {
     "cipc_min_dist2": (ctypes.c_int, [P, D, I32, ctypes.c_int32, D, D]),
     "cipc_ccd_stepsize": (ctypes.c_int, [P, D, D, ctypes.c_double, D, D]),
     "cipc_set_friction": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32]),
+    "cipc_set_friction_components": (ctypes.c_int, [P, ctypes.c_int32, I32, D]),
     "cipc_set_strain_limit": (ctypes.c_int, [P, D, D, D]),
     "cipc_set_fiber": (ctypes.c_int, [P, D, D]),
     "cipc_dirichlet_add_motion": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, D, D, D, ctypes.c_double]),

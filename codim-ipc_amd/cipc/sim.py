@@ -93,6 +93,8 @@ class Simulator:
         if rc != 0:
             raise CipcError(rc, "cipc_create failed")
         self.h = h
+        if P.get("compNodeRange") is not None:
+            self.set_friction_components(P["compNodeRange"], P["muComp"])
         self.nV = sd.nV
         self.nDBC = len(sd.dbc_idx)
         s = np.zeros(6, dtype=np.int32)
@@ -388,6 +390,12 @@ class Simulator:
     def set_friction(self, mu, epsv2=1e-6, iter_amt=1):
         """The step's mu, epsv2 and fricIterAmt arguments (IMPLICIT_EULER.h:167)."""
         self._check(self.lib.cipc_set_friction(self.h, float(mu), float(epsv2), int(iter_amt)))
+
+    def set_friction_components(self, comp_node_range, mu_comp):
+        """compNodeRange / muComp (Compute_Friction_Coef, FRICTION.h:114-171)."""
+        r = np.ascontiguousarray(comp_node_range, dtype=np.int32)
+        m = np.ascontiguousarray(mu_comp, dtype=np.float64).ravel()
+        self._check(self.lib.cipc_set_friction_components(self.h, len(r), _i(r), _d(m)))
 
     def friction_basis(self, x, C, kappa):
         """Compute_Friction_Basis (FRICTION.h:17-112): (stencils (n, 4), info (n, 9)); the

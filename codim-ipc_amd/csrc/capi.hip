@@ -492,6 +492,26 @@ int cipc_set_friction(cipc_ctx* ctx, double mu, double epsv2, int32_t iter_amt) 
     return CIPC_OK;
 }
 
+int cipc_set_friction_components(cipc_ctx* ctx, int32_t n_comp, const int32_t* comp_node_range, const double* mu_comp) {
+    if (!ctx || n_comp < 0 || (n_comp && (!comp_node_range || !mu_comp))) return CIPC_E_INVALID;
+    for (int i = 0; i < n_comp; ++i)
+        if (comp_node_range[i] <= (i ? comp_node_range[i - 1] : 0)) return CIPC_E_INVALID;
+    for (int i = 0; i < n_comp * n_comp; ++i)
+        if (mu_comp[i] < 0) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        c.nFricComp = n_comp;
+        if (n_comp) {
+            c.fricCompRange.ensure_exact(n_comp);
+            c.fricMuComp.ensure_exact((size_t)n_comp * n_comp);
+            CIPC_HIP(hipMemcpyAsync(c.fricCompRange.p, comp_node_range, n_comp * sizeof(int), hipMemcpyHostToDevice, c.s));
+            CIPC_HIP(hipMemcpyAsync(c.fricMuComp.p, mu_comp, (size_t)n_comp * n_comp * sizeof(double), hipMemcpyHostToDevice, c.s));
+            CIPC_HIP(hipStreamSynchronize(c.s));
+        }
+        return CIPC_OK;
+    });
+}
+
 int cipc_friction_basis(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, int32_t* out_cs,
                         double* out_info, int32_t cap, int32_t* n) {
     if (!ctx || !x || !n || n_cs < 0 || (n_cs && !cs) || cap < 0) return CIPC_E_INVALID;

@@ -157,6 +157,9 @@ struct Ctx {
     // lagged friction stencils (FEM/FRICTION.h): SoA [4][nF] encoding, SoA [9][nF] data
     // (closest point 2, tangent basis columns 2 x 3, normal force)
     int nF = 0;
+    int nFricComp = 0;                 // per-component coefficients (0: one mu for all)
+    DevBuf<int> fricCompRange;         // exclusive end vertex of each component
+    DevBuf<double> fricMuComp;         // nFricComp^2, entry c0 + c1 nFricComp
     bool fricOn = false;               // inside a step with mu > 0: potentials include friction
     DevBuf<int> fC;
     DevBuf<double> fI;

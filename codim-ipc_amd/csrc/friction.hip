@@ -63,9 +63,16 @@ __global__ void k_fric_flag(const int* __restrict__ C, int nC, uint32_t* __restr
     head[i] = (c0 >= 0 && (c2 < 0 || c3 < 0)) ? 0u : 1u;
 }
 
+// getCompI (FRICTION.h:114-124): first component whose range end exceeds the vertex
+CIPC_DEV int fric_comp(const int* __restrict__ range, int nComp, int v) {
+    for (int i = 0; i < nComp; ++i)
+        if (v < range[i]) return i;
+    return nComp - 1;
+}
+
 __global__ void k_fric_basis(const double* __restrict__ x, const int* __restrict__ C, int nC, const uint32_t* __restrict__ head,
                              const uint32_t* __restrict__ pos, int nF, double kappa, double xi2, double shat, int* __restrict__ fC,
-                             double* __restrict__ fI) {
+                             double* __restrict__ fI, const int* __restrict__ compRange, int nComp, const double* __restrict__ muComp) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nC || !head[i]) return;
     const int o = (int)pos[i];
@@ -118,7 +125,12 @@ __global__ void k_fric_basis(const double* __restrict__ x, const int* __restrict
             d2 = dist2_pt(p, t0, t1, t2);
         }
     }
-    const double v[9] = {cp0, cp1, b0.x, b0.y, b0.z, b1.x, b1.y, b1.z, -barrier_g(d2 - xi2, shat, kappa) * 2 * sqrt(d2)};
+    double lam = -barrier_g(d2 - xi2, shat, kappa) * 2 * sqrt(d2);
+    if (nComp) {   // Compute_Friction_Coef (FRICTION.h:127-171): coefficient of the two primitives' components
+        const int pa = c[0] >= 0 ? c[0] : -c[0] - 1, pb = c[0] >= 0 ? c[2] : c[1];
+        lam *= muComp[fric_comp(compRange, nComp, pa) + fric_comp(compRange, nComp, pb) * nComp];
+    }
+    const double v[9] = {cp0, cp1, b0.x, b0.y, b0.z, b1.x, b1.y, b1.z, lam};
 #pragma unroll
     for (int k = 0; k < 9; ++k) fI[(long)k * nF + o] = v[k];
 }
@@ -264,6 +276,8 @@ __global__ void k_fric_hess_keys(const int* __restrict__ fC, int nF, int nV, con
 
 // ------------------------------------------------------------------ host side
 static double fric_ev(const Ctx& c) { return std::sqrt(c.P.epsv2 * c.P.dt * c.P.dt); }
+// the coefficient multiplying every stencil: mu, or 1 when it comes per component
+static double fric_mu(const Ctx& c) { return c.nFricComp ? 1.0 : c.P.fricMu; }
 
 void friction_basis(Ctx& c, const double* x, const int* C, int nC, double kappa) {
     c.nF = 0;
@@ -282,7 +296,7 @@ void friction_basis(Ctx& c, const double* x, const int* C, int nC, double kappa)
     c.fI.ensure(9l * nF);
     const double xi2 = c.P.xi * c.P.xi, shat = c.dHat2 + 2 * std::sqrt(c.dHat2) * c.P.xi;
     hipLaunchKernelGGL(k_fric_basis, dim3(nblocks(nC)), dim3(kBlock), 0, c.s, x, C, nC, (const uint32_t*)head,
-                       (const uint32_t*)pos, nF, kappa, xi2, shat, c.fC.p, c.fI.p);
+                       (const uint32_t*)pos, nF, kappa, xi2, shat, c.fC.p, c.fI.p, c.fricCompRange.p, c.nFricComp, c.fricMuComp.p);
     CIPC_LAUNCH_CHECK();
 }
 
@@ -295,7 +309,7 @@ double friction_energy(Ctx& c, const double* x, const double* xn) {
     sum_partials(part, nb, c.scal.p + 48, c.s);
     CIPC_HIP(hipMemcpyAsync(c.hScal + 48, c.scal.p + 48, sizeof(double), hipMemcpyDeviceToHost, c.s));
     CIPC_HIP(hipStreamSynchronize(c.s));
-    return c.P.fricMu * c.hScal[48];
+    return fric_mu(c) * c.hScal[48];
 }
 
 void friction_gradient(Ctx& c, const double* x, const double* xn, double* g) {
@@ -304,7 +318,7 @@ void friction_gradient(Ctx& c, const double* x, const double* xn, double* g) {
     double* gl = c.fGLocal.ensure(3 * ns);
     int* gv = c.fGVert.ensure(ns);
     hipLaunchKernelGGL(k_fric_grad_local, dim3(nblocks(c.nF)), dim3(kBlock), 0, c.s, x, xn, c.fC.p, c.fI.p, c.nF, fric_ev(c),
-                       c.P.fricMu, gl, gv);
+                       fric_mu(c), gl, gv);
     CIPC_LAUNCH_CHECK();
     scatter_local_gradient(c, gl, gv, ns, g);
 }
@@ -312,7 +326,7 @@ void friction_gradient(Ctx& c, const double* x, const double* xn, double* g) {
 void friction_hess_local(Ctx& c, const double* x, const double* xn, double* hl, long ns, long off) {
     if (c.nF == 0) return;
     hipLaunchKernelGGL(k_fric_hess_local, dim3(nblocks(c.nF)), dim3(kBlock), 0, c.s, x, xn, c.fC.p, c.fI.p, c.nF, fric_ev(c),
-                       c.P.fricMu, hl, ns, off);
+                       fric_mu(c), hl, ns, off);
     CIPC_LAUNCH_CHECK();
 }
 

@@ -417,7 +417,7 @@ int advance_one_step(Ctx& c) {
     c.nCprev = 0;
     if (P.collision) { ta = now_ms(); constraint_set(c, c.x.p, c.C, c.nC); st.t_constraint_set_ms += now_ms() - ta; }
     // lagged friction stencils at the step start (IMPLICIT_EULER.h:419-427)
-    c.fricOn = P.collision && P.fricMu > 0;
+    c.fricOn = P.collision && (P.fricMu > 0 || c.nFricComp > 0);
     c.nF = 0;
     if (c.fricOn) friction_basis(c, c.x.p, c.C.p, c.nC, c.kappa);
     int bad = 0;

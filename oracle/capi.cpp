@@ -249,6 +249,11 @@ void orc_set_friction(void* h, double mu, double epsv2, int iterAmt) {
     Params& P = ((Scene*)h)->P;
     P.fricMu = mu; P.epsv2 = epsv2; P.fricIterAmt = iterAmt;
 }
+void orc_set_friction_components(void* h, int n, const int* range, const double* muComp) {
+    Params& P = ((Scene*)h)->P;
+    P.compNodeRange.assign(range, range + n);
+    P.muComp.assign(muComp, muComp + (size_t)n * n);
+}
 static FricSet toF(const int* c, int nF, const double* info) {
     FricSet F;
     F.C = toC(c, nF);

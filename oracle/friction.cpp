@@ -28,6 +28,18 @@ namespace orc {
 static inline V3 X3(const double* x, int i) { return {x[3 * i], x[3 * i + 1], x[3 * i + 2]}; }
 static inline V3 normalized(V3 a) { return (1.0 / std::sqrt(sq(a))) * a; }
 
+bool friction_by_component(const Scene& S) {
+    const size_t nc = S.P.compNodeRange.size();
+    return nc > 0 && S.P.muComp.size() == nc * nc;
+}
+// getCompI (FRICTION.h:114-124): first component whose range end exceeds the vertex
+static int comp_of(const Scene& S, int v) {
+    const std::vector<int>& r = S.P.compNodeRange;
+    for (size_t i = 0; i < r.size(); ++i)
+        if (v < r[i]) return (int)i;
+    return (int)r.size() - 1;
+}
+
 // vertex ids, count and weights of a friction stencil (info: cp0, cp1, basis cols, lambda)
 static int fric_stencil(const Constraint& cc, const double* info, int* vid, double* coef, double* mult) {
     const int* c = cc.c;
@@ -114,6 +126,11 @@ void friction_basis(const Scene& S, const double* x, const std::vector<Constrain
         inf[2] = b0.x; inf[3] = b0.y; inf[4] = b0.z;
         inf[5] = b1.x; inf[6] = b1.y; inf[7] = b1.z;
         inf[8] = -barrier_g(d2 - xi2, shat, kappa) * 2 * std::sqrt(d2);   // stencil weight 1
+        if (friction_by_component(S)) {   // Compute_Friction_Coef, FRICTION.h:127-171
+            const int a = c[0] >= 0 ? c[0] : -c[0] - 1, b = c[0] >= 0 ? c[2] : c[1];
+            const int nc = (int)S.P.compNodeRange.size();
+            inf[8] *= S.P.muComp[comp_of(S, a) + comp_of(S, b) * nc];
+        }
         F.C.push_back(cc);
         F.info.insert(F.info.end(), inf, inf + 9);
     }
@@ -147,6 +164,8 @@ static double f1_div_norm(double q, double ev) {
 }
 
 static double fric_eps(const Scene& S) { return std::sqrt(S.P.epsv2 * S.P.dt * S.P.dt); }
+// the coefficient multiplying every stencil: mu, or 1 when the coefficients come per component
+static double fric_mu(const Scene& S) { return friction_by_component(S) ? 1.0 : S.P.fricMu; }
 
 double friction_energy(const Scene& S, const double* x, const double* xn, const FricSet& F) {
     const double ev = fric_eps(S);
@@ -157,7 +176,7 @@ double friction_energy(const Scene& S, const double* x, const double* xn, const 
         fric_local(x, xn, F.C[k], inf, L);
         E += L.mult * inf[8] * f0_smooth(L.q, ev);
     }
-    return S.P.fricMu * E;
+    return fric_mu(S) * E;
 }
 
 void friction_grad(const Scene& S, const double* x, const double* xn, const FricSet& F, double* g) {
@@ -166,7 +185,7 @@ void friction_grad(const Scene& S, const double* x, const double* xn, const Fric
         FricLocal L;
         const double* inf = &F.info[9 * k];
         fric_local(x, xn, F.C[k], inf, L);
-        const double s = f1_div_norm(L.q, ev) * S.P.fricMu * L.mult * inf[8];
+        const double s = f1_div_norm(L.q, ev) * fric_mu(S) * L.mult * inf[8];
         V3 w = (s * L.u[0]) * L.b0 + (s * L.u[1]) * L.b1;
         for (int a = 0; a < L.nv; ++a) {
             g[3 * L.vid[a]] += L.coef[a] * w.x;
@@ -199,7 +218,7 @@ void friction_hess(const Scene& S, const double* x, const double* xn, const Fric
         FricLocal L;
         const double* inf = &F.info[9 * k];
         fric_local(x, xn, F.C[k], inf, L);
-        const double sc = S.P.fricMu * L.mult * inf[8];
+        const double sc = fric_mu(S) * L.mult * inf[8];
         const double f1d = f1_div_norm(L.q, ev);
         double M[4];
         if (L.q >= ev * ev) {

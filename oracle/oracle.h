@@ -106,6 +106,10 @@ struct Params {
     // lagged friction (Advance_One_Step_IE_Hinge's mu, epsv2, fricIterAmt)
     double fricMu = 0, epsv2 = 1e-6;
     int fricIterAmt = 1;
+    // per-component friction (compNodeRange: exclusive end vertex of each component;
+    // muComp: nComp x nComp coefficients, entry c0 + c1 nComp), FRICTION.h:114-171
+    std::vector<int> compNodeRange;
+    std::vector<double> muComp;
     // strain limiting (the step's kappa_s, s, sHat; upper limit index 0, lower 1)
     double slKappa[2] = {0, 0}, slS[2] = {1.01, 0}, slSHat[2] = {1, 1};
     // anisotropic (fibre) membrane: the step's fiberStiffMult (4) and inextLimit (3)
@@ -180,6 +184,7 @@ struct FricSet {
     std::vector<double> info;
 };
 void friction_basis(const Scene& S, const double* x, const std::vector<Constraint>& C, double kappa, FricSet& F);
+bool friction_by_component(const Scene& S);
 double friction_energy(const Scene& S, const double* x, const double* xn, const FricSet& F);
 void friction_grad(const Scene& S, const double* x, const double* xn, const FricSet& F, double* g);
 void friction_hess(const Scene& S, const double* x, const double* xn, const FricSet& F, std::vector<Block>& out);

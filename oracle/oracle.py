@@ -50,6 +50,7 @@ def lib():
         L.orc_membrane.argtypes = [D, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D, D, D]
         L.orc_solve.argtypes = [ctypes.c_int, I, I, D, D, D, ctypes.c_int, ctypes.c_double, I]
         L.orc_set_friction.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_set_friction_components.argtypes = [ctypes.c_void_p, ctypes.c_int, I, D]
         L.orc_set_strain_limit.argtypes = [ctypes.c_void_p, D, D, D]
         L.orc_set_fiber.argtypes = [ctypes.c_void_p, D, D]
         L.orc_fiber_tri.argtypes = [ctypes.c_void_p, D, ctypes.c_int, ctypes.c_int, D, D, D, D]
@@ -210,6 +211,10 @@ class OracleScene:
         self.nRod, self.nRodHinge, self.nParticle, self.codimBN0, self.codimBN1 = [int(v) for v in s]
         self.nV = sd.nV
         L.orc_set_friction(self.h, float(P.get("mu", 0.0)), float(P.get("epsv2", 1e-6)), int(P.get("fricIterAmt", 1)))
+        if P.get("compNodeRange") is not None:
+            self._comp = (np.ascontiguousarray(P["compNodeRange"], dtype=np.int32),
+                          np.ascontiguousarray(P["muComp"], dtype=np.float64).ravel())
+            L.orc_set_friction_components(self.h, len(self._comp[0]), _ip(self._comp[0]), _dp(self._comp[1]))
         self._sl = [np.array(P.get(k, d), dtype=np.float64) for k, d in
                     (("kappa_s", (0.0, 0.0)), ("s", (1.01, 0.0)), ("sHat", (1.0, 1.0)))]
         L.orc_set_strain_limit(self.h, _dp(self._sl[0]), _dp(self._sl[1]), _dp(self._sl[2]))

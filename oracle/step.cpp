@@ -216,7 +216,7 @@ int advance_one_step(Scene& S, StepStats& st) {
     std::vector<double> d2prev;
     if (P.collision) constraint_set(S, x.data(), C);
     // lagged friction set at the step start, IMPLICIT_EULER.h:419-437
-    const bool fric = P.collision && P.fricMu > 0;
+    const bool fric = P.collision && (P.fricMu > 0 || friction_by_component(S));
     FricSet F;
     const FricSet* Fp = fric ? &F : nullptr;
     if (fric) friction_basis(S, x.data(), C, S.kappa, F);

@@ -162,3 +162,23 @@ def test_friction_codim_multiplicity(require_gpu):
     sd2.params.update(mu=0.3, epsv2=1e-6)
     sd2.params["PNTol"] = 1e-6 * sd2.bbox_diag()
     _trajectory(sd2, 3)
+
+
+def test_friction_components_gpu(require_gpu):
+    """per-component coefficients (Compute_Friction_Coef): scaled normal forces against the
+    oracle, then three steps of the contact stack with a different coefficient per sheet pair"""
+    sd = scenes.scene_contact_stack()
+    mu = np.array([[0.1, 0.2, 0.3, 0.4], [0.2, 0.5, 0.6, 0.7], [0.3, 0.6, 0.8, 0.9], [0.4, 0.7, 0.9, 1.0]])
+    sd.params.update(compNodeRange=[144, 288, 432, sd.nV], muComp=mu.T.ravel(), epsv2=1e-6)
+    gpu = Simulator(sd)
+    orc = O.OracleScene(sd, threads=4)
+    x, _ = orc.get_state()
+    C = orc.constraint_set(x)
+    kappa = orc.scalars()["kappa"]
+    FCg, FIg = gpu.friction_basis(x, C, kappa)
+    FCo, FIo = orc.friction_basis(x, C, kappa)
+    assert np.array_equal(FCg, FCo)
+    assert rel(FIg[:, 8], FIo[:, 8]) <= 1e-12
+    gpu.close()
+    sd.params["PNTol"] = 1e-6 * sd.bbox_diag()
+    _trajectory(sd, 3)

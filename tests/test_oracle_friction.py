@@ -236,3 +236,35 @@ def test_friction_relag_iterations():
         res[it] = (np.abs(tot).max(), np.abs(gb).max())
     assert res[0][0] <= 1e-6 * res[0][1]
     assert res[1][0] >= 1e3 * res[0][0]
+
+
+def test_friction_components():
+    """Compute_Friction_Coef (FRICTION.h:114-171): each stencil's normal force is scaled by
+    the coefficient of its two primitives' components and mu is taken as 1; a uniform
+    coefficient matrix reproduces the single-mu potential."""
+    sd = scenes.scene_contact_stack()
+    nv, nS = sd.nV, 144
+    ranges = [nS, 2 * nS, 3 * nS, nv]
+    mu = np.array([[0.1, 0.2, 0.3, 0.4], [0.2, 0.5, 0.6, 0.7], [0.3, 0.6, 0.8, 0.9], [0.4, 0.7, 0.9, 1.0]])
+    base = O.OracleScene(sd)
+    x, _ = base.get_state()
+    C = base.constraint_set(x)
+    kappa = base.scalars()["kappa"]
+    F0, I0 = base.friction_basis(x, C, kappa)
+    comp = lambda v: int(np.searchsorted(ranges, v, side="right"))
+    for m in (mu, np.full((4, 4), 0.3)):
+        sd2 = scenes.scene_contact_stack()
+        sd2.params.update(compNodeRange=ranges, muComp=m.T.ravel())   # entry c0 + c1 n
+        orc = O.OracleScene(sd2)
+        F1, I1 = orc.friction_basis(x, C, kappa)
+        assert np.array_equal(F0, F1)
+        for c, a, b in zip(F0, I0, I1):
+            pa, pb = (c[0], c[2]) if c[0] >= 0 else (-c[0] - 1, c[1])
+            assert abs(b[8] - m[comp(pa), comp(pb)] * a[8]) <= 1e-15 * a[8]
+    # uniform 0.3 matrix == mu 0.3
+    sd3 = scenes.scene_contact_stack()
+    sd3.params.update(mu=0.3)
+    o3 = O.OracleScene(sd3)
+    xx = x + np.random.default_rng(1).standard_normal(x.size) * 1e-6
+    F3, I3 = o3.friction_basis(x, C, kappa)
+    assert abs(orc.friction_energy(xx, x, F1, I1) - o3.friction_energy(xx, x, F3, I3)) <= 1e-14 * o3.friction_energy(xx, x, F3, I3)
