@@ -128,7 +128,8 @@ def gpu_run(args, sd, form):
                 statuses=sorted(set(s["status"] for s in stats)), min_dist_final=min_d,
                 learned_solves=(info1["solves"] - info0["solves"]) if learned else None,
                 learned_fallbacks=(info1["fallbacks"] - info0["fallbacks"]) if learned else None,
-                spmv_avg_us=roof["avg_us"] if roof else None, spmv_frac=roof["frac"] if roof else None, roofline=roof)
+                spmv_avg_us=roof["avg_us"] if roof else None, spmv_frac=roof["frac"] if roof else None, roofline=roof,
+                _x=x, _energy=stats[-1]["energy"])
 
 
 def batch_run(args):
@@ -243,6 +244,19 @@ def main():
         if os.path.exists(precond.WEIGHTS):
             modes += ["mixed_learned", "fp32_learned"]
     runs = {m: gpu_run(args, sd, m) for m in modes}
+    # each reduced-precision form against the FP64 form after the same steps from the same
+    # state: max position difference relative to the FP64 displacement, and the last
+    # step's incremental-potential difference relative to its magnitude
+    if "fp64" in runs:
+        import numpy as np
+        x64, e64 = runs["fp64"]["_x"], runs["fp64"]["_energy"]
+        disp = float(np.abs(x64 - sd.X.ravel()).max())
+        for m, r in runs.items():
+            r["vs_fp64_max_dx_rel"] = float(np.abs(r["_x"] - x64).max()) / max(disp, 1e-300)
+            r["vs_fp64_energy_rel"] = abs(r["_energy"] - e64) / max(abs(e64), 1e-300)
+    for r in runs.values():
+        r.pop("_x")
+        r.pop("_energy")
     reduced = [m for m in modes if m != "fp64"] or modes
     head = runs[max(reduced, key=lambda m: runs[m]["steps_per_s"])]
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, sd)
@@ -270,7 +284,8 @@ def main():
                    "pcg_tol": args.pcg_tol, "friction_mu": args.mu, "strain_limit": args.strain_limit},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
-        "forms": {m: {k: (round(v, 4) if isinstance(v, float) and k != "min_dist_final" else v) for k, v in r.items() if k != "roofline"}
+        "forms": {m: {k: (round(v, 4) if isinstance(v, float) and k != "min_dist_final" and not k.startswith("vs_fp64") else v)
+                      for k, v in r.items() if k != "roofline"}
                   for m, r in runs.items()},
     }
     if "fp64" in runs and len(runs) > 1:
