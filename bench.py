@@ -192,6 +192,7 @@ def cpu_baseline(args, sd):
         newton += int(st["newton_iters"])
     el = time.perf_counter() - t0
     return dict(value=args.cpu_steps / el, unit="steps/s", cores=threads, kind="port",
+                ms_per_newton_iter=1e3 * el / max(newton, 1), newton_iters_per_step=newton / args.cpu_steps,
                 sample="first %d step(s) of the same scene on the FP64 C++ oracle (block-Jacobi PCG, rel-res %g), "
                        "%d OpenMP threads; %d Newton iterations in %.1f s" % (args.cpu_steps, args.pcg_tol, threads, newton, el))
 
