@@ -496,6 +496,7 @@ int cipc_set_friction_components(cipc_ctx* ctx, int32_t n_comp, const int32_t* c
     if (!ctx || n_comp < 0 || (n_comp && (!comp_node_range || !mu_comp))) return CIPC_E_INVALID;
     for (int i = 0; i < n_comp; ++i)
         if (comp_node_range[i] <= (i ? comp_node_range[i - 1] : 0)) return CIPC_E_INVALID;
+    if (n_comp && comp_node_range[n_comp - 1] < ctx->c.nV) return CIPC_E_INVALID;   // getCompI exits on an uncovered vertex
     for (int i = 0; i < n_comp * n_comp; ++i)
         if (mu_comp[i] < 0) return CIPC_E_INVALID;
     return guarded(ctx, [&]() {

@@ -182,7 +182,7 @@ int cipc_strain_limit_energy(cipc_ctx* ctx, const double* x, double* e, int32_t*
 int cipc_set_friction(cipc_ctx* ctx, double mu, double epsv2, int32_t iter_amt);
 /* Per-component friction coefficients (the step's compNodeRange / muComp arguments,
  * Compute_Friction_Coef FRICTION.h:114-171): component i holds the vertices below
- * comp_node_range[i] (increasing); a stencil's normal force is scaled by
+ * comp_node_range[i] (increasing, the last one >= n_vertices); a stencil's normal force is scaled by
  * mu_comp[c0 + c1 n_comp] for the components of its two primitives and mu is taken as 1.
  * n_comp = 0 returns to the single coefficient. */
 int cipc_set_friction_components(cipc_ctx* ctx, int32_t n_comp, const int32_t* comp_node_range, const double* mu_comp);
