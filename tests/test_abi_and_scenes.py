@@ -59,3 +59,37 @@ def test_splitmix64_known_values():
     u = scenes.splitmix64_uniform(0, 3)
     words = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
     assert np.array_equal(u, np.array([(w >> 11) * 2.0 ** -53 for w in words]))
+
+
+def test_feature_arguments_rejected_without_gpu():
+    """friction, strain-limit, fibre and Dirichlet entry points validate before any device
+    call: a NULL context, and scene descriptions with inconsistent limits"""
+    lib = cipc.load()
+    d3 = (ctypes.c_double * 3)(0, 0, 0)
+    d2 = (ctypes.c_double * 2)(1e3, 0)
+    assert lib.cipc_set_friction(None, 0.1, 1e-6, 1) == -1
+    assert lib.cipc_set_strain_limit(None, d2, d2, d2) == -1
+    assert lib.cipc_set_fiber(None, d3, d3) == -1
+    assert lib.cipc_dirichlet_add_motion(None, 0, 1, d3, d3, d3, 0.0) == -1
+    assert lib.cipc_step_dirichlet(None) == -1
+    assert lib.cipc_set_friction_components(None, 0, None, None) == -1
+    sd = scenes.scene_small(n=3, sheets=1, sphere_level=0)
+    h = ctypes.c_void_p()
+    for field, value in (("friction_mu", -0.1), ("strain_limit", ((1e3, 0.0), (1.0, 0.0), (1.01, 1.0))),
+                         ("fiber", ((1e3, 0.0, 0.0, 0.0), (0.0, 0.1, 0.1)))):
+        desc = _lib.SceneDesc()
+        desc.n_vertices = sd.nV
+        desc.x0 = sd.X.ctypes.data_as(_lib.D)
+        desc.n_triangles = sd.nT
+        desc.triangles = sd.tris.ctypes.data_as(_lib.I32)
+        if field == "friction_mu":
+            desc.friction_mu = value                  # negative coefficient
+        elif field == "strain_limit":
+            for i in range(2):                        # limit s below sHat
+                desc.strain_limit_kappa[i], desc.strain_limit_s[i], desc.strain_limit_shat[i] = value[0][i], value[1][i], value[2][i]
+        else:
+            for i in range(4):
+                desc.fiber_stiff_mult[i] = value[0][i]
+            for i in range(3):                        # a zero limit with the model on
+                desc.fiber_limit[i] = value[1][i]
+        assert lib.cipc_create(ctypes.byref(desc), ctypes.byref(h)) == -1, field
