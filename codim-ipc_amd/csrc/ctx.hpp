@@ -196,6 +196,7 @@ struct Ctx {
     bool warmStart = true;
     double lastWarmRatio = 1.0;       // ||b - H x0|| / ||b|| of the last warm-started solve
     DevBuf<double> warmR, warmX;
+    DevBuf<double> fallbackX, fallbackR;   // learned iterate and its residual when block Jacobi takes over
 
     // launch shapes and broad-phase cell size (set by the autotuner or the caller)
     int spmvGrid = 1024;               // workgroups of the PCG SpMV (multiple of 8)

@@ -768,8 +768,13 @@ static int solve_form(Ctx& c, const double* b, double* xsol, double tol, int max
                                 c.fq.ensure(n), c.fx.ensure(n), c.fb.ensure(n), learned);
 }
 
+static double norm2(Ctx& c, const double* a, long n);
+
 // The learned preconditioner runs with an iteration budget; when it does not converge
-// within it (or meets a non-positive curvature) the solve restarts with block Jacobi.
+// within it (or meets a non-positive curvature) the solve goes on with block Jacobi:
+// from the learned iterate when the budget ran out and that iterate's residual is below
+// half of ||b|| (the correction is solved to tol ||b|| / ||r0||, so the stopping rule
+// ||b - H x|| <= tol ||b|| is unchanged), else from zero.
 static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
     c.lastPrecondFallback = 0;
     const bool learned = c.precond == 1 && !c.lpOffThisStep;
@@ -803,6 +808,24 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
     c.precondFallbacks++;
     c.lpOffThisStep = true;
     int it2 = 0;
+    if (rc == 2) {
+        const long n = 3l * c.nV;
+        double* xl = c.fallbackX.ensure(n);
+        double* r0 = c.fallbackR.ensure(n);
+        CIPC_HIP(hipMemcpyAsync(xl, xsol, n * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+        const double bn2 = norm2(c, b, n);
+        const double rn2 = residual_norm2(c, b, xl, r0);
+        if (rn2 > 0 && rn2 < 0.25 * bn2) {
+            double rr2 = 0;
+            rc = solve_form(c, r0, xsol, tol * std::sqrt(bn2 / rn2), maxIter, &it2, &rr2, false);
+            hipLaunchKernelGGL(k_acc_f64, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const double*)xl, xsol);
+            CIPC_LAUNCH_CHECK();
+            if (relres) *relres = rr2 * std::sqrt(rn2 / bn2);
+            c.lastJacobiIters = it2;
+            if (iters) *iters = it + it2;
+            return rc;
+        }
+    }
     rc = solve_form(c, b, xsol, tol, maxIter, &it2, relres, false);
     c.lastJacobiIters = it2;
     if (iters) *iters = it + it2;
