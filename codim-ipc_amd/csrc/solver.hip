@@ -821,7 +821,7 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
             hipLaunchKernelGGL(k_acc_f64, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const double*)xl, xsol);
             CIPC_LAUNCH_CHECK();
             if (relres) *relres = rr2 * std::sqrt(rn2 / bn2);
-            c.lastJacobiIters = it2;
+            // lastJacobiIters (the learned budget's cap) keeps counting from-zero solves only
             if (iters) *iters = it + it2;
             return rc;
         }

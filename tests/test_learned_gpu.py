@@ -155,3 +155,25 @@ def test_training_cuts_pcg_iterations(setup):
     assert np.linalg.norm(H @ sol - b) <= 2e-6 * np.linalg.norm(b)
     print("block Jacobi %d iterations, learned %d" % (it_j, it_l))
     assert it_l < 0.8 * it_j
+
+
+def test_learned_fallback_continues_from_learned_iterate(setup):
+    """a budget too small for the (untrained, point-Jacobi) learned solve: block Jacobi
+    takes over from the learned iterate and the FP64 residual bar still holds"""
+    sd, gpu, orc, states = setup
+    x = states[1]
+    C = orc.constraint_set(x)
+    kappa = orc.scalars()["kappa"]
+    H = orc.hessian(x, C, kappa, which=3)
+    b = np.random.default_rng(4).standard_normal(x.size)
+    b.reshape(-1, 3)[sd.dbc_idx] = 0
+    gpu.precond_init(1)
+    gpu.set_preconditioner("learned", budget=30)
+    try:
+        sol, it, rr = gpu.linear_solve(x, C, kappa, b, tol=1e-8)
+        assert gpu.precond_info()["fallback"] == 1
+    finally:
+        gpu.set_preconditioner("jacobi")
+    assert it > 30
+    assert rr <= 1e-8
+    assert np.linalg.norm(H @ sol - b) <= 2e-8 * np.linalg.norm(b)
