@@ -9,11 +9,14 @@ from cipc import BatchStepper, Simulator, scenes
 pytestmark = pytest.mark.gpu
 
 
-def test_batch_equals_sequential(require_gpu):
+@pytest.mark.parametrize("features", ["plain", "friction+strain-limit"])
+def test_batch_equals_sequential(require_gpu, features):
     sds = scenes.scene_c5(seed=7, count=6, tri_min=500, tri_max=3000)
     for sd in sds:      # start just above the sphere so that contact happens within the steps
         top = sd.X[sd.n_cloth:, 1].max()
         sd.X[:sd.n_cloth, 1] += top + 0.004 - sd.X[:sd.n_cloth, 1].min()
+        if features != "plain":
+            sd.params.update(mu=0.3, epsv2=1e-6, kappa_s=(1e3, 0.0), s=(1.05, 0.0), sHat=(1.0, 1.0))
     a = [Simulator(sd) for sd in sds]
     b = [Simulator(sd) for sd in sds]
     batch = BatchStepper(a, threads=4)
