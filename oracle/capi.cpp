@@ -122,6 +122,19 @@ void orc_codim_arrays(void* h, double* rodInfo, int* rodHinge, double* rodHingeI
 void orc_scene_destroy(void* h) { delete (Scene*)h; }
 void orc_set_threads(void* h, int t) { ((Scene*)h)->P.threads = t > 0 ? t : 1; }
 void orc_set_solver(void* h, int solver, double tol) { ((Scene*)h)->P.solver = solver; ((Scene*)h)->P.pcgTol = tol; }
+// the strain-limit stiffness kappa_s[0] as the steps have adapted it (IMPLICIT_EULER.h:630)
+double orc_get_kappa_s(void* h) { return ((Scene*)h)->P.slKappa[0]; }
+// k > 0: orc_step returns after k Newton iterations (positions left at that iterate)
+void orc_set_max_newton(void* h, int k) { ((Scene*)h)->P.maxNewton = k > 0 ? k : 0; }
+// 0: Hessians are assembled from the unprojected local matrices (test switch)
+void orc_set_project_pd(void* h, int on) { ((Scene*)h)->P.projectPD = on ? 1 : 0; }
+// incremental potential before / after each Newton iteration's line search in the last step
+int orc_energy_trace(void* h, double* before, double* after, int cap) {
+    Scene* S = (Scene*)h;
+    int n = (int)S->eBefore.size();
+    for (int i = 0; i < n && i < cap; ++i) { before[i] = S->eBefore[i]; after[i] = S->eAfter[i]; }
+    return n;
+}
 // sizes: nT, nBN, nBE, nBT, nHinge
 void orc_scene_sizes(void* h, int* out) {
     Scene* S = (Scene*)h;
@@ -345,8 +358,10 @@ int orc_step(void* h, double* stats) {
     StepStats st;
     int rc = advance_one_step(*(Scene*)h, st);
     if (stats) {
-        double s[15] = {(double)st.newtonIters, (double)st.nContacts, (double)st.pcgIters, (double)st.useGDCount, (double)st.lsTrials,
-                        st.kappa, st.lastRes, st.E, st.tHessian, st.tSolve, st.tCS, st.tCCD, st.tEnergy, st.tGrad, st.tTotal};
+        const Scene& S = *(Scene*)h;
+        double s[17] = {(double)st.newtonIters, (double)st.nContacts, (double)st.pcgIters, (double)st.useGDCount, (double)st.lsTrials,
+                        st.kappa, st.lastRes, st.E, st.tHessian, st.tSolve, st.tCS, st.tCCD, st.tEnergy, st.tGrad, st.tTotal,
+                        S.P.slKappa[0], (double)st.kappaSDoublings};
         std::memcpy(stats, s, sizeof(s));
     }
     return rc;

@@ -211,7 +211,7 @@ void codim_hess(const Scene& S, const double* x, std::vector<Block>& out) {
         int v[2] = {S.rod[2 * r], S.rod[2 * r + 1]};
         double H[36];
         rod_spring_hess(X3(x, v[0]), X3(x, v[1]), S.rodInfo[3 * r], S.rodInfo[3 * r + 1], H);
-        make_pd(6, H);
+        if (S.P.projectPD) make_pd(6, H);
         double w = spring_w(S, r);
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) {
@@ -226,7 +226,7 @@ void codim_hess(const Scene& S, const double* x, std::vector<Block>& out) {
         const int* v = &S.rodHinge[3 * h];
         double H[81];
         rod_kappa2_hess(X3(x, v[0]), X3(x, v[1]), X3(x, v[2]), H);
-        make_pd(9, H);
+        if (S.P.projectPD) make_pd(9, H);
         double w = bend_w(S, h);
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) {

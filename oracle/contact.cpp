@@ -352,7 +352,7 @@ static bool eval_constraint(const Scene& S, const double* x, const Constraint& c
                 double bh = barrier_h(s, shat, kappa);
                 for (int i = 0; i < n; ++i)
                     for (int j = 0; j < n; ++j) L.H[i * n + j] = mult * (bh * gd[i] * gd[j] + bg * Hd[i * n + j]);
-                make_pd(n, L.H);
+                if (S.P.projectPD) make_pd(n, L.H);
             }
         }
     } else {
@@ -371,7 +371,7 @@ static bool eval_constraint(const Scene& S, const double* x, const Constraint& c
                     for (int j = 0; j < 12; ++j)
                         L.H[i * 12 + j] = bg * gd[i] * ge[j] + bg * ge[i] * gd[j] + b * He[i * 12 + j]
                                         + e * bh * gd[i] * gd[j] + e * bg * Hd[i * 12 + j];
-                make_pd(12, L.H);
+                if (S.P.projectPD) make_pd(12, L.H);
             }
         }
     }

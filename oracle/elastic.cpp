@@ -94,7 +94,7 @@ void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
         double H[81];
         membrane_hess(X3(x, v[0]), X3(x, v[1]), X3(x, v[2]), &S.Binv[3 * t], S.vol[t], S.lam, S.mu, H);
         for (int i = 0; i < 81; ++i) H[i] *= h2;
-        make_pd(9, H);
+        if (S.P.projectPD) make_pd(9, H);
         Block* dst = &out[base + offT[t]];
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) {
@@ -125,7 +125,7 @@ void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
         double s2 = h2 * S.kBend * 2 * I[1] / I[2];
         for (int i = 0; i < 12; ++i)
             for (int j = 0; j < 12; ++j) H[i * 12 + j] = s1 * H[i * 12 + j] + s2 * gl[i] * gl[j];
-        make_pd(12, H);
+        if (S.P.projectPD) make_pd(12, H);
         Block* dst = &out[baseH + offH[h]];
         for (int a = 0; a < 4; ++a)
             for (int b = 0; b < 4; ++b) {

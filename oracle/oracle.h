@@ -118,6 +118,8 @@ struct Params {
     double pcgTol = 1e-5;
     int pcgMaxIter = 100000;
     int threads = 1;
+    int maxNewton = 0;         // > 0: return after this many Newton iterations (a timed prefix of a step)
+    int projectPD = 1;         // 0: local Hessians assembled unprojected (finite-difference pins of the sums)
 };
 
 struct Scene {
@@ -151,6 +153,9 @@ struct Scene {
     // Dirichlet motion groups (FEM/BOUNDARY_CONDITION.h DBC_MOTION): entries [begin, end)
     struct DBCMotion { int begin, end; double v[3], center[3], axis[3], angVelDeg; };
     std::vector<DBCMotion> dbcMotion;
+    // per Newton iteration of the last step: incremental potential before and after the
+    // line search (the accepted iterate)
+    std::vector<double> eBefore, eAfter;
 };
 // Step_Dirichlet (BOUNDARY_CONDITION.h:107-155): rotate then translate every group's targets
 void step_dirichlet(Scene& S);
@@ -203,6 +208,7 @@ bool strain_limit_energy(const Scene& S, const double* x, double* E);
 void strain_limit_grad(const Scene& S, const double* x, double* g);
 void strain_limit_hess(const Scene& S, const double* x, std::vector<Block>& out);
 bool strain_limit_tri(const Scene& S, const double* x, int t, double* E, double* g9, double* H81, bool proj, double* sig);
+void strain_max_sv(const Scene& S, const double* x, std::vector<double>& smax);
 // anisotropic (fibre) membrane (FEM/Shell/ANISO_INEXT.h), replaces the Neo-Hookean membrane
 // when fiberK[0] > 0 or fiberK[1] > 0; energy adds to *E and returns false when infeasible
 void fiber_setup(Scene& S);
@@ -223,6 +229,7 @@ struct StepStats {
     int newtonIters = 0, nContacts = 0, pcgIters = 0, useGDCount = 0, lsTrials = 0;
     double kappa = 0, lastRes = 0, E = 0;
     double tHessian = 0, tSolve = 0, tCS = 0, tCCD = 0, tEnergy = 0, tGrad = 0, tTotal = 0;
+    int kappaSDoublings = 0;
 };
 int advance_one_step(Scene& S, StepStats& st);
 

@@ -42,6 +42,11 @@ def lib():
         L.orc_set_kappa.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_ccd.argtypes = [ctypes.c_void_p, D, D, ctypes.c_double, D]
         L.orc_set_solver.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.orc_set_max_newton.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_get_kappa_s.argtypes = [ctypes.c_void_p]
+        L.orc_get_kappa_s.restype = ctypes.c_double
+        L.orc_set_project_pd.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_energy_trace.argtypes = [ctypes.c_void_p, D, D, ctypes.c_int]
         L.orc_barrier.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, D]
         L.orc_energy.argtypes = [ctypes.c_void_p, D, D, I, ctypes.c_int, ctypes.c_double, D]
         L.orc_gradient.argtypes = [ctypes.c_void_p, D, I, ctypes.c_int, ctypes.c_double, D, D]
@@ -233,6 +238,29 @@ class OracleScene:
 
     def set_solver(self, solver, tol=1e-5):
         lib().orc_set_solver(self.h, solver, tol)
+
+    def set_strain_limit(self, kappa, s, sHat):
+        """the step's kappa_s, s, sHat (kappa_s[0] is then adapted in place by the steps)"""
+        self._sl = [np.ascontiguousarray(v, dtype=np.float64) for v in (kappa, s, sHat)]
+        lib().orc_set_strain_limit(self.h, _dp(self._sl[0]), _dp(self._sl[1]), _dp(self._sl[2]))
+
+    def kappa_s(self):
+        return lib().orc_get_kappa_s(self.h)
+
+    def set_project_pd(self, on):
+        """False: Hessians are assembled from unprojected local matrices (finite-difference pins)"""
+        lib().orc_set_project_pd(self.h, 1 if on else 0)
+
+    def energy_trace(self):
+        """(before, after): incremental potential before and after each Newton iteration's line
+        search in the last step"""
+        b, a = np.zeros(4096), np.zeros(4096)
+        n = lib().orc_energy_trace(self.h, _dp(b), _dp(a), 4096)
+        return b[:n], a[:n]
+
+    def set_max_newton(self, k):
+        """k > 0: step() returns after k Newton iterations (a timed prefix of a step); 0: off"""
+        lib().orc_set_max_newton(self.h, int(k))
 
     def scalars(self):
         out = np.zeros(6)
@@ -426,9 +454,10 @@ class OracleScene:
         return a, cl[0]
 
     STAT_NAMES = ("newton_iters", "contacts", "pcg_iters", "gd_fallbacks", "ls_trials", "kappa", "last_res", "E",
-                  "t_hessian", "t_solve", "t_constraint_set", "t_ccd", "t_energy", "t_grad", "t_total")
+                  "t_hessian", "t_solve", "t_constraint_set", "t_ccd", "t_energy", "t_grad", "t_total", "kappa_s",
+                  "kappa_s_doublings")
 
     def step(self):
-        st = np.zeros(15)
+        st = np.zeros(17)
         rc = lib().orc_step(self.h, _dp(st))
         return rc, dict(zip(self.STAT_NAMES, st.tolist()))

@@ -11,8 +11,8 @@
 // envelope Cholesky factorisation; a non-positive pivot reports "not SPD" exactly
 // where CHOLMOD reports CHOLMOD_NOT_POSDEF.  solve_pcg() is the block-Jacobi
 // preconditioned conjugate gradient the GPU path uses, with the same stopping rule.
-// Parity pins: ||Ax-b||/||b|| <= 1e-12 and agreement with scipy.sparse.linalg.spsolve
-// on committed fixtures (tests/test_oracle_solver.py).
+// Parity pins: agreement with scipy.sparse.linalg.spsolve on contact-stack Hessians
+// (tests/test_oracle_contact.py).
 #include "oracle.h"
 #include <algorithm>
 #include <parallel/algorithm>

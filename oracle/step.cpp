@@ -203,6 +203,24 @@ int advance_one_step(Scene& S, StepStats& st) {
     }
     double dbcAlpha = 1;
     if (P.collision) dbcAlpha = ccd_stepsize(S, x.data(), disp.data(), 1.0);
+    // a Dirichlet motion that would break a strain limit or the fibre limits is cut back
+    // by halving until the moved nodes leave every triangle feasible, IMPLICIT_EULER.h:
+    // 336-376 (only dbcAlpha survives: the nodes return to x_n below)
+    if ((P.slKappa[0] > 0 || P.fiberK[0] > 0) && !S.dbcIdx.empty()) {
+        for (int it = 0; it < 64; ++it) {
+            for (size_t k = 0; k < S.dbcIdx.size(); ++k)
+                for (int d = 0; d < 3; ++d) {
+                    const int i = 3 * S.dbcIdx[k] + d;
+                    x[i] = xn[i] + dbcAlpha * (S.dbcTarget[3 * k + d] - xn[i]);
+                }
+            double Et = 0;
+            bool ok = strain_limit_energy(S, x.data(), &Et);
+            if (fiber_active(S)) ok = fiber_energy(S, x.data(), &Et) && ok;
+            if (ok) break;
+            dbcAlpha /= 2;
+        }
+        x = xn;
+    }
     double dbcStiff = 0, dbcPenXn = 0;
     if (dbcAlpha == 1) {
         for (size_t k = 0; k < S.dbcIdx.size(); ++k)
@@ -213,7 +231,7 @@ int advance_one_step(Scene& S, StepStats& st) {
     }
 
     std::vector<Constraint> C, Cprev;
-    std::vector<double> d2prev;
+    std::vector<double> d2prev, strainPrev;
     if (P.collision) constraint_set(S, x.data(), C);
     // lagged friction set at the step start, IMPLICIT_EULER.h:419-437
     const bool fric = P.collision && (P.fricMu > 0 || friction_by_component(S));
@@ -226,6 +244,7 @@ int advance_one_step(Scene& S, StepStats& st) {
     if (bad) return -1;
     if (invalid) return -6;   // strain limit violated by the initial state
     int fricIterI = 0;
+    S.eBefore.clear(); S.eAfter.clear();
     std::deque<double> resRecord;
     bool useGD = false;
     double L2Norm = 0;
@@ -326,6 +345,8 @@ int advance_one_step(Scene& S, StepStats& st) {
             if (bad && valid) return -1;
             alpha /= 2.0;
         } while (E > Eprev || !valid);
+        S.eBefore.push_back(Eprev);
+        S.eAfter.push_back(E);
         Eprev = E;
         // kappa adaptation, IMPLICIT_EULER.h:566-595
         if (!Cprev.empty()) {
@@ -342,6 +363,24 @@ int advance_one_step(Scene& S, StepStats& st) {
         std::vector<double> d2;
         if (!C.empty()) constraint_dist2(S, x.data(), C, d2);
         Cprev = C; d2prev = d2;
+        // kappa_s adaptation, IMPLICIT_EULER.h:606-645: double the upper strain-limit
+        // stiffness (up to 1e5) when a triangle's largest singular value stayed within
+        // 1e-4 (s - sHat) of the limit in this and the previous Newton iteration
+        if (P.slKappa[0] > 0) {
+            std::vector<double> smax;
+            strain_max_sv(S, x.data(), smax);
+            const double thr = 1.0e-4 * (P.slS[0] - P.slSHat[0]);
+            bool upd = false;
+            if (!strainPrev.empty())
+                for (int t = 0; t < S.nT && !upd; ++t)
+                    upd = (P.slS[0] - strainPrev[t] < thr) && (P.slS[0] - smax[t] < thr);
+            if (upd && P.slKappa[0] < 1e5) {
+                S.P.slKappa[0] *= 2;
+                Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, nullptr, dbcStiff);
+                st.kappaSDoublings++;
+            }
+            strainPrev.swap(smax);
+        }
         // stopping rule, IMPLICIT_EULER.h:647-710, 866
         double s = 0;
         for (int i = 0; i < n; ++i) s += sol[i] * sol[i];
@@ -362,6 +401,7 @@ int advance_one_step(Scene& S, StepStats& st) {
         }
         st.lastRes = cur;
         if (PNIter > 2000) return -4;
+        if (P.maxNewton > 0 && PNIter >= P.maxNewton) break;   // timed prefix: stop at this iterate
         // penalty continuation, IMPLICIT_EULER.h:712-768
         if (dbcStiff) {
             const double progress = 1 - std::sqrt(dbc_dist2(S, x.data()) / dbcPenXn);

@@ -182,6 +182,18 @@ bool strain_limit_energy(const Scene& S, const double* x, double* E) {
     return valid;
 }
 
+// largest singular value of each triangle's F (1 for a triangle whose three nodes are
+// Dirichlet), Compute_Inextensibility (INEXT.h:72-114)
+void strain_max_sv(const Scene& S, const double* x, std::vector<double>& smax) {
+    smax.assign(S.nT, 1.0);
+    for (int t = 0; t < S.nT; ++t) {
+        if (tri_all_dbc(S, t)) continue;
+        SLTri T;
+        sl_svd(S, x, t, T);
+        smax[t] = T.sig[0];
+    }
+}
+
 void strain_limit_grad(const Scene& S, const double* x, double* g) {
     if (!sl_on(S)) return;
     for (int t = 0; t < S.nT; ++t) {

@@ -101,3 +101,38 @@ def _dbc_disp(sd, x, tgt):
     d = np.zeros_like(x).reshape(-1, 3)
     d[sd.dbc_idx] = tgt - x.reshape(-1, 3)[sd.dbc_idx]
     return d.ravel()
+
+
+def strain_limited_translation(n=9, speed=0.3):
+    """the pinned sheet with the 1.01 strain limit and all four corners translated along x
+    at `speed` (3 mm per step at 0.3 m/s: moved alone, the corners would stretch their
+    triangles by 8%, while the translated target shape itself is unstretched)"""
+    sd = pinned_sheet(n)
+    sd.params.update(kappa_s=(1e3, 0.0), s=(1.01, 0.0), sHat=(1.0, 1.0), collision=0)
+    return sd
+
+
+def test_motion_breaking_strain_limit_backtracks():
+    """IMPLICIT_EULER.h:336-376: the Dirichlet displacement is halved until the moved nodes
+    leave every triangle inside the strain limit; a cut motion takes the penalty path, which
+    brings the corners to within 1% of their targets with every triangle feasible.  Moving
+    the corners the whole way would start the step from an infeasible state."""
+    sd = strain_limited_translation()
+    orc = O.OracleScene(sd)
+    orc.dirichlet_add_motion(0, 4, (0.3, 0.0, 0.0))
+    probe = O.OracleScene(sd)
+    for s in range(3):
+        x0, _ = orc.get_state()
+        orc.step_dirichlet()
+        tgt = orc.dirichlet_targets()
+        moved = x0.reshape(-1, 3).copy()
+        moved[sd.dbc_idx] = tgt
+        assert not probe.strain_energy_grad(moved.ravel())[0]    # moving only the corners is infeasible
+        rc, _ = orc.step()
+        assert rc > 0, s
+        x, _ = orc.get_state()
+        X, X0 = x.reshape(-1, 3), x0.reshape(-1, 3)
+        gap = np.linalg.norm(tgt - X[sd.dbc_idx], axis=1)
+        full = np.linalg.norm(tgt - X0[sd.dbc_idx], axis=1)
+        assert np.sqrt((gap ** 2).sum() / (full ** 2).sum()) <= 0.01 + 1e-9
+        assert probe.strain_energy_grad(x)[0]

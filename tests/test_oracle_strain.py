@@ -193,3 +193,24 @@ def test_fiber_step_stays_feasible():
     x, _ = orc.get_state()
     e = np.array([orc.fiber_tri(x, t)[4] for t in range(sd.nT)])
     assert 0.02 < np.abs(e).max() < 0.03
+
+
+def test_kappa_s_doubles_when_a_triangle_stays_at_the_limit():
+    """IMPLICIT_EULER.h:606-645: with a weak barrier (kappa_s = 1) gravity drives the pinned
+    sheet's corner triangles to within 1e-4 (s - sHat) of the limit in successive Newton
+    iterations of the fourth step, and kappa_s doubles there (three times, to 8) and keeps
+    the new value in later steps; with kappa_s = 1e3 the limit is never approached that
+    closely and it never changes; every state stays inside the limit"""
+    for k0, expect in ((1.0, [1.0, 1.0, 1.0, 8.0, 8.0]), (1e3, [1e3] * 5)):
+        sd = hammock(k0)
+        orc = O.OracleScene(sd)
+        got = []
+        for _ in range(5):
+            rc, st = orc.step()
+            assert rc > 0
+            got.append(st["kappa_s"])
+            assert st["kappa_s"] == k0 * 2.0 ** round(np.log2(st["kappa_s"] / k0))
+            x, _ = orc.get_state()
+            assert max_sigma(orc, x, sd.nT) < 1.01
+        assert got == expect, got
+        assert orc.kappa_s() == expect[-1]
