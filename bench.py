@@ -5,28 +5,36 @@ triangles) falling onto a Dirichlet icosphere (r 0.2 m, level 4) and a 2-triangl
 rho 200, E 1e6, nu 0.3, t 1 mm, d_hat 1 mm, dt 0.01 s, Newton tol 1e-2 * bbox diagonal,
 Newton linear solve to relative residual 1e-5.  A "step" is one call of the implicit-Euler
 step (the reference's Advance_One_Step_IE_Hinge).  Inputs are generated on the host and
-uploaded before the timed region; positions stay resident in HBM.  The default timed
-window is steps 27-30 of the 50-step run: the 26 warm-up steps are the free fall (about
-4 Newton iterations per step, no contact), the timed ones the sheets landing on the
-sphere and on each other (tens of thousands of active constraints, about 10 Newton
-iterations per step); the warm-up rate is reported beside the timed one.
+uploaded before the timed region; positions stay resident in HBM.
 
-Solver forms timed on the same scene, each from the initial state:
-  fp64           block-Jacobi PCG in FP64;
+Window: the free fall (the first --fast-forward steps, 26 by default: about 4 Newton
+iterations and no contact per step) is simulated once, untimed, in the FP64 form; its end
+state (positions, velocities, barrier stiffness) is handed to every solver form, which
+then runs --warmup untimed and --steps timed steps from it.  Whatever --warmup and --steps
+are, the timed steps are therefore the sheets landing on the sphere and on each other
+(tens of thousands of active constraints, about 10 Newton iterations per step); contacts
+and Newton iterations of every timed step are reported.
+
+Solver forms, each from the same hand-over state:
+  fp64           block-Jacobi PCG in FP64 (the plain FP64 form);
+  fp32_learned   the mixed-precision learned form (headline `value`): FP32 Hessian blocks,
+                 FP32 Krylov vectors and SpMV accumulation inside FP64 iterative
+                 refinement on the true residual, preconditioned by the learned factor
+                 (weights trained by cipc.precond, codim-ipc_amd/cipc/weights/precond.npz),
+                 falling back to block Jacobi when its iteration budget runs out;
+and with --precision all also
   mixed          FP32 Hessian blocks, FP64 Krylov vectors / SpMV accumulation;
-  fp32           FP32 Hessian blocks, Krylov vectors and SpMV accumulation;
-  mixed_learned  the mixed form preconditioned by the learned factor (weights trained by
-  fp32_learned   cipc.precond, codim-ipc_amd/cipc/weights/precond.npz), falling back to
-                 block Jacobi when its iteration budget runs out; the same for fp32;
-the reduced ones inside FP64 iterative refinement stopped on the FP64 residual (same
-1e-5 relative bar).  `value` is the steps/s of the fastest reduced-precision form (named
-in `mode`); every form's figures are reported under `forms`.
+  fp32           FP32 blocks, vectors and accumulation, block-Jacobi;
+  mixed_learned  the mixed form with the learned factor.
+Every form meets the same FP64 relative residual 1e-5 in every Newton solve.
 
 The JSON line carries the roofline of the dominant kernel of the headline form (the PCG
 SpMV, timed from its own dispatch timestamps on the library's stream during the timed
-steps) and a CPU baseline: the FP64 oracle (a C++ restatement of the reference path, same
-PCG) on a bounded sample -- the first step of the same scene (free fall: the oracle would
-need minutes per contact step) -- on the host's cores.
+steps) and a CPU baseline: the FP64 oracle (a C++ restatement of the reference path) on
+a bounded sample -- the first Newton iterations of the first timed step, from the same
+state -- on the host's physical cores with OMP_PROC_BIND=close; its steps/s is that
+sample's ms per Newton iteration times the Newton iterations per step of the FP64 form
+over the timed steps (the same algorithm and stopping rule).
 """
 import argparse
 import json
@@ -59,25 +67,50 @@ MODE = {"fp64": "plain FP64, block-Jacobi PCG",
 PRECISION = {"fp64": "fp64", "mixed": "mixed", "fp32": "fp32", "mixed_learned": "mixed", "fp32_learned": "fp32"}
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the committed FETCH_SIZE /
-    WRITE_SIZE summary (profiles/), or None when no measurement matches."""
+def measured_traffic(kernel, workload, window):
+    """HBM bytes per launch of `kernel` on `workload` over the timed `window` (first and
+    last step) from the committed FETCH_SIZE / WRITE_SIZE summary (profiles/), or None
+    when no measurement matches."""
     path = os.path.join(ROOT, "profiles", "k_pcg_spmv_c2_hbm_traffic.json")
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
         t = json.load(f)
-    if not kernel.startswith(t["kernel"] + " ") or t["workload"] != workload:
+    if not kernel.startswith(t["kernel"] + " ") or t["workload"] != workload or t.get("window") != list(window):
         return None, None
     return int(t["traffic_bytes_per_launch"]), t["summary"]
 
 
-def gpu_run(args, sd, form):
+def fast_forward(args, sd):
+    """the untimed free fall shared by every form: (x, v, kappa) after args.fast_forward
+    FP64 steps from the initial state"""
+    import cipc
+    x, v = sd.X.ravel().copy(), None
+    kappa = None
+    if args.fast_forward <= 0:
+        return x, v, kappa, []
+    sim = cipc.Simulator(sd, pcg_tol=args.pcg_tol, precision="fp64")
+    stats = []
+    for k in range(args.fast_forward):
+        stats.append(sim.step())
+        print("[free fall] step %d/%d: %d contacts" % (k + 1, args.fast_forward, stats[-1]["contacts"]), file=sys.stderr, flush=True)
+    x, v = sim.get_state()
+    kappa = sim.scalars()["kappa"]
+    sim.close()
+    return x, v, kappa, stats
+
+
+def gpu_run(args, sd, form, state):
     import cipc
     import numpy as np
-    precision = form
+    x0, v0, kappa0 = state
     sim = cipc.Simulator(sd, pcg_tol=args.pcg_tol, precision=PRECISION[form])
-    if form.endswith("_learned"):
+    if v0 is not None:
+        sim.set_state(x0, v0)
+    if kappa0 is not None:
+        sim.set_kappa(kappa0)
+    learned = form.endswith("_learned")
+    if learned:
         from cipc import precond
         sim.precond_set_weights(precond.load())
         sim.set_preconditioner("learned", budget=args.learned_budget, reuse=args.learned_reuse)
@@ -87,16 +120,18 @@ def gpu_run(args, sd, form):
     wstats = []
     for k in range(args.warmup):
         wstats.append(sim.step())
-        print("[%s] warm-up step %d/%d" % (form, k + 1, args.warmup), file=sys.stderr, flush=True)
+        print("[%s] warm-up step %d/%d: %d contacts, %d Newton" % (form, k + 1, args.warmup, wstats[-1]["contacts"],
+                                                                  wstats[-1]["newton_iters"]), file=sys.stderr, flush=True)
     tw = time.perf_counter() - tw
+    start_state = sim.get_state() + (sim.scalars()["kappa"],)
     sim.set_profiling(True)
     stats = []
-    learned = form.endswith("_learned")
     info0 = sim.precond_info() if learned else None
     t0 = time.perf_counter()
     for k in range(args.steps):
         stats.append(sim.step())
-        print("[%s] step %d/%d" % (form, k + 1, args.steps), file=sys.stderr, flush=True)
+        print("[%s] step %d/%d: %d contacts, %d Newton" % (form, k + 1, args.steps, stats[-1]["contacts"], stats[-1]["newton_iters"]),
+              file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     info1 = sim.precond_info() if learned else None
     prof = sim.profile()
@@ -109,27 +144,28 @@ def gpu_run(args, sd, form):
     stage_ms = {k[2:-3]: round(sum(s[k] for s in stats) / args.steps, 2)
                 for k in ("t_total_ms", "t_constraint_set_ms", "t_ccd_ms", "t_hessian_ms", "t_solve_ms", "t_energy_ms", "t_gradient_ms")}
     roof = None
+    window = (args.fast_forward + args.warmup + 1, args.fast_forward + args.warmup + args.steps)
     if prof["launches"]:
         avg_s = prof["total_ms"] / prof["launches"] / 1e3
         ach = prof["bytes_per_launch"] / avg_s / 1e9
-        kname = kernel_name(precision, sim.tuning())
-        traffic, source = measured_traffic(kname, sd.name)
+        kname = kernel_name(form, sim.tuning())
+        traffic, source = measured_traffic(kname, sd.name, window)
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=traffic, traffic_source=source, kernel=kname, launches_timed=prof["launches"],
-                    pcg_iterations=pcg,
-                    avg_us=round(avg_s * 1e6, 2),
+                    traffic=traffic, traffic_source=source, kernel=kname, window_steps=list(window),
+                    launches_timed=prof["launches"], pcg_iterations=pcg, avg_us=round(avg_s * 1e6, 2),
                     bytes_per_launch=int(prof["bytes_per_launch"]), nnzb=prof["nnzb"], block_rows=prof["block_rows"])
-    wn = sum(s["newton_iters"] for s in wstats[1:])
-    return dict(precision=precision, steps_per_s=args.steps / elapsed,
+    return dict(precision=PRECISION[form], steps_per_s=args.steps / elapsed,
                 warmup_steps_per_s=(args.warmup / tw) if args.warmup else None, tuning=sim.tuning(),
-                warmup_pcg_iters_per_newton=(sum(s["pcg_iters"] for s in wstats[1:]) / wn) if wn else None, ms_per_step=1e3 * elapsed / args.steps,
+                ms_per_step=1e3 * elapsed / args.steps,
                 ms_per_newton_iter=1e3 * elapsed / max(newton, 1), newton_iters_per_step=newton / args.steps,
-                pcg_iters_per_newton=pcg / max(newton, 1), stage_ms_per_step=stage_ms, contacts_last_step=stats[-1]["contacts"],
+                pcg_iters_per_newton=pcg / max(newton, 1), stage_ms_per_step=stage_ms,
+                contacts_per_timed_step=[int(s["contacts"]) for s in stats],
+                newton_iters_per_timed_step=[int(s["newton_iters"]) for s in stats],
                 statuses=sorted(set(s["status"] for s in stats)), min_dist_final=min_d,
                 learned_solves=(info1["solves"] - info0["solves"]) if learned else None,
                 learned_fallbacks=(info1["fallbacks"] - info0["fallbacks"]) if learned else None,
                 spmv_avg_us=roof["avg_us"] if roof else None, spmv_frac=roof["frac"] if roof else None, roofline=roof,
-                _x=x, _energy=stats[-1]["energy"])
+                _x=x, _energy=stats[-1]["energy"], _start=start_state)
 
 
 def batch_run(args):
@@ -178,41 +214,89 @@ def batch_run(args):
     }
 
 
-def cpu_baseline(args, sd):
+def physical_cores():
+    """(physical cores this process may run on, CPU model) from `lscpu -p`, capped by
+    OMP_NUM_THREADS when that is set (the share of the host this job is given)"""
+    import subprocess
+    allowed = os.sched_getaffinity(0)
+    cores, model = set(), "?"
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        for line in out.splitlines():
+            if line.startswith("#"):
+                continue
+            cpu, core, sock = line.split(",")[:3]
+            if int(cpu) in allowed:
+                cores.add((sock, core))
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        cores = set(range(len(allowed)))
+    n = len(cores) or len(allowed)
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return n, model
+
+
+def cpu_baseline(args, sd, state, newton_per_step):
+    """the FP64 oracle on the first args.cpu_newton Newton iterations of the first timed
+    step (same start state as the GPU forms); steps/s = 1 / (ms per Newton iteration x the
+    FP64 GPU form's Newton iterations per timed step)"""
+    threads, model = physical_cores()
+    if args.cpu_threads:
+        threads = args.cpu_threads
+    # the OpenMP runtime reads these when the oracle library is first loaded
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    orc = O.OracleScene(sd, solver=1, pcg_tol=args.pcg_tol, threads=threads)
+    x, v, kappa = state
+    # the reference factors each Newton matrix with CHOLMOD; the oracle's stand-in (RCM +
+    # envelope Cholesky) is used up to 50k DOFs, beyond which its fill makes one
+    # factorisation take minutes and block-Jacobi PCG (same tolerance) is timed instead
+    direct = 3 * sd.nV <= 50000
+    orc = O.OracleScene(sd, solver=0 if direct else 1, pcg_tol=args.pcg_tol, threads=threads)
+    orc.set_state(x, v)
+    orc.set_kappa(kappa)
+    orc.set_max_newton(args.cpu_newton)
     t0 = time.perf_counter()
-    newton = 0
-    for _ in range(args.cpu_steps):
-        rc, st = orc.step()
-        if rc <= 0:
-            raise RuntimeError("oracle step failed: %d" % rc)
-        newton += int(st["newton_iters"])
+    rc, st = orc.step()
     el = time.perf_counter() - t0
-    return dict(value=args.cpu_steps / el, unit="steps/s", cores=threads, kind="port",
-                ms_per_newton_iter=1e3 * el / max(newton, 1), newton_iters_per_step=newton / args.cpu_steps,
-                sample="first %d step(s) of the same scene on the FP64 C++ oracle (block-Jacobi PCG, rel-res %g), "
-                       "%d OpenMP threads; %d Newton iterations in %.1f s" % (args.cpu_steps, args.pcg_tol, threads, newton, el))
+    if rc <= 0:
+        raise RuntimeError("oracle step failed: %d" % rc)
+    newton = int(st["newton_iters"])
+    ms_newton = 1e3 * el / newton
+    return dict(value=1e3 / (ms_newton * newton_per_step), unit="steps/s", cores=threads, kind="port",
+                ms_per_newton_iter=ms_newton, newton_iters_per_step=newton_per_step, contacts=int(st["contacts"]),
+                pcg_iters_per_newton=st["pcg_iters"] / newton,
+                solver="sparse Cholesky (RCM + envelope), stands in for CHOLMOD" if direct else
+                       "block-Jacobi PCG to rel-res %g (the envelope Cholesky standing in for CHOLMOD is infeasible at %d DOFs)"
+                       % (args.pcg_tol, 3 * sd.nV),
+                sample="first %d Newton iteration(s) of timed step 1 (%d contacts) from the same state on the FP64 C++ oracle "
+                       "(restatement of the reference path), %d OpenMP threads (OMP_PROC_BIND=close) on %s; %.1f s; steps/s "
+                       "extrapolated with the FP64 GPU form's %.2f Newton iterations per timed step"
+                       % (newton, int(st["contacts"]), threads, model, el, newton_per_step))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # default window: steps 27-30 of the 50-step run, the sheets hitting the sphere and each
-    # other (the first 26 steps are the free fall, run untimed)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=26)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fast-forward", type=int, default=26,
+                    help="untimed FP64 steps simulated once before every form's warm-up (the free fall)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"],
-                    help="c2 (default): 320k-triangle stack; c1: 20k-triangle stack; c3: the c1 stack with "
-                         "200,000 rod segments and 100,000 particles; c5: batch of independent scenes")
+                    help="c2 (default): 320k-triangle stack; c1: the 41x41 sheet (8,322 triangles with the obstacles); "
+                         "c3: the c1 scene with 200,000 rod segments and 100,000 particles; c5: batch of independent scenes")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes instead of tuning them during the warm-up")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--batch-form", default="fp32_learned", choices=["fp64", "fp32", "fp32_learned"])
-    ap.add_argument("--precision", default="all", choices=["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned", "all"])
+    ap.add_argument("--precision", default="headline", choices=["headline", "all", "fp64", "mixed", "fp32", "mixed_learned", "fp32_learned"],
+                    help="headline: fp64 and fp32_learned; all: the five forms")
     ap.add_argument("--learned-budget", type=int, default=2500)
     ap.add_argument("--learned-reuse", type=int, default=None,
                     help="Newton solves per network evaluation (default 1: every solve)")
@@ -222,7 +306,7 @@ def main():
     ap.add_argument("--strain-limit", type=float, default=0.0,
                     help="isotropic strain limit s (kappa_s = 1e3, sHat = 1); 0: off")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-newton", type=int, default=2, help="Newton iterations in the CPU baseline's sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -239,28 +323,34 @@ def main():
     if args.strain_limit > 0:
         sd.params.update(kappa_s=(1e3, 0.0), s=(args.strain_limit, 0.0), sHat=(1.0, 1.0))
 
-    modes = ["fp64", "mixed", "fp32"] if args.precision == "all" else [args.precision]
-    if args.precision == "all":
-        from cipc import precond
-        if os.path.exists(precond.WEIGHTS):
-            modes += ["mixed_learned", "fp32_learned"]
-    runs = {m: gpu_run(args, sd, m) for m in modes}
-    # each reduced-precision form against the FP64 form after the same steps from the same
-    # state: max position difference relative to the FP64 displacement, and the last
-    # step's incremental-potential difference relative to its magnitude
+    from cipc import precond
+    have_w = os.path.exists(precond.WEIGHTS)
+    modes = {"headline": ["fp64", "fp32_learned"], "all": ["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned"]}.get(
+        args.precision, [args.precision])
+    if not have_w:
+        modes = [m for m in modes if not m.endswith("_learned")]
+    x0, v0, kappa0, ff = fast_forward(args, sd)
+    runs = {m: gpu_run(args, sd, m, (x0, v0, kappa0)) for m in modes}
+    # each form against the FP64 form after the same steps from the same state: max position
+    # difference relative to the FP64 displacement over the timed steps, and the last step's
+    # incremental-potential difference relative to its magnitude
     if "fp64" in runs:
         import numpy as np
         x64, e64 = runs["fp64"]["_x"], runs["fp64"]["_energy"]
-        disp = float(np.abs(x64 - sd.X.ravel()).max())
+        disp = float(np.abs(x64 - runs["fp64"]["_start"][0]).max())
         for m, r in runs.items():
             r["vs_fp64_max_dx_rel"] = float(np.abs(r["_x"] - x64).max()) / max(disp, 1e-300)
             r["vs_fp64_energy_rel"] = abs(r["_energy"] - e64) / max(abs(e64), 1e-300)
+    head_form = "fp32_learned" if "fp32_learned" in runs else ([m for m in modes if m != "fp64"] or modes)[-1]
+    head = runs[head_form]
+    cpu = None
+    if not args.no_cpu_baseline:
+        ref = runs.get("fp64", head)
+        cpu = cpu_baseline(args, sd, ref["_start"], ref["newton_iters_per_step"])
     for r in runs.values():
-        r.pop("_x")
-        r.pop("_energy")
-    reduced = [m for m in modes if m != "fp64"] or modes
-    head = runs[max(reduced, key=lambda m: runs[m]["steps_per_s"])]
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args, sd)
+        for k in ("_x", "_energy", "_start"):
+            r.pop(k)
+    window = [args.fast_forward + args.warmup + 1, args.fast_forward + args.warmup + args.steps]
     out = {
         "metric": METRIC,
         "value": round(head["steps_per_s"], 4),
@@ -272,16 +362,20 @@ def main():
         "ms_per_newton_iter": round(head["ms_per_newton_iter"], 3),
         "newton_iters_per_step": round(head["newton_iters_per_step"], 3),
         "pcg_iters_per_newton": round(head["pcg_iters_per_newton"], 1),
+        "contacts_per_timed_step": head["contacts_per_timed_step"],
+        "newton_iters_per_timed_step": head["newton_iters_per_timed_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "mixed": "f32 blocks + f64", "fp32": "f32 + f64 refinement",
-                  "mixed_learned": "f32 blocks + f64, bf16 network", "fp32_learned": "f32 + f64 refinement, bf16 network"}[head["precision"]],
+                  "mixed_learned": "f32 blocks + f64, bf16 network", "fp32_learned": "f32 + f64 refinement, bf16 network"}[head_form],
         "data": "synthetic",
-        "mode": MODE[head["precision"]],
+        "mode": MODE[head_form],
         "config": {"workload": sd.name, "vertices": sd.nV, "triangles": sd.nT, "cloth_vertices": sd.n_cloth,
                    "rod_segments": 0 if sd.rods is None else len(sd.rods), "particles": 0 if sd.particles is None else len(sd.particles),
-                   "steps_simulated_per_form": args.warmup + args.steps, "dt": sd.params["dt"], "dhat": sd.params["dHat"],
+                   "untimed_free_fall_steps": args.fast_forward, "timed_window_steps": window,
+                   "free_fall_contacts_last": int(ff[-1]["contacts"]) if ff else None,
+                   "dt": sd.params["dt"], "dhat": sd.params["dHat"],
                    "pcg_tol": args.pcg_tol, "friction_mu": args.mu, "strain_limit": args.strain_limit},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
