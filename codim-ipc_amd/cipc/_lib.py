@@ -67,6 +67,11 @@ class SceneDesc(ctypes.Structure):
         ("strain_limit_shat", ctypes.c_double * 2),
         ("fiber_stiff_mult", ctypes.c_double * 4),
         ("fiber_limit", ctypes.c_double * 3),
+        ("n_tets", ctypes.c_int32),
+        ("tets", I32),
+        ("tet_density", ctypes.c_double),
+        ("tet_youngs", ctypes.c_double),
+        ("tet_poisson", ctypes.c_double),
     ]
 
 
@@ -88,6 +93,8 @@ class StepStats(ctypes.Structure):
         ("t_solve_ms", ctypes.c_double),
         ("t_energy_ms", ctypes.c_double),
         ("t_gradient_ms", ctypes.c_double),
+        ("kappa_s", ctypes.c_double),
+        ("kappa_s_doublings", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -139,7 +146,13 @@ PROTOTYPES = {
     "cipc_ccd_stepsize": (ctypes.c_int, [P, D, D, ctypes.c_double, D, D]),
     "cipc_set_friction": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32]),
     "cipc_set_friction_components": (ctypes.c_int, [P, ctypes.c_int32, I32, D]),
+    "cipc_get_tet_sizes": (ctypes.c_int, [P, I32]),
+    "cipc_set_graphs": (ctypes.c_int, [P, ctypes.c_int32]),
+    "cipc_set_static_solve": (ctypes.c_int, [P, ctypes.c_int32]),
+    "cipc_set_output_folder": (ctypes.c_int, [P, ctypes.c_char_p]),
+    "cipc_inversion_free_stepsize": (ctypes.c_int, [P, D, D, ctypes.c_double, D]),
     "cipc_set_strain_limit": (ctypes.c_int, [P, D, D, D]),
+    "cipc_get_strain_limit": (ctypes.c_int, [P, D, D, D]),
     "cipc_set_fiber": (ctypes.c_int, [P, D, D]),
     "cipc_dirichlet_add_motion": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, D, D, D, ctypes.c_double]),
     "cipc_step_dirichlet": (ctypes.c_int, [P]),

@@ -1,15 +1,18 @@
 """Training of the learned preconditioner on the Newton systems the simulator produces.
 
-Training data: cloth scenes of the two benchmark families -- the 41x41 single sheet over
-the icosphere and ground (2.5 cm spacing) and two-sheet 61x61 crops at the 320k stack's
-5 mm spacing -- with seeded drop heights and Young's moduli drawn log-uniform in
-[1e5, 1e7] Pa.  Each scene is stepped by the GPU simulator; after every step the state,
-its active constraint set and kappa define one Newton system (the Hessian the step's last
-Newton iteration solved).  The network is trained on all systems in rotation with Adam
-on the Kaporin loss (cipc_precond_train_step); weights and optimiser state move between
-scenes through get/set state.  The result is saved as weights/precond.npz.
+Training data (--spec c4, the default): scenes of the two benchmark families -- the 41x41
+single sheet over the icosphere and ground (2.5 cm spacing) and four-sheet 61x61 crops at
+the 320k stack's 5 mm spacing -- dropped from seeded heights uniform in [0.2, 0.8] m with
+Young's moduli log-uniform in [1e5, 1e7] Pa; a seeded fifth of the scenes also carries
+lagged friction (mu 0.4) and another fifth the 1.01 strain limit.  Each scene is stepped
+by the GPU simulator until it has lain in contact for `contact_steps` steps; every step's
+end state, its active constraint set and kappa define one Newton system, until `systems`
+systems are collected.  (--spec small: the earlier set of nine near-contact scenes.)
+The network is trained on all systems in rotation with Adam on the Kaporin loss
+(cipc_precond_train_step); weights and optimiser state move between scenes through
+get/set state.  The result is saved as weights/precond.npz.
 
-Usage:  python -m cipc.precond [--steps N] [--out PATH]
+Usage:  python -m cipc.precond [--spec c4|small] [--systems N] [--steps N] [--out PATH]
 """
 import argparse
 import math
@@ -56,6 +59,48 @@ def training_scenes(seed=0, n_coarse=6, n_fine=3, n_stack=0, stack_steps=24):
     return out
 
 
+def c4_scenes(seed=0, n=60, frac_c2=1 / 3, frac_friction=0.2, frac_strain=0.2):
+    """`n` scenes: 41x41 sheets and four-sheet 61x61 crops (5 mm spacing), drop height
+    uniform in [0.2, 0.8] m (at least 2 mm above the sphere top), E log-uniform in
+    [1e5, 1e7] Pa; seeded fractions with friction (mu 0.4) or the 1.01 strain limit"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        y0 = float(max(rng.uniform(0.2, 0.8), 0.202))
+        E = float(10 ** rng.uniform(5, 7))
+        if rng.uniform() < frac_c2:
+            sd = scenes.cloth_on_sphere(sheets=4, n=61, y0=y0, dy=0.005, seed=5000 + k, name="c4_s61_%d" % k)
+            sd = _crop(sd, rng)
+        else:
+            sd = scenes.cloth_on_sphere(sheets=1, n=41, y0=y0, seed=4000 + k, name="c4_c41_%d" % k)
+        sd.params["E"] = E
+        u = rng.uniform()
+        if u < frac_friction:
+            sd.params.update(mu=0.4, epsv2=1e-6, fricIterAmt=1)
+        elif u < frac_friction + frac_strain:
+            sd.params.update(kappa_s=(1e3, 0.0), s=(1.01, 0.0), sHat=(1.0, 1.0))
+        out.append(sd)
+    return out
+
+
+def collect_until_contact(sd, contact_steps, max_steps, log):
+    """step until the scene has been in contact for `contact_steps` steps (or max_steps):
+    one system per step"""
+    sim = Simulator(sd, precision="fp32")   # the systems are states; the solve form only sets the cost
+    systems = []
+    in_contact = 0
+    for s in range(max_steps):
+        st = sim.step()
+        x, _ = sim.get_state()
+        C = sim.constraint_set(x)
+        systems.append((x.copy(), C.copy(), sim.scalars()["kappa"]))
+        in_contact += 1 if len(C) else 0
+        if in_contact >= contact_steps:
+            break
+    log("  %s: %d systems (%d in contact), last step %d contacts" % (sd.name, len(systems), in_contact, st["contacts"]))
+    return sim, systems
+
+
 def collect(sd, steps, log):
     sim = Simulator(sd)
     systems = []
@@ -68,12 +113,22 @@ def collect(sd, steps, log):
     return sim, systems
 
 
-def train(steps=1500, lr=2e-3, lr_final=2e-4, n_probes=4, seed=0, sim_steps=12, log=print, n_stack=0, init=None):
+def train(steps=1500, lr=2e-3, lr_final=2e-4, n_probes=4, seed=0, sim_steps=12, log=print, n_stack=0, init=None, spec="c4",
+          systems=5000, contact_steps=30):
     t0 = time.time()
     data = []
-    for sd in training_scenes(seed, n_stack=n_stack):
-        sim, systems = collect(sd, getattr(sd, "train_steps", sim_steps), log)
-        data.append((sim, systems))
+    if spec == "c4":
+        total = 0
+        for sd in c4_scenes(seed):
+            sim, sys_ = collect_until_contact(sd, contact_steps, 90, log)
+            data.append((sim, sys_))
+            total += len(sys_)
+            if total >= systems:
+                break
+    else:
+        for sd in training_scenes(seed, n_stack=n_stack):
+            sim, sys_ = collect(sd, getattr(sd, "train_steps", sim_steps), log)
+            data.append((sim, sys_))
     log("collected %d systems from %d scenes in %.1f s" % (sum(len(s) for _, s in data), len(data), time.time() - t0))
     data[0][0].precond_init(seed + 1)
     if init is not None:
@@ -120,12 +175,17 @@ def main():
     ap.add_argument("--sim-steps", type=int, default=12)
     ap.add_argument("--stack", type=int, default=0, help="four-sheet crops stepped into deep contact")
     ap.add_argument("--init", default=None, help="start from these weights (npz)")
+    ap.add_argument("--spec", default="c4", choices=["c4", "small"])
+    ap.add_argument("--systems", type=int, default=5000, help="Newton systems to collect (spec c4)")
+    ap.add_argument("--contact-steps", type=int, default=30, help="steps in contact per scene (spec c4)")
     ap.add_argument("--out", default=WEIGHTS)
     args = ap.parse_args()
-    w, losses, _ = train(args.steps, args.lr, n_probes=args.probes, seed=args.seed, sim_steps=args.sim_steps,
-                         log=lambda m: print(m, flush=True), n_stack=args.stack, init=load(args.init) if args.init else None)
+    w, losses, data = train(args.steps, args.lr, n_probes=args.probes, seed=args.seed, sim_steps=args.sim_steps,
+                            log=lambda m: print(m, flush=True), n_stack=args.stack, init=load(args.init) if args.init else None,
+                            spec=args.spec, systems=args.systems, contact_steps=args.contact_steps)
     save(args.out, w, losses, dict(steps=args.steps, lr=args.lr, probes=args.probes, seed=args.seed, sim_steps=args.sim_steps,
-                                   stack=args.stack, init=args.init))
+                                   stack=args.stack, init=args.init, spec=args.spec, systems=sum(len(s) for _, s in data),
+                                   scenes=len(data)))
     print("saved", args.out, flush=True)
 
 

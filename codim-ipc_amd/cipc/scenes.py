@@ -102,6 +102,7 @@ class SceneData:
         self.name = name
         self.rods = None        # (n, 2) int32 rod segments or None
         self.particles = None   # (n,) int32 particle vertex ids or None
+        self.tets = None        # (n, 4) int32 volumetric tets or None (params tetRho, tetE, tetNu)
 
     @property
     def nV(self):
@@ -345,3 +346,65 @@ def scene_c5(seed=5, count=256, tri_min=1e3, tri_max=1e5):
         sd = cloth_on_sphere(sheets=1, n=n, y0=y0, seed=10000 + k, sphere_level=3, sphere_radius=r, name="c5_%03d" % k)
         out.append(sd)
     return out
+
+
+# ---------------------------------------------------------------- volumetric tets
+def tet_box(lo, hi, n):
+    """A box [lo, hi] cut into n^3 cubes of six tets each (the Kuhn subdivision along the
+    main diagonal: conforming across cubes), every tet positively oriented.
+    Returns (X (m, 3), tets (6 n^3, 4))."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    g = np.arange(n + 1)
+    I, J, K = np.meshgrid(g, g, g, indexing="ij")
+    X = lo + (hi - lo) * np.stack([I, J, K], -1).reshape(-1, 3) / n
+    vid = lambda i, j, k: (i * (n + 1) + j) * (n + 1) + k
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    tets = []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for pm in perms:
+                    c = [i, j, k]
+                    t = [vid(*c)]
+                    for ax in pm:
+                        c[ax] += 1
+                        t.append(vid(*c))
+                    a, b, cc, d = (X[v] for v in t)
+                    if np.linalg.det(np.stack([b - a, cc - a, d - a], 1)) < 0:
+                        t[1], t[2] = t[2], t[1]
+                    tets.append(t)
+    return X, np.array(tets, dtype=np.int32)
+
+
+TET_PARAMS = dict(tetRho=1000.0, tetE=1e5, tetNu=0.4)
+
+
+def with_tets(sd, X, tets, name=None, tet_params=None):
+    """Append a tet mesh (add_object_3D + initialize_added_objects) to a scene."""
+    out = SceneData(np.concatenate([sd.X, X]), sd.tris, sd.dbc_idx, sd.dbc_target, sd.params, name or sd.name)
+    out.n_cloth = getattr(sd, "n_cloth", 0)
+    out.rods, out.particles = sd.rods, sd.particles
+    out.tets = np.asarray(tets, dtype=np.int32) + sd.nV
+    out.params.update(TET_PARAMS if tet_params is None else tet_params)
+    out.params["PNTol"] = 1e-2 * out.bbox_diag()
+    return out
+
+
+def scene_tet_cloth(n=12, cells=4, gap=8e-4, seed=21):
+    """A soft tet cube (0.2 m, 6 cells^3 tets, rho 1000, E 1e5, nu 0.4) resting on the ground
+    (gap above it) with an n x n sheet (0.4 m) gap above the cube's top: cloth-tet contact,
+    tet-ground contact and the sheet draping over the cube's edges."""
+    X, T = cloth_grid(n, 0.0)
+    X = 0.4 * X
+    X[:, 1] = 0.2 + 2 * gap
+    u = splitmix64_uniform(seed, X.size).reshape(X.shape)
+    X = X + (2.0 * u - 1.0) * np.array([1e-4, 0.2 * gap, 1e-4])
+    n_cloth = X.shape[0]
+    Xg, Tg = ground(1.0)
+    Xa = np.concatenate([X, Xg])
+    Ta = np.concatenate([T, Tg + n_cloth])
+    dbc = np.arange(n_cloth, Xa.shape[0], dtype=np.int32)
+    base = SceneData(Xa, Ta, dbc, Xa[dbc].copy(), dict(DEFAULT_PARAMS), "tet_cloth")
+    base.n_cloth = n_cloth
+    Xt, Tt = tet_box((-0.1, gap, -0.1), (0.1, 0.2 + gap, 0.1), cells)
+    return with_tets(base, Xt, Tt, name="tet_cloth")

@@ -79,6 +79,15 @@ class Simulator:
         desc.rod_thickness = P.get("rodThick", 1e-3)
         desc.particle_density = P.get("partRho", 0.0)
         desc.particle_thickness = P.get("partThick", 1e-3)
+        tets = getattr(sd, "tets", None)
+        if tets is not None and len(tets):
+            tets = np.ascontiguousarray(tets, dtype=np.int32)
+            self._keep.append(tets)
+            desc.n_tets = len(tets)
+            desc.tets = _i(tets)
+            desc.tet_density = P.get("tetRho", 0.0)
+            desc.tet_youngs = P.get("tetE", 0.0)
+            desc.tet_poisson = P.get("tetNu", 0.3)
         desc.friction_mu = P.get("mu", 0.0)
         desc.friction_epsv2 = P.get("epsv2", 1e-6)
         desc.friction_iter_amt = int(P.get("fricIterAmt", 1))
@@ -151,6 +160,18 @@ class Simulator:
                         factor_lanes=factor_lanes, cell_mult=cell_mult)
         self._check(self.lib.cipc_set_tuning(self.h, ctypes.byref(t)))
 
+    def tet_sizes(self):
+        """(tets, contact triangles: the tets' surface then the shell's)"""
+        s = np.zeros(2, dtype=np.int32)
+        self._check(self.lib.cipc_get_tet_sizes(self.h, _i(s)))
+        return int(s[0]), int(s[1])
+
+    def inversion_free_stepsize(self, x, p, alpha=1.0):
+        """Compute_Inversion_Free_StepSize (IPC.h:1785-1877)"""
+        out = np.zeros(1)
+        self._check(self.lib.cipc_inversion_free_stepsize(self.h, _d(self._x(x)), _d(self._x(p)), float(alpha), _d(out)))
+        return out[0]
+
     def codim_setup(self):
         """Rod segment data, rod hinges and the boundary-node list (DISCRETE_ROD.h:87-165,
         IMPLICIT_EULER.h:243-277)."""
@@ -194,7 +215,20 @@ class Simulator:
     def solver_info(self):
         out = np.zeros(4, np.int32)
         self._check(self.lib.cipc_get_solver_info(self.h, _i(out)))
-        return dict(refinements=int(out[0]))
+        return dict(refinements=int(out[0]), graph_captures=int(out[1]), graph_launches=int(out[2]))
+
+    def set_output_folder(self, folder):
+        """the step's outputFolder: residual.txt, counter.txt, stretch.txt appended per step
+        (cipc_set_output_folder; None: off)"""
+        self._check(self.lib.cipc_set_output_folder(self.h, None if folder is None else str(folder).encode()))
+
+    def set_static_solve(self, enable=True):
+        """the step's staticSolve argument (cipc_set_static_solve)"""
+        self._check(self.lib.cipc_set_static_solve(self.h, 1 if enable else 0))
+
+    def set_graphs(self, enable=True):
+        """replay captured HIP graphs of the PCG chunks (cipc_set_graphs; on by default)"""
+        self._check(self.lib.cipc_set_graphs(self.h, 1 if enable else 0))
 
     # ------------------------------------------------------------ learned preconditioner
     def precond_init(self, seed=1):
@@ -260,7 +294,8 @@ class Simulator:
     def precond_info(self):
         out = np.zeros(8, np.int32)
         self._check(self.lib.cipc_precond_info(self.h, _i(out)))
-        return dict(fallback=int(out[0]), solves=int(out[1]), fallbacks=int(out[2]), last_status=int(out[3]), last_iters=int(out[4]))
+        return dict(fallback=int(out[0]), solves=int(out[1]), fallbacks=int(out[2]), last_status=int(out[3]), last_iters=int(out[4]),
+                    continued=int(out[5]), last_jacobi_iters=int(out[6]))
 
     def set_profiling(self, enable=True):
         self._check(self.lib.cipc_set_profiling(self.h, 1 if enable else 0))
@@ -379,6 +414,13 @@ class Simulator:
         k, s_, sh = (np.ascontiguousarray(v, dtype=np.float64) for v in (kappa, s, sHat))
         self._check(self.lib.cipc_set_strain_limit(self.h, _d(k), _d(s_), _d(sh)))
 
+    def strain_limit(self):
+        """(kappa_s, s, sHat) as the steps have left them (kappa_s[0] doubles in place at the
+        limit, IMPLICIT_EULER.h:606-645)"""
+        k, s_, sh = np.zeros(2), np.zeros(2), np.zeros(2)
+        self._check(self.lib.cipc_get_strain_limit(self.h, _d(k), _d(s_), _d(sh)))
+        return k, s_, sh
+
     def strain_limit_energy(self, x):
         """Compute_Inextensibility_Energy (INEXT.h:120-178): (valid, energy)."""
         e = np.zeros(1)
@@ -395,6 +437,8 @@ class Simulator:
         """compNodeRange / muComp (Compute_Friction_Coef, FRICTION.h:114-171)."""
         r = np.ascontiguousarray(comp_node_range, dtype=np.int32)
         m = np.ascontiguousarray(mu_comp, dtype=np.float64).ravel()
+        if len(m) != len(r) * len(r):
+            raise ValueError("mu_comp needs %d coefficients (n_components^2), got %d" % (len(r) * len(r), len(m)))
         self._check(self.lib.cipc_set_friction_components(self.h, len(r), _i(r), _d(m)))
 
     def friction_basis(self, x, C, kappa):

@@ -1,6 +1,8 @@
 // C ABI (include/cipc.h).  Every entry point converts HIP failures and invalid
 // arguments into negative status codes and records a message for cipc_last_error().
 #include "ctx.hpp"
+#include <stdexcept>
+#include <string>
 #include <cstdio>
 #include <cmath>
 #include "learn.hpp"
@@ -60,7 +62,14 @@ static void upload_x(cipc_ctx* ctx, DevBuf<double>& b, const double* h, size_t n
     CIPC_HIP(hipMemcpyAsync(b.p, h, n * sizeof(double), hipMemcpyHostToDevice, ctx->c.s));
 }
 // constraints arrive as int4 rows; the device keeps them SoA [4][n]
+// Every entry of the reference encoding is a vertex v, a -v-1, a -multiplicity or -1
+// (IPC.h:144-661), so all lie in [-nV, nV); anything else would be read out of bounds on
+// the device and is rejected here (CIPC_E_INVALID through guarded()).
 static void upload_cs(cipc_ctx* ctx, const int32_t* cs, int32_t n) {
+    const int nV = ctx->c.nV;
+    for (size_t i = 0; i < 4 * (size_t)n; ++i)
+        if (cs[i] < -nV || cs[i] >= nV)
+            throw std::invalid_argument("constraint " + std::to_string(i / 4) + " has an entry outside [-n_vertices, n_vertices)");
     std::vector<int> soa(4 * (size_t)std::max(n, 1));
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 4; ++k) soa[(size_t)k * n + i] = cs[4 * (size_t)i + k];
@@ -86,6 +95,10 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
         if (d->rod_segments[i] < 0 || d->rod_segments[i] >= d->n_vertices) return CIPC_E_INVALID;
     for (int i = 0; i < d->n_rod_segments; ++i)
         if (d->rod_segments[2 * i] == d->rod_segments[2 * i + 1]) return CIPC_E_INVALID;
+    if (d->n_tets < 0 || (d->n_tets && !d->tets)) return CIPC_E_INVALID;
+    for (int i = 0; i < 4 * d->n_tets; ++i)
+        if (d->tets[i] < 0 || d->tets[i] >= d->n_vertices) return CIPC_E_INVALID;
+    if (d->n_tets && !(d->tet_density >= 0 && d->tet_youngs > 0 && d->tet_poisson > -1 && d->tet_poisson < 0.5)) return CIPC_E_INVALID;
     for (int i = 0; i < d->n_particles; ++i)
         if (d->particles[i] < 0 || d->particles[i] >= d->n_vertices) return CIPC_E_INVALID;
     if (d->n_vertices >= (1 << 21) - 1) return CIPC_E_INVALID;   // contact-key packing limit
@@ -115,6 +128,8 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
                 P.slKappa[i] = d->strain_limit_kappa[i]; P.slS[i] = d->strain_limit_s[i]; P.slSHat[i] = d->strain_limit_shat[i];
             }
         c.hRod.assign(d->rod_segments, d->rod_segments + 2 * (size_t)d->n_rod_segments);
+        c.hTet.assign(d->tets, d->tets + 4 * (size_t)d->n_tets);
+        P.tetRho = d->tet_density; P.tetE = d->tet_youngs; P.tetNu = d->tet_poisson;
         c.hPart.assign(d->particles, d->particles + (size_t)d->n_particles);
         CIPC_HIP(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
         CIPC_HIP(hipHostMalloc((void**)&c.hScal, 64 * sizeof(double), hipHostMallocDefault));
@@ -462,6 +477,36 @@ int cipc_set_strain_limit(cipc_ctx* ctx, const double* kappa, const double* s, c
     return CIPC_OK;
 }
 
+int cipc_get_tet_sizes(const cipc_ctx* ctx, int32_t* out) {
+    if (!ctx || !out) return CIPC_E_INVALID;
+    out[0] = ctx->c.nTet;
+    out[1] = ctx->c.nBT;
+    return CIPC_OK;
+}
+
+int cipc_inversion_free_stepsize(cipc_ctx* ctx, const double* x, const double* p, double alpha, double* out) {
+    if (!ctx || !x || !p || !out || !(alpha > 0)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        const size_t n = 3 * (size_t)c.nV;
+        upload_x(ctx, ctx->qx, x, n);
+        upload_x(ctx, ctx->qp, p, n);
+        *out = inversion_free_stepsize(c, ctx->qx.p, ctx->qp.p, alpha);
+        return CIPC_OK;
+    });
+}
+
+int cipc_get_strain_limit(const cipc_ctx* ctx, double* kappa, double* s, double* shat) {
+    if (!ctx) return CIPC_E_INVALID;
+    const Params& P = ctx->c.P;
+    for (int i = 0; i < 2; ++i) {
+        if (kappa) kappa[i] = P.slKappa[i];
+        if (s) s[i] = P.slS[i];
+        if (shat) shat[i] = P.slSHat[i];
+    }
+    return CIPC_OK;
+}
+
 int cipc_strain_limit_energy(cipc_ctx* ctx, const double* x, double* e, int32_t* valid) {
     if (!ctx || !x || !e || !valid) return CIPC_E_INVALID;
     return guarded(ctx, [&]() {
@@ -640,6 +685,27 @@ int cipc_set_precision(cipc_ctx* ctx, int32_t mode) {
 int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out) {
     if (!ctx || !out) return CIPC_E_INVALID;
     out[0] = ctx->c.lastRefinements;
+    out[1] = (int32_t)ctx->c.pcgGraphCaptures;
+    out[2] = (int32_t)ctx->c.pcgGraphLaunches;
+    return CIPC_OK;
+}
+
+int cipc_set_output_folder(cipc_ctx* ctx, const char* folder) {
+    if (!ctx) return CIPC_E_INVALID;
+    ctx->c.outputFolder = folder ? folder : "";
+    return CIPC_OK;
+}
+
+int cipc_set_static_solve(cipc_ctx* ctx, int32_t enable) {
+    if (!ctx) return CIPC_E_INVALID;
+    ctx->c.P.staticSolve = enable ? 1 : 0;
+    return CIPC_OK;
+}
+
+int cipc_set_graphs(cipc_ctx* ctx, int32_t enable) {
+    if (!ctx) return CIPC_E_INVALID;
+    ctx->c.useGraphs = enable != 0;
+    if (!enable) ctx->c.pcgGraph.reset();
     return CIPC_OK;
 }
 
@@ -781,6 +847,8 @@ int cipc_precond_info(const cipc_ctx* ctx, int32_t* out) {
     out[2] = (int32_t)ctx->c.precondFallbacks;
     out[3] = ctx->c.lastLearnedRc;
     out[4] = ctx->c.lastLearnedIters;
+    out[5] = ctx->c.lastFallbackContinued;
+    out[6] = ctx->c.lastJacobiIters;
     return CIPC_OK;
 }
 

@@ -659,10 +659,10 @@ double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, doub
     // point-triangle
     uint32_t* lgCountT = c.counters.p + 18;
     uint32_t* lgCountE = c.counters.p + 19;
-    build_vox_cells(c, c.tri.p, c.nT, 3, vlo, vhi, G, c.ccdTlo, c.ccdThi, c.ccdTbox, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT,
+    build_vox_cells(c, c.btri.p, c.nBT, 3, vlo, vhi, G, c.ccdTlo, c.ccdThi, c.ccdTbox, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT,
                     lgCountT);
-    if (c.nBN && c.nT) {
-        hipLaunchKernelGGL(k_ccd_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, p, c.bNode.p, c.nBN, c.tri.p, c.nT, c.dbc.p, vlo, vhi, c.nV,
+    if (c.nBN && c.nBT) {
+        hipLaunchKernelGGL(k_ccd_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, p, c.bNode.p, c.nBN, c.btri.p, c.nBT, c.dbc.p, vlo, vhi, c.nV,
                            (const int4*)c.ccdTbox.p, G, c.cellStart.p, c.cellEnd.p, c.u32c.p, c.lgListT.p, lgCountT, alpha, xi, gmin,
                            c.counters.p + 13);
         CIPC_LAUNCH_CHECK();

@@ -447,7 +447,7 @@ static void constraint_set_impl(Ctx& c, const double* x, DevBuf<int>& out, int& 
 
     uint32_t* lgCountT = c.counters.p + 16;
     uint32_t* lgCountE = c.counters.p + 17;
-    build_cells(c, x, c.tri.p, c.nT, 3, G, c.triLo, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT, lgCountT);
+    build_cells(c, x, c.btri.p, c.nBT, 3, G, c.triLo, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT, lgCountT);
     build_cells(c, x, c.bEdge.p, c.nBE, 2, G, c.edgeLo, c.k64b, c.u32d, c.ecellStart, c.ecellEnd, c.lgFlagE, c.lgListE, lgCountE);
     if (c.nPart)
         build_cells(c, x, c.partSlot.p, c.nPart, 1, G, c.pLo, c.pKeys, c.pVals, c.pcellStart, c.pcellEnd, c.lgFlagP, c.lgListP,
@@ -465,7 +465,7 @@ static void constraint_set_impl(Ctx& c, const double* x, DevBuf<int>& out, int& 
         o.dcnt = c.counters.p + 1;
         CIPC_HIP(hipMemsetAsync(c.counters.p, 0, 4 * sizeof(uint32_t), c.s));
         // point-triangle kept entries fill [0, cap), edge-edge kept entries [cap, 2 cap)
-        hipLaunchKernelGGL(k_query_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, c.bNode.p, c.nBN, c.tri.p, c.nT, c.dbc.p, G,
+        hipLaunchKernelGGL(k_query_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, c.bNode.p, c.nBN, c.btri.p, c.nBT, c.dbc.p, G,
                            c.cellStart.p, c.cellEnd.p, c.u32c.p, c.triLo.p, c.lgListT.p, lgCountT, dHat, dHat2, c.nV, o);
         CIPC_LAUNCH_CHECK();
         CSOut oEE = o;
@@ -509,7 +509,7 @@ static void constraint_set_impl(Ctx& c, const double* x, DevBuf<int>& out, int& 
         out.ensure((size_t)4 * std::max(nC, 1));
         if (nPT) {
             fill_iota(si, nPT, c.s);
-            radix_sort_pairs(o.key, si, nPT, bits_for((uint64_t)c.nBN * c.nT), c.pw, c.s);
+            radix_sort_pairs(o.key, si, nPT, bits_for((uint64_t)c.nBN * c.nBT), c.pw, c.s);
             hipLaunchKernelGGL(k_gather_c4, dim3(nblocks(nPT)), dim3(kBlock), 0, c.s, si, o.c4, (int)nPT, out.p, nC, 0);
             CIPC_LAUNCH_CHECK();
         }
@@ -562,9 +562,12 @@ struct LocalC {
 
 // decode a constraint (reference encoding) into its 4-vertex stencil; for mollified
 // entries the stencil is the two edges (edgeVInd of IPC.h:831, 845, 859)
-__device__ __forceinline__ void decode(const int* __restrict__ C, int nC, int i, int* cc, int& kind, int* vid, int& nv, double& mult) {
+// entry k of constraint i at C[k * sk + i * si]: the library's SoA layout (sk = nC, si =
+// 1) or int4 rows (sk = 1, si = 4)
+__device__ __forceinline__ void decode(const int* __restrict__ C, long sk, long si, int i, int* cc, int& kind, int* vid, int& nv,
+                                       double& mult) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) cc[k] = C[(long)k * nC + i];
+    for (int k = 0; k < 4; ++k) cc[k] = C[(long)k * sk + (long)i * si];
     mult = 1;
     if (cc[0] >= 0) {
         nv = 4;
@@ -591,18 +594,30 @@ __device__ __forceinline__ double con_dist2(const double* __restrict__ x, int ki
     }
 }
 
+// ROWS: C is int4 rows checked against [0, nV) after decoding (device-resident input the
+// host never saw; an entry out of range contributes nothing and sets bit 2 of *bad),
+// else the library's own SoA set
+template <bool ROWS>
 __global__ void k_barrier_energy(const double* __restrict__ x, const double* __restrict__ X0, const int* __restrict__ C, int nC,
-                                 double kappa, double xi2, double shat, double* __restrict__ partial, uint32_t* __restrict__ bad) {
+                                 double kappa, double xi2, double shat, double* __restrict__ partial, uint32_t* __restrict__ bad,
+                                 int nV) {
     __shared__ double lds[4];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0;
     if (i < nC) {
         int cc[4], vid[4], kind, nv;
         double mult;
-        decode(C, nC, i, cc, kind, vid, nv, mult);
-        double d2 = con_dist2(x, kind, vid);
+        decode(C, ROWS ? 1 : nC, ROWS ? 4 : 1, i, cc, kind, vid, nv, mult);
+        bool inRange = true;
+        if (ROWS) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) inRange = inRange && cc[k] >= -nV && cc[k] < nV && (k >= nv || (vid[k] >= 0 && vid[k] < nV));
+        }
+        double d2 = inRange ? con_dist2(x, kind, vid) : 1.0;
         double s = d2 - xi2;
-        if (s <= 0) {
+        if (!inRange) {
+            atomicOr(bad, 2u);
+        } else if (s <= 0) {
             atomicOr(bad, 1u);
         } else {
             double b = barrier_e(s, shat, kappa);
@@ -703,7 +718,7 @@ __global__ void k_barrier_grad_local(const double* __restrict__ x, const double*
     if (i >= nC) return;
     int cc[4], vid[4], kind, nv;
     double mult;
-    decode(C, nC, i, cc, kind, vid, nv, mult);
+    decode(C, nC, 1, i, cc, kind, vid, nv, mult);
     double d2 = con_dist2(x, kind, vid);
     double s = d2 - xi2;
     long ns = 4l * nC;
@@ -745,7 +760,7 @@ __global__ void __launch_bounds__(64) k_barrier_hess_build(const double* __restr
     if (i >= nC) return;
     int cc[4], vid[4], kind, nv;
     double mult;
-    decode(C, nC, i, cc, kind, vid, nv, mult);
+    decode(C, nC, 1, i, cc, kind, vid, nv, mult);
     double d2 = con_dist2(x, kind, vid);
     double s = d2 - xi2;
     double* H = Hc + 144l * i;
@@ -809,7 +824,7 @@ __global__ void k_con_dist2(const double* __restrict__ x, const int* __restrict_
     if (i < nC) {
         int cc[4], vid[4], kind, nv;
         double mult;
-        decode(C, nC, i, cc, kind, vid, nv, mult);
+        decode(C, nC, 1, i, cc, kind, vid, nv, mult);
         v = con_dist2(x, kind, vid);
         if (d2) d2[i] = v;
     }
@@ -831,7 +846,8 @@ double barrier_energy(Ctx& c, const double* x, const int* C, int nC, double kapp
     int nb = nblocks(nC);
     double* part = c.red.ensure(nb);
     CIPC_HIP(hipMemsetAsync(c.counters.p + 12, 0, sizeof(uint32_t), c.s));
-    hipLaunchKernelGGL(k_barrier_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, part, c.counters.p + 12);
+    hipLaunchKernelGGL(k_barrier_energy<false>, dim3(nb), dim3(kBlock), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, part, c.counters.p + 12,
+                       c.nV);
     CIPC_LAUNCH_CHECK();
     sum_partials(part, nb, c.scal.p + 1, c.s);
     CIPC_HIP(hipMemcpyAsync(c.hScal + 1, c.scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, c.s));
@@ -908,7 +924,7 @@ __global__ void k_hess_keys(const int* __restrict__ C, int nC, int nV, const uin
     int i = (int)(q / 16), a = (int)((q / 4) % 4), b = (int)(q % 4);
     int cc[4], vid[4], kind, nv;
     double mult;
-    decode(C, nC, i, cc, kind, vid, nv, mult);
+    decode(C, nC, 1, i, cc, kind, vid, nv, mult);
     const bool keep = a < nv && b < nv && !dbc[vid[a]] && !dbc[vid[b]];
     key[q] = keep ? (uint64_t)vid[a] * (uint64_t)nV + (uint64_t)vid[b] : (uint64_t)nV * (uint64_t)nV;
 }

@@ -25,6 +25,12 @@ struct Params {
     int precision = 0;            // 0: FP64 PCG; 1: FP32 blocks + FP64 vectors; 2: all-FP32 PCG (1, 2 inside FP64 refinement)
     // rods and particles
     double rodRho = 0, rodE = 0, rodBendMult = 1, rodThick = 1e-3;
+    // volumetric tets (density, Young's modulus, Poisson ratio)
+    double tetRho = 0, tetE = 0, tetNu = 0.3;
+    // the step's staticSolve argument: shell terms weighted 1 instead of h^2, the body
+    // force -x.(m g) instead of inertia, no velocity update (IMPLICIT_EULER.h:196-217,
+    // 646-666, 893-899; INC_POTENTIAL.h:118-130, 233-249, 383-386)
+    int staticSolve = 0;
     double partRho = 0, partThick = 1e-3;
     // lagged friction (Advance_One_Step_IE_Hinge's mu, epsv2, fricIterAmt)
     double fricMu = 0, epsv2 = 1e-6;
@@ -41,6 +47,22 @@ struct Params {
 constexpr uint32_t kLargeCells = 512;
 
 struct LearnedPrecond;
+
+// an instantiated graph of one 16-iteration PCG chunk and the launch arguments it was
+// captured with (replayed while they stay the same)
+struct PcgGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uintptr_t> key;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+    PcgGraph() = default;
+    PcgGraph(const PcgGraph&) = delete;
+    PcgGraph& operator=(const PcgGraph&) = delete;
+    ~PcgGraph() { reset(); }
+};
 
 struct GridParams {
     double lb[3];
@@ -84,6 +106,13 @@ struct Ctx {
     DevBuf<int> rodHinge;     // SoA [3][nRH]
     DevBuf<double> rodHW;     // bending weight h^2 E t^4 pi / (64 lsum)
     DevBuf<int> partSlot;     // particles as point primitives (vertex ids, [1][nPart])
+    // volumetric tets: SoA [4][nTet] vertices, Dm^-1 SoA [9][nTet] (row-major entries), rest
+    // volumes; contact triangles (the tets' surface first, then the shell) SoA [3][nBT]
+    int nTet = 0, nBT = 0;
+    double tetLam = 0, tetMu = 0;
+    std::vector<int> hTet, hBTri;
+    DevBuf<int> tetIdx, btri;
+    DevBuf<double> tetIB, tetVol;
     DevBuf<int> pLo;
     DevBuf<uint64_t> pKeys;
     DevBuf<uint32_t> pVals, pcellStart, pcellEnd;
@@ -154,6 +183,12 @@ struct Ctx {
     DevBuf<uint8_t> dbcFixed;          // per vertex: Dirichlet with zero prescribed motion
     double dbcStiff = 0;
 
+    // kappa_s rule: per triangle "largest singular value near the upper limit" at the
+    // current and the previous Newton iterate (ping-pong)
+    DevBuf<uint8_t> slNear0, slNear1;
+    int slNearCur = 0;
+    int kappaSDoublings = 0;           // in the last step
+
     // lagged friction stencils (FEM/FRICTION.h): SoA [4][nF] encoding, SoA [9][nF] data
     // (closest point 2, tangent basis columns 2 x 3, normal force)
     int nF = 0;
@@ -190,13 +225,20 @@ struct Ctx {
     // after a fallback the rest of the step's solves use block Jacobi; a learned attempt
     // is also cut off at 1.25x the iterations of the last block-Jacobi solve
     bool lpOffThisStep = false;
-    int lastJacobiIters = 0;
+    int lastJacobiIters = 0;          // iterations of the last from-zero block-Jacobi solve
+    bool solveWarm = false;           // inside the correction solve of a warm-started Newton solve
+    int lastFallbackContinued = 0;    // the last fallback continued from the learned iterate
     // Newton solves after a line search that cut the step to alpha < 1 start from
     // (1 - alpha) p_prev, the quadratic model's prediction of the next direction
     bool warmStart = true;
     double lastWarmRatio = 1.0;       // ||b - H x0|| / ||b|| of the last warm-started solve
     DevBuf<double> warmR, warmX;
     DevBuf<double> fallbackX, fallbackR;   // learned iterate and its residual when block Jacobi takes over
+
+    // PCG chunks after the first replay a captured graph (cipc_set_graphs)
+    bool useGraphs = true;
+    PcgGraph pcgGraph;
+    long pcgGraphCaptures = 0, pcgGraphLaunches = 0;
 
     // launch shapes and broad-phase cell size (set by the autotuner or the caller)
     int spmvGrid = 1024;               // workgroups of the PCG SpMV (multiple of 8)
@@ -224,6 +266,9 @@ struct Ctx {
 
     // statistics of the last step
     cipc_step_stats stats{};
+    // per-step text logs of the reference's step (residual.txt, counter.txt, stretch.txt)
+    // appended under this folder when it is set
+    std::string outputFolder;
 
     // optional event timing of the dominant PCG kernel (elastic SpMV)
     int profiling = 0;
@@ -252,6 +297,9 @@ double friction_energy(Ctx& c, const double* x, const double* xn);
 void friction_gradient(Ctx& c, const double* x, const double* xn, double* g);   // g +=
 void friction_hess_local(Ctx& c, const double* x, const double* xn, double* hl, long ns, long off);
 void friction_hess_keys(Ctx& c, uint64_t* key);
+// weight of the shell's membrane, bending, strain-limit and fibre terms: h^2, or 1 in a
+// static solve (staticSolve ? 1.0 : h in the reference; rods and tets keep h^2)
+inline double shell_h2(const Ctx& c) { return c.P.staticSolve ? 1.0 : c.P.dt * c.P.dt; }
 // rows/columns projected out of the Newton system: every Dirichlet node, or only the
 // fixed ones while the Dirichlet penalty is on
 inline const uint8_t* proj_mask(const Ctx& c) { return c.dbcStiff ? c.dbcFixed.p : c.dbc.p; }
@@ -269,11 +317,21 @@ bool strain_limit_on(const Ctx& c);
 double strain_limit_energy(Ctx& c, const double* x);
 void strain_limit_grad_add(Ctx& c, const double* x, double* gl);
 void strain_limit_hess_add(Ctx& c, const double* x, double* hl);
+// kappa_s rule (IMPLICIT_EULER.h:606-645): records which triangles' largest singular value
+// lies within 1e-4 (s - sHat) of the upper limit at x; true when one of them did at the
+// previous call as well (first: no previous iterate in this step)
+bool strain_limit_near_twice(Ctx& c, const double* x, bool first);
 // anisotropic (fibre) membrane (strain.hip): replaces the membrane's energy and local slots
 bool fiber_on(const Ctx& c);
 double fiber_energy(Ctx& c, const double* x);               // +inf when infeasible
 void fiber_grad_local(Ctx& c, const double* x, double* gl);
 void fiber_hess_local(Ctx& c, const double* x, double* hl);
+// volumetric tets (tet.hip): energy +inf when a tet is inverted
+double tet_energy(Ctx& c, const double* x);
+bool tets_valid(Ctx& c, const double* x);
+void tet_grad_local(Ctx& c, const double* x, double* gl, long off);
+void tet_hess_local(Ctx& c, const double* x, double* hl, long off);
+double inversion_free_stepsize(Ctx& c, const double* x, const double* p, double alpha);
 double min_dist2(Ctx& c, const double* x, const int* C, int nC, double* d2out);
 double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, double* clamped, int* zeroToi);
 double elastic_energy(Ctx& c, const double* x, double* memb, double* bend);

@@ -84,6 +84,17 @@ __global__ void k_inertia_energy(const double* __restrict__ x, const double* __r
     double s = block_sum_256(e, lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
+// static solve: the body force's potential -sum x . (m g) over nodes with mass
+// (INC_POTENTIAL.h:118-130; b = M g as in Augment_Mass_Matrix_And_Body_Force)
+__global__ void k_body_energy(const double* __restrict__ x, const double* __restrict__ m, int nV, double gx, double gy, double gz,
+                              double* __restrict__ partial) {
+    __shared__ double lds[4];
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (v < nV && m[v] > 0) e = -(x[3 * v] * (m[v] * gx) + x[3 * v + 1] * (m[v] * gy) + x[3 * v + 2] * (m[v] * gz));
+    double s = block_sum_256(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
 
 static double finish_sum(Ctx& c, const double* part, int nb, int slot) {
     sum_partials(part, nb, c.scal.p + slot, c.s);
@@ -93,7 +104,7 @@ static double finish_sum(Ctx& c, const double* part, int nb, int slot) {
 }
 
 double elastic_energy(Ctx& c, const double* x, double* memb, double* bend) {
-    const double h2 = c.P.dt * c.P.dt;
+    const double h2 = shell_h2(c);
     double Em = 0, Eb = 0;
     if (c.nT && !fiber_on(c)) {   // the fibre model replaces the membrane (INC_POTENTIAL.h:81-92)
         int nb = nblocks(c.nT);
@@ -124,6 +135,7 @@ double elastic_energy(Ctx& c, const double* x, double* memb, double* bend) {
         CIPC_LAUNCH_CHECK();
         Eb += finish_sum(c, part, nb, 4);
     }
+    if (c.nTet) Em += tet_energy(c, x);   // +inf when a tet is inverted
     if (memb) *memb = Em;
     if (bend) *bend = Eb;
     return Em + Eb;
@@ -131,7 +143,10 @@ double elastic_energy(Ctx& c, const double* x, double* memb, double* bend) {
 double inertia_energy(Ctx& c, const double* x) {
     int nb = nblocks(c.nV);
     double* part = c.red.ensure(nb);
-    hipLaunchKernelGGL(k_inertia_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.xt.p, c.mass.p, c.nV, part);
+    if (c.P.staticSolve)
+        hipLaunchKernelGGL(k_body_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.mass.p, c.nV, c.P.grav[0], c.P.grav[1], c.P.grav[2], part);
+    else
+        hipLaunchKernelGGL(k_inertia_energy, dim3(nb), dim3(kBlock), 0, c.s, x, c.xt.p, c.mass.p, c.nV, part);
     CIPC_LAUNCH_CHECK();
     return finish_sum(c, part, nb, 5);
 }
@@ -204,7 +219,7 @@ __global__ void k_rod_bend_grad_local(const double* __restrict__ x, const int* _
 // g[v] = sum of local slots + m (x - xt)
 __global__ void k_el_grad_gather(const double* __restrict__ gl, long ns, const int* __restrict__ off, const int* __restrict__ slot,
                                  const double* __restrict__ x, const double* __restrict__ xt, const double* __restrict__ m, int nV,
-                                 int withInertia, double* __restrict__ g) {
+                                 int withInertia, double gx, double gy, double gz, double* __restrict__ g) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nV) return;
     double a = 0, b = 0, d = 0;
@@ -212,7 +227,9 @@ __global__ void k_el_grad_gather(const double* __restrict__ gl, long ns, const i
         long q = slot[k];
         a += gl[3 * q]; b += gl[3 * q + 1]; d += gl[3 * q + 2];
     }
-    if (withInertia) {
+    if (withInertia == 2 && m[v] > 0) {   // static solve: - m g (INC_POTENTIAL.h:233-249)
+        a -= m[v] * gx; b -= m[v] * gy; d -= m[v] * gz;
+    } else if (withInertia == 1) {
         a += m[v] * (x[3 * v] - xt[3 * v]);
         b += m[v] * (x[3 * v + 1] - xt[3 * v + 1]);
         d += m[v] * (x[3 * v + 2] - xt[3 * v + 2]);
@@ -221,7 +238,7 @@ __global__ void k_el_grad_gather(const double* __restrict__ gl, long ns, const i
 }
 
 static void elastic_grad_impl(Ctx& c, const double* x, double* g, int withInertia) {
-    const double h2 = c.P.dt * c.P.dt;
+    const double h2 = shell_h2(c);
     long ns = c.nGSlots;
     double* gl = c.gLocal.ensure(3 * std::max<long>(ns, 1));
     if (c.nT && fiber_on(c)) {
@@ -249,8 +266,9 @@ static void elastic_grad_impl(Ctx& c, const double* x, double* g, int withInerti
                            3l * c.nT + 4l * c.nH + 2l * c.nRod);
         CIPC_LAUNCH_CHECK();
     }
+    tet_grad_local(c, x, gl, 3l * c.nT + 4l * c.nH + 2l * c.nRod + 3l * c.nRH);
     hipLaunchKernelGGL(k_el_grad_gather, dim3(nblocks(c.nV)), dim3(kBlock), 0, c.s, gl, ns, c.gplanOff.p, c.gplanSlot.p, x, c.xt.p,
-                       c.mass.p, c.nV, withInertia, g);
+                       c.mass.p, c.nV, withInertia && c.P.staticSolve ? 2 : withInertia, c.P.grav[0], c.P.grav[1], c.P.grav[2], g);
     CIPC_LAUNCH_CHECK();
 }
 void elastic_gradient(Ctx& c, const double* x, double* g) { elastic_grad_impl(c, x, g, 1); }
@@ -476,7 +494,7 @@ __global__ void k_el_assemble(const double* __restrict__ hl, long ns, const int*
         for (int e = 0; e < 9; ++e) acc[e] += hl[9 * q + e];
     }
     int r = rowOf[b], cl = col[b];
-    if (r == cl) { acc[0] += m[r]; acc[4] += m[r]; acc[8] += m[r]; }
+    if (m && r == cl) { acc[0] += m[r]; acc[4] += m[r]; acc[8] += m[r]; }   // no mass in a static solve
     if (dbc[r] || dbc[cl]) {
 #pragma unroll
         for (int e = 0; e < 9; ++e) acc[e] = (r == cl && (e == 0 || e == 4 || e == 8)) ? 1.0 : 0.0;
@@ -486,7 +504,7 @@ __global__ void k_el_assemble(const double* __restrict__ hl, long ns, const int*
 }
 
 void elastic_hessian(Ctx& c, const double* x) {
-    const double h2 = c.P.dt * c.P.dt;
+    const double h2 = shell_h2(c);
     long ns = c.nElSlots;
     double* hl = c.elLocal.ensure(9 * std::max<long>(ns, 1));
     if (c.nT && fiber_on(c)) {
@@ -515,9 +533,10 @@ void elastic_hessian(Ctx& c, const double* x) {
                            9l * c.nT + 16l * c.nH + 4l * c.nRod);
         CIPC_LAUNCH_CHECK();
     }
+    tet_hess_local(c, x, hl, 9l * c.nT + 16l * c.nH + 4l * c.nRod + 9l * c.nRH);
     BSR& A = c.Ael;
     hipLaunchKernelGGL(k_el_assemble, dim3(nblocks(A.nnzb)), dim3(kBlock), 0, c.s, hl, ns, c.planOff.p, c.planSlot.p, A.rowOf.p,
-                       A.col.p, A.nnzb, c.mass.p, proj_mask(c), A.val.p);
+                       A.col.p, A.nnzb, c.P.staticSolve ? nullptr : c.mass.p, proj_mask(c), A.val.p);
     CIPC_LAUNCH_CHECK();
 }
 
@@ -526,8 +545,8 @@ void build_elastic_plans(Ctx& c) {
     const int nT = c.nT, nH = c.nH, nV = c.nV;
     // gradient plan: slot -> vertex
     {
-        const int nR = c.nRod, nRH = c.nRH;
-        long ns = 3l * nT + 4l * nH + 2l * nR + 3l * nRH;
+        const int nR = c.nRod, nRH = c.nRH, nTet = c.nTet;
+        long ns = 3l * nT + 4l * nH + 2l * nR + 3l * nRH + 4l * nTet;
         c.nGSlots = (int)ns;
         std::vector<std::pair<int, int>> vs;
         vs.reserve(ns);
@@ -540,6 +559,9 @@ void build_elastic_plans(Ctx& c) {
             for (int a = 0; a < 2; ++a) vs.push_back({c.hRod[2 * r + a], (int)(oR + 2l * r + a)});
         for (int h = 0; h < nRH; ++h)
             for (int a = 0; a < 3; ++a) vs.push_back({c.hRodHinge[3 * h + a], (int)(oRH + 3l * h + a)});
+        const long oTet = oRH + 3l * nRH;
+        for (int t = 0; t < nTet; ++t)
+            for (int a = 0; a < 4; ++a) vs.push_back({c.hTet[4 * t + a], (int)(oTet + 4l * t + a)});
         std::sort(vs.begin(), vs.end());
         std::vector<int> off(nV + 1, 0), slot(ns);
         for (auto& p : vs) off[p.first + 1]++;
@@ -550,8 +572,8 @@ void build_elastic_plans(Ctx& c) {
     }
     // Hessian pattern and plan
     {
-        const int nR = c.nRod, nRH = c.nRH;
-        long ns = 9l * nT + 16l * nH + 4l * nR + 9l * nRH;
+        const int nR = c.nRod, nRH = c.nRH, nTet = c.nTet;
+        long ns = 9l * nT + 16l * nH + 4l * nR + 9l * nRH + 16l * nTet;
         c.nElSlots = (int)ns;
         std::vector<std::pair<uint64_t, int>> ks;
         ks.reserve(ns + nV);
@@ -572,6 +594,11 @@ void build_elastic_plans(Ctx& c) {
             for (int a = 0; a < 4; ++a)
                 for (int b = 0; b < 4; ++b)
                     ks.push_back({(uint64_t)c.hHinge[4 * h + a] * nV + c.hHinge[4 * h + b], (int)(9l * nT + 16l * h + 4 * a + b)});
+        const long oTet = oRH + 9l * nRH;
+        for (int t = 0; t < nTet; ++t)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    ks.push_back({(uint64_t)c.hTet[4 * t + a] * nV + c.hTet[4 * t + b], (int)(oTet + 16l * t + 4 * a + b)});
         for (int v = 0; v < nV; ++v) ks.push_back({(uint64_t)v * nV + v, -1});
         std::sort(ks.begin(), ks.end());
         std::vector<int> rowOf, col, off, slot;

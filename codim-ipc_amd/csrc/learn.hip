@@ -557,7 +557,9 @@ __global__ void k_lp_assemble_G(int nL, int nV, int nnzb, const int* __restrict_
 template <int LANES, int BS, class V>
 __global__ __launch_bounds__(BS) void k_lp_spmv_bf16(int nrow, int rowsPerXcd, const int* __restrict__ rp, const int* __restrict__ col,
                                                      const uint32_t* __restrict__ val, int nnz, const V* __restrict__ x,
-                                                     V* __restrict__ y, const V* __restrict__ r, double* __restrict__ part) {
+                                                     V* __restrict__ y, const V* __restrict__ r, double* __restrict__ part,
+                                                     const int* __restrict__ done) {
+    if (done && *done) return;   // the PCG has converged: later iterations of the chunk do nothing
     __shared__ double lds[BS / 64];
     constexpr int RPG = BS / LANES;
     const int xb = blockIdx.x % 8, jb = blockIdx.x / 8, nj = gridDim.x / 8;
@@ -1130,22 +1132,30 @@ constexpr int kLpSpmvBlock = 256;
 int lp_apply_partials(const Ctx& c) { return c.lpGrid; }
 
 template <int LANES, class V>
-static void lp_spmv_go(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part) {
+static void lp_spmv_go(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part,
+                       const int* done) {
     constexpr int RPG = kLpSpmvBlock / LANES;
     const int nV = c.lp->nV;
     int rpx = (nV + 7) / 8;
     rpx = (rpx + RPG - 1) / RPG * RPG;
     hipLaunchKernelGGL((k_lp_spmv_bf16<LANES, kLpSpmvBlock, V>), dim3(c.lpGrid), dim3(kLpSpmvBlock), 0, c.s, nV, rpx, rp, col, val, nnz,
-                       x, y, r, part);
+                       x, y, r, part, done);
 }
 template <class V>
-static void lp_spmv(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part) {
+static void lp_spmv(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part,
+                    const int* done = nullptr) {
     switch (c.lpLanes) {
-        case 2: lp_spmv_go<2, V>(c, rp, col, val, nnz, x, y, r, part); break;
-        case 4: lp_spmv_go<4, V>(c, rp, col, val, nnz, x, y, r, part); break;
-        case 16: lp_spmv_go<16, V>(c, rp, col, val, nnz, x, y, r, part); break;
-        default: lp_spmv_go<8, V>(c, rp, col, val, nnz, x, y, r, part); break;
+        case 2: lp_spmv_go<2, V>(c, rp, col, val, nnz, x, y, r, part, done); break;
+        case 4: lp_spmv_go<4, V>(c, rp, col, val, nnz, x, y, r, part, done); break;
+        case 16: lp_spmv_go<16, V>(c, rp, col, val, nnz, x, y, r, part, done); break;
+        default: lp_spmv_go<8, V>(c, rp, col, val, nnz, x, y, r, part, done); break;
     }
+}
+
+std::vector<uintptr_t> lp_graph_key(const Ctx& c) {
+    const LearnedPrecond& lp = *c.lp;
+    return {(uintptr_t)c.lpGrid, (uintptr_t)c.lpLanes, (uintptr_t)lp.nV, (uintptr_t)lp.nLow, (uintptr_t)lp.gLrp.p, (uintptr_t)lp.gLcol.p,
+            (uintptr_t)lp.FL.p, (uintptr_t)lp.gUrp.p, (uintptr_t)lp.gUcol.p, (uintptr_t)lp.FU.p};
 }
 
 template <class V>
@@ -1161,13 +1171,13 @@ template int lp_apply<double>(Ctx&, const double*, double*, double*, double*);
 template int lp_apply<float>(Ctx&, const float*, float*, float*, double*);
 
 template <class V>
-void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part) {
+void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part, const int* done) {
     LearnedPrecond& lp = LP(c);
-    lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, y, z, r, part);
+    lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, y, z, r, part, done);
     CIPC_LAUNCH_CHECK();
 }
-template void lp_apply_upper<double>(Ctx&, const double*, double*, const double*, double*);
-template void lp_apply_upper<float>(Ctx&, const float*, float*, const float*, double*);
+template void lp_apply_upper<double>(Ctx&, const double*, double*, const double*, double*, const int*);
+template void lp_apply_upper<float>(Ctx&, const float*, float*, const float*, double*, const int*);
 
 double lp_loss_grad(Ctx& c, int nProbes, uint64_t seed, float* gradOut) {
     LearnedPrecond& lp = LP(c);

@@ -92,7 +92,9 @@ int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part);
 int lp_apply_partials(const Ctx& c);
 // z = F^T y (the second product of lp_apply) with partial sums of r.z
 template <class V>
-void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part);
+void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part, const int* done = nullptr);
+// the factor products' launch arguments (part of a captured PCG chunk's key)
+std::vector<uintptr_t> lp_graph_key(const Ctx& c);
 // unpack the 9 BF16 entries of a block stored as five words (SoA [5][n]) at position b
 template <class V>
 __device__ __forceinline__ void lp_unpack9(const uint32_t* __restrict__ val, long n, long b, V* a) {

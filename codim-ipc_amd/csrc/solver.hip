@@ -187,7 +187,7 @@ __device__ __forceinline__ void bsr_row_ek(int lane, int eb, int ee, int cb, int
 // first; then p_new = z + beta p_old (p_new = z at it = 0, WITHP = false), q = A p_new,
 // partial p.q.  Row bounds of the next row group are loaded one group ahead.
 template <class M, class V, int LANES, int BS, bool WITHP, int UNR = 1>
-__global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2, const int* __restrict__ erp,
+__global__ __launch_bounds__(BS) void k_pcg_spmv(const int* __restrict__ itDev, int itArg, int maxIt, double tol2, const int* __restrict__ erp,
                                                  const int* __restrict__ ecol, const M* __restrict__ ev, int enz,
                                                  const int* __restrict__ crp, const int* __restrict__ ccol, const M* __restrict__ cv,
                                                  int cnz, int nrow, int rowsPerXcd, const V* __restrict__ z,
@@ -207,6 +207,7 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(int it, int maxIt, double tol2,
     int beg = live ? erp[row] : 0, end = live ? erp[row + 1] : 0;
     int cbeg = live && cnz > 0 ? crp[row] : 0, cend = live && cnz > 0 ? crp[row + 1] : 0;
     if (flags[F_DONE]) return;
+    const int it = itDev ? *itDev + itArg : itArg;   // a captured chunk reads its base iteration
 
     double arz = 0, arr = 0;
 #pragma unroll 4
@@ -284,11 +285,12 @@ static auto spmv_kernel(int lanes, int unr, bool withP) -> decltype(&k_pcg_spmv<
 // Iteration `it`: alpha = r.z / p.q, x += alpha p, r -= alpha q, z = Dinv r, partials
 // of r.z and r.r (written as partU[block], partU[nU + block]).
 template <class V, int BS, bool JAC>
-__global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPerXcd, V* __restrict__ x, V* __restrict__ r,
-                                                   V* __restrict__ z, const V* __restrict__ p, const V* __restrict__ q,
+__global__ __launch_bounds__(BS) void k_pcg_update(const int* __restrict__ itDev, int itArg, int nrow, int rowsPerXcd, V* __restrict__ x,
+                                                   V* __restrict__ r, V* __restrict__ z, const V* __restrict__ p, const V* __restrict__ q,
                                                    const V* __restrict__ dinv, const double* __restrict__ partS, int nS,
                                                    double* __restrict__ partU, double* __restrict__ sc, int* __restrict__ flags) {
     if (flags[F_DONE]) return;
+    const int it = itDev ? *itDev + itArg : itArg;
     __shared__ double lds[BS / 64];
     double a = 0;
     for (int i = threadIdx.x; i < nS; i += BS) a += partS[i];
@@ -347,13 +349,15 @@ __global__ __launch_bounds__(BS) void k_pcg_update(int it, int nrow, int rowsPer
 // factor F, partial sums of r'.r'.  Every row of F stores its diagonal block; the lane
 // that meets it updates the row's r and x.  Fixed grid, rows mapped XCD-aware.
 template <int LANES, int BS, class V>
-__global__ __launch_bounds__(BS) void k_pcg_update_lower(int it, int nrow, int rowsPerXcd, const int* __restrict__ rp,
+__global__ __launch_bounds__(BS) void k_pcg_update_lower(const int* __restrict__ itDev, int itArg, int nrow, int rowsPerXcd,
+                                                         const int* __restrict__ rp,
                                                          const int* __restrict__ col, const uint32_t* __restrict__ val, int nnz,
                                                          V* __restrict__ x, const V* __restrict__ p, const V* __restrict__ q,
                                                          const V* __restrict__ rOld, V* __restrict__ rNew, V* __restrict__ y,
                                                          const double* __restrict__ partS, int nS, double* __restrict__ partRR,
                                                          double* __restrict__ sc, int* __restrict__ flags) {
     if (flags[F_DONE]) return;
+    const int it = itDev ? *itDev + itArg : itArg;
     __shared__ double lds[BS / 64];
     constexpr int RPG = BS / LANES;
     const int xb = blockIdx.x % kXcd, jb = blockIdx.x / kXcd, nj = gridDim.x / kXcd;
@@ -532,6 +536,15 @@ __global__ __launch_bounds__(kBlock) void k_residual(const int* __restrict__ erp
     if (threadIdx.x == 0) part[blockIdx.x] = br;
 }
 
+// device iteration counter of the captured chunks: set before the first replay, advanced
+// by the chunk's last node
+__global__ void k_it_set(int* itDev, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *itDev = v;
+}
+__global__ void k_it_advance(int* itDev, int by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *itDev += by;
+}
+
 // ---- one preconditioned CG solve in precision V; returns 0 converged, 1 not SPD
 // (p^T A p <= 0), 2 iteration limit
 template <class M, class V>
@@ -584,7 +597,42 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     const double tol2 = tol * tol;
     const double mb = (double)sizeof(M), vb = (double)sizeof(V);
     const double spmvBytes = (9 * mb + 4) * ((double)E.nnzb + K.nnzb) + 4.0 * (nV + 1) * (K.nnzb > 0 ? 2 : 1) + 4 * 3 * vb * nV;
+    int* itDev = (int*)(c.counters.p + 24);   // base iteration of a captured chunk
+    // one iteration's launches: iteration `it` (itDev == nullptr) or base + k (captured)
+    auto launch_iteration = [&](int it, int k, const int* itd, int nS, int sLanes, int sUnroll, int rpxS, bool timed) {
+        const V* pold = pbuf + (size_t)((it + 1) & 1) * n;
+        V* pnew = pbuf + (size_t)(it & 1) * n;
+        const int itArg = itd ? k : it;
+        auto kern = spmv_kernel<M, V>(sLanes, sUnroll, it > 0);
+        const double* partRR = learned && it > 0 ? partF : partU + nU;
+        const int nRR = learned && it > 0 ? nP : nU;
+        if (timed)
+            hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.evPool[2 * k], c.evPool[2 * k + 1], 0, itd, itArg, maxIter, tol2,
+                                  E.rowptr.p, E.col.p, valE, E.nnzb, K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew,
+                                  q, partRZ, nRZ, partRR, nRR, partS, sc, flags);
+        else
+            hipLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, itd, itArg, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
+                               K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, partRZ, nRZ, partRR, nRR, partS,
+                               sc, flags);
+        if (!learned) {
+            hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock, true>), dim3(nU), dim3(kUpdBlock), 0, c.s, itd, itArg, nV, rpxU, xsol, r, z,
+                               (const V*)pnew, (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
+        } else {
+            LearnedPrecond& lp = *c.lp;
+            V* rNew = rb[(it + 1) & 1];
+            auto fused = c.lpLanes == 2    ? k_pcg_update_lower<2, kSpmvBlock, V>
+                         : c.lpLanes == 4  ? k_pcg_update_lower<4, kSpmvBlock, V>
+                         : c.lpLanes == 16 ? k_pcg_update_lower<16, kSpmvBlock, V>
+                                           : k_pcg_update_lower<8, kSpmvBlock, V>;
+            const int rpxL = rows_per_xcd(nV, kSpmvBlock / c.lpLanes);
+            hipLaunchKernelGGL(fused, dim3(nP), dim3(kSpmvBlock), 0, c.s, itd, itArg, nV, rpxL, (const int*)lp.gLrp.p, (const int*)lp.gLcol.p,
+                               (const uint32_t*)lp.FL.p, lp.nLow, xsol, (const V*)pnew, (const V*)q, (const V*)rb[it & 1], rNew, ltmp,
+                               (const double*)partS, nS, partF, sc, flags);
+            lp_apply_upper<V>(c, (const V*)ltmp, z, (const V*)rNew, partP, flags);
+        }
+    };
     int it = 0;
+    bool itDevSet = false;
     for (;;) {
         int launched = 0;
         int cand = -1, nS = c.spmvGrid, sLanes = c.spmvLanes, sUnroll = c.spmvUnroll;
@@ -593,32 +641,58 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
             nS = T.spmvCand[cand][0]; sLanes = T.spmvCand[cand][1]; sUnroll = T.spmvCand[cand][2];
         }
         const int rpxS = rows_per_xcd(nV, kSpmvBlock / sLanes);
-        for (int k = 0; k < chunk && it <= maxIter; ++k, ++it, ++launched) {
-            const V* pold = pbuf + (size_t)((it + 1) & 1) * n;
-            V* pnew = pbuf + (size_t)(it & 1) * n;
-            auto kern = spmv_kernel<M, V>(sLanes, sUnroll, it > 0);
-            const bool timed = sampling && k % kProfEvery == kProfFirst;
-            hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, timed ? c.evPool[2 * k] : nullptr,
-                                  timed ? c.evPool[2 * k + 1] : nullptr, 0, it, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
-                                  K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, partRZ, nRZ,
-                                  (const double*)(learned && it > 0 ? partF : partU + nU), learned && it > 0 ? nP : nU, partS, sc,
-                                  flags);
-            if (!learned) {
-                hipLaunchKernelGGL((k_pcg_update<V, kUpdBlock, true>), dim3(nU), dim3(kUpdBlock), 0, c.s, it, nV, rpxU, xsol, r, z,
-                                   (const V*)pnew, (const V*)q, dinv, (const double*)partS, nS, partU, sc, flags);
-            } else {
+        // a whole chunk after the first, with no launch timed, replays one captured graph of
+        // its 16 iterations (launch arguments identical from chunk to chunk: the iteration
+        // number comes from the device counter, the buffer parities repeat every 16)
+        const bool asGraph = c.useGraphs && !sampling && it > 0 && it % chunk == 0 && it + chunk - 1 <= maxIter;
+        if (asGraph) {
+            std::vector<uintptr_t> key = {(uintptr_t)(void*)spmv_kernel<M, V>(sLanes, sUnroll, true), (uintptr_t)nS, (uintptr_t)rpxS,
+                                          (uintptr_t)maxIter, (uintptr_t)std::llround(std::log2(tol2) * 1e6),
+                                          (uintptr_t)E.rowptr.p, (uintptr_t)E.col.p, (uintptr_t)valE, (uintptr_t)E.nnzb,
+                                          (uintptr_t)K.rowptr.p, (uintptr_t)K.col.p, (uintptr_t)valK, (uintptr_t)K.nnzb, (uintptr_t)nV,
+                                          (uintptr_t)z, (uintptr_t)pbuf, (uintptr_t)q, (uintptr_t)partRZ, (uintptr_t)nRZ,
+                                          (uintptr_t)partF, (uintptr_t)partU, (uintptr_t)nU, (uintptr_t)nP, (uintptr_t)partS,
+                                          (uintptr_t)sc, (uintptr_t)flags, (uintptr_t)xsol, (uintptr_t)r, (uintptr_t)dinv,
+                                          (uintptr_t)learned, (uintptr_t)ltmp, (uintptr_t)rb[1], (uintptr_t)partP, (uintptr_t)sizeof(M),
+                                          (uintptr_t)sizeof(V)};
+            if (learned) {
                 LearnedPrecond& lp = *c.lp;
-                V* rNew = rb[(it + 1) & 1];
-                auto fused = c.lpLanes == 2    ? k_pcg_update_lower<2, kSpmvBlock, V>
-                             : c.lpLanes == 4  ? k_pcg_update_lower<4, kSpmvBlock, V>
-                             : c.lpLanes == 16 ? k_pcg_update_lower<16, kSpmvBlock, V>
-                                               : k_pcg_update_lower<8, kSpmvBlock, V>;
-                const int rpxL = rows_per_xcd(nV, kSpmvBlock / c.lpLanes);
-                hipLaunchKernelGGL(fused, dim3(nP), dim3(kSpmvBlock), 0, c.s, it, nV, rpxL, (const int*)lp.gLrp.p, (const int*)lp.gLcol.p,
-                                   (const uint32_t*)lp.FL.p, lp.nLow, xsol, (const V*)pnew, (const V*)q, (const V*)rb[it & 1], rNew, ltmp,
-                                   (const double*)partS, nS, partF, sc, flags);
-                lp_apply_upper<V>(c, (const V*)ltmp, z, (const V*)rNew, partP);
+                const std::vector<uintptr_t> lk = lp_graph_key(c);
+                key.insert(key.end(), lk.begin(), lk.end());
             }
+            if (!c.pcgGraph.exec || c.pcgGraph.key != key) {
+                c.pcgGraph.reset();
+                hipGraph_t g = nullptr;
+                CIPC_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+                try {
+                    for (int k = 0; k < chunk; ++k) launch_iteration(it + k, k, itDev, nS, sLanes, sUnroll, rpxS, false);
+                    hipLaunchKernelGGL(k_it_advance, dim3(1), dim3(64), 0, c.s, itDev, chunk);
+                } catch (...) {
+                    (void)hipStreamEndCapture(c.s, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                CIPC_HIP(hipStreamEndCapture(c.s, &g));
+                hipGraphExec_t ex = nullptr;
+                const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                CIPC_HIP(ie);
+                c.pcgGraph.exec = ex;
+                c.pcgGraph.key = key;
+                c.pcgGraphCaptures++;
+            }
+            if (!itDevSet) {
+                hipLaunchKernelGGL(k_it_set, dim3(1), dim3(64), 0, c.s, itDev, it);
+                itDevSet = true;
+            }
+            CIPC_HIP(hipGraphLaunch(c.pcgGraph.exec, c.s));
+            c.pcgGraphLaunches++;
+            it += chunk;
+            launched = chunk;
+        } else {
+            itDevSet = false;
+            for (int k = 0; k < chunk && it <= maxIter; ++k, ++it, ++launched)
+                launch_iteration(it, k, nullptr, nS, sLanes, sUnroll, rpxS, sampling && k % kProfEvery == kProfFirst);
         }
         CIPC_LAUNCH_CHECK();
         CIPC_HIP(hipMemcpyAsync(hf, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, c.s));
@@ -778,10 +852,11 @@ static double norm2(Ctx& c, const double* a, long n);
 static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres) {
     c.lastPrecondFallback = 0;
     const bool learned = c.precond == 1 && !c.lpOffThisStep;
+    c.lastFallbackContinued = 0;
     if (!learned) {
         int it = 0;
         const int rc = solve_form(c, b, xsol, tol, maxIter, &it, relres, false);
-        c.lastJacobiIters = it;
+        if (!c.solveWarm) c.lastJacobiIters = it;   // the learned budget's cap counts from-zero solves only
         if (iters) *iters = it;
         return rc;
     }
@@ -822,12 +897,13 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
             CIPC_LAUNCH_CHECK();
             if (relres) *relres = rr2 * std::sqrt(rn2 / bn2);
             // lastJacobiIters (the learned budget's cap) keeps counting from-zero solves only
+            c.lastFallbackContinued = 1;
             if (iters) *iters = it + it2;
             return rc;
         }
     }
     rc = solve_form(c, b, xsol, tol, maxIter, &it2, relres, false);
-    c.lastJacobiIters = it2;
+    if (!c.solveWarm) c.lastJacobiIters = it2;
     if (iters) *iters = it + it2;
     return rc;
 }
@@ -947,7 +1023,15 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     c.lastWarmRatio = bn2 > 0 ? std::sqrt(rn2 / bn2) : 1.0;
     if (!(rn2 < 0.25 * bn2)) return pcg_solve_tuned(c, b, xsol, tol, maxIter, iters, relres);
     double rr = 0;
-    const int rc = pcg_solve_tuned(c, r0, xsol, tol * std::sqrt(bn2 / rn2), maxIter, iters, &rr);
+    c.solveWarm = true;   // a correction solve: its iteration count is not a from-zero count
+    int rc;
+    try {
+        rc = pcg_solve_tuned(c, r0, xsol, tol * std::sqrt(bn2 / rn2), maxIter, iters, &rr);
+    } catch (...) {
+        c.solveWarm = false;
+        throw;
+    }
+    c.solveWarm = false;
     hipLaunchKernelGGL(k_acc_f64, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, x0, xsol);
     CIPC_LAUNCH_CHECK();
     if (relres) *relres = rr * std::sqrt(rn2 / bn2);

@@ -15,6 +15,10 @@
 #include "ctx.hpp"
 #include "dev_math.hpp"
 #include <algorithm>
+#include <cstdio>
+#include <array>
+#include <stdexcept>
+#include <string>
 #include <chrono>
 #include <cmath>
 #include <deque>
@@ -63,24 +67,49 @@ void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri
     }
     c.hTri = tri;
     c.nT = (int)tri.size() / 3;
-    // edge2tri + primitives, MESHIO.h:728-766
+    // contact triangles: the surface of the tets (Find_Surface_TriMesh, MESHIO.h:486-544:
+    // faces keyed as written in a std::map, kept when no rotation of the reverse exists)
+    // followed by the shell (IMPLICIT_EULER.h:235-241)
+    c.nTet = (int)c.hTet.size() / 4;
+    c.hBTri.clear();
+    {
+        std::map<std::array<int, 3>, int> faces;
+        for (int t = 0; t < c.nTet; ++t) {
+            const int* v = &c.hTet[4 * t];
+            faces[{v[0], v[2], v[1]}] = t; faces[{v[0], v[3], v[2]}] = t;
+            faces[{v[0], v[1], v[3]}] = t; faces[{v[1], v[2], v[3]}] = t;
+        }
+        for (const auto& f : faces) {
+            const auto& k = f.first;
+            if (faces.count({k[2], k[1], k[0]}) || faces.count({k[1], k[0], k[2]}) || faces.count({k[0], k[2], k[1]})) continue;
+            c.hBTri.insert(c.hBTri.end(), k.begin(), k.end());
+        }
+    }
+    c.hBTri.insert(c.hBTri.end(), tri.begin(), tri.end());
+    c.nBT = (int)c.hBTri.size() / 3;
+    // edge2tri + primitives, MESHIO.h:728-766 (hinges from the shell's edges, primitives
+    // from every contact triangle)
     std::map<std::pair<int, int>, int> edge2tri;
-    std::set<std::pair<int, int>> edgeSet;
+    std::set<std::pair<int, int>> edgeSet, primEdgeSet;
     std::vector<char> isBN(c.nV, 0);
     for (int t = 0; t < c.nT; ++t) {
         const int* v = &tri[3 * t];
         edge2tri[{v[0], v[1]}] = t; edge2tri[{v[1], v[2]}] = t; edge2tri[{v[2], v[0]}] = t;
     }
-    for (int t = 0; t < c.nT; ++t) {
-        const int* v = &tri[3 * t];
-        if (!edgeSet.count({v[1], v[0]})) edgeSet.insert({v[0], v[1]});
-        if (!edgeSet.count({v[2], v[1]})) edgeSet.insert({v[1], v[2]});
-        if (!edgeSet.count({v[0], v[2]})) edgeSet.insert({v[2], v[0]});
+    auto add_edges = [](std::set<std::pair<int, int>>& es, const int* v) {
+        if (!es.count({v[1], v[0]})) es.insert({v[0], v[1]});
+        if (!es.count({v[2], v[1]})) es.insert({v[1], v[2]});
+        if (!es.count({v[0], v[2]})) es.insert({v[2], v[0]});
+    };
+    for (int t = 0; t < c.nT; ++t) add_edges(edgeSet, &tri[3 * t]);
+    for (int t = 0; t < c.nBT; ++t) {
+        const int* v = &c.hBTri[3 * t];
+        add_edges(primEdgeSet, v);
         isBN[v[0]] = isBN[v[1]] = isBN[v[2]] = 1;
     }
     c.hBNode.clear(); c.hBEdge.clear();
     for (int i = 0; i < c.nV; ++i) if (isBN[i]) c.hBNode.push_back(i);
-    for (auto& e : edgeSet) { c.hBEdge.push_back(e.first); c.hBEdge.push_back(e.second); }
+    for (auto& e : primEdgeSet) { c.hBEdge.push_back(e.first); c.hBEdge.push_back(e.second); }
     c.nBN = (int)c.hBNode.size();
     c.nBE = (int)c.hBEdge.size() / 2;
     // inverse rest metric, hinges, DISCRETE_SHELL.h:132-213
@@ -153,6 +182,29 @@ void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri
         c.hMass[v[0]] += mp; c.hMass[v[1]] += mp; c.hMass[v[2]] += mp;
         c.hVol[t] = area * P.thick;
     }
+    // tets: rest basis Dm^-1, volume det(Dm) / 6, lumped mass vol rho / 4 per node
+    // (FEM/DEFORMATION_GRADIENT.h:39-77, FEM/ELEM_TO_NODE.h:11-46), Lame parameters
+    // (Physics/NEOHOOKEAN.h:46-47)
+    std::vector<double> tetIB(9 * (size_t)c.nTet), tetVol(c.nTet);
+    for (int t = 0; t < c.nTet; ++t) {
+        const int* v = &c.hTet[4 * t];
+        double p0[3], q[3][3];
+        hv(X, v[0], p0);
+        for (int k = 0; k < 3; ++k) { double pk[3]; hv(X, v[k + 1], pk); hsub(pk, p0, q[k]); }
+        // Dm row-major: column k = q[k]
+        const double M[9] = {q[0][0], q[1][0], q[2][0], q[0][1], q[1][1], q[2][1], q[0][2], q[1][2], q[2][2]};
+        const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) + M[2] * (M[3] * M[7] - M[4] * M[6]);
+        if (!(det > 0)) throw std::invalid_argument("tet " + std::to_string(t) + " is inverted or degenerate in the rest shape");
+        double* I = &tetIB[9 * (size_t)t];
+        I[0] = (M[4] * M[8] - M[5] * M[7]) / det; I[1] = (M[2] * M[7] - M[1] * M[8]) / det; I[2] = (M[1] * M[5] - M[2] * M[4]) / det;
+        I[3] = (M[5] * M[6] - M[3] * M[8]) / det; I[4] = (M[0] * M[8] - M[2] * M[6]) / det; I[5] = (M[2] * M[3] - M[0] * M[5]) / det;
+        I[6] = (M[3] * M[7] - M[4] * M[6]) / det; I[7] = (M[1] * M[6] - M[0] * M[7]) / det; I[8] = (M[0] * M[4] - M[1] * M[3]) / det;
+        tetVol[t] = det / 6.0;
+        const double mt = tetVol[t] * P.tetRho / 4;
+        for (int k = 0; k < 4; ++k) c.hMass[v[k]] += mt;
+    }
+    c.tetLam = P.tetE * P.tetNu / ((1.0 + P.tetNu) * (1.0 - 2.0 * P.tetNu));
+    c.tetMu = P.tetE / (2.0 * (1.0 + P.tetNu));
     c.lam = P.E * P.nu / (1.0 - P.nu * P.nu);
     c.mu = P.E / (2.0 * (1.0 + P.nu));
     c.kBend = P.bendMult * (P.E * std::pow(P.thick, 3) / (24 * (1.0 - P.nu * P.nu)));
@@ -249,6 +301,12 @@ void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri
     upload(c.hinge, soa_i(c.hHinge, c.nH, 4), s);
     upload(c.hInfo, soa_d(c.hHingeInfo, c.nH, 3), s);
     upload(c.bNode, c.hBNode, s);
+    upload(c.btri, soa_i(c.hBTri, c.nBT, 3), s);
+    if (c.nTet) {
+        upload(c.tetIdx, soa_i(c.hTet, c.nTet, 4), s);
+        upload(c.tetIB, soa_d(tetIB, c.nTet, 9), s);
+        upload(c.tetVol, tetVol, s);
+    }
     upload(c.bEdge, soa_i(c.hBEdge, c.nBE, 2), s);
     if (c.nRod) {
         // spring weight h^2 pi t^2 / 4 (ROD_STRETCHING.h:22-24), bending weight
@@ -308,6 +366,15 @@ __global__ void k_dbc_set(int n, const int* __restrict__ idx, const double* __re
 #pragma unroll
     for (int k = 0; k < 3; ++k) x[3 * v + k] = tgt[3 * i + k];
 }
+// Dirichlet nodes moved the fraction alpha of the way from x_n to their targets
+__global__ void k_dbc_blend(int n, const int* __restrict__ idx, const double* __restrict__ tgt, const double* __restrict__ xn,
+                            double alpha, double* __restrict__ x) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = idx[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[3 * v + k] = xn[3 * v + k] + alpha * (tgt[3 * i + k] - xn[3 * v + k]);
+}
 // rhs = -g with Dirichlet rows zeroed (IMPLICIT_EULER.h:476-491)
 __global__ void k_rhs(int nV, double* __restrict__ g, const uint8_t* __restrict__ dbc, double* __restrict__ rhs) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -344,6 +411,122 @@ __global__ void k_kappa_flag(int n, const double* __restrict__ d2prev, const dou
     if (d2prev[i] < 1e-18 && d2cur[i] < d2prev[i]) atomicOr(flag, 1u);
 }
 
+// residual.txt statistics of a Newton direction: per workgroup the sum of the nodes'
+// |p_i| and the largest |p component| (IMPLICIT_EULER.h:648-663)
+__global__ void k_res_stats(int nV, const double* __restrict__ p, double* __restrict__ part) {
+    __shared__ double lds[4];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double mag = 0, mx = 0;
+    if (v < nV) {
+        const double a = p[3 * v], b = p[3 * v + 1], c = p[3 * v + 2];
+        mag = sqrt(a * a + b * b + c * c);
+        mx = fmax(fabs(a), fmax(fabs(b), fabs(c)));
+    }
+    const double s = block_sum_256(mag, lds);
+    double m = mx;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    __shared__ double lm[4];
+    if ((threadIdx.x & 63) == 0) lm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s;
+        part[2 * blockIdx.x + 1] = fmax(fmax(lm[0], lm[1]), fmax(lm[2], lm[3]));
+    }
+}
+// stretch.txt statistics (Compute_Max_And_Avg_Stretch, DISCRETE_SHELL.h:602-674): per
+// triangle not all Dirichlet, F = A B^-1 with A the deformed edge frame
+// [[|e1|, e1.e2/|e1|], [0, |e1 x e2|/|e1|]] and B the factor of the rest metric; per
+// workgroup: max sigma_max, sum and count of sigma_max > 1, min sigma_min, sum and count
+// of sigma_min < 1
+__global__ void k_stretch_stats(const double* __restrict__ x, const int* __restrict__ tri, int nT, const double* __restrict__ IB,
+                                const uint8_t* __restrict__ dbc, double* __restrict__ part) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double smax = 1.0, ssum = 0, scnt = 0, cmin = 1.0, csum = 0, ccnt = 0;
+    if (t < nT) {
+        const int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+        if (!(dbc[a] && dbc[b] && dbc[c])) {
+            const d3 e1 = ld3(x, b) - ld3(x, a), e2 = ld3(x, c) - ld3(x, a);
+            const double l = sqrt(sq(e1));
+            const double A00 = l, A01 = dot(e1, e2) / l, A11 = sqrt(sq(cross(e1, e2))) / l;
+            const double i00 = IB[t], i01 = IB[nT + t], i11 = IB[2 * nT + t];   // B^-1 (upper triangular)
+            const double F00 = A00 * i00, F01 = A00 * i01 + A01 * i11, F11 = A11 * i11;
+            const double p = F00 * F00, q = F00 * F01, r = F01 * F01 + F11 * F11;   // F^T F
+            const double hm = 0.5 * (p - r), rt = sqrt(hm * hm + q * q);
+            const double s0 = sqrt(0.5 * (p + r) + rt), s1 = sqrt(fmax(0.5 * (p + r) - rt, 0.0));
+            smax = s0;
+            if (s0 > 1) { ssum = s0; scnt = 1; }
+            cmin = s1;
+            if (s1 < 1) { csum = s1; ccnt = 1; }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        smax = fmax(smax, __shfl_xor(smax, o, 64));
+        cmin = fmin(cmin, __shfl_xor(cmin, o, 64));
+        ssum += __shfl_xor(ssum, o, 64); scnt += __shfl_xor(scnt, o, 64);
+        csum += __shfl_xor(csum, o, 64); ccnt += __shfl_xor(ccnt, o, 64);
+    }
+    __shared__ double L[4][6];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { L[w][0] = smax; L[w][1] = ssum; L[w][2] = scnt; L[w][3] = cmin; L[w][4] = csum; L[w][5] = ccnt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double o6[6] = {L[0][0], L[0][1], L[0][2], L[0][3], L[0][4], L[0][5]};
+        for (int k = 1; k < 4; ++k) {
+            o6[0] = fmax(o6[0], L[k][0]); o6[1] += L[k][1]; o6[2] += L[k][2];
+            o6[3] = fmin(o6[3], L[k][3]); o6[4] += L[k][4]; o6[5] += L[k][5];
+        }
+        for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = o6[k];
+    }
+}
+
+static void append_line(const Ctx& c, const char* file, const char* line) {
+    const std::string path = c.outputFolder + "/" + file;
+    FILE* f = std::fopen(path.c_str(), "a+");
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::fputs(line, f);
+    std::fclose(f);
+}
+// residual.txt line of a Newton iteration: PNIter avgResMag maxRes Eprev L2Norm
+static void log_residual(Ctx& c, int pnIter, double E, double L2, double hres) {
+    const int nb = nblocks(c.nV);
+    double* part = c.red.ensure(2 * (size_t)nb);
+    hipLaunchKernelGGL(k_res_stats, dim3(nb), dim3(kBlock), 0, c.s, c.nV, c.p.p, part);
+    CIPC_LAUNCH_CHECK();
+    std::vector<double> h(2 * (size_t)nb);
+    CIPC_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    double sum = 0, mx = 0;
+    for (int b = 0; b < nb; ++b) { sum += h[2 * b]; mx = std::max(mx, h[2 * b + 1]); }
+    char line[256];
+    std::snprintf(line, sizeof(line), "%d %le %le %le %le\n", pnIter, sum / (c.nV - c.nDBC) / hres, mx / hres, E, L2);
+    append_line(c, "residual.txt", line);
+}
+// counter.txt (PNIter, contacts) and stretch.txt (maxs avgs minc avgc) at the step's end
+static void log_step_end(Ctx& c, int pnIter) {
+    char line[256];
+    if (c.P.collision) std::snprintf(line, sizeof(line), "%d %d\n", pnIter, c.nC);
+    else std::snprintf(line, sizeof(line), "%d\n", pnIter);
+    append_line(c, "counter.txt", line);
+    if (!c.nT) return;
+    const int nb = nblocks(c.nT);
+    double* part = c.red.ensure(6 * (size_t)nb);
+    hipLaunchKernelGGL(k_stretch_stats, dim3(nb), dim3(kBlock), 0, c.s, c.x.p, c.tri.p, c.nT, c.slIB.p, c.dbc.p, part);
+    CIPC_LAUNCH_CHECK();
+    std::vector<double> h(6 * (size_t)nb);
+    CIPC_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    double maxs = 1, ssum = 0, scnt = 0, minc = 1, csum = 0, ccnt = 0;
+    for (int b = 0; b < nb; ++b) {
+        const double* o = &h[6 * (size_t)b];
+        maxs = std::max(maxs, o[0]); ssum += o[1]; scnt += o[2];
+        minc = std::min(minc, o[3]); csum += o[4]; ccnt += o[5];
+    }
+    std::snprintf(line, sizeof(line), "%le %le %le %le\n", maxs, scnt ? ssum / scnt : 0.0, minc, ccnt ? csum / ccnt : 0.0);
+    append_line(c, "stretch.txt", line);
+}
+
 static double sumsq(Ctx& c, const double* a, long n) {
     int nb = nblocks(n);
     double* part = c.red.ensure(nb);
@@ -358,6 +541,7 @@ static double sumsq(Ctx& c, const double* a, long n) {
 // incremental potential at x with constraint set C (INC_POTENTIAL.h:15-152)
 // +inf when a strain limit is reached (INC_POTENTIAL.h:75-80: the potential is invalid)
 static double inc_potential(Ctx& c, const double* x, const int* C, int nC, int* bad) {
+    if (c.nTet && !tets_valid(c, x)) { if (bad) *bad = 0; return INFINITY; }   // an inverted tet, INC_POTENTIAL.h:98-107
     double Esl = strain_limit_energy(c, x);
     if (fiber_on(c)) Esl += fiber_energy(c, x);   // Check_Fiber_Feasibility + Compute_Fiber_Energy
     if (std::isinf(Esl)) { if (bad) *bad = 0; return Esl; }
@@ -382,14 +566,23 @@ int advance_one_step(Ctx& c) {
     st = cipc_step_stats{};
     c.lpStepStart = true;
     c.lpOffThisStep = false;
+    c.kappaSDoublings = 0;
     double t0 = now_ms(), ta;
     hipStream_t s = c.s;
     static_assert(sizeof(int) == 4, "");
     // x_n, x_tilde (IMPLICIT_EULER.h:195-217)
+    const bool stat = P.staticSolve != 0;
+    if (stat && P.collision && (P.fricMu > 0 || c.nFricComp > 0)) {
+        c.lastError = "a static solve has no previous state for lagged friction (the reference leaves Xn unset)";
+        return CIPC_E_UNSUPPORTED;
+    }
+    const double hres = stat ? 1.0 : h;   // residual scale, IMPLICIT_EULER.h:662-665
     CIPC_HIP(hipMemcpyAsync(c.xn.p, c.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_predict, dim3(nblocks(c.nV)), dim3(kBlock), 0, s, c.nV, c.x.p, c.v.p, c.mass.p, h, P.grav[0], P.grav[1], P.grav[2],
-                       c.xt.p);
-    CIPC_LAUNCH_CHECK();
+    if (!stat) {
+        hipLaunchKernelGGL(k_predict, dim3(nblocks(c.nV)), dim3(kBlock), 0, s, c.nV, c.x.p, c.v.p, c.mass.p, h, P.grav[0], P.grav[1],
+                           P.grav[2], c.xt.p);
+        CIPC_LAUNCH_CHECK();
+    }
     // Dirichlet motion and its CCD (IMPLICIT_EULER.h:304-401): move the nodes when the CCD
     // allows the whole displacement, else keep them and pull them with the penalty
     c.dbcStiff = 0;
@@ -404,6 +597,22 @@ int advance_one_step(Ctx& c) {
         if (P.collision) {
             int z = 0;
             dbcAlpha = ccd_stepsize(c, c.x.p, c.p.p, 1.0, nullptr, &z);
+        }
+        // a motion that would break a strain limit or the fibre limits is cut back by
+        // halving until the moved nodes leave every triangle feasible (IMPLICIT_EULER.h:
+        // 336-376); only dbcAlpha survives, the nodes stay at x_n here
+        if ((strain_limit_on(c) || (c.P.fiberK[0] > 0 && c.nT) || c.nTet)) {
+            for (int it = 0; it < 64; ++it) {
+                hipLaunchKernelGGL(k_dbc_blend, dim3(nblocks(c.nDBC)), dim3(kBlock), 0, s, c.nDBC, c.dbcIdxD.p, c.dbcTgt.p, c.xn.p,
+                                   dbcAlpha, c.x.p);
+                CIPC_LAUNCH_CHECK();
+                double Ef = strain_limit_energy(c, c.x.p);
+                if (fiber_on(c)) Ef += fiber_energy(c, c.x.p);
+                if (c.nTet && !tets_valid(c, c.x.p)) Ef = INFINITY;   // IMPLICIT_EULER.h:364-374
+                if (!std::isinf(Ef)) break;
+                dbcAlpha /= 2;
+            }
+            CIPC_HIP(hipMemcpyAsync(c.x.p, c.xn.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         }
         if (dbcAlpha == 1) {
             hipLaunchKernelGGL(k_dbc_set, dim3(nblocks(c.nDBC)), dim3(kBlock), 0, s, c.nDBC, c.dbcIdxD.p, c.dbcTgt.p, c.x.p);
@@ -483,13 +692,15 @@ int advance_one_step(Ctx& c) {
         // line search (IMPLICIT_EULER.h:9-149)
         CIPC_HIP(hipMemcpyAsync(c.xprev.p, c.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         double alpha = 1.0;
-        // strain-limit feasibility first (IMPLICIT_EULER.h:74-92), then the CCD from there
-        if (strain_limit_on(c) || (c.P.fiberK[0] > 0 && c.nT)) {
+        // feasibility first -- strain limits, fibre limits, uninverted tets
+        // (IMPLICIT_EULER.h:74-92) -- then the CCD from there
+        if (strain_limit_on(c) || (c.P.fiberK[0] > 0 && c.nT) || c.nTet) {
             for (;;) {
                 hipLaunchKernelGGL(k_axpy_to, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c.xprev.p, alpha, c.p.p, c.x.p);
                 CIPC_LAUNCH_CHECK();
                 double Ef = strain_limit_energy(c, c.x.p);
                 if (fiber_on(c)) Ef += fiber_energy(c, c.x.p);
+                if (c.nTet && !tets_valid(c, c.x.p)) Ef = INFINITY;
                 if (!std::isinf(Ef)) break;
                 alpha /= 2;
             }
@@ -548,10 +759,23 @@ int advance_one_step(Ctx& c) {
             c.d2prev.ensure(c.nC);
             min_dist2(c, c.x.p, c.C.p, c.nC, c.d2prev.p);
         }
+        // kappa_s adaptation (IMPLICIT_EULER.h:606-645): double the upper strain-limit
+        // stiffness, up to 1e5, when a triangle's largest singular value stayed within
+        // 1e-4 (s - sHat) of the limit at this and the previous Newton iterate; the
+        // caller's kappa_s is updated in place (cipc_get_strain_limit reads it back)
+        if (strain_limit_on(c)) {
+            const bool upd = strain_limit_near_twice(c, c.x.p, PNIter == 0);
+            if (upd && c.P.slKappa[0] < 1e5) {
+                c.P.slKappa[0] *= 2;
+                c.kappaSDoublings++;
+                Eprev = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
+            }
+        }
         // stopping rule (IMPLICIT_EULER.h:647-710, 866)
         double ss = sumsq(c, c.p.p, n);
-        L2Norm = std::sqrt(ss / (c.nV - c.nDBC)) / h;
+        L2Norm = std::sqrt(ss / (c.nV - c.nDBC)) / hres;
         PNIter++;
+        if (!c.outputFolder.empty()) log_residual(c, PNIter, Eprev, L2Norm, hres);
         resRecord.push_back(L2Norm);
         if (resRecord.size() > 3) resRecord.pop_front();
         double cur = L2Norm;
@@ -591,7 +815,7 @@ int advance_one_step(Ctx& c) {
                 friction_basis(c, c.x.p, c.C.p, c.nC, c.kappa);
                 havePrev = false;
                 if (!newton_direction()) { c.lastError = "friction update: Newton matrix not positive definite"; return CIPC_E_NO_CONVERGENCE; }
-                L2Norm = std::sqrt(sumsq(c, c.p.p, n) / (c.nV - c.nDBC)) / h;
+                L2Norm = std::sqrt(sumsq(c, c.p.p, n) / (c.nV - c.nDBC)) / hres;
                 if (L2Norm > P.PNTol) {
                     Eprev = inc_potential(c, c.x.p, c.C.p, c.nC, &bad);
                     reuseDir = true;
@@ -600,12 +824,17 @@ int advance_one_step(Ctx& c) {
         }
     } while (resRecord.size() < 3 || L2Norm > P.PNTol);
     c.fricOn = false;
-    hipLaunchKernelGGL(k_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c.x.p, c.xn.p, h, c.v.p);
-    CIPC_LAUNCH_CHECK();
+    if (!stat) {   // velocities stay as they are after a static solve (IMPLICIT_EULER.h:893-899)
+        hipLaunchKernelGGL(k_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c.x.p, c.xn.p, h, c.v.p);
+        CIPC_LAUNCH_CHECK();
+    }
     CIPC_HIP(hipStreamSynchronize(s));
+    if (!c.outputFolder.empty()) log_step_end(c, PNIter);
     st.newton_iters = PNIter;
     st.contacts = c.nC;
     st.kappa = c.kappa;
+    st.kappa_s = c.P.slKappa[0];
+    st.kappa_s_doublings = c.kappaSDoublings;
     st.energy = Eprev;
     st.t_total_ms = now_ms() - t0;
     st.status = CIPC_OK;

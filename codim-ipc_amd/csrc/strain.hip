@@ -106,6 +106,27 @@ __global__ void k_sl_energy(const double* __restrict__ x, const int* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// Compute_Inextensibility (INEXT.h:72-114) for the kappa_s rule of IMPLICIT_EULER.h:
+// 606-645: near[t] = 1 when triangle t's largest singular value lies within thr of the
+// upper limit s0 (a triangle whose three nodes are Dirichlet counts as 1, never near);
+// *flag is set when a triangle is near now and was near at the previous iterate
+__global__ void k_sl_near_limit(const double* __restrict__ x, const int* __restrict__ tri, int nT, const double* __restrict__ IB,
+                                const uint8_t* __restrict__ dbc, double s0, double thr, const uint8_t* __restrict__ prev,
+                                uint8_t* __restrict__ near, uint32_t* __restrict__ flag) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const int a = tri[t], b = tri[nT + t], c = tri[2 * nT + t];
+    double smax = 1.0;
+    if (!sl_all_dbc(dbc, a, b, c)) {
+        SLSvd S;
+        sl_svd(ld3(x, a), ld3(x, b), ld3(x, c), IB[t], IB[nT + t], IB[2 * nT + t], S);
+        smax = S.sig[0];
+    }
+    const uint8_t n = (s0 - smax < thr) ? 1 : 0;
+    near[t] = n;
+    if (n && prev && prev[t]) atomicOr(flag, 1u);
+}
+
 // vec(u_a v_b^T) with columns stacked (F column 0 then column 1)
 CIPC_DEV void sl_dir(const SLSvd& S, int ua, int vb, double* o) {
     const d3 u = S.u[ua];
@@ -371,7 +392,7 @@ static SLPar sl_par(const Ctx& c) {
     SLPar P;
     P.k0 = c.P.slKappa[0]; P.s0 = c.P.slS[0]; P.sh0 = c.P.slSHat[0];
     P.k1 = c.P.slKappa[1]; P.s1 = c.P.slS[1]; P.sh1 = c.P.slSHat[1];
-    P.h2 = c.P.dt * c.P.dt;
+    P.h2 = shell_h2(c);
     return P;
 }
 bool strain_limit_on(const Ctx& c) { return c.P.slKappa[0] > 0 && c.nT > 0; }
@@ -386,6 +407,20 @@ double strain_limit_energy(Ctx& c, const double* x) {
     CIPC_HIP(hipMemcpyAsync(c.hScal + 49, c.scal.p + 49, sizeof(double), hipMemcpyDeviceToHost, c.s));
     CIPC_HIP(hipStreamSynchronize(c.s));
     return c.hScal[49];
+}
+bool strain_limit_near_twice(Ctx& c, const double* x, bool first) {
+    if (!strain_limit_on(c)) return false;
+    uint8_t* cur = (c.slNearCur ? c.slNear1 : c.slNear0).ensure(c.nT);
+    const uint8_t* prev = first ? nullptr : (c.slNearCur ? c.slNear0 : c.slNear1).p;
+    const double thr = 1.0e-4 * (c.P.slS[0] - c.P.slSHat[0]);
+    CIPC_HIP(hipMemsetAsync(c.counters.p + 15, 0, sizeof(uint32_t), c.s));
+    hipLaunchKernelGGL(k_sl_near_limit, dim3(nblocks(c.nT)), dim3(kBlock), 0, c.s, x, c.tri.p, c.nT, c.slIB.p, c.dbc.p, c.P.slS[0], thr,
+                       prev, cur, c.counters.p + 15);
+    CIPC_LAUNCH_CHECK();
+    CIPC_HIP(hipMemcpyAsync(c.hCnt + 15, c.counters.p + 15, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+    CIPC_HIP(hipStreamSynchronize(c.s));
+    c.slNearCur ^= 1;
+    return c.hCnt[15] != 0;
 }
 void strain_limit_grad_add(Ctx& c, const double* x, double* gl) {
     if (!strain_limit_on(c)) return;
@@ -402,7 +437,7 @@ static FBPar fb_par(const Ctx& c) {
     FBPar P;
     for (int i = 0; i < 4; ++i) P.k[i] = c.P.fiberK[i];
     for (int i = 0; i < 3; ++i) P.L[i] = c.P.fiberLimit[i];
-    P.h2 = c.P.dt * c.P.dt;
+    P.h2 = shell_h2(c);
     return P;
 }
 bool fiber_on(const Ctx& c) { return (c.P.fiberK[0] > 0 || c.P.fiberK[1] > 0) && c.nT > 0; }

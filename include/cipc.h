@@ -84,12 +84,25 @@ typedef struct cipc_scene_desc {
      * Neo-Hookean membrane: log barriers on the Green strain components E00, E00 E11, E11,
      * E01 of the rest x-z frame, limits |E00| < fiber_limit[0], |E11| < [1], |E01| < [2]. */
     double fiber_stiff_mult[4], fiber_limit[3];
+    /* volumetric tets inside the shell step (add_object_3D + initialize_added_objects of
+     * FEMDiscreteShellBase.py, the step's tet / tetAttr / tetElasticity arguments):
+     * Neo-Hookean (Physics/NEOHOOKEAN.h) with Young's modulus tet_youngs and Poisson ratio
+     * tet_poisson, lumped mass tet_density vol / 4 per node (FEM/ELEM_TO_NODE.h:11-46);
+     * their surface triangles (Find_Surface_TriMesh) join the contact primitives ahead of
+     * the shell's.  Every rest tet must be positively oriented.  Zero count: no tets. */
+    int32_t n_tets;
+    const int32_t* tets;             /* 4 n_tets vertex ids */
+    double tet_density, tet_youngs, tet_poisson;
 } cipc_scene_desc;
 
 typedef struct cipc_step_stats {
     int32_t newton_iters, contacts, pcg_iters, gd_fallbacks, ls_trials, status;
     double kappa, last_residual, energy;
     double t_total_ms, t_constraint_set_ms, t_ccd_ms, t_hessian_ms, t_solve_ms, t_energy_ms, t_gradient_ms;
+    /* upper strain-limit stiffness kappa_s[0] after the step and how often the step doubled
+     * it (IMPLICIT_EULER.h:606-645; the reference mutates its kappa_s argument in place) */
+    double kappa_s;
+    int32_t kappa_s_doublings;
 } cipc_step_stats;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -108,6 +121,14 @@ int cipc_get_setup(const cipc_ctx* ctx, double* mass, int32_t* hinges, double* h
  * out[0..4] = n_rod_segments, n_rod_hinges, n_particles, first rod-node slot and first
  * particle slot among the boundary nodes */
 int cipc_get_codim_sizes(const cipc_ctx* ctx, int32_t* out);
+/* out[0] = n_tets, out[1] = contact triangles (the tets' surface, then the shell's) */
+int cipc_get_tet_sizes(const cipc_ctx* ctx, int32_t* out);
+/* Compute_Inversion_Free_StepSize (FEM/IPC.h:1785-1877): the largest step in (0, alpha]
+ * along p (3 n_vertices) from x that keeps every tet's volume above 20% of its volume at
+ * x (smallest positive root of the cubic det(x + t p) = 0.2 det(x), IPC.h:1751-1782).
+ * The shell step itself keeps tets uninverted by halving its line search
+ * (IMPLICIT_EULER.h:74-92), as the reference's shell step does. */
+int cipc_inversion_free_stepsize(cipc_ctx* ctx, const double* x, const double* p, double alpha, double* out);
 /* rod_info 3 per segment (E, rest length, thickness), rod_hinges 3 per hinge (x0, shared
  * node, x2), rod_hinge_info 3 per hinge (E * bend mult, rest |e0| + |e1|, thickness),
  * bnodes (n_boundary_nodes) */
@@ -169,7 +190,25 @@ int cipc_get_dirichlet_targets(cipc_ctx* ctx, double* targets);
 int cipc_set_dirichlet_targets(cipc_ctx* ctx, const double* targets);
 /* Strain-limit parameters for the following steps (kappa_s, s, sHat of
  * Advance_One_Step_IE_Hinge, IMPLICIT_EULER.h:162); same meaning as in cipc_scene_desc. */
+/* The step's staticSolve argument (IMPLICIT_EULER.h:170): enable != 0 makes the following
+ * steps static solves -- the shell's membrane, bending, strain-limit and fibre terms
+ * weighted by 1 instead of h^2 (rods and tets keep h^2, as in the reference), the body
+ * force potential -x.(M g) instead of inertia, no mass in the Hessian, the Newton residual
+ * not divided by h, velocities left unchanged (IMPLICIT_EULER.h:196-217, 646-666, 893-899,
+ * INC_POTENTIAL.h:118-130, 233-249, 383-386).  Lagged friction is refused in a static
+ * solve (CIPC_E_UNSUPPORTED): the reference has no previous state for it there. */
+int cipc_set_static_solve(cipc_ctx* ctx, int32_t enable);
+/* The step's outputFolder argument (IMPLICIT_EULER.h:187): with a folder set (NULL or ""
+ * unsets it) each step appends, as the reference does, one line per Newton iteration to
+ * residual.txt ("PNIter avgResMag maxRes Eprev L2Norm", :648-670), "PNIter contacts" to
+ * counter.txt (:868-875) and "maxs avgs minc avgc" of the triangles' in-plane stretch to
+ * stretch.txt (:877-883, Compute_Max_And_Avg_Stretch, DISCRETE_SHELL.h:602-674). */
+int cipc_set_output_folder(cipc_ctx* ctx, const char* folder);
 int cipc_set_strain_limit(cipc_ctx* ctx, const double* kappa, const double* s, const double* shat);
+/* kappa_s, s, sHat as the steps have left them: the step doubles kappa_s[0] (up to 1e5)
+ * when a triangle stays at the limit, in place, as the reference does with its kappa_s
+ * argument (IMPLICIT_EULER.h:606-645).  Null outputs are skipped. */
+int cipc_get_strain_limit(const cipc_ctx* ctx, double* kappa, double* s, double* shat);
 /* fiberStiffMult (4) and inextLimit (3) for the following steps; same meaning as in
  * cipc_scene_desc. */
 int cipc_set_fiber(cipc_ctx* ctx, const double* stiff_mult, const double* limit);
@@ -218,9 +257,15 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
  * mode 2 (fp32): FP32 blocks, FP32 Krylov vectors, FP32 SpMV accumulation.
  * Modes 1 and 2 run inside an FP64 iterative refinement that stops when the FP64
  * residual ||b - H x|| <= pcg_tol ||b||.  Affects cipc_step and cipc_linear_solve;
- * out[0] of cipc_get_solver_info = refinement sweeps of the last solve (0 in mode 0). */
+ * out[0] of cipc_get_solver_info = refinement sweeps of the last solve (0 in mode 0),
+ * out[1], out[2] = PCG chunk graphs captured and replayed so far (cipc_set_graphs). */
 int cipc_set_precision(cipc_ctx* ctx, int32_t mode);
 int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out);
+/* No reference counterpart.  enable != 0 (the default): every 16-iteration PCG chunk after
+ * a solve's first runs as one replay of a captured HIP graph (its launches identical from
+ * chunk to chunk; re-captured when the matrix, buffers or launch shapes change); 0: the
+ * chunks' kernels are launched one by one.  Both give the same bits. */
+int cipc_set_graphs(cipc_ctx* ctx, int32_t enable);
 
 /* ---- learned preconditioner (no counterpart in the reference) ------------
  * A 4-layer message-passing network (hidden 64, BF16 weights, FP32 accumulation) over
@@ -263,7 +308,9 @@ int cipc_precond_set_adam(cipc_ctx* ctx, const float* m, const float* v, int32_t
 /* out[0] = 1 if the last solve fell back to block Jacobi; out[1], out[2] = learned
  * solves so far and how many of them fell back; out[3], out[4] = status (0 converged,
  * 1 non-positive curvature, 2 budget/refinement limit) and iterations of the last
- * learned attempt */
+ * learned attempt; out[5] = 1 if that fallback continued from the learned iterate (else
+ * it restarted from zero); out[6] = iterations of the last from-zero block-Jacobi solve
+ * (the cap of the learned budget: 1.25x this).  out holds 7 values. */
 int cipc_precond_info(const cipc_ctx* ctx, int32_t* out);
 
 /* ---- launch shapes and autotuning (no counterpart in the reference) -------

@@ -1,6 +1,7 @@
 // Oracle C entry points for ctypes (tests/, smoke, bench cpu_baseline only).
 // TEST INFRASTRUCTURE ONLY (see oracle.h).
 #include "oracle.h"
+#include <algorithm>
 #include <cstring>
 
 using namespace orc;
@@ -79,10 +80,15 @@ void orc_rod_kappa2(const double* x, double* k2, double* g, double* H) {
 // params: rho, E, nu, thick, bendMult, dt, dHat, xi, PNTol, gx, gy, gz, kappaStiffMult, pcgTol
 // iparams: collision, solver, pcgMaxIter, threads
 // codim (may be null when nRod = nPart = 0): rodRho, rodE, rodBendMult, rodThick, partRho, partThick
-void* orc_scene_create_codim(int nV, const double* X0, int nT, const int* tri, int nDBC, const int* dbcIdx,
-                             const double* dbcTarget, const double* params, const int* iparams, int nRod, const int* rods,
-                             int nPart, const int* parts, const double* codim) {
+// tets (may be null when nTet = 0): 4 per tet; tetParams: density, E, nu
+void* orc_scene_create_full(int nV, const double* X0, int nT, const int* tri, int nDBC, const int* dbcIdx,
+                            const double* dbcTarget, const double* params, const int* iparams, int nRod, const int* rods,
+                            int nPart, const int* parts, const double* codim, int nTet, const int* tets, const double* tetParams) {
     Scene* S = new Scene();
+    if (nTet) {
+        S->tet.assign(tets, tets + 4 * (size_t)nTet);
+        S->P.tetRho = tetParams[0]; S->P.tetE = tetParams[1]; S->P.tetNu = tetParams[2];
+    }
     if (codim) {
         S->P.rodRho = codim[0]; S->P.rodE = codim[1]; S->P.rodBendMult = codim[2]; S->P.rodThick = codim[3];
         S->P.partRho = codim[4]; S->P.partThick = codim[5];
@@ -102,6 +108,40 @@ void* orc_scene_create_codim(int nV, const double* X0, int nT, const int* tri, i
     S->dbcTarget.assign(dbcTarget, dbcTarget + 3 * nDBC);
     scene_init(*S);
     return S;
+}
+void* orc_scene_create_codim(int nV, const double* X0, int nT, const int* tri, int nDBC, const int* dbcIdx,
+                             const double* dbcTarget, const double* params, const int* iparams, int nRod, const int* rods,
+                             int nPart, const int* parts, const double* codim) {
+    return orc_scene_create_full(nV, X0, nT, tri, nDBC, dbcIdx, dbcTarget, params, iparams, nRod, rods, nPart, parts, codim, 0, nullptr,
+                                 nullptr);
+}
+// 1 when the scene was set up (0: e.g. an inverted rest tet)
+int orc_scene_ok(void* h) { return ((Scene*)h)->setupError.empty() ? 1 : 0; }
+// ---- tets
+int orc_tet_count(void* h) { return (int)((Scene*)h)->tet.size() / 4; }
+// surface triangles of the tets (Find_Surface_TriMesh): returns the count, writes up to cap
+int orc_tet_surface(void* h, int* out, int cap) {
+    std::vector<int> tri;
+    tet_surface(*(Scene*)h, tri);
+    const int n = (int)tri.size() / 3;
+    for (int i = 0; i < 3 * std::min(n, cap); ++i) out[i] = tri[i];
+    return n;
+}
+// rest data: Dm^-1 (9 per tet, row-major), volume; lambda and mu
+void orc_tet_arrays(void* h, double* IB, double* vol, double* lamMu) {
+    Scene* S = (Scene*)h;
+    if (IB) std::memcpy(IB, S->tetIB.data(), sizeof(double) * S->tetIB.size());
+    if (vol) std::memcpy(vol, S->tetVol.data(), sizeof(double) * S->tetVol.size());
+    if (lamMu) { lamMu[0] = S->tetLam; lamMu[1] = S->tetMu; }
+}
+// one tet's 12x12 Hessian with weight w (h^2 vol in the step)
+void orc_tet_hess_local(void* h, const double* x, int t, double w, int project, double* H) {
+    tet_hess_local(*(Scene*)h, x, t, w, project != 0, H);
+}
+void orc_nh_dPdF(const double* F, double lam, double mu, int project, double* D) { nh_dPdF(F, lam, mu, 1.0, project != 0, D); }
+double orc_cubic_root(double a, double b, double c, double d) { return cubic_root(a, b, c, d, 1.e-6); }
+double orc_inversion_free_stepsize(void* h, const double* x, const double* p, double alpha) {
+    return inversion_free_stepsize(*(Scene*)h, x, p, alpha);
 }
 void* orc_scene_create(int nV, const double* X0, int nT, const int* tri, int nDBC, const int* dbcIdx,
                        const double* dbcTarget, const double* params, const int* iparams) {
@@ -126,6 +166,7 @@ void orc_set_solver(void* h, int solver, double tol) { ((Scene*)h)->P.solver = s
 double orc_get_kappa_s(void* h) { return ((Scene*)h)->P.slKappa[0]; }
 // k > 0: orc_step returns after k Newton iterations (positions left at that iterate)
 void orc_set_max_newton(void* h, int k) { ((Scene*)h)->P.maxNewton = k > 0 ? k : 0; }
+void orc_set_static_solve(void* h, int on) { ((Scene*)h)->P.staticSolve = on ? 1 : 0; }
 // 0: Hessians are assembled from the unprojected local matrices (test switch)
 void orc_set_project_pd(void* h, int on) { ((Scene*)h)->P.projectPD = on ? 1 : 0; }
 // incremental potential before / after each Newton iteration's line search in the last step

@@ -25,7 +25,7 @@ static inline bool hinge_all_dbc(const Scene& S, int h) {
 }
 
 double elastic_energy(const Scene& S, const double* x, double* membOut, double* bendOut) {
-    const double h2 = S.P.dt * S.P.dt;
+    const double h2 = shell_h2(S);
     double Em = 0, Eb = 0;
     const bool memb = !fiber_active(S);   // the fibre model replaces the membrane (INC_POTENTIAL.h:81-92)
     for (int t = 0; t < S.nT && memb; ++t) {
@@ -44,13 +44,14 @@ double elastic_energy(const Scene& S, const double* x, double* membOut, double* 
         }
     }
     codim_energy(S, x, &Em, &Eb);
+    if (!S.tet.empty()) Em += tet_energy(S, x);   // volumetric Neo-Hookean with the stretching terms
     if (membOut) *membOut = Em;
     if (bendOut) *bendOut = Eb;
     return Em + Eb;
 }
 
 void elastic_grad(const Scene& S, const double* x, double* g) {
-    const double h2 = S.P.dt * S.P.dt;
+    const double h2 = shell_h2(S);
     const bool memb = !fiber_active(S);
     for (int t = 0; t < S.nT && memb; ++t) {
         if (tri_all_dbc(S, t)) continue;
@@ -76,12 +77,13 @@ void elastic_grad(const Scene& S, const double* x, double* g) {
         }
     }
     codim_grad(S, x, g);
+    if (!S.tet.empty()) tet_grad(S, x, g);
 }
 
 // Local blocks are written straight into their final slots of `out` (the reference's
 // per-element triplet offsets, MEMBRANE.h:170-180, BENDING.h:443-455).
 void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
-    const double h2 = S.P.dt * S.P.dt;
+    const double h2 = shell_h2(S);
     std::vector<long> offT(S.nT + 1, 0);
     const bool memb = !fiber_active(S);
     for (int t = 0; t < S.nT; ++t) offT[t + 1] = offT[t] + ((tri_all_dbc(S, t) || !memb) ? 0 : 9);
@@ -105,6 +107,7 @@ void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
             }
     }
     codim_hess(S, x, out);
+    if (!S.tet.empty()) tet_hess(S, x, out);
     if (S.P.bendMult == 0) return;
     int nH = (int)S.hinge.size() / 4;
     std::vector<long> offH(nH + 1, 0);
@@ -137,8 +140,18 @@ void elastic_hess(const Scene& S, const double* x, std::vector<Block>& out) {
     }
 }
 
+double shell_h2(const Scene& S) { return S.P.staticSolve ? 1.0 : S.P.dt * S.P.dt; }
+
+// inertia 1/2 (x - xt)^T M (x - xt), or in a static solve the body force's -x . (M g)
+// (INC_POTENTIAL.h:118-144)
 double inertia_energy(const Scene& S, const double* x, const double* xt) {
     double E = 0;
+    if (S.P.staticSolve) {
+        for (int i = 0; i < S.nV; ++i)
+            if (S.mass[i] > 0)
+                for (int k = 0; k < 3; ++k) E -= x[3 * i + k] * (S.mass[i] * S.P.grav[k]);
+        return E;
+    }
     for (int i = 0; i < S.nV; ++i)
         for (int k = 0; k < 3; ++k) {
             double d = x[3 * i + k] - xt[3 * i + k];

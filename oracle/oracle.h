@@ -102,6 +102,8 @@ struct Params {
     // rods (Initialize_Discrete_Rod) and particles (Initialize_Discrete_Particle)
     double rodRho = 0, rodE = 0, rodBendMult = 1, rodThick = 1e-3;
     double partRho = 0, partThick = 1e-3;
+    // volumetric tets (add_object_3D + initialize_added_objects): density, E, nu
+    double tetRho = 0, tetE = 0, tetNu = 0.3;
     int collision = 1;
     // lagged friction (Advance_One_Step_IE_Hinge's mu, epsv2, fricIterAmt)
     double fricMu = 0, epsv2 = 1e-6;
@@ -120,7 +122,13 @@ struct Params {
     int threads = 1;
     int maxNewton = 0;         // > 0: return after this many Newton iterations (a timed prefix of a step)
     int projectPD = 1;         // 0: local Hessians assembled unprojected (finite-difference pins of the sums)
+    int staticSolve = 0;       // the step's staticSolve argument
 };
+
+struct Scene;
+// weight of the shell's membrane, bending, strain-limit and fibre terms: h^2, 1 in a static
+// solve (staticSolve ? 1.0 : h, INC_POTENTIAL.h:90-94)
+double shell_h2(const Scene& S);
 
 struct Scene {
     Params P;
@@ -145,6 +153,10 @@ struct Scene {
     std::vector<int> rodHinge;         // 3 per rod hinge (x0, x1 = shared node, x2)
     std::vector<double> rodHingeInfo;  // E * bend mult, rest |e0| + |e1|, thickness
     std::vector<int> particle;
+    // tets (FEM/DEFORMATION_GRADIENT.h:39-77): 4 per tet, Dm^-1 row-major (9 per tet), rest volume
+    std::vector<int> tet;
+    std::vector<double> tetIB, tetVol;
+    double tetLam = 0, tetMu = 0;
     int codimBN0 = 0, codimBN1 = 0;    // boundary-node slots where rod nodes / particles start
     double kappa = 0, kappaMax = 0, dHat2 = 0;
     double bboxDiag = 0;
@@ -156,6 +168,7 @@ struct Scene {
     // per Newton iteration of the last step: incremental potential before and after the
     // line search (the accepted iterate)
     std::vector<double> eBefore, eAfter;
+    std::string setupError;            // non-empty when the scene could not be set up
 };
 // Step_Dirichlet (BOUNDARY_CONDITION.h:107-155): rotate then translate every group's targets
 void step_dirichlet(Scene& S);
@@ -209,6 +222,18 @@ void strain_limit_grad(const Scene& S, const double* x, double* g);
 void strain_limit_hess(const Scene& S, const double* x, std::vector<Block>& out);
 bool strain_limit_tri(const Scene& S, const double* x, int t, double* E, double* g9, double* H81, bool proj, double* sig);
 void strain_max_sv(const Scene& S, const double* x, std::vector<double>& smax);
+
+// ---------------------------------------------------------------- volumetric tets (tet.cpp)
+bool tet_setup(Scene& S);                                   // rest basis, volume, lumped mass; false on an inverted rest tet
+void tet_surface(const Scene& S, std::vector<int>& tri);    // Find_Surface_TriMesh
+bool tets_valid(const Scene& S, const double* x);           // every det F > 0
+double tet_energy(const Scene& S, const double* x);         // h^2 sum vol psi (valid states only)
+void tet_grad(const Scene& S, const double* x, double* g);  // g +=
+void tet_hess(const Scene& S, const double* x, std::vector<Block>& out);
+void tet_hess_local(const Scene& S, const double* x, int t, double w, bool project, double* H144);
+void nh_dPdF(const double* F, double lam, double mu, double w, bool project, double* D81);
+double cubic_root(double a, double b, double c, double d, double tol);
+double inversion_free_stepsize(const Scene& S, const double* x, const double* p, double alpha);
 // anisotropic (fibre) membrane (FEM/Shell/ANISO_INEXT.h), replaces the Neo-Hookean membrane
 // when fiberK[0] > 0 or fiberK[1] > 0; energy adds to *E and returns false when infeasible
 void fiber_setup(Scene& S);

@@ -34,6 +34,17 @@ def lib():
         L.orc_dist2_unclassified.restype = ctypes.c_double
         L.orc_scene_create.restype = ctypes.c_void_p
         L.orc_scene_create_codim.restype = ctypes.c_void_p
+        L.orc_scene_create_full.restype = ctypes.c_void_p
+        L.orc_scene_ok.argtypes = [ctypes.c_void_p]
+        L.orc_tet_count.argtypes = [ctypes.c_void_p]
+        L.orc_tet_surface.argtypes = [ctypes.c_void_p, I, ctypes.c_int]
+        L.orc_tet_arrays.argtypes = [ctypes.c_void_p, D, D, D]
+        L.orc_tet_hess_local.argtypes = [ctypes.c_void_p, D, ctypes.c_int, ctypes.c_double, ctypes.c_int, D]
+        L.orc_nh_dPdF.argtypes = [D, ctypes.c_double, ctypes.c_double, ctypes.c_int, D]
+        L.orc_cubic_root.argtypes = [ctypes.c_double] * 4
+        L.orc_cubic_root.restype = ctypes.c_double
+        L.orc_inversion_free_stepsize.argtypes = [ctypes.c_void_p, D, D, ctypes.c_double]
+        L.orc_inversion_free_stepsize.restype = ctypes.c_double
         L.orc_ccd.restype = ctypes.c_double
         for name in ("orc_scene_destroy", "orc_scene_sizes", "orc_scene_scalars", "orc_scene_arrays", "orc_get_state",
                      "orc_set_state", "orc_set_kappa", "orc_constraint_set", "orc_energy", "orc_gradient", "orc_hessian",
@@ -46,6 +57,7 @@ def lib():
         L.orc_get_kappa_s.argtypes = [ctypes.c_void_p]
         L.orc_get_kappa_s.restype = ctypes.c_double
         L.orc_set_project_pd.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_static_solve.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_energy_trace.argtypes = [ctypes.c_void_p, D, D, ctypes.c_int]
         L.orc_barrier.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, D]
         L.orc_energy.argtypes = [ctypes.c_void_p, D, D, I, ctypes.c_int, ctypes.c_double, D]
@@ -172,6 +184,20 @@ def rod_kappa2(x):
     return k2[0], g, H
 
 
+def nh_dPdF(F, lam, mu, project=True):
+    """d2 psi / dF2 of the Neo-Hookean energy (9x9, vec(F) column-major), projected
+    block-wise as NEOHOOKEAN.h:192-300 does"""
+    Fr = np.ascontiguousarray(np.asarray(F, dtype=np.float64).reshape(3, 3)).ravel()
+    D = np.zeros(81)
+    lib().orc_nh_dPdF(_dp(Fr), float(lam), float(mu), int(project), _dp(D))
+    return D.reshape(9, 9)
+
+
+def cubic_root(a, b, c, d):
+    """smallest positive real root of a t^3 + b t^2 + c t + d (IPC.h:1751-1782), < 0 if none"""
+    return lib().orc_cubic_root(a, b, c, d)
+
+
 def solve(A_csr, b, solver=0, tol=1e-5):
     import scipy.sparse as sp
     A = sp.csr_matrix(A_csr)
@@ -205,10 +231,16 @@ class OracleScene:
         self._codim = np.array([P.get("rodRho", 0.0), P.get("rodE", 0.0), P.get("rodBendMult", 1.0), P.get("rodThick", 1e-3),
                                 P.get("partRho", 0.0), P.get("partThick", 1e-3)], dtype=np.float64)
         self._rods, self._parts = rods, parts
-        self.h = ctypes.c_void_p(L.orc_scene_create_codim(
+        tets = getattr(sd, "tets", None)
+        tets = np.zeros((0, 4), np.int32) if tets is None else np.ascontiguousarray(tets, dtype=np.int32)
+        self._tets = tets
+        self._tetp = np.array([P.get("tetRho", 0.0), P.get("tetE", 0.0), P.get("tetNu", 0.3)], dtype=np.float64)
+        self.h = ctypes.c_void_p(L.orc_scene_create_full(
             ctypes.c_int(sd.nV), _dp(sd.X), ctypes.c_int(sd.nT), _ip(sd.tris), ctypes.c_int(len(sd.dbc_idx)),
             _ip(sd.dbc_idx), _dp(sd.dbc_target), _dp(self._params), _ip(self._iparams), ctypes.c_int(len(rods)), _ip(rods),
-            ctypes.c_int(len(parts)), _ip(parts), _dp(self._codim)))
+            ctypes.c_int(len(parts)), _ip(parts), _dp(self._codim), ctypes.c_int(len(tets)), _ip(tets), _dp(self._tetp)))
+        if not L.orc_scene_ok(self.h):
+            raise ValueError("oracle scene set-up failed (an inverted rest tet)")
         s = np.zeros(5, dtype=np.int32)
         L.orc_scene_sizes(self.h, _ip(s))
         self.nT, self.nBN, self.nBE, self.nBT, self.nH = [int(v) for v in s]
@@ -239,6 +271,32 @@ class OracleScene:
     def set_solver(self, solver, tol=1e-5):
         lib().orc_set_solver(self.h, solver, tol)
 
+    # ---- volumetric tets
+    def tet_surface(self):
+        """Find_Surface_TriMesh (MESHIO.h:486-544) of the scene's tets, (n, 3)"""
+        n = lib().orc_tet_surface(self.h, None, 0)
+        out = np.zeros(3 * max(n, 1), dtype=np.int32)
+        lib().orc_tet_surface(self.h, _ip(out), n)
+        return out[:3 * n].reshape(-1, 3)
+
+    def tet_arrays(self):
+        n = lib().orc_tet_count(self.h)
+        IB, vol, lm = np.zeros(9 * n), np.zeros(n), np.zeros(2)
+        lib().orc_tet_arrays(self.h, _dp(IB), _dp(vol), _dp(lm))
+        return dict(IB=IB.reshape(-1, 3, 3), vol=vol, lam=lm[0], mu=lm[1])
+
+    def tet_hess_local(self, x, t, w=1.0, project=True):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        H = np.zeros(144)
+        lib().orc_tet_hess_local(self.h, _dp(x), int(t), float(w), int(project), _dp(H))
+        return H.reshape(12, 12)
+
+    def inversion_free_stepsize(self, x, p, alpha=1.0):
+        """Compute_Inversion_Free_StepSize (IPC.h:1785-1877)"""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        p = np.ascontiguousarray(p, dtype=np.float64).ravel()
+        return lib().orc_inversion_free_stepsize(self.h, _dp(x), _dp(p), float(alpha))
+
     def set_strain_limit(self, kappa, s, sHat):
         """the step's kappa_s, s, sHat (kappa_s[0] is then adapted in place by the steps)"""
         self._sl = [np.ascontiguousarray(v, dtype=np.float64) for v in (kappa, s, sHat)]
@@ -246,6 +304,10 @@ class OracleScene:
 
     def kappa_s(self):
         return lib().orc_get_kappa_s(self.h)
+
+    def set_static_solve(self, on):
+        """the step's staticSolve argument"""
+        lib().orc_set_static_solve(self.h, 1 if on else 0)
 
     def set_project_pd(self, on):
         """False: Hessians are assembled from unprojected local matrices (finite-difference pins)"""

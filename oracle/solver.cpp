@@ -49,7 +49,7 @@ void assemble(const Scene& S, const std::vector<Block>& blocks, CSR& A, const st
         for (long j = start[u]; j < start[u + 1]; ++j) {
             long id = key[j].second;
             if (id >= 0) for (int k = 0; k < 9; ++k) s[k] += blocks[id].b[k];
-            else { double m = S.mass[-1 - id]; s[0] += m; s[4] += m; s[8] += m; }
+            else if (!S.P.staticSolve) { double m = S.mass[-1 - id]; s[0] += m; s[4] += m; s[8] += m; }   // no mass in a static solve
         }
         int r = (int)(key[start[u]].first / S.nV), c = (int)(key[start[u]].first % S.nV);
         brow[u] = r; bcol[u] = c;

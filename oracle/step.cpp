@@ -21,6 +21,7 @@
 #include <set>
 #include <chrono>
 #include <deque>
+#include <cmath>
 #include <cstdio>
 
 namespace orc {
@@ -40,29 +41,37 @@ void scene_init(Scene& S) {
     }
     S.tri = kept;
     S.nT = (int)S.tri.size() / 3;
+    // contact triangles: the tets' surface first, then the shell (IMPLICIT_EULER.h:235-241)
+    std::vector<int> primTri;
+    tet_surface(S, primTri);
+    primTri.insert(primTri.end(), S.tri.begin(), S.tri.end());
     if (S.x.empty()) S.x = S.X0;
     if (S.v.empty()) S.v.assign(3 * S.nV, 0.0);
     S.isDBC.assign(S.nV, 0);
     for (int d : S.dbcIdx) S.isDBC[d] = 1;
     // edge2tri and surface primitives, MESHIO.h:728-766
     std::map<std::pair<int, int>, int> edge2tri;
-    std::set<std::pair<int, int>> edgeSet;
+    std::set<std::pair<int, int>> edgeSet, primEdgeSet;
     std::vector<char> isBN(S.nV, 0);
     for (int t = 0; t < S.nT; ++t) {
         const int* v = &S.tri[3 * t];
         edge2tri[{v[0], v[1]}] = t; edge2tri[{v[1], v[2]}] = t; edge2tri[{v[2], v[0]}] = t;
     }
-    for (int t = 0; t < S.nT; ++t) {
-        const int* v = &S.tri[3 * t];
-        if (!edgeSet.count({v[1], v[0]})) edgeSet.insert({v[0], v[1]});
-        if (!edgeSet.count({v[2], v[1]})) edgeSet.insert({v[1], v[2]});
-        if (!edgeSet.count({v[0], v[2]})) edgeSet.insert({v[2], v[0]});
+    auto add_edges = [](std::set<std::pair<int, int>>& es, const int* v) {
+        if (!es.count({v[1], v[0]})) es.insert({v[0], v[1]});
+        if (!es.count({v[2], v[1]})) es.insert({v[1], v[2]});
+        if (!es.count({v[0], v[2]})) es.insert({v[2], v[0]});
+    };
+    for (int t = 0; t < S.nT; ++t) add_edges(edgeSet, &S.tri[3 * t]);        // hinges: the shell's edges
+    for (size_t t = 0; t < primTri.size() / 3; ++t) {                        // primitives: shell + tet surface
+        const int* v = &primTri[3 * t];
+        add_edges(primEdgeSet, v);
         isBN[v[0]] = isBN[v[1]] = isBN[v[2]] = 1;
     }
     S.bNode.clear(); S.bEdge.clear();
     for (int i = 0; i < S.nV; ++i) if (isBN[i]) S.bNode.push_back(i);
-    for (auto& e : edgeSet) { S.bEdge.push_back(e.first); S.bEdge.push_back(e.second); }
-    S.bTri = S.tri;
+    for (auto& e : primEdgeSet) { S.bEdge.push_back(e.first); S.bEdge.push_back(e.second); }
+    S.bTri = primTri;
     // rest first fundamental form (inverted) and hinges, DISCRETE_SHELL.h:132-213
     S.Binv.resize(3 * S.nT);
     for (int t = 0; t < S.nT; ++t) {
@@ -108,7 +117,9 @@ void scene_init(Scene& S) {
     S.lam = P.E * P.nu / (1.0 - P.nu * P.nu);
     S.mu = P.E / (2.0 * (1.0 + P.nu));
     S.kBend = P.bendMult * (P.E * std::pow(P.thick, 3) / (24 * (1.0 - P.nu * P.nu)));
-    // rods and particles (initialize_rod / initialize_particle run before initialize_OIPC)
+    // tets (initialize_added_objects), rods and particles (initialize_rod /
+    // initialize_particle) run before initialize_OIPC
+    if (!tet_setup(S)) { S.setupError = "a rest tet is inverted or degenerate"; }
     codim_init(S);
     strain_limit_setup(S);
     fiber_setup(S);
@@ -167,6 +178,11 @@ static double inc_potential(const Scene& S, const double* x, const double* xt, c
                             double kappa, bool withContact, int* bad, const double* xn = nullptr,
                             const FricSet* F = nullptr, bool* invalid = nullptr, double dbcStiff = 0) {
     double E = 0;
+    if (!S.tet.empty() && !tets_valid(S, x)) {   // an inverted tet, INC_POTENTIAL.h:98-107
+        if (invalid) *invalid = true;
+        if (bad) *bad = 0;
+        return INFINITY;
+    }
     bool ok = strain_limit_energy(S, x, &E);
     if (fiber_active(S)) ok = fiber_energy(S, x, &E) && ok;   // Check_Fiber_Feasibility, INC_POTENTIAL.h:81-88
     if (invalid) *invalid = !ok;
@@ -184,9 +200,12 @@ int advance_one_step(Scene& S, StepStats& st) {
     double t0 = now();
     st = StepStats();
     std::vector<double>& x = S.x;
-    std::vector<double> xn = x, xt(n);
-    // x_tilde = x + h v + h^2 M^{-1} b, IMPLICIT_EULER.h:195-217 (b = m g)
-    for (int i = 0; i < S.nV; ++i)
+    std::vector<double> xn = x, xt(n, 0.0);
+    const bool stat = P.staticSolve != 0;
+    if (stat && P.collision && (P.fricMu > 0 || friction_by_component(S))) return -7;   // no previous state for friction
+    const double hres = stat ? 1.0 : h;   // residual scale, IMPLICIT_EULER.h:662-665
+    // x_tilde = x + h v + h^2 M^{-1} b, IMPLICIT_EULER.h:195-217 (b = m g); unused in a static solve
+    for (int i = 0; i < S.nV && !stat; ++i)
         for (int k = 0; k < 3; ++k) {
             double a = S.mass[i] > 0 ? P.grav[k] : 0.0;
             xt[3 * i + k] = x[3 * i + k] + h * S.v[3 * i + k] + h * h * a;
@@ -206,7 +225,7 @@ int advance_one_step(Scene& S, StepStats& st) {
     // a Dirichlet motion that would break a strain limit or the fibre limits is cut back
     // by halving until the moved nodes leave every triangle feasible, IMPLICIT_EULER.h:
     // 336-376 (only dbcAlpha survives: the nodes return to x_n below)
-    if ((P.slKappa[0] > 0 || P.fiberK[0] > 0) && !S.dbcIdx.empty()) {
+    if ((P.slKappa[0] > 0 || P.fiberK[0] > 0 || !S.tet.empty()) && !S.dbcIdx.empty()) {
         for (int it = 0; it < 64; ++it) {
             for (size_t k = 0; k < S.dbcIdx.size(); ++k)
                 for (int d = 0; d < 3; ++d) {
@@ -216,6 +235,7 @@ int advance_one_step(Scene& S, StepStats& st) {
             double Et = 0;
             bool ok = strain_limit_energy(S, x.data(), &Et);
             if (fiber_active(S)) ok = fiber_energy(S, x.data(), &Et) && ok;
+            if (!S.tet.empty()) ok = tets_valid(S, x.data()) && ok;   // IMPLICIT_EULER.h:364-374
             if (ok) break;
             dbcAlpha /= 2;
         }
@@ -259,7 +279,8 @@ int advance_one_step(Scene& S, StepStats& st) {
         strain_limit_grad(S, x.data(), g.data());
         if (fiber_active(S)) fiber_grad(S, x.data(), g.data());
         for (int i = 0; i < S.nV; ++i)
-            for (int k = 0; k < 3; ++k) g[3 * i + k] += S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
+            for (int k = 0; k < 3; ++k)
+                g[3 * i + k] += stat ? (S.mass[i] > 0 ? -S.mass[i] * P.grav[k] : 0.0) : S.mass[i] * (x[3 * i + k] - xt[3 * i + k]);
         if (P.collision) barrier_grad(S, x.data(), C, S.kappa, g.data());
         if (fric) friction_grad(S, x.data(), xn.data(), F, g.data());
         if (dbcStiff) {   // Shell/DIRICHLET.h:50-70; only fixed nodes projected
@@ -303,13 +324,14 @@ int advance_one_step(Scene& S, StepStats& st) {
         // line search, IMPLICIT_EULER.h:9-149
         xprev = x;
         double alpha = 1.0;
-        // strain-limit feasibility first, IMPLICIT_EULER.h:74-92
-        if (P.slKappa[0] > 0 || P.fiberK[0] > 0) {
+        // feasibility first (strain limits, fibre limits, uninverted tets), IMPLICIT_EULER.h:74-92
+        if (P.slKappa[0] > 0 || P.fiberK[0] > 0 || !S.tet.empty()) {
             for (;;) {
                 for (int i = 0; i < n; ++i) x[i] = xprev[i] + alpha * sol[i];
                 double Et = 0;
                 bool ok = strain_limit_energy(S, x.data(), &Et);
                 if (fiber_active(S)) ok = fiber_energy(S, x.data(), &Et) && ok;
+                if (!S.tet.empty()) ok = tets_valid(S, x.data()) && ok;
                 if (ok) break;
                 alpha /= 2;
             }
@@ -384,7 +406,7 @@ int advance_one_step(Scene& S, StepStats& st) {
         // stopping rule, IMPLICIT_EULER.h:647-710, 866
         double s = 0;
         for (int i = 0; i < n; ++i) s += sol[i] * sol[i];
-        L2Norm = std::sqrt(s / (S.nV - (int)S.dbcIdx.size())) / h;
+        L2Norm = std::sqrt(s / (S.nV - (int)S.dbcIdx.size())) / hres;
         PNIter++;
         resRecord.push_back(L2Norm);
         if (resRecord.size() > 3) resRecord.pop_front();
@@ -459,12 +481,13 @@ int advance_one_step(Scene& S, StepStats& st) {
                 if (rc != 0) return -5;   // IMPLICIT_EULER.h:826-831 exits on a non-SPD matrix
                 double s2 = 0;
                 for (int i = 0; i < n; ++i) s2 += sol[i] * sol[i];
-                L2Norm = std::sqrt(s2 / (S.nV - (int)S.dbcIdx.size())) / h;
+                L2Norm = std::sqrt(s2 / (S.nV - (int)S.dbcIdx.size())) / hres;
                 if (L2Norm > P.PNTol) Eprev = inc_potential(S, x.data(), xt.data(), C, S.kappa, P.collision, &bad, xn.data(), Fp, nullptr, dbcStiff);
             }
         }
     } while (resRecord.size() < 3 || L2Norm > P.PNTol);
-    for (int i = 0; i < n; ++i) S.v[i] = (x[i] - xn[i]) / h;
+    if (!stat)   // velocities stay as they are after a static solve, IMPLICIT_EULER.h:893-899
+        for (int i = 0; i < n; ++i) S.v[i] = (x[i] - xn[i]) / h;
     st.newtonIters = PNIter;
     st.nContacts = (int)C.size();
     st.kappa = S.kappa;

@@ -118,7 +118,7 @@ bool strain_limit_tri(const Scene& S, const double* x, int t, double* E, double*
     SLTri T;
     sl_svd(S, x, t, T);
     if (sigOut) { sigOut[0] = T.sig[0]; sigOut[1] = T.sig[1]; }
-    const double w = S.P.dt * S.P.dt * S.vol[t];
+    const double w = shell_h2(S) * S.vol[t];
     double f[2], g[2], hh[2];
     for (int i = 0; i < 2; ++i)
         if (!sl_phi(S, T.sig[i], &f[i], &g[i], &hh[i])) return false;
@@ -269,7 +269,7 @@ bool fiber_tri(const Scene& S, const double* x, int t, double* E, double* g9, do
     const double L0 = P.fiberLimit[0], L1 = P.fiberLimit[1], L2 = P.fiberLimit[2];
     if (std::fabs(E00) >= L0 || std::fabs(E11) >= L1 || std::fabs(E01) >= L2) return false;
     const double* k = P.fiberK;
-    const double w = P.dt * P.dt * S.vol[t];
+    const double w = shell_h2(S) * S.vol[t];
     const double m00 = L0 * L0, m11 = L1 * L1, m01 = L0 * L1, m2 = L2 * L2;
     const double s00 = E00 * E00, s11 = E11 * E11, sx = E00 * E11, s2 = E01 * E01;
     if (E) *E = w * (k[0] / 2 * eta0(s00, m00) + k[1] * eta0(sx, m01) + k[2] / 2 * eta0(s11, m11) + k[3] * eta0(s2, m2));

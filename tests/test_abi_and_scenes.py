@@ -93,3 +93,15 @@ def test_feature_arguments_rejected_without_gpu():
             for i in range(3):                        # a zero limit with the model on
                 desc.fiber_limit[i] = value[1][i]
         assert lib.cipc_create(ctypes.byref(desc), ctypes.byref(h)) == -1, field
+
+
+def test_friction_component_table_size_checked():
+    """set_friction_components rejects a coefficient table that is not n_components^2
+    before anything reaches the library"""
+    import pytest
+
+    class NoLib:
+        h = None
+        lib = None
+    with pytest.raises(ValueError):
+        cipc.Simulator.set_friction_components(NoLib(), [10, 20], [0.1, 0.2, 0.3])

@@ -11,7 +11,7 @@ import pytest
 import oracle as O
 from bars import trajectory_bar
 from cipc import Simulator
-from test_oracle_dirichlet import pinned_sheet, _dbc_disp
+from test_oracle_dirichlet import pinned_sheet, _dbc_disp, strain_limited_translation
 
 pytestmark = pytest.mark.gpu
 
@@ -86,6 +86,26 @@ def test_blocked_motion_penalty(require_gpu):
 
     x = _run(sd, (4, 7, (0.0, -0.3, 0.0)), 3, check)
     assert x.reshape(-1, 3)[: sd.n_cloth, 1].min() < -1e-3
+
+
+def test_motion_breaking_strain_limit(require_gpu):
+    """the corners translated 3 mm per step with the 1.01 strain limit on: the Dirichlet
+    displacement is halved until the moved corners leave every triangle feasible
+    (IMPLICIT_EULER.h:336-376), the penalty then brings them within 1% of their targets"""
+    sd = strain_limited_translation()
+    probe = O.OracleScene(sd)
+
+    def check(gpu, orc, x0, x, tgt):
+        X, X0 = x.reshape(-1, 3), x0.reshape(-1, 3)
+        moved = X0.copy()
+        moved[sd.dbc_idx] = tgt
+        assert not probe.strain_energy_grad(moved.ravel())[0]
+        gap = np.linalg.norm(tgt - X[sd.dbc_idx], axis=1)
+        full = np.linalg.norm(tgt - X0[sd.dbc_idx], axis=1)
+        assert np.sqrt((gap ** 2).sum() / (full ** 2).sum()) <= 0.01 + 1e-9
+        assert probe.strain_energy_grad(x)[0]
+
+    _run(sd, (0, 4, (0.3, 0.0, 0.0)), 3, check)
 
 
 def test_write_obj(require_gpu, tmp_path):

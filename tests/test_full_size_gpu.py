@@ -65,7 +65,7 @@ def test_c2_contact_state(require_gpu):
         ag, clg = gpu.ccd_stepsize(x, p, 1.0)
         print("ccd", scale, ag, ao, clg, flush=True)
         assert clg == clo
-        assert ao * (1 - 1e-9) <= ag <= ao * (1 + 1e-12)
+        assert ao * (1 - 1e-9) <= ag <= ao      # never later than the FP64 oracle
     gpu.close()
 
 

@@ -171,7 +171,9 @@ def test_learned_fallback_continues_from_learned_iterate(setup):
     gpu.set_preconditioner("learned", budget=30)
     try:
         sol, it, rr = gpu.linear_solve(x, C, kappa, b, tol=1e-8)
-        assert gpu.precond_info()["fallback"] == 1
+        info = gpu.precond_info()
+        assert info["fallback"] == 1
+        assert info["continued"] == 1          # block Jacobi went on from the learned iterate
     finally:
         gpu.set_preconditioner("jacobi")
     assert it > 30

@@ -133,3 +133,33 @@ def test_fiber_trajectory(require_gpu):
         assert np.abs(e).max() < 0.03
     assert np.abs(e).max() > 0.02          # the limit is active
     gpu.close()
+
+
+def test_kappa_s_doubling_matches_oracle(require_gpu):
+    """IMPLICIT_EULER.h:606-645 on both sides: with kappa_s = 1 the fourth step doubles it
+    three times; each step starts from the GPU's state and kappa_s on both sides; kappa_s
+    after every step equal, positions to the trajectory bar, the doubled value read back
+    through cipc_get_strain_limit and kept by later steps"""
+    sd = hammock(1.0)
+    gpu = Simulator(sd, pcg_tol=1e-10)
+    orc = O.OracleScene(sd)
+    probe = O.OracleScene(sd)
+    ks = []
+    for s in range(5):
+        x0, v0 = gpu.get_state()
+        kg = gpu.strain_limit()[0]
+        orc.set_state(x0, v0)
+        orc.set_strain_limit(kg, sd.params["s"], sd.params["sHat"])
+        st = gpu.step()
+        rc, so = orc.step()
+        assert rc > 0 and st["status"] == 0, (s, st, rc)
+        assert st["kappa_s"] == so["kappa_s"] == gpu.strain_limit()[0][0], (s, st["kappa_s"], so["kappa_s"])
+        assert st["kappa_s_doublings"] == so["kappa_s_doublings"]
+        ks.append(st["kappa_s"])
+        xg, _ = gpu.get_state()
+        xo, _ = orc.get_state()
+        disp = np.abs(xo - x0).max()
+        assert np.abs(xg - xo).max() <= trajectory_bar(sd, disp), (s, st["newton_iters"], so)
+        assert max_sigma(probe, xg, sd.nT) < 1.01
+    assert ks == [1.0, 1.0, 1.0, 8.0, 8.0], ks
+    gpu.close()
