@@ -182,6 +182,8 @@ def batch_run(args):
         if args.batch_form.endswith("_learned"):
             sim.precond_set_weights(precond.load())
             sim.set_preconditioner("learned", budget=args.learned_budget)
+        if args.autotune:
+            sim.autotune()          # each scene tunes its own launch shapes during the warm-up
         sims.append(sim)
     setup_s = time.perf_counter() - t0
     b = cipc.BatchStepper(sims, threads=args.batch_threads)
@@ -211,6 +213,8 @@ def batch_run(args):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
                        "max": round(float(lat.max()), 2)},
         "newton_iters_per_scene_step": round(newton / (n * args.steps), 2), "setup_s": round(setup_s, 1),
+        "autotune": bool(args.autotune),
+        "pcg_chunk_graph_replays": int(sum(s.solver_info()["graph_launches"] for s in sims)),
     }
 
 

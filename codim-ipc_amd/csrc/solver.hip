@@ -37,6 +37,7 @@
 #include "learn.hpp"
 #include <chrono>
 #include <cstdlib>
+#include <cstring>
 #include <hip/hip_ext.h>
 #include <cmath>
 
@@ -646,8 +647,10 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
         // number comes from the device counter, the buffer parities repeat every 16)
         const bool asGraph = c.useGraphs && !sampling && it > 0 && it % chunk == 0 && it + chunk - 1 <= maxIter;
         if (asGraph) {
+            uint64_t tolBits = 0;
+            std::memcpy(&tolBits, &tol2, sizeof(tolBits));
             std::vector<uintptr_t> key = {(uintptr_t)(void*)spmv_kernel<M, V>(sLanes, sUnroll, true), (uintptr_t)nS, (uintptr_t)rpxS,
-                                          (uintptr_t)maxIter, (uintptr_t)std::llround(std::log2(tol2) * 1e6),
+                                          (uintptr_t)maxIter, (uintptr_t)tolBits,
                                           (uintptr_t)E.rowptr.p, (uintptr_t)E.col.p, (uintptr_t)valE, (uintptr_t)E.nnzb,
                                           (uintptr_t)K.rowptr.p, (uintptr_t)K.col.p, (uintptr_t)valK, (uintptr_t)K.nnzb, (uintptr_t)nV,
                                           (uintptr_t)z, (uintptr_t)pbuf, (uintptr_t)q, (uintptr_t)partRZ, (uintptr_t)nRZ,
