@@ -138,6 +138,7 @@ PROTOTYPES = {
     "cipc_step": (ctypes.c_int, [P, ctypes.POINTER(StepStats)]),
     "cipc_constraint_set": (ctypes.c_int, [P, D, I32, ctypes.c_int32, I32]),
     "cipc_barrier_energy": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, D]),
+    "cipc_barrier_energy_noalloc": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, D]),
     "cipc_barrier_gradient": (ctypes.c_int, [P, D, I32, ctypes.c_int32, ctypes.c_double, D]),
     "cipc_elastic_energy": (ctypes.c_int, [P, D, D]),
     "cipc_elastic_gradient": (ctypes.c_int, [P, D, D]),

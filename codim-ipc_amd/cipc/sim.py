@@ -337,6 +337,15 @@ class Simulator:
         self._check(self.lib.cipc_barrier_energy(self.h, _d(x), _i(C), len(C), float(kappa), _d(e)))
         return e[0]
 
+    def barrier_energy_noalloc(self, x, C, kappa):
+        """Compute_Barrier (IPC.h:742-941) through the entry point that stages everything in
+        the library's static per-GPU scratch array (no device allocation; calls that overlap
+        on one GPU race)."""
+        x, C = self._x(x), self._cs(C)
+        e = np.zeros(1)
+        self._check(self.lib.cipc_barrier_energy_noalloc(self.h, _d(x), _i(C), len(C), float(kappa), _d(e)))
+        return e[0]
+
     def barrier_gradient(self, x, C, kappa):
         """Compute_Barrier_Gradient (IPC.h:943-1256)."""
         x, C = self._x(x), self._cs(C)

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -20,7 +21,36 @@ struct cipc_ctx {
     Ctx c;
     DevBuf<double> qx, qp, qb, qsol, qout;
     DevBuf<int> qC;
+    unsigned char* scratch = nullptr;   // device address of this device's `scratch` array
 };
+
+// ---- static scratch of the allocation-free entry point (cipc_barrier_energy_noalloc)
+// One array per GPU, alive until the process ends.  Its host-side name is only a stand-in:
+// device code reaches it through the address hipGetSymbolAddress returns for the current
+// device, resolved once per device when the first context is created there.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+unsigned char* g_scratchPtr[kMaxDevices] = {};
+
+unsigned char* resolve_scratch() {
+    int dev = 0;
+    CIPC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void* ptr = nullptr;
+        CIPC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratchPtr[dev] = static_cast<unsigned char*>(ptr);
+    }
+    return g_scratchPtr[dev];
+}
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
 
 namespace cipc {
 void ctx_setup(Ctx& c, const std::vector<double>& X, const std::vector<int>& tri);
@@ -132,6 +162,7 @@ int cipc_create(const cipc_scene_desc* d, cipc_ctx** out) {
         P.tetRho = d->tet_density; P.tetE = d->tet_youngs; P.tetNu = d->tet_poisson;
         c.hPart.assign(d->particles, d->particles + (size_t)d->n_particles);
         CIPC_HIP(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
+        ctx->scratch = resolve_scratch();
         CIPC_HIP(hipHostMalloc((void**)&c.hScal, 64 * sizeof(double), hipHostMallocDefault));
         CIPC_HIP(hipHostMalloc((void**)&c.hCnt, 64 * sizeof(uint32_t), hipHostMallocDefault));
         c.nV = d->n_vertices;
@@ -305,6 +336,48 @@ int cipc_barrier_energy(cipc_ctx* ctx, const double* x, const int32_t* cs, int32
         int bad = 0;
         *e = barrier_energy(c, ctx->qx.p, ctx->qC.p, n_cs, kappa, &bad);
         if (bad) { c.lastError = "non-positive constraint distance"; return CIPC_E_PENETRATION; }
+        return CIPC_OK;
+    });
+}
+
+int cipc_barrier_energy_noalloc(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, double* e) {
+    if (!ctx || !x || !e || n_cs < 0 || (n_cs && !cs)) return CIPC_E_INVALID;
+    return guarded(ctx, [&]() {
+        Ctx& c = ctx->c;
+        unsigned char* base = ctx->scratch;
+        if (!base) throw std::runtime_error("the scratch array of this context's device is not resolved");
+        // layout: x [3 nV doubles] | constraint rows [4 n_cs ints] | block partials | sum | flags
+        const size_t nx = 3 * (size_t)c.nV, nb = (size_t)nblocks(n_cs);
+        const size_t offC = align256(nx * sizeof(double));
+        const size_t offP = offC + align256(4 * (size_t)n_cs * sizeof(int32_t));
+        const size_t offO = offP + align256(std::max<size_t>(nb, 1) * sizeof(double));
+        const size_t offB = offO + 256, need = offB + 256;
+        if (need > kScratchBytes) {
+            c.lastError = "cipc_barrier_energy_noalloc: " + std::to_string(c.nV) + " vertices and " + std::to_string(n_cs) +
+                          " constraints need " + std::to_string(need) + " bytes of scratch, more than its " +
+                          std::to_string(kScratchBytes) + "; use cipc_barrier_energy";
+            return CIPC_E_CAPACITY;
+        }
+        const int nV = c.nV;
+        for (size_t i = 0; i < 4 * (size_t)n_cs; ++i)   // same rule as every other constraint input
+            if (cs[i] < -nV || cs[i] >= nV) {
+                c.lastError = "constraint " + std::to_string(i / 4) + " has an entry outside [-n_vertices, n_vertices)";
+                return CIPC_E_INVALID;
+            }
+        double* xd = reinterpret_cast<double*>(base);
+        int* Cd = reinterpret_cast<int*>(base + offC);
+        double* part = reinterpret_cast<double*>(base + offP);
+        double* out = reinterpret_cast<double*>(base + offO);
+        uint32_t* bad = reinterpret_cast<uint32_t*>(base + offB);
+        CIPC_HIP(hipMemcpyAsync(xd, x, nx * sizeof(double), hipMemcpyHostToDevice, c.s));
+        if (n_cs) CIPC_HIP(hipMemcpyAsync(Cd, cs, 4 * (size_t)n_cs * sizeof(int32_t), hipMemcpyHostToDevice, c.s));
+        barrier_energy_rows(c, xd, Cd, n_cs, kappa, part, out, bad);
+        CIPC_HIP(hipMemcpyAsync(c.hScal + 62, out, sizeof(double), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipMemcpyAsync(c.hCnt + 62, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, c.s));
+        CIPC_HIP(hipStreamSynchronize(c.s));
+        *e = c.hScal[62];
+        if (c.hCnt[62] & 2u) { c.lastError = "constraint entry outside the vertex range"; return CIPC_E_INVALID; }
+        if (c.hCnt[62]) { c.lastError = "non-positive constraint distance"; return CIPC_E_PENETRATION; }
         return CIPC_OK;
     });
 }

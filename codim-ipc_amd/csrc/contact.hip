@@ -857,6 +857,30 @@ double barrier_energy(Ctx& c, const double* x, const int* C, int nC, double kapp
     return c.hScal[1];
 }
 
+__global__ void k_clear_u32(uint32_t* __restrict__ a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0u;
+}
+// The same energy over int4 constraint rows (C[4 i + k], the reference's layout) with every
+// buffer supplied by the caller: per-block partials part[nblocks(nC)], the sum *out and
+// the validity bits *bad (bit 1: a distance at or below the offset, bit 2: an entry
+// outside the vertex range), all device memory.  *bad is cleared by a fill kernel.
+// Nothing is allocated and nothing is synchronised; the launches go to c.s.
+void barrier_energy_rows(Ctx& c, const double* x, const int* Crows, int nC, double kappa, double* part, double* out, uint32_t* bad) {
+    const double xi2 = c.P.xi * c.P.xi, shat = c.dHat2 + 2 * std::sqrt(c.dHat2) * c.P.xi;   // IPC.h:756-757
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, c.s, bad, 1);
+    CIPC_LAUNCH_CHECK();
+    if (nC == 0) {
+        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, c.s, reinterpret_cast<uint32_t*>(out), 2);   // +0.0
+        CIPC_LAUNCH_CHECK();
+        return;
+    }
+    const int nb = nblocks(nC);
+    hipLaunchKernelGGL(k_barrier_energy<true>, dim3(nb), dim3(kBlock), 0, c.s, x, c.X0.p, Crows, nC, kappa, xi2, shat, part, bad, c.nV);
+    CIPC_LAUNCH_CHECK();
+    sum_partials(part, nb, out, c.s);
+}
+
 __global__ void k_vert_bounds(const uint64_t* __restrict__ key, long n, int nV, uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

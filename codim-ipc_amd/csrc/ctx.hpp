@@ -287,6 +287,8 @@ void minmax6_final(const double* part, int nb, double* out6, hipStream_t s);   /
 void ctx_init_device(Ctx& c);                                      // upload + derived data
 void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut);
 double barrier_energy(Ctx& c, const double* x, const int* C, int nC, double kappa, int* bad);
+// int4-row constraints, caller-supplied device partials / sum / flags; allocates nothing
+void barrier_energy_rows(Ctx& c, const double* x, const int* Crows, int nC, double kappa, double* part, double* out, uint32_t* bad);
 void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kappa, double* g);
 // fills Acon; with fricXn the lagged friction blocks (displacement x - fricXn) are added
 void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa, const double* fricXn = nullptr);

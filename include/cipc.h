@@ -156,6 +156,16 @@ int cipc_batch_step(cipc_ctx* const* ctxs, int32_t n, int32_t threads, cipc_step
 int cipc_constraint_set(cipc_ctx* ctx, const double* x, int32_t* out, int32_t cap, int32_t* n);
 /* Compute_Barrier (IPC.h:742-941) */
 int cipc_barrier_energy(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, double* e);
+/* No counterpart in the reference: the same result as cipc_barrier_energy for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  x, the constraint rows (int4 rows, reference
+ * encoding), the per-block partials, the sum and the validity flags are staged in a
+ * static 32 MiB device array per GPU; the call makes no hipMalloc / hipFree.  Inputs
+ * that need more than 32 MiB (24 n_vertices + 16 n_cs + 8 ceil(n_cs / 256) bytes, each
+ * rounded up to 256, + 512) return CIPC_E_CAPACITY.  Every context on one GPU shares that
+ * GPU's array and the last call's contents stay in it: calls that overlap in time on one
+ * GPU (two host threads, two contexts) race and must be serialised by the caller. */
+int cipc_barrier_energy_noalloc(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, double* e);
 /* Compute_Barrier_Gradient (IPC.h:943-1256); g overwritten (3 n_vertices) */
 int cipc_barrier_gradient(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t n_cs, double kappa, double* g);
 /* Compute_Membrane_Energy + Compute_Bending_Energy (MEMBRANE.h:7-50, BENDING.h:52-75): out[0..1] */
