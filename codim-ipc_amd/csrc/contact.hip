@@ -141,15 +141,29 @@ __device__ __forceinline__ uint64_t dup_key(int c0, int c1, int c2, int nV) {
     uint64_t a = (uint64_t)(nV - 1 - (-c0 - 1)), b = (uint64_t)c1, d = (uint64_t)(c2 + 1);
     return (a * (uint64_t)nV + b) * (uint64_t)(nV + 1) + d;
 }
+// Output slots are reserved once per wavefront: the lanes emitting at this point (whatever
+// the divergence) are found by a ballot, the lowest of them adds their count to the
+// counter, and each lane takes base + its rank among them (mbcnt).  The slot order is
+// arbitrary; the set is sorted by key afterwards, so the result is deterministic.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* cnt) {
+    const uint64_t act = __ballot(1);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(act));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    return base + rank;
+}
 __device__ __forceinline__ void emit_kept(const CSOut& o, uint64_t key, int c0, int c1, int c2, int c3) {
-    uint32_t i = atomicAdd(o.cnt, 1u);
+    const uint32_t i = wave_reserve(o.cnt);
     if (i < o.cap) {
         o.key[i] = key;
         o.c4[4 * (long)i] = c0; o.c4[4 * (long)i + 1] = c1; o.c4[4 * (long)i + 2] = c2; o.c4[4 * (long)i + 3] = c3;
     }
 }
 __device__ __forceinline__ void emit_dup(const CSOut& o, uint64_t key) {
-    uint32_t i = atomicAdd(o.dcnt, 1u);
+    const uint32_t i = wave_reserve(o.dcnt);
     if (i < o.dcap) o.dkey[i] = key;
 }
 

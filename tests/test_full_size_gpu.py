@@ -94,7 +94,8 @@ def test_empty_constraint_inputs(require_gpu):
 @pytest.mark.parametrize("layout", ["crossing", "parallel"])
 def test_ccd_edge_edge_closed_form_gpu(require_gpu, layout):
     """Same cases as the oracle's closed-form edge-edge test: exact time of impact d0 / v,
-    the GPU step within [0.85, 1] of it and equal to the oracle's to rounding."""
+    the GPU step within [0.85, 1] of it, never later than the oracle's and equal to it to
+    rounding."""
     d0 = 0.01
     A = [[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, -1.0, 0.0]]
     if layout == "crossing":
@@ -116,5 +117,5 @@ def test_ccd_edge_edge_closed_form_gpu(require_gpu, layout):
         toi = d0 / v
         assert clg == clo
         assert 0.85 * toi <= ag <= toi
-        assert abs(ag - ao) <= 1e-12 * ao
+        assert ao * (1 - 1e-12) <= ag <= ao      # never later than the FP64 oracle
     gpu.close()
