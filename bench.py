@@ -17,15 +17,16 @@ Newton iterations of every timed step are reported.
 
 Solver forms, each from the same hand-over state:
   fp64           block-Jacobi PCG in FP64 (the plain FP64 form);
-  fp32_learned   the mixed-precision learned form (headline `value`): FP32 Hessian blocks,
-                 FP32 Krylov vectors and SpMV accumulation inside FP64 iterative
-                 refinement on the true residual, preconditioned by the learned factor
-                 (weights trained by cipc.precond, codim-ipc_amd/cipc/weights/precond.npz),
-                 falling back to block Jacobi when its iteration budget runs out;
+  mixed_learned  the mixed-precision learned form (headline `value`): FP32 Hessian blocks
+                 (half the SpMV's block bytes), FP64 Krylov vectors and SpMV accumulation,
+                 inside FP64 iterative refinement on the true residual, preconditioned by
+                 the learned factor (BF16 network and factor; weights trained by
+                 cipc.precond, codim-ipc_amd/cipc/weights/precond.npz), falling back to
+                 block Jacobi when its iteration budget runs out;
+  fp32_learned   the same with FP32 Krylov vectors and FP32 SpMV accumulation as well;
 and with --precision all also
-  mixed          FP32 Hessian blocks, FP64 Krylov vectors / SpMV accumulation;
-  fp32           FP32 blocks, vectors and accumulation, block-Jacobi;
-  mixed_learned  the mixed form with the learned factor.
+  mixed          FP32 Hessian blocks, FP64 Krylov vectors / SpMV accumulation, block-Jacobi;
+  fp32           FP32 blocks, vectors and accumulation, block-Jacobi.
 Every form meets the same FP64 relative residual 1e-5 in every Newton solve.
 
 The JSON line carries the roofline of the dominant kernel of the headline form (the PCG
@@ -78,15 +79,23 @@ def measured_traffic(kernel, workload, window):
         t = json.load(f)
     w = t.get("window")
     inside = isinstance(w, list) and len(w) == 2 and window[0] <= w[0] <= w[1] <= window[1]
-    if not kernel.startswith(t["kernel"] + " ") or t["workload"] != workload or not inside:
+    # the summary names the kernel family (block / vector types); every launch shape of it
+    # reads and writes the same arrays
+    if not kernel.startswith(t["kernel"]) or t["workload"] != workload or not inside:
         return None, None
     return int(t["traffic_bytes_per_launch"]), t["summary"]
 
 
 def fast_forward(args, sd):
     """the untimed free fall shared by every form: (x, v, kappa) after args.fast_forward
-    FP64 steps from the initial state"""
+    FP64 steps from the initial state, or the state --load-state names (written by an
+    earlier run's --save-state; its step number is stored with it)"""
     import cipc
+    import numpy as np
+    if args.load_state:
+        with np.load(args.load_state, allow_pickle=False) as z:
+            args.fast_forward = int(z["step"])
+            return z["x"].copy(), z["v"].copy(), float(z["kappa"]), []
     x, v = sd.X.ravel().copy(), None
     kappa = None
     if args.fast_forward <= 0:
@@ -128,6 +137,8 @@ def gpu_run(args, sd, form, state):
                                                                   wstats[-1]["newton_iters"]), file=sys.stderr, flush=True)
     tw = time.perf_counter() - tw
     start_state = sim.get_state() + (sim.scalars()["kappa"],)
+    if args.save_state:
+        np.savez(args.save_state, x=start_state[0], v=start_state[1], kappa=start_state[2], step=args.fast_forward + args.warmup)
     sim.set_profiling(True)
     stats = []
     info0 = sim.precond_info() if learned else None
@@ -301,6 +312,10 @@ def main():
                          "c3: the c1 scene with 200,000 rod segments and 100,000 particles; c5: batch of independent scenes")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes instead of tuning them during the warm-up")
+    ap.add_argument("--save-state", default=None,
+                    help="write the timed window's start state (after the warm-up; x, v, kappa, step) to this .npz")
+    ap.add_argument("--load-state", default=None,
+                    help="start from a state written by --save-state instead of the free fall (profiling runs)")
     ap.add_argument("--no-graphs", dest="graphs", action="store_false",
                     help="launch every PCG iteration's kernels one by one instead of replaying captured 16-iteration "
                          "graphs (same kernels and results; for tracing tools that do not follow graph launches)")
@@ -308,8 +323,9 @@ def main():
     ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--batch-form", default="fp32_learned", choices=["fp64", "fp32", "fp32_learned"])
     ap.add_argument("--precision", default="headline", choices=["headline", "all", "fp64", "mixed", "fp32", "mixed_learned", "fp32_learned"],
-                    help="headline: fp64 and fp32_learned; all: the five forms")
-    ap.add_argument("--learned-budget", type=int, default=2500)
+                    help="headline: fp64, mixed_learned and fp32_learned; all: the five forms")
+    ap.add_argument("--learned-budget", type=int, default=6000,
+                    help="PCG iterations after which a learned solve falls back to block Jacobi (continuing from its iterate)")
     ap.add_argument("--learned-reuse", type=int, default=None,
                     help="Newton solves per network evaluation (default 1: every solve)")
     ap.add_argument("--pcg-tol", type=float, default=1e-5)
@@ -337,7 +353,7 @@ def main():
 
     from cipc import precond
     have_w = os.path.exists(precond.WEIGHTS)
-    modes = {"headline": ["fp64", "fp32_learned"], "all": ["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned"]}.get(
+    modes = {"headline": ["fp64", "mixed_learned", "fp32_learned"], "all": ["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned"]}.get(
         args.precision, [args.precision])
     if not have_w:
         modes = [m for m in modes if not m.endswith("_learned")]
@@ -353,7 +369,8 @@ def main():
         for m, r in runs.items():
             r["vs_fp64_max_dx_rel"] = float(np.abs(r["_x"] - x64).max()) / max(disp, 1e-300)
             r["vs_fp64_energy_rel"] = abs(r["_energy"] - e64) / max(abs(e64), 1e-300)
-    head_form = "fp32_learned" if "fp32_learned" in runs else ([m for m in modes if m != "fp64"] or modes)[-1]
+    # the metric's "mixed+learned" form; the FP32-vector variant is reported beside it
+    head_form = next((m for m in ("mixed_learned", "fp32_learned") if m in runs), ([m for m in modes if m != "fp64"] or modes)[-1])
     head = runs[head_form]
     cpu = None
     if not args.no_cpu_baseline:
@@ -386,6 +403,7 @@ def main():
         "config": {"workload": sd.name, "vertices": sd.nV, "triangles": sd.nT, "cloth_vertices": sd.n_cloth,
                    "rod_segments": 0 if sd.rods is None else len(sd.rods), "particles": 0 if sd.particles is None else len(sd.particles),
                    "untimed_free_fall_steps": args.fast_forward, "timed_window_steps": window,
+                   "start_state": os.path.basename(args.load_state) if args.load_state else "initial",
                    "free_fall_contacts_last": int(ff[-1]["contacts"]) if ff else None,
                    "dt": sd.params["dt"], "dhat": sd.params["dHat"],
                    "pcg_tol": args.pcg_tol, "friction_mu": args.mu, "strain_limit": args.strain_limit},

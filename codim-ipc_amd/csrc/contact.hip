@@ -903,11 +903,14 @@ __global__ void k_vert_bounds(const uint64_t* __restrict__ key, long n, int nV, 
     if (i == 0 || key[i - 1] != k) start[k] = (uint32_t)i;
     if (i == n - 1 || key[i + 1] != k) end[k] = (uint32_t)(i + 1);
 }
-__global__ void k_grad_keys(const int* __restrict__ gv, long n, int nV, uint64_t* __restrict__ key) {
+// slots of vertices flagged in `skip` (rows the caller projects out) are not gathered: a
+// static obstacle's vertex (a ground corner) can sit in 10^5 stencils, which one thread
+// would otherwise sum serially
+__global__ void k_grad_keys(const int* __restrict__ gv, long n, int nV, const uint8_t* __restrict__ skip, uint64_t* __restrict__ key) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int v = gv[i];
-    key[i] = v < 0 ? (uint64_t)nV : (uint64_t)v;
+    key[i] = (v < 0 || (skip && skip[v])) ? (uint64_t)nV : (uint64_t)v;
 }
 __global__ void k_grad_gather(const double* __restrict__ gl, long ns, const uint32_t* __restrict__ slot, const uint32_t* __restrict__ start,
                               const uint32_t* __restrict__ end, int nV, double* __restrict__ g) {
@@ -923,7 +926,7 @@ __global__ void k_grad_gather(const double* __restrict__ gl, long ns, const uint
 }
 
 // g += barrier gradient (deterministic: slots sorted by vertex, summed in slot order)
-void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kappa, double* g) {
+void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kappa, double* g, const uint8_t* skip) {
     if (nC == 0) return;
     const double xi2 = c.P.xi * c.P.xi, shat = c.dHat2 + 2 * std::sqrt(c.dHat2) * c.P.xi;
     long ns = 4l * nC;
@@ -931,15 +934,15 @@ void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kapp
     int* gv = c.conGVert.ensure(ns);
     hipLaunchKernelGGL(k_barrier_grad_local, dim3(nblocks(nC)), dim3(kBlock), 0, c.s, x, c.X0.p, C, nC, kappa, xi2, shat, gl, gv);
     CIPC_LAUNCH_CHECK();
-    scatter_local_gradient(c, gl, gv, ns, g);
+    scatter_local_gradient(c, gl, gv, ns, g, skip);
 }
 
 // g += local gradient slots (SoA [3][ns], vertex gv per slot, -1 unused), summed per
-// vertex in slot order (radix sort by vertex, stable)
-void scatter_local_gradient(Ctx& c, const double* gl, const int* gv, long ns, double* g) {
+// vertex in slot order (radix sort by vertex, stable); vertices flagged in skip keep g
+void scatter_local_gradient(Ctx& c, const double* gl, const int* gv, long ns, double* g, const uint8_t* skip) {
     uint64_t* key = c.k64a.ensure(ns);
     uint32_t* slot = c.u32c.ensure(ns);
-    hipLaunchKernelGGL(k_grad_keys, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, gv, ns, c.nV, key);
+    hipLaunchKernelGGL(k_grad_keys, dim3(nblocks(ns)), dim3(kBlock), 0, c.s, gv, ns, c.nV, skip, key);
     CIPC_LAUNCH_CHECK();
     fill_iota(slot, ns, c.s);
     radix_sort_pairs(key, slot, ns, bits_for((uint64_t)c.nV + 1), c.pw, c.s);

@@ -289,14 +289,15 @@ void constraint_set(Ctx& c, const double* x, DevBuf<int>& out, int& nOut);
 double barrier_energy(Ctx& c, const double* x, const int* C, int nC, double kappa, int* bad);
 // int4-row constraints, caller-supplied device partials / sum / flags; allocates nothing
 void barrier_energy_rows(Ctx& c, const double* x, const int* Crows, int nC, double kappa, double* part, double* out, uint32_t* bad);
-void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kappa, double* g);
+// g += barrier gradient; rows flagged in skip (projected out by the caller) are left alone
+void barrier_gradient(Ctx& c, const double* x, const int* C, int nC, double kappa, double* g, const uint8_t* skip = nullptr);
 // fills Acon; with fricXn the lagged friction blocks (displacement x - fricXn) are added
 void barrier_hessian(Ctx& c, const double* x, const int* C, int nC, double kappa, const double* fricXn = nullptr);
-void scatter_local_gradient(Ctx& c, const double* gl, const int* gv, long ns, double* g);
+void scatter_local_gradient(Ctx& c, const double* gl, const int* gv, long ns, double* g, const uint8_t* skip = nullptr);
 // lagged friction (friction.hip): basis at x for constraint set C into c.fC / c.fI / c.nF
 void friction_basis(Ctx& c, const double* x, const int* C, int nC, double kappa);
 double friction_energy(Ctx& c, const double* x, const double* xn);
-void friction_gradient(Ctx& c, const double* x, const double* xn, double* g);   // g +=
+void friction_gradient(Ctx& c, const double* x, const double* xn, double* g, const uint8_t* skip = nullptr);   // g +=
 void friction_hess_local(Ctx& c, const double* x, const double* xn, double* hl, long ns, long off);
 void friction_hess_keys(Ctx& c, uint64_t* key);
 // weight of the shell's membrane, bending, strain-limit and fibre terms: h^2, or 1 in a

@@ -312,7 +312,7 @@ double friction_energy(Ctx& c, const double* x, const double* xn) {
     return fric_mu(c) * c.hScal[48];
 }
 
-void friction_gradient(Ctx& c, const double* x, const double* xn, double* g) {
+void friction_gradient(Ctx& c, const double* x, const double* xn, double* g, const uint8_t* skip) {
     if (c.nF == 0) return;
     const long ns = 4l * c.nF;
     double* gl = c.fGLocal.ensure(3 * ns);
@@ -320,7 +320,7 @@ void friction_gradient(Ctx& c, const double* x, const double* xn, double* g) {
     hipLaunchKernelGGL(k_fric_grad_local, dim3(nblocks(c.nF)), dim3(kBlock), 0, c.s, x, xn, c.fC.p, c.fI.p, c.nF, fric_ev(c),
                        fric_mu(c), gl, gv);
     CIPC_LAUNCH_CHECK();
-    scatter_local_gradient(c, gl, gv, ns, g);
+    scatter_local_gradient(c, gl, gv, ns, g, skip);
 }
 
 void friction_hess_local(Ctx& c, const double* x, const double* xn, double* hl, long ns, long off) {

@@ -773,7 +773,9 @@ static int refine(Ctx& c, const M* vE, const M* vK, const V* dinv, const double*
     const double floorTol = sizeof(V) == sizeof(float) ? 1e-6 : 1e-12;
     // aim just below the target: the inner residual drifts from the true FP64 residual by
     // the rounding of the FP32 blocks, and pushing the inner solve further costs more
-    // iterations than the short second sweep it would save (measured on the 320k stack)
+    // iterations than the short second sweep it would save (measured on the 320k stack:
+    // the drift is additive -- the FP32 blocks' error times the solution -- so an inner
+    // target below it only over-solves the sweep)
     constexpr double kInnerMargin = 0.9;
     for (int k = 0; k < kMaxRefine; ++k) {
         const double inner = std::max(floorTol, std::min(0.5, kInnerMargin * tol * bnorm / rnorm));

@@ -647,8 +647,9 @@ int advance_one_step(Ctx& c) {
         // gradient (INC_POTENTIAL.h:154-275)
         ta = now_ms();
         elastic_gradient(c, c.x.p, c.g.p);
-        if (P.collision) barrier_gradient(c, c.x.p, c.C.p, c.nC, c.kappa, c.g.p);
-        if (c.fricOn) friction_gradient(c, c.x.p, c.xn.p, c.g.p);
+        // rows that k_rhs projects out need no barrier / friction terms
+        if (P.collision) barrier_gradient(c, c.x.p, c.C.p, c.nC, c.kappa, c.g.p, proj_mask(c));
+        if (c.fricOn) friction_gradient(c, c.x.p, c.xn.p, c.g.p, proj_mask(c));
         dbc_penalty_grad(c, c.x.p, c.g.p);   // IMPLICIT_EULER.h:467-475
         hipLaunchKernelGGL(k_rhs, dim3(nblocks(c.nV)), dim3(kBlock), 0, s, c.nV, c.g.p, proj_mask(c), c.rhs.p);
         CIPC_LAUNCH_CHECK();
