@@ -324,7 +324,7 @@ def main():
     ap.add_argument("--batch-form", default="fp32_learned", choices=["fp64", "fp32", "fp32_learned"])
     ap.add_argument("--precision", default="headline", choices=["headline", "all", "fp64", "mixed", "fp32", "mixed_learned", "fp32_learned"],
                     help="headline: fp64, mixed_learned and fp32_learned; all: the five forms")
-    ap.add_argument("--learned-budget", type=int, default=6000,
+    ap.add_argument("--learned-budget", type=int, default=2500,
                     help="PCG iterations after which a learned solve falls back to block Jacobi (continuing from its iterate)")
     ap.add_argument("--learned-reuse", type=int, default=None,
                     help="Newton solves per network evaluation (default 1: every solve)")

@@ -367,7 +367,7 @@ __device__ __forceinline__ void ee_ccd_candidate_box(const double* __restrict__ 
 __global__ void k_ccd_pt(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bNode, int nBN,
                          const int* __restrict__ tri, int nT, const uint8_t* __restrict__ dbc, const int* __restrict__ vlo,
                          const int* __restrict__ vhi, int nV, const int4* __restrict__ tbox, VoxGrid G,
-                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals,
+                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const int4* __restrict__ crec,
                          const int* __restrict__ lgList, const uint32_t* __restrict__ lgCount, double alpha0, double xi,
                          unsigned long long* __restrict__ gmin, uint32_t* __restrict__ zero) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -387,8 +387,8 @@ __global__ void k_ccd_pt(const double* __restrict__ x, const double* __restrict_
                 uint64_t cell = (uint64_t)ix + (uint64_t)G.edim[0] * ((uint64_t)iy + (uint64_t)G.edim[1] * (uint64_t)iz);
                 uint32_t st = start[cell], en = end[cell];
                 for (uint32_t k = st; k < en; ++k) {
-                    int t = (int)vals[k];
-                    const VBox T = ld_box(tbox, t);
+                    const VBox T = ld_box(crec, (int)k);   // the cell's records are contiguous
+                    const int t = T.b.z;
                     if (ix != max(qlo[0], ecell(T.a.x, G.f, G.edim[0])) || iy != max(qlo[1], ecell(T.a.y, G.f, G.edim[1])) ||
                         iz != max(qlo[2], ecell(T.a.z, G.f, G.edim[2])))
                         continue;
@@ -407,7 +407,7 @@ __global__ void k_ccd_pt(const double* __restrict__ x, const double* __restrict_
 // larger index plus every large edge; large edges take large partners of larger index
 __global__ void k_ccd_ee(const double* __restrict__ x, const double* __restrict__ p, const int* __restrict__ bEdge, int nBE,
                          const uint8_t* __restrict__ dbc, const int4* __restrict__ ebox, VoxGrid G,
-                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals,
+                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const int4* __restrict__ crec,
                          const uint8_t* __restrict__ lgFlag, const int* __restrict__ lgList, const uint32_t* __restrict__ lgCount,
                          double alpha0, double xi, unsigned long long* __restrict__ gmin, uint32_t* __restrict__ zero) {
     int eI = blockIdx.x * blockDim.x + threadIdx.x;
@@ -429,9 +429,9 @@ __global__ void k_ccd_ee(const double* __restrict__ x, const double* __restrict_
                     uint64_t cell = (uint64_t)ix + (uint64_t)G.edim[0] * ((uint64_t)iy + (uint64_t)G.edim[1] * (uint64_t)iz);
                     uint32_t st = start[cell], en = end[cell];
                     for (uint32_t k = st; k < en; ++k) {
-                        int eJ = (int)vals[k];
+                        const VBox B = ld_box(crec, (int)k);   // the cell's records are contiguous
+                        const int eJ = B.b.z;
                         if (eJ <= eI) continue;
-                        const VBox B = ld_box(ebox, eJ);
                         if (ix != max(qlo[0], ecell(B.a.x, G.f, G.edim[0])) || iy != max(qlo[1], ecell(B.a.y, G.f, G.edim[1])) ||
                             iz != max(qlo[2], ecell(B.a.z, G.f, G.edim[2])))
                             continue;
@@ -553,6 +553,19 @@ static int bits_for2(uint64_t n) {
     return b;
 }
 
+// primitive boxes in cell order: record k = box of the k-th (cell, primitive) entry, its
+// spare third word of the second half holding the primitive, so a query walks a cell's
+// candidates through contiguous memory instead of gathering one box per candidate
+__global__ void k_cell_records(const uint32_t* __restrict__ vals, long total, const int4* __restrict__ box, int4* __restrict__ rec) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const int v = (int)vals[k];
+    int4 hi = box[2l * v + 1];
+    hi.z = v;
+    rec[2 * k] = box[2l * v];
+    rec[2 * k + 1] = hi;
+}
+
 static long build_vox_cells(Ctx& c, const int* verts, int n, int nv, const int* vlo, const int* vhi, const VoxGrid& G, DevBuf<int>& plo,
                             DevBuf<int>& phi, DevBuf<int>& box, DevBuf<uint64_t>& keys, DevBuf<uint32_t>& vals, DevBuf<uint32_t>& start,
                             DevBuf<uint32_t>& end, DevBuf<uint8_t>& lgFlag, DevBuf<int>& lgList, uint32_t* lgCount) {
@@ -659,20 +672,30 @@ double ccd_stepsize(Ctx& c, const double* x, const double* p, double alpha, doub
     // point-triangle
     uint32_t* lgCountT = c.counters.p + 18;
     uint32_t* lgCountE = c.counters.p + 19;
-    build_vox_cells(c, c.btri.p, c.nBT, 3, vlo, vhi, G, c.ccdTlo, c.ccdThi, c.ccdTbox, c.k64a, c.u32c, c.cellStart, c.cellEnd, c.lgFlagT, c.lgListT,
-                    lgCountT);
+    const long totT = build_vox_cells(c, c.btri.p, c.nBT, 3, vlo, vhi, G, c.ccdTlo, c.ccdThi, c.ccdTbox, c.k64a, c.u32c, c.cellStart, c.cellEnd,
+                                      c.lgFlagT, c.lgListT, lgCountT);
+    int4* recT = (int4*)c.ccdTrec.ensure(8 * (size_t)std::max(totT, 1L));
+    if (totT > 0) {
+        hipLaunchKernelGGL(k_cell_records, dim3(nblocks(totT)), dim3(kBlock), 0, c.s, c.u32c.p, totT, (const int4*)c.ccdTbox.p, recT);
+        CIPC_LAUNCH_CHECK();
+    }
     if (c.nBN && c.nBT) {
         hipLaunchKernelGGL(k_ccd_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, p, c.bNode.p, c.nBN, c.btri.p, c.nBT, c.dbc.p, vlo, vhi, c.nV,
-                           (const int4*)c.ccdTbox.p, G, c.cellStart.p, c.cellEnd.p, c.u32c.p, c.lgListT.p, lgCountT, alpha, xi, gmin,
+                           (const int4*)c.ccdTbox.p, G, c.cellStart.p, c.cellEnd.p, (const int4*)recT, c.lgListT.p, lgCountT, alpha, xi, gmin,
                            c.counters.p + 13);
         CIPC_LAUNCH_CHECK();
     }
     // edge-edge
-    build_vox_cells(c, c.bEdge.p, c.nBE, 2, vlo, vhi, G, c.ccdElo, c.ccdEhi, c.ccdEbox, c.k64b, c.u32d, c.ecellStart, c.ecellEnd, c.lgFlagE,
-                    c.lgListE, lgCountE);
+    const long totE = build_vox_cells(c, c.bEdge.p, c.nBE, 2, vlo, vhi, G, c.ccdElo, c.ccdEhi, c.ccdEbox, c.k64b, c.u32d, c.ecellStart,
+                                      c.ecellEnd, c.lgFlagE, c.lgListE, lgCountE);
+    int4* recE = (int4*)c.ccdErec.ensure(8 * (size_t)std::max(totE, 1L));
+    if (totE > 0) {
+        hipLaunchKernelGGL(k_cell_records, dim3(nblocks(totE)), dim3(kBlock), 0, c.s, c.u32d.p, totE, (const int4*)c.ccdEbox.p, recE);
+        CIPC_LAUNCH_CHECK();
+    }
     if (c.nBE) {
         hipLaunchKernelGGL(k_ccd_ee, dim3(nblocks(c.nBE)), dim3(kBlock), 0, c.s, x, p, c.bEdge.p, c.nBE, c.dbc.p, (const int4*)c.ccdEbox.p, G,
-                           c.ecellStart.p, c.ecellEnd.p, c.u32d.p, c.lgFlagE.p, c.lgListE.p, lgCountE, alpha, xi, gmin,
+                           c.ecellStart.p, c.ecellEnd.p, (const int4*)recE, c.lgFlagE.p, c.lgListE.p, lgCountE, alpha, xi, gmin,
                            c.counters.p + 13);
         CIPC_LAUNCH_CHECK();
     }

@@ -120,6 +120,7 @@ struct Ctx {
     DevBuf<int> lgListP;
     DevBuf<int> ccdPlo, ccdPhi;
     DevBuf<int> ccdTbox, ccdEbox, ccdPbox;   // the same voxel boxes as 32-byte records (lo xyz, hi xyz, 2 pad)
+    DevBuf<int> ccdTrec, ccdErec;            // those records in cell order (third pad word = primitive)
 
     // device state
     DevBuf<double> x, xn, xt, v, xprev, X0, mass, p, g, rhs;

@@ -44,7 +44,7 @@
 namespace cipc {
 
 // solver scalars (sc[]) and flags
-enum { S_RZ0 = 0, S_RZ1, S_BB, S_RR };
+enum { S_RZ0 = 0, S_RZ1, S_BB, S_RR, S_TOL2 };   // S_TOL2: the squared relative target the SpMV tests
 enum { F_DONE = 0, F_ITERS, F_FAIL };
 
 constexpr int kSpmvBlock = 256;
@@ -188,7 +188,7 @@ __device__ __forceinline__ void bsr_row_ek(int lane, int eb, int ee, int cb, int
 // first; then p_new = z + beta p_old (p_new = z at it = 0, WITHP = false), q = A p_new,
 // partial p.q.  Row bounds of the next row group are loaded one group ahead.
 template <class M, class V, int LANES, int BS, bool WITHP, int UNR = 1>
-__global__ __launch_bounds__(BS) void k_pcg_spmv(const int* __restrict__ itDev, int itArg, int maxIt, double tol2, const int* __restrict__ erp,
+__global__ __launch_bounds__(BS) void k_pcg_spmv(const int* __restrict__ itDev, int itArg, int maxIt, const int* __restrict__ erp,
                                                  const int* __restrict__ ecol, const M* __restrict__ ev, int enz,
                                                  const int* __restrict__ crp, const int* __restrict__ ccol, const M* __restrict__ cv,
                                                  int cnz, int nrow, int rowsPerXcd, const V* __restrict__ z,
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(BS) void k_pcg_spmv(const int* __restrict__ itDev, 
     const double rr = block_allsum<BS>(arr, lds);
     const double bb = it == 0 ? rr : sc[S_BB];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (rr <= tol2 * bb || it >= maxIt) {
+    if (rr <= sc[S_TOL2] * bb || it >= maxIt) {
         if (lead) {
             if (it == 0) sc[S_BB] = rr;
             sc[S_RR] = rr;
@@ -596,6 +596,10 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
         }
     }
     const double tol2 = tol * tol;
+    // the target is read by the SpMV from device memory, so captured chunks serve every
+    // tolerance (refinement sweeps change it)
+    c.hScal[36] = tol2;
+    CIPC_HIP(hipMemcpyAsync(sc + S_TOL2, c.hScal + 36, sizeof(double), hipMemcpyHostToDevice, c.s));
     const double mb = (double)sizeof(M), vb = (double)sizeof(V);
     const double spmvBytes = (9 * mb + 4) * ((double)E.nnzb + K.nnzb) + 4.0 * (nV + 1) * (K.nnzb > 0 ? 2 : 1) + 4 * 3 * vb * nV;
     int* itDev = (int*)(c.counters.p + 24);   // base iteration of a captured chunk
@@ -608,11 +612,11 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
         const double* partRR = learned && it > 0 ? partF : partU + nU;
         const int nRR = learned && it > 0 ? nP : nU;
         if (timed)
-            hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.evPool[2 * k], c.evPool[2 * k + 1], 0, itd, itArg, maxIter, tol2,
+            hipExtLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, c.evPool[2 * k], c.evPool[2 * k + 1], 0, itd, itArg, maxIter,
                                   E.rowptr.p, E.col.p, valE, E.nnzb, K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew,
                                   q, partRZ, nRZ, partRR, nRR, partS, sc, flags);
         else
-            hipLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, itd, itArg, maxIter, tol2, E.rowptr.p, E.col.p, valE, E.nnzb,
+            hipLaunchKernelGGL(kern, dim3(nS), dim3(kSpmvBlock), 0, c.s, itd, itArg, maxIter, E.rowptr.p, E.col.p, valE, E.nnzb,
                                K.rowptr.p, K.col.p, valK, K.nnzb, nV, rpxS, (const V*)z, pold, pnew, q, partRZ, nRZ, partRR, nRR, partS,
                                sc, flags);
         if (!learned) {
@@ -647,10 +651,9 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
         // number comes from the device counter, the buffer parities repeat every 16)
         const bool asGraph = c.useGraphs && !sampling && it > 0 && it % chunk == 0 && it + chunk - 1 <= maxIter;
         if (asGraph) {
-            uint64_t tolBits = 0;
-            std::memcpy(&tolBits, &tol2, sizeof(tolBits));
+            // (the target tolerance is read from sc[S_TOL2] at run time, not captured)
             std::vector<uintptr_t> key = {(uintptr_t)(void*)spmv_kernel<M, V>(sLanes, sUnroll, true), (uintptr_t)nS, (uintptr_t)rpxS,
-                                          (uintptr_t)maxIter, (uintptr_t)tolBits,
+                                          (uintptr_t)maxIter,
                                           (uintptr_t)E.rowptr.p, (uintptr_t)E.col.p, (uintptr_t)valE, (uintptr_t)E.nnzb,
                                           (uintptr_t)K.rowptr.p, (uintptr_t)K.col.p, (uintptr_t)valK, (uintptr_t)K.nnzb, (uintptr_t)nV,
                                           (uintptr_t)z, (uintptr_t)pbuf, (uintptr_t)q, (uintptr_t)partRZ, (uintptr_t)nRZ,
