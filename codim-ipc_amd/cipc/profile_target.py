@@ -11,7 +11,9 @@ learned factor (the network's forward pass on the matrix cores), one Newton line
 (right-hand side: the step's first one, -grad of the incremental potential at x) in the
 given precision form with the learned or block-Jacobi preconditioner, and the CCD step
 size along that solve's direction.  Launch shapes are the library defaults (no tuning),
-graph capture is off (every kernel is its own dispatch, as tracers expect).
+graph capture is off (every kernel is its own dispatch, as tracers expect).  The solve
+also reports the PCG SpMV's mean dispatch time (event-timed on every 8th launch) and its
+algorithmic bytes per launch at this state.
 """
 import argparse
 import sys
@@ -65,9 +67,15 @@ def main():
         xt = (x.reshape(-1, 3) + h * v.reshape(-1, 3) + h * h * np.where(mass[:, None] > 0, grav[None, :], 0.0)).ravel()
         b = -(sim.elastic_gradient(x) + gb + np.repeat(mass, 3) * (x - xt))
         b.reshape(-1, 3)[dbc] = 0
+        sim.set_profiling(True)
         p, it, rel = sim.linear_solve(x, C, kappa, b, tol=1e-5)
+        pr = sim.profile()
+        sim.set_profiling(False)
         log("linear solve (%s, %s): %d PCG iterations, relative residual %.2e" % (a.precision, "learned" if a.learned else "block Jacobi",
                                                                                  it, rel))
+        if pr["launches"]:
+            log("PCG SpMV: %d timed launches, %.2f us mean, %d algorithmic bytes per launch (%d block rows, %d stored blocks)"
+                % (pr["launches"], 1e3 * pr["total_ms"] / pr["launches"], pr["bytes_per_launch"], pr["block_rows"], pr["nnzb"]))
         alpha, _ = sim.ccd_stepsize(x, p, 1.0)
         log("CCD step size %.6g" % alpha)
     sim.close()

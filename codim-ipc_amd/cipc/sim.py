@@ -215,7 +215,17 @@ class Simulator:
     def solver_info(self):
         out = np.zeros(4, np.int32)
         self._check(self.lib.cipc_get_solver_info(self.h, _i(out)))
-        return dict(refinements=int(out[0]), graph_captures=int(out[1]), graph_launches=int(out[2]))
+        return dict(refinements=int(out[0]), graph_captures=int(out[1]), graph_launches=int(out[2]),
+                    direct_solves=int(out[3]))
+
+    LINEAR_SOLVER = {"pcg": 0, "direct": 1}
+
+    def set_linear_solver(self, solver):
+        """'pcg' (default; a solve that reaches pcg_max_iter is redone by the dense direct
+        solve when the system is small enough) or 'direct' (dense FP64 Cholesky for every
+        Newton solve, as the reference's Solve_Direct; at most 24576 unknowns)
+        (cipc_set_linear_solver)"""
+        self._check(self.lib.cipc_set_linear_solver(self.h, self.LINEAR_SOLVER[solver]))
 
     def set_output_folder(self, folder):
         """the step's outputFolder: residual.txt, counter.txt, stretch.txt appended per step

@@ -742,6 +742,7 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
         if (relres) *relres = rr;
         if (rc == 1) { c.lastError = "matrix not positive definite"; return CIPC_E_NO_CONVERGENCE; }
         if (rc == 2) { c.lastError = "PCG iteration limit"; return CIPC_E_NO_CONVERGENCE; }
+        if (rc == 3) { c.lastError = "system too large for the direct solve"; return CIPC_E_UNSUPPORTED; }
         return CIPC_OK;
     });
 }
@@ -760,6 +761,22 @@ int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out) {
     out[0] = ctx->c.lastRefinements;
     out[1] = (int32_t)ctx->c.pcgGraphCaptures;
     out[2] = (int32_t)ctx->c.pcgGraphLaunches;
+    out[3] = (int32_t)ctx->c.directSolves;
+    return CIPC_OK;
+}
+
+int cipc_set_linear_solver(cipc_ctx* ctx, int32_t mode) {
+    if (!ctx) return CIPC_E_INVALID;
+    Ctx& c = ctx->c;
+    if (mode < 0 || mode > 1) {
+        c.lastError = "linear solver must be 0 (PCG, direct fallback) or 1 (direct)";
+        return CIPC_E_INVALID;
+    }
+    if (mode == 1 && 3l * c.nV > kDirectMaxDof) {
+        c.lastError = "direct solve limited to " + std::to_string(kDirectMaxDof) + " unknowns";
+        return CIPC_E_UNSUPPORTED;
+    }
+    c.P.linSolver = mode;
     return CIPC_OK;
 }
 

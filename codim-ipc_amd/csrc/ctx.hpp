@@ -23,6 +23,9 @@ struct Params {
     int pcgMaxIter = 100000;
     int collision = 1;
     int precision = 0;            // 0: FP64 PCG; 1: FP32 blocks + FP64 vectors; 2: all-FP32 PCG (1, 2 inside FP64 refinement)
+    // 0: PCG, with the dense direct solve taking over a solve that reaches pcgMaxIter when
+    // the system has at most kDirectMaxDof unknowns; 1: dense direct solve for every solve
+    int linSolver = 0;
     // rods and particles
     double rodRho = 0, rodE = 0, rodBendMult = 1, rodThick = 1e-3;
     // volumetric tets (density, Young's modulus, Poisson ratio)
@@ -47,6 +50,9 @@ struct Params {
 constexpr uint32_t kLargeCells = 512;
 
 struct LearnedPrecond;
+
+// largest system (unknowns) of the dense direct solve: 24576^2 FP64 = 4.5 GiB of matrix
+constexpr int kDirectMaxDof = 24576;
 
 // an instantiated graph of one 16-iteration PCG chunk and the launch arguments it was
 // captured with (replayed while they stay the same)
@@ -235,6 +241,10 @@ struct Ctx {
     double lastWarmRatio = 1.0;       // ||b - H x0|| / ||b|| of the last warm-started solve
     DevBuf<double> warmR, warmX;
     DevBuf<double> fallbackX, fallbackR;   // learned iterate and its residual when block Jacobi takes over
+    // dense direct solve: padded row-major matrix / factor, right-hand side, pivot flag
+    DevBuf<double> denseA, denseY;
+    DevBuf<int> denseBad;
+    long directSolves = 0;
 
     // PCG chunks after the first replay a captured graph (cipc_set_graphs)
     bool useGraphs = true;
@@ -342,6 +352,11 @@ double elastic_energy(Ctx& c, const double* x, double* memb, double* bend);
 double inertia_energy(Ctx& c, const double* x);
 void elastic_gradient(Ctx& c, const double* x, double* g);        // g = elastic + inertia (overwrites)
 void elastic_hessian(Ctx& c, const double* x);                    // fills Ael (incl. mass, DBC projection)
+// dense FP64 Cholesky solve of (Ael + Acon) x = b (direct.hip); 0 ok, 1 non-positive
+// pivot, 3 more than kDirectMaxDof unknowns
+int direct_solve(Ctx& c, const double* b, double* x);
+// Newton linear solve (PCG or the direct solve, as P.linSolver selects); 0 ok, 1 matrix
+// not positive definite, 2 iteration limit, 3 too large for the direct solve
 // x0: optional initial guess (nullptr: zero)
 int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres, const double* x0 = nullptr);
 // start the autotuner: bit 1 PCG launch shapes, bit 2 broad-phase cell size

@@ -1021,7 +1021,7 @@ static double norm2(Ctx& c, const double* a, long n) {
 // With an initial guess x0: r0 = b - H x0 in FP64; when ||r0|| < ||b|| / 2 the solve runs
 // on H d = r0 to the tolerance that gives ||b - H (x0 + d)|| <= tol ||b||, and x = x0 + d;
 // otherwise x0 is ignored.  The stopping rule is the same as from a zero start.
-int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres, const double* x0) {
+static int pcg_solve_from(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres, const double* x0) {
     if (!x0) return pcg_solve_tuned(c, b, xsol, tol, maxIter, iters, relres);
     const long n = 3l * c.nV;
     if (c.Acon.nnzb == 0) c.Acon.rowptr.ensure(c.nV + 1);
@@ -1044,6 +1044,37 @@ int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, in
     CIPC_LAUNCH_CHECK();
     if (relres) *relres = rr * std::sqrt(rn2 / bn2);
     return rc;
+}
+
+// dense direct solve and its FP64 relative residual ||b - H x|| / ||b||
+static int direct_checked(Ctx& c, const double* b, double* xsol, double* relres) {
+    const long n = 3l * c.nV;
+    const int rc = direct_solve(c, b, xsol);
+    if (rc == 0 && relres) {
+        const double bn2 = norm2(c, b, n);
+        const double rn2 = residual_norm2(c, b, xsol, c.warmR.ensure(n));
+        *relres = bn2 > 0 ? std::sqrt(rn2 / bn2) : 0.0;
+    }
+    return rc;
+}
+
+// The reference factorises every Newton matrix (Solve_Direct, DIRECT_SOLVER.h:14-66).
+// With linSolver 1 the dense direct solve does the same here; with linSolver 0 the PCG
+// solves, and the direct solve takes over a solve that reached maxIter when the system
+// has at most kDirectMaxDof unknowns.  A non-positive pivot returns 1, as the reference's
+// CHOLMOD_NOT_POSDEF, and the step turns to the gradient direction.
+int pcg_solve(Ctx& c, const double* b, double* xsol, double tol, int maxIter, int* iters, double* relres, const double* x0) {
+    const long n = 3l * c.nV;
+    if (c.Acon.nnzb == 0) c.Acon.rowptr.ensure(c.nV + 1);
+    if (c.P.linSolver == 1) {
+        if (iters) *iters = 0;
+        return direct_checked(c, b, xsol, relres);
+    }
+    int it = 0;
+    const int rc = pcg_solve_from(c, b, xsol, tol, maxIter, &it, relres, x0);
+    if (iters) *iters = it;
+    if (rc != 2 || n > kDirectMaxDof) return rc;
+    return direct_checked(c, b, xsol, relres);
 }
 
 void autotune_start(Ctx& c, int what) {

@@ -267,10 +267,21 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
  * mode 2 (fp32): FP32 blocks, FP32 Krylov vectors, FP32 SpMV accumulation.
  * Modes 1 and 2 run inside an FP64 iterative refinement that stops when the FP64
  * residual ||b - H x|| <= pcg_tol ||b||.  Affects cipc_step and cipc_linear_solve;
- * out[0] of cipc_get_solver_info = refinement sweeps of the last solve (0 in mode 0),
- * out[1], out[2] = PCG chunk graphs captured and replayed so far (cipc_set_graphs). */
+ * out (4 entries) of cipc_get_solver_info: out[0] = refinement sweeps of the last solve
+ * (0 in mode 0), out[1], out[2] = PCG chunk graphs captured and replayed so far
+ * (cipc_set_graphs), out[3] = dense direct solves so far (cipc_set_linear_solver). */
 int cipc_set_precision(cipc_ctx* ctx, int32_t mode);
 int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out);
+/* Linear solver of the Newton systems (Solve_Direct, DIRECT_SOLVER.h:14-66: CHOLMOD
+ * analyse + factorise + solve of every Newton matrix; a non-positive pivot returns
+ * CHOLMOD_NOT_POSDEF and the step takes the gradient direction, IMPLICIT_EULER.h:515-554).
+ * mode 0 (default): PCG as configured above; a solve that reaches pcg_max_iter is
+ * redone by the dense direct solve when the system has at most 24576 unknowns.
+ * mode 1: every Newton solve (cipc_step, cipc_linear_solve) is a dense FP64 Cholesky
+ * factorisation and solve on the GPU; CIPC_E_UNSUPPORTED for scenes above 24576
+ * unknowns (8192 vertices).  A non-positive pivot makes the step take the gradient
+ * direction, as in the reference. */
+int cipc_set_linear_solver(cipc_ctx* ctx, int32_t mode);
 /* No reference counterpart.  enable != 0 (the default): every 16-iteration PCG chunk after
  * a solve's first runs as one replay of a captured HIP graph (its launches identical from
  * chunk to chunk; re-captured when the matrix, buffers or launch shapes change); 0: the

@@ -194,16 +194,86 @@ def test_linear_solve_mixed_refinement(stack, precision):
     assert sweeps >= (2 if precision == "fp32" else 1)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+def test_direct_solve(stack):
+    """Dense FP64 Cholesky of the Newton matrix on the GPU (linear solver 'direct', the
+    reference's Solve_Direct).  On the library's own assembled matrix the residual is of
+    the order of LAPACK's dense solve of that matrix (at most 10x it: a backward-stable
+    solve's round-off) and the solution equals scipy's sparse LU solution to 1e-9; on the
+    oracle's assembled matrix (equal to 1e-9, different PSD eigensolvers) the residual meets
+    the 2e-8 bar of the PCG test."""
+    sd, gpu, orc, states = stack
+    import scipy.sparse.linalg as spla
+    kappa = orc.scalars()["kappa"]
+    x = states[1]
+    C = orc.constraint_set(x)
+    H = orc.hessian(x, C, kappa, which=3)
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(x.size)
+    b.reshape(-1, 3)[sd.dbc_idx] = 0
+    before = gpu.solver_info()["direct_solves"]
+    gpu.set_linear_solver("direct")
+    try:
+        sol, it, rr = gpu.linear_solve(x, C, kappa, b, tol=1e-8)
+    finally:
+        gpu.set_linear_solver("pcg")
+    assert gpu.solver_info()["direct_solves"] == before + 1
+    assert it == 0
+    assert rr <= 1e-10
+    Hg = gpu.hessian(x, C, kappa, which=3)
+    lapack = np.linalg.norm(Hg @ np.linalg.solve(Hg.toarray(), b) - b)
+    assert np.linalg.norm(Hg @ sol - b) <= 10 * lapack + 1e-14 * np.linalg.norm(b)
+    assert rel(sol, spla.spsolve(Hg.tocsc(), b)) <= 1e-9
+    assert np.linalg.norm(H @ sol - b) <= 2e-8 * np.linalg.norm(b)
+
+
+def test_direct_solve_takes_over_at_iteration_limit(require_gpu):
+    """A PCG solve that reaches pcg_max_iter is redone by the direct solve (systems of at
+    most 24576 unknowns), so the solve still meets its tolerance."""
+    sd = scenes.scene_contact_stack()
+    gpu = Simulator(sd, pcg_max_iter=3)
+    orc = O.OracleScene(sd, threads=4)
+    kappa = orc.scalars()["kappa"]
+    x = orc.get_state()[0]
+    C = orc.constraint_set(x)
+    H = orc.hessian(x, C, kappa, which=3)
+    b = np.random.default_rng(9).standard_normal(x.size)
+    b.reshape(-1, 3)[sd.dbc_idx] = 0
+    sol, it, rr = gpu.linear_solve(x, C, kappa, b, tol=1e-8)
+    assert it == 3
+    assert gpu.solver_info()["direct_solves"] == 1
+    assert rr <= 1e-10
+    Hg = gpu.hessian(x, C, kappa, which=3)
+    lapack = np.linalg.norm(Hg @ np.linalg.solve(Hg.toarray(), b) - b)
+    assert np.linalg.norm(Hg @ sol - b) <= 10 * lapack + 1e-14 * np.linalg.norm(b)
+    assert np.linalg.norm(H @ sol - b) <= 2e-8 * np.linalg.norm(b)
+
+
+def test_direct_solve_size_limit(require_gpu):
+    """The dense direct solve is refused above 24576 unknowns (8192 vertices)."""
+    sd = scenes.scene_cloth_on_ground(n=100)
+    assert 3 * sd.X.shape[0] > 24576
+    gpu = Simulator(sd)
+    with pytest.raises(Exception) as e:
+        gpu.set_linear_solver("direct")
+    assert getattr(e.value, "code", None) == -5
+    small = Simulator(scenes.scene_contact_stack())
+    small.set_linear_solver("direct")
+    small.set_linear_solver("pcg")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32", "direct"])
 def test_step_trajectory(require_gpu, precision):
     """Five steps of the contact stack, tight Newton tolerance; FP64 oracle.  Before each
     step the oracle takes the GPU's state (positions, velocities, barrier stiffness), so
     every step is compared from the same input (bars.trajectory_bar: 1e-6 of the step's
     displacement or the Newton stopping resolution).  The mixed solver refines to the same
-    1e-10 linear residual, so the same bar applies."""
+    1e-10 linear residual, and the direct solve (dense Cholesky) solves exactly, so the
+    same bar applies."""
     sd = scenes.scene_contact_stack()
     sd.params["PNTol"] = 1e-6 * sd.bbox_diag()
-    gpu = Simulator(sd, pcg_tol=1e-10, precision=precision)
+    gpu = Simulator(sd, pcg_tol=1e-10, precision="fp64" if precision == "direct" else precision)
+    if precision == "direct":
+        gpu.set_linear_solver("direct")
     orc = O.OracleScene(sd, threads=4)
     for s in range(5):
         x0, v0 = gpu.get_state()
@@ -218,6 +288,8 @@ def test_step_trajectory(require_gpu, precision):
         assert np.abs(xg - xo).max() <= trajectory_bar(sd, disp), (s, st["newton_iters"], so)
         d2, m = gpu.min_dist2(xg, gpu.constraint_set(xg)) if len(gpu.constraint_set(xg)) else (None, 1.0)
         assert m > 0
+    if precision == "direct":
+        assert gpu.solver_info()["direct_solves"] > 0 and st["pcg_iters"] == 0
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
