@@ -71,10 +71,11 @@ PRECISION = {"fp64": "fp64", "mixed": "mixed", "fp32": "fp32", "mixed_learned": 
 def measured_traffic(kernel, workload, window):
     """HBM bytes per launch of `kernel` on `workload` from the committed FETCH_SIZE /
     WRITE_SIZE summary (profiles/) when its measured steps lie inside the timed `window`
-    (first and last step), or None when no measurement matches."""
+    (first and last step): (bytes, algorithmic bytes of the same launches, summary path),
+    or Nones when no measurement matches."""
     path = os.path.join(ROOT, "profiles", "k_pcg_spmv_c2_hbm_traffic.json")
     if not os.path.exists(path):
-        return None, None
+        return None, None, None
     with open(path) as f:
         t = json.load(f)
     w = t.get("window")
@@ -82,8 +83,8 @@ def measured_traffic(kernel, workload, window):
     # the summary names the kernel family (block / vector types); every launch shape of it
     # reads and writes the same arrays
     if not kernel.startswith(t["kernel"]) or t["workload"] != workload or not inside:
-        return None, None
-    return int(t["traffic_bytes_per_launch"]), t["summary"]
+        return None, None, None
+    return int(t["traffic_bytes_per_launch"]), int(t["algorithmic_bytes_per_launch"]), t["summary"]
 
 
 def fast_forward(args, sd):
@@ -164,9 +165,13 @@ def gpu_run(args, sd, form, state):
         avg_s = prof["total_ms"] / prof["launches"] / 1e3
         ach = prof["bytes_per_launch"] / avg_s / 1e9
         kname = kernel_name(form, sim.tuning())
-        traffic, source = measured_traffic(kname, sd.name, window)
+        traffic, traffic_alg, source = measured_traffic(kname, sd.name, window)
+        # the counters are read at one state of the window (its first step's start), so
+        # next to `traffic` stand the algorithmic bytes of those same launches
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=traffic, traffic_source=source, kernel=kname, window_steps=list(window),
+                    traffic=traffic, traffic_algorithmic=traffic_alg,
+                    traffic_ratio=round(traffic / traffic_alg, 4) if traffic else None, traffic_source=source,
+                    kernel=kname, window_steps=list(window),
                     launches_timed=prof["launches"], pcg_iterations=pcg, avg_us=round(avg_s * 1e6, 2),
                     bytes_per_launch=int(prof["bytes_per_launch"]), nnzb=prof["nnzb"], block_rows=prof["block_rows"])
     return dict(precision=PRECISION[form], steps_per_s=args.steps / elapsed,
