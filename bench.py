@@ -52,7 +52,9 @@ KERNEL = {"fp64": ("k_pcg_spmv<double,double,%d,256,true,%d>", "PCG SpMV, FP64 b
           "mixed": ("k_pcg_spmv<float,double,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP64 vectors"),
           "fp32": ("k_pcg_spmv<float,float,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP32 vectors"),
           "mixed_learned": ("k_pcg_spmv<float,double,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP64 vectors"),
-          "fp32_learned": ("k_pcg_spmv<float,float,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP32 vectors")}
+          "fp32_learned": ("k_pcg_spmv<float,float,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP32 vectors"),
+          "mixed_learned_coarse": ("k_pcg_spmv<float,double,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP64 vectors"),
+          "fp32_learned_coarse": ("k_pcg_spmv<float,float,%d,256,true,%d>", "PCG SpMV, FP32 blocks, FP32 vectors")}
 
 
 def kernel_name(precision, tuning):
@@ -64,8 +66,13 @@ MODE = {"fp64": "plain FP64, block-Jacobi PCG",
         "mixed": "FP32 Hessian blocks, FP64 vectors, FP64 iterative refinement, block-Jacobi PCG",
         "fp32": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, block-Jacobi PCG",
         "mixed_learned": "FP32 Hessian blocks, FP64 vectors, FP64 iterative refinement, learned-factor PCG (block-Jacobi fallback)",
-        "fp32_learned": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, learned-factor PCG (block-Jacobi fallback)"}
-PRECISION = {"fp64": "fp64", "mixed": "mixed", "fp32": "fp32", "mixed_learned": "mixed", "fp32_learned": "fp32"}
+        "fp32_learned": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, learned-factor PCG (block-Jacobi fallback)",
+        "mixed_learned_coarse": "FP32 Hessian blocks, FP64 vectors, FP64 iterative refinement, learned-factor PCG with the "
+                                "rigid-body coarse level (block-Jacobi fallback)",
+        "fp32_learned_coarse": "FP32 blocks/vectors/SpMV accumulation, FP64 iterative refinement, learned-factor PCG with the "
+                               "rigid-body coarse level (block-Jacobi fallback)"}
+PRECISION = {"fp64": "fp64", "mixed": "mixed", "fp32": "fp32", "mixed_learned": "mixed", "fp32_learned": "fp32",
+             "mixed_learned_coarse": "mixed", "fp32_learned_coarse": "fp32"}
 
 
 def measured_traffic(kernel, workload, window):
@@ -123,11 +130,13 @@ def gpu_run(args, sd, form, state):
         sim.set_state(x0, v0)
     if kappa0 is not None:
         sim.set_kappa(kappa0)
-    learned = form.endswith("_learned")
+    learned = "_learned" in form
     if learned:
         from cipc import precond
         sim.precond_set_weights(precond.load())
         sim.set_preconditioner("learned", budget=args.learned_budget, reuse=args.learned_reuse)
+        if form.endswith("_coarse"):
+            sim.set_coarse(True, every=args.coarse_every, max_unknowns=args.coarse_max)
     if args.autotune:
         sim.autotune()          # settles during the warm-up steps
     tw = time.perf_counter()
@@ -184,6 +193,7 @@ def gpu_run(args, sd, form, state):
                 statuses=sorted(set(s["status"] for s in stats)), min_dist_final=min_d,
                 learned_solves=(info1["solves"] - info0["solves"]) if learned else None,
                 learned_fallbacks=(info1["fallbacks"] - info0["fallbacks"]) if learned else None,
+                coarse=dict(sim.coarse_info(), every=args.coarse_every, max_unknowns=args.coarse_max) if form.endswith("_coarse") else None,
                 spmv_avg_us=roof["avg_us"] if roof else None, spmv_frac=roof["frac"] if roof else None, roofline=roof,
                 _x=x, _energy=stats[-1]["energy"], _start=start_state)
 
@@ -327,8 +337,14 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--batch-form", default="fp32_learned", choices=["fp64", "fp32", "fp32_learned"])
-    ap.add_argument("--precision", default="headline", choices=["headline", "all", "fp64", "mixed", "fp32", "mixed_learned", "fp32_learned"],
-                    help="headline: fp64, mixed_learned and fp32_learned; all: the five forms")
+    ap.add_argument("--precision", default="headline",
+                    choices=["headline", "all", "fp64", "mixed", "fp32", "mixed_learned", "fp32_learned", "mixed_learned_coarse",
+                             "fp32_learned_coarse"],
+                    help="headline: fp64, mixed_learned and mixed_learned_coarse; all: the seven forms")
+    ap.add_argument("--coarse-max", type=int, default=4000,
+                    help="largest coarse problem (unknowns) of the *_coarse forms")
+    ap.add_argument("--coarse-every", type=int, default=1,
+                    help="Newton solves per coarse set-up of the *_coarse forms (always at a step's first solve)")
     ap.add_argument("--learned-budget", type=int, default=2500,
                     help="PCG iterations after which a learned solve falls back to block Jacobi (continuing from its iterate)")
     ap.add_argument("--learned-reuse", type=int, default=None,
@@ -358,10 +374,11 @@ def main():
 
     from cipc import precond
     have_w = os.path.exists(precond.WEIGHTS)
-    modes = {"headline": ["fp64", "mixed_learned", "fp32_learned"], "all": ["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned"]}.get(
+    modes = {"headline": ["fp64", "mixed_learned", "mixed_learned_coarse"],
+             "all": ["fp64", "mixed", "fp32", "mixed_learned", "fp32_learned", "mixed_learned_coarse", "fp32_learned_coarse"]}.get(
         args.precision, [args.precision])
     if not have_w:
-        modes = [m for m in modes if not m.endswith("_learned")]
+        modes = [m for m in modes if "_learned" not in m]
     x0, v0, kappa0, ff = fast_forward(args, sd)
     runs = {m: gpu_run(args, sd, m, (x0, v0, kappa0)) for m in modes}
     # each form against the FP64 form after the same steps from the same state: max position
@@ -374,8 +391,10 @@ def main():
         for m, r in runs.items():
             r["vs_fp64_max_dx_rel"] = float(np.abs(r["_x"] - x64).max()) / max(disp, 1e-300)
             r["vs_fp64_energy_rel"] = abs(r["_energy"] - e64) / max(abs(e64), 1e-300)
-    # the metric's "mixed+learned" form; the FP32-vector variant is reported beside it
-    head_form = next((m for m in ("mixed_learned", "fp32_learned") if m in runs), ([m for m in modes if m != "fp64"] or modes)[-1])
+    # the metric's "mixed+learned" form: with the coarse level when it ran, else the learned
+    # factor alone; the other forms are reported beside it
+    head_form = next((m for m in ("mixed_learned_coarse", "mixed_learned", "fp32_learned_coarse", "fp32_learned") if m in runs),
+                     ([m for m in modes if m != "fp64"] or modes)[-1])
     head = runs[head_form]
     cpu = None
     if not args.no_cpu_baseline:
@@ -402,7 +421,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "mixed": "f32 blocks + f64", "fp32": "f32 + f64 refinement",
-                  "mixed_learned": "f32 blocks + f64, bf16 network", "fp32_learned": "f32 + f64 refinement, bf16 network"}[head_form],
+                  "mixed_learned": "f32 blocks + f64, bf16 network", "fp32_learned": "f32 + f64 refinement, bf16 network",
+                  "mixed_learned_coarse": "f32 blocks + f64, bf16 network, f64 coarse factor (bf16 inverse)",
+                  "fp32_learned_coarse": "f32 + f64 refinement, bf16 network, f64 coarse factor (bf16 inverse)"}[head_form],
         "data": "synthetic",
         "mode": MODE[head_form],
         "config": {"workload": sd.name, "vertices": sd.nV, "triangles": sd.nT, "cloth_vertices": sd.n_cloth,

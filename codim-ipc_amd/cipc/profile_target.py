@@ -2,7 +2,7 @@
 saved contact state, with a progress line after every stage.
 
     python -m cipc.profile_target STATE.npz [--workload c2] [--precision mixed]
-                                           [--no-learned] [--repeat N]
+                                           [--no-learned] [--coarse MAX_UNKNOWNS] [--repeat N]
 
 STATE.npz is written by `bench.py --save-state` (x, v, kappa, step).  Stages, in order:
 constraint set (Compute_Constraint_Set), barrier energy and gradient, the projected
@@ -32,6 +32,9 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "fp32"])
     ap.add_argument("--no-learned", dest="learned", action="store_false")
     ap.add_argument("--repeat", type=int, default=1)
+    ap.add_argument("--coarse", type=int, default=-1, metavar="MAX_UNKNOWNS",
+                    help="two-level correction of the learned preconditioner with at most this many coarse unknowns "
+                         "(0: the library's bound)")
     a = ap.parse_args()
     t0 = time.time()
 
@@ -46,6 +49,8 @@ def main():
     if a.learned:
         sim.precond_set_weights(precond.load())
         sim.set_preconditioner("learned", budget=20000)
+        if a.coarse >= 0:
+            sim.set_coarse(True, max_unknowns=a.coarse)
     sim.set_state(x, v)
     sim.set_kappa(kappa)
     mass = sim.setup_arrays()["mass"]
@@ -68,11 +73,16 @@ def main():
         b = -(sim.elastic_gradient(x) + gb + np.repeat(mass, 3) * (x - xt))
         b.reshape(-1, 3)[dbc] = 0
         sim.set_profiling(True)
+        ts = time.time()
         p, it, rel = sim.linear_solve(x, C, kappa, b, tol=1e-5)
+        ts = time.time() - ts
         pr = sim.profile()
         sim.set_profiling(False)
-        log("linear solve (%s, %s): %d PCG iterations, relative residual %.2e" % (a.precision, "learned" if a.learned else "block Jacobi",
-                                                                                 it, rel))
+        form = ("learned + coarse" if a.coarse >= 0 else "learned") if a.learned else "block Jacobi"
+        log("linear solve (%s, %s): %d PCG iterations, relative residual %.2e, %.1f ms with assembly" % (a.precision, form, it, rel,
+                                                                                                       1e3 * ts))
+        if a.coarse >= 0:
+            log("coarse level: %s" % sim.coarse_info())
         if pr["launches"]:
             log("PCG SpMV: %d timed launches, %.2f us mean, %d algorithmic bytes per launch (%d block rows, %d stored blocks)"
                 % (pr["launches"], 1e3 * pr["total_ms"] / pr["launches"], pr["bytes_per_launch"], pr["block_rows"], pr["nnzb"]))

@@ -218,6 +218,19 @@ class Simulator:
         return dict(refinements=int(out[0]), graph_captures=int(out[1]), graph_launches=int(out[2]),
                     direct_solves=int(out[3]))
 
+    def set_coarse(self, enable=True, every=1, max_unknowns=0):
+        """two-level correction of the learned preconditioner: rigid-body modes on
+        aggregates of the elastic vertex graph, Galerkin coarse matrix set up at a step's
+        first solve and then every `every` solves, at most `max_unknowns` coarse unknowns
+        (0: the library's bound) (cipc_set_coarse; learned forms only)"""
+        self._check(self.lib.cipc_set_coarse(self.h, int(every) if enable else 0, int(max_unknowns)))
+
+    def coarse_info(self):
+        out = np.zeros(5)
+        self._check(self.lib.cipc_get_coarse_info(self.h, _d(out)))
+        return dict(aggregates=int(out[0]), unknowns=int(out[1]), setups=int(out[2]), last_setup_ms=float(out[3]),
+                    ready=bool(out[4]))
+
     LINEAR_SOLVER = {"pcg": 0, "direct": 1}
 
     def set_linear_solver(self, solver):

@@ -734,8 +734,11 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
         int it = 0;
         double rr = 0;
         c.lpStepStart = true;   // a standalone solve evaluates the learned network afresh
+        c.cg.stepStart = true;
         c.lpOffThisStep = false;
+        c.solveX = ctx->qx.p;
         int rc = pcg_solve(c, ctx->qb.p, sd, tol > 0 ? tol : c.P.pcgTol, c.P.pcgMaxIter, &it, &rr);
+        c.solveX = nullptr;
         CIPC_HIP(hipMemcpyAsync(sol, sd, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
         CIPC_HIP(hipStreamSynchronize(c.s));
         if (iters) *iters = it;
@@ -762,6 +765,26 @@ int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out) {
     out[1] = (int32_t)ctx->c.pcgGraphCaptures;
     out[2] = (int32_t)ctx->c.pcgGraphLaunches;
     out[3] = (int32_t)ctx->c.directSolves;
+    return CIPC_OK;
+}
+
+int cipc_set_coarse(cipc_ctx* ctx, int32_t every, int32_t max_unknowns) {
+    if (!ctx || every < 0 || max_unknowns < 0) return CIPC_E_INVALID;
+    Ctx& c = ctx->c;
+    c.coarseOn = every > 0;
+    c.cg.every = every > 0 ? every : 1;
+    c.cg.maxDof = max_unknowns;
+    c.cg.ready = false;
+    return CIPC_OK;
+}
+int cipc_get_coarse_info(const cipc_ctx* ctx, double* out) {
+    if (!ctx || !out) return CIPC_E_INVALID;
+    const CoarseSpace& g = ctx->c.cg;
+    out[0] = g.built ? g.nAgg : -1;
+    out[1] = g.nc;
+    out[2] = (double)g.setups;
+    out[3] = g.lastSetupMs;
+    out[4] = g.ready ? 1 : 0;
     return CIPC_OK;
 }
 

@@ -51,6 +51,36 @@ constexpr uint32_t kLargeCells = 512;
 
 struct LearnedPrecond;
 
+// Two-level correction of the learned preconditioner (coarse.hip): rigid-body modes
+// (three translations, three rotations about the centroid) on aggregates of the elastic
+// vertex graph, Galerkin coarse matrix P^T H P factorised densely, its inverse applied as
+// two triangular products per PCG iteration.
+constexpr int kCoarseMaxDof = 12288;      // largest coarse problem (6 per aggregate)
+constexpr int kCoarseMaxNb = 256;         // neighbour aggregates of one aggregate (Galerkin row)
+constexpr int kCoarseMaxVerts = 4096;     // vertices of one aggregate
+struct CoarseSpace {
+    bool built = false;                   // aggregates of the current elastic pattern
+    bool ready = false;                   // factor valid for the current Hessian
+    int nAgg = 0, nc = 0, npad = 0;
+    int nnzbBuilt = -1, nVBuilt = -1;
+    DevBuf<int> agg, aggPtr, aggVerts;    // per vertex aggregate (-1: not in the coarse space), CSR of members
+    DevBuf<double> cent;                  // per aggregate: centroid xyz, 1 / rms radius
+    DevBuf<double> d;                     // per vertex: (x - centroid) / rms radius
+    DevBuf<double> A;                     // dense coarse matrix, then its Cholesky factor, then L^-1 (npad^2)
+    DevBuf<double> W;                     // dense work: L^-1 (npad^2)
+    DevBuf<double> diag0, tileInv;        // coarse diagonal before factorisation; inverses of the 64x64 diagonal tiles
+    DevBuf<int> dropped;                  // coarse unknowns whose pivot vanished (degenerate modes)
+    DevBuf<uint16_t> W16, WT16;           // L^-1 and its transpose in BF16, row-major npad^2
+    DevBuf<double> rc, wc, zc;            // coarse vectors
+    DevBuf<int> flag;                     // Galerkin overflow flag
+    int maxDof = 0;                       // largest coarse problem asked for (0: kCoarseMaxDof)
+    int builtMaxDof = -1;
+    int every = 1, sinceSetup = 0;        // set-up at a step's first solve and then every `every` solves
+    bool stepStart = true;
+    double lastSetupMs = 0;
+    long setups = 0;
+};
+
 // largest system (unknowns) of the dense direct solve: 24576^2 FP64 = 4.5 GiB of matrix
 constexpr int kDirectMaxDof = 24576;
 
@@ -241,6 +271,10 @@ struct Ctx {
     double lastWarmRatio = 1.0;       // ||b - H x0|| / ||b|| of the last warm-started solve
     DevBuf<double> warmR, warmX;
     DevBuf<double> fallbackX, fallbackR;   // learned iterate and its residual when block Jacobi takes over
+    // two-level correction of the learned preconditioner (cipc_set_coarse)
+    bool coarseOn = false;
+    CoarseSpace cg;
+    const double* solveX = nullptr;         // positions of the Newton iterate being solved (rotation modes)
     // dense direct solve: padded row-major matrix / factor, right-hand side, pivot flag
     DevBuf<double> denseA, denseY;
     DevBuf<int> denseBad;
@@ -355,6 +389,19 @@ void elastic_hessian(Ctx& c, const double* x);                    // fills Ael (
 // dense FP64 Cholesky solve of (Ael + Acon) x = b (direct.hip); 0 ok, 1 non-positive
 // pivot, 3 more than kDirectMaxDof unknowns
 int direct_solve(Ctx& c, const double* b, double* x);
+// tiled dense Cholesky in place of a padded row-major npad x npad matrix (lower factor;
+// direct.hip).  bad: set on a non-positive pivot.  tileInv (npad / 64 tiles of 64 x 64):
+// the diagonal tiles' inverses are kept there and the panels become products; nullptr:
+// panels by substitution.  With diag0 / dropped: a pivot below tau x its original
+// diagonal is dropped (pivot at that scale, zero column) and flagged instead.
+void dense_cholesky(Ctx& c, double* A, int npad, int* bad, double* tileInv, const double* diag0 = nullptr, int* dropped = nullptr,
+                    double tau = 0);
+// W = L^-1 of a dense_cholesky factor whose diagonal-tile inverses are in tileInv
+void dense_lower_inverse(Ctx& c, const double* L, double* W, int npad, const double* tileInv);
+// two-level correction (coarse.hip)
+bool coarse_prepare(Ctx& c, const double* x);            // aggregates (once), Galerkin matrix, factor; false: unavailable
+template <class V>
+void coarse_restrict_solve(Ctx& c, const V* r, const int* done);   // c.cg.zc = A_c^-1 P^T r
 // Newton linear solve (PCG or the direct solve, as P.linSolver selects); 0 ok, 1 matrix
 // not positive definite, 2 iteration limit, 3 too large for the direct solve
 // x0: optional initial guess (nullptr: zero)

@@ -40,45 +40,69 @@ __global__ void k_dense_scatter(const int* __restrict__ rowOf, const int* __rest
         }
 }
 
-// Cholesky of diagonal tile k in LDS (lower, in place); a non-positive pivot sets *bad and
-// continues with pivot 1 so that the factorisation stays finite
-__global__ __launch_bounds__(kDirectBlock) void k_potrf_tile(double* __restrict__ a, long ld, int k, int* __restrict__ bad) {
+// Cholesky of diagonal tile k in LDS (lower, in place), one wave: thread i owns row i of
+// the tile.  A non-positive pivot sets *bad and continues with pivot 1 so that the
+// factorisation stays finite.  With diag0, a pivot at or below tau x diag0 (the unknown's
+// diagonal before elimination: a mode the matrix does not see) is dropped instead: a
+// pivot at the unknown's own scale, zero column, dropped[] = 1.  With inv, the inverse of
+// the factored tile is written there too (column c by forward substitution, thread c).
+__global__ __launch_bounds__(kDT) void k_potrf_tile(double* __restrict__ a, long ld, int k, int* __restrict__ bad,
+                                                    const double* __restrict__ diag0, int* __restrict__ dropped, double tau,
+                                                    double* __restrict__ inv) {
     __shared__ double T[kDT][kDT + 1];
+    __shared__ int drop;
     const long o = (long)k * kDT;
-    for (int e = threadIdx.x; e < kDT * kDT; e += kDirectBlock) {
-        const int i = e / kDT, j = e % kDT;
-        T[i][j] = a[(o + i) * ld + o + j];
-    }
+    const int i = threadIdx.x;
+    for (int r = 0; r < kDT; ++r) T[r][i] = a[(o + r) * ld + o + i];
     __syncthreads();
     for (int j = 0; j < kDT; ++j) {
-        if (threadIdx.x == 0) {
+        if (i == j) {
             double d = T[j][j];
-            if (!(d > 0)) {
+            int dr = 0;
+            if (diag0) {
+                if (!(d > tau * diag0[o + j])) {
+                    dr = 1;
+                    d = diag0[o + j] > 0 ? diag0[o + j] : 1.0;
+                }
+                dropped[o + j] = dr;
+            } else if (!(d > 0)) {
                 atomicOr(bad, 1);
                 d = 1.0;
             }
             T[j][j] = sqrt(d);
+            drop = dr;
         }
         __syncthreads();
-        const double djj = T[j][j];
-        for (int i = j + 1 + threadIdx.x; i < kDT; i += kDirectBlock) T[i][j] /= djj;
+        const bool dr = drop;
+        if (i > j) T[i][j] = dr ? 0.0 : T[i][j] / T[j][j];
         __syncthreads();
-        // trailing lower part of the tile
-        const int m = kDT - j - 1;
-        for (int e = threadIdx.x; e < m * m; e += kDirectBlock) {
-            const int i = j + 1 + e / m, l = j + 1 + e % m;
-            if (l <= i) T[i][l] -= T[i][j] * T[l][j];
+        if (!dr && i > j) {
+            const double lij = T[i][j];
+            for (int l = j + 1; l <= i; ++l) T[i][l] -= lij * T[l][j];
         }
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < kDT * kDT; e += kDirectBlock) {
-        const int i = e / kDT, j = e % kDT;
-        a[(o + i) * ld + o + j] = j <= i ? T[i][j] : 0.0;
+    for (int r = 0; r < kDT; ++r) a[(o + r) * ld + o + i] = i <= r ? T[r][i] : 0.0;
+    if (inv) {
+        // column i of the inverse: X[r][i] = (delta_ri - sum_{i <= m < r} L[r][m] X[m][i]) / L[r][r]
+        double* dst = inv + (long)k * kDT * kDT;
+        __syncthreads();
+        double x[kDT];
+#pragma unroll
+        for (int r = 0; r < kDT; ++r) {
+            double s = r == i ? 1.0 : 0.0;
+#pragma unroll
+            for (int m = 0; m < r; ++m) s -= (m >= i) ? T[r][m] * x[m] : 0.0;
+            x[r] = r < i ? 0.0 : s / T[r][r];
+        }
+#pragma unroll
+        for (int r = 0; r < kDT; ++r) dst[r * kDT + i] = x[r];
     }
 }
 
-// panel: tiles (i, k), i > k, become A_ik L_kk^-T (each row solved forward against L_kk)
-__global__ __launch_bounds__(kDirectBlock) void k_trsm_tiles(double* __restrict__ a, long ld, int k) {
+// panel: tiles (i, k), i > k, become A_ik L_kk^-T (each row solved forward against L_kk);
+// columns of dropped pivots are zero
+__global__ __launch_bounds__(kDirectBlock) void k_trsm_tiles(double* __restrict__ a, long ld, int k, const int* __restrict__ dropped) {
     __shared__ double L[kDT][kDT + 1];
     __shared__ double X[kDT][kDT + 1];
     const int i = k + 1 + blockIdx.x;
@@ -89,13 +113,13 @@ __global__ __launch_bounds__(kDirectBlock) void k_trsm_tiles(double* __restrict_
         X[r][c] = a[(oi + r) * ld + ok + c];
     }
     __syncthreads();
-    // 4 threads per row would race on the running sum; one thread per row, 64 rows
+    // one thread per row (the running sums of a row are sequential)
     if (threadIdx.x < kDT) {
         const int r = threadIdx.x;
         for (int c = 0; c < kDT; ++c) {
             double s = X[r][c];
             for (int m = 0; m < c; ++m) s -= X[r][m] * L[c][m];
-            X[r][c] = s / L[c][c];
+            X[r][c] = (dropped && dropped[ok + c]) ? 0.0 : s / L[c][c];
         }
     }
     __syncthreads();
@@ -105,37 +129,128 @@ __global__ __launch_bounds__(kDirectBlock) void k_trsm_tiles(double* __restrict_
     }
 }
 
+// C (64 x 64 tile, row-major, leading dimension ldc) -= P Q^T  (BT) or P Q  (!BT), P and Q
+// 64 x 64 tiles (ldp, ldq); overwrite: C = P Q instead.  Both operands are staged in LDS
+// (Pt[l][r] = P[r][l]; Qt[l][c] = Q[c][l] or Q[l][c]) so that a thread's four rows and four
+// columns are consecutive words; each thread owns a 4 x 4 block of C.  All reads of P and
+// Q precede the barrier, so C may be the tile Q.
+template <bool BT>
+__device__ __forceinline__ void tile_update(double* __restrict__ C, long ldc, const double* P, long ldp, const double* Q, long ldq,
+                                            double (*Pt)[kDT + 2], double (*Qt)[kDT + 2], bool overwrite,
+                                            const int* colMask = nullptr) {
+    // colMask (BT only): output columns c with colMask[c] != 0 come out zero
+    for (int e = threadIdx.x; e < kDT * kDT; e += kDirectBlock) {
+        const int r = e / kDT, l = e % kDT;
+        Pt[l][r] = P[r * ldp + l];
+        if (BT) Qt[l][r] = (colMask && colMask[r]) ? 0.0 : Q[r * ldq + l];
+        else Qt[r][l] = Q[r * ldq + l];
+    }
+    __syncthreads();
+    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+#pragma unroll 4
+    for (int l = 0; l < kDT; ++l) {
+        double pv[4], qv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            pv[i] = Pt[l][4 * tr + i];
+            qv[i] = Qt[l][4 * tc + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = fma(pv[i], qv[j], acc[i][j]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            double* dst = C + (4 * tr + i) * ldc + 4 * tc + j;
+            *dst = overwrite ? acc[i][j] : *dst - acc[i][j];
+        }
+}
+
 // trailing update of the lower tiles (i, j), k < j <= i: A_ij -= L_ik L_jk^T; block b
 // enumerates the lower triangle of the trailing tile matrix row by row
 __global__ __launch_bounds__(kDirectBlock) void k_syrk_tiles(double* __restrict__ a, long ld, int k, int nt) {
-    __shared__ double P[kDT][kDT + 1];
-    __shared__ double Q[kDT][kDT + 1];
+    __shared__ double Pt[kDT][kDT + 2];
+    __shared__ double Qt[kDT][kDT + 2];
     const int m = nt - k - 1;
     int b = blockIdx.x, ii = 0;
     while (b >= ii + 1) { b -= ii + 1; ++ii; }   // row ii of the trailing triangle holds ii + 1 tiles
     if (ii >= m) return;
     const int i = k + 1 + ii, j = k + 1 + b;
     const long ok = (long)k * kDT, oi = (long)i * kDT, oj = (long)j * kDT;
-    for (int e = threadIdx.x; e < kDT * kDT; e += kDirectBlock) {
-        const int r = e / kDT, c = e % kDT;
-        P[r][c] = a[(oi + r) * ld + ok + c];
-        Q[r][c] = a[(oj + r) * ld + ok + c];
+    tile_update<true>(a + oi * ld + oj, ld, a + oi * ld + ok, ld, a + oj * ld + ok, ld, Pt, Qt, false);
+}
+
+// panel by product with the inverted diagonal tile: A_ik <- A_ik inv(L_kk)^T, columns of
+// dropped pivots zero
+__global__ __launch_bounds__(kDirectBlock) void k_panel_tiles(double* __restrict__ a, long ld, int k, const double* __restrict__ inv,
+                                                             const int* __restrict__ dropped) {
+    __shared__ double Pt[kDT][kDT + 2];
+    __shared__ double Qt[kDT][kDT + 2];
+    const int i = k + 1 + blockIdx.x;
+    const long ok = (long)k * kDT, oi = (long)i * kDT;
+    double* Aik = a + oi * ld + ok;
+    tile_update<true>(Aik, ld, Aik, ld, inv + (long)k * kDT * kDT, kDT, Pt, Qt, true, dropped ? dropped + ok : nullptr);
+}
+
+void dense_cholesky(Ctx& c, double* A, int npad, int* bad, double* tileInv, const double* diag0, int* dropped, double tau) {
+    const int nt = npad / kDT;
+    const long ld = npad;
+    for (int k = 0; k < nt; ++k) {
+        // the diagonal tile's inverse (tileInv) turns the panel into a product; without it
+        // the panel rows are solved by substitution (the direct solve's exact path)
+        hipLaunchKernelGGL(k_potrf_tile, dim3(1), dim3(kDT), 0, c.s, A, ld, k, bad, diag0, dropped, tau, tileInv);
+        const int m = nt - k - 1;
+        if (m > 0) {
+            if (tileInv) hipLaunchKernelGGL(k_panel_tiles, dim3(m), dim3(kDirectBlock), 0, c.s, A, ld, k, (const double*)tileInv,
+                                            (const int*)dropped);
+            else hipLaunchKernelGGL(k_trsm_tiles, dim3(m), dim3(kDirectBlock), 0, c.s, A, ld, k, (const int*)dropped);
+            hipLaunchKernelGGL(k_syrk_tiles, dim3(m * (m + 1) / 2), dim3(kDirectBlock), 0, c.s, A, ld, k, nt);
+        }
+        CIPC_LAUNCH_CHECK();
     }
-    __syncthreads();
-    // 256 threads, 16 outputs each: thread t covers row t / 4 and columns (t % 4) + 4 s
-    const int r = threadIdx.x / 4, c0 = threadIdx.x % 4;
-    double acc[16];
-#pragma unroll
-    for (int s = 0; s < 16; ++s) acc[s] = 0.0;
-    for (int l = 0; l < kDT; ++l) {
-        const double pv = P[r][l];
-#pragma unroll
-        for (int s = 0; s < 16; ++s) acc[s] += pv * Q[c0 + 4 * s][l];
-    }
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        const int c = c0 + 4 * s;
-        a[(oi + r) * ld + oj + c] -= acc[s];
+}
+
+// L^-1 by blocked forward substitution on the identity, right-looking over tile rows k:
+// row k:  W_kj = inv(L_kk) W_kj  (j <= k; W_kj holds the updated right-hand side)
+// update: W_ij -= L_ik W_kj      (i > k, j <= k)
+__global__ __launch_bounds__(kDirectBlock) void k_inv_row(const double* __restrict__ inv, double* __restrict__ W, long ld, int k) {
+    __shared__ double Pt[kDT][kDT + 2];
+    __shared__ double Qt[kDT][kDT + 2];
+    const long ok = (long)k * kDT, oj = (long)blockIdx.x * kDT;   // tile column j = blockIdx.x <= k
+    double* Wkj = W + ok * ld + oj;
+    tile_update<false>(Wkj, ld, inv + (long)k * kDT * kDT, kDT, Wkj, ld, Pt, Qt, true);
+}
+__global__ __launch_bounds__(kDirectBlock) void k_inv_update(const double* __restrict__ L, double* __restrict__ W, long ld, int k) {
+    __shared__ double Pt[kDT][kDT + 2];
+    __shared__ double Qt[kDT][kDT + 2];
+    const int j = blockIdx.x % (k + 1), i = k + 1 + blockIdx.x / (k + 1);
+    const long ok = (long)k * kDT, oi = (long)i * kDT, oj = (long)j * kDT;
+    tile_update<false>(W + oi * ld + oj, ld, L + oi * ld + ok, ld, W + ok * ld + oj, ld, Pt, Qt, false);
+}
+__global__ void k_dense_identity(double* __restrict__ a, long ld, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[(long)i * ld + i] = 1.0;
+}
+
+// W = L^-1; tileInv holds the inverses of L's diagonal tiles (dense_cholesky with tileInv)
+void dense_lower_inverse(Ctx& c, const double* L, double* W, int npad, const double* tileInv) {
+    const int nt = npad / kDT;
+    const long ld = npad;
+    hipLaunchKernelGGL(k_dense_zero, dim3(2048), dim3(kBlock), 0, c.s, W, (long)npad * npad);
+    hipLaunchKernelGGL(k_dense_identity, dim3(nblocks(npad)), dim3(kBlock), 0, c.s, W, ld, npad);
+    CIPC_LAUNCH_CHECK();
+    for (int k = 0; k < nt; ++k) {
+        hipLaunchKernelGGL(k_inv_row, dim3(k + 1), dim3(kDirectBlock), 0, c.s, tileInv, W, ld, k);
+        if (k + 1 < nt) hipLaunchKernelGGL(k_inv_update, dim3((nt - k - 1) * (k + 1)), dim3(kDirectBlock), 0, c.s, L, W, ld, k);
+        CIPC_LAUNCH_CHECK();
     }
 }
 
@@ -206,15 +321,7 @@ int direct_solve(Ctx& c, const double* b, double* x) {
     }
     int* bad = c.denseBad.ensure(1);
     CIPC_HIP(hipMemsetAsync(bad, 0, sizeof(int), c.s));
-    for (int k = 0; k < nt; ++k) {
-        hipLaunchKernelGGL(k_potrf_tile, dim3(1), dim3(kDirectBlock), 0, c.s, A, ld, k, bad);
-        const int m = nt - k - 1;
-        if (m > 0) {
-            hipLaunchKernelGGL(k_trsm_tiles, dim3(m), dim3(kDirectBlock), 0, c.s, A, ld, k);
-            hipLaunchKernelGGL(k_syrk_tiles, dim3(m * (m + 1) / 2), dim3(kDirectBlock), 0, c.s, A, ld, k, nt);
-        }
-        CIPC_LAUNCH_CHECK();
-    }
+    dense_cholesky(c, A, npad, bad, nullptr);
     double* y = c.denseY.ensure(npad);
     CIPC_HIP(hipMemsetAsync(y, 0, npad * sizeof(double), c.s));
     CIPC_HIP(hipMemcpyAsync(y, b, n * sizeof(double), hipMemcpyDeviceToDevice, c.s));

@@ -558,7 +558,8 @@ template <int LANES, int BS, class V>
 __global__ __launch_bounds__(BS) void k_lp_spmv_bf16(int nrow, int rowsPerXcd, const int* __restrict__ rp, const int* __restrict__ col,
                                                      const uint32_t* __restrict__ val, int nnz, const V* __restrict__ x,
                                                      V* __restrict__ y, const V* __restrict__ r, double* __restrict__ part,
-                                                     const int* __restrict__ done) {
+                                                     const int* __restrict__ done, const int* __restrict__ cgAgg,
+                                                     const double* __restrict__ cgD, const double* __restrict__ cgZ) {
     if (done && *done) return;   // the PCG has converged: later iterations of the chunk do nothing
     __shared__ double lds[BS / 64];
     constexpr int RPG = BS / LANES;
@@ -597,6 +598,15 @@ __global__ __launch_bounds__(BS) void k_lp_spmv_bf16(int nrow, int rowsPerXcd, c
 #pragma unroll
             for (int k = 0; k < 3; ++k) acc[k] += __shfl_xor(acc[k], o, LANES);
         if (row < nrow && lane == 0) {
+            // two-level correction: y_i += T_i zc = t_a + w_a x d_i (rows of the coarse space)
+            const int ca = cgZ ? cgAgg[row] : -1;
+            if (ca >= 0) {
+                const double* zc = cgZ + 6 * ca;
+                const double d0 = cgD[3 * row], d1 = cgD[3 * row + 1], d2 = cgD[3 * row + 2];
+                acc[0] += (V)(zc[0] + zc[4] * d2 - zc[5] * d1);
+                acc[1] += (V)(zc[1] + zc[5] * d0 - zc[3] * d2);
+                acc[2] += (V)(zc[2] + zc[3] * d1 - zc[4] * d0);
+            }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 y[3 * row + k] = acc[k];
@@ -1133,22 +1143,24 @@ int lp_apply_partials(const Ctx& c) { return c.lpGrid; }
 
 template <int LANES, class V>
 static void lp_spmv_go(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part,
-                       const int* done) {
+                       const int* done, bool coarse) {
     constexpr int RPG = kLpSpmvBlock / LANES;
     const int nV = c.lp->nV;
     int rpx = (nV + 7) / 8;
     rpx = (rpx + RPG - 1) / RPG * RPG;
+    const CoarseSpace& g = c.cg;
     hipLaunchKernelGGL((k_lp_spmv_bf16<LANES, kLpSpmvBlock, V>), dim3(c.lpGrid), dim3(kLpSpmvBlock), 0, c.s, nV, rpx, rp, col, val, nnz,
-                       x, y, r, part, done);
+                       x, y, r, part, done, coarse ? (const int*)g.agg.p : nullptr, coarse ? (const double*)g.d.p : nullptr,
+                       coarse ? (const double*)g.zc.p : nullptr);
 }
 template <class V>
 static void lp_spmv(Ctx& c, const int* rp, const int* col, const uint32_t* val, int nnz, const V* x, V* y, const V* r, double* part,
-                    const int* done = nullptr) {
+                    const int* done = nullptr, bool coarse = false) {
     switch (c.lpLanes) {
-        case 2: lp_spmv_go<2, V>(c, rp, col, val, nnz, x, y, r, part, done); break;
-        case 4: lp_spmv_go<4, V>(c, rp, col, val, nnz, x, y, r, part, done); break;
-        case 16: lp_spmv_go<16, V>(c, rp, col, val, nnz, x, y, r, part, done); break;
-        default: lp_spmv_go<8, V>(c, rp, col, val, nnz, x, y, r, part, done); break;
+        case 2: lp_spmv_go<2, V>(c, rp, col, val, nnz, x, y, r, part, done, coarse); break;
+        case 4: lp_spmv_go<4, V>(c, rp, col, val, nnz, x, y, r, part, done, coarse); break;
+        case 16: lp_spmv_go<16, V>(c, rp, col, val, nnz, x, y, r, part, done, coarse); break;
+        default: lp_spmv_go<8, V>(c, rp, col, val, nnz, x, y, r, part, done, coarse); break;
     }
 }
 
@@ -1159,25 +1171,26 @@ std::vector<uintptr_t> lp_graph_key(const Ctx& c) {
 }
 
 template <class V>
-int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part) {
+int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part, bool coarse) {
     LearnedPrecond& lp = LP(c);
-    // tmp = F r;  z = F^T tmp  (+ partial sums of r.z), F = G S
+    // tmp = F r;  z = F^T tmp (+ P zc, the coarse correction of r already in c.cg.zc)
+    // (+ partial sums of r.z), F = G S
     lp_spmv<V>(c, (const int*)lp.gLrp.p, (const int*)lp.gLcol.p, (const uint32_t*)lp.FL.p, lp.nLow, r, tmp, (const V*)nullptr, nullptr);
-    lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, (const V*)tmp, z, r, part);
+    lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, (const V*)tmp, z, r, part, nullptr, coarse);
     CIPC_LAUNCH_CHECK();
     return c.lpGrid;
 }
-template int lp_apply<double>(Ctx&, const double*, double*, double*, double*);
-template int lp_apply<float>(Ctx&, const float*, float*, float*, double*);
+template int lp_apply<double>(Ctx&, const double*, double*, double*, double*, bool);
+template int lp_apply<float>(Ctx&, const float*, float*, float*, double*, bool);
 
 template <class V>
-void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part, const int* done) {
+void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part, const int* done, bool coarse) {
     LearnedPrecond& lp = LP(c);
-    lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, y, z, r, part, done);
+    lp_spmv<V>(c, (const int*)lp.gUrp.p, (const int*)lp.gUcol.p, (const uint32_t*)lp.FU.p, lp.nLow, y, z, r, part, done, coarse);
     CIPC_LAUNCH_CHECK();
 }
-template void lp_apply_upper<double>(Ctx&, const double*, double*, const double*, double*, const int*);
-template void lp_apply_upper<float>(Ctx&, const float*, float*, const float*, double*, const int*);
+template void lp_apply_upper<double>(Ctx&, const double*, double*, const double*, double*, const int*, bool);
+template void lp_apply_upper<float>(Ctx&, const float*, float*, const float*, double*, const int*, bool);
 
 double lp_loss_grad(Ctx& c, int nProbes, uint64_t seed, float* gradOut) {
     LearnedPrecond& lp = LP(c);

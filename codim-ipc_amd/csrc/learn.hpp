@@ -86,13 +86,14 @@ void lp_prepare(Ctx& c);
 // Hessian (F = G S); false when there is no forward pass to reuse for this pattern
 bool lp_refresh(Ctx& c);
 // z = S G^T G S r (V = double or float vectors); tmp holds 3 nV values; when part !=
-// nullptr the per-workgroup partial sums of r.z are written there (returns their count)
+// nullptr the per-workgroup partial sums of r.z are written there (returns their count).
+// coarse: z also receives P zc, the two-level correction already solved into c.cg.zc
 template <class V>
-int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part);
+int lp_apply(Ctx& c, const V* r, V* z, V* tmp, double* part, bool coarse = false);
 int lp_apply_partials(const Ctx& c);
 // z = F^T y (the second product of lp_apply) with partial sums of r.z
 template <class V>
-void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part, const int* done = nullptr);
+void lp_apply_upper(Ctx& c, const V* y, V* z, const V* r, double* part, const int* done = nullptr, bool coarse = false);
 // the factor products' launch arguments (part of a captured PCG chunk's key)
 std::vector<uintptr_t> lp_graph_key(const Ctx& c);
 // unpack the 9 BF16 entries of a block stored as five words (SoA [5][n]) at position b

@@ -572,6 +572,8 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     // r.z partials: the update kernel's (block Jacobi) or the preconditioner product's
     const double* partRZ = learned ? partP : partU;
     const int nRZ = learned ? nP : nU;
+    // two-level correction of the learned preconditioner (coarse.hip)
+    const bool coarse = learned && c.coarseOn && c.cg.ready;
     V* ltmp = nullptr;
     V* rb[2] = {r, r};                        // learned form: r ping-pongs between r and a second buffer
 
@@ -581,7 +583,8 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
         static_assert(sizeof(double) >= sizeof(V), "");
         ltmp = (V*)c.lpTmp.ensure((size_t)2 * n);
         rb[1] = ltmp + n;
-        lp_apply<V>(c, (const V*)r, z, ltmp, partP);
+        if (coarse) coarse_restrict_solve<V>(c, (const V*)r, nullptr);
+        lp_apply<V>(c, (const V*)r, z, ltmp, partP, coarse);
     }
 
     const int chunk = 16;
@@ -633,7 +636,8 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
             hipLaunchKernelGGL(fused, dim3(nP), dim3(kSpmvBlock), 0, c.s, itd, itArg, nV, rpxL, (const int*)lp.gLrp.p, (const int*)lp.gLcol.p,
                                (const uint32_t*)lp.FL.p, lp.nLow, xsol, (const V*)pnew, (const V*)q, (const V*)rb[it & 1], rNew, ltmp,
                                (const double*)partS, nS, partF, sc, flags);
-            lp_apply_upper<V>(c, (const V*)ltmp, z, (const V*)rNew, partP, flags);
+            if (coarse) coarse_restrict_solve<V>(c, (const V*)rNew, flags);
+            lp_apply_upper<V>(c, (const V*)ltmp, z, (const V*)rNew, partP, flags, coarse);
         }
     };
     int it = 0;
@@ -665,6 +669,14 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
                 LearnedPrecond& lp = *c.lp;
                 const std::vector<uintptr_t> lk = lp_graph_key(c);
                 key.insert(key.end(), lk.begin(), lk.end());
+            }
+            if (coarse) {
+                const CoarseSpace& g = c.cg;
+                key.insert(key.end(), {(uintptr_t)g.nAgg, (uintptr_t)g.npad, (uintptr_t)g.agg.p, (uintptr_t)g.aggPtr.p,
+                                       (uintptr_t)g.aggVerts.p, (uintptr_t)g.d.p, (uintptr_t)g.rc.p, (uintptr_t)g.wc.p, (uintptr_t)g.zc.p,
+                                       (uintptr_t)g.W16.p, (uintptr_t)g.WT16.p});
+            } else {
+                key.push_back(0);
             }
             if (!c.pcgGraph.exec || c.pcgGraph.key != key) {
                 c.pcgGraph.reset();
@@ -872,6 +884,14 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
         lp_prepare(c);
         c.lpSinceForward = 0;
         c.lpStepStart = false;
+    }
+    if (c.coarseOn) {
+        CoarseSpace& g = c.cg;
+        if (g.stepStart || ++g.sinceSetup >= g.every || !g.ready) {
+            coarse_prepare(c, c.solveX ? c.solveX : c.x.p);
+            g.sinceSetup = 0;
+            g.stepStart = false;
+        }
     }
     c.lpSinceForward++;
     c.precondSolves++;

@@ -565,6 +565,7 @@ int advance_one_step(Ctx& c) {
     cipc_step_stats& st = c.stats;
     st = cipc_step_stats{};
     c.lpStepStart = true;
+    c.cg.stepStart = true;
     c.lpOffThisStep = false;
     c.kappaSDoublings = 0;
     double t0 = now_ms(), ta;
@@ -677,6 +678,7 @@ int advance_one_step(Ctx& c) {
                 CIPC_LAUNCH_CHECK();
                 x0 = wx;
             }
+            c.solveX = c.x.p;
             int rc = pcg_solve(c, c.rhs.p, c.p.p, P.pcgTol, P.pcgMaxIter, &it, &rr, x0);
             st.pcg_iters += it;
             st.t_solve_ms += now_ms() - ta;

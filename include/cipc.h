@@ -305,6 +305,22 @@ int cipc_set_preconditioner(cipc_ctx* ctx, int32_t mode, int32_t budget);
  * (>= 1; default 1: at every solve); the solves in between reuse its factor G with the
  * diagonal scaling S of their own Hessian */
 int cipc_set_precond_reuse(cipc_ctx* ctx, int32_t every);
+/* Two-level correction of the learned preconditioner (no counterpart in the reference,
+ * which factorises every Newton matrix, DIRECT_SOLVER.h:14-66): every > 0 makes the
+ * learned solves use M^-1 = F^T F + P X P^T, P = rigid-body modes (three translations,
+ * three rotations about the centroid) on aggregates of the elastic vertex graph
+ * (Dirichlet vertices and vertices without elastic neighbours excluded), X = W^T W with W
+ * the inverse Cholesky factor of A_c = P^T H P rounded to BF16.  A_c is assembled and
+ * factorised densely at the first solve of each step and then every `every` solves
+ * (every = 0: off).  max_unknowns bounds the coarse problem (0: 12288; the aggregates of
+ * a root-and-neighbours pass over the aggregate graph, merged pairwise while above the
+ * bound); with an aggregate of more than 256 neighbours the solve runs without the
+ * coarse level.  Affects the learned forms only (cipc_set_preconditioner mode 1); the
+ * stopping rule is unchanged. */
+int cipc_set_coarse(cipc_ctx* ctx, int32_t every, int32_t max_unknowns);
+/* out (5 entries): aggregates (-1 before the first build), coarse unknowns, coarse
+ * set-ups so far, wall ms of the last set-up, 1 when the last set-up succeeded */
+int cipc_get_coarse_info(const cipc_ctx* ctx, double* out);
 /* Initial guess of the Newton solves (no counterpart in the reference, whose direct solve
  * needs none): enable != 0 (the default) starts a solve that follows a line search cut
  * to alpha < 1 from (1 - alpha) p_prev, the quadratic model's next direction, when that
