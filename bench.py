@@ -137,6 +137,7 @@ def gpu_run(args, sd, form, state):
         sim.set_preconditioner("learned", budget=args.learned_budget, reuse=args.learned_reuse)
         if form.endswith("_coarse"):
             sim.set_coarse(True, every=args.coarse_every, max_unknowns=args.coarse_max)
+            sim.set_preconditioner("learned", budget=args.coarse_budget, reuse=args.learned_reuse)
     if args.autotune:
         sim.autotune()          # settles during the warm-up steps
     tw = time.perf_counter()
@@ -343,8 +344,10 @@ def main():
                     help="headline: fp64, mixed_learned and mixed_learned_coarse; all: the seven forms")
     ap.add_argument("--coarse-max", type=int, default=4000,
                     help="largest coarse problem (unknowns) of the *_coarse forms")
-    ap.add_argument("--coarse-every", type=int, default=1,
+    ap.add_argument("--coarse-every", type=int, default=4,
                     help="Newton solves per coarse set-up of the *_coarse forms (always at a step's first solve)")
+    ap.add_argument("--coarse-budget", type=int, default=1500,
+                    help="PCG iterations after which a learned + coarse solve falls back to block Jacobi")
     ap.add_argument("--learned-budget", type=int, default=2500,
                     help="PCG iterations after which a learned solve falls back to block Jacobi (continuing from its iterate)")
     ap.add_argument("--learned-reuse", type=int, default=None,

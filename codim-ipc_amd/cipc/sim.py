@@ -226,10 +226,10 @@ class Simulator:
         self._check(self.lib.cipc_set_coarse(self.h, int(every) if enable else 0, int(max_unknowns)))
 
     def coarse_info(self):
-        out = np.zeros(5)
+        out = np.zeros(7)
         self._check(self.lib.cipc_get_coarse_info(self.h, _d(out)))
         return dict(aggregates=int(out[0]), unknowns=int(out[1]), setups=int(out[2]), last_setup_ms=float(out[3]),
-                    ready=bool(out[4]))
+                    ready=bool(out[4]), failures=int(out[5]), fp64_retries=int(out[6]))
 
     LINEAR_SOLVER = {"pcg": 0, "direct": 1}
 

@@ -785,6 +785,8 @@ int cipc_get_coarse_info(const cipc_ctx* ctx, double* out) {
     out[2] = (double)g.setups;
     out[3] = g.lastSetupMs;
     out[4] = g.ready ? 1 : 0;
+    out[5] = (double)g.failures;
+    out[6] = (double)ctx->c.fp64Retries;
     return CIPC_OK;
 }
 

@@ -9,10 +9,12 @@
 // which stays symmetric positive definite.  The coarse problem (6 unknowns per aggregate,
 // at most kCoarseMaxDof) is dense: its Cholesky factor L and the explicit inverse L^-1
 // are computed at a step's first solve and then every `every` solves (tiled kernels of
-// direct.hip), and every PCG iteration applies A_c^-1 ~ W^T W, W = L^-1 rounded to BF16,
+// direct.hip), and every PCG iteration applies A_c^-1 ~ W^T W, W = L^-1 rounded to FP32,
 // as two triangular products (FP64 accumulation).  W^T W is positive semidefinite
-// whatever the rounding, so M^-1 stays SPD.  No counterpart in the reference, which factorises H directly
-// (DIRECT_SOLVER.h:14-66).
+// whatever the rounding, so M^-1 stays SPD; the rounding error of W acts like
+// sqrt(cond(A_c)) x the unit roundoff in the energy norm, and contact makes A_c
+// ill-conditioned (with BF16 storage the PCG stalled in deep contact), hence FP32.
+// No counterpart in the reference, which factorises H directly (DIRECT_SOLVER.h:14-66).
 //
 // Aggregates are built once per elastic pattern on the host: greedy root-and-neighbours
 // aggregation of the vertex graph (a vertex all of whose neighbours are free becomes a
@@ -180,8 +182,8 @@ static void coarse_build(Ctx& c) {
     const size_t np2 = (size_t)g.npad * g.npad;
     g.A.ensure_exact(np2);
     g.W.ensure_exact(np2);
-    g.W16.ensure_exact(np2);
-    g.WT16.ensure_exact(np2);
+    g.W32.ensure_exact(np2);
+    g.WT32.ensure_exact(np2);
     g.tileInv.ensure_exact((size_t)(g.npad / kCgTile) * kCgTile * kCgTile);
     g.diag0.ensure_exact(g.npad);
     g.dropped.ensure_exact(g.npad);
@@ -391,30 +393,25 @@ __global__ void k_cg_diag(const double* __restrict__ A, long ld, int npad, doubl
     if (i < npad) diag0[i] = A[(long)i * ld + i];
 }
 
-__device__ __forceinline__ uint16_t cg_bf16(double v) {
-    const uint32_t u = __float_as_uint((float)v);
-    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);   // round to nearest even
-}
-
-// W16 = L^-1 (lower tiles) and WT16 = its transpose (upper tiles), BF16; one workgroup
+// W32 = L^-1 (lower tiles) and WT32 = its transpose (upper tiles), FP32; one workgroup
 // per lower tile, staged through LDS so that both writes are coalesced
-__global__ __launch_bounds__(kCgBlock) void k_cg_to_bf16(const double* __restrict__ W, long ld, uint16_t* __restrict__ W16,
-                                                         uint16_t* __restrict__ WT16) {
-    __shared__ uint16_t T[kCgTile][kCgTile + 2];
+__global__ __launch_bounds__(kCgBlock) void k_cg_to_f32(const double* __restrict__ W, long ld, float* __restrict__ W32,
+                                                        float* __restrict__ WT32) {
+    __shared__ float T[kCgTile][kCgTile + 1];
     int b = blockIdx.x, ti = 0;
     while (b >= ti + 1) { b -= ti + 1; ++ti; }
     const int tj = b;
     const long oi = (long)ti * kCgTile, oj = (long)tj * kCgTile;
     for (int e = threadIdx.x; e < kCgTile * kCgTile; e += kCgBlock) {
         const int r = e / kCgTile, q = e % kCgTile;
-        const uint16_t v = cg_bf16(W[(oi + r) * ld + oj + q]);
+        const float v = (float)W[(oi + r) * ld + oj + q];
         T[r][q] = v;
-        W16[(oi + r) * ld + oj + q] = v;
+        W32[(oi + r) * ld + oj + q] = v;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < kCgTile * kCgTile; e += kCgBlock) {
         const int r = e / kCgTile, q = e % kCgTile;
-        WT16[(oj + r) * ld + oi + q] = T[q][r];
+        WT32[(oj + r) * ld + oi + q] = T[q][r];
     }
 }
 
@@ -451,28 +448,26 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_restrict(int nAgg, const int* _
     }
 }
 
-// y = W x over the lower (W = L^-1) or upper (W = L^-T) triangle of a row-major BF16
-// matrix; one wave per row, eight entries (16 bytes) per lane load, FP64 accumulation
+// y = W x over the lower (W = L^-1) or upper (W = L^-T) triangle of a row-major FP32
+// matrix; one wave per row, four entries (16 bytes) per lane load, FP64 accumulation
 template <bool UPPER>
-__global__ __launch_bounds__(kCgBlock) void k_cg_gemv(const uint16_t* __restrict__ W, int npad, const double* __restrict__ x,
+__global__ __launch_bounds__(kCgBlock) void k_cg_gemv(const float* __restrict__ W, int npad, const double* __restrict__ x,
                                                       double* __restrict__ y, const int* __restrict__ done) {
     if (done && *done) return;
     const int row = blockIdx.x * (kCgBlock / 64) + threadIdx.x / 64, lane = threadIdx.x & 63;
     if (row >= npad) return;
-    const uint4* w = reinterpret_cast<const uint4*>(W + (long)row * npad);
+    const float4* w = reinterpret_cast<const float4*>(W + (long)row * npad);
     double acc = 0;
-    // whole 8-entry groups; the row's first / last group is masked to the triangle
-    const int g0 = (UPPER ? row : 0) >> 3, g1 = ((UPPER ? npad - 1 : row) >> 3) + 1;
+    // whole 4-entry groups; the row's first / last group is masked to the triangle
+    const int g0 = (UPPER ? row : 0) >> 2, g1 = ((UPPER ? npad - 1 : row) >> 2) + 1;
     for (int g = g0 + lane; g < g1; g += 64) {
-        const uint4 wv = w[g];
-        const uint32_t words[4] = {wv.x, wv.y, wv.z, wv.w};
-        const int j = 8 * g;
+        const float4 wv = w[g];
+        const float e[4] = {wv.x, wv.y, wv.z, wv.w};
+        const int j = 4 * g;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t wd = words[k >> 1];
-            const float e = __uint_as_float((k & 1) ? (wd & 0xFFFF0000u) : (wd << 16));
+        for (int k = 0; k < 4; ++k) {
             const bool in = UPPER ? (j + k >= row) : (j + k <= row);
-            if (in) acc += (double)e * x[j + k];
+            if (in) acc += (double)e[k] * x[j + k];
         }
     }
     acc = wave_sum(acc);
@@ -509,13 +504,16 @@ bool coarse_prepare(Ctx& c, const double* x) {
     dense_cholesky(c, g.A.p, g.npad, g.flag.p, g.tileInv.p, g.diag0.p, g.dropped.p, 1e-9);
     dense_lower_inverse(c, g.A.p, g.W.p, g.npad, g.tileInv.p);
     const int nt = g.npad / kCgTile;
-    hipLaunchKernelGGL(k_cg_to_bf16, dim3(nt * (nt + 1) / 2), dim3(kCgBlock), 0, c.s, (const double*)g.W.p, ld, g.W16.p, g.WT16.p);
+    hipLaunchKernelGGL(k_cg_to_f32, dim3(nt * (nt + 1) / 2), dim3(kCgBlock), 0, c.s, (const double*)g.W.p, ld, g.W32.p, g.WT32.p);
     CIPC_LAUNCH_CHECK();
     CIPC_HIP(hipMemcpyAsync(c.hCnt + 49, g.flag.p, sizeof(int), hipMemcpyDeviceToHost, c.s));
     CIPC_HIP(hipStreamSynchronize(c.s));
     g.setups++;
     g.lastSetupMs = wall_ms() - t0;
-    if (c.hCnt[49] & 1) return false;   // an aggregate with too many neighbours: no coarse level this time
+    if (c.hCnt[49] & 1) {   // an aggregate with too many neighbours: no coarse level this time
+        g.failures++;
+        return false;
+    }
     g.ready = true;
     return true;
 }
@@ -525,9 +523,9 @@ void coarse_restrict_solve(Ctx& c, const V* r, const int* done) {
     CoarseSpace& g = c.cg;
     hipLaunchKernelGGL((k_cg_restrict<V>), dim3(g.nAgg), dim3(kCgBlock), 0, c.s, g.nAgg, (const int*)g.aggPtr.p,
                        (const int*)g.aggVerts.p, (const double*)g.d.p, r, g.rc.p, done);
-    hipLaunchKernelGGL(k_cg_gemv<false>, dim3(nblocks(g.npad, kCgBlock / 64)), dim3(kCgBlock), 0, c.s, (const uint16_t*)g.W16.p, g.npad,
+    hipLaunchKernelGGL(k_cg_gemv<false>, dim3(nblocks(g.npad, kCgBlock / 64)), dim3(kCgBlock), 0, c.s, (const float*)g.W32.p, g.npad,
                        (const double*)g.rc.p, g.wc.p, done);
-    hipLaunchKernelGGL(k_cg_gemv<true>, dim3(nblocks(g.npad, kCgBlock / 64)), dim3(kCgBlock), 0, c.s, (const uint16_t*)g.WT16.p, g.npad,
+    hipLaunchKernelGGL(k_cg_gemv<true>, dim3(nblocks(g.npad, kCgBlock / 64)), dim3(kCgBlock), 0, c.s, (const float*)g.WT32.p, g.npad,
                        (const double*)g.wc.p, g.zc.p, done);
     CIPC_LAUNCH_CHECK();
 }

@@ -70,7 +70,7 @@ struct CoarseSpace {
     DevBuf<double> W;                     // dense work: L^-1 (npad^2)
     DevBuf<double> diag0, tileInv;        // coarse diagonal before factorisation; inverses of the 64x64 diagonal tiles
     DevBuf<int> dropped;                  // coarse unknowns whose pivot vanished (degenerate modes)
-    DevBuf<uint16_t> W16, WT16;           // L^-1 and its transpose in BF16, row-major npad^2
+    DevBuf<float> W32, WT32;              // L^-1 and its transpose in FP32, row-major npad^2
     DevBuf<double> rc, wc, zc;            // coarse vectors
     DevBuf<int> flag;                     // Galerkin overflow flag
     int maxDof = 0;                       // largest coarse problem asked for (0: kCoarseMaxDof)
@@ -78,7 +78,7 @@ struct CoarseSpace {
     int every = 1, sinceSetup = 0;        // set-up at a step's first solve and then every `every` solves
     bool stepStart = true;
     double lastSetupMs = 0;
-    long setups = 0;
+    long setups = 0, failures = 0;        // set-ups, and those that found an aggregate with too many neighbours
 };
 
 // largest system (unknowns) of the dense direct solve: 24576^2 FP64 = 4.5 GiB of matrix
@@ -274,6 +274,7 @@ struct Ctx {
     // two-level correction of the learned preconditioner (cipc_set_coarse)
     bool coarseOn = false;
     CoarseSpace cg;
+    long fp64Retries = 0;                   // reduced-precision learned + coarse solves redone with FP64 blocks
     const double* solveX = nullptr;         // positions of the Newton iterate being solved (rotation modes)
     // dense direct solve: padded row-major matrix / factor, right-hand side, pivot flag
     DevBuf<double> denseA, denseY;

@@ -674,7 +674,7 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
                 const CoarseSpace& g = c.cg;
                 key.insert(key.end(), {(uintptr_t)g.nAgg, (uintptr_t)g.npad, (uintptr_t)g.agg.p, (uintptr_t)g.aggPtr.p,
                                        (uintptr_t)g.aggVerts.p, (uintptr_t)g.d.p, (uintptr_t)g.rc.p, (uintptr_t)g.wc.p, (uintptr_t)g.zc.p,
-                                       (uintptr_t)g.W16.p, (uintptr_t)g.WT16.p});
+                                       (uintptr_t)g.W32.p, (uintptr_t)g.WT32.p});
             } else {
                 key.push_back(0);
             }
@@ -807,19 +807,21 @@ static int refine(Ctx& c, const M* vE, const M* vK, const V* dinv, const double*
         int rc = pcg_run<M, V>(c, vE, vK, rhs, d, wr, wz, wp, wq, dinv, inner, std::max(1, maxIter - total), &it, &rr, learned);
         total += it;
         c.lastRefinements = k + 1;
-        if (rc == 1) {
-            if (iters) *iters = total;
-            return 1;
-        }
+        // A breakdown (p.q <= 0) of the reduced-precision inner solve keeps its iterate up
+        // to the last good iteration: the FP32 blocks' rounding can make a direction that H
+        // barely resists indefinite, which says nothing about H itself.  The FP64
+        // refinement goes on from the true residual while the sweeps make progress.
         if constexpr (sizeof(V) == sizeof(double))
             hipLaunchKernelGGL(k_acc_f64, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const double*)d, xsol);
         else
             hipLaunchKernelGGL(k_acc_f32, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const float*)d, xsol);
         CIPC_LAUNCH_CHECK();
+        const double rprev = rnorm;
         rnorm = std::sqrt(residual_norm2(c, b, xsol, r));
         if (relres) *relres = rnorm / bnorm;
         if (iters) *iters = total;
         if (rnorm <= tol * bnorm) return 0;
+        if (rc == 1 && !(rnorm < 0.9 * rprev)) return 1;
         if (total >= maxIter) return 2;
     }
     return 2;
@@ -898,7 +900,9 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
     int it = 0;
     double rr = 0;
     int budget = std::min(maxIter, c.lpBudget);
-    if (c.lastJacobiIters > 0) budget = std::min(budget, std::max(200, c.lastJacobiIters + c.lastJacobiIters / 4));
+    // (with the coarse level the learned solve is far stronger than block Jacobi: no cap)
+    if (c.lastJacobiIters > 0 && !(c.coarseOn && c.cg.ready))
+        budget = std::min(budget, std::max(200, c.lastJacobiIters + c.lastJacobiIters / 4));
     int rc = solve_form(c, b, xsol, tol, budget, &it, &rr, true);
     c.lastLearnedRc = rc;
     c.lastLearnedIters = it;
@@ -907,9 +911,50 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
         if (relres) *relres = rr;
         return 0;
     }
+    // With the coarse level, a reduced-precision solve that fails is retried with FP64
+    // blocks and the same preconditioner, from the learned iterate: on the hardest contact
+    // systems the FP32 blocks' rounding (about 6e-8 relative) stalls the refinement, while
+    // the two-level preconditioner itself still converges in a few hundred iterations.
+    if (c.coarseOn && c.cg.ready && c.P.precision != 0) {
+        const long n = 3l * c.nV;
+        double* xl = c.fallbackX.ensure(n);
+        double* r0 = c.fallbackR.ensure(n);
+        CIPC_HIP(hipMemcpyAsync(xl, xsol, n * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+        const double bn2 = norm2(c, b, n);
+        const double rn2 = residual_norm2(c, b, xl, r0);
+        const bool from = rn2 > 0 && rn2 < bn2;
+        const int saved = c.P.precision;
+        c.P.precision = 0;
+        int it3 = 0;
+        double rr3 = 0;
+        int rc3;
+        try {
+            rc3 = from ? solve_form(c, r0, xsol, tol * std::sqrt(bn2 / rn2), budget, &it3, &rr3, true)
+                       : solve_form(c, b, xsol, tol, budget, &it3, &rr3, true);
+        } catch (...) {
+            c.P.precision = saved;
+            throw;
+        }
+        c.P.precision = saved;
+        c.fp64Retries++;
+        if (from) {
+            hipLaunchKernelGGL(k_acc_f64, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const double*)xl, xsol);
+            CIPC_LAUNCH_CHECK();
+            rr3 *= std::sqrt(rn2 / bn2);
+        }
+        it += it3;
+        if (rc3 == 0) {
+            if (iters) *iters = it;
+            if (relres) *relres = rr3;
+            return 0;
+        }
+        rc = rc3;
+    }
     c.lastPrecondFallback = 1;
     c.precondFallbacks++;
-    c.lpOffThisStep = true;
+    // the rest of the step stays on block Jacobi, unless the coarse level is on: then the
+    // hard solve is an exception and the next ones are learned again
+    if (!(c.coarseOn && c.cg.ready)) c.lpOffThisStep = true;
     int it2 = 0;
     if (rc == 2) {
         const long n = 3l * c.nV;

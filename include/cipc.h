@@ -318,8 +318,11 @@ int cipc_set_precond_reuse(cipc_ctx* ctx, int32_t every);
  * coarse level.  Affects the learned forms only (cipc_set_preconditioner mode 1); the
  * stopping rule is unchanged. */
 int cipc_set_coarse(cipc_ctx* ctx, int32_t every, int32_t max_unknowns);
-/* out (5 entries): aggregates (-1 before the first build), coarse unknowns, coarse
- * set-ups so far, wall ms of the last set-up, 1 when the last set-up succeeded */
+/* out (7 entries): aggregates (-1 before the first build), coarse unknowns, coarse
+ * set-ups so far, wall ms of the last set-up, 1 when the last set-up succeeded, set-ups
+ * that found an aggregate with more than 256 neighbours (solved without the coarse level),
+ * reduced-precision solves that failed with the coarse level and were redone with FP64
+ * blocks (same preconditioner, from their iterate; block Jacobi only if that fails too) */
 int cipc_get_coarse_info(const cipc_ctx* ctx, double* out);
 /* Initial guess of the Newton solves (no counterpart in the reference, whose direct solve
  * needs none): enable != 0 (the default) starts a solve that follows a line search cut
