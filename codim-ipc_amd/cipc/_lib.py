@@ -170,7 +170,7 @@ PROTOTYPES = {
     "cipc_set_precision": (ctypes.c_int, [P, ctypes.c_int32]),
     "cipc_get_solver_info": (ctypes.c_int, [P, I32]),
     "cipc_set_linear_solver": (ctypes.c_int, [P, ctypes.c_int32]),
-    "cipc_set_coarse": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32]),
+    "cipc_set_coarse": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "cipc_get_coarse_info": (ctypes.c_int, [P, D]),
     "cipc_precond_num_params": (ctypes.c_int32, []),
     "cipc_precond_init": (ctypes.c_int, [P, ctypes.c_uint64]),

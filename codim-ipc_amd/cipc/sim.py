@@ -218,12 +218,13 @@ class Simulator:
         return dict(refinements=int(out[0]), graph_captures=int(out[1]), graph_launches=int(out[2]),
                     direct_solves=int(out[3]))
 
-    def set_coarse(self, enable=True, every=1, max_unknowns=0):
+    def set_coarse(self, enable=True, every=1, max_unknowns=0, bf16=False):
         """two-level correction of the learned preconditioner: rigid-body modes on
         aggregates of the elastic vertex graph, Galerkin coarse matrix set up at a step's
         first solve and then every `every` solves, at most `max_unknowns` coarse unknowns
-        (0: the library's bound) (cipc_set_coarse; learned forms only)"""
-        self._check(self.lib.cipc_set_coarse(self.h, int(every) if enable else 0, int(max_unknowns)))
+        (0: the library's bound), inverse factor in FP32 or BF16 (cipc_set_coarse; learned
+        forms only)"""
+        self._check(self.lib.cipc_set_coarse(self.h, int(every) if enable else 0, int(max_unknowns), 1 if bf16 else 0))
 
     def coarse_info(self):
         out = np.zeros(7)

@@ -768,12 +768,13 @@ int cipc_get_solver_info(const cipc_ctx* ctx, int32_t* out) {
     return CIPC_OK;
 }
 
-int cipc_set_coarse(cipc_ctx* ctx, int32_t every, int32_t max_unknowns) {
-    if (!ctx || every < 0 || max_unknowns < 0) return CIPC_E_INVALID;
+int cipc_set_coarse(cipc_ctx* ctx, int32_t every, int32_t max_unknowns, int32_t flags) {
+    if (!ctx || every < 0 || max_unknowns < 0 || (flags & ~1)) return CIPC_E_INVALID;
     Ctx& c = ctx->c;
     c.coarseOn = every > 0;
     c.cg.every = every > 0 ? every : 1;
     c.cg.maxDof = max_unknowns;
+    c.cg.bf16 = (flags & 1) != 0;
     c.cg.ready = false;
     return CIPC_OK;
 }

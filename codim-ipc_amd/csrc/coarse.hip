@@ -182,8 +182,13 @@ static void coarse_build(Ctx& c) {
     const size_t np2 = (size_t)g.npad * g.npad;
     g.A.ensure_exact(np2);
     g.W.ensure_exact(np2);
-    g.W32.ensure_exact(np2);
-    g.WT32.ensure_exact(np2);
+    if (g.bf16) {
+        g.W16.ensure_exact(np2);
+        g.WT16.ensure_exact(np2);
+    } else {
+        g.W32.ensure_exact(np2);
+        g.WT32.ensure_exact(np2);
+    }
     g.tileInv.ensure_exact((size_t)(g.npad / kCgTile) * kCgTile * kCgTile);
     g.diag0.ensure_exact(g.npad);
     g.dropped.ensure_exact(g.npad);
@@ -393,25 +398,32 @@ __global__ void k_cg_diag(const double* __restrict__ A, long ld, int npad, doubl
     if (i < npad) diag0[i] = A[(long)i * ld + i];
 }
 
-// W32 = L^-1 (lower tiles) and WT32 = its transpose (upper tiles), FP32; one workgroup
-// per lower tile, staged through LDS so that both writes are coalesced
-__global__ __launch_bounds__(kCgBlock) void k_cg_to_f32(const double* __restrict__ W, long ld, float* __restrict__ W32,
-                                                        float* __restrict__ WT32) {
-    __shared__ float T[kCgTile][kCgTile + 1];
+__device__ __forceinline__ float cg_store(double v, float*) { return (float)v; }
+__device__ __forceinline__ uint16_t cg_store(double v, uint16_t*) {
+    const uint32_t u = __float_as_uint((float)v);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);   // BF16, round to nearest even
+}
+
+// W = L^-1 (lower tiles) and WT = its transpose (upper tiles) in the storage type S (FP32
+// or BF16); one workgroup per lower tile, staged through LDS so that both writes are
+// coalesced
+template <class S>
+__global__ __launch_bounds__(kCgBlock) void k_cg_store(const double* __restrict__ W, long ld, S* __restrict__ Ws, S* __restrict__ WTs) {
+    __shared__ S T[kCgTile][kCgTile + 2];
     int b = blockIdx.x, ti = 0;
     while (b >= ti + 1) { b -= ti + 1; ++ti; }
     const int tj = b;
     const long oi = (long)ti * kCgTile, oj = (long)tj * kCgTile;
     for (int e = threadIdx.x; e < kCgTile * kCgTile; e += kCgBlock) {
         const int r = e / kCgTile, q = e % kCgTile;
-        const float v = (float)W[(oi + r) * ld + oj + q];
+        const S v = cg_store(W[(oi + r) * ld + oj + q], (S*)nullptr);
         T[r][q] = v;
-        W32[(oi + r) * ld + oj + q] = v;
+        Ws[(oi + r) * ld + oj + q] = v;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < kCgTile * kCgTile; e += kCgBlock) {
         const int r = e / kCgTile, q = e % kCgTile;
-        WT32[(oj + r) * ld + oi + q] = T[q][r];
+        WTs[(oj + r) * ld + oi + q] = T[q][r];
     }
 }
 
@@ -448,30 +460,53 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_restrict(int nAgg, const int* _
     }
 }
 
-// y = W x over the lower (W = L^-1) or upper (W = L^-T) triangle of a row-major FP32
-// matrix; one wave per row, four entries (16 bytes) per lane load, FP64 accumulation
-template <bool UPPER>
-__global__ __launch_bounds__(kCgBlock) void k_cg_gemv(const float* __restrict__ W, int npad, const double* __restrict__ x,
+// 16 bytes of a row as doubles: four FP32 or eight BF16 entries
+__device__ __forceinline__ void cg_unpack(const float4& w, const float*, double* e) {
+    e[0] = w.x; e[1] = w.y; e[2] = w.z; e[3] = w.w;
+}
+__device__ __forceinline__ void cg_unpack(const float4& w, const uint16_t*, double* e) {
+    const uint32_t words[4] = {__float_as_uint(w.x), __float_as_uint(w.y), __float_as_uint(w.z), __float_as_uint(w.w)};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[k] = (double)__uint_as_float((k & 1) ? (words[k >> 1] & 0xFFFF0000u) : (words[k >> 1] << 16));
+}
+
+// y = W x over the lower (W = L^-1) or upper (W = L^-T) triangle of a row-major matrix of
+// FP32 or BF16 entries, FP64 accumulation.  One workgroup per pair of rows (i, npad-1-i),
+// whose triangle parts together hold npad + 1 entries, so every workgroup streams the
+// same amount; 16 bytes per lane load; fixed-order reduction.
+template <class S, bool UPPER>
+__global__ __launch_bounds__(kCgBlock) void k_cg_gemv(const S* __restrict__ W, int npad, const double* __restrict__ x,
                                                       double* __restrict__ y, const int* __restrict__ done) {
     if (done && *done) return;
-    const int row = blockIdx.x * (kCgBlock / 64) + threadIdx.x / 64, lane = threadIdx.x & 63;
-    if (row >= npad) return;
-    const float4* w = reinterpret_cast<const float4*>(W + (long)row * npad);
-    double acc = 0;
-    // whole 4-entry groups; the row's first / last group is masked to the triangle
-    const int g0 = (UPPER ? row : 0) >> 2, g1 = ((UPPER ? npad - 1 : row) >> 2) + 1;
-    for (int g = g0 + lane; g < g1; g += 64) {
-        const float4 wv = w[g];
-        const float e[4] = {wv.x, wv.y, wv.z, wv.w};
-        const int j = 4 * g;
+    constexpr int E = 16 / sizeof(S);   // entries per 16-byte group
+    __shared__ double part[2][kCgBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const bool in = UPPER ? (j + k >= row) : (j + k <= row);
-            if (in) acc += (double)e[k] * x[j + k];
+    for (int h = 0; h < 2; ++h) {
+        const int row = h == 0 ? (int)blockIdx.x : npad - 1 - (int)blockIdx.x;
+        const float4* w = reinterpret_cast<const float4*>(W + (long)row * npad);
+        double acc = 0;
+        // whole groups; the row's first / last group is masked to the triangle
+        const int g0 = (UPPER ? row : 0) / E, g1 = (UPPER ? npad - 1 : row) / E + 1;
+        for (int g = g0 + (int)threadIdx.x; g < g1; g += kCgBlock) {
+            double e[E];
+            cg_unpack(w[g], (const S*)nullptr, e);
+            const int j = E * g;
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const bool in = UPPER ? (j + k >= row) : (j + k <= row);
+                if (in) acc += e[k] * x[j + k];
+            }
         }
+        acc = wave_sum(acc);
+        if (lane == 0) part[h][wv] = acc;
     }
-    acc = wave_sum(acc);
-    if (lane == 0) y[row] = acc;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int h = threadIdx.x;
+        const int row = h == 0 ? (int)blockIdx.x : npad - 1 - (int)blockIdx.x;
+        y[row] = (part[h][0] + part[h][1]) + (part[h][2] + part[h][3]);
+    }
 }
 
 static double wall_ms() {
@@ -504,7 +539,17 @@ bool coarse_prepare(Ctx& c, const double* x) {
     dense_cholesky(c, g.A.p, g.npad, g.flag.p, g.tileInv.p, g.diag0.p, g.dropped.p, 1e-9);
     dense_lower_inverse(c, g.A.p, g.W.p, g.npad, g.tileInv.p);
     const int nt = g.npad / kCgTile;
-    hipLaunchKernelGGL(k_cg_to_f32, dim3(nt * (nt + 1) / 2), dim3(kCgBlock), 0, c.s, (const double*)g.W.p, ld, g.W32.p, g.WT32.p);
+    if (g.bf16) {
+        g.W16.ensure_exact((size_t)g.npad * g.npad);
+        g.WT16.ensure_exact((size_t)g.npad * g.npad);
+        hipLaunchKernelGGL(k_cg_store<uint16_t>, dim3(nt * (nt + 1) / 2), dim3(kCgBlock), 0, c.s, (const double*)g.W.p, ld, g.W16.p,
+                           g.WT16.p);
+    } else {
+        g.W32.ensure_exact((size_t)g.npad * g.npad);
+        g.WT32.ensure_exact((size_t)g.npad * g.npad);
+        hipLaunchKernelGGL(k_cg_store<float>, dim3(nt * (nt + 1) / 2), dim3(kCgBlock), 0, c.s, (const double*)g.W.p, ld, g.W32.p,
+                           g.WT32.p);
+    }
     CIPC_LAUNCH_CHECK();
     CIPC_HIP(hipMemcpyAsync(c.hCnt + 49, g.flag.p, sizeof(int), hipMemcpyDeviceToHost, c.s));
     CIPC_HIP(hipStreamSynchronize(c.s));
@@ -523,10 +568,17 @@ void coarse_restrict_solve(Ctx& c, const V* r, const int* done) {
     CoarseSpace& g = c.cg;
     hipLaunchKernelGGL((k_cg_restrict<V>), dim3(g.nAgg), dim3(kCgBlock), 0, c.s, g.nAgg, (const int*)g.aggPtr.p,
                        (const int*)g.aggVerts.p, (const double*)g.d.p, r, g.rc.p, done);
-    hipLaunchKernelGGL(k_cg_gemv<false>, dim3(nblocks(g.npad, kCgBlock / 64)), dim3(kCgBlock), 0, c.s, (const float*)g.W32.p, g.npad,
-                       (const double*)g.rc.p, g.wc.p, done);
-    hipLaunchKernelGGL(k_cg_gemv<true>, dim3(nblocks(g.npad, kCgBlock / 64)), dim3(kCgBlock), 0, c.s, (const float*)g.WT32.p, g.npad,
-                       (const double*)g.wc.p, g.zc.p, done);
+    if (g.bf16) {
+        hipLaunchKernelGGL((k_cg_gemv<uint16_t, false>), dim3(g.npad / 2), dim3(kCgBlock), 0, c.s, (const uint16_t*)g.W16.p, g.npad,
+                           (const double*)g.rc.p, g.wc.p, done);
+        hipLaunchKernelGGL((k_cg_gemv<uint16_t, true>), dim3(g.npad / 2), dim3(kCgBlock), 0, c.s, (const uint16_t*)g.WT16.p, g.npad,
+                           (const double*)g.wc.p, g.zc.p, done);
+    } else {
+        hipLaunchKernelGGL((k_cg_gemv<float, false>), dim3(g.npad / 2), dim3(kCgBlock), 0, c.s, (const float*)g.W32.p, g.npad,
+                           (const double*)g.rc.p, g.wc.p, done);
+        hipLaunchKernelGGL((k_cg_gemv<float, true>), dim3(g.npad / 2), dim3(kCgBlock), 0, c.s, (const float*)g.WT32.p, g.npad,
+                           (const double*)g.wc.p, g.zc.p, done);
+    }
     CIPC_LAUNCH_CHECK();
 }
 template void coarse_restrict_solve<double>(Ctx&, const double*, const int*);

@@ -71,6 +71,8 @@ struct CoarseSpace {
     DevBuf<double> diag0, tileInv;        // coarse diagonal before factorisation; inverses of the 64x64 diagonal tiles
     DevBuf<int> dropped;                  // coarse unknowns whose pivot vanished (degenerate modes)
     DevBuf<float> W32, WT32;              // L^-1 and its transpose in FP32, row-major npad^2
+    DevBuf<uint16_t> W16, WT16;           // the same in BF16 (bf16 storage)
+    bool bf16 = false;
     DevBuf<double> rc, wc, zc;            // coarse vectors
     DevBuf<int> flag;                     // Galerkin overflow flag
     int maxDof = 0;                       // largest coarse problem asked for (0: kCoarseMaxDof)

@@ -40,61 +40,83 @@ __global__ void k_dense_scatter(const int* __restrict__ rowOf, const int* __rest
         }
 }
 
-// Cholesky of diagonal tile k in LDS (lower, in place), one wave: thread i owns row i of
-// the tile.  A non-positive pivot sets *bad and continues with pivot 1 so that the
-// factorisation stays finite.  With diag0, a pivot at or below tau x diag0 (the unknown's
-// diagonal before elimination: a mode the matrix does not see) is dropped instead: a
-// pivot at the unknown's own scale, zero column, dropped[] = 1.  With inv, the inverse of
-// the factored tile is written there too (column c by forward substitution, thread c).
+// Cholesky of diagonal tile k (lower, in place), one wave: thread i keeps row i of the
+// tile in registers, so the trailing updates are register FMAs against a broadcast copy of
+// the current column, whose LDS reads carry no condition and are issued together (a single
+// wave has nothing else to hide their latency behind).  A non-positive pivot sets *bad and
+// continues with pivot 1 so that the factorisation stays finite.  With diag0, a pivot at
+// or below tau x diag0 (the unknown's diagonal before elimination: a mode the matrix does
+// not see) is dropped instead: a pivot at the unknown's own scale, zero column,
+// dropped[] = 1.  With inv, the inverse of the factored tile is written there too (column
+// c by forward substitution in thread c's registers).
 __global__ __launch_bounds__(kDT) void k_potrf_tile(double* __restrict__ a, long ld, int k, int* __restrict__ bad,
                                                     const double* __restrict__ diag0, int* __restrict__ dropped, double tau,
                                                     double* __restrict__ inv) {
     __shared__ double T[kDT][kDT + 1];
+    __shared__ double colj[kDT];
+    __shared__ double piv;
     __shared__ int drop;
     const long o = (long)k * kDT;
     const int i = threadIdx.x;
+    // the pivot of column j is thread j's: its original diagonal is loaded up front
+    const double myDiag0 = diag0 ? diag0[o + i] : 0.0;
     for (int r = 0; r < kDT; ++r) T[r][i] = a[(o + r) * ld + o + i];
     __syncthreads();
+    double row[kDT];   // entries right of the diagonal hold garbage and are never stored
+#pragma unroll
+    for (int l = 0; l < kDT; ++l) row[l] = T[i][l];
+    __syncthreads();
     for (int j = 0; j < kDT; ++j) {
+        double lij = 0.0;
+#pragma unroll
+        for (int l = 0; l < kDT; ++l) lij = l == j ? row[l] : lij;
         if (i == j) {
-            double d = T[j][j];
+            double d = lij;
             int dr = 0;
             if (diag0) {
-                if (!(d > tau * diag0[o + j])) {
+                if (!(d > tau * myDiag0)) {
                     dr = 1;
-                    d = diag0[o + j] > 0 ? diag0[o + j] : 1.0;
+                    d = myDiag0 > 0 ? myDiag0 : 1.0;
                 }
                 dropped[o + j] = dr;
             } else if (!(d > 0)) {
                 atomicOr(bad, 1);
                 d = 1.0;
             }
-            T[j][j] = sqrt(d);
+            piv = sqrt(d);
             drop = dr;
         }
         __syncthreads();
+        const double pv = piv;
         const bool dr = drop;
-        if (i > j) T[i][j] = dr ? 0.0 : T[i][j] / T[j][j];
+        if (i == j) lij = pv;
+        else if (i > j) lij = dr ? 0.0 : lij / pv;
+        else lij = 0.0;
+        T[i][j] = lij;          // column j of L (zero above the diagonal)
+        colj[i] = i > j ? lij : 0.0;
         __syncthreads();
-        if (!dr && i > j) {
-            const double lij = T[i][j];
-            for (int l = j + 1; l <= i; ++l) T[i][l] -= lij * T[l][j];
+        // row[l] -= L_ij L_lj: colj is zero for l <= j, and entries right of the diagonal
+        // are never read, so the loop needs no condition
+        if (i > j) {
+#pragma unroll
+            for (int l = 0; l < kDT; ++l) row[l] = fma(-lij, colj[l], row[l]);
         }
         __syncthreads();
     }
-    for (int r = 0; r < kDT; ++r) a[(o + r) * ld + o + i] = i <= r ? T[r][i] : 0.0;
+    for (int r = 0; r < kDT; ++r) a[(o + r) * ld + o + i] = T[r][i];
     if (inv) {
-        // column i of the inverse: X[r][i] = (delta_ri - sum_{i <= m < r} L[r][m] X[m][i]) / L[r][r]
-        double* dst = inv + (long)k * kDT * kDT;
-        __syncthreads();
+        // column i of the inverse: X[r] = (delta_ri - sum_{m < r} L[r][m] X[m]) / L[r][r],
+        // X[m] = 0 for m < i; every thread walks the same (r, m) sequence, so the L reads
+        // are broadcasts
         double x[kDT];
 #pragma unroll
         for (int r = 0; r < kDT; ++r) {
-            double s = r == i ? 1.0 : 0.0;
+            double sacc = r == i ? 1.0 : 0.0;
 #pragma unroll
-            for (int m = 0; m < r; ++m) s -= (m >= i) ? T[r][m] * x[m] : 0.0;
-            x[r] = r < i ? 0.0 : s / T[r][r];
+            for (int m = 0; m < r; ++m) sacc = fma(-T[r][m], x[m], sacc);
+            x[r] = r < i ? 0.0 : sacc / T[r][r];
         }
+        double* dst = inv + (long)k * kDT * kDT;
 #pragma unroll
         for (int r = 0; r < kDT; ++r) dst[r * kDT + i] = x[r];
     }

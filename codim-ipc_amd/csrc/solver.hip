@@ -674,7 +674,8 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
                 const CoarseSpace& g = c.cg;
                 key.insert(key.end(), {(uintptr_t)g.nAgg, (uintptr_t)g.npad, (uintptr_t)g.agg.p, (uintptr_t)g.aggPtr.p,
                                        (uintptr_t)g.aggVerts.p, (uintptr_t)g.d.p, (uintptr_t)g.rc.p, (uintptr_t)g.wc.p, (uintptr_t)g.zc.p,
-                                       (uintptr_t)g.W32.p, (uintptr_t)g.WT32.p});
+                                       (uintptr_t)g.W32.p, (uintptr_t)g.WT32.p, (uintptr_t)g.W16.p, (uintptr_t)g.WT16.p,
+                                       (uintptr_t)g.bf16});
             } else {
                 key.push_back(0);
             }
@@ -774,6 +775,9 @@ __global__ void k_acc_f64(long n, const double* __restrict__ d, double* __restri
 template <class M, class V>
 static int refine(Ctx& c, const M* vE, const M* vK, const V* dinv, const double* b, double* xsol, double tol, int maxIter, int* iters,
                   double* relres, V* wr, V* wz, V* wp, V* wq, V* d, V* rb, bool learned) {
+    // with the coarse level a stalled refinement returns at once (2): the caller redoes the
+    // solve with FP64 blocks instead of spending the rest of the budget
+    const bool stallExit = learned && c.coarseOn && c.cg.ready;
     const long n = 3l * c.nV;
     double* r = c.rres.ensure(n);
     CIPC_HIP(hipMemsetAsync(xsol, 0, n * sizeof(double), c.s));
@@ -822,6 +826,9 @@ static int refine(Ctx& c, const M* vE, const M* vK, const V* dinv, const double*
         if (iters) *iters = total;
         if (rnorm <= tol * bnorm) return 0;
         if (rc == 1 && !(rnorm < 0.9 * rprev)) return 1;
+        // a sweep after the first that gains less than 4x: the FP32 blocks' rounding holds
+        // the refinement back
+        if (stallExit && k >= 1 && !(rnorm < 0.25 * rprev)) return 2;
         if (total >= maxIter) return 2;
     }
     return 2;

@@ -310,14 +310,16 @@ int cipc_set_precond_reuse(cipc_ctx* ctx, int32_t every);
  * learned solves use M^-1 = F^T F + P X P^T, P = rigid-body modes (three translations,
  * three rotations about the centroid) on aggregates of the elastic vertex graph
  * (Dirichlet vertices and vertices without elastic neighbours excluded), X = W^T W with W
- * the inverse Cholesky factor of A_c = P^T H P rounded to BF16.  A_c is assembled and
+ * the inverse Cholesky factor of A_c = P^T H P rounded to FP32.  A_c is assembled and
  * factorised densely at the first solve of each step and then every `every` solves
  * (every = 0: off).  max_unknowns bounds the coarse problem (0: 12288; the aggregates of
  * a root-and-neighbours pass over the aggregate graph, merged pairwise while above the
  * bound); with an aggregate of more than 256 neighbours the solve runs without the
- * coarse level.  Affects the learned forms only (cipc_set_preconditioner mode 1); the
- * stopping rule is unchanged. */
-int cipc_set_coarse(cipc_ctx* ctx, int32_t every, int32_t max_unknowns);
+ * coarse level.  flags bit 0: W stored in BF16 instead of FP32 (half the bytes per
+ * iteration; X stays positive semidefinite, its error grows with sqrt(cond(A_c))).
+ * Affects the learned forms only (cipc_set_preconditioner mode 1); the stopping rule is
+ * unchanged. */
+int cipc_set_coarse(cipc_ctx* ctx, int32_t every, int32_t max_unknowns, int32_t flags);
 /* out (7 entries): aggregates (-1 before the first build), coarse unknowns, coarse
  * set-ups so far, wall ms of the last set-up, 1 when the last set-up succeeded, set-ups
  * that found an aggregate with more than 256 neighbours (solved without the coarse level),
