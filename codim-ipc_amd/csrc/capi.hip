@@ -736,6 +736,7 @@ int cipc_linear_solve(cipc_ctx* ctx, const double* x, const int32_t* cs, int32_t
         c.lpStepStart = true;   // a standalone solve evaluates the learned network afresh
         c.cg.stepStart = true;
         c.lpOffThisStep = false;
+        c.fp64ThisStep = false;
         c.solveX = ctx->qx.p;
         int rc = pcg_solve(c, ctx->qb.p, sd, tol > 0 ? tol : c.P.pcgTol, c.P.pcgMaxIter, &it, &rr);
         c.solveX = nullptr;
@@ -971,6 +972,7 @@ int cipc_precond_info(const cipc_ctx* ctx, int32_t* out) {
 int cipc_set_profiling(cipc_ctx* ctx, int32_t enable) {
     if (!ctx) return CIPC_E_INVALID;
     ctx->c.profiling = enable ? 1 : 0;
+    ctx->c.profPrecision = ctx->c.P.precision;
     ctx->c.profLaunches = 0;
     ctx->c.profMs = 0;
     ctx->c.profBytes = 0;

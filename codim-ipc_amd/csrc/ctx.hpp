@@ -277,6 +277,7 @@ struct Ctx {
     bool coarseOn = false;
     CoarseSpace cg;
     long fp64Retries = 0;                   // reduced-precision learned + coarse solves redone with FP64 blocks
+    bool fp64ThisStep = false;              // a retry happened in this step: its remaining solves use FP64 blocks
     const double* solveX = nullptr;         // positions of the Newton iterate being solved (rotation modes)
     // dense direct solve: padded row-major matrix / factor, right-hand side, pivot flag
     DevBuf<double> denseA, denseY;
@@ -320,6 +321,7 @@ struct Ctx {
 
     // optional event timing of the dominant PCG kernel (elastic SpMV)
     int profiling = 0;
+    int profPrecision = 0;                 // the solver form's precision when profiling started
     std::vector<hipEvent_t> evPool;
     long profLaunches = 0;
     double profMs = 0;

@@ -590,7 +590,10 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
     const int chunk = 16;
     int* hf = (int*)(c.hCnt + 32);
     // sampled SpMV timing: for the caller's profile and for the autotuner's SpMV phase
-    const bool sampling = c.profiling || spmvTuning;
+    // (the caller's profile samples the form's own block type: FP64-block retries of a
+    // reduced-precision form are not its SpMV)
+    const bool profThis = c.profiling && (sizeof(M) == sizeof(double)) == (c.profPrecision == 0);
+    const bool sampling = profThis || spmvTuning;
     if (sampling && c.evPool.size() < 2 * chunk) {
         for (size_t i = c.evPool.size(); i < 2 * chunk; ++i) {
             hipEvent_t e;
@@ -724,7 +727,7 @@ static int pcg_run(Ctx& c, const M* valE, const M* valK, const V* b, V* xsol, V*
             for (int k = kProfFirst; k < launched && base + k < done; k += kProfEvery) {
                 float ms = 0;
                 CIPC_HIP(hipEventElapsedTime(&ms, c.evPool[2 * k], c.evPool[2 * k + 1]));
-                if (c.profiling) {
+                if (profThis) {
                     c.profMs += ms;
                     c.profLaunches++;
                     c.profBytes += spmvBytes;
@@ -910,7 +913,25 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
     // (with the coarse level the learned solve is far stronger than block Jacobi: no cap)
     if (c.lastJacobiIters > 0 && !(c.coarseOn && c.cg.ready))
         budget = std::min(budget, std::max(200, c.lastJacobiIters + c.lastJacobiIters / 4));
-    int rc = solve_form(c, b, xsol, tol, budget, &it, &rr, true);
+    int rc;
+    // FP64-block attempts with the coarse level get four times the budget: block Jacobi,
+    // the only thing left after them, is far slower on the systems that reach them
+    const int budget64 = std::min(maxIter, 4 * budget);
+    const bool ranFp64 = c.fp64ThisStep && c.coarseOn && c.cg.ready && c.P.precision != 0;
+    if (ranFp64) {
+        // hard systems come in runs: after a retry the step's remaining solves start in FP64
+        const int saved = c.P.precision;
+        c.P.precision = 0;
+        try {
+            rc = solve_form(c, b, xsol, tol, budget64, &it, &rr, true);
+        } catch (...) {
+            c.P.precision = saved;
+            throw;
+        }
+        c.P.precision = saved;
+    } else {
+        rc = solve_form(c, b, xsol, tol, budget, &it, &rr, true);
+    }
     c.lastLearnedRc = rc;
     c.lastLearnedIters = it;
     if (rc == 0) {
@@ -922,7 +943,7 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
     // blocks and the same preconditioner, from the learned iterate: on the hardest contact
     // systems the FP32 blocks' rounding (about 6e-8 relative) stalls the refinement, while
     // the two-level preconditioner itself still converges in a few hundred iterations.
-    if (c.coarseOn && c.cg.ready && c.P.precision != 0) {
+    if (!ranFp64 && c.coarseOn && c.cg.ready && c.P.precision != 0) {
         const long n = 3l * c.nV;
         double* xl = c.fallbackX.ensure(n);
         double* r0 = c.fallbackR.ensure(n);
@@ -936,14 +957,15 @@ static int pcg_solve_impl(Ctx& c, const double* b, double* xsol, double tol, int
         double rr3 = 0;
         int rc3;
         try {
-            rc3 = from ? solve_form(c, r0, xsol, tol * std::sqrt(bn2 / rn2), budget, &it3, &rr3, true)
-                       : solve_form(c, b, xsol, tol, budget, &it3, &rr3, true);
+            rc3 = from ? solve_form(c, r0, xsol, tol * std::sqrt(bn2 / rn2), budget64, &it3, &rr3, true)
+                       : solve_form(c, b, xsol, tol, budget64, &it3, &rr3, true);
         } catch (...) {
             c.P.precision = saved;
             throw;
         }
         c.P.precision = saved;
         c.fp64Retries++;
+        c.fp64ThisStep = true;
         if (from) {
             hipLaunchKernelGGL(k_acc_f64, dim3(nblocks(n)), dim3(kBlock), 0, c.s, n, (const double*)xl, xsol);
             CIPC_LAUNCH_CHECK();

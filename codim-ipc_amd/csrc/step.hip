@@ -567,6 +567,7 @@ int advance_one_step(Ctx& c) {
     c.lpStepStart = true;
     c.cg.stepStart = true;
     c.lpOffThisStep = false;
+    c.fp64ThisStep = false;
     c.kappaSDoublings = 0;
     double t0 = now_ms(), ta;
     hipStream_t s = c.s;
