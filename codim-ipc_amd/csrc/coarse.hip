@@ -21,7 +21,8 @@
 // root and takes them; the rest join the neighbouring aggregate they share most edges
 // with), then the same once on the aggregate graph (about 10^2 vertices per aggregate),
 // then pairwise matching of neighbouring aggregates while the coarse problem is above the
-// size asked for (the set-up grows with its cube, each iteration with its square).  Dirichlet vertices and vertices without elastic neighbours (free
+// size asked for (the set-up grows with its cube, each iteration with its square); if
+// separate components keep it above (thousands of rods), only the largest aggregates stay.  Dirichlet vertices and vertices without elastic neighbours (free
 // particles) stay outside the coarse space; the smoother alone treats them.
 #include "ctx.hpp"
 #include <algorithm>
@@ -152,7 +153,28 @@ static void coarse_build(Ctx& c) {
             if (agg[v] >= 0) agg[v] = up[agg[v]];
         na = na2;
     }
-    g.nAgg = 6l * na > kCoarseMaxDof ? 0 : na;
+    // still too many (components that cannot merge: thousands of separate rods): keep the
+    // largest aggregates, the rest stay with the smoother
+    if (na > 0 && 6l * na > maxDof) {
+        std::vector<int> size(na, 0);
+        for (int v = 0; v < nV; ++v)
+            if (agg[v] >= 0) size[agg[v]]++;
+        std::vector<int> order(na);
+        for (int a = 0; a < na; ++a) order[a] = a;
+        std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return size[p] > size[q]; });
+        const int keep = maxDof / 6;
+        std::vector<int> remap(na, -1);
+        for (int r = 0; r < keep; ++r) remap[order[r]] = r;
+        // renumber the kept ones in their original order (deterministic, locality kept)
+        std::vector<int> kept;
+        for (int a = 0; a < na; ++a)
+            if (remap[a] >= 0) kept.push_back(a);
+        for (int r = 0; r < (int)kept.size(); ++r) remap[kept[r]] = r;
+        for (int v = 0; v < nV; ++v)
+            if (agg[v] >= 0) agg[v] = remap[agg[v]];
+        na = (int)kept.size();
+    }
+    g.nAgg = na;
     g.nVBuilt = nV;
     g.nnzbBuilt = E.nnzb;
     g.builtMaxDof = g.maxDof;

@@ -96,3 +96,30 @@ def test_coarse_step_trajectory(require_gpu):
         disp = np.abs(xo - x0).max()
         assert np.abs(xg - xo).max() <= trajectory_bar(sd, disp), (s, st["newton_iters"], so)
     assert gpu.coarse_info()["setups"] > 0
+
+
+@pytest.mark.parametrize("max_unknowns", [0, 12])
+def test_coarse_with_rods_particles_and_a_tight_bound(require_gpu, max_unknowns):
+    """Rods, particles and a sheet: particles (no elastic neighbours) stay outside the coarse
+    space, collinear rod aggregates lose their degenerate rotation; with a bound of 12
+    unknowns only the two largest aggregates stay.  The solve meets the oracle bar."""
+    sd = scenes.scene_codim_contact()
+    orc = O.OracleScene(sd, threads=4)
+    x = orc.get_state()[0].copy()
+    kappa = orc.scalars()["kappa"]
+    C = orc.constraint_set(x)
+    H = orc.hessian(x, C, kappa, which=3)
+    b = np.random.default_rng(21).standard_normal(x.size)
+    b.reshape(-1, 3)[sd.dbc_idx] = 0
+    sim = Simulator(sd, precision="fp64")
+    sim.precond_set_weights(precond.load())
+    sim.set_preconditioner("learned", budget=20000)
+    sim.set_coarse(True, max_unknowns=max_unknowns)
+    sol, it, rr = sim.linear_solve(x, C, kappa, b, tol=1e-8)
+    info = sim.coarse_info()
+    print(max_unknowns, info, it)
+    assert info["ready"] and info["aggregates"] > 0
+    if max_unknowns:
+        assert info["unknowns"] <= max_unknowns
+    assert rr <= 1e-8
+    assert np.linalg.norm(H @ sol - b) <= 2e-8 * np.linalg.norm(b)
