@@ -27,6 +27,18 @@ struct HipError : std::runtime_error {
 
 constexpr int kBlock = 256;
 inline int nblocks(long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
+// XCD-aware item mapping for per-item kernels whose neighbouring items read neighbouring
+// data: workgroups are dispatched to the 8 XCDs round robin, so workgroup b serves logical
+// block (b % 8) * per + b / 8 and each XCD's L2 sees one contiguous eighth of the items.
+// The grid must be xcd_grid(n) (a multiple of 8).  Only for uniform work per item: where
+// the work clusters in space (CCD candidate walks in a contact region) the round-robin
+// spread balances the XCDs better (measured: k_ccd_ee 14.9 ms round robin, 22.8 ms
+// contiguous; k_query_pt 0.72 ms round robin, 0.65 ms contiguous).
+inline int xcd_grid(long n, int bs = kBlock) { return 8 * ((nblocks(n, bs) + 7) / 8); }
+__device__ __forceinline__ int xcd_item() {
+    const int per = gridDim.x / 8;
+    return ((int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8)) * (int)blockDim.x + (int)threadIdx.x;
+}
 
 // Grow-only device allocation; contents are not preserved on growth.
 template <class T>

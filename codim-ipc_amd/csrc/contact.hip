@@ -255,7 +255,7 @@ __global__ void k_query_pt(const double* __restrict__ x, const int* __restrict__
                            const uint32_t* __restrict__ end, const uint32_t* __restrict__ vals, const int* __restrict__ triLo,
                            const int* __restrict__ lgList, const uint32_t* __restrict__ lgCount, double dHat, double dHat2,
                            int nV, CSOut o) {
-    int svI = blockIdx.x * blockDim.x + threadIdx.x;
+    int svI = xcd_item();
     if (svI >= nBN) return;
     int vI = bNode[svI];
     d3 p = ld3(x, vI);
@@ -479,7 +479,7 @@ static void constraint_set_impl(Ctx& c, const double* x, DevBuf<int>& out, int& 
         o.dcnt = c.counters.p + 1;
         CIPC_HIP(hipMemsetAsync(c.counters.p, 0, 4 * sizeof(uint32_t), c.s));
         // point-triangle kept entries fill [0, cap), edge-edge kept entries [cap, 2 cap)
-        hipLaunchKernelGGL(k_query_pt, dim3(nblocks(c.nBN)), dim3(kBlock), 0, c.s, x, c.bNode.p, c.nBN, c.btri.p, c.nBT, c.dbc.p, G,
+        hipLaunchKernelGGL(k_query_pt, dim3(xcd_grid(c.nBN)), dim3(kBlock), 0, c.s, x, c.bNode.p, c.nBN, c.btri.p, c.nBT, c.dbc.p, G,
                            c.cellStart.p, c.cellEnd.p, c.u32c.p, c.triLo.p, c.lgListT.p, lgCountT, dHat, dHat2, c.nV, o);
         CIPC_LAUNCH_CHECK();
         CSOut oEE = o;
