@@ -97,6 +97,20 @@ void chunk_sum(const double* in, long n, double* out, PrimWork& w, hipStream_t s
 // sum of per-block partials already in device memory (fixed order)
 void sum_partials(const double* partials, int n, double* out, hipStream_t s);
 
+// a slot in an output list for every active lane of the wave: the active lanes are found
+// by a ballot, the lowest adds their count to the counter, and each lane takes base + its
+// rank among them (one atomic per wave; the slot order is arbitrary)
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* cnt) {
+    const uint64_t act = __ballot(1);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(act));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    return base + rank;
+}
+
 // device reductions inside kernels
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -145,16 +145,6 @@ __device__ __forceinline__ uint64_t dup_key(int c0, int c1, int c2, int nV) {
 // the divergence) are found by a ballot, the lowest of them adds their count to the
 // counter, and each lane takes base + its rank among them (mbcnt).  The slot order is
 // arbitrary; the set is sorted by key afterwards, so the result is deterministic.
-__device__ __forceinline__ uint32_t wave_reserve(uint32_t* cnt) {
-    const uint64_t act = __ballot(1);
-    const int lane = (int)__lane_id();
-    const int leader = __ffsll((unsigned long long)act) - 1;
-    const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(act));
-    base = (uint32_t)__shfl((int)base, leader, 64);
-    return base + rank;
-}
 __device__ __forceinline__ void emit_kept(const CSOut& o, uint64_t key, int c0, int c1, int c2, int c3) {
     const uint32_t i = wave_reserve(o.cnt);
     if (i < o.cap) {
