@@ -512,7 +512,11 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_gemv(const S* __restrict__ W, i
         const int g0 = (UPPER ? row : 0) / E, g1 = (UPPER ? npad - 1 : row) / E + 1;
         for (int g = g0 + (int)threadIdx.x; g < g1; g += kCgBlock) {
             double e[E];
-            cg_unpack(w[g], (const S*)nullptr, e);
+            // streamed once per iteration: non-temporal, so that it does not push the
+            // SpMV's matrix out of the last-level cache
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            const f4 raw = __builtin_nontemporal_load(reinterpret_cast<const f4*>(w) + g);
+            cg_unpack(make_float4(raw.x, raw.y, raw.z, raw.w), (const S*)nullptr, e);
             const int j = E * g;
 #pragma unroll
             for (int k = 0; k < E; ++k) {
