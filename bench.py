@@ -73,6 +73,17 @@ MODE = {"fp64": "plain FP64, block-Jacobi PCG",
                                "rigid-body coarse level (block-Jacobi fallback)"}
 PRECISION = {"fp64": "fp64", "mixed": "mixed", "fp32": "fp32", "mixed_learned": "mixed", "fp32_learned": "fp32",
              "mixed_learned_coarse": "mixed", "fp32_learned_coarse": "fp32"}
+# Launch shapes pinned on the 320k-triangle stack (the ones the autotuner settles on there
+# in every measured run).  A shape only reorders floating-point reductions, but the
+# deep-contact trajectory amplifies last-bit differences: with the tuner's timing-based
+# choice made during the warm-up, the learned form's window took 1398 PCG iterations per
+# Newton iteration in some runs and 1724 in others.  Pinned shapes give the same bits, and
+# the same iteration counts, on every run.
+PINNED_SHAPES = {
+    "fp64": dict(spmv_grid=1024, spmv_lanes=16, spmv_unroll=1, factor_grid=2048, factor_lanes=4, cell_mult=0.5),
+    "mixed": dict(spmv_grid=1024, spmv_lanes=8, spmv_unroll=1, factor_grid=1024, factor_lanes=4, cell_mult=0.5),
+    "fp32": dict(spmv_grid=1024, spmv_lanes=8, spmv_unroll=2, factor_grid=2048, factor_lanes=4, cell_mult=0.5),
+}
 
 
 def measured_traffic(kernel, workload, window):
@@ -142,6 +153,8 @@ def gpu_run(args, sd, form, state):
             sim.set_preconditioner("learned", budget=args.coarse_budget, reuse=args.learned_reuse if args.learned_reuse else 2)
     if args.autotune:
         sim.autotune()          # settles during the warm-up steps
+    else:
+        sim.set_tuning(**PINNED_SHAPES[PRECISION[form]])
     tw = time.perf_counter()
     wstats = []
     for k in range(args.warmup):
@@ -329,8 +342,11 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"],
                     help="c2 (default): 320k-triangle stack; c1: the 41x41 sheet (8,322 triangles with the obstacles); "
                          "c3: the c1 scene with 200,000 rod segments and 100,000 particles; c5: batch of independent scenes")
+    ap.add_argument("--autotune", dest="autotune", action="store_true", default=None,
+                    help="tune the launch shapes during the warm-up (default for c1, c3, c5; on c2 the pinned shapes "
+                         "are used so that every run follows the same trajectory)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="keep the default launch shapes instead of tuning them during the warm-up")
+                    help="keep fixed launch shapes (the pinned ones; c5: the library defaults)")
     ap.add_argument("--save-state", default=None,
                     help="write the timed window's start state (after the warm-up; x, v, kappa, step) to this .npz")
     ap.add_argument("--load-state", default=None,
@@ -350,7 +366,7 @@ def main():
     ap.add_argument("--coarse-every", type=int, default=4,
                     help="Newton solves per coarse set-up of the *_coarse forms (always at a step's first solve)")
     ap.add_argument("--coarse-bf16", action="store_true", help="BF16 inverse coarse factor (default FP32)")
-    ap.add_argument("--coarse-budget", type=int, default=800,
+    ap.add_argument("--coarse-budget", type=int, default=1200,
                     help="PCG iterations after which a learned + coarse solve falls back to block Jacobi")
     ap.add_argument("--learned-budget", type=int, default=2500,
                     help="PCG iterations after which a learned solve falls back to block Jacobi (continuing from its iterate)")
@@ -365,6 +381,8 @@ def main():
     ap.add_argument("--cpu-newton", type=int, default=2, help="Newton iterations in the CPU baseline's sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
+    if args.autotune is None:
+        args.autotune = args.workload != "c2"
     if args.gpus != 1:
         print("bench.py: this build runs one scene on one GPU; --gpus %d is not supported (use --gpus 1)" % args.gpus,
               file=sys.stderr)
@@ -439,7 +457,8 @@ def main():
                    "start_state": os.path.basename(args.load_state) if args.load_state else "initial",
                    "free_fall_contacts_last": int(ff[-1]["contacts"]) if ff else None,
                    "dt": sd.params["dt"], "dhat": sd.params["dHat"],
-                   "pcg_tol": args.pcg_tol, "friction_mu": args.mu, "strain_limit": args.strain_limit},
+                   "pcg_tol": args.pcg_tol, "friction_mu": args.mu, "strain_limit": args.strain_limit,
+                   "launch_shapes": "autotuned in the warm-up" if args.autotune else "pinned"},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
         "forms": {m: {k: (round(v, 4) if isinstance(v, float) and k != "min_dist_final" and not k.startswith("vs_fp64") else v)
