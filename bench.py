@@ -147,7 +147,7 @@ def gpu_run(args, sd, form, state):
         sim.precond_set_weights(precond.load())
         sim.set_preconditioner("learned", budget=args.learned_budget, reuse=args.learned_reuse)
         if form.endswith("_coarse"):
-            sim.set_coarse(True, every=args.coarse_every, max_unknowns=args.coarse_max, bf16=args.coarse_bf16)
+            sim.set_coarse(True, every=args.coarse_every, max_unknowns=args.coarse_max, bf16=args.coarse_storage == "bf16")
             # with the coarse level the network runs every second solve by default (the
             # coarse level carries the smooth error, the factor's staleness costs little)
             sim.set_preconditioner("learned", budget=args.coarse_budget, reuse=args.learned_reuse if args.learned_reuse else 2)
@@ -209,7 +209,7 @@ def gpu_run(args, sd, form, state):
                 statuses=sorted(set(s["status"] for s in stats)), min_dist_final=min_d,
                 learned_solves=(info1["solves"] - info0["solves"]) if learned else None,
                 learned_fallbacks=(info1["fallbacks"] - info0["fallbacks"]) if learned else None,
-                coarse=dict(sim.coarse_info(), every=args.coarse_every, max_unknowns=args.coarse_max, bf16=args.coarse_bf16)
+                coarse=dict(sim.coarse_info(), every=args.coarse_every, max_unknowns=args.coarse_max, bf16=args.coarse_storage == "bf16")
                 if form.endswith("_coarse") else None,
                 spmv_avg_us=roof["avg_us"] if roof else None, spmv_frac=roof["frac"] if roof else None, roofline=roof,
                 _x=x, _energy=stats[-1]["energy"], _start=start_state)
@@ -365,7 +365,8 @@ def main():
                     help="largest coarse problem (unknowns) of the *_coarse forms")
     ap.add_argument("--coarse-every", type=int, default=4,
                     help="Newton solves per coarse set-up of the *_coarse forms (always at a step's first solve)")
-    ap.add_argument("--coarse-bf16", action="store_true", help="BF16 inverse coarse factor (default FP32)")
+    ap.add_argument("--coarse-storage", default="bf16", choices=["bf16", "fp32"],
+                    help="storage of the coarse level's inverse factor in the per-iteration products")
     ap.add_argument("--coarse-budget", type=int, default=1200,
                     help="PCG iterations after which a learned + coarse solve falls back to block Jacobi")
     ap.add_argument("--learned-budget", type=int, default=2500,

@@ -493,45 +493,86 @@ __device__ __forceinline__ void cg_unpack(const float4& w, const uint16_t*, doub
 }
 
 // y = W x over the lower (W = L^-1) or upper (W = L^-T) triangle of a row-major matrix of
-// FP32 or BF16 entries, FP64 accumulation.  One workgroup per pair of rows (i, npad-1-i),
-// whose triangle parts together hold npad + 1 entries, so every workgroup streams the
-// same amount; 16 bytes per lane load; fixed-order reduction.
+// FP32 or BF16 entries, FP64 accumulation.  One workgroup per pair of bands of kCgBand
+// rows, (kCgBand b ..) and (npad - kCgBand (b+1) ..), whose triangle parts together hold
+// about the same number of entries for every b; 16 bytes per lane load, kCgChunk of them
+// per row issued before any is used.  Each lane's slice of x is loaded once and used for
+// the band's kCgBand rows: with one row per workgroup the x loads (twice the matrix's
+// bytes for FP32, four times for BF16, served by L2) cost more than the matrix stream.
+// Fixed-order reduction.
+constexpr int kCgBand = 4;
+static_assert(kCgTile % (2 * kCgBand) == 0, "the padded size must split into band pairs");
 template <class S, bool UPPER>
 __global__ __launch_bounds__(kCgBlock) void k_cg_gemv(const S* __restrict__ W, int npad, const double* __restrict__ x,
                                                       double* __restrict__ y, const int* __restrict__ done) {
     if (done && *done) return;
     constexpr int E = 16 / sizeof(S);   // entries per 16-byte group
-    __shared__ double part[2][kCgBlock / 64];
+    constexpr int R = kCgBand;
+    constexpr int C = sizeof(S) == 4 ? 1 : 2;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    __shared__ double part[2 * R][kCgBlock / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rA = R * (int)blockIdx.x, rB = npad - R - R * (int)blockIdx.x;
+    // each band's groups: the union of its rows' triangle parts (entries outside a row's
+    // part are masked below)
+    const int a0 = (UPPER ? rA : 0) / E, a1 = (UPPER ? npad - 1 : rA + R - 1) / E + 1;
+    const int b0 = (UPPER ? rB : 0) / E, b1 = (UPPER ? npad - 1 : rB + R - 1) / E + 1;
+    const int nA = a1 - a0, T = nA + (b1 - b0);
+    double accA[R], accB[R];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int row = h == 0 ? (int)blockIdx.x : npad - 1 - (int)blockIdx.x;
-        const float4* w = reinterpret_cast<const float4*>(W + (long)row * npad);
-        double acc = 0;
-        // whole groups; the row's first / last group is masked to the triangle
-        const int g0 = (UPPER ? row : 0) / E, g1 = (UPPER ? npad - 1 : row) / E + 1;
-        for (int g = g0 + (int)threadIdx.x; g < g1; g += kCgBlock) {
-            double e[E];
+    for (int q = 0; q < R; ++q) accA[q] = accB[q] = 0;
+    for (int base = 0; base < T; base += C * kCgBlock) {
+        f4 raw[C][R];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int t = base + u * kCgBlock + (int)threadIdx.x;
+            const int r = t < nA ? rA : rB;
+            const int g = t < nA ? a0 + t : b0 + t - nA;
             // streamed once per iteration: non-temporal, so that it does not push the
             // SpMV's matrix out of the last-level cache
-            typedef float f4 __attribute__((ext_vector_type(4)));
-            const f4 raw = __builtin_nontemporal_load(reinterpret_cast<const f4*>(w) + g);
-            cg_unpack(make_float4(raw.x, raw.y, raw.z, raw.w), (const S*)nullptr, e);
-            const int j = E * g;
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const bool in = UPPER ? (j + k >= row) : (j + k <= row);
-                if (in) acc += e[k] * x[j + k];
+            for (int q = 0; q < R; ++q)
+                raw[u][q] = t < T ? __builtin_nontemporal_load(reinterpret_cast<const f4*>(W + (long)(r + q) * npad) + g) : f4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int t = base + u * kCgBlock + (int)threadIdx.x;
+            if (t < T) {
+                const bool isA = t < nA;
+                const int r = isA ? rA : rB;
+                const int j = E * (isA ? a0 + t : b0 + t - nA);
+                double xv[E];
+#pragma unroll
+                for (int k = 0; k < E; ++k) xv[k] = x[j + k];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    double e[E];
+                    cg_unpack(make_float4(raw[u][q].x, raw[u][q].y, raw[u][q].z, raw[u][q].w), (const S*)nullptr, e);
+                    double s = 0;
+#pragma unroll
+                    for (int k = 0; k < E; ++k) {
+                        const bool in = UPPER ? (j + k >= r + q) : (j + k <= r + q);
+                        if (in) s += e[k] * xv[k];
+                    }
+                    if (isA) accA[q] += s;
+                    else accB[q] += s;
+                }
             }
         }
-        acc = wave_sum(acc);
-        if (lane == 0) part[h][wv] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const double va = wave_sum(accA[q]);
+        const double vb = wave_sum(accB[q]);
+        if (lane == 0) {
+            part[q][wv] = va;
+            part[R + q][wv] = vb;
+        }
     }
     __syncthreads();
-    if (threadIdx.x < 2) {
-        const int h = threadIdx.x;
-        const int row = h == 0 ? (int)blockIdx.x : npad - 1 - (int)blockIdx.x;
-        y[row] = (part[h][0] + part[h][1]) + (part[h][2] + part[h][3]);
+    if (threadIdx.x < 2 * R) {
+        const int q = threadIdx.x;
+        y[q < R ? rA + q : rB + q - R] = (part[q][0] + part[q][1]) + (part[q][2] + part[q][3]);
     }
 }
 
@@ -595,14 +636,14 @@ void coarse_restrict_solve(Ctx& c, const V* r, const int* done) {
     hipLaunchKernelGGL((k_cg_restrict<V>), dim3(g.nAgg), dim3(kCgBlock), 0, c.s, g.nAgg, (const int*)g.aggPtr.p,
                        (const int*)g.aggVerts.p, (const double*)g.d.p, r, g.rc.p, done);
     if (g.bf16) {
-        hipLaunchKernelGGL((k_cg_gemv<uint16_t, false>), dim3(g.npad / 2), dim3(kCgBlock), 0, c.s, (const uint16_t*)g.W16.p, g.npad,
+        hipLaunchKernelGGL((k_cg_gemv<uint16_t, false>), dim3(g.npad / (2 * kCgBand)), dim3(kCgBlock), 0, c.s, (const uint16_t*)g.W16.p, g.npad,
                            (const double*)g.rc.p, g.wc.p, done);
-        hipLaunchKernelGGL((k_cg_gemv<uint16_t, true>), dim3(g.npad / 2), dim3(kCgBlock), 0, c.s, (const uint16_t*)g.WT16.p, g.npad,
+        hipLaunchKernelGGL((k_cg_gemv<uint16_t, true>), dim3(g.npad / (2 * kCgBand)), dim3(kCgBlock), 0, c.s, (const uint16_t*)g.WT16.p, g.npad,
                            (const double*)g.wc.p, g.zc.p, done);
     } else {
-        hipLaunchKernelGGL((k_cg_gemv<float, false>), dim3(g.npad / 2), dim3(kCgBlock), 0, c.s, (const float*)g.W32.p, g.npad,
+        hipLaunchKernelGGL((k_cg_gemv<float, false>), dim3(g.npad / (2 * kCgBand)), dim3(kCgBlock), 0, c.s, (const float*)g.W32.p, g.npad,
                            (const double*)g.rc.p, g.wc.p, done);
-        hipLaunchKernelGGL((k_cg_gemv<float, true>), dim3(g.npad / 2), dim3(kCgBlock), 0, c.s, (const float*)g.WT32.p, g.npad,
+        hipLaunchKernelGGL((k_cg_gemv<float, true>), dim3(g.npad / (2 * kCgBand)), dim3(kCgBlock), 0, c.s, (const float*)g.WT32.p, g.npad,
                            (const double*)g.wc.p, g.zc.p, done);
     }
     CIPC_LAUNCH_CHECK();

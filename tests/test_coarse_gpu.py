@@ -32,21 +32,22 @@ def ground(require_gpu):
     return sd, x, C, kappa, H, b
 
 
-def learned_sim(sd, precision, coarse, graphs=True):
+def learned_sim(sd, precision, coarse, graphs=True, bf16=False):
     sim = Simulator(sd, precision=precision)
     sim.precond_set_weights(precond.load())
     sim.set_preconditioner("learned", budget=20000)
-    sim.set_coarse(coarse)
+    sim.set_coarse(coarse, bf16=bf16)
     sim.set_graphs(graphs)
     return sim
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
-def test_coarse_solve_meets_bar_and_cuts_iterations(ground, precision):
+@pytest.mark.parametrize("precision,storage", [("fp64", "fp32"), ("mixed", "fp32"), ("fp32", "fp32"), ("mixed", "bf16"),
+                                               ("fp32", "bf16")])
+def test_coarse_solve_meets_bar_and_cuts_iterations(ground, precision, storage):
     sd, x, C, kappa, H, b = ground
     its = {}
     for coarse in (False, True):
-        sim = learned_sim(sd, precision, coarse)
+        sim = learned_sim(sd, precision, coarse, bf16=storage == "bf16")
         sol, it, rr = sim.linear_solve(x, C, kappa, b, tol=1e-8)
         assert sim.precond_info()["fallback"] == 0
         assert rr <= 1e-8
@@ -61,11 +62,12 @@ def test_coarse_solve_meets_bar_and_cuts_iterations(ground, precision):
     assert its[True] < 0.7 * its[False]
 
 
-def test_coarse_solves_repeat_bits_with_and_without_graphs(ground):
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_coarse_solves_repeat_bits_with_and_without_graphs(ground, storage):
     sd, x, C, kappa, H, b = ground
     sols = []
     for graphs in (True, True, False):
-        sim = learned_sim(sd, "mixed", True, graphs)
+        sim = learned_sim(sd, "mixed", True, graphs, bf16=storage == "bf16")
         sol, it, rr = sim.linear_solve(x, C, kappa, b, tol=1e-10)
         sols.append((sol, it))
         sim.close()
