@@ -40,85 +40,115 @@ __global__ void k_dense_scatter(const int* __restrict__ rowOf, const int* __rest
         }
 }
 
-// Cholesky of diagonal tile k (lower, in place), one wave: thread i keeps row i of the
-// tile in registers, so the trailing updates are register FMAs against a broadcast copy of
-// the current column, whose LDS reads carry no condition and are issued together (a single
-// wave has nothing else to hide their latency behind).  A non-positive pivot sets *bad and
-// continues with pivot 1 so that the factorisation stays finite.  With diag0, a pivot at
-// or below tau x diag0 (the unknown's diagonal before elimination: a mode the matrix does
-// not see) is dropped instead: a pivot at the unknown's own scale, zero column,
-// dropped[] = 1.  With inv, the inverse of the factored tile is written there too (column
-// c by forward substitution in thread c's registers).
-__global__ __launch_bounds__(kDT) void k_potrf_tile(double* __restrict__ a, long ld, int k, int* __restrict__ bad,
-                                                    const double* __restrict__ diag0, int* __restrict__ dropped, double tau,
-                                                    double* __restrict__ inv) {
+// Cholesky of diagonal tile k (lower, in place) by four waves: wave h keeps columns
+// [16h, 16h+16) of every row in registers, so the trailing update of a column is 16
+// register FMAs per lane against a broadcast copy of the column in LDS.  Per column two
+// barriers: after the column's entries are published, and after its multipliers are
+// (both LDS buffers alternate between columns, so no third is needed).  The entry (i, j)
+// of the column being eliminated is published by the wave that holds it (picked from
+// registers by a select tree on the bits of j); every lane derives the pivot and its
+// row's multiplier from it with the same operations, so all lanes hold the same bits.
+// A non-positive pivot sets *bad and continues with pivot 1 so that the factorisation
+// stays finite.  With diag0, a pivot at or below tau x diag0 (the unknown's diagonal
+// before elimination: a mode the matrix does not see) is dropped instead: a pivot at the
+// unknown's own scale, zero column, dropped[] = 1.  With inv, the inverse of the factored
+// tile is written there too.  Every sum runs in a fixed order: trailing updates in
+// column order, substitution sums in index order.
+static_assert(kDirectBlock == 4 * kDT, "the tile factor runs four waves of one lane per row");
+__global__ __launch_bounds__(kDirectBlock) void k_potrf_tile(double* __restrict__ a, long ld, int k, int* __restrict__ bad,
+                                                             const double* __restrict__ diag0, int* __restrict__ dropped, double tau,
+                                                             double* __restrict__ inv) {
     __shared__ double T[kDT][kDT + 1];
-    __shared__ double colj[kDT];
-    __shared__ double piv;
-    __shared__ int drop;
+    __shared__ double colj[2][kDT];
+    __shared__ double cur[2][kDT];
+    __shared__ double dg0[kDT];
+    const int tid = threadIdx.x, i = tid & 63, h = tid >> 6;
     const long o = (long)k * kDT;
-    const int i = threadIdx.x;
-    // the pivot of column j is thread j's: its original diagonal is loaded up front
-    const double myDiag0 = diag0 ? diag0[o + i] : 0.0;
-    for (int r = 0; r < kDT; ++r) T[r][i] = a[(o + r) * ld + o + i];
+    if (h == 0) dg0[i] = diag0 ? diag0[o + i] : 0.0;
+    for (int r = h; r < kDT; r += 4) T[r][i] = a[(o + r) * ld + o + i];
     __syncthreads();
-    double row[kDT];   // entries right of the diagonal hold garbage and are never stored
+    double row[16];
 #pragma unroll
-    for (int l = 0; l < kDT; ++l) row[l] = T[i][l];
-    __syncthreads();
+    for (int q = 0; q < 16; ++q) row[q] = T[i][16 * h + q];
     for (int j = 0; j < kDT; ++j) {
-        double lij = 0.0;
+        const int b = j & 1;
+        if (h == (j >> 4)) {
+            double t4[4];
+            const bool s8 = j & 8, s4 = j & 4;
 #pragma unroll
-        for (int l = 0; l < kDT; ++l) lij = l == j ? row[l] : lij;
-        if (i == j) {
-            double d = lij;
-            int dr = 0;
-            if (diag0) {
-                if (!(d > tau * myDiag0)) {
-                    dr = 1;
-                    d = myDiag0 > 0 ? myDiag0 : 1.0;
-                }
-                dropped[o + j] = dr;
-            } else if (!(d > 0)) {
-                atomicOr(bad, 1);
-                d = 1.0;
-            }
-            piv = sqrt(d);
-            drop = dr;
+            for (int q = 0; q < 4; ++q) t4[q] = s8 ? (s4 ? row[q + 12] : row[q + 8]) : (s4 ? row[q + 4] : row[q]);
+            const bool s2 = j & 2, s1 = j & 1;
+            cur[b][i] = s2 ? (s1 ? t4[3] : t4[2]) : (s1 ? t4[1] : t4[0]);
         }
         __syncthreads();
-        const double pv = piv;
-        const bool dr = drop;
+        double d = cur[b][j];
+        const double rij = cur[b][i];
+        int dr = 0;
+        bool neg = false;
+        if (diag0) {
+            const double d0 = dg0[j];
+            if (!(d > tau * d0)) {
+                dr = 1;
+                d = d0 > 0 ? d0 : 1.0;
+            }
+        } else if (!(d > 0)) {
+            neg = true;
+            d = 1.0;
+        }
+        if (tid == j) {
+            if (diag0) dropped[o + j] = dr;
+            else if (neg) atomicOr(bad, 1);
+        }
+        const double pv = sqrt(d);
+        double lij;
         if (i == j) lij = pv;
-        else if (i > j) lij = dr ? 0.0 : lij / pv;
+        else if (i > j) lij = dr ? 0.0 : rij / pv;
         else lij = 0.0;
-        T[i][j] = lij;          // column j of L (zero above the diagonal)
-        colj[i] = i > j ? lij : 0.0;
+        if (h == 0) {
+            T[i][j] = lij;
+            colj[b][i] = i > j ? lij : 0.0;
+        }
         __syncthreads();
-        // row[l] -= L_ij L_lj: colj is zero for l <= j, and entries right of the diagonal
-        // are never read, so the loop needs no condition
         if (i > j) {
 #pragma unroll
-            for (int l = 0; l < kDT; ++l) row[l] = fma(-lij, colj[l], row[l]);
+            for (int q = 0; q < 16; ++q) row[q] = fma(-lij, colj[b][16 * h + q], row[q]);
         }
-        __syncthreads();
     }
-    for (int r = 0; r < kDT; ++r) a[(o + r) * ld + o + i] = T[r][i];
+    __syncthreads();
+    for (int r = h; r < kDT; r += 4) a[(o + r) * ld + o + i] = T[r][i];
     if (inv) {
-        // column i of the inverse: X[r] = (delta_ri - sum_{m < r} L[r][m] X[m]) / L[r][r],
-        // X[m] = 0 for m < i; every thread walks the same (r, m) sequence, so the L reads
-        // are broadcasts
-        double x[kDT];
+        // column i of the inverse, rows [16h, 16h+16) in wave h: X[r] = (delta_ri -
+        // sum_{m<r} L[r][m] X[m]) / L[r][r] with the sum taken in m order, quarter by
+        // quarter: wave s finishes its rows, then the later waves add their terms
+        // from them
+        __shared__ double X[kDT][kDT + 1];
+        double sacc[16];
 #pragma unroll
-        for (int r = 0; r < kDT; ++r) {
-            double sacc = r == i ? 1.0 : 0.0;
+        for (int q = 0; q < 16; ++q) sacc[q] = 16 * h + q == i ? 1.0 : 0.0;
+        for (int st = 0; st < 4; ++st) {
+            if (h == st) {
+                double xq[16];
 #pragma unroll
-            for (int m = 0; m < r; ++m) sacc = fma(-T[r][m], x[m], sacc);
-            x[r] = r < i ? 0.0 : sacc / T[r][r];
+                for (int q = 0; q < 16; ++q) {
+                    const int r = 16 * st + q;
+#pragma unroll
+                    for (int m = 0; m < q; ++m) sacc[q] = fma(-T[r][16 * st + m], xq[m], sacc[q]);
+                    xq[q] = r < i ? 0.0 : sacc[q] / T[r][r];
+                    X[r][i] = xq[q];
+                }
+            }
+            __syncthreads();
+            if (h > st) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int r = 16 * h + q;
+#pragma unroll
+                    for (int m = 0; m < 16; ++m) sacc[q] = fma(-T[r][16 * st + m], X[16 * st + m][i], sacc[q]);
+                }
+            }
         }
         double* dst = inv + (long)k * kDT * kDT;
-#pragma unroll
-        for (int r = 0; r < kDT; ++r) dst[r * kDT + i] = x[r];
+        for (int r = h; r < kDT; r += 4) dst[r * kDT + i] = X[r][i];
     }
 }
 
@@ -228,7 +258,7 @@ void dense_cholesky(Ctx& c, double* A, int npad, int* bad, double* tileInv, cons
     for (int k = 0; k < nt; ++k) {
         // the diagonal tile's inverse (tileInv) turns the panel into a product; without it
         // the panel rows are solved by substitution (the direct solve's exact path)
-        hipLaunchKernelGGL(k_potrf_tile, dim3(1), dim3(kDT), 0, c.s, A, ld, k, bad, diag0, dropped, tau, tileInv);
+        hipLaunchKernelGGL(k_potrf_tile, dim3(1), dim3(kDirectBlock), 0, c.s, A, ld, k, bad, diag0, dropped, tau, tileInv);
         const int m = nt - k - 1;
         if (m > 0) {
             if (tileInv) hipLaunchKernelGGL(k_panel_tiles, dim3(m), dim3(kDirectBlock), 0, c.s, A, ld, k, (const double*)tileInv,
