@@ -201,6 +201,7 @@ static void coarse_build(Ctx& c) {
     upload(g.agg, agg, c.s);
     upload(g.aggPtr, aptr, c.s);
     upload(g.aggVerts, members, c.s);
+    g.nMembers = (int)members.size();
     const size_t np2 = (size_t)g.npad * g.npad;
     g.A.ensure_exact(np2);
     g.W.ensure_exact(np2);
@@ -219,6 +220,7 @@ static void coarse_build(Ctx& c) {
     g.zc.ensure_exact(g.npad);
     g.cent.ensure_exact(4l * na);
     g.d.ensure_exact(3l * nV);
+    g.dm.ensure_exact(3l * (size_t)members.size());
     g.flag.ensure_exact(1);
     CIPC_HIP(hipMemsetAsync(g.rc.p, 0, g.npad * sizeof(double), c.s));
     CIPC_HIP(hipMemsetAsync(g.d.p, 0, 3l * nV * sizeof(double), c.s));
@@ -263,6 +265,14 @@ __global__ void k_cg_offsets(int nV, const int* __restrict__ agg, const double* 
     const double s = cent[4 * a + 3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) d[3 * v + k] = (x[3 * v + k] - cent[4 * a + k]) * s;
+}
+
+__global__ void k_cg_member_offsets(int nM, const int* __restrict__ aggVerts, const double* __restrict__ d, double* __restrict__ dm) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nM) return;
+    const int v = aggVerts[q];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dm[3 * q + k] = d[3 * v + k];
 }
 
 __global__ void k_cg_zero_identity(double* __restrict__ a, long n2, int nc, int npad) {
@@ -453,22 +463,43 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_store(const double* __restrict_
 // fixed order
 template <class V>
 __global__ __launch_bounds__(kCgBlock) void k_cg_restrict(int nAgg, const int* __restrict__ aggPtr, const int* __restrict__ aggVerts,
-                                                          const double* __restrict__ d, const V* __restrict__ r, double* __restrict__ rc,
+                                                          const double* __restrict__ dm, const V* __restrict__ r, double* __restrict__ rc,
                                                           const int* __restrict__ done) {
     if (done && *done) return;
     __shared__ double part[kCgBlock / 64][6];
     const int a = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int q1 = aggPtr[a + 1];
     double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int q = aggPtr[a] + threadIdx.x; q < aggPtr[a + 1]; q += kCgBlock) {
-        const int v = aggVerts[q];
-        const double r0 = (double)r[3 * v], r1 = (double)r[3 * v + 1], r2 = (double)r[3 * v + 2];
-        const double d0 = d[3 * v], d1 = d[3 * v + 1], d2 = d[3 * v + 2];
-        s[0] += r0;
-        s[1] += r1;
-        s[2] += r2;
-        s[3] += d1 * r2 - d2 * r1;
-        s[4] += d2 * r0 - d0 * r2;
-        s[5] += d0 * r1 - d1 * r0;
+    // members q, q + kCgBlock, ... of this lane, two per pass with all loads issued first
+    // (the offsets are stored in member order, so only r is gathered); summed in member
+    // order
+    for (int q = aggPtr[a] + threadIdx.x; q < q1; q += 2 * kCgBlock) {
+        const int qb = q + kCgBlock;
+        const bool hb = qb < q1;
+        const int va = aggVerts[q], vb = aggVerts[hb ? qb : q];
+        double dv[2][3], rv[2][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            dv[0][k] = dm[3 * q + k];
+            dv[1][k] = dm[3 * (hb ? qb : q) + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            rv[0][k] = (double)r[3 * va + k];
+            rv[1][k] = (double)r[3 * vb + k];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (u == 1 && !hb) break;
+            const double* dd = dv[u];
+            const double* rr = rv[u];
+            s[0] += rr[0];
+            s[1] += rr[1];
+            s[2] += rr[2];
+            s[3] += dd[1] * rr[2] - dd[2] * rr[1];
+            s[4] += dd[2] * rr[0] - dd[0] * rr[2];
+            s[5] += dd[0] * rr[1] - dd[1] * rr[0];
+        }
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) s[k] = wave_sum(s[k]);
@@ -593,6 +624,8 @@ bool coarse_prepare(Ctx& c, const double* x) {
     hipLaunchKernelGGL(k_cg_centroids, dim3(nblocks(g.nAgg, kCgBlock / 64)), dim3(kCgBlock), 0, c.s, g.nAgg, (const int*)g.aggPtr.p,
                        (const int*)g.aggVerts.p, x, g.cent.p);
     hipLaunchKernelGGL(k_cg_offsets, dim3(nblocks(c.nV)), dim3(kBlock), 0, c.s, c.nV, (const int*)g.agg.p, x, (const double*)g.cent.p, g.d.p);
+    hipLaunchKernelGGL(k_cg_member_offsets, dim3(nblocks(g.nMembers)), dim3(kBlock), 0, c.s, g.nMembers, (const int*)g.aggVerts.p,
+                       (const double*)g.d.p, g.dm.p);
     hipLaunchKernelGGL(k_cg_zero_identity, dim3(2048), dim3(kBlock), 0, c.s, g.A.p, (long)g.npad * g.npad, g.nc, g.npad);
     CIPC_HIP(hipMemsetAsync(g.flag.p, 0, sizeof(int), c.s));
     CIPC_LAUNCH_CHECK();
@@ -634,7 +667,7 @@ template <class V>
 void coarse_restrict_solve(Ctx& c, const V* r, const int* done) {
     CoarseSpace& g = c.cg;
     hipLaunchKernelGGL((k_cg_restrict<V>), dim3(g.nAgg), dim3(kCgBlock), 0, c.s, g.nAgg, (const int*)g.aggPtr.p,
-                       (const int*)g.aggVerts.p, (const double*)g.d.p, r, g.rc.p, done);
+                       (const int*)g.aggVerts.p, (const double*)g.dm.p, r, g.rc.p, done);
     if (g.bf16) {
         hipLaunchKernelGGL((k_cg_gemv<uint16_t, false>), dim3(g.npad / (2 * kCgBand)), dim3(kCgBlock), 0, c.s, (const uint16_t*)g.W16.p, g.npad,
                            (const double*)g.rc.p, g.wc.p, done);

@@ -61,11 +61,12 @@ constexpr int kCoarseMaxVerts = 4096;     // vertices of one aggregate
 struct CoarseSpace {
     bool built = false;                   // aggregates of the current elastic pattern
     bool ready = false;                   // factor valid for the current Hessian
-    int nAgg = 0, nc = 0, npad = 0;
+    int nAgg = 0, nc = 0, npad = 0, nMembers = 0;
     int nnzbBuilt = -1, nVBuilt = -1;
     DevBuf<int> agg, aggPtr, aggVerts;    // per vertex aggregate (-1: not in the coarse space), CSR of members
     DevBuf<double> cent;                  // per aggregate: centroid xyz, 1 / rms radius
     DevBuf<double> d;                     // per vertex: (x - centroid) / rms radius
+    DevBuf<double> dm;                    // the same per member, in aggVerts order (restriction reads it in sequence)
     DevBuf<double> A;                     // dense coarse matrix, then its Cholesky factor, then L^-1 (npad^2)
     DevBuf<double> W;                     // dense work: L^-1 (npad^2)
     DevBuf<double> diag0, tileInv;        // coarse diagonal before factorisation; inverses of the 64x64 diagonal tiles
