@@ -15,18 +15,26 @@ are, the timed steps are therefore the sheets landing on the sphere and on each 
 (10^5 - 5 x 10^5 active constraints, 6 - 30 Newton iterations per step); contacts and
 Newton iterations of every timed step are reported.
 
-Solver forms, each from the same hand-over state:
-  fp64           block-Jacobi PCG in FP64 (the plain FP64 form);
-  mixed_learned  the mixed-precision learned form (headline `value`): FP32 Hessian blocks
-                 (half the SpMV's block bytes), FP64 Krylov vectors and SpMV accumulation,
-                 inside FP64 iterative refinement on the true residual, preconditioned by
-                 the learned factor (BF16 network and factor; weights trained by
-                 cipc.precond, codim-ipc_amd/cipc/weights/precond.npz), falling back to
-                 block Jacobi when its iteration budget runs out;
-  fp32_learned   the same with FP32 Krylov vectors and FP32 SpMV accumulation as well;
+Solver forms, each from the same hand-over state (default: the first three):
+  fp64                  block-Jacobi PCG in FP64 (the plain FP64 form);
+  mixed_learned         FP32 Hessian blocks (half the SpMV's block bytes), FP64 Krylov
+                        vectors and SpMV accumulation, inside FP64 iterative refinement on
+                        the true residual, preconditioned by the learned factor (BF16
+                        network and factor; weights trained by cipc.precond,
+                        codim-ipc_amd/cipc/weights/precond.npz), falling back to block
+                        Jacobi when its iteration budget runs out;
+  mixed_learned_coarse  the same plus the rigid-body coarse level on the learned factor
+                        (headline `value`): Galerkin coarse matrix of each solve's own
+                        Hessian, its inverse Cholesky factor stored in BF16
+                        (--coarse-storage), set up at a step's first solve and every
+                        --coarse-every solves; a reduced-precision solve that fails is
+                        redone with FP64 blocks;
 and with --precision all also
+  fp32_learned, fp32_learned_coarse   FP32 Krylov vectors and SpMV accumulation as well;
   mixed          FP32 Hessian blocks, FP64 Krylov vectors / SpMV accumulation, block-Jacobi;
   fp32           FP32 blocks, vectors and accumulation, block-Jacobi.
+Launch shapes are pinned on this workload (PINNED_SHAPES), so that every run follows the
+same trajectory; --autotune tunes them during the warm-up instead.
 Every form meets the same FP64 relative residual 1e-5 in every Newton solve.
 
 The JSON line carries the roofline of the dominant kernel of the headline form (the PCG
